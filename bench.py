@@ -1,0 +1,156 @@
+"""Benchmark of the rendering hot path on one MI355X.
+
+Step = one closest-hit traversal launch over the whole ray set of the traversal
+workload (1M-triangle soup SBVH, 8,294,400 rays: 4,147,200 coherent primary rays
+from a 1920x1080 pinhole at 2 spp + 4,147,200 incoherent cosine-hemisphere rays),
+rays and hits resident in HBM. Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--variant", type=int, default=0)
+    ap.add_argument("--lds-stack", type=int, default=16)
+    ap.add_argument("--block", type=int, default=128)
+    ap.add_argument("--ntris", type=int, default=1_000_000)
+    ap.add_argument("--cpu-sample-stride", type=int, default=16)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(scenes are replicated per GPU and rays shard without exchange; see DESIGN.md). "
+              "Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import trenchanttracer_amd as tt
+    from trenchanttracer_amd import workloads
+
+    if not torch.cuda.is_available():
+        log("no GPU visible")
+        sys.exit(1)
+
+    mesh, bvh, rays, n_prim = workloads.traversal(ntris=args.ntris, log=log)
+    n = len(rays)
+    ctx = tt.Context(0)
+    scene = tt.DeviceScene(ctx, mesh, bvh)
+    info = scene.info()
+    log(f"scene on device: {info}")
+
+    d_rays = torch.from_numpy(rays).to("cuda")
+    d_hits = torch.empty((n, 4), dtype=torch.int32, device="cuda")
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+    cfg = tt.trace_cfg(variant=args.variant, lds_stack=args.lds_stack, block=args.block)
+
+    for _ in range(args.warmup):
+        scene.trace_closest(d_rays, n, d_hits, cfg, stream=sh)
+    torch.cuda.synchronize()
+    log(f"warmup done ({args.warmup} steps)")
+
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ev[i][0].record(stream)
+        scene.trace_closest(d_rays, n, d_hits, cfg, stream=sh)
+        ev[i][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = [a.elapsed_time(b) for a, b in ev]
+    kmean = float(np.mean(kernel_ms))
+    ms_per_step = wall * 1e3 / args.steps
+    mrays = n * args.steps / wall / 1e6
+    log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step wall, kernel {kmean:.3f} ms avg, {mrays:.1f} Mrays/s")
+
+    hits = d_hits.cpu().numpy()
+    hit_frac = float((hits[:, 0] >= 0).mean())
+
+    result = {
+        "metric": "Mrays/s closest-hit",
+        "value": round(mrays, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"traversal: {args.ntris}-triangle log-normal soup, SBVH alpha 1e-5, {n} rays "
+                        f"({n_prim} primary 1920x1080x2spp + {n - n_prim} incoherent), closest hit",
+            "rays": n, "triangles": args.ntris, "tri_refs": info["num_tris"], "nodes": info["num_nodes"],
+            "variant": "exact" if args.variant == 0 else "speculative", "lds_stack": args.lds_stack,
+            "block": args.block, "hit_fraction": round(hit_frac, 4),
+        },
+        "kernel_ms": round(kmean, 4),
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+
+    if not args.no_cpu_baseline:
+        # cpu_baseline leg: the oracle (single-lane restatement of the reference
+        # traversal) timed on host cores over a strided sample of the same rays;
+        # its per-ray visit counts give the algorithmic bytes, and its hits check
+        # the GPU's on the sample.
+        from oracle import oracle as O
+
+        stride = args.cpu_sample_stride
+        idx = np.arange(0, n, stride)
+        sample = rays[idx]
+        cb = O.CompactBVH(bvh.export_compact())
+        cores = min(16, os.cpu_count() or 1)
+        oh, st = cb.trace(sample, nthreads=cores)
+        cpu_mrays = len(sample) / st["seconds"] / 1e6
+        bpr = workloads.bytes_per_ray(st["inner"], st["tris"], len(sample))
+        gpu_ids = hits[idx, 0]
+        o_ids = O.hit_ids(oh)
+        mism = int((gpu_ids != o_ids).sum())
+        exact = bool(np.array_equal(hits[idx].view(np.int32), oh.view(np.int32)))
+        achieved = bpr * n / (kmean * 1e-3) / 1e9
+        result["roofline"] = {
+            "bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
+            "frac": round(achieved / 8000.0, 4), "traffic": None,
+            "bytes_per_ray": round(bpr, 1),
+            "node_visits_per_ray": round(st["inner"] / len(sample), 2),
+            "tri_tests_per_ray": round(st["tris"] / len(sample), 2),
+        }
+        result["cpu_baseline"] = {
+            "value": round(cpu_mrays, 3), "unit": "Mrays/s", "cores": cores, "kind": "port",
+            "sample": f"every {stride}th ray of the same set ({len(sample)} rays), oracle single-lane traversal, "
+                      f"{st['seconds']:.2f}s wall on {cores} threads",
+        }
+        result["parity_sample"] = {"rays": int(len(sample)), "hit_id_mismatches": mism, "bit_exact": exact}
+        log(f"cpu baseline {cpu_mrays:.2f} Mrays/s on {cores} threads; sample mismatches {mism}; "
+            f"bytes/ray {bpr:.1f}; achieved {achieved:.0f} GB/s")
+
+    print(json.dumps(result), flush=True)
+    scene.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,240 @@
+/*
+ * tt.h — C ABI of the MI355X (gfx950) rendering hot path.
+ *
+ * Plain C types only: pointers, sizes, POD structs with asserted sizes. No torch
+ * or HIP types cross this boundary (device buffers are passed as void* / typed
+ * pointers to device memory, streams as void*).
+ *
+ * Every entry point returns a tt_status (0 = TT_OK). A failing call never
+ * throws; the message of the last failure on the calling thread is returned by
+ * tt_last_error(). Host objects (tt_mesh, tt_bvh) need no GPU. Device objects
+ * (tt_ctx, tt_scene) own all device memory they allocate; the caller owns every
+ * buffer it passes in. A tt_ctx is not thread-safe: use one per host thread.
+ *
+ * Reference seams replaced (paths relative to the reference repository):
+ *   - BVH::BVH + BVH::createHolder          src/bvh/BVH.cpp:3-63,358      -> tt_bvh_build
+ *   - BVHCompact::createCompact             src/bvh/BVHCompact.cpp:88-250 -> tt_bvh_export_compact
+ *   - BVHCompact(FILE*) / BVHCompact::save  src/bvh/BVHCompact.cpp:23-79,319-375 -> tt_bvh_save / tt_bvh_load
+ *   - Renderer::init / Renderer::Renderer   src/core/Renderer.cpp:8-57    -> tt_ctx_create + tt_scene_create
+ *   - intersect(ray, bvhCompact, true)      include/geometry/IntersectKernel.cuh:78-296 -> tt_trace_closest
+ *   - intersect(ray, bvhCompact, false)     include/geometry/IntersectKernel.cuh:265-269 -> tt_trace_any
+ *   - Renderer::render / pathTracingKernel  src/core/RenderKernel.cu:97-233 -> tt_render
+ *   - hdrSample                             include/hdr/HDRKernel.cuh:6-42 -> tt_env_* (lookup + importance sampling)
+ */
+#ifndef TT_H
+#define TT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+typedef enum {
+    TT_OK = 0,
+    TT_ERR_INVALID = 1,     /* bad argument (null pointer, size out of range) */
+    TT_ERR_NOMEM = 2,       /* host or device allocation failed */
+    TT_ERR_DEVICE = 3,      /* HIP runtime error */
+    TT_ERR_IO = 4,          /* file could not be read / written / is corrupt */
+    TT_ERR_UNSUPPORTED = 5  /* feature not available (e.g. no GPU) */
+} tt_status;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *tt_last_error(void);
+/* ABI version; bumped on any layout change. */
+int tt_abi_version(void);
+
+/* ------------------------------------------------------------------ records */
+/* Ray, 32 B: {origin, tmin, direction, tmax}. Same fields as Ray
+ * (include/geometry/Ray.h:10-13); hits need tmin < t < tmax (strict). */
+typedef struct tt_ray {
+    float ox, oy, oz, tmin;
+    float dx, dy, dz, tmax;
+} tt_ray;
+
+/* Hit, 16 B: original triangle id (-1 = miss), distance t and the barycentrics
+ * (u, v) of the accepted triangle. The reference Hit (include/geometry/Hit.h:9-15)
+ * stores index + distance; u and v are added here. */
+typedef struct tt_hit {
+    int32_t tri_id;
+    float t, u, v;
+} tt_hit;
+
+/* Builder statistics (reference prints them at src/bvh/BVH.cpp:8-9,45-54 and
+ * src/bvh/BVHCompact.cpp:40-42). */
+typedef struct tt_bvh_stats {
+    int32_t num_tris;          /* input triangles */
+    int32_t num_refs;          /* triangle references in leaves (>= num_tris with spatial splits) */
+    int32_t num_duplicates;    /* numDuplicates of the reference builder */
+    int32_t num_inner;         /* inner nodes */
+    int32_t num_leaves;        /* leaf nodes */
+    int32_t max_depth;         /* deepest leaf level (root = 0) */
+    float   sah;               /* top-down SAH cost (BVHNode::computeSubtreeProbabilities) */
+    float   root_lo[3], root_hi[3];
+    double  build_seconds;
+} tt_bvh_stats;
+
+/* Sizes of the reference's compact arrays (BVHCompact.h:42-48), in elements. */
+typedef struct tt_compact_sizes {
+    uint32_t nodes;        /* float4 entries (4 per inner node) */
+    uint32_t vertices;     /* float4 entries (3 per triangle + 1 terminator per leaf) */
+    uint32_t normals;      /* float4 entries (same as vertices) */
+    uint32_t tri_indices;  /* int32 entries (same as vertices) */
+    uint32_t mat_indices;  /* int32 entries (same as vertices) */
+    uint32_t leaf_count;
+    uint32_t tri_count;
+} tt_compact_sizes;
+
+/* ------------------------------------------------------------------ host mesh */
+typedef struct tt_mesh tt_mesh;
+
+/* Flat triangle mesh, as Scene (src/core/Scene.cpp:7-23): vertices, normals,
+ * per-triangle vertex indices, normal indices and material index. normals /
+ * tri_normals / tri_mats may be NULL (face normals / material 0 are used). */
+tt_status tt_mesh_create(const float *verts, int32_t nverts,
+                         const int32_t *tris, int32_t ntris,
+                         const float *normals, int32_t nnormals,
+                         const int32_t *tri_normals, const int32_t *tri_mats,
+                         tt_mesh **out);
+void tt_mesh_destroy(tt_mesh *m);
+tt_status tt_mesh_info(const tt_mesh *m, int32_t *nverts, int32_t *ntris, int32_t *nnormals);
+/* Copies out any of the arrays (pass NULL to skip one). */
+tt_status tt_mesh_get(const tt_mesh *m, float *verts, int32_t *tris, float *normals,
+                      int32_t *tri_normals, int32_t *tri_mats);
+/* Appends src (vertex/normal indices rebased, material ids offset by mat_offset). */
+tt_status tt_mesh_append(tt_mesh *dst, const tt_mesh *src, int32_t mat_offset);
+
+/* Procedural generators (deterministic for a given seed).
+ * Soup: ntris equilateral triangles, centroid ~ U[0,1]^3, side ~ LogNormal(mu, sigma),
+ *       uniformly random orientation; face normals. */
+tt_status tt_mesh_gen_soup(int32_t ntris, uint64_t seed, float mu, float sigma, tt_mesh **out);
+/* Icosphere subdivided `subdiv` times (20*4^subdiv tris), outward CCW winding,
+ * radius*(1 + amp*fbm(p)) radial displacement (amp = 0 disables; `octaves` of
+ * value noise), centred at c; smooth vertex normals. */
+tt_status tt_mesh_gen_icosphere(int32_t subdiv, float radius, const float c[3],
+                                float amp, int32_t octaves, uint64_t seed,
+                                int32_t mat, tt_mesh **out);
+/* Parallelogram p0,p1,p2 as two triangles (p0,p1,p2) and (p1,p1+p2-p0,p2)
+ * (reference Plane, src/geometry/Plane.cpp; winding kept). */
+tt_status tt_mesh_gen_quad(const float p0[3], const float p1[3], const float p2[3],
+                           int32_t mat, tt_mesh **out);
+/* Per-face material assignment: fractions f[0..k) of faces get mats[0..k). */
+tt_status tt_mesh_assign_materials(tt_mesh *m, int32_t k, const float *fractions,
+                                   const int32_t *mats, uint64_t seed);
+
+/* Ray sets of the traversal workload.
+ * Primary: width*height*spp pinhole rays (eye, look-at target, fovx degrees),
+ *   jittered in-pixel, ordered in 8x8 pixel tiles (spp-major inside a tile).
+ * Incoherent: n rays from uniform points on uniformly chosen triangles, cosine
+ *   distributed about the front normal normalize(e1 x e2). */
+tt_status tt_rays_primary(int32_t width, int32_t height, int32_t spp, const float eye[3],
+                          const float target[3], float fovx_deg, float tmin, float tmax,
+                          uint64_t seed, tt_ray *out);
+tt_status tt_rays_incoherent(const tt_mesh *m, int64_t n, float tmin, float tmax,
+                             uint64_t seed, tt_ray *out);
+
+/* ------------------------------------------------------------------ host SBVH */
+typedef struct tt_bvh tt_bvh;
+
+/* SBVH build (Stich et al. 2009), decision-for-decision as BVH::BVH
+ * (src/bvh/BVH.cpp:3-341): SAH node/tri cost 1/1, 32 spatial bins, max depth 64,
+ * spatial splits above depth 48 with overlap >= split_alpha * root area.
+ * nthreads <= 0 uses all cores; the result does not depend on nthreads. */
+tt_status tt_bvh_build(const tt_mesh *m, float split_alpha, int32_t nthreads, tt_bvh **out);
+void tt_bvh_destroy(tt_bvh *b);
+tt_status tt_bvh_get_stats(const tt_bvh *b, tt_bvh_stats *out);
+
+/* Reference compact layout (BVHCompact::createCompact, src/bvh/BVHCompact.cpp:88-250):
+ * query sizes with all pointers NULL, then copy. nodes: 16 int32 per inner node
+ * (child = float4 offset, leaf = ~vertex offset); vertices/normals: float4 per
+ * vertex, leaf terminator {0x80000000,0,0,0}; tri/mat indices padded 3 per tri. */
+tt_status tt_bvh_export_compact(const tt_bvh *b, tt_compact_sizes *sizes,
+                                int32_t *nodes, float *vertices, float *normals,
+                                int32_t *tri_indices, int32_t *mat_indices);
+
+/* Device layout: nodes (64 B each: c0.xy bounds, c1.xy bounds, z bounds, {c0,c1,n0,n1})
+ * with inner child = node index, leaf child = ~first triangle, n = leaf triangle
+ * count; triangles 48 B each: {v0, id}, {v1-v0, mat}, {v2-v0, 0}. */
+tt_status tt_bvh_export_device_layout(const tt_bvh *b, int64_t *num_nodes, int64_t *num_tris,
+                                      int32_t *nodes, float *tris);
+
+/* Cache file: the reference's bvh.cache format (7 x u32 header + the compact
+ * arrays, src/bvh/BVHCompact.cpp:319-375) followed by a trailer holding the
+ * device layout. tt_bvh_load restores a tt_bvh that can be passed to
+ * tt_scene_create; it fails with TT_ERR_IO on a short or corrupt file. */
+tt_status tt_bvh_save(const tt_bvh *b, const char *path);
+tt_status tt_bvh_load(const char *path, tt_bvh **out);
+
+/* ------------------------------------------------------------------ device */
+typedef struct tt_ctx tt_ctx;
+typedef struct tt_scene tt_scene;
+
+tt_status tt_ctx_create(int32_t device, tt_ctx **out);
+void tt_ctx_destroy(tt_ctx *ctx);
+/* Synchronises the context's stream. */
+tt_status tt_ctx_sync(tt_ctx *ctx);
+
+/* Materials. REF_* reproduce the reference samplers (reference render mode);
+ * the others are the physically based set (path-tracer mode). */
+typedef enum {
+    TT_MAT_DIFFUSE = 0,     /* Lambert; ref: diffSample (include/material/DiffImpl.cuh:12-36) */
+    TT_MAT_CONDUCTOR = 1,   /* GGX conductor (VNDF sampling); ref mode: metalSample Phong lobe */
+    TT_MAT_DIELECTRIC = 2,  /* smooth dielectric, Fresnel; ref mode: refrSample (IOR 1.4) */
+    TT_MAT_EMITTER = 3      /* emission only (black BSDF) */
+} tt_mat_type;
+
+typedef struct tt_material {
+    int32_t type;           /* tt_mat_type */
+    float albedo[3];        /* diffuse colour / conductor F0 colour / dielectric tint */
+    float emission[3];
+    float roughness;        /* GGX alpha = roughness^2 (conductor) */
+    float ior;              /* dielectric */
+    float pad[3];
+} tt_material;              /* 48 B */
+
+/* Uploads the SBVH device layout, per-triangle shading normals and materials.
+ * The env map (RGB float, width x height, equirectangular, may be NULL = black)
+ * gets its marginal/conditional CDF built on the GPU (FP64 prefix sums). */
+tt_status tt_scene_create(tt_ctx *ctx, const tt_mesh *mesh, const tt_bvh *bvh,
+                          const tt_material *mats, int32_t nmats,
+                          const float *env_rgb, int32_t env_w, int32_t env_h,
+                          float env_scale, tt_scene **out);
+void tt_scene_destroy(tt_scene *s);
+tt_status tt_scene_info(const tt_scene *s, int64_t *num_nodes, int64_t *num_tris, int64_t *device_bytes);
+/* Copies the env-map sampling distribution built on the GPU: marginal CDF over
+ * rows (env_h + 1 floats), conditional CDFs (env_h x (env_w + 1) floats) and the
+ * FP64 total weight (sum of luminance * sin(theta) over texels). */
+tt_status tt_env_export_cdf(const tt_scene *s, float *marg, float *cond, double *total);
+
+/* Traversal launch tuning. variant: 0 = per-lane exact (single-lane order; hit ids
+ * bit-identical to the host reference traversal), 1 = speculative while-while
+ * (Aila-Laine postponed leaf + wave ballot). lds_stack: LDS stack entries per lane
+ * (8, 16, 24 or 32); block: threads per block (64..1024, multiple of 64). */
+typedef struct tt_trace_cfg {
+    int32_t variant;
+    int32_t lds_stack;
+    int32_t block;
+    int32_t persistent;      /* 0 = one ray per lane; >0 = persistent waves fetching this many rays per wave batch */
+} tt_trace_cfg;
+
+/* Closest hit for n rays. rays / hits are DEVICE pointers; stream is a hipStream_t
+ * (NULL = the context stream). Asynchronous: synchronise before reading hits.
+ * cfg may be NULL (autotuned default). */
+tt_status tt_trace_closest(tt_scene *s, const tt_ray *rays, int64_t n, tt_hit *hits,
+                           const tt_trace_cfg *cfg, void *stream);
+/* Any hit (shadow rays): hit_flags[i] = 1 if some triangle is hit in (tmin, tmax). */
+tt_status tt_trace_any(tt_scene *s, const tt_ray *rays, int64_t n, int32_t *hit_flags,
+                       const tt_trace_cfg *cfg, void *stream);
+/* Host-buffer convenience wrappers (copy in, trace, copy out, synchronise). */
+tt_status tt_trace_closest_host(tt_scene *s, const tt_ray *rays, int64_t n, tt_hit *hits,
+                                const tt_trace_cfg *cfg);
+tt_status tt_trace_any_host(tt_scene *s, const tt_ray *rays, int64_t n, int32_t *hit_flags,
+                            const tt_trace_cfg *cfg);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* TT_H */

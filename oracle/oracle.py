@@ -1,0 +1,110 @@
+"""ctypes wrapper of the oracle (oracle/_build/liboracle.so).
+
+Test infrastructure: imported only by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker. Never imported by the product.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+
+
+class o_bvh(C.Structure):
+    _fields_ = [("nodes", C.c_void_p), ("vertices", C.c_void_p), ("normals", C.c_void_p),
+                ("tri_idx", C.c_void_p), ("mat_idx", C.c_void_p), ("n_nodes_f4", C.c_int64),
+                ("n_vertices", C.c_int64)]
+
+
+class o_counts(C.Structure):
+    _fields_ = [("inner", C.c_int64), ("tris", C.c_int64), ("leaves", C.c_int64)]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        from . import build as _b  # noqa
+    lib = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    lib.o_philox4x32_10.argtypes = [P, P, P]
+    lib.o_philox4x32_10.restype = None
+    lib.o_intersect.argtypes = [C.POINTER(o_bvh), P, C.c_int, P, P, C.POINTER(o_counts)]
+    lib.o_intersect.restype = C.c_int
+    lib.o_brute_batch.argtypes = [P, P, C.c_int32, P, C.c_int64, P]
+    lib.o_brute_batch.restype = None
+    lib.o_trace_counts.argtypes = [C.POINTER(o_bvh), P, C.c_int64, C.c_int, P, P]
+    lib.o_trace_counts.restype = None
+    lib.o_trace_batch.argtypes = [C.POINTER(o_bvh), P, C.c_int64, C.c_int, P, C.c_int, C.POINTER(o_counts)]
+    lib.o_trace_batch.restype = C.c_double
+    lib.o_tri_test.argtypes = [P, P, P, P, C.c_float, P, P, P]
+    lib.o_tri_test.restype = C.c_int
+    for name in ("o_sinf", "o_cosf", "o_logf", "o_expf"):
+        if hasattr(lib, name):
+            getattr(lib, name).argtypes = [C.c_float]
+            getattr(lib, name).restype = C.c_float
+    if hasattr(lib, "o_powf"):
+        lib.o_powf.argtypes = [C.c_float, C.c_float]
+        lib.o_powf.restype = C.c_float
+    return lib
+
+
+lib = _load()
+
+
+def _p(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def philox(ctr, key):
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    o = np.empty(4, np.uint32)
+    lib.o_philox4x32_10(_p(c), _p(k), _p(o))
+    return o
+
+
+class CompactBVH:
+    """Holds the reference compact arrays (numpy) and the C view of them."""
+
+    def __init__(self, compact):
+        self.a = {k: np.ascontiguousarray(v) for k, v in compact.items() if isinstance(v, np.ndarray)}
+        self.c = o_bvh(_p(self.a["nodes"]), _p(self.a["vertices"]), _p(self.a["normals"]), _p(self.a["tri_idx"]),
+                       _p(self.a["mat_idx"]), len(self.a["nodes"]), len(self.a["vertices"]))
+
+    def trace(self, rays, any_hit=False, nthreads=None):
+        r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        hits = np.empty((len(r), 4), np.float32)
+        cnt = o_counts()
+        nt = nthreads or min(8, os.cpu_count() or 1)
+        secs = lib.o_trace_batch(C.byref(self.c), _p(r), len(r), 1 if any_hit else 0, _p(hits), nt, C.byref(cnt))
+        return hits, {"inner": cnt.inner, "tris": cnt.tris, "leaves": cnt.leaves, "seconds": secs, "threads": nt}
+
+    def trace_counts(self, rays, any_hit=False):
+        r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        hits = np.empty((len(r), 4), np.float32)
+        cnt = np.empty((len(r), 3), np.int64)
+        lib.o_trace_counts(C.byref(self.c), _p(r), len(r), 1 if any_hit else 0, _p(hits), _p(cnt))
+        return hits, cnt
+
+
+def brute(verts, tris, rays):
+    v = np.ascontiguousarray(verts, dtype=np.float32)
+    t = np.ascontiguousarray(tris, dtype=np.int32)
+    r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+    hits = np.empty((len(r), 4), np.float32)
+    lib.o_brute_batch(_p(v), _p(t), len(t), _p(r), len(r), _p(hits))
+    return hits
+
+
+def tri_test(v0, v1, v2, ray, tmax=None):
+    a = [np.ascontiguousarray(x, dtype=np.float32) for x in (v0, v1, v2)]
+    r = np.ascontiguousarray(ray, dtype=np.float32)
+    t, u, v = C.c_float(), C.c_float(), C.c_float()
+    ok = lib.o_tri_test(_p(a[0]), _p(a[1]), _p(a[2]), _p(r), float(r[7] if tmax is None else tmax),
+                        C.byref(t), C.byref(u), C.byref(v))
+    return bool(ok), t.value, u.value, v.value
+
+
+def hit_ids(h):
+    return np.ascontiguousarray(h[:, 0]).view(np.int32)

@@ -1,0 +1,87 @@
+/*
+ * tt_oracle.h — CPU restatement of the reference rendering hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker: it is loaded by the
+ * tests, by __graft_entry__.smoke() and by the cpu_baseline leg of bench.py,
+ * never by the product library (trenchanttracer_amd), which has no CPU
+ * fallback. Built with gcc -O2 -ffp-contract=off (no -march), so each float
+ * operation is one IEEE binary32 operation in source order.
+ *
+ * Pinning (what checks each function, see tests/):
+ *   o_philox4x32_10      Random123 known-answer vectors (tests/test_oracle.py)
+ *   o_intersect          == o_brute_closest on seeded scenes (argmin over all
+ *                        triangles of the same triangle test; independent of
+ *                        the tree) + closed-form triangle cases
+ *   o_tri_test           closed forms: det = 2*area*cos, back faces rejected,
+ *                        edge inclusive, t = tmin exclusive, |det| <= 1e-5 rejected
+ *   o_brute_closest      brute force itself; checked on hand-made scenes with
+ *                        known answers
+ *   o_trace_batch        == o_intersect per ray (threads only split the work)
+ *   o_sinf/o_cosf/...    float64 numpy within stated ulp bounds
+ *   o_render_ref         restated megakernel; checked by white furnace and
+ *                        closed-form cases (tests/test_render_oracle.py)
+ */
+#ifndef TT_ORACLE_H
+#define TT_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { float ox, oy, oz, tmin, dx, dy, dz, tmax; } o_ray;
+typedef struct { int32_t tri; float t, u, v; } o_hit;
+typedef struct { int64_t inner, tris, leaves; } o_counts;
+
+/* Reference compact BVH (BVHCompact.h:30-48): nodes as int32 (4 per float4),
+ * vertices / normals as float4, index arrays int32. */
+typedef struct {
+    const int32_t *nodes;
+    const float *vertices;
+    const float *normals;
+    const int32_t *tri_idx;
+    const int32_t *mat_idx;
+    int64_t n_nodes_f4;   /* float4 entries in nodes */
+    int64_t n_vertices;   /* float4 entries in vertices */
+} o_bvh;
+
+void o_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* Möller–Trumbore as IntersectKernel.cuh:230-254. Returns 1 if accepted with
+ * tmin < t < tmax. */
+int o_tri_test(const float v0[3], const float v1[3], const float v2[3], const o_ray *r, float tmax,
+               float *t, float *u, float *v);
+
+/* Single-lane restatement of intersect() (IntersectKernel.cuh:78-296). any != 0
+ * stops at the first accepted triangle. normal_out (may be NULL) receives the
+ * interpolated unnormalised normal. counts (may be NULL) accumulate node visits,
+ * triangle tests and processed leaves. Returns 0, or -1 on stack overflow. */
+int o_intersect(const o_bvh *b, const o_ray *r, int any, o_hit *out, float normal_out[3], o_counts *counts);
+
+/* Argmin over all triangles (v0.x -0.0 rewritten to +0.0 as the compact layout
+ * does), first index wins ties. */
+void o_brute_closest(const float *verts, const int32_t *tris, int32_t ntris, const o_ray *r, o_hit *out);
+
+void o_brute_batch(const float *verts, const int32_t *tris, int32_t ntris, const o_ray *rays, int64_t n,
+                   o_hit *hits);
+void o_trace_counts(const o_bvh *b, const o_ray *rays, int64_t n, int any, o_hit *hits, int64_t *counts3);
+
+/* Batch traversal on nthreads host threads (static contiguous chunks).
+ * counts_out (may be NULL) receives the totals. Returns wall seconds. */
+double o_trace_batch(const o_bvh *b, const o_ray *rays, int64_t n, int any, o_hit *hits,
+                     int nthreads, o_counts *counts_out);
+
+/* Float math used by the restated samplers (Cody–Waite reduction + Taylor
+ * polynomials; |error| <= 2 ulp on the documented ranges). */
+float o_sinf(float x);
+float o_cosf(float x);
+float o_logf(float x);
+float o_expf(float x);
+float o_powf(float x, float y);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,149 @@
+"""GPU traversal parity: HIP kernels through the C ABI vs the oracle."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def _scene(tt, ctx, mesh):
+    b = tt.BVH.build(mesh)
+    return b, tt.DeviceScene(ctx, mesh, b)
+
+
+def _rays(tt, mesh, n_prim_w=40, n_prim_h=30, n_inc=3000, seed=3):
+    return np.concatenate([tt.rays_primary(n_prim_w, n_prim_h, 2, (0.5, 0.5, -1.0), (0.5, 0.5, 0.0), seed=seed),
+                           tt.rays_incoherent(mesh, n_inc, seed=seed)])
+
+
+def _check_exact(oracle, oh, ids, t, u, v):
+    o_ids = oracle.hit_ids(oh)
+    assert np.array_equal(ids, o_ids), f"{(ids != o_ids).sum()} hit id mismatches"
+    assert np.array_equal(t.view(np.int32), oh[:, 1].view(np.int32))
+    assert np.array_equal(u.view(np.int32), oh[:, 2].view(np.int32))
+    assert np.array_equal(v.view(np.int32), oh[:, 3].view(np.int32))
+
+
+@pytest.mark.parametrize("ntris,seed", [(1, 1), (2, 2), (300, 3), (5000, 4), (60000, 5)])
+def test_trace_exact_matches_oracle(tt, oracle, ctx, ntris, seed):
+    _say(f"trace exact ntris={ntris} seed={seed}")
+    m = tt.Mesh.soup(ntris, seed=seed, mu=float(np.log(0.2 if ntris < 100 else 0.02)))
+    b, s = _scene(tt, ctx, m)
+    rays = _rays(tt, m, seed=seed)
+    oh, _ = oracle.CompactBVH(b.export_compact()).trace(rays)
+    for lds in (8, 16, 24, 32):
+        for block in (64, 128, 256):
+            ids, t, u, v = s.intersect(rays, tt.trace_cfg(0, lds, block))
+            _check_exact(oracle, oh, ids, t, u, v)
+    s.close()
+
+
+def test_trace_speculative_near_ties_only(tt, oracle, ctx):
+    _say("trace speculative")
+    m = tt.Mesh.soup(50000, seed=7)
+    b, s = _scene(tt, ctx, m)
+    rays = _rays(tt, m, 120, 90, 20000, seed=7)
+    oh, _ = oracle.CompactBVH(b.export_compact()).trace(rays)
+    ids, t, u, v = s.intersect(rays, tt.trace_cfg(1, 16, 128))
+    o_ids = oracle.hit_ids(oh)
+    bad = np.nonzero(ids != o_ids)[0]
+    # a differing id must be a near tie: both distances within 1e-6 relative
+    for i in bad:
+        assert ids[i] >= 0 and o_ids[i] >= 0
+        assert abs(t[i] - oh[i, 1]) <= 1e-6 * max(1.0, abs(oh[i, 1]))
+    assert len(bad) <= 1e-4 * len(rays)
+    s.close()
+
+
+def test_any_hit_matches_closest(tt, oracle, ctx):
+    _say("trace any")
+    m = tt.Mesh.soup(20000, seed=8)
+    b, s = _scene(tt, ctx, m)
+    rays = _rays(tt, m, seed=8)
+    oh, _ = oracle.CompactBVH(b.export_compact()).trace(rays)
+    oa, _ = oracle.CompactBVH(b.export_compact()).trace(rays, any_hit=True)
+    for variant in (0, 1):
+        flags = s.occluded(rays, tt.trace_cfg(variant, 16, 128))
+        assert np.array_equal(flags, (oracle.hit_ids(oh) >= 0).astype(np.int32))
+        assert np.array_equal(flags, (oracle.hit_ids(oa) >= 0).astype(np.int32))
+    s.close()
+
+
+def test_ragged_and_empty_inputs(tt, oracle, ctx):
+    _say("trace ragged/empty")
+    m = tt.Mesh.soup(3000, seed=9)
+    b, s = _scene(tt, ctx, m)
+    rays = _rays(tt, m, seed=9)
+    cb = oracle.CompactBVH(b.export_compact())
+    for n in (1, 63, 65, 127, 129, 1000):
+        oh, _ = cb.trace(rays[:n])
+        ids, t, u, v = s.intersect(rays[:n])
+        _check_exact(oracle, oh, ids, t, u, v)
+    ids, *_ = s.intersect(rays[:0])
+    assert len(ids) == 0
+    s.close()
+    # empty scene: every ray misses with t = tmax
+    e = tt.Mesh.from_arrays(np.zeros((0, 3)), np.zeros((0, 3)))
+    b, s = _scene(tt, ctx, e)
+    ids, t, u, v = s.intersect(rays[:100])
+    assert (ids == -1).all() and np.array_equal(t, rays[:100, 7])
+    s.close()
+
+
+def test_icosphere_and_cornell_exact(tt, oracle, ctx):
+    _say("trace icosphere + cornell")
+    sphere = tt.Mesh.icosphere(5, 0.3, (0.5, 0.5, 0.5), amp=0.05, octaves=6, seed=3)
+    cornell = tt.Mesh.quad([-1.5, -1, -1], [1.5, -1, -1], [-1.5, 1, -1])
+    cornell.append(tt.Mesh.quad([-1.5, -1, -1], [-1.5, 1, -1], [-1.5, -1, 4]))
+    cornell.append(tt.Mesh.icosphere(4, 0.3, (0.0, -0.7, 1.5)))
+    for m in (sphere, cornell):
+        b, s = _scene(tt, ctx, m)
+        rays = np.concatenate([tt.rays_primary(64, 64, 1, (0.0, 0.2, 5.0), (0.0, -0.2, 0.0), seed=4),
+                               tt.rays_incoherent(m, 4000, seed=4)])
+        oh, _ = oracle.CompactBVH(b.export_compact()).trace(rays)
+        ids, t, u, v = s.intersect(rays)
+        _check_exact(oracle, oh, ids, t, u, v)
+        s.close()
+
+
+def test_device_pointer_entry_point(tt, oracle, ctx):
+    import torch
+    _say("trace device pointers")
+    m = tt.Mesh.soup(10000, seed=10)
+    b, s = _scene(tt, ctx, m)
+    rays = _rays(tt, m, seed=10)
+    d_r = torch.from_numpy(rays).cuda()
+    d_h = torch.empty((len(rays), 4), dtype=torch.int32, device="cuda")
+    st = torch.cuda.current_stream()
+    s.trace_closest(d_r, len(rays), d_h, tt.trace_cfg(0, 16, 128), stream=st.cuda_stream)
+    torch.cuda.synchronize()
+    h = d_h.cpu().numpy()
+    oh, _ = oracle.CompactBVH(b.export_compact()).trace(rays)
+    assert np.array_equal(h.view(np.int32), oh.view(np.int32))
+    s.close()
+
+
+def test_full_traversal_workload_bit_exact(tt, oracle, ctx):
+    """1M-triangle soup, 8,294,400 rays: every hit bit-identical to the oracle."""
+    from trenchanttracer_amd import workloads
+    _say("full traversal workload: building")
+    mesh, bvh, rays, n_prim = workloads.traversal(log=_say)
+    s = tt.DeviceScene(ctx, mesh, bvh)
+    ids, t, u, v = s.intersect(rays, tt.trace_cfg(0, 16, 128))
+    _say("full traversal workload: oracle")
+    oh, st = oracle.CompactBVH(bvh.export_compact()).trace(rays, nthreads=min(16, os.cpu_count() or 1))
+    _check_exact(oracle, oh, ids, t, u, v)
+    assert 0.05 < (ids >= 0).mean() < 0.95
+    # speculative variant: only near ties may differ
+    ids1, t1, *_ = s.intersect(rays, tt.trace_cfg(1, 16, 128))
+    bad = np.nonzero(ids1 != ids)[0]
+    _say(f"speculative variant: {len(bad)} differing ids of {len(rays)}")
+    for i in bad[:1000]:
+        assert abs(t1[i] - t[i]) <= 1e-6 * max(1.0, abs(t[i]))
+    s.close()

@@ -1,0 +1,291 @@
+"""MI355X-native rendering hot path of TrenchantTracer.
+
+Python mirror of the reference's host interface over the C ABI (include/tt.h):
+
+    Mesh        ~ Scene (src/core/Scene.cpp): vertices, normals, triangles, materials
+    BVH         ~ BVH + BVHCompact (src/bvh/BVH.cpp, src/bvh/BVHCompact.cpp)
+    Context     ~ device selection + stream (src/main.cpp:97-98)
+    DeviceScene ~ Renderer's device copies (src/core/Renderer.cpp:8-57)
+    DeviceScene.intersect / trace_closest / trace_any ~ intersect() (IntersectKernel.cuh:78)
+
+Device entry points run the HIP kernels of libtt.so; they raise if the library
+or a GPU is missing (there is no CPU fallback).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+from ._native import TTError, check, lib, tt_trace_cfg
+
+__all__ = ["Mesh", "BVH", "Context", "DeviceScene", "TTError", "rays_primary", "rays_incoherent",
+           "trace_cfg", "MAT_DIFFUSE", "MAT_CONDUCTOR", "MAT_DIELECTRIC", "MAT_EMITTER", "material"]
+
+MAT_DIFFUSE, MAT_CONDUCTOR, MAT_DIELECTRIC, MAT_EMITTER = 0, 1, 2, 3
+
+
+def _fptr(a):
+    return a.ctypes.data_as(N.PF)
+
+
+def _vp(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def _f3(x):
+    return (C.c_float * 3)(*[float(v) for v in x])
+
+
+class Mesh:
+    """Host triangle mesh (flat, as the reference Scene)."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle) if not isinstance(handle, C.c_void_p) else handle
+
+    @classmethod
+    def from_arrays(cls, verts, tris, normals=None, tri_normals=None, tri_mats=None):
+        v = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 3)
+        t = np.ascontiguousarray(tris, dtype=np.int32).reshape(-1, 3)
+        n = None if normals is None else np.ascontiguousarray(normals, dtype=np.float32).reshape(-1, 3)
+        tn = None if tri_normals is None else np.ascontiguousarray(tri_normals, dtype=np.int32).reshape(-1, 3)
+        tm = None if tri_mats is None else np.ascontiguousarray(tri_mats, dtype=np.int32).reshape(-1)
+        h = C.c_void_p()
+        check(lib.tt_mesh_create(_vp(v), len(v), _vp(t), len(t), _vp(n), 0 if n is None else len(n), _vp(tn),
+                                 _vp(tm), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def soup(cls, ntris, seed=1, mu=float(np.log(0.01)), sigma=0.5):
+        h = C.c_void_p()
+        check(lib.tt_mesh_gen_soup(int(ntris), int(seed), float(mu), float(sigma), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def icosphere(cls, subdiv, radius=1.0, center=(0, 0, 0), amp=0.0, octaves=6, seed=1, mat=0):
+        h = C.c_void_p()
+        check(lib.tt_mesh_gen_icosphere(int(subdiv), float(radius), _f3(center), float(amp), int(octaves),
+                                        int(seed), int(mat), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def quad(cls, p0, p1, p2, mat=0):
+        h = C.c_void_p()
+        check(lib.tt_mesh_gen_quad(_f3(p0), _f3(p1), _f3(p2), int(mat), C.byref(h)))
+        return cls(h)
+
+    def append(self, other, mat_offset=0):
+        check(lib.tt_mesh_append(self._h, other._h, int(mat_offset)))
+        return self
+
+    def assign_materials(self, fractions, mats, seed=1):
+        f = np.ascontiguousarray(fractions, dtype=np.float32)
+        m = np.ascontiguousarray(mats, dtype=np.int32)
+        check(lib.tt_mesh_assign_materials(self._h, len(f), _fptr(f), m.ctypes.data_as(N.PI32), int(seed)))
+
+    def info(self):
+        nv, nt, nn = C.c_int32(), C.c_int32(), C.c_int32()
+        check(lib.tt_mesh_info(self._h, C.byref(nv), C.byref(nt), C.byref(nn)))
+        return nv.value, nt.value, nn.value
+
+    @property
+    def ntris(self):
+        return self.info()[1]
+
+    def arrays(self):
+        nv, nt, nn = self.info()
+        v = np.empty((nv, 3), np.float32)
+        t = np.empty((nt, 3), np.int32)
+        n = np.empty((nn, 3), np.float32)
+        tn = np.empty((nt, 3), np.int32)
+        tm = np.empty(nt, np.int32)
+        check(lib.tt_mesh_get(self._h, _vp(v), _vp(t), _vp(n), _vp(tn), _vp(tm)))
+        return {"verts": v, "tris": t, "normals": n, "tri_normals": tn, "tri_mats": tm}
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib.tt_mesh_destroy(h)
+            self._h = None
+
+
+class BVH:
+    """Host SBVH: BVH::BVH (src/bvh/BVH.cpp:3-63) + BVHCompact (BVHCompact.cpp:88-250)."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @classmethod
+    def build(cls, mesh, split_alpha=1.0e-5, nthreads=0):
+        h = C.c_void_p()
+        check(lib.tt_bvh_build(mesh._h, float(split_alpha), int(nthreads), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def load(cls, path):
+        h = C.c_void_p()
+        check(lib.tt_bvh_load(str(path).encode(), C.byref(h)))
+        return cls(h)
+
+    def save(self, path):
+        check(lib.tt_bvh_save(self._h, str(path).encode()))
+
+    def stats(self):
+        s = N.tt_bvh_stats()
+        check(lib.tt_bvh_get_stats(self._h, C.byref(s)))
+        return {"num_tris": s.num_tris, "num_refs": s.num_refs, "num_duplicates": s.num_duplicates,
+                "num_inner": s.num_inner, "num_leaves": s.num_leaves, "max_depth": s.max_depth,
+                "sah": s.sah, "root_lo": list(s.root_lo), "root_hi": list(s.root_hi),
+                "build_seconds": s.build_seconds}
+
+    def export_compact(self):
+        """Reference compact arrays (nodes int32 [n,4], vertices/normals float32 [n,4], tri/mat int32)."""
+        sz = N.tt_compact_sizes()
+        check(lib.tt_bvh_export_compact(self._h, C.byref(sz), None, None, None, None, None))
+        nodes = np.empty((sz.nodes, 4), np.int32)
+        verts = np.empty((sz.vertices, 4), np.float32)
+        norms = np.empty((sz.normals, 4), np.float32)
+        tri = np.empty(sz.tri_indices, np.int32)
+        mat = np.empty(sz.mat_indices, np.int32)
+        check(lib.tt_bvh_export_compact(self._h, C.byref(sz), _vp(nodes), _vp(verts), _vp(norms), _vp(tri),
+                                        _vp(mat)))
+        return {"nodes": nodes, "vertices": verts, "normals": norms, "tri_idx": tri, "mat_idx": mat,
+                "leaf_count": sz.leaf_count, "tri_count": sz.tri_count}
+
+    def device_layout(self):
+        nn, nt = C.c_int64(), C.c_int64()
+        check(lib.tt_bvh_export_device_layout(self._h, C.byref(nn), C.byref(nt), None, None))
+        nodes = np.empty((nn.value, 16), np.int32)
+        tris = np.empty((nt.value, 12), np.float32)
+        check(lib.tt_bvh_export_device_layout(self._h, None, None, _vp(nodes), _vp(tris)))
+        return nodes, tris
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib.tt_bvh_destroy(h)
+            self._h = None
+
+
+def rays_primary(width, height, spp, eye, target, fovx=45.0, tmin=0.0, tmax=1e20, seed=2):
+    out = np.empty((width * height * spp, 8), np.float32)
+    check(lib.tt_rays_primary(int(width), int(height), int(spp), _f3(eye), _f3(target), float(fovx), float(tmin),
+                              float(tmax), int(seed), _vp(out)))
+    return out
+
+
+def rays_incoherent(mesh, n, tmin=1e-4, tmax=1e20, seed=2):
+    out = np.empty((int(n), 8), np.float32)
+    check(lib.tt_rays_incoherent(mesh._h, int(n), float(tmin), float(tmax), int(seed), _vp(out)))
+    return out
+
+
+def trace_cfg(variant=0, lds_stack=16, block=128, persistent=0):
+    return tt_trace_cfg(int(variant), int(lds_stack), int(block), int(persistent))
+
+
+def material(type, albedo=(0.8, 0.8, 0.8), emission=(0, 0, 0), roughness=0.0, ior=1.5):
+    m = N.tt_material()
+    m.type = int(type)
+    m.albedo[:] = [float(x) for x in albedo]
+    m.emission[:] = [float(x) for x in emission]
+    m.roughness = float(roughness)
+    m.ior = float(ior)
+    return m
+
+
+class Context:
+    """One GPU and its stream (tt_ctx)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        check(lib.tt_ctx_create(int(device), C.byref(h)))
+        self._h = h
+
+    def sync(self):
+        check(lib.tt_ctx_sync(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib.tt_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+def _dev_ptr(x):
+    """Device address of a torch tensor or an int."""
+    if hasattr(x, "data_ptr"):
+        if not x.is_cuda:
+            raise ValueError("expected a device tensor")
+        if not x.is_contiguous():
+            raise ValueError("expected a contiguous tensor")
+        return C.c_void_p(x.data_ptr())
+    return C.c_void_p(int(x))
+
+
+class DeviceScene:
+    """Device copy of a built scene (tt_scene): SBVH layout, normals, materials, env map."""
+
+    def __init__(self, ctx, mesh, bvh, materials=None, env=None, env_scale=1.0):
+        self.ctx = ctx
+        mats = None
+        nm = 0
+        if materials:
+            nm = len(materials)
+            mats = (N.tt_material * nm)(*materials)
+        env_arr = None
+        ew = eh = 0
+        if env is not None:
+            env_arr = np.ascontiguousarray(env, dtype=np.float32)
+            eh, ew = env_arr.shape[:2]
+        h = C.c_void_p()
+        check(lib.tt_scene_create(ctx._h, mesh._h if mesh is not None else None, bvh._h,
+                                  C.cast(mats, C.c_void_p) if mats is not None else None, nm,
+                                  _vp(env_arr) if env_arr is not None else None, ew, eh, float(env_scale),
+                                  C.byref(h)))
+        self._h = h
+
+    def info(self):
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        check(lib.tt_scene_info(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return {"num_nodes": a.value, "num_tris": b.value, "device_bytes": c.value}
+
+    def env_cdf(self, w, h):
+        marg = np.empty(h + 1, np.float32)
+        cond = np.empty((h, w + 1), np.float32)
+        tot = C.c_double()
+        check(lib.tt_env_export_cdf(self._h, _vp(marg), _vp(cond), C.byref(tot)))
+        return marg, cond, tot.value
+
+    def trace_closest(self, rays, n, hits, cfg=None, stream=None):
+        """rays / hits: device tensors (n x 8 float32 / n x 4 int32|float32) or device addresses."""
+        check(lib.tt_trace_closest(self._h, _dev_ptr(rays), int(n), _dev_ptr(hits),
+                                   C.byref(cfg) if cfg is not None else None,
+                                   C.c_void_p(stream) if stream else None))
+
+    def trace_any(self, rays, n, flags, cfg=None, stream=None):
+        check(lib.tt_trace_any(self._h, _dev_ptr(rays), int(n), _dev_ptr(flags),
+                               C.byref(cfg) if cfg is not None else None,
+                               C.c_void_p(stream) if stream else None))
+
+    def intersect(self, rays, cfg=None):
+        """Host arrays in and out: rays (n, 8) float32 -> (tri_id int32, t, u, v float32)."""
+        r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        out = np.empty((len(r), 4), np.float32)
+        check(lib.tt_trace_closest_host(self._h, _vp(r), len(r), _vp(out), C.byref(cfg) if cfg is not None else None))
+        return out[:, 0].view(np.int32).copy(), out[:, 1].copy(), out[:, 2].copy(), out[:, 3].copy()
+
+    def occluded(self, rays, cfg=None):
+        r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        out = np.empty(len(r), np.int32)
+        check(lib.tt_trace_any_host(self._h, _vp(r), len(r), _vp(out), C.byref(cfg) if cfg is not None else None))
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib.tt_scene_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()

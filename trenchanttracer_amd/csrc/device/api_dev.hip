@@ -1,0 +1,240 @@
+// C ABI, device half: context, scene upload, traversal entry points.
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../host/tt_host.h"
+#include "dev_common.h"
+#include "scene_dev.h"
+
+using namespace tt;
+
+#define TT_HIP(call, what)                                                                       \
+    do {                                                                                         \
+        hipError_t e_ = (call);                                                                  \
+        if (e_ != hipSuccess) return fail(TT_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+namespace tt {
+
+tt_status dev_alloc(tt_ctx *ctx, void **p, size_t bytes, const char *what) {
+    (void)ctx;
+    *p = nullptr;
+    if (bytes == 0) return TT_OK;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(e == hipErrorOutOfMemory ? TT_ERR_NOMEM : TT_ERR_DEVICE,
+                    std::string(what) + ": hipMalloc failed: " + hipGetErrorString(e));
+    }
+    return TT_OK;
+}
+
+tt_status resolve_cfg(const tt_trace_cfg *cfg, tt_trace_cfg &out) {
+    out.variant = 0;
+    out.lds_stack = 16;
+    out.block = 128;
+    out.persistent = 0;
+    if (cfg) out = *cfg;
+    if (out.variant < 0 || out.variant > 1) return fail(TT_ERR_INVALID, "trace cfg: variant must be 0 or 1");
+    if (out.lds_stack != 8 && out.lds_stack != 16 && out.lds_stack != 24 && out.lds_stack != 32)
+        return fail(TT_ERR_INVALID, "trace cfg: lds_stack must be 8, 16, 24 or 32");
+    if (out.block < 64 || out.block > 256 || out.block % 64 != 0)
+        return fail(TT_ERR_INVALID, "trace cfg: block must be 64, 128, 192 or 256");
+    return TT_OK;
+}
+
+}  // namespace tt
+
+extern "C" {
+
+tt_status tt_ctx_create(int32_t device, tt_ctx **out) {
+    if (!out) return fail(TT_ERR_INVALID, "tt_ctx_create: null out");
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) return fail(TT_ERR_UNSUPPORTED, "tt_ctx_create: no HIP device available");
+    if (device < 0 || device >= count) return fail(TT_ERR_INVALID, "tt_ctx_create: device index out of range");
+    TT_HIP(hipSetDevice(device), "hipSetDevice");
+    auto *c = new (std::nothrow) tt_ctx();
+    if (!c) return fail(TT_ERR_NOMEM, "tt_ctx_create: host allocation failed");
+    c->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+        c->num_cus = prop.multiProcessorCount;
+        std::strncpy(c->arch, prop.gcnArchName, sizeof(c->arch) - 1);
+    }
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(TT_ERR_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    *out = c;
+    return TT_OK;
+}
+
+void tt_ctx_destroy(tt_ctx *ctx) {
+    if (!ctx) return;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    if (ctx->scratch) hipFree(ctx->scratch);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+tt_status tt_ctx_sync(tt_ctx *ctx) {
+    if (!ctx) return fail(TT_ERR_INVALID, "tt_ctx_sync: null ctx");
+    TT_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return TT_OK;
+}
+
+tt_status tt_scene_create(tt_ctx *ctx, const tt_mesh *mesh, const tt_bvh *bvh, const tt_material *mats, int32_t nmats,
+                          const float *env_rgb, int32_t env_w, int32_t env_h, float env_scale, tt_scene **out) {
+    if (!ctx || !bvh || !out) return fail(TT_ERR_INVALID, "tt_scene_create: null argument");
+    if (nmats < 0 || (nmats > 0 && !mats)) return fail(TT_ERR_INVALID, "tt_scene_create: bad materials");
+    if (env_rgb && (env_w <= 0 || env_h <= 0 || (int64_t)env_w * env_h > (1LL << 28)))
+        return fail(TT_ERR_INVALID, "tt_scene_create: bad env map size");
+    const Flat &f = bvh->b.flat;
+    if (f.dev_nodes.size() % 16 != 0 || f.dev_tris.size() % 12 != 0)
+        return fail(TT_ERR_INVALID, "tt_scene_create: corrupt device layout");
+    TT_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    auto s = new (std::nothrow) tt_scene();
+    if (!s) return fail(TT_ERR_NOMEM, "tt_scene_create: host allocation failed");
+    s->ctx = ctx;
+    s->num_nodes = (int64_t)(f.dev_nodes.size() / 16);
+    s->num_tris = (int64_t)(f.dev_tris.size() / 12);
+    if (s->num_nodes > 0x7FFFFFFF / 4 || s->num_tris > 0x7FFFFFFF / 3) {
+        delete s;
+        return fail(TT_ERR_INVALID, "tt_scene_create: tree too large for 32-bit node / triangle addressing");
+    }
+    tt_status st = TT_OK;
+    auto up = [&](void **dst, const void *src, size_t bytes, const char *what) {
+        if (st != TT_OK) return;
+        st = dev_alloc(ctx, dst, bytes, what);
+        if (st != TT_OK || bytes == 0) return;
+        s->bytes += bytes;
+        hipError_t e = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+        if (e != hipSuccess) st = fail(TT_ERR_DEVICE, std::string(what) + ": hipMemcpy: " + hipGetErrorString(e));
+    };
+    up((void **)&s->nodes, f.dev_nodes.data(), f.dev_nodes.size() * 4, "scene nodes");
+    up((void **)&s->tris, f.dev_tris.data(), f.dev_tris.size() * 4, "scene triangles");
+    if (mesh && st == TT_OK) {
+        // per-triangle shading normals (3 float4) and material ids, by original triangle id
+        const Mesh &m = mesh->m;
+        std::vector<float> tn((size_t)m.ntris() * 12);
+        for (int32_t i = 0; i < m.ntris(); ++i)
+            for (int k = 0; k < 3; ++k) {
+                V3 n = m.normals[m.tri_n[3 * (size_t)i + k]];
+                float *d = &tn[(size_t)i * 12 + 4 * k];
+                d[0] = n.x;
+                d[1] = n.y;
+                d[2] = n.z;
+                d[3] = 0.0f;
+            }
+        s->num_orig_tris = m.ntris();
+        up((void **)&s->tri_normals, tn.data(), tn.size() * 4, "scene normals");
+        up((void **)&s->tri_mat, m.tri_mat.data(), m.tri_mat.size() * 4, "scene material ids");
+        for (int32_t x : m.tri_mat)
+            if (x < 0 || (nmats > 0 && x >= nmats)) {
+                if (st == TT_OK) st = fail(TT_ERR_INVALID, "tt_scene_create: triangle material id out of range");
+                break;
+            }
+    }
+    if (nmats > 0) up((void **)&s->mats, mats, sizeof(tt_material) * (size_t)nmats, "scene materials");
+    s->num_mats = nmats;
+    if (st == TT_OK && env_rgb) st = build_env(s, env_rgb, env_w, env_h, env_scale);
+    if (st != TT_OK) {
+        tt_scene_destroy(s);
+        return st;
+    }
+    *out = s;
+    return TT_OK;
+}
+
+void tt_scene_destroy(tt_scene *s) {
+    if (!s) return;
+    hipSetDevice(s->ctx->device);
+    hipStreamSynchronize(s->ctx->stream);
+    void *ptrs[] = {s->nodes, s->tris, s->tri_normals, s->tri_mat, s->mats,
+                    s->env_rgb, s->env_marg_cdf, s->env_cond_cdf, s->env_pdf_scale};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    delete s;
+}
+
+tt_status tt_scene_info(const tt_scene *s, int64_t *num_nodes, int64_t *num_tris, int64_t *device_bytes) {
+    if (!s) return fail(TT_ERR_INVALID, "tt_scene_info: null scene");
+    if (num_nodes) *num_nodes = s->num_nodes;
+    if (num_tris) *num_tris = s->num_tris;
+    if (device_bytes) *device_bytes = (int64_t)s->bytes;
+    return TT_OK;
+}
+
+static tt_status trace_impl(tt_scene *s, const tt_ray *rays, int64_t n, tt_hit *hits, int32_t *flags,
+                            const tt_trace_cfg *cfg, void *stream, bool any) {
+    if (!s) return fail(TT_ERR_INVALID, "trace: null scene");
+    if (n < 0 || (n > 0 && (!rays || (any ? !flags : !hits)))) return fail(TT_ERR_INVALID, "trace: bad buffers");
+    if (n >= (int64_t)1 << 40) return fail(TT_ERR_INVALID, "trace: too many rays");
+    tt_trace_cfg c;
+    tt_status st = resolve_cfg(cfg, c);
+    if (st != TT_OK) return st;
+    if (n == 0) return TT_OK;
+    TraceArgs a{};
+    a.nodes = reinterpret_cast<const float4 *>(s->nodes);
+    a.tris = reinterpret_cast<const float4 *>(s->tris);
+    a.rays = rays;
+    a.hits = hits;
+    a.flags = flags;
+    a.n = n;
+    a.num_nodes = (int32_t)s->num_nodes;
+    hipStream_t hs = stream ? (hipStream_t)stream : s->ctx->stream;
+    TT_HIP(hipSetDevice(s->ctx->device), "hipSetDevice");
+    TT_HIP(launch_trace(a, c.variant, any, c.lds_stack, c.block, c.persistent, s->ctx->num_cus, hs), "trace launch");
+    return TT_OK;
+}
+
+tt_status tt_trace_closest(tt_scene *s, const tt_ray *rays, int64_t n, tt_hit *hits, const tt_trace_cfg *cfg,
+                           void *stream) {
+    return trace_impl(s, rays, n, hits, nullptr, cfg, stream, false);
+}
+
+tt_status tt_trace_any(tt_scene *s, const tt_ray *rays, int64_t n, int32_t *hit_flags, const tt_trace_cfg *cfg,
+                       void *stream) {
+    return trace_impl(s, rays, n, nullptr, hit_flags, cfg, stream, true);
+}
+
+static tt_status host_trace(tt_scene *s, const tt_ray *rays, int64_t n, void *out, size_t out_elem,
+                            const tt_trace_cfg *cfg, bool any) {
+    if (!s) return fail(TT_ERR_INVALID, "trace_host: null scene");
+    if (n < 0 || (n > 0 && (!rays || !out))) return fail(TT_ERR_INVALID, "trace_host: bad buffers");
+    if (n == 0) return TT_OK;
+    tt_ctx *ctx = s->ctx;
+    TT_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    size_t need = (size_t)n * (sizeof(tt_ray) + out_elem);
+    if (ctx->scratch_bytes < need) {
+        if (ctx->scratch) hipFree(ctx->scratch);
+        ctx->scratch = nullptr;
+        ctx->scratch_bytes = 0;
+        tt_status st = dev_alloc(ctx, &ctx->scratch, need, "trace_host scratch");
+        if (st != TT_OK) return st;
+        ctx->scratch_bytes = need;
+    }
+    tt_ray *dr = (tt_ray *)ctx->scratch;
+    void *dout = (char *)ctx->scratch + (size_t)n * sizeof(tt_ray);
+    TT_HIP(hipMemcpyAsync(dr, rays, (size_t)n * sizeof(tt_ray), hipMemcpyHostToDevice, ctx->stream), "copy rays");
+    tt_status st = any ? tt_trace_any(s, dr, n, (int32_t *)dout, cfg, ctx->stream)
+                       : tt_trace_closest(s, dr, n, (tt_hit *)dout, cfg, ctx->stream);
+    if (st != TT_OK) return st;
+    TT_HIP(hipMemcpyAsync(out, dout, (size_t)n * out_elem, hipMemcpyDeviceToHost, ctx->stream), "copy hits");
+    TT_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return TT_OK;
+}
+
+tt_status tt_trace_closest_host(tt_scene *s, const tt_ray *rays, int64_t n, tt_hit *hits, const tt_trace_cfg *cfg) {
+    return host_trace(s, rays, n, hits, sizeof(tt_hit), cfg, false);
+}
+
+tt_status tt_trace_any_host(tt_scene *s, const tt_ray *rays, int64_t n, int32_t *hit_flags, const tt_trace_cfg *cfg) {
+    return host_trace(s, rays, n, hit_flags, sizeof(int32_t), cfg, true);
+}
+
+}  // extern "C"

@@ -1,0 +1,30 @@
+// Device-side shared definitions (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../../include/tt.h"
+
+namespace tt {
+
+constexpr int kSentinel = 0x76543210;  // IntersectKernel.cuh:9
+constexpr int kStackCap = 72;          // >= 64 outstanding pushes (tree depth <= 64)
+
+struct TraceArgs {
+    const float4 *nodes;   // 4 float4 per node
+    const float4 *tris;    // 3 float4 per triangle record
+    const tt_ray *rays;
+    tt_hit *hits;          // closest-hit output (or nullptr)
+    int32_t *flags;        // any-hit output (or nullptr)
+    int64_t n;
+    int32_t num_nodes;
+    int32_t *work_counter; // persistent variant: next ray batch
+    int32_t batch;         // persistent variant: rays per wave fetch
+};
+
+// Launchers (trace.hip).
+hipError_t launch_trace(const TraceArgs &a, int variant, bool any, int lds_stack, int block, int persistent,
+                        int num_cus, hipStream_t s);
+
+}  // namespace tt

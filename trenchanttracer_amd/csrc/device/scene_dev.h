@@ -1,0 +1,46 @@
+// Device objects behind the C ABI handles.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../../include/tt.h"
+
+struct tt_ctx {
+    int device = 0;
+    int num_cus = 256;
+    char arch[64] = {0};
+    hipStream_t stream = nullptr;
+    void *scratch = nullptr;  // staging for the host-buffer wrappers
+    size_t scratch_bytes = 0;
+};
+
+struct tt_scene {
+    tt_ctx *ctx = nullptr;
+    // SBVH device layout
+    void *nodes = nullptr;       // float4 x 4 per node
+    void *tris = nullptr;        // float4 x 3 per triangle record
+    int64_t num_nodes = 0;
+    int64_t num_tris = 0;
+    // shading data by original triangle id
+    void *tri_normals = nullptr;  // float4 x 3 per triangle
+    int32_t *tri_mat = nullptr;
+    int32_t num_orig_tris = 0;
+    tt_material *mats = nullptr;
+    int32_t num_mats = 0;
+    // environment map (equirectangular) and its sampling distribution
+    float *env_rgb = nullptr;        // float4 per texel (rgb, luminance weight)
+    float *env_marg_cdf = nullptr;   // H + 1 floats
+    float *env_cond_cdf = nullptr;   // H * (W + 1) floats
+    double *env_pdf_scale = nullptr; // [0] = total weight (FP64)
+    int32_t env_w = 0, env_h = 0;
+    float env_scale = 1.0f;
+    double env_total = 0.0;
+    size_t bytes = 0;
+};
+
+namespace tt {
+tt_status dev_alloc(tt_ctx *ctx, void **p, size_t bytes, const char *what);
+tt_status build_env(tt_scene *s, const float *rgb, int32_t w, int32_t h, float scale);
+}  // namespace tt

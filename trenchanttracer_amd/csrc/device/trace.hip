@@ -1,0 +1,252 @@
+// Closest-hit / any-hit traversal of the SBVH on gfx950 (wave64).
+//
+// Per lane: one ray, the while-while loop of Aila & Laine ("Understanding the
+// Efficiency of Ray Traversal on GPUs", HPG 2009) in the form of the reference
+// intersect() (include/geometry/IntersectKernel.cuh:78-296):
+//   - ray precompute idir / ood with the 2^-80 clamp                (:92-101)
+//   - two child boxes per 64-byte node, slab entry/exit as 3-operand integer
+//     min/max of the float bit patterns (v_min3_i32 / v_max3_i32 restate the
+//     PTX vmin/vmax.s32 forms), traverse iff entry <= exit, entry >= tmin,
+//     entry <= 1e20                                                  (:126-147)
+//   - near child first, far child pushed                             (:149-167)
+//   - first leaf postponed                                           (:172-178)
+//   - Möller–Trumbore on v0 and the two edges, front faces only with
+//     |det| > 1e-5, tmin < t < best                                  (:230-257)
+// Arithmetic is unfused (fp contract off) and division IEEE, so for a given
+// visiting order every decision is bit-identical to the host restatement.
+//
+// Two loop variants:
+//   VARIANT 0 (exact): a lane leaves the node loop as soon as it holds a leaf,
+//     i.e. the single-lane order. Hit ids, t, u, v are bit-identical to the
+//     host traversal of the same tree.
+//   VARIANT 1 (speculative): lanes keep descending until no lane of the wave is
+//     still searching for its first leaf (__any ballot, :192-203). Results differ
+//     from the single-lane order only where a triangle lies on a box boundary
+//     within rounding of the current hit distance.
+// Stack: the top kLds entries of each lane live in LDS ([entry][lane] so the
+// lanes of a wave hit distinct banks), deeper entries in per-lane scratch.
+#include "dev_common.h"
+
+#pragma clang fp contract(off)
+
+namespace tt {
+namespace {
+
+__device__ __forceinline__ int fb(float f) { return __float_as_int(f); }
+__device__ __forceinline__ float bf(int i) { return __int_as_float(i); }
+
+// fminf / fmaxf of the slab code with -0 < +0 (v_minimum3_f32 / v_maximum3_f32).
+__device__ __forceinline__ float fmn(float a, float b) { return __builtin_elementwise_minimum(a, b); }
+__device__ __forceinline__ float fmx(float a, float b) { return __builtin_elementwise_maximum(a, b); }
+
+// spanBeginKepler / spanEndKepler (IntersectKernel.cuh:54-60).
+__device__ __forceinline__ float span_begin(float a0, float a1, float b0, float b1, float c0, float c1, float d) {
+    int z = max(min(fb(c0), fb(c1)), fb(d));
+    return bf(max(max(fb(fmn(a0, a1)), fb(fmn(b0, b1))), z));
+}
+__device__ __forceinline__ float span_end(float a0, float a1, float b0, float b1, float c0, float c1, float d) {
+    int z = min(max(fb(c0), fb(c1)), fb(d));
+    return bf(min(min(fb(fmx(a0, a1)), fb(fmx(b0, b1))), z));
+}
+
+template <int S>
+struct Stack {
+    int *lds;       // this lane's column: lds[k * stride]
+    int stride;
+    int sp;         // entries held (the sentinel lies below entry 0)
+    int spill[kStackCap - S];
+
+    __device__ __forceinline__ void push(int v) {
+        if (sp < S) {
+            lds[sp * stride] = v;
+        } else if (sp < kStackCap) {
+            spill[sp - S] = v;
+        }
+        sp++;
+    }
+    __device__ __forceinline__ int pop() {
+        if (sp <= 0) return kSentinel;
+        sp--;
+        if (sp < S) return lds[sp * stride];
+        if (sp < kStackCap) return spill[sp - S];
+        return kSentinel;
+    }
+};
+
+template <int VARIANT, bool ANY, int S>
+__global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
+    extern __shared__ int lds_stack[];
+    const int tid = threadIdx.x;
+    const int64_t rid = (int64_t)blockIdx.x * blockDim.x + tid;
+    const bool valid = rid < a.n;
+
+    float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = make_float4(0.f, 0.f, 1.f, 0.f);
+    if (valid) {
+        const float4 *rp = reinterpret_cast<const float4 *>(a.rays + rid);
+        r0 = rp[0];
+        r1 = rp[1];
+    }
+    const float ox = r0.x, oy = r0.y, oz = r0.z, tmin = r0.w;
+    const float dx = r1.x, dy = r1.y, dz = r1.z, tmax = r1.w;
+
+    const float ooeps = 0x1p-80f;
+    const float idirx = 1.0f / (fabsf(dx) > ooeps ? dx : copysignf(ooeps, dx));
+    const float idiry = 1.0f / (fabsf(dy) > ooeps ? dy : copysignf(ooeps, dy));
+    const float idirz = 1.0f / (fabsf(dz) > ooeps ? dz : copysignf(ooeps, dz));
+    const float oodx = ox * idirx, oody = oy * idiry, oodz = oz * idirz;
+
+    Stack<S> st;
+    st.lds = lds_stack + tid;
+    st.stride = blockDim.x;
+    st.sp = 0;
+
+    int leaf_addr = 0;
+    int node_addr = (valid && a.num_nodes > 0) ? 0 : kSentinel;
+    float hit_t = tmax, hit_u = 0.0f, hit_v = 0.0f;
+    int hit_id = -1;
+
+    while (node_addr != kSentinel) {
+        while (node_addr >= 0 && node_addr != kSentinel) {
+            const float4 *np = a.nodes + (int64_t)node_addr * 4;
+            const float4 n0xy = np[0];
+            const float4 n1xy = np[1];
+            const float4 nz = np[2];
+            const int4 ch = reinterpret_cast<const int4 *>(np)[3];
+
+            const float c0lox = n0xy.x * idirx - oodx;
+            const float c0hix = n0xy.y * idirx - oodx;
+            const float c0loy = n0xy.z * idiry - oody;
+            const float c0hiy = n0xy.w * idiry - oody;
+            const float c0loz = nz.x * idirz - oodz;
+            const float c0hiz = nz.y * idirz - oodz;
+            const float c1loz = nz.z * idirz - oodz;
+            const float c1hiz = nz.w * idirz - oodz;
+            const float c0min = span_begin(c0lox, c0hix, c0loy, c0hiy, c0loz, c0hiz, tmin);
+            const float c0max = span_end(c0lox, c0hix, c0loy, c0hiy, c0loz, c0hiz, hit_t);
+            const float c1lox = n1xy.x * idirx - oodx;
+            const float c1hix = n1xy.y * idirx - oodx;
+            const float c1loy = n1xy.z * idiry - oody;
+            const float c1hiy = n1xy.w * idiry - oody;
+            const float c1min = span_begin(c1lox, c1hix, c1loy, c1hiy, c1loz, c1hiz, tmin);
+            const float c1max = span_end(c1lox, c1hix, c1loy, c1hiy, c1loz, c1hiz, hit_t);
+
+            const float ray_tmax = 1e20f;
+            const bool t0 = (c0min <= c0max) && (c0min >= tmin) && (c0min <= ray_tmax);
+            const bool t1 = (c1min <= c1max) && (c1min >= tmin) && (c1min <= ray_tmax);
+
+            if (!t0 && !t1) {
+                node_addr = st.pop();
+            } else {
+                int far = ch.y;
+                node_addr = t0 ? ch.x : ch.y;
+                if (t0 && t1) {
+                    if (c1min < c0min) {
+                        int tmp = node_addr;
+                        node_addr = far;
+                        far = tmp;
+                    }
+                    st.push(far);
+                }
+            }
+            if (node_addr < 0 && leaf_addr >= 0) {
+                leaf_addr = node_addr;
+                node_addr = st.pop();
+            }
+            if (VARIANT == 0) {
+                if (leaf_addr < 0) break;
+            } else {
+                if (!__any(leaf_addr >= 0)) break;
+            }
+        }
+
+        while (leaf_addr < 0) {
+            int tri = ~leaf_addr;
+            for (;;) {
+                const float4 *tp = a.tris + (int64_t)tri * 3;
+                const float4 A = tp[0];
+                const float4 B = tp[1];
+                const float4 C = tp[2];
+                // Möller–Trumbore, IntersectKernel.cuh:230-254 (edges precomputed: same bits)
+                const float tvx = ox - A.x, tvy = oy - A.y, tvz = oz - A.z;
+                const float pvx = dy * C.z - dz * C.y;
+                const float pvy = dz * C.x - dx * C.z;
+                const float pvz = dx * C.y - dy * C.x;
+                const float det = B.x * pvx + B.y * pvy + B.z * pvz;
+                const float inv = 1.0f / det;
+                const float u = (tvx * pvx + tvy * pvy + tvz * pvz) * inv;
+                const float qvx = tvy * B.z - tvz * B.y;
+                const float qvy = tvz * B.x - tvx * B.z;
+                const float qvz = tvx * B.y - tvy * B.x;
+                const float v = (dx * qvx + dy * qvy + dz * qvz) * inv;
+                const float EPS = 0.00001f;
+                bool ok;
+                if (det > EPS) {
+                    ok = !(u < 0.0f || u > 1.0f) && !(v < 0.0f || (u + v) > 1.0f);
+                } else if (det < -EPS) {
+                    ok = !(u > 0.0f || u < 1.0f) && !(v > 0.0f || (u + v) < 1.0f);
+                } else {
+                    ok = false;
+                }
+                if (ok) {
+                    const float t = (C.x * qvx + C.y * qvy + C.z * qvz) * inv;
+                    if (tmin < t && t < hit_t) {
+                        hit_t = t;
+                        hit_u = u;
+                        hit_v = v;
+                        hit_id = fb(A.w);
+                        if (ANY) {
+                            node_addr = kSentinel;
+                            break;
+                        }
+                    }
+                }
+                if (fb(C.w) != 0) break;
+                ++tri;
+            }
+            leaf_addr = node_addr;
+            if (node_addr < 0) node_addr = st.pop();
+        }
+    }
+
+    if (!valid) return;
+    if (ANY) {
+        a.flags[rid] = hit_id != -1 ? 1 : 0;
+    } else {
+        if (hit_id == -1) {
+            hit_u = 0.0f;
+            hit_v = 0.0f;
+        }
+        float4 out = make_float4(bf(hit_id), hit_t, hit_u, hit_v);
+        *reinterpret_cast<float4 *>(a.hits + rid) = out;
+    }
+}
+
+template <int V, bool ANY>
+hipError_t launch_s(const TraceArgs &a, int S, int block, hipStream_t s) {
+    const int64_t grid = (a.n + block - 1) / block;
+    const size_t lds = (size_t)S * block * sizeof(int);
+    switch (S) {
+        case 8: hipLaunchKernelGGL((k_trace<V, ANY, 8>), dim3((unsigned)grid), dim3(block), lds, s, a); break;
+        case 16: hipLaunchKernelGGL((k_trace<V, ANY, 16>), dim3((unsigned)grid), dim3(block), lds, s, a); break;
+        case 24: hipLaunchKernelGGL((k_trace<V, ANY, 24>), dim3((unsigned)grid), dim3(block), lds, s, a); break;
+        case 32: hipLaunchKernelGGL((k_trace<V, ANY, 32>), dim3((unsigned)grid), dim3(block), lds, s, a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_trace(const TraceArgs &a, int variant, bool any, int lds_stack, int block, int persistent,
+                        int num_cus, hipStream_t s) {
+    (void)persistent;
+    (void)num_cus;
+    if (a.n <= 0) return hipSuccess;
+    if (block < 64 || block > 256 || block % 64 != 0) return hipErrorInvalidValue;
+    if ((a.n + block - 1) / block > 0x7FFFFFFF) return hipErrorInvalidValue;
+    if (variant == 0) return any ? launch_s<0, true>(a, lds_stack, block, s) : launch_s<0, false>(a, lds_stack, block, s);
+    if (variant == 1) return any ? launch_s<1, true>(a, lds_stack, block, s) : launch_s<1, false>(a, lds_stack, block, s);
+    return hipErrorInvalidValue;
+}
+
+}  // namespace tt
