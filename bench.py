@@ -41,14 +41,9 @@ def main():
         sys.exit(2)
 
     import numpy as np
-    import torch
 
     import trenchanttracer_amd as tt
     from trenchanttracer_amd import workloads
-
-    if not torch.cuda.is_available():
-        log("no GPU visible")
-        sys.exit(1)
 
     mesh, bvh, rays, n_prim = workloads.traversal(ntris=args.ntris, log=log)
     n = len(rays)
@@ -57,33 +52,32 @@ def main():
     info = scene.info()
     log(f"scene on device: {info}")
 
-    d_rays = torch.from_numpy(rays).to("cuda")
-    d_hits = torch.empty((n, 4), dtype=torch.int32, device="cuda")
-    stream = torch.cuda.current_stream()
-    sh = stream.cuda_stream
+    d_rays = tt.DeviceBuffer.from_array(ctx, rays)
+    d_hits = tt.DeviceBuffer(ctx, n * 16)
     cfg = tt.trace_cfg(variant=args.variant, lds_stack=args.lds_stack, block=args.block)
 
     for _ in range(args.warmup):
-        scene.trace_closest(d_rays, n, d_hits, cfg, stream=sh)
-    torch.cuda.synchronize()
+        scene.trace_closest(d_rays, n, d_hits, cfg)
+    ctx.sync()
     log(f"warmup done ({args.warmup} steps)")
 
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
-    torch.cuda.synchronize()
+    # HIP events on the context stream, the stream the traversal kernel runs on
+    timer = tt.Timer(ctx, 2 * args.steps)
+    ctx.sync()
     t0 = time.perf_counter()
     for i in range(args.steps):
-        ev[i][0].record(stream)
-        scene.trace_closest(d_rays, n, d_hits, cfg, stream=sh)
-        ev[i][1].record(stream)
-    torch.cuda.synchronize()
+        timer.record(2 * i)
+        scene.trace_closest(d_rays, n, d_hits, cfg)
+        timer.record(2 * i + 1)
+    ctx.sync()
     wall = time.perf_counter() - t0
-    kernel_ms = [a.elapsed_time(b) for a, b in ev]
+    kernel_ms = [timer.elapsed(2 * i, 2 * i + 1) for i in range(args.steps)]
     kmean = float(np.mean(kernel_ms))
     ms_per_step = wall * 1e3 / args.steps
     mrays = n * args.steps / wall / 1e6
     log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step wall, kernel {kmean:.3f} ms avg, {mrays:.1f} Mrays/s")
 
-    hits = d_hits.cpu().numpy()
+    hits = d_hits.download((n, 4), np.int32)
     hit_frac = float((hits[:, 0] >= 0).mean())
 
     result = {

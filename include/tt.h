@@ -176,6 +176,22 @@ void tt_ctx_destroy(tt_ctx *ctx);
 /* Synchronises the context's stream. */
 tt_status tt_ctx_sync(tt_ctx *ctx);
 
+/* Device memory and timing on the context's device / stream (plumbing for callers
+ * that hold no other HIP runtime). Copies are synchronous. Timer: `n` HIP event
+ * slots recorded on the context stream; elapsed = ms between two recorded slots. */
+tt_status tt_dev_alloc(tt_ctx *ctx, size_t bytes, void **out);
+tt_status tt_dev_free(tt_ctx *ctx, void *p);
+tt_status tt_memcpy_h2d(tt_ctx *ctx, void *dst, const void *src, size_t bytes);
+tt_status tt_memcpy_d2h(tt_ctx *ctx, void *dst, const void *src, size_t bytes);
+tt_status tt_memset_d(tt_ctx *ctx, void *dst, int value, size_t bytes);
+typedef struct tt_timer tt_timer;
+tt_status tt_timer_create(tt_ctx *ctx, int32_t n, tt_timer **out);
+tt_status tt_timer_record(tt_timer *t, int32_t slot);
+tt_status tt_timer_elapsed(tt_timer *t, int32_t a, int32_t b, float *ms);
+void tt_timer_destroy(tt_timer *t);
+/* The context's stream (hipStream_t) as an opaque pointer. */
+void *tt_ctx_stream(tt_ctx *ctx);
+
 /* Materials. REF_* reproduce the reference samplers (reference render mode);
  * the others are the physically based set (path-tracer mode). */
 typedef enum {

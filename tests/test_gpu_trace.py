@@ -113,17 +113,15 @@ def test_icosphere_and_cornell_exact(tt, oracle, ctx):
 
 
 def test_device_pointer_entry_point(tt, oracle, ctx):
-    import torch
     _say("trace device pointers")
     m = tt.Mesh.soup(10000, seed=10)
     b, s = _scene(tt, ctx, m)
     rays = _rays(tt, m, seed=10)
-    d_r = torch.from_numpy(rays).cuda()
-    d_h = torch.empty((len(rays), 4), dtype=torch.int32, device="cuda")
-    st = torch.cuda.current_stream()
-    s.trace_closest(d_r, len(rays), d_h, tt.trace_cfg(0, 16, 128), stream=st.cuda_stream)
-    torch.cuda.synchronize()
-    h = d_h.cpu().numpy()
+    d_r = tt.DeviceBuffer.from_array(ctx, rays)
+    d_h = tt.DeviceBuffer(ctx, len(rays) * 16)
+    s.trace_closest(d_r, len(rays), d_h, tt.trace_cfg(0, 16, 128))
+    ctx.sync()
+    h = d_h.download((len(rays), 4), np.int32)
     oh, _ = oracle.CompactBVH(b.export_compact()).trace(rays)
     assert np.array_equal(h.view(np.int32), oh.view(np.int32))
     s.close()
@@ -139,7 +137,7 @@ def test_full_traversal_workload_bit_exact(tt, oracle, ctx):
     _say("full traversal workload: oracle")
     oh, st = oracle.CompactBVH(bvh.export_compact()).trace(rays, nthreads=min(16, os.cpu_count() or 1))
     _check_exact(oracle, oh, ids, t, u, v)
-    assert 0.05 < (ids >= 0).mean() < 0.95
+    assert 0.05 < (ids >= 0).mean() < 0.995
     # speculative variant: only near ties may differ
     ids1, t1, *_ = s.intersect(rays, tt.trace_cfg(1, 16, 128))
     bad = np.nonzero(ids1 != ids)[0]

@@ -12,13 +12,14 @@ Device entry points run the HIP kernels of libtt.so; they raise if the library
 or a GPU is missing (there is no CPU fallback).
 """
 import ctypes as C
+import weakref
 
 import numpy as np
 
 from . import _native as N
 from ._native import TTError, check, lib, tt_trace_cfg
 
-__all__ = ["Mesh", "BVH", "Context", "DeviceScene", "TTError", "rays_primary", "rays_incoherent",
+__all__ = ["Mesh", "BVH", "Context", "DeviceScene", "DeviceBuffer", "Timer", "TTError", "rays_primary", "rays_incoherent",
            "trace_cfg", "MAT_DIFFUSE", "MAT_CONDUCTOR", "MAT_DIELECTRIC", "MAT_EMITTER", "material"]
 
 MAT_DIFFUSE, MAT_CONDUCTOR, MAT_DIELECTRIC, MAT_EMITTER = 0, 1, 2, 3
@@ -200,12 +201,22 @@ class Context:
         h = C.c_void_p()
         check(lib.tt_ctx_create(int(device), C.byref(h)))
         self._h = h
+        self._children = weakref.WeakSet()  # scenes, buffers, timers: released before the context
+
+    def _adopt(self, obj):
+        self._children.add(obj)
 
     def sync(self):
         check(lib.tt_ctx_sync(self._h))
 
+    @property
+    def stream(self):
+        return lib.tt_ctx_stream(self._h)
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
+            for c in list(self._children):
+                c.close()
             lib.tt_ctx_destroy(self._h)
             self._h = None
 
@@ -213,14 +224,84 @@ class Context:
         self.close()
 
 
+class DeviceBuffer:
+    """Device allocation owned by a Context (tt_dev_alloc)."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        check(lib.tt_dev_alloc(ctx._h, self.nbytes, C.byref(p)))
+        self.ptr = p.value or 0
+        ctx._adopt(self)
+
+    @classmethod
+    def from_array(cls, ctx, a):
+        a = np.ascontiguousarray(a)
+        b = cls(ctx, a.nbytes)
+        b.upload(a)
+        return b
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        if a.nbytes > self.nbytes:
+            raise ValueError("array larger than the device buffer")
+        check(lib.tt_memcpy_h2d(self.ctx._h, C.c_void_p(self.ptr), C.c_void_p(a.ctypes.data), a.nbytes))
+
+    def download(self, shape, dtype):
+        out = np.empty(shape, dtype)
+        if out.nbytes > self.nbytes:
+            raise ValueError("requested more bytes than the device buffer holds")
+        check(lib.tt_memcpy_d2h(self.ctx._h, C.c_void_p(out.ctypes.data), C.c_void_p(self.ptr), out.nbytes))
+        return out
+
+    def zero(self):
+        check(lib.tt_memset_d(self.ctx._h, C.c_void_p(self.ptr), 0, self.nbytes))
+
+    def free(self):
+        if getattr(self, "ptr", 0) and self.ctx._h is not None:
+            lib.tt_dev_free(self.ctx._h, C.c_void_p(self.ptr))
+        self.ptr = 0
+
+    close = free
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Timer:
+    """HIP events recorded on the context stream (tt_timer)."""
+
+    def __init__(self, ctx, n):
+        h = C.c_void_p()
+        check(lib.tt_timer_create(ctx._h, int(n), C.byref(h)))
+        self._h = h
+        ctx._adopt(self)
+
+    def record(self, slot):
+        check(lib.tt_timer_record(self._h, int(slot)))
+
+    def elapsed(self, a, b):
+        ms = C.c_float()
+        check(lib.tt_timer_elapsed(self._h, int(a), int(b), C.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib.tt_timer_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
 def _dev_ptr(x):
-    """Device address of a torch tensor or an int."""
-    if hasattr(x, "data_ptr"):
-        if not x.is_cuda:
-            raise ValueError("expected a device tensor")
-        if not x.is_contiguous():
-            raise ValueError("expected a contiguous tensor")
-        return C.c_void_p(x.data_ptr())
+    """Device address of a DeviceBuffer or an int."""
+    if isinstance(x, DeviceBuffer):
+        return C.c_void_p(x.ptr)
     return C.c_void_p(int(x))
 
 
@@ -245,6 +326,7 @@ class DeviceScene:
                                   _vp(env_arr) if env_arr is not None else None, ew, eh, float(env_scale),
                                   C.byref(h)))
         self._h = h
+        ctx._adopt(self)
 
     def info(self):
         a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
@@ -259,7 +341,7 @@ class DeviceScene:
         return marg, cond, tot.value
 
     def trace_closest(self, rays, n, hits, cfg=None, stream=None):
-        """rays / hits: device tensors (n x 8 float32 / n x 4 int32|float32) or device addresses."""
+        """rays / hits: DeviceBuffers (n x 32 B / n x 16 B) or device addresses; stream: hipStream_t or None."""
         check(lib.tt_trace_closest(self._h, _dev_ptr(rays), int(n), _dev_ptr(hits),
                                    C.byref(cfg) if cfg is not None else None,
                                    C.c_void_p(stream) if stream else None))

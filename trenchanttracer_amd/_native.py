@@ -104,6 +104,16 @@ PROTOS = {
     "tt_scene_destroy": (None, [P]),
     "tt_scene_info": (C.c_int, [P, PI64, PI64, PI64]),
     "tt_env_export_cdf": (C.c_int, [P, P, P, PD]),
+    "tt_dev_alloc": (C.c_int, [P, C.c_size_t, C.POINTER(P)]),
+    "tt_dev_free": (C.c_int, [P, P]),
+    "tt_memcpy_h2d": (C.c_int, [P, P, P, C.c_size_t]),
+    "tt_memcpy_d2h": (C.c_int, [P, P, P, C.c_size_t]),
+    "tt_memset_d": (C.c_int, [P, P, C.c_int, C.c_size_t]),
+    "tt_timer_create": (C.c_int, [P, I32, C.POINTER(P)]),
+    "tt_timer_record": (C.c_int, [P, I32]),
+    "tt_timer_elapsed": (C.c_int, [P, I32, I32, C.POINTER(C.c_float)]),
+    "tt_timer_destroy": (None, [P]),
+    "tt_ctx_stream": (P, [P]),
     "tt_trace_closest": (C.c_int, [P, P, I64, P, C.POINTER(tt_trace_cfg), P]),
     "tt_trace_any": (C.c_int, [P, P, I64, P, C.POINTER(tt_trace_cfg), P]),
     "tt_trace_closest_host": (C.c_int, [P, P, I64, P, C.POINTER(tt_trace_cfg)]),

@@ -87,6 +87,89 @@ tt_status tt_ctx_sync(tt_ctx *ctx) {
     return TT_OK;
 }
 
+tt_status tt_dev_alloc(tt_ctx *ctx, size_t bytes, void **out) {
+    if (!ctx || !out) return fail(TT_ERR_INVALID, "tt_dev_alloc: bad arguments");
+    TT_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    return dev_alloc(ctx, out, bytes, "tt_dev_alloc");
+}
+
+tt_status tt_dev_free(tt_ctx *ctx, void *p) {
+    if (!ctx) return fail(TT_ERR_INVALID, "tt_dev_free: null ctx");
+    if (!p) return TT_OK;
+    TT_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    TT_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    TT_HIP(hipFree(p), "hipFree");
+    return TT_OK;
+}
+
+tt_status tt_memcpy_h2d(tt_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return fail(TT_ERR_INVALID, "tt_memcpy_h2d: bad arguments");
+    if (!bytes) return TT_OK;
+    TT_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    TT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream), "hipMemcpyAsync h2d");
+    TT_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return TT_OK;
+}
+
+tt_status tt_memcpy_d2h(tt_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return fail(TT_ERR_INVALID, "tt_memcpy_d2h: bad arguments");
+    if (!bytes) return TT_OK;
+    TT_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    TT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync d2h");
+    TT_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return TT_OK;
+}
+
+tt_status tt_memset_d(tt_ctx *ctx, void *dst, int value, size_t bytes) {
+    if (!ctx || (bytes && !dst)) return fail(TT_ERR_INVALID, "tt_memset_d: bad arguments");
+    if (!bytes) return TT_OK;
+    TT_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    TT_HIP(hipMemsetAsync(dst, value, bytes, ctx->stream), "hipMemsetAsync");
+    TT_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return TT_OK;
+}
+
+void *tt_ctx_stream(tt_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+tt_status tt_timer_create(tt_ctx *ctx, int32_t n, tt_timer **out) {
+    if (!ctx || !out || n <= 0 || n > (1 << 20)) return fail(TT_ERR_INVALID, "tt_timer_create: bad arguments");
+    TT_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    auto *t = new (std::nothrow) tt_timer();
+    if (!t) return fail(TT_ERR_NOMEM, "tt_timer_create: host allocation failed");
+    t->ctx = ctx;
+    t->ev.resize(n, nullptr);
+    for (int32_t i = 0; i < n; ++i) {
+        hipError_t e = hipEventCreate(&t->ev[i]);
+        if (e != hipSuccess) {
+            tt_timer_destroy(t);
+            return fail(TT_ERR_DEVICE, std::string("hipEventCreate: ") + hipGetErrorString(e));
+        }
+    }
+    *out = t;
+    return TT_OK;
+}
+
+tt_status tt_timer_record(tt_timer *t, int32_t slot) {
+    if (!t || slot < 0 || slot >= (int32_t)t->ev.size()) return fail(TT_ERR_INVALID, "tt_timer_record: bad slot");
+    TT_HIP(hipEventRecord(t->ev[slot], t->ctx->stream), "hipEventRecord");
+    return TT_OK;
+}
+
+tt_status tt_timer_elapsed(tt_timer *t, int32_t a, int32_t b, float *ms) {
+    if (!t || !ms || a < 0 || b < 0 || a >= (int32_t)t->ev.size() || b >= (int32_t)t->ev.size())
+        return fail(TT_ERR_INVALID, "tt_timer_elapsed: bad slots");
+    TT_HIP(hipEventSynchronize(t->ev[b]), "hipEventSynchronize");
+    TT_HIP(hipEventElapsedTime(ms, t->ev[a], t->ev[b]), "hipEventElapsedTime");
+    return TT_OK;
+}
+
+void tt_timer_destroy(tt_timer *t) {
+    if (!t) return;
+    for (hipEvent_t e : t->ev)
+        if (e) hipEventDestroy(e);
+    delete t;
+}
+
 tt_status tt_scene_create(tt_ctx *ctx, const tt_mesh *mesh, const tt_bvh *bvh, const tt_material *mats, int32_t nmats,
                           const float *env_rgb, int32_t env_w, int32_t env_h, float env_scale, tt_scene **out) {
     if (!ctx || !bvh || !out) return fail(TT_ERR_INVALID, "tt_scene_create: null argument");

@@ -5,6 +5,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "../../../include/tt.h"
 
 struct tt_ctx {
@@ -38,6 +40,11 @@ struct tt_scene {
     float env_scale = 1.0f;
     double env_total = 0.0;
     size_t bytes = 0;
+};
+
+struct tt_timer {
+    tt_ctx *ctx = nullptr;
+    std::vector<hipEvent_t> ev;
 };
 
 namespace tt {
