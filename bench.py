@@ -28,7 +28,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--variant", type=int, default=0)
     ap.add_argument("--lds-stack", type=int, default=16)
-    ap.add_argument("--block", type=int, default=128)
+    ap.add_argument("--block", type=int, default=64)
     ap.add_argument("--ntris", type=int, default=1_000_000)
     ap.add_argument("--cpu-sample-stride", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -80,6 +80,30 @@ def main():
     hits = d_hits.download((n, 4), np.int32)
     hit_frac = float((hits[:, 0] >= 0).mean())
 
+    # the speculative while-while variant (wave-ballot leaf postponement), timed
+    # beside the exact one; its hits are compared with the exact variant's
+    spec_cfg = tt.trace_cfg(variant=1, lds_stack=args.lds_stack, block=args.block)
+    d_hits2 = tt.DeviceBuffer(ctx, n * 16)
+    scene.trace_closest(d_rays, n, d_hits2, spec_cfg)
+    spec_ms = []
+    for i in range(min(args.steps, 5)):
+        timer.record(0)
+        scene.trace_closest(d_rays, n, d_hits2, spec_cfg)
+        timer.record(1)
+        spec_ms.append(timer.elapsed(0, 1))
+    hits2 = d_hits2.download((n, 4), np.int32)
+    spec = {"mrays_kernel": round(n / float(np.median(spec_ms)) / 1e3, 1),
+            "hit_id_diffs_vs_exact": int((hits2[:, 0] != hits[:, 0]).sum())}
+    d_hits2.free()
+    log(f"speculative variant: {spec}")
+
+    # measured node-fetch roofline: random 64-byte gathers over a buffer the size
+    # of the device BVH (nodes + triangle records)
+    probe = tt.DeviceBuffer(ctx, info["device_bytes"])
+    gather_gbps = max(tt.gather64_gbps(ctx, probe, info["device_bytes"], seed=s) for s in (1, 2, 3))
+    probe.free()
+    log(f"random 64-byte gather over {info['device_bytes'] / 1e6:.0f} MB: {gather_gbps:.0f} GB/s")
+
     result = {
         "metric": "Mrays/s closest-hit",
         "value": round(mrays, 2),
@@ -101,6 +125,7 @@ def main():
             "block": args.block, "hit_fraction": round(hit_frac, 4),
         },
         "kernel_ms": round(kmean, 4),
+        "speculative_variant": spec,
         "roofline": None,
         "cpu_baseline": None,
     }
@@ -129,6 +154,8 @@ def main():
             "bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
             "frac": round(achieved / 8000.0, 4), "traffic": None,
             "bytes_per_ray": round(bpr, 1),
+            "node_fetch_roofline_gbps": round(gather_gbps, 1),
+            "node_fetch_frac": round(achieved / gather_gbps, 4),
             "node_visits_per_ray": round(st["inner"] / len(sample), 2),
             "tri_tests_per_ray": round(st["tris"] / len(sample), 2),
         }

@@ -184,6 +184,14 @@ tt_status tt_dev_free(tt_ctx *ctx, void *p);
 tt_status tt_memcpy_h2d(tt_ctx *ctx, void *dst, const void *src, size_t bytes);
 tt_status tt_memcpy_d2h(tt_ctx *ctx, void *dst, const void *src, size_t bytes);
 tt_status tt_memset_d(tt_ctx *ctx, void *dst, int value, size_t bytes);
+/* Device-to-device copy by a plain streaming kernel (16-byte loads/stores); its
+ * HBM counters calibrate traffic measurements of the other kernels. bytes must
+ * be a multiple of 16 and both pointers 16-byte aligned. Asynchronous. */
+tt_status tt_dev_copy(tt_ctx *ctx, void *dst, const void *src, size_t bytes);
+/* Node-fetch roofline probe: `fetches` independent 64-byte reads (4 x 16 B) at
+ * uniformly random 64-byte-aligned offsets of buf[0, bytes), 16 per lane; returns
+ * the kernel time in ms. bytes >= 64 and buf 64-byte aligned. Synchronous. */
+tt_status tt_bench_gather64(tt_ctx *ctx, const void *buf, size_t bytes, int64_t fetches, uint32_t seed, float *ms);
 typedef struct tt_timer tt_timer;
 tt_status tt_timer_create(tt_ctx *ctx, int32_t n, tt_timer **out);
 tt_status tt_timer_record(tt_timer *t, int32_t slot);

@@ -104,24 +104,32 @@ def test_build_deterministic_across_threads(tt):
 
 
 def test_compact_and_device_layout_agree(tt):
+    """Walk both layouts from the root: same boxes, same child order, same triangles."""
     m = tt.Mesh.soup(2000, seed=9)
     b = tt.BVH.build(m)
     c = b.export_compact()
     dn, dt = b.device_layout()
-    assert len(dn) * 4 == len(c["nodes"])
     rn = c["nodes"].reshape(-1, 16)
-    assert np.array_equal(rn[:, :12], dn[:, :12])
     dbits = dt.view(np.int32)
-    # walk every leaf in both layouts
-    for k in range(len(dn)):
+    vbits = c["vertices"].view(np.int32)
+    stack = [(0, 0)]
+    seen = 0
+    live = []
+    while stack:
+        rk, dk = stack.pop()
+        seen += 1
+        live.append(dk)
+        assert np.array_equal(rn[rk, :12], dn[dk, :12])
         for i in range(2):
-            rc, dc = rn[k, 12 + i], dn[k, 12 + i]
+            rc, dc = rn[rk, 12 + i], dn[dk, 12 + i]
+            assert (rc >= 0) == (dc >= 0)
             if rc >= 0:
-                assert dc * 4 == rc
+                assert rc % 4 == 0
+                stack.append((rc // 4, dc))
                 continue
             off, j = ~rc, ~dc
             got = []
-            while c["vertices"].view(np.int32)[off, 0] != np.int32(-2 ** 31):
+            while vbits[off, 0] != np.int32(-2 ** 31):
                 v0, v1, v2 = c["vertices"][off:off + 3, :3]
                 rec = dt[j]
                 assert np.array_equal(rec[0:3], v0)
@@ -132,8 +140,13 @@ def test_compact_and_device_layout_agree(tt):
                 got.append(j)
                 off += 3
                 j += 1
-            assert len(got) == dn[k, 14 + i]
+            assert len(got) == dn[dk, 14 + i]
             assert dbits[got[-1], 11] == 1 and all(dbits[g, 11] == 0 for g in got[:-1])
+    assert seen == len(rn)
+    # inner siblings share a 128-byte line
+    for k in live:
+        if dn[k, 12] >= 0 and dn[k, 13] >= 0:
+            assert dn[k, 12] % 2 == 0 and dn[k, 13] == dn[k, 12] + 1
 
 
 def test_negative_zero_vertex_sanitised(tt):
@@ -154,8 +167,7 @@ def test_degenerate_scenes_flatten(tt, n):
     m = tt.Mesh.soup(n, seed=1) if n else tt.Mesh.from_arrays(np.zeros((0, 3)), np.zeros((0, 3)))
     b = tt.BVH.build(m)
     dn, dt = b.device_layout()
-    assert len(dn) == 1          # wrapped root
-    assert dn[0, 12] < 0 and dn[0, 13] < 0
+    assert dn[0, 12] < 0 and dn[0, 13] < 0   # wrapped root: two leaves
     c = b.export_compact()
     assert c["leaf_count"] == 2
 

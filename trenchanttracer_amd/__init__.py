@@ -255,6 +255,10 @@ class DeviceBuffer:
         check(lib.tt_memcpy_d2h(self.ctx._h, C.c_void_p(out.ctypes.data), C.c_void_p(self.ptr), out.nbytes))
         return out
 
+    def copy_from(self, other, nbytes=None):
+        n = self.nbytes if nbytes is None else int(nbytes)
+        check(lib.tt_dev_copy(self.ctx._h, C.c_void_p(self.ptr), C.c_void_p(other.ptr), n))
+
     def zero(self):
         check(lib.tt_memset_d(self.ctx._h, C.c_void_p(self.ptr), 0, self.nbytes))
 
@@ -270,6 +274,13 @@ class DeviceBuffer:
             self.free()
         except Exception:
             pass
+
+
+def gather64_gbps(ctx, buf, nbytes, fetches=1 << 26, seed=1):
+    """Random 64-byte gather bandwidth (GB/s) over the first nbytes of a DeviceBuffer."""
+    ms = C.c_float()
+    check(lib.tt_bench_gather64(ctx._h, C.c_void_p(buf.ptr), int(nbytes), int(fetches), int(seed), C.byref(ms)))
+    return fetches * 64 / (ms.value * 1e-3) / 1e9
 
 
 class Timer:

@@ -109,6 +109,8 @@ PROTOS = {
     "tt_memcpy_h2d": (C.c_int, [P, P, P, C.c_size_t]),
     "tt_memcpy_d2h": (C.c_int, [P, P, P, C.c_size_t]),
     "tt_memset_d": (C.c_int, [P, P, C.c_int, C.c_size_t]),
+    "tt_bench_gather64": (C.c_int, [P, P, C.c_size_t, I64, U32, C.POINTER(C.c_float)]),
+    "tt_dev_copy": (C.c_int, [P, P, P, C.c_size_t]),
     "tt_timer_create": (C.c_int, [P, I32, C.POINTER(P)]),
     "tt_timer_record": (C.c_int, [P, I32]),
     "tt_timer_elapsed": (C.c_int, [P, I32, I32, C.POINTER(C.c_float)]),

@@ -129,6 +129,82 @@ tt_status tt_memset_d(tt_ctx *ctx, void *dst, int value, size_t bytes) {
     return TT_OK;
 }
 
+namespace {
+__global__ __launch_bounds__(256) void k_copy16(float4 *__restrict__ dst, const float4 *__restrict__ src, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+constexpr int kGatherPerLane = 16;
+
+__global__ __launch_bounds__(256) void k_gather64(const float4 *__restrict__ buf, uint64_t nblocks, uint32_t seed,
+                                                  int64_t lanes, uint32_t *__restrict__ sink) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= lanes) return;
+    uint32_t h = (uint32_t)t * 0x9E3779B9u ^ seed;
+    uint32_t acc = 0;
+    float4 v[kGatherPerLane][4];
+#pragma unroll
+    for (int i = 0; i < kGatherPerLane; ++i) {
+        h ^= h << 13;
+        h ^= h >> 17;
+        h ^= h << 5;
+        uint64_t blk = ((uint64_t)h * (uint64_t)(h ^ 0x5bd1e995u)) % nblocks;
+        const float4 *p = buf + blk * 4;
+        v[i][0] = p[0];
+        v[i][1] = p[1];
+        v[i][2] = p[2];
+        v[i][3] = p[3];
+    }
+#pragma unroll
+    for (int i = 0; i < kGatherPerLane; ++i)
+        for (int k = 0; k < 4; ++k)
+            acc ^= __float_as_uint(v[i][k].x) ^ __float_as_uint(v[i][k].y) ^ __float_as_uint(v[i][k].z) ^
+                   __float_as_uint(v[i][k].w);
+    sink[t] = acc;
+}
+
+}  // namespace
+
+tt_status tt_bench_gather64(tt_ctx *ctx, const void *buf, size_t bytes, int64_t fetches, uint32_t seed, float *ms) {
+    if (!ctx || !buf || !ms || bytes < 64 || fetches <= 0 || ((uintptr_t)buf % 64))
+        return fail(TT_ERR_INVALID, "tt_bench_gather64: bad arguments");
+    TT_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    int64_t lanes = (fetches + kGatherPerLane - 1) / kGatherPerLane;
+    if ((lanes + 255) / 256 > 0x7FFFFFFF) return fail(TT_ERR_INVALID, "tt_bench_gather64: too many fetches");
+    void *sink = nullptr;
+    tt_status st = dev_alloc(ctx, &sink, (size_t)lanes * 4, "gather sink");
+    if (st != TT_OK) return st;
+    hipEvent_t a, b;
+    hipEventCreate(&a);
+    hipEventCreate(&b);
+    hipEventRecord(a, ctx->stream);
+    hipLaunchKernelGGL(k_gather64, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, ctx->stream,
+                       (const float4 *)buf, (uint64_t)(bytes / 64), seed, lanes, (uint32_t *)sink);
+    hipError_t e = hipGetLastError();
+    hipEventRecord(b, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    if (e == hipSuccess) e = hipEventElapsedTime(ms, a, b);
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    hipFree(sink);
+    if (e != hipSuccess) return fail(TT_ERR_DEVICE, std::string("gather probe: ") + hipGetErrorString(e));
+    return TT_OK;
+}
+
+tt_status tt_dev_copy(tt_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src)) || bytes % 16 || ((uintptr_t)dst | (uintptr_t)src) % 16)
+        return fail(TT_ERR_INVALID, "tt_dev_copy: bad arguments");
+    if (!bytes) return TT_OK;
+    TT_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    size_t n = bytes / 16;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > (size_t)ctx->num_cus * 64) blocks = (size_t)ctx->num_cus * 64;
+    hipLaunchKernelGGL(k_copy16, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, (float4 *)dst, (const float4 *)src, n);
+    TT_HIP(hipGetLastError(), "copy launch");
+    return TT_OK;
+}
+
 void *tt_ctx_stream(tt_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
 
 tt_status tt_timer_create(tt_ctx *ctx, int32_t n, tt_timer **out) {

@@ -320,6 +320,7 @@ static tt_status compact_to_device(Flat &f) {
             f.dev_nodes[k * 16 + 12 + c] = ~lc.first;
             f.dev_nodes[k * 16 + 14 + c] = lc.second;
         }
+    pair_align(f.dev_nodes);
     return TT_OK;
 }
 
@@ -353,7 +354,7 @@ tt_status tt_bvh_load(const char *path, tt_bvh **out) {
         uint32_t ver;
         int64_t dn, dt;
         if (!rd(fp, &ver, 4) || ver != 1 || !rd(fp, &dn, 8) || !rd(fp, &dt, 8) || dn < 0 || dt < 0 ||
-            dn != (int64_t)hdr[0] / 4 || dt > (int64_t)hdr[3])
+            dn < (int64_t)hdr[0] / 4 || dn > 2 * ((int64_t)hdr[0] / 4) + 2 || dt > (int64_t)hdr[3])
             return fail(TT_ERR_IO, "Error reading BVH cache file: bad trailer");
         if (!rd(fp, &b->b.stats, sizeof(tt_bvh_stats))) return fail(TT_ERR_IO, "Error reading BVH cache file: bad trailer");
         f.dev_nodes.resize((size_t)dn * 16);

@@ -13,7 +13,8 @@
 //      terminator fetch and the edges are not recomputed. A leaf without
 //      triangles holds one degenerate record (zero edges: det = 0, never
 //      accepted).
-// Both use the same node order, so node k of (2) is float4 offset 4k of (1).
+// The device nodes are then renumbered so that inner siblings share one 128-byte
+// line (pair_align); node order does not change any traversal decision.
 #include <climits>
 #include <cstring>
 
@@ -34,6 +35,54 @@ inline float bitsf(int32_t i) {
 }
 
 }  // namespace
+
+// Renumbers the device nodes so that two inner siblings share one 128-byte
+// L2 line (slots 2k, 2k+1), in depth-first order from the root. The root takes
+// slot 0 and slot 1 stays unused; a lone inner child takes half of a fresh
+// pair and the next lone child fills the other half. Only memory positions
+// change: children keep their order (c0, c1), so traversal decisions, and
+// therefore results, are the same.
+void pair_align(std::vector<int32_t> &nodes) {
+    const int32_t n = (int32_t)(nodes.size() / 16);
+    if (n <= 1) return;
+    std::vector<int32_t> slot(n, -1);
+    int32_t next = 2, pending = -1;
+    slot[0] = 0;
+    std::vector<int32_t> stack = {0};
+    while (!stack.empty()) {
+        int32_t k = stack.back();
+        stack.pop_back();
+        int32_t c0 = nodes[(size_t)k * 16 + 12], c1 = nodes[(size_t)k * 16 + 13];
+        if (c0 >= 0 && c1 >= 0) {
+            slot[c0] = next;
+            slot[c1] = next + 1;
+            next += 2;
+        } else {
+            for (int32_t c : {c0, c1}) {
+                if (c < 0) continue;
+                if (pending >= 0) {
+                    slot[c] = pending;
+                    pending = -1;
+                } else {
+                    slot[c] = next;
+                    pending = next + 1;
+                    next += 2;
+                }
+            }
+        }
+        if (c1 >= 0) stack.push_back(c1);
+        if (c0 >= 0) stack.push_back(c0);
+    }
+    std::vector<int32_t> out((size_t)next * 16, 0);
+    for (int32_t k = 0; k < n; ++k) {
+        int32_t *d = &out[(size_t)slot[k] * 16];
+        std::memcpy(d, &nodes[(size_t)k * 16], 64);
+        for (int c = 0; c < 2; ++c)
+            if (d[12 + c] >= 0) d[12 + c] = slot[d[12 + c]];
+    }
+    // unused slots stay zero; no child pointer refers to them
+    nodes.swap(out);
+}
 
 void flatten(const Mesh &m, BVH &b) {
     Flat &f = b.flat;
@@ -153,6 +202,7 @@ void flatten(const Mesh &m, BVH &b) {
         dn[14] = dev_cnt[0];
         dn[15] = dev_cnt[1];
     }
+    pair_align(f.dev_nodes);
 }
 
 }  // namespace tt

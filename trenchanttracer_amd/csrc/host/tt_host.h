@@ -111,6 +111,7 @@ struct BVH {
 void build_sbvh(const Mesh &m, float split_alpha, int nthreads, BVH &out);
 // flatten.cpp
 void flatten(const Mesh &m, BVH &b);
+void pair_align(std::vector<int32_t> &dev_nodes);
 // gen.cpp helpers
 void face_normal_fill(Mesh &m);
 
