@@ -257,6 +257,64 @@ tt_status tt_trace_closest_host(tt_scene *s, const tt_ray *rays, int64_t n, tt_h
 tt_status tt_trace_any_host(tt_scene *s, const tt_ray *rays, int64_t n, int32_t *hit_flags,
                             const tt_trace_cfg *cfg);
 
+/* ------------------------------------------------------------------ render */
+/* Pinhole / thin-lens camera with the fields of CameraMeta (include/core/Camera.h:8-16). */
+typedef struct tt_camera {
+    float eye[3], view[3], up[3];
+    float fov[2];            /* degrees: fov[0] horizontal, fov[1] vertical */
+    float aperture, focal;   /* aperture radius (lens sampled if > 1e-5), focal distance */
+    int32_t width, height;
+} tt_camera;
+
+/* Orbit camera as Camera::Camera + setFovx + getCameraMeta (src/core/Camera.cpp:12-44,
+ * 115-140): eye = center + (sin yaw cos pitch, sin pitch, cos yaw cos pitch) * radius,
+ * view = -(that direction), up = +y, fov[1] from fov[0] and the aspect ratio with the
+ * reference's PI = 3.14156265f in the degree conversions. */
+tt_status tt_camera_from_orbit(const float center[3], float yaw, float pitch, float radius, float fovx_deg,
+                               int32_t width, int32_t height, float focal, float aperture, tt_camera *out);
+
+typedef enum {
+    TT_RENDER_REFERENCE = 0,  /* the reference megakernel's semantics: emission on hit, BSDF sampling only,
+                                 reference samplers (Lambert / Phong-30 metal / IOR dielectric), front-face
+                                 triangle test; bit-identical to the host restatement for equal inputs */
+    TT_RENDER_PATH = 1        /* path tracer: Lambert / GGX conductor (VNDF) / smooth dielectric, next-event
+                                 estimation of emitters and the env map with MIS (power heuristic), Russian
+                                 roulette, two-sided triangle test */
+} tt_render_mode;
+
+typedef struct tt_render_params {
+    int32_t spp;              /* samples per pixel (one wavefront pass each) */
+    int32_t max_depth;        /* path segments (the reference hard-codes 10 bounces, RenderKernel.cu:35) */
+    int32_t rr_depth;         /* Russian roulette from this segment on (path mode) */
+    uint32_t seed;            /* Philox key */
+    int32_t mode;             /* tt_render_mode */
+    int32_t guided;           /* path mode: 1 = learned guiding proposal (one-sample MIS with the BSDF) */
+    int32_t sample_offset;    /* index of the first sample (progressive rendering) */
+    int32_t accumulate_fp64;  /* accumulate pixel sums in FP64 (oracle renders) */
+    float pi, two_pi;         /* camera / env constants (reference: 3.14156265f and 6.2831853f) */
+    float tmin;               /* reference mode ray tmin (0.01, RenderKernel.cu:110) */
+    float reserved;
+} tt_render_params;
+
+typedef struct tt_render_stats {
+    int64_t rays_extension;   /* closest-hit rays traced */
+    int64_t rays_shadow;      /* any-hit rays traced */
+    int64_t samples;          /* width * height * spp */
+    double seconds;           /* GPU time of the render (HIP events) */
+    int32_t iterations;       /* wavefront iterations launched */
+    int32_t pad;
+} tt_render_stats;
+
+/* Renders one image. rgb_out (host, width*height*3 floats, row 0 = top) receives the
+ * per-pixel mean over the spp samples; rgb64_out (host, may be NULL) the FP64 mean
+ * when accumulate_fp64 is set. Synchronous. */
+tt_status tt_render(tt_scene *s, const tt_camera *cam, const tt_render_params *p, float *rgb_out,
+                    double *rgb64_out, tt_render_stats *stats);
+
+/* Reproducibility hook for the render math: applies op (0 sin, 1 cos, 2 pow(x, y)) of
+ * the device sampler math to n device values; y is read for op 2. Synchronous. */
+tt_status tt_debug_math(tt_ctx *ctx, int32_t op, const float *x, const float *y, float *out, int64_t n);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

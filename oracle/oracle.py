@@ -22,6 +22,21 @@ class o_counts(C.Structure):
     _fields_ = [("inner", C.c_int64), ("tris", C.c_int64), ("leaves", C.c_int64)]
 
 
+class o_camera(C.Structure):
+    _fields_ = [("eye", C.c_float * 3), ("view", C.c_float * 3), ("up", C.c_float * 3), ("fov", C.c_float * 2),
+                ("aperture", C.c_float), ("focal", C.c_float), ("width", C.c_int32), ("height", C.c_int32)]
+
+
+class o_material(C.Structure):
+    _fields_ = [("type", C.c_int32), ("albedo", C.c_float * 3), ("emission", C.c_float * 3),
+                ("roughness", C.c_float), ("ior", C.c_float), ("pad", C.c_float * 3)]
+
+
+class o_ref_params(C.Structure):
+    _fields_ = [("spp", C.c_int32), ("max_depth", C.c_int32), ("seed", C.c_uint32), ("sample_offset", C.c_int32),
+                ("pi", C.c_float), ("two_pi", C.c_float), ("tmin", C.c_float), ("tmax", C.c_float)]
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         from . import build as _b  # noqa
@@ -39,6 +54,9 @@ def _load():
     lib.o_trace_batch.restype = C.c_double
     lib.o_tri_test.argtypes = [P, P, P, P, C.c_float, P, P, P]
     lib.o_tri_test.restype = C.c_int
+    lib.o_render_ref.argtypes = [C.POINTER(o_bvh), P, P, P, C.POINTER(o_camera), C.POINTER(o_ref_params), C.c_int32,
+                                 C.c_int32, P, C.c_int]
+    lib.o_render_ref.restype = C.c_int
     for name in ("o_sinf", "o_cosf", "o_logf", "o_expf"):
         if hasattr(lib, name):
             getattr(lib, name).argtypes = [C.c_float]
@@ -108,3 +126,38 @@ def tri_test(v0, v1, v2, ray, tmax=None):
 
 def hit_ids(h):
     return np.ascontiguousarray(h[:, 0]).view(np.int32)
+
+
+def tri_normal_table(mesh_arrays):
+    """12 floats (3 float4) per triangle: the shading normals of its vertices."""
+    n = mesh_arrays["normals"][mesh_arrays["tri_normals"]]  # (T, 3, 3)
+    out = np.zeros((len(n), 3, 4), np.float32)
+    out[:, :, :3] = n
+    return out.reshape(-1, 12)
+
+
+def render_ref(compact_bvh, mesh_arrays, materials, cam, spp, max_depth, seed, pi=3.14156265,
+               two_pi=6.2831853071795864769, tmin=0.01, rows=None, nthreads=None, sample_offset=0):
+    """Reference-semantics render on the CPU: (H, W, 3) float32 image (rows outside `rows` stay 0)."""
+    oc = o_camera()
+    for f in ("eye", "view", "up", "fov"):
+        getattr(oc, f)[:] = list(getattr(cam, f))
+    oc.aperture, oc.focal, oc.width, oc.height = cam.aperture, cam.focal, cam.width, cam.height
+    mats = (o_material * len(materials))()
+    for k, m in enumerate(materials):
+        mats[k].type = m.type
+        mats[k].albedo[:] = list(m.albedo)
+        mats[k].emission[:] = list(m.emission)
+        mats[k].roughness, mats[k].ior = m.roughness, m.ior
+    prm = o_ref_params(int(spp), int(max_depth), int(seed), int(sample_offset), float(pi), float(two_pi),
+                       float(tmin), 1e20)
+    tn = tri_normal_table(mesh_arrays)
+    tm = np.ascontiguousarray(mesh_arrays["tri_mats"], np.int32)
+    out = np.zeros((cam.height, cam.width, 3), np.float32)
+    y0, y1 = rows if rows is not None else (0, cam.height)
+    nt = nthreads or min(8, os.cpu_count() or 1)
+    rc = lib.o_render_ref(C.byref(compact_bvh.c), _p(tn), _p(tm), C.cast(mats, C.c_void_p), C.byref(oc),
+                          C.byref(prm), int(y0), int(y1), _p(out), nt)
+    if rc != 0:
+        raise RuntimeError("o_render_ref failed")
+    return out

@@ -1,0 +1,294 @@
+/*
+ * tt_oracle_render.c — CPU restatement of the reference path tracer
+ * (src/core/RenderKernel.cu:27-218 with DiffImpl.cuh, MetalImpl.cuh, RefrImpl.cuh,
+ * HDRKernel.cuh) for image parity of the reference render mode. TEST
+ * INFRASTRUCTURE: the checker, never part of the product.
+ *
+ * Differences from the reference that the GPU reference mode shares:
+ *   - random numbers: Philox-4x32-10 addressed by (pixel, sample, bounce), uniform
+ *     in (0, 1] like curand_uniform; cuRAND's XORWOW stream is not reproduced;
+ *   - sinf / cosf / powf: the functions below (the reference calls CUDA's);
+ *   - the miss colour is black (no environment image in this restatement);
+ *   - the accumulation buffer starts at 0 (the reference memsets it to bytes of 1,
+ *     i.e. 2.4e-38 per channel, Renderer.cpp:28).
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stdint.h>
+#include <string.h>
+
+#include "tt_oracle.h"
+
+/* ---------------------------------------------------------------- float math */
+/* Cody–Waite reduction by pi/2 with a three-part constant (12 + 12 + 24 bits),
+ * then Taylor polynomials of sin (to r^9) and cos (to r^10) on |r| <= pi/4. */
+static float o_sincos(float x, float *c_out) {
+    float k = rintf(x * 0x1.45f306p-1f);
+    float r = x - k * 0x1.92p+0f;
+    r = r - k * 0x1.fb4p-12f;
+    r = r - k * 0x1.4442d2p-24f;
+    float r2 = r * r;
+    float ps = -0x1.555556p-3f + r2 * (0x1.111112p-7f + r2 * (-0x1.a01a02p-13f + r2 * 0x1.71de3ap-19f));
+    float s = r + r * r2 * ps;
+    float pc = 0x1.555556p-5f + r2 * (-0x1.6c16c2p-10f + r2 * (0x1.a01a02p-16f + r2 * -0x1.27e4fcp-22f));
+    float c = 1.0f - 0.5f * r2 + r2 * r2 * pc;
+    int q = ((int)k) & 3;
+    float sn, cs;
+    if (q == 0) { sn = s; cs = c; }
+    else if (q == 1) { sn = c; cs = -s; }
+    else if (q == 2) { sn = -s; cs = -c; }
+    else { sn = -c; cs = s; }
+    *c_out = cs;
+    return sn;
+}
+
+float o_sinf(float x) { float c; return o_sincos(x, &c); }
+float o_cosf(float x) { float c; o_sincos(x, &c); return c; }
+
+static inline uint32_t fbits(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+static inline float bitsf(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+
+/* log2 by x = m 2^e, m in [sqrt(1/2), sqrt(2)), ln m = 2 atanh((m-1)/(m+1)) series to f^11 */
+static float o_log2f(float x) {
+    int e = 0;
+    if (x < 0x1p-126f) { x = x * 0x1p24f; e = -24; }
+    uint32_t b = fbits(x);
+    e += (int)((b >> 23) & 0xffu) - 127;
+    float m = bitsf((b & 0x7fffffu) | 0x3f800000u);
+    if (m > 0x1.6a09e6p+0f) { m = m * 0.5f; e = e + 1; }
+    float f = (m - 1.0f) / (m + 1.0f);
+    float f2 = f * f;
+    float p = f2 * (0x1.555556p-2f + f2 * (0x1.99999ap-3f + f2 * (0x1.24924ap-3f + f2 * (0x1.c71c72p-4f + f2 * 0x1.745d18p-4f))));
+    float ln_m = (f + f * p) * 2.0f;
+    return (float)e + ln_m * 0x1.715476p+0f;
+}
+
+/* 2^z = 2^n e^(g ln 2), |g| <= 1/2, Taylor to t^8 */
+static float o_exp2f(float z) {
+    if (z < -150.0f) return 0.0f;
+    if (z >= 128.0f) return INFINITY;
+    float n = rintf(z);
+    float g = z - n;
+    float t = g * 0x1.62e430p-1f;
+    float p = 1.0f + t * (1.0f + t * (0.5f + t * (0x1.555556p-3f + t * (0x1.555556p-5f + t * (0x1.111112p-7f +
+              t * (0x1.6c16c2p-10f + t * (0x1.a01a02p-13f + t * 0x1.a01a02p-16f)))))));
+    int ni = (int)n;
+    if (ni < -126) { p = p * 0x1p-64f; ni += 64; }
+    return p * bitsf((uint32_t)(ni + 127) << 23);
+}
+
+float o_logf(float x) { return o_log2f(x) * 0x1.62e430p-1f; }
+float o_expf(float x) { return o_exp2f(x * 0x1.715476p+0f); }
+/* x in [0, 1], y > 0 */
+float o_powf(float x, float y) {
+    if (x == 0.0f) return 0.0f;
+    return o_exp2f(y * o_log2f(x));
+}
+
+/* ---------------------------------------------------------------- Vec3f ops
+ * operand order of include/math/LinearMath.h */
+typedef struct { float x, y, z; } v3;
+static inline v3 V(float x, float y, float z) { v3 r = {x, y, z}; return r; }
+static inline v3 vadd(v3 a, v3 b) { return V(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline v3 vsub3(v3 a, v3 b) { return V(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline v3 vmul(v3 a, v3 b) { return V(a.x * b.x, a.y * b.y, a.z * b.z); }
+static inline v3 vscale(v3 a, float s) { return V(a.x * s, a.y * s, a.z * s); }
+static inline float vdot3(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline v3 vcross3(v3 a, v3 b) { return V(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+/* Vec3f::normalize (LinearMath.h:101-107): three divisions by the norm */
+static inline v3 vnorm(v3 a) {
+    float n = sqrtf(a.x * a.x + a.y * a.y + a.z * a.z);
+    return V(a.x / n, a.y / n, a.z / n);
+}
+
+/* ---------------------------------------------------------------- RNG */
+/* uniform in (0, 1] from the top 24 bits */
+static inline float u01c(uint32_t x) { return (float)((x >> 8) + 1u) * (1.0f / 16777216.0f); }
+
+static void draws(uint32_t pix, uint32_t sample, uint32_t slot, uint32_t seed, float u[4]) {
+    uint32_t ctr[4] = {pix, sample, slot, 0u}, key[2] = {seed, 0x5EEDu}, o[4];
+    o_philox4x32_10(ctr, key, o);
+    for (int i = 0; i < 4; ++i) u[i] = u01c(o[i]);
+}
+
+/* ---------------------------------------------------------------- samplers */
+typedef struct {
+    v3 o, d;
+    float tmin, tmax;
+} ray3;
+
+/* diffSample, DiffImpl.cuh:12-36 */
+static ray3 diff_sample(const o_material *m, const float u[4], ray3 r, v3 point, v3 nl, v3 *mask) {
+    float phi = (float)(2.0 * 3.14159265358979323846 * (double)u[0]);
+    float r2 = u[1];
+    float r2s = sqrtf(r2);
+    v3 w = vnorm(nl);
+    v3 a = fabsf(w.x) >= 0.1f ? V(0, 1, 0) : V(1, 0, 0); /* (double)|w.x| > .1 */
+    v3 uu = vnorm(vcross3(a, w));
+    v3 vv = vcross3(w, uu);
+    float c, s = o_sincos(phi, &c);
+    v3 d = vadd(vadd(vscale(vscale(uu, c), r2s), vscale(vscale(vv, s), r2s)), vscale(w, sqrtf(1 - r2)));
+    r.d = vnorm(d);
+    r.o = vadd(point, vscale(nl, 0.001f));
+    *mask = vmul(*mask, V(m->albedo[0], m->albedo[1], m->albedo[2]));
+    return r;
+}
+
+/* metalSample, MetalImpl.cuh:13-41 (Phong lobe, exponent 30) */
+static ray3 metal_sample(const o_material *m, const float u[4], ray3 r, v3 point, v3 n, v3 nl, v3 *mask) {
+    float phi = (float)(2.0 * 3.14159265358979323846 * (double)u[0]);
+    float r2 = u[1];
+    float cos_t = o_powf(1 - r2, 1.0f / (30.0f + 1));
+    float sin_t = sqrtf(1 - cos_t * cos_t);
+    v3 w = vnorm(vsub3(r.d, vscale(vscale(n, 2.0f), vdot3(n, r.d))));
+    v3 a = fabsf(w.x) >= 0.1f ? V(0, 1, 0) : V(1, 0, 0);
+    v3 uu = vnorm(vcross3(a, w));
+    v3 vv = vcross3(w, uu);
+    float c, s = o_sincos(phi, &c);
+    v3 d = vadd(vadd(vscale(vscale(uu, c), sin_t), vscale(vscale(vv, s), sin_t)), vscale(w, cos_t));
+    r.d = vnorm(d);
+    r.o = vadd(point, vscale(nl, 0.0001f));
+    *mask = vmul(*mask, V(m->albedo[0], m->albedo[1], m->albedo[2]));
+    return r;
+}
+
+/* refrSample, RefrImpl.cuh:12-61 (IOR from the material; the reference fixes 1.4) */
+static ray3 refr_sample(const o_material *m, const float u[4], ray3 r, v3 point, v3 n, v3 nl, v3 *mask) {
+    int into = vdot3(n, nl) > 0;
+    float nc = 1.0f, nt = m->ior;
+    float nnt = into ? nc / nt : nt / nc;
+    float ddn = vdot3(r.d, nl);
+    float cos2t = 1.0f - nnt * nnt * (1.f - ddn * ddn);
+    if (cos2t < 0.0f) {
+        r.d = vnorm(vsub3(r.d, vscale(vscale(n, 2.0f), vdot3(n, r.d))));
+        r.o = vadd(point, vscale(nl, 0.001f));
+    } else {
+        v3 tdir = vscale(r.d, nnt);
+        tdir = vsub3(tdir, vscale(n, (float)(into ? 1 : -1) * (ddn * nnt + sqrtf(cos2t))));
+        tdir = vnorm(tdir);
+        float R0 = (nt - nc) * (nt - nc) / (nt + nc) * (nt + nc);
+        float c = 1.f - (into ? -ddn : vdot3(tdir, n));
+        float Re = R0 + (1.f - R0) * c * c * c * c * c;
+        float Tr = 1 - Re;
+        float P = .25f + .5f * Re;
+        float RP = Re / P;
+        float TP = Tr / (1.f - P);
+        if (u[0] < 0.2f) { /* (double)u < 0.2 */
+            *mask = vscale(*mask, RP);
+            r.d = vnorm(vsub3(r.d, vscale(vscale(n, 2.0f), vdot3(n, r.d))));
+            r.o = vadd(point, vscale(nl, 0.001f));
+        } else {
+            *mask = vscale(*mask, TP);
+            r.d = vnorm(tdir);
+            r.o = vadd(point, vscale(nl, 0.001f));
+        }
+    }
+    return r;
+}
+
+/* ---------------------------------------------------------------- render */
+typedef struct {
+    const o_bvh *b;
+    const float *tri_normals;
+    const int32_t *tri_mat;
+    const o_material *mats;
+    const o_camera *cam;
+    const o_ref_params *p;
+    float *out;
+    int32_t y0, y1;
+} o_rjob;
+
+static void render_rows(const o_rjob *j) {
+    const o_camera *cam = j->cam;
+    const o_ref_params *p = j->p;
+    const int W = cam->width, H = cam->height;
+    /* camera basis and extents, RenderKernel.cu:131-146 */
+    v3 pos = V(cam->eye[0], cam->eye[1], cam->eye[2]);
+    v3 view = vnorm(V(cam->view[0], cam->view[1], cam->view[2]));
+    v3 up = vnorm(V(cam->up[0], cam->up[1], cam->up[2]));
+    v3 hax = vnorm(vcross3(view, up));
+    v3 vax = vnorm(vcross3(hax, view));
+    v3 middle = vadd(pos, view);
+    v3 horizontal = vscale(hax, tanf(cam->fov[0] * 0.5f * (p->pi / 180)));
+    v3 vertical = vscale(vax, tanf(-cam->fov[1] * 0.5f * (p->pi / 180)));
+    const float inv_spp = 1.0f / (float)p->spp;
+
+    for (int32_t Y = j->y0; Y < j->y1; ++Y)
+        for (int32_t X = 0; X < W; ++X) {
+            uint32_t pix = (uint32_t)(Y * W + X);
+            v3 final = V(0, 0, 0);
+            for (int s = 0; s < p->spp; ++s) {
+                uint32_t sg = (uint32_t)(p->sample_offset + s);
+                float u[4];
+                draws(pix, sg, 0u, p->seed, u);
+                float jx = u[0] - 0.5f, jy = u[1] - 0.5f;
+                float sx = (jx + (float)X) / (float)(W - 1);
+                float sy = (jy + (float)Y) / (float)(H - 1);
+                v3 on_plane = vadd(vadd(middle, vscale(horizontal, (2 * sx) - 1)), vscale(vertical, (2 * sy) - 1));
+                v3 on_image = vadd(pos, vscale(vsub3(on_plane, pos), cam->focal));
+                v3 ap = pos;
+                if (cam->aperture > 1e-5f) { /* (double)aperture > 0.00001 */
+                    float angle = p->two_pi * u[2];
+                    float dist = cam->aperture * sqrtf(u[3]);
+                    float ca, sa = o_sincos(angle, &ca);
+                    ap = vadd(vadd(pos, vscale(hax, ca * dist)), vscale(vax, sa * dist));
+                }
+                ray3 r;
+                r.d = vnorm(vnorm(vsub3(on_image, ap)));
+                r.o = ap;
+                r.tmin = p->tmin;
+                r.tmax = p->tmax;
+
+                /* renderKernel, RenderKernel.cu:27-95 */
+                v3 mask = V(1, 1, 1), acc = V(0, 0, 0);
+                for (int bounce = 0; bounce < p->max_depth; ++bounce) {
+                    o_ray q = {r.o.x, r.o.y, r.o.z, r.tmin, r.d.x, r.d.y, r.d.z, r.tmax};
+                    o_hit h;
+                    o_intersect(j->b, &q, 0, &h, NULL, NULL);
+                    if (h.tri < 0 || h.t > 1e19f) break; /* miss: black environment */
+                    const float *tn = j->tri_normals + 12 * (int64_t)h.tri;
+                    float w = 1 - h.u - h.v;
+                    v3 nrm = V(tn[0] * w + tn[4] * h.u + tn[8] * h.v, tn[1] * w + tn[5] * h.u + tn[9] * h.v,
+                               tn[2] * w + tn[6] * h.u + tn[10] * h.v);
+                    v3 n = vnorm(nrm);
+                    v3 nl = vdot3(n, r.d) < 0 ? n : vscale(n, -1);
+                    v3 point = vadd(r.o, vscale(r.d, h.t));
+                    const o_material *m = &j->mats[j->tri_mat[h.tri]];
+                    acc = vadd(acc, vmul(mask, V(m->emission[0], m->emission[1], m->emission[2])));
+                    draws(pix, sg, (uint32_t)(bounce + 1), p->seed, u);
+                    if (m->type == 0) r = diff_sample(m, u, r, point, nl, &mask);
+                    else if (m->type == 1) r = metal_sample(m, u, r, point, n, nl, &mask);
+                    else if (m->type == 2) r = refr_sample(m, u, r, point, n, nl, &mask);
+                    else break; /* emitter: emission only */
+                }
+                final = vadd(final, vscale(acc, inv_spp));
+            }
+            float *o = j->out + 3 * (int64_t)pix;
+            o[0] = final.x;
+            o[1] = final.y;
+            o[2] = final.z;
+        }
+}
+
+static void *render_worker(void *arg) {
+    render_rows((const o_rjob *)arg);
+    return NULL;
+}
+
+int o_render_ref(const o_bvh *b, const float *tri_normals, const int32_t *tri_mat, const o_material *mats,
+                 const o_camera *cam, const o_ref_params *p, int32_t y0, int32_t y1, float *out_rgb, int nthreads) {
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    if (y0 < 0 || y1 > cam->height || y0 > y1) return -1;
+    o_rjob jobs[256];
+    pthread_t th[256];
+    int32_t rows = y1 - y0;
+    for (int k = 0; k < nthreads; ++k) {
+        o_rjob jb = {b, tri_normals, tri_mat, mats, cam, p, out_rgb, y0 + rows * k / nthreads,
+                     y0 + rows * (k + 1) / nthreads};
+        jobs[k] = jb;
+        pthread_create(&th[k], NULL, render_worker, &jobs[k]);
+    }
+    for (int k = 0; k < nthreads; ++k) pthread_join(th[k], NULL);
+    return 0;
+}

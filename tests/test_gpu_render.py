@@ -1,0 +1,140 @@
+"""GPU render parity: reference mode vs the oracle restatement; path mode vs
+closed forms (furnace) and vs the reference mode's converged mean."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def test_device_math_bit_identical_to_oracle(tt, oracle, ctx):
+    _say("device math")
+    rng = np.random.default_rng(1)
+    x = np.concatenate([np.float32(np.linspace(0, 6.2831853, 20001)), rng.uniform(0, 6.3, 20000).astype(np.float32)])
+    for op, fn in ((0, oracle.lib.o_sinf), (1, oracle.lib.o_cosf)):
+        got = tt.debug_math(ctx, op, x)
+        want = np.array([fn(float(v)) for v in x], np.float32)
+        assert np.array_equal(got.view(np.int32), want.view(np.int32)), op
+    k = rng.integers(0, 1 << 24, 40000)
+    px = (1.0 - (k + 1) * 2.0 ** -24).astype(np.float32)
+    py = np.full_like(px, np.float32(1.0) / np.float32(31.0))
+    got = tt.debug_math(ctx, 2, px, py)
+    want = np.array([oracle.lib.o_powf(float(a), float(b)) for a, b in zip(px, py)], np.float32)
+    assert np.array_equal(got.view(np.int32), want.view(np.int32))
+
+
+def _ref_case(tt, oracle, ctx, mesh, mats, cam, spp, depth, seed):
+    b = tt.BVH.build(mesh)
+    s = tt.DeviceScene(ctx, mesh, b, mats)
+    img, st = s.render(cam, tt.render_params(spp=spp, max_depth=depth, seed=seed, mode=tt.RENDER_REFERENCE))
+    ref = oracle.render_ref(oracle.CompactBVH(b.export_compact()), mesh.arrays(), mats, cam, spp=spp,
+                            max_depth=depth, seed=seed)
+    s.close()
+    return img, ref, st
+
+
+def test_reference_render_cornell_matches_oracle(tt, oracle, ctx):
+    """Cornell box + icosphere-4, 256x256, 16 spp, depth 8, seed 1 (the CPU-reference case).
+    Tolerance |dL| <= 1e-3 * max(1, L) per pixel; expected bit-identical."""
+    from trenchanttracer_amd import workloads
+    _say("reference render cornell")
+    mesh, mats, cam = workloads.cornell()
+    img, ref, st = _ref_case(tt, oracle, ctx, mesh, mats, cam, 16, 8, 1)
+    tol = 1e-3 * np.maximum(1.0, np.abs(ref))
+    assert np.all(np.abs(img - ref) <= tol)
+    same = np.mean(img.view(np.int32) == ref.view(np.int32))
+    _say(f"cornell reference render: {same * 100:.3f}% of channels bit-identical, stats {st}")
+    assert same == 1.0
+    assert img.mean() > 0.01
+
+
+def test_reference_render_metal_and_glass_match_oracle(tt, oracle, ctx):
+    _say("reference render metal/glass")
+    from trenchanttracer_amd import workloads
+    mesh, mats, cam = workloads.cornell(subdiv=3, width=96, height=80)
+    mesh.append(tt.Mesh.icosphere(3, 0.35, (-0.7, -0.65, 0.8), mat=5))
+    mesh.append(tt.Mesh.icosphere(3, 0.35, (0.7, -0.65, 1.0), mat=6))
+    mats = mats + [tt.material(tt.MAT_CONDUCTOR, (0.9, 0.8, 0.6)), tt.material(tt.MAT_DIELECTRIC, (1, 1, 1), ior=1.4)]
+    img, ref, _ = _ref_case(tt, oracle, ctx, mesh, mats, cam, 8, 10, 7)
+    assert np.array_equal(img.view(np.int32), ref.view(np.int32))
+
+
+def _furnace_scene(tt, ctx, mat, env_value=1.0):
+    sphere = tt.Mesh.icosphere(5, 1.0, (0, 0, 0), mat=0)
+    env = np.full((16, 32, 3), env_value, np.float32)
+    b = tt.BVH.build(sphere)
+    return tt.DeviceScene(ctx, sphere, b, [mat], env=env)
+
+
+@pytest.mark.parametrize("kind", ["diffuse", "glass", "smooth_metal"])
+def test_path_mode_furnace(tt, ctx, kind):
+    """Uniform environment of radiance 1 around a unit sphere: a convex diffuse sphere
+    reflects albedo * 1; a clear glass sphere (absorption-free, Fresnel-split) and a
+    near-smooth conductor with F0 = 1 return 1."""
+    _say(f"furnace {kind}")
+    if kind == "diffuse":
+        mat, want = tt.material(tt.MAT_DIFFUSE, (0.5, 0.5, 0.5)), 0.5
+    elif kind == "glass":
+        mat, want = tt.material(tt.MAT_DIELECTRIC, (1, 1, 1), ior=1.5), 1.0
+    else:
+        mat, want = tt.material(tt.MAT_CONDUCTOR, (1, 1, 1), roughness=0.05), 1.0
+    s = _furnace_scene(tt, ctx, mat)
+    cam = tt.camera((0, 0, 3.0), (0, 0, -1), fov=(40, 40), width=64, height=64)
+    img, st = s.render(cam, tt.render_params(spp=256, max_depth=64, rr_depth=8, seed=5, mode=tt.RENDER_PATH))
+    yy, xx = np.mgrid[0:64, 0:64]
+    r = np.hypot(xx - 31.5, yy - 31.5)
+    inside = r < 20          # the sphere covers radius ~27 px
+    outside = r > 31
+    assert abs(img[outside].mean() - 1.0) < 1e-5
+    m = img[inside].mean()
+    _say(f"furnace {kind}: sphere mean {m:.4f} (want {want}), stats {st}")
+    assert abs(m - want) < 0.01 * max(want, 0.5)
+    s.close()
+
+
+def test_path_mode_agrees_with_reference_mode(tt, ctx):
+    """Diffuse-only Cornell box: BSDF-only sampling (reference mode) and NEE + MIS +
+    Russian roulette (path mode) estimate the same radiance."""
+    from trenchanttracer_amd import workloads
+    _say("path vs reference mean")
+    mesh, mats, cam = workloads.cornell(subdiv=3, width=48, height=48)
+    b = tt.BVH.build(mesh)
+    s = tt.DeviceScene(ctx, mesh, b, mats)
+    ref, _ = s.render(cam, tt.render_params(spp=4096, max_depth=8, seed=1, mode=tt.RENDER_REFERENCE))
+    pt, st = s.render(cam, tt.render_params(spp=1024, max_depth=8, rr_depth=3, seed=2, mode=tt.RENDER_PATH,
+                                            pi=tt.REF_PI))
+    # 8x8 blocks: means agree within a few percent
+    rb = ref.reshape(6, 8, 6, 8, 3).mean(axis=(1, 3))
+    pb = pt.reshape(6, 8, 6, 8, 3).mean(axis=(1, 3))
+    lit = rb.sum(-1) > 0.05
+    rel = np.abs(rb - pb)[lit] / np.maximum(rb[lit], 1e-3)
+    _say(f"path vs reference: global {pt.mean():.4f} vs {ref.mean():.4f}, block rel err median {np.median(rel):.4f} "
+         f"max {rel.max():.4f}, stats {st}")
+    assert abs(pt.mean() - ref.mean()) < 0.02 * ref.mean()
+    assert np.median(rel) < 0.03
+    s.close()
+
+
+def test_env_cdf_matches_fp64_numpy(tt, ctx):
+    _say("env cdf")
+    rng = np.random.default_rng(4)
+    h, w = 64, 128
+    env = rng.gamma(0.5, 1.0, (h, w, 3)).astype(np.float32)
+    env[10:12, 50:52] = 1e4
+    sphere = tt.Mesh.icosphere(1, 0.1, (0, 0, 0))
+    s = tt.DeviceScene(ctx, sphere, tt.BVH.build(sphere), [tt.material(0)], env=env)
+    marg, cond, total = s.env_cdf(w, h)
+    lum = 0.2126 * env[..., 0].astype(np.float64) + 0.7152 * env[..., 1] + 0.0722 * env[..., 2]
+    wgt = lum * np.sin(np.pi * (np.arange(h) + 0.5) / h)[:, None]
+    rows = wgt.sum(1)
+    assert abs(total - rows.sum()) <= 1e-12 * rows.sum()
+    want_m = np.concatenate([[0], np.cumsum(rows) / rows.sum()]).astype(np.float32)
+    assert np.max(np.abs(marg - want_m)) <= 1.2e-7
+    want_c = np.concatenate([np.zeros((h, 1)), np.cumsum(wgt, 1) / rows[:, None]], 1).astype(np.float32)
+    assert np.max(np.abs(cond - want_c)) <= 1.2e-7
+    s.close()

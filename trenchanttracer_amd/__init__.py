@@ -194,6 +194,44 @@ def material(type, albedo=(0.8, 0.8, 0.8), emission=(0, 0, 0), roughness=0.0, io
     return m
 
 
+REF_PI = 3.14156265        # the reference's PI (src/core/Renderer.cpp:5, Camera.cpp:5)
+REF_TWO_PI = 6.2831853071795864769
+RENDER_REFERENCE, RENDER_PATH = 0, 1
+
+
+def camera(eye, view, up=(0, 1, 0), fov=(45.0, 45.0), width=256, height=256, aperture=0.0, focal=1.0):
+    c = N.tt_camera()
+    c.eye[:] = [float(x) for x in eye]
+    c.view[:] = [float(x) for x in view]
+    c.up[:] = [float(x) for x in up]
+    c.fov[:] = [float(x) for x in fov]
+    c.aperture = float(aperture)
+    c.focal = float(focal)
+    c.width, c.height = int(width), int(height)
+    return c
+
+
+def camera_from_orbit(center, yaw, pitch, radius=4.0, fovx=45.0, width=256, height=256, focal=1.0, aperture=0.0):
+    """Camera::Camera + setFovx + getCameraMeta (src/core/Camera.cpp:12-44, 115-140)."""
+    c = N.tt_camera()
+    check(lib.tt_camera_from_orbit(_f3(center), float(yaw), float(pitch), float(radius), float(fovx), int(width),
+                                   int(height), float(focal), float(aperture), C.byref(c)))
+    return c
+
+
+def render_params(spp=16, max_depth=8, rr_depth=4, seed=1, mode=RENDER_REFERENCE, guided=0, sample_offset=0,
+                  fp64=False, pi=None, two_pi=None, tmin=0.01):
+    p = N.tt_render_params()
+    p.spp, p.max_depth, p.rr_depth, p.seed = int(spp), int(max_depth), int(rr_depth), int(seed)
+    p.mode, p.guided, p.sample_offset, p.accumulate_fp64 = int(mode), int(guided), int(sample_offset), int(bool(fp64))
+    if pi is None:
+        pi = REF_PI if mode == RENDER_REFERENCE else float(np.pi)
+    p.pi = float(pi)
+    p.two_pi = float(REF_TWO_PI if two_pi is None else two_pi)
+    p.tmin = float(tmin)
+    return p
+
+
 class Context:
     """One GPU and its stream (tt_ctx)."""
 
@@ -283,6 +321,19 @@ def gather64_gbps(ctx, buf, nbytes, fetches=1 << 26, seed=1):
     return fetches * 64 / (ms.value * 1e-3) / 1e9
 
 
+def debug_math(ctx, op, x, y=None):
+    """Device sampler math (op 0 sin, 1 cos, 2 pow) on host arrays, through device buffers."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.ascontiguousarray(x if y is None else y, dtype=np.float32)
+    bx, by = DeviceBuffer.from_array(ctx, x), DeviceBuffer.from_array(ctx, y)
+    bo = DeviceBuffer(ctx, x.nbytes)
+    check(lib.tt_debug_math(ctx._h, int(op), C.c_void_p(bx.ptr), C.c_void_p(by.ptr), C.c_void_p(bo.ptr), len(x)))
+    out = bo.download(x.shape, np.float32)
+    for b in (bx, by, bo):
+        b.free()
+    return out
+
+
 class Timer:
     """HIP events recorded on the context stream (tt_timer)."""
 
@@ -368,6 +419,16 @@ class DeviceScene:
         out = np.empty((len(r), 4), np.float32)
         check(lib.tt_trace_closest_host(self._h, _vp(r), len(r), _vp(out), C.byref(cfg) if cfg is not None else None))
         return out[:, 0].view(np.int32).copy(), out[:, 1].copy(), out[:, 2].copy(), out[:, 3].copy()
+
+    def render(self, cam, params, want_fp64=False):
+        """Host image (height, width, 3) float32 (+ float64 image when FP64 accumulation is on) and stats."""
+        out = np.empty((cam.height, cam.width, 3), np.float32)
+        out64 = np.empty((cam.height, cam.width, 3), np.float64) if want_fp64 else None
+        st = N.tt_render_stats()
+        check(lib.tt_render(self._h, C.byref(cam), C.byref(params), _vp(out), _vp(out64), C.byref(st)))
+        stats = {"rays_extension": st.rays_extension, "rays_shadow": st.rays_shadow, "samples": st.samples,
+                 "seconds": st.seconds, "iterations": st.iterations}
+        return (out, out64, stats) if want_fp64 else (out, stats)
 
     def occluded(self, rays, cfg=None):
         r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)

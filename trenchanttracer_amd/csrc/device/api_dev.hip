@@ -1,4 +1,5 @@
 // C ABI, device half: context, scene upload, traversal entry points.
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -292,6 +293,51 @@ tt_status tt_scene_create(tt_ctx *ctx, const tt_mesh *mesh, const tt_bvh *bvh, c
         s->num_orig_tris = m.ntris();
         up((void **)&s->tri_normals, tn.data(), tn.size() * 4, "scene normals");
         up((void **)&s->tri_mat, m.tri_mat.data(), m.tri_mat.size() * 4, "scene material ids");
+        // geometric normals and areas; emissive triangles for light sampling
+        std::vector<float> geo((size_t)m.ntris() * 4);
+        std::vector<int32_t> etri;
+        std::vector<float> ecdf(1, 0.0f), everts;
+        double earea = 0.0;
+        for (int32_t i = 0; i < m.ntris(); ++i) {
+            V3 v0 = m.verts[m.tri_v[3 * (size_t)i]], v1 = m.verts[m.tri_v[3 * (size_t)i + 1]],
+               v2 = m.verts[m.tri_v[3 * (size_t)i + 2]];
+            double ax = (double)v1.x - v0.x, ay = (double)v1.y - v0.y, az = (double)v1.z - v0.z;
+            double bx = (double)v2.x - v0.x, by = (double)v2.y - v0.y, bz = (double)v2.z - v0.z;
+            double cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
+            double len = std::sqrt(cx * cx + cy * cy + cz * cz);
+            double area = 0.5 * len;
+            float *g = &geo[(size_t)i * 4];
+            g[0] = len > 0 ? (float)(cx / len) : 0.0f;
+            g[1] = len > 0 ? (float)(cy / len) : 0.0f;
+            g[2] = len > 0 ? (float)(cz / len) : 1.0f;
+            g[3] = (float)area;
+            int32_t mid = m.tri_mat[i];
+            if (nmats > 0 && mid >= 0 && mid < nmats && area > 0) {
+                const tt_material &mm = mats[mid];
+                if (mm.emission[0] > 0 || mm.emission[1] > 0 || mm.emission[2] > 0) {
+                    etri.push_back(i);
+                    earea += area;
+                    ecdf.push_back((float)earea);
+                    const V3 vs[3] = {v0, v1, v2};
+                    for (const V3 &v : vs) {
+                        everts.push_back(v.x);
+                        everts.push_back(v.y);
+                        everts.push_back(v.z);
+                        everts.push_back(0.0f);
+                    }
+                }
+            }
+        }
+        up((void **)&s->tri_geo, geo.data(), geo.size() * 4, "scene geometric normals");
+        if (!etri.empty()) {
+            for (float &c : ecdf) c = (float)(c / earea);
+            ecdf.back() = 1.0f;
+            s->num_emit = (int32_t)etri.size();
+            s->emit_area = (float)earea;
+            up((void **)&s->emit_tri, etri.data(), etri.size() * 4, "emitter ids");
+            up((void **)&s->emit_cdf, ecdf.data(), ecdf.size() * 4, "emitter cdf");
+            up((void **)&s->emit_verts, everts.data(), everts.size() * 4, "emitter vertices");
+        }
         for (int32_t x : m.tri_mat)
             if (x < 0 || (nmats > 0 && x >= nmats)) {
                 if (st == TT_OK) st = fail(TT_ERR_INVALID, "tt_scene_create: triangle material id out of range");
@@ -313,8 +359,9 @@ void tt_scene_destroy(tt_scene *s) {
     if (!s) return;
     hipSetDevice(s->ctx->device);
     hipStreamSynchronize(s->ctx->stream);
-    void *ptrs[] = {s->nodes, s->tris, s->tri_normals, s->tri_mat, s->mats,
-                    s->env_rgb, s->env_marg_cdf, s->env_cond_cdf, s->env_pdf_scale};
+    free_render_state(s);
+    void *ptrs[] = {s->nodes, s->tris, s->tri_normals, s->tri_mat, s->mats, s->tri_geo, s->emit_tri, s->emit_cdf,
+                    s->emit_verts, s->env_rgb, s->env_marg_cdf, s->env_cond_cdf, s->env_pdf_scale};
     for (void *p : ptrs)
         if (p) hipFree(p);
     delete s;

@@ -21,10 +21,17 @@ struct TraceArgs {
     int32_t num_nodes;
     int32_t *work_counter; // persistent variant: next ray batch
     int32_t batch;         // persistent variant: rays per wave fetch
+    const int32_t *n_dev;  // if set: ray count read on the device (capped at n)
+    int32_t *reset0;       // if set: zeroed by the first lane of block 0
+    int32_t *reset1;
 };
+
+// Triangle acceptance: the reference's test (front faces only, det > 1e-5) or
+// a two-sided test with no epsilon (accept any det != 0) for the path tracer.
+enum TriMode { kTriReference = 0, kTriTwoSided = 1 };
 
 // Launchers (trace.hip).
 hipError_t launch_trace(const TraceArgs &a, int variant, bool any, int lds_stack, int block, int persistent,
-                        int num_cus, hipStream_t s);
+                        int num_cus, hipStream_t s, int tri_mode = kTriReference);
 
 }  // namespace tt

@@ -1,0 +1,986 @@
+// Wavefront path tracing on gfx950: one pass per sample over all pixels, each
+// bounce = extend (closest-hit traversal of the ray queue) -> shade (materials,
+// emission, next-event rays, continuation rays appended to the next queue with
+// a wave64 ballot + prefix popcount and one atomic per wave) -> shadow (any-hit
+// traversal of the next-event queue) -> apply (unoccluded contributions).
+// Queue lengths stay on the device; kernels read them, so the host never waits
+// between stages.
+//
+// Reference mode restates the reference megakernel (src/core/RenderKernel.cu:27-218)
+// and its samplers (include/material/DiffImpl.cuh:12-36, MetalImpl.cuh:13-41,
+// RefrImpl.cuh:12-61); for the same Philox stream it computes the same bits as
+// the host restatement. Path mode is the physically based tracer (no reference
+// counterpart): Lambert, GGX conductor with visible-normal sampling (Heitz 2018),
+// smooth dielectric, next-event estimation of emitters and of the environment
+// map (marginal / conditional CDF, FP64 prefix sums), power-heuristic MIS,
+// Russian roulette.
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../common/philox.h"
+#include "../host/tt_host.h"
+#include "dev_common.h"
+#include "scene_dev.h"
+#include "ttmath.h"
+
+#pragma clang fp contract(off)
+
+struct RenderState {
+    int32_t cap = 0;
+    tt_ray *rays[2] = {nullptr, nullptr};
+    int32_t *path[2] = {nullptr, nullptr};
+    tt_hit *hits = nullptr;
+    float4 *thr = nullptr;       // reference: mask | path: throughput, w = pdf of the last BSDF sample
+    float4 *L = nullptr;         // reference: path radiance
+    int32_t *flags = nullptr;    // path: bit 0 = last bounce was specular
+    tt_ray *srays = nullptr;     // next-event rays
+    float4 *scontrib = nullptr;  // contribution (rgb) and pixel (w, as int bits)
+    int32_t *socc = nullptr;     // any-hit flags
+    int32_t *counts = nullptr;   // [0], [1] queues, [2] next-event queue
+    unsigned long long *stats = nullptr;  // [0] extension rays, [1] shadow rays
+    float *acc = nullptr;        // per pixel rgb
+    double *acc64 = nullptr;     // per pixel rgb (FP64 accumulation)
+    int32_t acc64_cap = 0;
+};
+
+namespace tt {
+
+tt_status dev_alloc(tt_ctx *ctx, void **p, size_t bytes, const char *what);
+
+namespace {
+
+struct f3 {
+    float x, y, z;
+};
+__device__ __forceinline__ f3 F3(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, f3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ f3 cross(f3 a, f3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+// Vec3f::normalize (LinearMath.h:101-107)
+__device__ __forceinline__ f3 vnorm(f3 a) {
+    float n = sqrtf(a.x * a.x + a.y * a.y + a.z * a.z);
+    return {a.x / n, a.y / n, a.z / n};
+}
+__device__ __forceinline__ f3 xyz(float4 v) { return {v.x, v.y, v.z}; }
+__device__ __forceinline__ float maxc(f3 a) { return fmaxf(a.x, fmaxf(a.y, a.z)); }
+
+__device__ __forceinline__ float u01c(uint32_t x) { return (float)((x >> 8) + 1u) * (1.0f / 16777216.0f); }
+
+// Four uniforms in (0, 1] for (pixel, sample, slot, lane); lane 0 is the stream of
+// the reference mode (the host restatement draws the same).
+__device__ __forceinline__ void draws(uint32_t pix, uint32_t sample, uint32_t slot, uint32_t lane, uint32_t seed,
+                                      float u[4]) {
+    u32x4 o = philox4x32_10({pix, sample, slot, lane}, seed, 0x5EEDu);
+    u[0] = u01c(o.x);
+    u[1] = u01c(o.y);
+    u[2] = u01c(o.z);
+    u[3] = u01c(o.w);
+}
+
+// Appends the lanes with `want` to a queue: one atomic per wave, slots in lane order.
+__device__ __forceinline__ int wave_append(bool want, int32_t *counter) {
+    const unsigned long long m = __ballot(want);
+    if (m == 0ull) return -1;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, __popcll(m));
+    base = __shfl(base, leader);
+    const int off = __popcll(m & ((1ull << lane) - 1ull));
+    return want ? base + off : -1;
+}
+
+struct CamArgs {
+    f3 pos, hax, vax, middle, horizontal, vertical;
+    float focal, aperture, two_pi;
+};
+
+struct GenArgs {
+    CamArgs cam;
+    int32_t w, h, mode;
+    uint32_t sample, seed;
+    float tmin;
+    tt_ray *rays;
+    int32_t *path;
+    float4 *thr, *L;
+    int32_t *flags;
+    int32_t *counts;
+};
+
+__global__ __launch_bounds__(256) void k_generate(GenArgs a) {
+    const int32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p == 0) {
+        a.counts[0] = a.w * a.h;
+        a.counts[1] = 0;
+        a.counts[2] = 0;
+    }
+    if (p >= a.w * a.h) return;
+    const int32_t X = p % a.w, Y = p / a.w;
+    float u[4];
+    draws((uint32_t)p, a.sample, 0u, 0u, a.seed, u);
+    // camera ray, RenderKernel.cu:152-197
+    const float jx = u[0] - 0.5f, jy = u[1] - 0.5f;
+    const float sx = (jx + (float)X) / (float)(a.w - 1);
+    const float sy = (jy + (float)Y) / (float)(a.h - 1);
+    const CamArgs &c = a.cam;
+    f3 on_plane = (c.middle + c.horizontal * ((2 * sx) - 1)) + c.vertical * ((2 * sy) - 1);
+    f3 on_image = c.pos + (on_plane - c.pos) * c.focal;
+    f3 ap = c.pos;
+    if (c.aperture > 1e-5f) {
+        float angle = c.two_pi * u[2];
+        float dist = c.aperture * sqrtf(u[3]);
+        float ca, sa = tm_sincos(angle, &ca);
+        ap = (c.pos + c.hax * (ca * dist)) + c.vax * (sa * dist);
+    }
+    f3 d = vnorm(vnorm(on_image - ap));
+    a.rays[p] = {ap.x, ap.y, ap.z, a.tmin, d.x, d.y, d.z, 1e20f};
+    a.path[p] = p;
+    a.thr[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    if (a.L) a.L[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (a.flags) a.flags[p] = 0;
+}
+
+struct SceneView {
+    const float4 *tri_normals;
+    const float4 *tri_geo;
+    const int32_t *tri_mat;
+    const tt_material *mats;
+    int32_t num_mats;
+    // environment
+    const float4 *env;
+    const float *env_marg, *env_cond;
+    int32_t env_w, env_h;
+    float env_scale, env_inv_total;
+    // emitters
+    const int32_t *emit_tri;
+    const float *emit_cdf;
+    const float4 *emit_verts;
+    int32_t num_emit;
+    float emit_inv_area;
+};
+
+struct ShadeArgs {
+    SceneView sv;
+    tt_ray *rays_in, *rays_out;
+    int32_t *path_in, *path_out;
+    const tt_hit *hits;
+    float4 *thr, *L;
+    int32_t *flags;
+    int32_t *counts;
+    int32_t cur, nxt;
+    tt_ray *srays;
+    float4 *scontrib;
+    float *acc;
+    double *acc64;
+    unsigned long long *stats;
+    int32_t bounce, max_depth, rr_depth;
+    uint32_t sample, seed;
+    float inv_spp, pi, two_pi;
+};
+
+// hdrSample (HDRKernel.cuh:6-42): nearest texel, x2; the row index is clamped
+// (the reference reads one row past the end for d.y near -1 with its PI).
+__device__ f3 env_reference(const SceneView &sv, f3 d, float pi, float two_pi) {
+    if (!sv.env) return F3(0, 0, 0);
+    float lx = atan2f(d.x, d.z);
+    lx = lx < 0.f ? lx + two_pi : lx;
+    float ly = acosf(d.y);
+    float u = lx / two_pi, v = ly / pi;
+    int u2 = min(max((int)(u * sv.env_w), 0), sv.env_w - 1);
+    int v2 = min(max((int)(v * sv.env_h), 0), sv.env_h - 1);
+    float4 t = sv.env[u2 + v2 * sv.env_w];
+    return F3(t.x, t.y, t.z) * (2.0f * sv.env_scale);
+}
+
+// ------------------------------------------------------------------ reference shade
+__global__ __launch_bounds__(256) void k_shade_reference(ShadeArgs a) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t n = a.counts[a.cur];
+    if (i == 0) atomicAdd(&a.stats[0], (unsigned long long)n);
+    bool cont = false;
+    int32_t p = 0;
+    tt_ray nr{};
+    if (i < n) {
+        p = a.path_in[i];
+        const tt_ray r = a.rays_in[i];
+        const tt_hit h = a.hits[i];
+        f3 mask = xyz(a.thr[p]);
+        f3 Lp = xyz(a.L[p]);
+        const f3 ro = F3(r.ox, r.oy, r.oz), rd = F3(r.dx, r.dy, r.dz);
+        bool finish = false;
+        if (h.tri_id < 0 || h.t > 1e19f) {
+            Lp = Lp + mask * env_reference(a.sv, rd, a.pi, a.two_pi);
+            finish = true;
+        } else {
+            const float4 *tn = a.sv.tri_normals + 3 * (int64_t)h.tri_id;
+            const float w = 1 - h.u - h.v;
+            const f3 n0 = xyz(tn[0]), n1 = xyz(tn[1]), n2 = xyz(tn[2]);
+            f3 nrm = (n0 * w + n1 * h.u) + n2 * h.v;
+            f3 nn = vnorm(nrm);
+            f3 nl = dot(nn, rd) < 0 ? nn : nn * -1.0f;
+            f3 point = ro + rd * h.t;
+            const tt_material m = a.sv.mats[a.sv.tri_mat[h.tri_id]];
+            Lp = Lp + mask * F3(m.emission[0], m.emission[1], m.emission[2]);
+            if (a.bounce == a.max_depth - 1 || m.type == TT_MAT_EMITTER) {
+                finish = true;
+            } else {
+                float u[4];
+                draws((uint32_t)p, a.sample, (uint32_t)(a.bounce + 1), 0u, a.seed, u);
+                f3 no = ro, nd = rd;
+                const f3 alb = F3(m.albedo[0], m.albedo[1], m.albedo[2]);
+                if (m.type == TT_MAT_DIFFUSE) {
+                    // diffSample
+                    float phi = (float)(2.0 * 3.14159265358979323846 * (double)u[0]);
+                    float r2 = u[1];
+                    float r2s = sqrtf(r2);
+                    f3 ww = vnorm(nl);
+                    f3 ax = fabsf(ww.x) >= 0.1f ? F3(0, 1, 0) : F3(1, 0, 0);
+                    f3 uu = vnorm(cross(ax, ww));
+                    f3 vv = cross(ww, uu);
+                    float c, s = tm_sincos(phi, &c);
+                    nd = vnorm(((uu * c) * r2s + (vv * s) * r2s) + ww * sqrtf(1 - r2));
+                    no = point + nl * 0.001f;
+                    mask = mask * alb;
+                } else if (m.type == TT_MAT_CONDUCTOR) {
+                    // metalSample (Phong lobe, exponent 30)
+                    float phi = (float)(2.0 * 3.14159265358979323846 * (double)u[0]);
+                    float r2 = u[1];
+                    float cos_t = tm_pow(1 - r2, 1.0f / (30.0f + 1));
+                    float sin_t = sqrtf(1 - cos_t * cos_t);
+                    f3 ww = vnorm(rd - (nn * 2.0f) * dot(nn, rd));
+                    f3 ax = fabsf(ww.x) >= 0.1f ? F3(0, 1, 0) : F3(1, 0, 0);
+                    f3 uu = vnorm(cross(ax, ww));
+                    f3 vv = cross(ww, uu);
+                    float c, s = tm_sincos(phi, &c);
+                    nd = vnorm(((uu * c) * sin_t + (vv * s) * sin_t) + ww * cos_t);
+                    no = point + nl * 0.0001f;
+                    mask = mask * alb;
+                } else {
+                    // refrSample
+                    bool into = dot(nn, nl) > 0;
+                    float nc = 1.0f, nt = m.ior;
+                    float nnt = into ? nc / nt : nt / nc;
+                    float ddn = dot(rd, nl);
+                    float cos2t = 1.0f - nnt * nnt * (1.f - ddn * ddn);
+                    if (cos2t < 0.0f) {
+                        nd = vnorm(rd - (nn * 2.0f) * dot(nn, rd));
+                        no = point + nl * 0.001f;
+                    } else {
+                        f3 tdir = rd * nnt;
+                        tdir = tdir - nn * ((float)(into ? 1 : -1) * (ddn * nnt + sqrtf(cos2t)));
+                        tdir = vnorm(tdir);
+                        float R0 = (nt - nc) * (nt - nc) / (nt + nc) * (nt + nc);
+                        float c = 1.f - (into ? -ddn : dot(tdir, nn));
+                        float Re = R0 + (1.f - R0) * c * c * c * c * c;
+                        float Tr = 1 - Re;
+                        float P = .25f + .5f * Re;
+                        float RP = Re / P;
+                        float TP = Tr / (1.f - P);
+                        if (u[0] < 0.2f) {
+                            mask = mask * RP;
+                            nd = vnorm(rd - (nn * 2.0f) * dot(nn, rd));
+                        } else {
+                            mask = mask * TP;
+                            nd = vnorm(tdir);
+                        }
+                        no = point + nl * 0.001f;
+                    }
+                }
+                nr = {no.x, no.y, no.z, r.tmin, nd.x, nd.y, nd.z, r.tmax};
+                cont = true;
+                a.thr[p] = make_float4(mask.x, mask.y, mask.z, 0.0f);
+                a.L[p] = make_float4(Lp.x, Lp.y, Lp.z, 0.0f);
+            }
+        }
+        if (finish) {
+            float *o = a.acc + 3 * (int64_t)p;
+            o[0] = o[0] + Lp.x * a.inv_spp;
+            o[1] = o[1] + Lp.y * a.inv_spp;
+            o[2] = o[2] + Lp.z * a.inv_spp;
+        }
+    }
+    const int slot = wave_append(cont, &a.counts[a.nxt]);
+    if (cont) {
+        a.rays_out[slot] = nr;
+        a.path_out[slot] = p;
+    }
+}
+
+// ------------------------------------------------------------------ path mode
+struct Frame {
+    f3 t, b, n;
+};
+// Duff et al. 2017, "Building an orthonormal basis, revisited"
+__device__ __forceinline__ Frame make_frame(f3 n) {
+    float s = n.z >= 0.0f ? 1.0f : -1.0f;
+    float a = -1.0f / (s + n.z);
+    float b = n.x * n.y * a;
+    return {F3(1.0f + s * n.x * n.x * a, s * b, -s * n.x), F3(b, s + n.y * n.y * a, -n.y), n};
+}
+__device__ __forceinline__ f3 to_local(const Frame &f, f3 v) { return F3(dot(v, f.t), dot(v, f.b), dot(v, f.n)); }
+__device__ __forceinline__ f3 to_world(const Frame &f, f3 v) { return (f.t * v.x + f.b * v.y) + f.n * v.z; }
+
+__device__ __forceinline__ float power_heuristic(float a, float b) {
+    float a2 = a * a, b2 = b * b;
+    return a2 + b2 > 0.0f ? a2 / (a2 + b2) : 0.0f;
+}
+
+// GGX with alpha = roughness^2 (clamped), height-correlated Smith masking.
+__device__ __forceinline__ float ggx_alpha(float rough) { return fmaxf(rough * rough, 1e-4f); }
+__device__ __forceinline__ float ggx_D(f3 h, float al) {
+    float x = h.x / al, y = h.y / al;
+    float q = x * x + y * y + h.z * h.z;
+    return 1.0f / (3.14159265f * al * al * q * q);
+}
+__device__ __forceinline__ float ggx_lambda(f3 w, float al) {
+    float z2 = w.z * w.z;
+    if (z2 <= 0.0f) return 0.0f;
+    float t2 = fmaxf(0.0f, 1.0f - z2) / z2;
+    return 0.5f * (-1.0f + sqrtf(1.0f + al * al * t2));
+}
+__device__ __forceinline__ f3 schlick(f3 f0, float c) {
+    float m = fminf(fmaxf(1.0f - c, 0.0f), 1.0f);
+    float m5 = m * m * m * m * m;
+    return f0 + (F3(1, 1, 1) - f0) * m5;
+}
+// Fresnel reflectance of a smooth dielectric, eta = eta_i / eta_t.
+__device__ __forceinline__ float fresnel_dielectric(float cosi, float eta, float *cost_out) {
+    float sin2t = eta * eta * fmaxf(0.0f, 1.0f - cosi * cosi);
+    if (sin2t >= 1.0f) {
+        *cost_out = 0.0f;
+        return 1.0f;
+    }
+    float cost = sqrtf(1.0f - sin2t);
+    float rs = (eta * cosi - cost) / (eta * cosi + cost);
+    float rp = (cosi - eta * cost) / (cosi + eta * cost);
+    *cost_out = cost;
+    return 0.5f * (rs * rs + rp * rp);
+}
+
+// BSDF value times |cos(wi)| and the sampling pdf for a non-delta material;
+// wo, wi in the local frame of the shading normal (z up, wo.z > 0).
+__device__ void bsdf_eval(const tt_material &m, f3 wo, f3 wi, f3 *f_cos, float *pdf) {
+    const f3 alb = F3(m.albedo[0], m.albedo[1], m.albedo[2]);
+    *f_cos = F3(0, 0, 0);
+    *pdf = 0.0f;
+    if (wi.z <= 0.0f || wo.z <= 0.0f) return;
+    if (m.type == TT_MAT_DIFFUSE) {
+        *f_cos = alb * (wi.z * 0.318309886f);
+        *pdf = wi.z * 0.318309886f;
+    } else if (m.type == TT_MAT_CONDUCTOR) {
+        float al = ggx_alpha(m.roughness);
+        f3 h = vnorm(wo + wi);
+        float D = ggx_D(h, al);
+        float G2 = 1.0f / (1.0f + ggx_lambda(wo, al) + ggx_lambda(wi, al));
+        float G1 = 1.0f / (1.0f + ggx_lambda(wo, al));
+        f3 F = schlick(alb, dot(wo, h));
+        *f_cos = F * (D * G2 / (4.0f * wo.z));
+        *pdf = G1 * D / (4.0f * wo.z);
+    }
+}
+
+// Samples wi; returns weight = f |cos| / pdf. *delta for specular events.
+__device__ bool bsdf_sample(const tt_material &m, f3 wo, bool front, const float u[4], f3 *wi, f3 *weight, float *pdf,
+                            bool *delta) {
+    const f3 alb = F3(m.albedo[0], m.albedo[1], m.albedo[2]);
+    *delta = false;
+    if (m.type == TT_MAT_DIFFUSE) {
+        float r = sqrtf(u[0]), phi = 6.28318531f * u[1];
+        float c, s = tm_sincos(phi, &c);
+        f3 d = F3(r * c, r * s, sqrtf(fmaxf(0.0f, 1.0f - u[0])));
+        *wi = d;
+        *weight = alb;
+        *pdf = d.z * 0.318309886f;
+        return d.z > 0.0f;
+    }
+    if (m.type == TT_MAT_CONDUCTOR) {
+        // visible-normal sampling (Heitz 2018, "Sampling the GGX Distribution of Visible Normals")
+        float al = ggx_alpha(m.roughness);
+        f3 vh = vnorm(F3(al * wo.x, al * wo.y, wo.z));
+        float lensq = vh.x * vh.x + vh.y * vh.y;
+        f3 t1 = lensq > 0.0f ? F3(-vh.y, vh.x, 0.0f) * (1.0f / sqrtf(lensq)) : F3(1, 0, 0);
+        f3 t2 = cross(vh, t1);
+        float r = sqrtf(u[0]), phi = 6.28318531f * u[1];
+        float c, s = tm_sincos(phi, &c);
+        float p1 = r * c, p2 = r * s;
+        float sv = 0.5f * (1.0f + vh.z);
+        p2 = (1.0f - sv) * sqrtf(fmaxf(0.0f, 1.0f - p1 * p1)) + sv * p2;
+        f3 nh = (t1 * p1 + t2 * p2) + vh * sqrtf(fmaxf(0.0f, 1.0f - p1 * p1 - p2 * p2));
+        f3 h = vnorm(F3(al * nh.x, al * nh.y, fmaxf(1e-6f, nh.z)));
+        float woh = dot(wo, h);
+        f3 d = h * (2.0f * woh) - wo;
+        if (d.z <= 0.0f || woh <= 0.0f) return false;
+        float D = ggx_D(h, al);
+        float l_o = ggx_lambda(wo, al), l_i = ggx_lambda(d, al);
+        float G1 = 1.0f / (1.0f + l_o);
+        float G2 = 1.0f / (1.0f + l_o + l_i);
+        *wi = d;
+        *weight = schlick(alb, woh) * (G2 / G1);
+        *pdf = G1 * D / (4.0f * wo.z);
+        return true;
+    }
+    if (m.type == TT_MAT_DIELECTRIC) {
+        float eta = front ? 1.0f / m.ior : m.ior;
+        float cost;
+        float F = fresnel_dielectric(wo.z, eta, &cost);
+        *delta = true;
+        *pdf = 0.0f;
+        if (u[2] <= F) {
+            *wi = F3(-wo.x, -wo.y, wo.z);
+        } else {
+            *wi = F3(-eta * wo.x, -eta * wo.y, -cost);
+        }
+        *weight = alb;
+        return true;
+    }
+    return false;
+}
+
+// Environment lookup (nearest texel) and the density of sampling direction d.
+__device__ f3 env_eval(const SceneView &sv, f3 d, float *pdf) {
+    *pdf = 0.0f;
+    if (!sv.env) return F3(0, 0, 0);
+    float phi = atan2f(d.x, d.z);
+    if (phi < 0.0f) phi += 6.28318531f;
+    float u = phi * 0.159154943f;
+    float v = acosf(fminf(1.0f, fmaxf(-1.0f, d.y))) * 0.318309886f;
+    int i = min(max((int)(u * sv.env_w), 0), sv.env_w - 1);
+    int j = min(max((int)(v * sv.env_h), 0), sv.env_h - 1);
+    float4 t = sv.env[(int64_t)j * sv.env_w + i];
+    float sin_t = sqrtf(fmaxf(0.0f, 1.0f - d.y * d.y));
+    if (sin_t > 0.0f && sv.env_inv_total > 0.0f)
+        *pdf = t.w * sv.env_inv_total * (float)sv.env_w * (float)sv.env_h / (19.7392088f * sin_t);
+    return F3(t.x, t.y, t.z) * sv.env_scale;
+}
+
+// Largest index k in [0, n) with cdf[k] < u (cdf[0] = 0 < u).
+__device__ __forceinline__ int cdf_search(const float *cdf, int n, float u) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (cdf[mid] < u) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ f3 env_sample(const SceneView &sv, const float u[4], f3 *dir, float *pdf) {
+    int j = cdf_search(sv.env_marg, sv.env_h, u[0]);
+    int i = cdf_search(sv.env_cond + (int64_t)j * (sv.env_w + 1), sv.env_w, u[1]);
+    float theta = 3.14159265f * ((float)j + (1.0f - u[2])) / (float)sv.env_h;
+    float phi = 6.28318531f * ((float)i + (1.0f - u[3])) / (float)sv.env_w;
+    float ct, st = tm_sincos(theta, &ct);
+    float cp, sp = tm_sincos(phi, &cp);
+    *dir = F3(st * sp, ct, st * cp);
+    float4 t = sv.env[(int64_t)j * sv.env_w + i];
+    *pdf = st > 0.0f ? t.w * sv.env_inv_total * (float)sv.env_w * (float)sv.env_h / (19.7392088f * st) : 0.0f;
+    return F3(t.x, t.y, t.z) * sv.env_scale;
+}
+
+__global__ __launch_bounds__(256) void k_shade_path(ShadeArgs a) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t n = a.counts[a.cur];
+    if (i == 0) atomicAdd(&a.stats[0], (unsigned long long)n);
+    bool cont = false, shadow = false;
+    int32_t p = 0;
+    tt_ray nr{}, sr{};
+    f3 scontrib = F3(0, 0, 0);
+    const SceneView &sv = a.sv;
+    const float p_env = sv.env ? (sv.num_emit > 0 ? 0.5f : 1.0f) : 0.0f;
+    const float p_emit = sv.num_emit > 0 ? 1.0f - p_env : 0.0f;
+    if (i < n) {
+        p = a.path_in[i];
+        const tt_ray r = a.rays_in[i];
+        const tt_hit h = a.hits[i];
+        const float4 tp = a.thr[p];
+        f3 thr = xyz(tp);
+        const float last_pdf = tp.w;
+        const bool last_delta = (a.flags[p] & 1) != 0;
+        const f3 ro = F3(r.ox, r.oy, r.oz), rd = F3(r.dx, r.dy, r.dz);
+        f3 add = F3(0, 0, 0);
+        if (h.tri_id < 0) {
+            float pdf_env;
+            f3 le = env_eval(sv, rd, &pdf_env);
+            float w = (a.bounce == 0 || last_delta) ? 1.0f : power_heuristic(last_pdf, pdf_env * p_env);
+            add = thr * le * w;
+        } else {
+            const float4 g = sv.tri_geo[h.tri_id];
+            const f3 ng = xyz(g);
+            const f3 wo = rd * -1.0f;
+            const bool front = dot(wo, ng) > 0.0f;
+            const f3 ngf = front ? ng : ng * -1.0f;
+            const float4 *tn = sv.tri_normals + 3 * (int64_t)h.tri_id;
+            const float bw = 1.0f - h.u - h.v;
+            f3 ns = vnorm((xyz(tn[0]) * bw + xyz(tn[1]) * h.u) + xyz(tn[2]) * h.v);
+            if (dot(ns, ngf) < 0.0f) ns = ns * -1.0f;
+            const f3 point = ro + rd * h.t;
+            const tt_material m = sv.mats[sv.tri_mat[h.tri_id]];
+            const f3 em = F3(m.emission[0], m.emission[1], m.emission[2]);
+            if (front && maxc(em) > 0.0f) {
+                float w = 1.0f;
+                if (a.bounce > 0 && !last_delta) {
+                    float cos_l = dot(wo, ng);
+                    float pdf_l = p_emit * sv.emit_inv_area * h.t * h.t / fmaxf(cos_l, 1e-12f);
+                    w = power_heuristic(last_pdf, pdf_l);
+                }
+                add = thr * em * w;
+            }
+            if (a.bounce < a.max_depth - 1 && m.type != TT_MAT_EMITTER) {
+                const float eps = 1e-4f * fmaxf(1.0f, fmaxf(fabsf(point.x), fmaxf(fabsf(point.y), fabsf(point.z))));
+                const Frame fr = make_frame(ns);
+                const f3 wol = to_local(fr, wo);
+                float ul[4], ub[4];
+                draws((uint32_t)p, a.sample, (uint32_t)(a.bounce + 1), 1u, a.seed, ul);
+                draws((uint32_t)p, a.sample, (uint32_t)(a.bounce + 1), 2u, a.seed, ub);
+                const bool smooth = m.type == TT_MAT_DIELECTRIC;
+                // next-event estimation
+                if (!smooth && (p_env > 0.0f || p_emit > 0.0f) && wol.z > 0.0f) {
+                    f3 wi, le;
+                    float pdf_l = 0.0f, dist = 1e20f;
+                    bool pick_env = ul[0] <= p_env;
+                    float uu[4] = {pick_env ? ul[0] / p_env : (ul[0] - p_env) / p_emit, ul[1], ul[2], ul[3]};
+                    uu[0] = fminf(fmaxf(uu[0], 1e-7f), 1.0f);
+                    if (pick_env) {
+                        le = env_sample(sv, uu, &wi, &pdf_l);
+                        pdf_l *= p_env;
+                    } else {
+                        int k = cdf_search(sv.emit_cdf, sv.num_emit, uu[0]);
+                        const float4 *v = sv.emit_verts + 3 * (int64_t)k;
+                        float su = sqrtf(uu[1]);
+                        f3 y = (xyz(v[0]) * (1.0f - su) + xyz(v[1]) * (su * (1.0f - uu[2]))) + xyz(v[2]) * (su * uu[2]);
+                        int t = sv.emit_tri[k];
+                        f3 nl = xyz(sv.tri_geo[t]);
+                        f3 dv = y - point;
+                        float d2 = dot(dv, dv);
+                        dist = sqrtf(d2);
+                        wi = dv * (1.0f / dist);
+                        float cos_l = -dot(wi, nl);
+                        const tt_material lm = sv.mats[sv.tri_mat[t]];
+                        le = F3(lm.emission[0], lm.emission[1], lm.emission[2]);
+                        pdf_l = cos_l > 0.0f ? p_emit * sv.emit_inv_area * d2 / cos_l : 0.0f;
+                    }
+                    if (pdf_l > 0.0f && dot(wi, ngf) > 0.0f) {
+                        f3 f_cos;
+                        float pdf_b;
+                        bsdf_eval(m, wol, to_local(fr, wi), &f_cos, &pdf_b);
+                        if (maxc(f_cos) > 0.0f) {
+                            float w = power_heuristic(pdf_l, pdf_b);
+                            scontrib = thr * f_cos * le * (w / pdf_l);
+                            f3 so = point + ngf * eps;
+                            sr = {so.x, so.y, so.z, 0.0f, wi.x, wi.y, wi.z, pick_env ? 1e20f : dist * (1.0f - 2e-4f)};
+                            shadow = maxc(scontrib) > 0.0f;
+                        }
+                    }
+                }
+                // BSDF sampling
+                f3 wil, weight;
+                float pdf_b;
+                bool delta;
+                if (wol.z > 0.0f && bsdf_sample(m, wol, front, ub, &wil, &weight, &pdf_b, &delta)) {
+                    f3 wi = to_world(fr, wil);
+                    float side = dot(wi, ngf);
+                    bool transmit = delta && wil.z < 0.0f;
+                    bool ok = transmit ? side < 0.0f : side > 0.0f;
+                    if (ok) {
+                        thr = thr * weight;
+                        bool alive = maxc(thr) > 0.0f;
+                        if (alive && a.bounce + 1 >= a.rr_depth) {
+                            float q = fminf(0.95f, maxc(thr));
+                            if (ub[3] > q) alive = false;
+                            else thr = thr * (1.0f / q);
+                        }
+                        if (alive) {
+                            f3 no = point + (side > 0.0f ? ngf : ngf * -1.0f) * eps;
+                            nr = {no.x, no.y, no.z, 0.0f, wi.x, wi.y, wi.z, 1e20f};
+                            cont = true;
+                            a.thr[p] = make_float4(thr.x, thr.y, thr.z, pdf_b);
+                            a.flags[p] = delta ? 1 : 0;
+                        }
+                    }
+                }
+            }
+        }
+        if (maxc(add) > 0.0f) {
+            if (a.acc64) {
+                double *o = a.acc64 + 3 * (int64_t)p;
+                o[0] += (double)add.x;
+                o[1] += (double)add.y;
+                o[2] += (double)add.z;
+            } else {
+                float *o = a.acc + 3 * (int64_t)p;
+                o[0] += add.x;
+                o[1] += add.y;
+                o[2] += add.z;
+            }
+        }
+    }
+    const int slot = wave_append(cont, &a.counts[a.nxt]);
+    if (cont) {
+        a.rays_out[slot] = nr;
+        a.path_out[slot] = p;
+    }
+    const int sslot = wave_append(shadow, &a.counts[2]);
+    if (shadow) {
+        a.srays[sslot] = sr;
+        a.scontrib[sslot] = make_float4(scontrib.x, scontrib.y, scontrib.z, __int_as_float(p));
+    }
+}
+
+struct ApplyArgs {
+    const int32_t *counts;
+    const int32_t *occ;
+    const float4 *contrib;
+    float *acc;
+    double *acc64;
+    unsigned long long *stats;
+};
+
+__global__ __launch_bounds__(256) void k_apply_shadow(ApplyArgs a) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t n = a.counts[2];
+    if (i == 0) atomicAdd(&a.stats[1], (unsigned long long)n);
+    if (i >= n || a.occ[i]) return;
+    const float4 c = a.contrib[i];
+    const int32_t p = __float_as_int(c.w);
+    if (a.acc64) {
+        double *o = a.acc64 + 3 * (int64_t)p;
+        o[0] += (double)c.x;
+        o[1] += (double)c.y;
+        o[2] += (double)c.z;
+    } else {
+        float *o = a.acc + 3 * (int64_t)p;
+        o[0] += c.x;
+        o[1] += c.y;
+        o[2] += c.z;
+    }
+}
+
+__global__ void k_zero(float *acc, double *acc64, unsigned long long *stats, int64_t n3) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n3) {
+        if (acc) acc[i] = 0.0f;
+        if (acc64) acc64[i] = 0.0;
+    }
+    if (i < 2 && stats) stats[i] = 0ull;
+}
+
+__global__ void k_debug_math(int op, const float *x, const float *y, float *out, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float c;
+    if (op == 0) out[i] = tm_sincos(x[i], &c);
+    else if (op == 1) {
+        tm_sincos(x[i], &c);
+        out[i] = c;
+    } else out[i] = tm_pow(x[i], y[i]);
+}
+
+// Host-side camera basis: the same float operations as the device code and the
+// host restatement (RenderKernel.cu:131-146, evaluated once per render).
+struct HF3 {
+    float x, y, z;
+};
+HF3 hnorm(HF3 a) {
+    float n = sqrtf(a.x * a.x + a.y * a.y + a.z * a.z);
+    return {a.x / n, a.y / n, a.z / n};
+}
+HF3 hcross(HF3 a, HF3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+HF3 hadd(HF3 a, HF3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+HF3 hscale(HF3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+f3 tof3(HF3 a) { return {a.x, a.y, a.z}; }
+
+CamArgs camera_args(const tt_camera &c, float pi, float two_pi) {
+    HF3 pos = {c.eye[0], c.eye[1], c.eye[2]};
+    HF3 view = hnorm({c.view[0], c.view[1], c.view[2]});
+    HF3 up = hnorm({c.up[0], c.up[1], c.up[2]});
+    HF3 hax = hnorm(hcross(view, up));
+    HF3 vax = hnorm(hcross(hax, view));
+    HF3 middle = hadd(pos, view);
+    HF3 horizontal = hscale(hax, tanf(c.fov[0] * 0.5f * (pi / 180)));
+    HF3 vertical = hscale(vax, tanf(-c.fov[1] * 0.5f * (pi / 180)));
+    CamArgs a;
+    a.pos = tof3(pos);
+    a.hax = tof3(hax);
+    a.vax = tof3(vax);
+    a.middle = tof3(middle);
+    a.horizontal = tof3(horizontal);
+    a.vertical = tof3(vertical);
+    a.focal = c.focal;
+    a.aperture = c.aperture;
+    a.two_pi = two_pi;
+    return a;
+}
+
+tt_status ensure_state(tt_scene *s, int32_t cap, bool fp64) {
+    RenderState *rs = s->rs;
+    if (rs && rs->cap >= cap && (!fp64 || rs->acc64_cap >= cap)) return TT_OK;
+    free_render_state(s);
+    rs = new (std::nothrow) RenderState();
+    if (!rs) return fail(TT_ERR_NOMEM, "render state: host allocation failed");
+    s->rs = rs;
+    rs->cap = cap;
+    tt_status st = TT_OK;
+    auto al = [&](void **p, size_t bytes, const char *what) {
+        if (st == TT_OK) st = dev_alloc(s->ctx, p, bytes, what);
+    };
+    for (int k = 0; k < 2; ++k) {
+        al((void **)&rs->rays[k], (size_t)cap * sizeof(tt_ray), "render rays");
+        al((void **)&rs->path[k], (size_t)cap * 4, "render path ids");
+    }
+    al((void **)&rs->hits, (size_t)cap * sizeof(tt_hit), "render hits");
+    al((void **)&rs->thr, (size_t)cap * 16, "render throughput");
+    al((void **)&rs->L, (size_t)cap * 16, "render radiance");
+    al((void **)&rs->flags, (size_t)cap * 4, "render flags");
+    al((void **)&rs->srays, (size_t)cap * sizeof(tt_ray), "render shadow rays");
+    al((void **)&rs->scontrib, (size_t)cap * 16, "render shadow contributions");
+    al((void **)&rs->socc, (size_t)cap * 4, "render shadow flags");
+    al((void **)&rs->counts, 16, "render queue counts");
+    al((void **)&rs->stats, 16, "render stats");
+    al((void **)&rs->acc, (size_t)cap * 12, "render accumulation");
+    if (fp64) {
+        al((void **)&rs->acc64, (size_t)cap * 24, "render FP64 accumulation");
+        rs->acc64_cap = cap;
+    }
+    if (st != TT_OK) free_render_state(s);
+    return st;
+}
+
+}  // namespace
+
+void free_render_state(tt_scene *s) {
+    RenderState *rs = s->rs;
+    if (!rs) return;
+    void *ptrs[] = {rs->rays[0], rs->rays[1], rs->path[0], rs->path[1], rs->hits, rs->thr, rs->L, rs->flags,
+                    rs->srays, rs->scontrib, rs->socc, rs->counts, rs->stats, rs->acc, rs->acc64};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    delete rs;
+    s->rs = nullptr;
+}
+
+}  // namespace tt
+
+using namespace tt;
+
+#define TT_HIPR(call, what)                                                                       \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess) return fail(TT_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+extern "C" {
+
+tt_status tt_camera_from_orbit(const float center[3], float yaw, float pitch, float radius, float fovx_deg,
+                               int32_t width, int32_t height, float focal, float aperture, tt_camera *out) {
+    if (!center || !out || width <= 0 || height <= 0) return fail(TT_ERR_INVALID, "tt_camera_from_orbit: bad arguments");
+    const float PI = 3.14156265f;  // Camera.cpp:5
+    // Camera::setFovx (Camera.cpp:115-119) with radiansToDegrees / degreesToRadians (:107-113)
+    float fovy_rad = std::atan(std::tan(fovx_deg / 180.0f * PI * 0.5f) * (float)height / (float)width) * 2.0f;
+    float fovy = fovy_rad * 180.0f / PI;
+    // Camera::getCameraMeta (Camera.cpp:121-140)
+    float xd = std::sin(yaw) * std::cos(pitch);
+    float yd = std::sin(pitch);
+    float zd = std::cos(yaw) * std::cos(pitch);
+    float m1 = (float)(-1.0);
+    tt_camera c{};
+    c.view[0] = xd * m1;
+    c.view[1] = yd * m1;
+    c.view[2] = zd * m1;
+    c.eye[0] = center[0] + xd * radius;
+    c.eye[1] = center[1] + yd * radius;
+    c.eye[2] = center[2] + zd * radius;
+    c.up[0] = 0.0f;
+    c.up[1] = 1.0f;
+    c.up[2] = 0.0f;
+    c.fov[0] = fovx_deg;
+    c.fov[1] = fovy;
+    c.aperture = aperture;
+    c.focal = focal;
+    c.width = width;
+    c.height = height;
+    *out = c;
+    return TT_OK;
+}
+
+tt_status tt_render(tt_scene *s, const tt_camera *cam, const tt_render_params *p, float *rgb_out, double *rgb64_out,
+                    tt_render_stats *stats) {
+    if (!s || !cam || !p || !rgb_out) return fail(TT_ERR_INVALID, "tt_render: null argument");
+    if (cam->width < 2 || cam->height < 2 || (int64_t)cam->width * cam->height > (1 << 26))
+        return fail(TT_ERR_INVALID, "tt_render: resolution out of range");
+    if (p->spp <= 0 || p->max_depth <= 0 || p->max_depth > 1024 || p->sample_offset < 0)
+        return fail(TT_ERR_INVALID, "tt_render: spp / max_depth out of range");
+    if (p->mode != TT_RENDER_REFERENCE && p->mode != TT_RENDER_PATH) return fail(TT_ERR_INVALID, "tt_render: bad mode");
+    if (p->guided) return fail(TT_ERR_UNSUPPORTED, "tt_render: guided rendering needs a guide (tt_guide_render)");
+    if (!s->tri_normals || !s->tri_mat || !s->mats) return fail(TT_ERR_INVALID, "tt_render: scene has no shading data");
+    const bool fp64 = p->accumulate_fp64 != 0;
+    if (p->mode == TT_RENDER_REFERENCE && fp64)
+        return fail(TT_ERR_INVALID, "tt_render: FP64 accumulation is a path-mode option");
+    tt_ctx *ctx = s->ctx;
+    TT_HIPR(hipSetDevice(ctx->device), "hipSetDevice");
+    const int32_t W = cam->width, H = cam->height, cap = W * H;
+    tt_status st = ensure_state(s, cap, fp64);
+    if (st != TT_OK) return st;
+    RenderState *rs = s->rs;
+    hipStream_t hs = ctx->stream;
+
+    const CamArgs cargs = camera_args(*cam, p->pi, p->two_pi);
+    SceneView sv{};
+    sv.tri_normals = (const float4 *)s->tri_normals;
+    sv.tri_geo = (const float4 *)s->tri_geo;
+    sv.tri_mat = s->tri_mat;
+    sv.mats = s->mats;
+    sv.num_mats = s->num_mats;
+    sv.env = (const float4 *)s->env_rgb;
+    sv.env_marg = s->env_marg_cdf;
+    sv.env_cond = s->env_cond_cdf;
+    sv.env_w = s->env_w;
+    sv.env_h = s->env_h;
+    sv.env_scale = s->env_scale;
+    sv.env_inv_total = s->env_total > 0.0 ? (float)(1.0 / s->env_total) : 0.0f;
+    if (p->mode == TT_RENDER_PATH && s->env_total <= 0.0) sv.env = nullptr;  // black map: nothing to sample
+    sv.emit_tri = s->emit_tri;
+    sv.emit_cdf = s->emit_cdf;
+    sv.emit_verts = (const float4 *)s->emit_verts;
+    sv.num_emit = s->num_emit;
+    sv.emit_inv_area = s->emit_area > 0.0f ? 1.0f / s->emit_area : 0.0f;
+
+    const int64_t n3 = (int64_t)cap * 3;
+    const unsigned gblocks = (unsigned)((cap + 255) / 256);
+    hipEvent_t e0, e1;
+    TT_HIPR(hipEventCreate(&e0), "hipEventCreate");
+    TT_HIPR(hipEventCreate(&e1), "hipEventCreate");
+    hipEventRecord(e0, hs);
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, hs, rs->acc, fp64 ? rs->acc64 : nullptr,
+                       rs->stats, n3);
+    const int tri_mode = p->mode == TT_RENDER_REFERENCE ? kTriReference : kTriTwoSided;
+    int32_t iters = 0;
+    for (int32_t smp = 0; smp < p->spp; ++smp) {
+        GenArgs g{};
+        g.cam = cargs;
+        g.w = W;
+        g.h = H;
+        g.mode = p->mode;
+        g.sample = (uint32_t)(p->sample_offset + smp);
+        g.seed = p->seed;
+        g.tmin = p->mode == TT_RENDER_REFERENCE ? p->tmin : 0.0f;
+        g.rays = rs->rays[0];
+        g.path = rs->path[0];
+        g.thr = rs->thr;
+        g.L = rs->L;
+        g.flags = rs->flags;
+        g.counts = rs->counts;
+        hipLaunchKernelGGL(k_generate, dim3(gblocks), dim3(256), 0, hs, g);
+        for (int32_t b = 0; b < p->max_depth; ++b) {
+            const int cur = b & 1, nxt = cur ^ 1;
+            TraceArgs ta{};
+            ta.nodes = (const float4 *)s->nodes;
+            ta.tris = (const float4 *)s->tris;
+            ta.rays = rs->rays[cur];
+            ta.hits = rs->hits;
+            ta.n = cap;
+            ta.num_nodes = (int32_t)s->num_nodes;
+            ta.n_dev = rs->counts + cur;
+            ta.reset0 = rs->counts + nxt;
+            ta.reset1 = rs->counts + 2;
+            TT_HIPR(launch_trace(ta, 0, false, 16, 64, 0, ctx->num_cus, hs, tri_mode), "render extend");
+            ShadeArgs sa{};
+            sa.sv = sv;
+            sa.rays_in = rs->rays[cur];
+            sa.rays_out = rs->rays[nxt];
+            sa.path_in = rs->path[cur];
+            sa.path_out = rs->path[nxt];
+            sa.hits = rs->hits;
+            sa.thr = rs->thr;
+            sa.L = rs->L;
+            sa.flags = rs->flags;
+            sa.counts = rs->counts;
+            sa.cur = cur;
+            sa.nxt = nxt;
+            sa.srays = rs->srays;
+            sa.scontrib = rs->scontrib;
+            sa.acc = rs->acc;
+            sa.acc64 = fp64 ? rs->acc64 : nullptr;
+            sa.stats = rs->stats;
+            sa.bounce = b;
+            sa.max_depth = p->max_depth;
+            sa.rr_depth = p->rr_depth;
+            sa.sample = (uint32_t)(p->sample_offset + smp);
+            sa.seed = p->seed;
+            sa.inv_spp = 1.0f / (float)p->spp;
+            sa.pi = p->pi;
+            sa.two_pi = p->two_pi;
+            if (p->mode == TT_RENDER_REFERENCE) {
+                hipLaunchKernelGGL(k_shade_reference, dim3(gblocks), dim3(256), 0, hs, sa);
+            } else {
+                hipLaunchKernelGGL(k_shade_path, dim3(gblocks), dim3(256), 0, hs, sa);
+                TraceArgs sh{};
+                sh.nodes = (const float4 *)s->nodes;
+                sh.tris = (const float4 *)s->tris;
+                sh.rays = rs->srays;
+                sh.flags = rs->socc;
+                sh.n = cap;
+                sh.num_nodes = (int32_t)s->num_nodes;
+                sh.n_dev = rs->counts + 2;
+                TT_HIPR(launch_trace(sh, 0, true, 16, 64, 0, ctx->num_cus, hs, kTriTwoSided), "render shadow");
+                ApplyArgs aa{rs->counts, rs->socc, rs->scontrib, rs->acc, fp64 ? rs->acc64 : nullptr, rs->stats};
+                hipLaunchKernelGGL(k_apply_shadow, dim3(gblocks), dim3(256), 0, hs, aa);
+            }
+            TT_HIPR(hipGetLastError(), "render launch");
+            ++iters;
+        }
+    }
+    hipEventRecord(e1, hs);
+    std::vector<float> acc((size_t)n3);
+    std::vector<double> acc64;
+    unsigned long long hstats[2] = {0, 0};
+    hipError_t e = hipMemcpyAsync(acc.data(), rs->acc, (size_t)n3 * 4, hipMemcpyDeviceToHost, hs);
+    if (e == hipSuccess && fp64) {
+        acc64.resize((size_t)n3);
+        e = hipMemcpyAsync(acc64.data(), rs->acc64, (size_t)n3 * 8, hipMemcpyDeviceToHost, hs);
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(hstats, rs->stats, 16, hipMemcpyDeviceToHost, hs);
+    if (e == hipSuccess) e = hipStreamSynchronize(hs);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (e != hipSuccess) return fail(TT_ERR_DEVICE, std::string("tt_render: ") + hipGetErrorString(e));
+    if (p->mode == TT_RENDER_REFERENCE) {
+        std::memcpy(rgb_out, acc.data(), (size_t)n3 * 4);
+    } else if (fp64) {
+        for (int64_t k = 0; k < n3; ++k) rgb_out[k] = (float)(acc64[k] / (double)p->spp);
+        if (rgb64_out)
+            for (int64_t k = 0; k < n3; ++k) rgb64_out[k] = acc64[k] / (double)p->spp;
+    } else {
+        const float inv = 1.0f / (float)p->spp;
+        for (int64_t k = 0; k < n3; ++k) rgb_out[k] = acc[k] * inv;
+    }
+    if (stats) {
+        stats->rays_extension = (int64_t)hstats[0];
+        stats->rays_shadow = (int64_t)hstats[1];
+        stats->samples = (int64_t)cap * p->spp;
+        stats->seconds = ms * 1e-3;
+        stats->iterations = iters;
+    }
+    return TT_OK;
+}
+
+tt_status tt_debug_math(tt_ctx *ctx, int32_t op, const float *x, const float *y, float *out, int64_t n) {
+    if (!ctx || op < 0 || op > 2 || n < 0 || (n > 0 && (!x || !out || (op == 2 && !y))))
+        return fail(TT_ERR_INVALID, "tt_debug_math: bad arguments");
+    if (n == 0) return TT_OK;
+    TT_HIPR(hipSetDevice(ctx->device), "hipSetDevice");
+    hipLaunchKernelGGL(k_debug_math, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, op, x, y, out, n);
+    TT_HIPR(hipGetLastError(), "debug math launch");
+    TT_HIPR(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return TT_OK;
+}
+
+}  // extern "C"

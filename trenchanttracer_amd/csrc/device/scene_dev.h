@@ -29,8 +29,15 @@ struct tt_scene {
     void *tri_normals = nullptr;  // float4 x 3 per triangle
     int32_t *tri_mat = nullptr;
     int32_t num_orig_tris = 0;
+    float *tri_geo = nullptr;     // float4 per triangle: unit geometric normal (e1 x e2), area
     tt_material *mats = nullptr;
     int32_t num_mats = 0;
+    // emissive triangles (area-proportional light sampling)
+    int32_t *emit_tri = nullptr;  // original triangle ids
+    float *emit_cdf = nullptr;    // num_emit + 1
+    float *emit_verts = nullptr;  // float4 x 3 per emitter
+    int32_t num_emit = 0;
+    float emit_area = 0.0f;
     // environment map (equirectangular) and its sampling distribution
     float *env_rgb = nullptr;        // float4 per texel (rgb, luminance weight)
     float *env_marg_cdf = nullptr;   // H + 1 floats
@@ -40,6 +47,7 @@ struct tt_scene {
     float env_scale = 1.0f;
     double env_total = 0.0;
     size_t bytes = 0;
+    struct RenderState *rs = nullptr;  // wavefront buffers, sized on first render
 };
 
 struct tt_timer {
@@ -50,4 +58,5 @@ struct tt_timer {
 namespace tt {
 tt_status dev_alloc(tt_ctx *ctx, void **p, size_t bytes, const char *what);
 tt_status build_env(tt_scene *s, const float *rgb, int32_t w, int32_t h, float scale);
+void free_render_state(tt_scene *s);
 }  // namespace tt

@@ -73,12 +73,18 @@ struct Stack {
     }
 };
 
-template <int VARIANT, bool ANY, int S>
+template <int VARIANT, bool ANY, int S, int TRI>
 __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
     extern __shared__ int lds_stack[];
     const int tid = threadIdx.x;
     const int64_t rid = (int64_t)blockIdx.x * blockDim.x + tid;
-    const bool valid = rid < a.n;
+    int64_t n = a.n;
+    if (a.n_dev) n = min((int64_t)*a.n_dev, n);
+    if (blockIdx.x == 0 && tid == 0) {
+        if (a.reset0) *a.reset0 = 0;
+        if (a.reset1) *a.reset1 = 0;
+    }
+    const bool valid = rid < n;
 
     float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = make_float4(0.f, 0.f, 1.f, 0.f);
     if (valid) {
@@ -178,14 +184,18 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
                 const float qvy = tvz * B.x - tvx * B.z;
                 const float qvz = tvx * B.y - tvy * B.x;
                 const float v = (dx * qvx + dy * qvy + dz * qvz) * inv;
-                const float EPS = 0.00001f;
                 bool ok;
-                if (det > EPS) {
-                    ok = !(u < 0.0f || u > 1.0f) && !(v < 0.0f || (u + v) > 1.0f);
-                } else if (det < -EPS) {
-                    ok = !(u > 0.0f || u < 1.0f) && !(v > 0.0f || (u + v) < 1.0f);
+                if (TRI == kTriReference) {
+                    const float EPS = 0.00001f;
+                    if (det > EPS) {
+                        ok = !(u < 0.0f || u > 1.0f) && !(v < 0.0f || (u + v) > 1.0f);
+                    } else if (det < -EPS) {
+                        ok = !(u > 0.0f || u < 1.0f) && !(v > 0.0f || (u + v) < 1.0f);
+                    } else {
+                        ok = false;
+                    }
                 } else {
-                    ok = false;
+                    ok = det != 0.0f && u >= 0.0f && u <= 1.0f && v >= 0.0f && (u + v) <= 1.0f;
                 }
                 if (ok) {
                     const float t = (C.x * qvx + C.y * qvy + C.z * qvz) * inv;
@@ -221,31 +231,38 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
     }
 }
 
-template <int V, bool ANY>
+template <int V, bool ANY, int TRI>
 hipError_t launch_s(const TraceArgs &a, int S, int block, hipStream_t s) {
     const int64_t grid = (a.n + block - 1) / block;
     const size_t lds = (size_t)S * block * sizeof(int);
     switch (S) {
-        case 8: hipLaunchKernelGGL((k_trace<V, ANY, 8>), dim3((unsigned)grid), dim3(block), lds, s, a); break;
-        case 16: hipLaunchKernelGGL((k_trace<V, ANY, 16>), dim3((unsigned)grid), dim3(block), lds, s, a); break;
-        case 24: hipLaunchKernelGGL((k_trace<V, ANY, 24>), dim3((unsigned)grid), dim3(block), lds, s, a); break;
-        case 32: hipLaunchKernelGGL((k_trace<V, ANY, 32>), dim3((unsigned)grid), dim3(block), lds, s, a); break;
+        case 8: hipLaunchKernelGGL((k_trace<V, ANY, 8, TRI>), dim3((unsigned)grid), dim3(block), lds, s, a); break;
+        case 16: hipLaunchKernelGGL((k_trace<V, ANY, 16, TRI>), dim3((unsigned)grid), dim3(block), lds, s, a); break;
+        case 24: hipLaunchKernelGGL((k_trace<V, ANY, 24, TRI>), dim3((unsigned)grid), dim3(block), lds, s, a); break;
+        case 32: hipLaunchKernelGGL((k_trace<V, ANY, 32, TRI>), dim3((unsigned)grid), dim3(block), lds, s, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
+template <int TRI>
+hipError_t launch_v(const TraceArgs &a, int variant, bool any, int S, int block, hipStream_t s) {
+    if (variant == 0) return any ? launch_s<0, true, TRI>(a, S, block, s) : launch_s<0, false, TRI>(a, S, block, s);
+    if (variant == 1) return any ? launch_s<1, true, TRI>(a, S, block, s) : launch_s<1, false, TRI>(a, S, block, s);
+    return hipErrorInvalidValue;
+}
+
 }  // namespace
 
 hipError_t launch_trace(const TraceArgs &a, int variant, bool any, int lds_stack, int block, int persistent,
-                        int num_cus, hipStream_t s) {
+                        int num_cus, hipStream_t s, int tri_mode) {
     (void)persistent;
     (void)num_cus;
     if (a.n <= 0) return hipSuccess;
     if (block < 64 || block > 256 || block % 64 != 0) return hipErrorInvalidValue;
     if ((a.n + block - 1) / block > 0x7FFFFFFF) return hipErrorInvalidValue;
-    if (variant == 0) return any ? launch_s<0, true>(a, lds_stack, block, s) : launch_s<0, false>(a, lds_stack, block, s);
-    if (variant == 1) return any ? launch_s<1, true>(a, lds_stack, block, s) : launch_s<1, false>(a, lds_stack, block, s);
+    if (tri_mode == kTriReference) return launch_v<kTriReference>(a, variant, any, lds_stack, block, s);
+    if (tri_mode == kTriTwoSided) return launch_v<kTriTwoSided>(a, variant, any, lds_stack, block, s);
     return hipErrorInvalidValue;
 }
 

@@ -39,3 +39,41 @@ def bytes_per_ray(inner, tris, nrays):
     """Algorithmic bytes of the device layout: ray (32) + hit (16) + 64 per node
     visit + 48 per triangle test, with visits counted by the single-lane traversal."""
     return 32.0 + 16.0 + (64.0 * inner + 48.0 * tris) / max(1, nrays)
+
+
+# Cornell box of the reference config (config/cornell_dragon.json:19-62): back,
+# left, right, ceiling and floor quads plus the ceiling light quad, each as two
+# triangles (p0, p1, p2), (p1, p1 + p2 - p0, p2); every wall faces the interior.
+CORNELL_QUADS = [
+    ("back", [-1.5, -1, -1], [1.5, -1, -1], [-1.5, 1, -1]),
+    ("left", [-1.5, -1, -1], [-1.5, 1, -1], [-1.5, -1, 4]),
+    ("right", [1.5, -1, -1], [1.5, -1, 4], [1.5, 1, -1]),
+    ("ceiling", [-1.5, 1, -1], [1.5, 1, -1], [-1.5, 1, 4]),
+    ("floor", [-30, -1, -30], [-30, -1, 30], [30, -1, -30]),
+    ("light", [-0.3, 0.9999, 1.2], [0.3, 0.9999, 1.2], [-0.3, 0.9999, 1.8]),
+]
+
+
+def cornell(subdiv=4, sphere_radius=0.3, sphere_center=(0.0, -0.7, 1.5), width=256, height=256):
+    """Cornell box + icosphere (the CPU-reference render case).
+
+    Materials: 0 grey diffuse 0.75, 1 red (0.75, 0.25, 0.25), 2 green (0.25, 0.75, 0.25),
+    3 light (diffuse albedo 0, emission 15), 4 sphere diffuse 0.8. Camera: the orbit
+    camera of the config (centre (0, 0.2, 3), yaw 0, pitch 0.1, radius 4, focal 1) with
+    fovx 45 and no aperture. Returns (mesh, materials, camera).
+    """
+    from . import MAT_DIFFUSE, camera_from_orbit, material
+    mat_of = {"back": 0, "left": 1, "right": 2, "ceiling": 0, "floor": 0, "light": 3}
+    mesh = None
+    for name, p0, p1, p2 in CORNELL_QUADS:
+        q = Mesh.quad(p0, p1, p2, mat=mat_of[name])
+        mesh = q if mesh is None else mesh.append(q)
+    mesh.append(Mesh.icosphere(subdiv, sphere_radius, sphere_center, mat=4))
+    mats = [material(MAT_DIFFUSE, (0.75, 0.75, 0.75)),
+            material(MAT_DIFFUSE, (0.75, 0.25, 0.25)),
+            material(MAT_DIFFUSE, (0.25, 0.75, 0.25)),
+            material(MAT_DIFFUSE, (0.0, 0.0, 0.0), emission=(15.0, 15.0, 15.0)),
+            material(MAT_DIFFUSE, (0.8, 0.8, 0.8))]
+    cam = camera_from_orbit((0.0, 0.2, 3.0), 0.0, 0.1, radius=4.0, fovx=45.0, width=width, height=height,
+                            focal=1.0, aperture=0.0)
+    return mesh, mats, cam
