@@ -32,6 +32,7 @@ def main():
     ap.add_argument("--ntris", type=int, default=1_000_000)
     ap.add_argument("--cpu-sample-stride", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-render", action="store_true", help="skip the render workloads (secondary metrics)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -168,9 +169,60 @@ def main():
         log(f"cpu baseline {cpu_mrays:.2f} Mrays/s on {cores} threads; sample mismatches {mism}; "
             f"bytes/ray {bpr:.1f}; achieved {achieved:.0f} GB/s")
 
-    print(json.dumps(result), flush=True)
     scene.close()
+    d_rays.free()
+    d_hits.free()
+    if not args.no_render:
+        result["render"] = render_bench(tt, ctx, np, args.no_cpu_baseline)
+
+    print(json.dumps(result), flush=True)
     ctx.close()
+
+
+def render_bench(tt, ctx, np, no_cpu):
+    """Secondary metrics: samples/s of the path tracer on the full path-trace case
+    (1,310,720-tri displaced icosphere, 40/40/20 diffuse/GGX/dielectric, 2048x1024
+    sky + sun, 1920x1080, 64 spp, max depth 16, Russian roulette from depth 4), and
+    of the reference-semantics render of the Cornell case (256x256, 16 spp, depth 8)
+    beside the oracle's CPU restatement of it."""
+    from trenchanttracer_amd import workloads
+    out = {}
+    mesh, mats, env, cam = workloads.pathtrace()
+    bvh = tt.BVH.build(mesh)
+    s = tt.DeviceScene(ctx, mesh, bvh, mats, env=env)
+    prm = tt.render_params(spp=64, max_depth=16, rr_depth=4, seed=1, mode=tt.RENDER_PATH)
+    s.render(cam, tt.render_params(spp=4, max_depth=16, rr_depth=4, seed=7, mode=tt.RENDER_PATH))
+    img, st = s.render(cam, prm)
+    rays = st["rays_extension"] + st["rays_shadow"]
+    out["pathtrace_1080p_64spp"] = {
+        "samples_per_s": round(st["samples"] / st["seconds"], 1), "seconds": round(st["seconds"], 4),
+        "mrays_per_s": round(rays / st["seconds"] / 1e6, 1), "rays_extension": st["rays_extension"],
+        "rays_shadow": st["rays_shadow"], "mean_radiance": round(float(img.mean()), 5)}
+    log(f"path trace 1080p 64spp: {out['pathtrace_1080p_64spp']}")
+    s.close()
+
+    mesh, mats, cam = workloads.cornell()
+    bvh = tt.BVH.build(mesh)
+    s = tt.DeviceScene(ctx, mesh, bvh, mats)
+    prm = tt.render_params(spp=16, max_depth=8, seed=1, mode=tt.RENDER_REFERENCE)
+    s.render(cam, prm)
+    img, st = s.render(cam, prm)
+    c1 = {"samples_per_s": round(st["samples"] / st["seconds"], 1), "seconds": round(st["seconds"], 5),
+          "rays_extension": st["rays_extension"]}
+    if not no_cpu:
+        from oracle import oracle as O
+        cores = min(16, os.cpu_count() or 1)
+        t0 = time.perf_counter()
+        ref = O.render_ref(O.CompactBVH(bvh.export_compact()), mesh.arrays(), mats, cam, spp=16, max_depth=8,
+                           seed=1, nthreads=cores)
+        cpu_s = time.perf_counter() - t0
+        c1["cpu_baseline"] = {"samples_per_s": round(st["samples"] / cpu_s, 1), "cores": cores, "kind": "port",
+                              "seconds": round(cpu_s, 3)}
+        c1["bit_identical_to_cpu"] = bool(np.array_equal(img.view(np.int32), ref.view(np.int32)))
+    out["reference_cornell_256_16spp"] = c1
+    log(f"reference cornell: {c1}")
+    s.close()
+    return out
 
 
 if __name__ == "__main__":

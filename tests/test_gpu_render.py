@@ -84,12 +84,13 @@ def test_path_mode_furnace(tt, ctx, kind):
     else:
         mat, want = tt.material(tt.MAT_CONDUCTOR, (1, 1, 1), roughness=0.05), 1.0
     s = _furnace_scene(tt, ctx, mat)
-    cam = tt.camera((0, 0, 3.0), (0, 0, -1), fov=(40, 40), width=64, height=64)
+    cam = tt.camera((0, 0, 3.0), (0, 0, -1), fov=(60, 60), width=64, height=64)
     img, st = s.render(cam, tt.render_params(spp=256, max_depth=64, rr_depth=8, seed=5, mode=tt.RENDER_PATH))
     yy, xx = np.mgrid[0:64, 0:64]
     r = np.hypot(xx - 31.5, yy - 31.5)
-    inside = r < 20          # the sphere covers radius ~27 px
-    outside = r > 31
+    # the sphere subtends asin(1/3) = 19.47 deg: radius tan(19.47)/tan(30) * 32 = 19.6 px
+    inside = r < 15
+    outside = r > 24
     assert abs(img[outside].mean() - 1.0) < 1e-5
     m = img[inside].mean()
     _say(f"furnace {kind}: sphere mean {m:.4f} (want {want}), stats {st}")
@@ -129,7 +130,8 @@ def test_env_cdf_matches_fp64_numpy(tt, ctx):
     sphere = tt.Mesh.icosphere(1, 0.1, (0, 0, 0))
     s = tt.DeviceScene(ctx, sphere, tt.BVH.build(sphere), [tt.material(0)], env=env)
     marg, cond, total = s.env_cdf(w, h)
-    lum = 0.2126 * env[..., 0].astype(np.float64) + 0.7152 * env[..., 1] + 0.0722 * env[..., 2]
+    e64 = env.astype(np.float64)
+    lum = 0.2126 * e64[..., 0] + 0.7152 * e64[..., 1] + 0.0722 * e64[..., 2]
     wgt = lum * np.sin(np.pi * (np.arange(h) + 0.5) / h)[:, None]
     rows = wgt.sum(1)
     assert abs(total - rows.sum()) <= 1e-12 * rows.sum()

@@ -14,6 +14,7 @@
 // smooth dielectric, next-event estimation of emitters and of the environment
 // map (marginal / conditional CDF, FP64 prefix sums), power-heuristic MIS,
 // Russian roulette.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -43,6 +44,7 @@ struct RenderState {
     float *acc = nullptr;        // per pixel rgb
     double *acc64 = nullptr;     // per pixel rgb (FP64 accumulation)
     int32_t acc64_cap = 0;
+    int32_t pix_cap = 0;
 };
 
 namespace tt {
@@ -104,7 +106,7 @@ struct CamArgs {
 
 struct GenArgs {
     CamArgs cam;
-    int32_t w, h, mode;
+    int32_t w, h, mode, k;   // k samples per pixel in flight: path id = j * w * h + pixel
     uint32_t sample, seed;
     float tmin;
     tt_ray *rays;
@@ -116,15 +118,17 @@ struct GenArgs {
 
 __global__ __launch_bounds__(256) void k_generate(GenArgs a) {
     const int32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t npix = a.w * a.h;
     if (p == 0) {
-        a.counts[0] = a.w * a.h;
+        a.counts[0] = npix * a.k;
         a.counts[1] = 0;
         a.counts[2] = 0;
     }
-    if (p >= a.w * a.h) return;
-    const int32_t X = p % a.w, Y = p / a.w;
+    if (p >= npix * a.k) return;
+    const int32_t pix = p % npix;
+    const int32_t X = pix % a.w, Y = pix / a.w;
     float u[4];
-    draws((uint32_t)p, a.sample, 0u, 0u, a.seed, u);
+    draws((uint32_t)pix, a.sample + (uint32_t)(p / npix), 0u, 0u, a.seed, u);
     // camera ray, RenderKernel.cu:152-197
     const float jx = u[0] - 0.5f, jy = u[1] - 0.5f;
     const float sx = (jx + (float)X) / (float)(a.w - 1);
@@ -181,6 +185,7 @@ struct ShadeArgs {
     double *acc64;
     unsigned long long *stats;
     int32_t bounce, max_depth, rr_depth;
+    int32_t npix;     // path id = j * npix + pixel, sample = sample + j
     uint32_t sample, seed;
     float inv_spp, pi, two_pi;
 };
@@ -497,6 +502,8 @@ __global__ __launch_bounds__(256) void k_shade_path(ShadeArgs a) {
     const float p_emit = sv.num_emit > 0 ? 1.0f - p_env : 0.0f;
     if (i < n) {
         p = a.path_in[i];
+        const int32_t pix = p % a.npix;
+        const uint32_t smp = a.sample + (uint32_t)(p / a.npix);
         const tt_ray r = a.rays_in[i];
         const tt_hit h = a.hits[i];
         const float4 tp = a.thr[p];
@@ -537,8 +544,8 @@ __global__ __launch_bounds__(256) void k_shade_path(ShadeArgs a) {
                 const Frame fr = make_frame(ns);
                 const f3 wol = to_local(fr, wo);
                 float ul[4], ub[4];
-                draws((uint32_t)p, a.sample, (uint32_t)(a.bounce + 1), 1u, a.seed, ul);
-                draws((uint32_t)p, a.sample, (uint32_t)(a.bounce + 1), 2u, a.seed, ub);
+                draws((uint32_t)pix, smp, (uint32_t)(a.bounce + 1), 1u, a.seed, ul);
+                draws((uint32_t)pix, smp, (uint32_t)(a.bounce + 1), 2u, a.seed, ub);
                 const bool smooth = m.type == TT_MAT_DIELECTRIC;
                 // next-event estimation
                 if (!smooth && (p_env > 0.0f || p_emit > 0.0f) && wol.z > 0.0f) {
@@ -609,15 +616,15 @@ __global__ __launch_bounds__(256) void k_shade_path(ShadeArgs a) {
         }
         if (maxc(add) > 0.0f) {
             if (a.acc64) {
-                double *o = a.acc64 + 3 * (int64_t)p;
-                o[0] += (double)add.x;
-                o[1] += (double)add.y;
-                o[2] += (double)add.z;
+                double *o = a.acc64 + 3 * (int64_t)pix;
+                atomicAdd(o, (double)add.x);
+                atomicAdd(o + 1, (double)add.y);
+                atomicAdd(o + 2, (double)add.z);
             } else {
-                float *o = a.acc + 3 * (int64_t)p;
-                o[0] += add.x;
-                o[1] += add.y;
-                o[2] += add.z;
+                float *o = a.acc + 3 * (int64_t)pix;
+                atomicAdd(o, add.x);
+                atomicAdd(o + 1, add.y);
+                atomicAdd(o + 2, add.z);
             }
         }
     }
@@ -629,7 +636,7 @@ __global__ __launch_bounds__(256) void k_shade_path(ShadeArgs a) {
     const int sslot = wave_append(shadow, &a.counts[2]);
     if (shadow) {
         a.srays[sslot] = sr;
-        a.scontrib[sslot] = make_float4(scontrib.x, scontrib.y, scontrib.z, __int_as_float(p));
+        a.scontrib[sslot] = make_float4(scontrib.x, scontrib.y, scontrib.z, __int_as_float(p % a.npix));
     }
 }
 
@@ -651,14 +658,14 @@ __global__ __launch_bounds__(256) void k_apply_shadow(ApplyArgs a) {
     const int32_t p = __float_as_int(c.w);
     if (a.acc64) {
         double *o = a.acc64 + 3 * (int64_t)p;
-        o[0] += (double)c.x;
-        o[1] += (double)c.y;
-        o[2] += (double)c.z;
+        atomicAdd(o, (double)c.x);
+        atomicAdd(o + 1, (double)c.y);
+        atomicAdd(o + 2, (double)c.z);
     } else {
         float *o = a.acc + 3 * (int64_t)p;
-        o[0] += c.x;
-        o[1] += c.y;
-        o[2] += c.z;
+        atomicAdd(o, c.x);
+        atomicAdd(o + 1, c.y);
+        atomicAdd(o + 2, c.z);
     }
 }
 
@@ -718,14 +725,15 @@ CamArgs camera_args(const tt_camera &c, float pi, float two_pi) {
     return a;
 }
 
-tt_status ensure_state(tt_scene *s, int32_t cap, bool fp64) {
+tt_status ensure_state(tt_scene *s, int32_t cap, int32_t npix, bool fp64) {
     RenderState *rs = s->rs;
-    if (rs && rs->cap >= cap && (!fp64 || rs->acc64_cap >= cap)) return TT_OK;
+    if (rs && rs->cap >= cap && rs->pix_cap >= npix && (!fp64 || rs->acc64_cap >= npix)) return TT_OK;
     free_render_state(s);
     rs = new (std::nothrow) RenderState();
     if (!rs) return fail(TT_ERR_NOMEM, "render state: host allocation failed");
     s->rs = rs;
     rs->cap = cap;
+    rs->pix_cap = npix;
     tt_status st = TT_OK;
     auto al = [&](void **p, size_t bytes, const char *what) {
         if (st == TT_OK) st = dev_alloc(s->ctx, p, bytes, what);
@@ -743,10 +751,10 @@ tt_status ensure_state(tt_scene *s, int32_t cap, bool fp64) {
     al((void **)&rs->socc, (size_t)cap * 4, "render shadow flags");
     al((void **)&rs->counts, 16, "render queue counts");
     al((void **)&rs->stats, 16, "render stats");
-    al((void **)&rs->acc, (size_t)cap * 12, "render accumulation");
+    al((void **)&rs->acc, (size_t)npix * 12, "render accumulation");
     if (fp64) {
-        al((void **)&rs->acc64, (size_t)cap * 24, "render FP64 accumulation");
-        rs->acc64_cap = cap;
+        al((void **)&rs->acc64, (size_t)npix * 24, "render FP64 accumulation");
+        rs->acc64_cap = npix;
     }
     if (st != TT_OK) free_render_state(s);
     return st;
@@ -824,8 +832,16 @@ tt_status tt_render(tt_scene *s, const tt_camera *cam, const tt_render_params *p
         return fail(TT_ERR_INVALID, "tt_render: FP64 accumulation is a path-mode option");
     tt_ctx *ctx = s->ctx;
     TT_HIPR(hipSetDevice(ctx->device), "hipSetDevice");
-    const int32_t W = cam->width, H = cam->height, cap = W * H;
-    tt_status st = ensure_state(s, cap, fp64);
+    const int32_t W = cam->width, H = cam->height, npix = W * H;
+    // path mode keeps k samples per pixel in flight (about 8M paths) so that
+    // deep bounces still fill the GPU; reference mode runs one sample per pass,
+    // which keeps the per-pixel float sums in sample order
+    int32_t k = 1;
+    if (p->mode == TT_RENDER_PATH) {
+        k = std::max(1, std::min(p->spp, (8 << 20) / npix));
+    }
+    const int32_t cap = npix * k;
+    tt_status st = ensure_state(s, cap, npix, fp64);
     if (st != TT_OK) return st;
     RenderState *rs = s->rs;
     hipStream_t hs = ctx->stream;
@@ -851,7 +867,7 @@ tt_status tt_render(tt_scene *s, const tt_camera *cam, const tt_render_params *p
     sv.num_emit = s->num_emit;
     sv.emit_inv_area = s->emit_area > 0.0f ? 1.0f / s->emit_area : 0.0f;
 
-    const int64_t n3 = (int64_t)cap * 3;
+    const int64_t n3 = (int64_t)npix * 3;
     const unsigned gblocks = (unsigned)((cap + 255) / 256);
     hipEvent_t e0, e1;
     TT_HIPR(hipEventCreate(&e0), "hipEventCreate");
@@ -861,12 +877,14 @@ tt_status tt_render(tt_scene *s, const tt_camera *cam, const tt_render_params *p
                        rs->stats, n3);
     const int tri_mode = p->mode == TT_RENDER_REFERENCE ? kTriReference : kTriTwoSided;
     int32_t iters = 0;
-    for (int32_t smp = 0; smp < p->spp; ++smp) {
+    for (int32_t smp = 0; smp < p->spp; smp += k) {
+        const int32_t kk = std::min(k, p->spp - smp);
         GenArgs g{};
         g.cam = cargs;
         g.w = W;
         g.h = H;
         g.mode = p->mode;
+        g.k = kk;
         g.sample = (uint32_t)(p->sample_offset + smp);
         g.seed = p->seed;
         g.tmin = p->mode == TT_RENDER_REFERENCE ? p->tmin : 0.0f;
@@ -889,7 +907,10 @@ tt_status tt_render(tt_scene *s, const tt_camera *cam, const tt_render_params *p
             ta.n_dev = rs->counts + cur;
             ta.reset0 = rs->counts + nxt;
             ta.reset1 = rs->counts + 2;
-            TT_HIPR(launch_trace(ta, 0, false, 16, 64, 0, ctx->num_cus, hs, tri_mode), "render extend");
+            // reference mode: the exact loop (hits identical to the host restatement);
+            // path mode: the speculative while-while loop
+            const int variant = p->mode == TT_RENDER_REFERENCE ? 0 : 1;
+            TT_HIPR(launch_trace(ta, variant, false, 16, 64, 0, ctx->num_cus, hs, tri_mode), "render extend");
             ShadeArgs sa{};
             sa.sv = sv;
             sa.rays_in = rs->rays[cur];
@@ -911,6 +932,7 @@ tt_status tt_render(tt_scene *s, const tt_camera *cam, const tt_render_params *p
             sa.bounce = b;
             sa.max_depth = p->max_depth;
             sa.rr_depth = p->rr_depth;
+            sa.npix = npix;
             sa.sample = (uint32_t)(p->sample_offset + smp);
             sa.seed = p->seed;
             sa.inv_spp = 1.0f / (float)p->spp;
@@ -928,7 +950,7 @@ tt_status tt_render(tt_scene *s, const tt_camera *cam, const tt_render_params *p
                 sh.n = cap;
                 sh.num_nodes = (int32_t)s->num_nodes;
                 sh.n_dev = rs->counts + 2;
-                TT_HIPR(launch_trace(sh, 0, true, 16, 64, 0, ctx->num_cus, hs, kTriTwoSided), "render shadow");
+                TT_HIPR(launch_trace(sh, 1, true, 16, 64, 0, ctx->num_cus, hs, kTriTwoSided), "render shadow");
                 ApplyArgs aa{rs->counts, rs->socc, rs->scontrib, rs->acc, fp64 ? rs->acc64 : nullptr, rs->stats};
                 hipLaunchKernelGGL(k_apply_shadow, dim3(gblocks), dim3(256), 0, hs, aa);
             }
@@ -965,7 +987,7 @@ tt_status tt_render(tt_scene *s, const tt_camera *cam, const tt_render_params *p
     if (stats) {
         stats->rays_extension = (int64_t)hstats[0];
         stats->rays_shadow = (int64_t)hstats[1];
-        stats->samples = (int64_t)cap * p->spp;
+        stats->samples = (int64_t)npix * p->spp;
         stats->seconds = ms * 1e-3;
         stats->iterations = iters;
     }
