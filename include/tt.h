@@ -315,6 +315,19 @@ tt_status tt_render(tt_scene *s, const tt_camera *cam, const tt_render_params *p
  * the device sampler math to n device values; y is read for op 2. Synchronous. */
 tt_status tt_debug_math(tt_ctx *ctx, int32_t op, const float *x, const float *y, float *out, int64_t n);
 
+/* Sampling hooks for the path-mode BSDFs and the env-map distribution (the same
+ * device functions the shade stage calls), for distribution tests.
+ * tt_debug_bsdf_sample: n samples for a fixed local wo (z = shading normal), front =
+ *   ray on the side the geometric normal points to; out: n x 8 floats
+ *   {wi.xyz, weight.xyz (f cos / pdf), pdf, delta}. out_eval (may be NULL): n x 4
+ *   {f cos .xyz, pdf} of bsdf_eval(wo, wi) for the sampled wi.
+ * tt_debug_env_sample: n directions from the scene's env distribution; out: n x 8
+ *   {dir.xyz, pdf, Le.xyz, pdf of env_eval(dir)}.
+ * Host buffers; synchronous. */
+tt_status tt_debug_bsdf_sample(tt_ctx *ctx, const tt_material *m, const float wo[3], int32_t front, int64_t n,
+                               uint32_t seed, float *out, float *out_eval);
+tt_status tt_debug_env_sample(tt_scene *s, int64_t n, uint32_t seed, float *out);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

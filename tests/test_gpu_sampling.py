@@ -1,0 +1,135 @@
+"""Path-mode sampling on the GPU vs the FP64 densities: chi-square goodness of fit,
+pdf / weight consistency, Fresnel split, Snell's law, env texel distribution."""
+import sys
+
+import numpy as np
+import pytest
+from scipy import stats
+
+pytestmark = pytest.mark.gpu
+
+NTH, NPH = 10, 16
+
+
+def _say(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def _hist(wi):
+    ok = wi[:, 2] > 0
+    c = wi[ok, 2]
+    ph = np.arctan2(wi[ok, 1], wi[ok, 0]) % (2 * np.pi)
+    a = np.minimum(((1 - c) * NTH).astype(int), NTH - 1)
+    b = np.minimum((ph / (2 * np.pi) * NPH).astype(int), NPH - 1)
+    h = np.zeros((NTH, NPH))
+    np.add.at(h, (a, b), 1)
+    return h
+
+
+def _chi2(obs, expected_p, n):
+    exp = expected_p * n
+    m = exp > 20
+    chi = ((obs[m] - exp[m]) ** 2 / exp[m]).sum()
+    dof = m.sum() - 1
+    return stats.chi2.sf(chi, dof), chi, dof
+
+
+def test_lambert_sampling(tt, ctx):
+    from oracle import bsdf_ref as B  # noqa: F401
+    n = 400000
+    m = tt.material(tt.MAT_DIFFUSE, (0.3, 0.5, 0.7))
+    out, ev = tt.debug_bsdf_sample(ctx, m, (0.3, 0.1, np.sqrt(1 - 0.1)), n, seed=3)
+    wi = out[:, :3].astype(np.float64)
+    assert np.allclose(out[:, 3:6], [0.3, 0.5, 0.7])
+    assert np.allclose(out[:, 6], wi[:, 2] / np.pi, rtol=1e-5)
+    assert np.allclose(ev[:, 3], out[:, 6], rtol=1e-5)
+    expected = np.array([[(((1 - a / NTH) ** 2 - (1 - (a + 1) / NTH) ** 2) / 2) / NPH * 2
+                          for _ in range(NPH)] for a in range(NTH)]) / 2 * 2
+    expected = expected / expected.sum()
+    p, chi, dof = _chi2(_hist(wi), expected, n)
+    _say(f"lambert chi2 {chi:.1f} dof {dof} p {p:.3g}")
+    assert p > 1e-4
+
+
+@pytest.mark.parametrize("rough,theta_o", [(0.1, 20), (0.3, 45), (0.6, 70), (0.9, 10)])
+def test_ggx_vndf_sampling(tt, ctx, rough, theta_o):
+    from oracle import bsdf_ref as B
+    n = 400000
+    f0 = (0.9, 0.7, 0.4)
+    m = tt.material(tt.MAT_CONDUCTOR, f0, roughness=rough)
+    to = np.radians(theta_o)
+    wo = np.array([np.sin(to), 0.0, np.cos(to)])
+    out, ev = tt.debug_bsdf_sample(ctx, m, wo, n, seed=5)
+    valid = out[:, 6] > 0
+    wi = out[valid, :3].astype(np.float64)
+    fcos, pdf = B.conductor_eval(wo, wi, rough, f0)
+    # density and weight of every sample agree with the FP64 formulas
+    assert np.allclose(out[valid, 6], pdf, rtol=2e-3, atol=1e-6)
+    assert np.allclose(out[valid, 3:6], fcos / pdf[:, None], rtol=2e-3, atol=1e-6)
+    # bsdf_eval on the sampled direction returns the same density
+    assert np.allclose(ev[valid, 3], out[valid, 6], rtol=1e-3, atol=1e-6)
+    # distribution: chi-square against the binned VNDF-induced density
+    expected = B.sphere_bin_probs(lambda d: B.conductor_eval(wo, d, rough, f0)[1], NTH, NPH)
+    miss = max(0.0, 1 - expected.sum())
+    assert abs((~valid).mean() - miss) < 0.01
+    p, chi, dof = _chi2(_hist(out[:, :3].astype(np.float64)), expected, n)
+    _say(f"ggx rough {rough} theta {theta_o}: chi2 {chi:.1f} dof {dof} p {p:.3g}, below-horizon {(~valid).mean():.4f}")
+    assert p > 1e-4
+
+
+@pytest.mark.parametrize("front,theta_o", [(True, 30), (True, 75), (False, 20), (False, 50)])
+def test_dielectric_fresnel_split_and_snell(tt, ctx, front, theta_o):
+    from oracle import bsdf_ref as B
+    n = 200000
+    ior = 1.5
+    m = tt.material(tt.MAT_DIELECTRIC, (1, 1, 1), ior=ior)
+    to = np.radians(theta_o)
+    wo = np.array([np.sin(to), 0.0, np.cos(to)])
+    out, _ = tt.debug_bsdf_sample(ctx, m, wo, n, front=front, seed=7)
+    eta = 1 / ior if front else ior
+    F = B.fresnel_dielectric(np.cos(to), eta)
+    refl = out[:, 2] > 0
+    sd = np.sqrt(F * (1 - F) / n)
+    _say(f"dielectric front={front} theta={theta_o}: reflected {refl.mean():.4f} Fresnel {F:.4f}")
+    assert abs(refl.mean() - F) < 5 * sd + 1e-6
+    assert np.all(out[:, 7] == 1)
+    assert np.allclose(out[refl, :3], [-wo[0], -wo[1], wo[2]], atol=1e-6)
+    if F < 1:
+        t = out[~refl, :3]
+        sin_t = np.hypot(t[:, 0], t[:, 1])
+        assert np.allclose(sin_t, eta * np.sin(to), atol=1e-5)
+        assert np.allclose(np.linalg.norm(t, axis=1), 1, atol=1e-5)
+
+
+def test_env_importance_sampling(tt, ctx):
+    from oracle import bsdf_ref as B
+    rng = np.random.default_rng(2)
+    h, w = 32, 64
+    env = rng.gamma(1.0, 1.0, (h, w, 3)).astype(np.float32)
+    env[5, 10] = 400.0
+    sphere = tt.Mesh.icosphere(1, 0.1, (0, 0, 0))
+    s = tt.DeviceScene(ctx, sphere, tt.BVH.build(sphere), [tt.material(0)], env=env)
+    n = 1 << 20
+    out = s.env_samples(n, seed=9)
+    d = out[:, :3].astype(np.float64)
+    assert np.allclose(np.linalg.norm(d, axis=1), 1, atol=1e-5)
+    # pdf returned by the sampler == pdf of looking the direction up
+    assert np.allclose(out[:, 3], out[:, 7], rtol=1e-4)
+    # texel of each direction, as the lookup maps it
+    u = (np.arctan2(d[:, 0], d[:, 2]) % (2 * np.pi)) / (2 * np.pi)
+    v = np.arccos(np.clip(d[:, 1], -1, 1)) / np.pi
+    ti = np.minimum((u * w).astype(int), w - 1)
+    tj = np.minimum((v * h).astype(int), h - 1)
+    cnt = np.zeros((h, w))
+    np.add.at(cnt, (tj, ti), 1)
+    probs = B.env_texel_probs(env)
+    p, chi, dof = _chi2(cnt, probs, n)
+    _say(f"env chi2 {chi:.1f} dof {dof} p {p:.3g}")
+    assert p > 1e-4
+    # Le equals the texel radiance; pdf equals the solid-angle density of the texel
+    # (directions within rounding of a texel edge may resolve to the neighbour in FP64)
+    same = np.all(np.isclose(out[:, 4:7], env[tj, ti], rtol=1e-6), axis=1)
+    assert same.mean() > 0.9999
+    dens = probs[tj, ti] * w * h / (2 * np.pi ** 2 * np.hypot(d[:, 0], d[:, 2]))
+    assert np.isclose(out[same, 3], dens[same], rtol=2e-3).all()
+    s.close()

@@ -334,6 +334,15 @@ def debug_math(ctx, op, x, y=None):
     return out
 
 
+def debug_bsdf_sample(ctx, mat, wo, n, front=True, seed=1):
+    """n samples of the path-mode BSDF for local wo: (samples n x 8, eval n x 4)."""
+    out = np.empty((int(n), 8), np.float32)
+    ev = np.empty((int(n), 4), np.float32)
+    check(lib.tt_debug_bsdf_sample(ctx._h, C.byref(mat), _f3(wo), int(bool(front)), int(n), int(seed), _vp(out),
+                                   _vp(ev)))
+    return out, ev
+
+
 class Timer:
     """HIP events recorded on the context stream (tt_timer)."""
 
@@ -429,6 +438,11 @@ class DeviceScene:
         stats = {"rays_extension": st.rays_extension, "rays_shadow": st.rays_shadow, "samples": st.samples,
                  "seconds": st.seconds, "iterations": st.iterations}
         return (out, out64, stats) if want_fp64 else (out, stats)
+
+    def env_samples(self, n, seed=1):
+        out = np.empty((int(n), 8), np.float32)
+        check(lib.tt_debug_env_sample(self._h, int(n), int(seed), _vp(out)))
+        return out
 
     def occluded(self, rays, cfg=None):
         r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)

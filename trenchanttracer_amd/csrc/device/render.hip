@@ -459,7 +459,7 @@ __device__ f3 env_eval(const SceneView &sv, f3 d, float *pdf) {
     int i = min(max((int)(u * sv.env_w), 0), sv.env_w - 1);
     int j = min(max((int)(v * sv.env_h), 0), sv.env_h - 1);
     float4 t = sv.env[(int64_t)j * sv.env_w + i];
-    float sin_t = sqrtf(fmaxf(0.0f, 1.0f - d.y * d.y));
+    float sin_t = sqrtf(d.x * d.x + d.z * d.z);  // no cancellation near the poles
     if (sin_t > 0.0f && sv.env_inv_total > 0.0f)
         *pdf = t.w * sv.env_inv_total * (float)sv.env_w * (float)sv.env_h / (19.7392088f * sin_t);
     return F3(t.x, t.y, t.z) * sv.env_scale;
@@ -477,6 +477,10 @@ __device__ __forceinline__ int cdf_search(const float *cdf, int n, float u) {
 }
 
 __device__ f3 env_sample(const SceneView &sv, const float u[4], f3 *dir, float *pdf) {
+    // texel from the marginal / conditional CDFs, uniform (u, v) inside it; the
+    // density and radiance are those of the texel the direction looks up, so
+    // sampling and env_eval agree even where rounding moves a direction across a
+    // texel edge
     int j = cdf_search(sv.env_marg, sv.env_h, u[0]);
     int i = cdf_search(sv.env_cond + (int64_t)j * (sv.env_w + 1), sv.env_w, u[1]);
     float theta = 3.14159265f * ((float)j + (1.0f - u[2])) / (float)sv.env_h;
@@ -484,9 +488,7 @@ __device__ f3 env_sample(const SceneView &sv, const float u[4], f3 *dir, float *
     float ct, st = tm_sincos(theta, &ct);
     float cp, sp = tm_sincos(phi, &cp);
     *dir = F3(st * sp, ct, st * cp);
-    float4 t = sv.env[(int64_t)j * sv.env_w + i];
-    *pdf = st > 0.0f ? t.w * sv.env_inv_total * (float)sv.env_w * (float)sv.env_h / (19.7392088f * st) : 0.0f;
-    return F3(t.x, t.y, t.z) * sv.env_scale;
+    return env_eval(sv, *dir, pdf);
 }
 
 __global__ __launch_bounds__(256) void k_shade_path(ShadeArgs a) {
@@ -676,6 +678,62 @@ __global__ void k_zero(float *acc, double *acc64, unsigned long long *stats, int
         if (acc64) acc64[i] = 0.0;
     }
     if (i < 2 && stats) stats[i] = 0ull;
+}
+
+__global__ void k_debug_bsdf(tt_material m, f3 wo, bool front, int64_t n, uint32_t seed, float *out, float *out_eval) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float u[4];
+    draws((uint32_t)i, (uint32_t)(i >> 32), 0x0D0Bu, 2u, seed, u);
+    f3 wi = F3(0, 0, 0), w = F3(0, 0, 0);
+    float pdf = 0.0f;
+    bool delta = false;
+    bool ok = bsdf_sample(m, wo, front, u, &wi, &w, &pdf, &delta);
+    if (!ok) {
+        wi = F3(0, 0, 0);
+        w = F3(0, 0, 0);
+        pdf = 0.0f;
+    }
+    float *o = out + 8 * i;
+    o[0] = wi.x;
+    o[1] = wi.y;
+    o[2] = wi.z;
+    o[3] = w.x;
+    o[4] = w.y;
+    o[5] = w.z;
+    o[6] = pdf;
+    o[7] = delta ? 1.0f : 0.0f;
+    if (out_eval) {
+        f3 fc;
+        float pe;
+        bsdf_eval(m, wo, wi, &fc, &pe);
+        float *e = out_eval + 4 * i;
+        e[0] = fc.x;
+        e[1] = fc.y;
+        e[2] = fc.z;
+        e[3] = pe;
+    }
+}
+
+__global__ void k_debug_env(SceneView sv, int64_t n, uint32_t seed, float *out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float u[4];
+    draws((uint32_t)i, (uint32_t)(i >> 32), 0x0E0Bu, 1u, seed, u);
+    f3 d;
+    float pdf;
+    f3 le = env_sample(sv, u, &d, &pdf);
+    float pe;
+    env_eval(sv, d, &pe);
+    float *o = out + 8 * i;
+    o[0] = d.x;
+    o[1] = d.y;
+    o[2] = d.z;
+    o[3] = pdf;
+    o[4] = le.x;
+    o[5] = le.y;
+    o[6] = le.z;
+    o[7] = pe;
 }
 
 __global__ void k_debug_math(int op, const float *x, const float *y, float *out, int64_t n) {
@@ -991,6 +1049,54 @@ tt_status tt_render(tt_scene *s, const tt_camera *cam, const tt_render_params *p
         stats->seconds = ms * 1e-3;
         stats->iterations = iters;
     }
+    return TT_OK;
+}
+
+tt_status tt_debug_bsdf_sample(tt_ctx *ctx, const tt_material *m, const float wo[3], int32_t front, int64_t n,
+                               uint32_t seed, float *out, float *out_eval) {
+    if (!ctx || !m || !wo || !out || n <= 0 || n > (1LL << 26)) return fail(TT_ERR_INVALID, "tt_debug_bsdf_sample: bad arguments");
+    TT_HIPR(hipSetDevice(ctx->device), "hipSetDevice");
+    float *d_out = nullptr, *d_eval = nullptr;
+    tt_status st = dev_alloc(ctx, (void **)&d_out, (size_t)n * 32, "debug bsdf");
+    if (st == TT_OK && out_eval) st = dev_alloc(ctx, (void **)&d_eval, (size_t)n * 16, "debug bsdf eval");
+    if (st != TT_OK) {
+        if (d_out) hipFree(d_out);
+        return st;
+    }
+    hipLaunchKernelGGL(k_debug_bsdf, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, *m,
+                       f3{wo[0], wo[1], wo[2]}, front != 0, n, seed, d_out, d_eval);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, (size_t)n * 32, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess && out_eval) e = hipMemcpyAsync(out_eval, d_eval, (size_t)n * 16, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    hipFree(d_out);
+    if (d_eval) hipFree(d_eval);
+    if (e != hipSuccess) return fail(TT_ERR_DEVICE, std::string("tt_debug_bsdf_sample: ") + hipGetErrorString(e));
+    return TT_OK;
+}
+
+tt_status tt_debug_env_sample(tt_scene *s, int64_t n, uint32_t seed, float *out) {
+    if (!s || !out || n <= 0 || n > (1LL << 26)) return fail(TT_ERR_INVALID, "tt_debug_env_sample: bad arguments");
+    if (!s->env_rgb || s->env_total <= 0.0) return fail(TT_ERR_INVALID, "tt_debug_env_sample: scene has no env map");
+    tt_ctx *ctx = s->ctx;
+    TT_HIPR(hipSetDevice(ctx->device), "hipSetDevice");
+    SceneView sv{};
+    sv.env = (const float4 *)s->env_rgb;
+    sv.env_marg = s->env_marg_cdf;
+    sv.env_cond = s->env_cond_cdf;
+    sv.env_w = s->env_w;
+    sv.env_h = s->env_h;
+    sv.env_scale = s->env_scale;
+    sv.env_inv_total = (float)(1.0 / s->env_total);
+    float *d_out = nullptr;
+    tt_status st = dev_alloc(ctx, (void **)&d_out, (size_t)n * 32, "debug env");
+    if (st != TT_OK) return st;
+    hipLaunchKernelGGL(k_debug_env, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, sv, n, seed, d_out);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, (size_t)n * 32, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    hipFree(d_out);
+    if (e != hipSuccess) return fail(TT_ERR_DEVICE, std::string("tt_debug_env_sample: ") + hipGetErrorString(e));
     return TT_OK;
 }
 
