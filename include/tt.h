@@ -293,7 +293,11 @@ typedef struct tt_render_params {
     int32_t accumulate_fp64;  /* accumulate pixel sums in FP64 (oracle renders) */
     float pi, two_pi;         /* camera / env constants (reference: 3.14156265f and 6.2831853f) */
     float tmin;               /* reference mode ray tmin (0.01, RenderKernel.cu:110) */
-    float reserved;
+    int32_t use_graph;        /* capture one sample pass in a hipGraph and replay it */
+    int32_t paths_in_flight;  /* path mode: target paths per pass (0 = about 8M); rounded to whole samples */
+    int32_t lds_stack;        /* traversal LDS stack entries per lane (0 = 16) */
+    int32_t block;            /* traversal block size (0 = 64) */
+    int32_t variant;          /* path-mode traversal loop: -1 speculative (default), 0 exact, 1 speculative */
 } tt_render_params;
 
 typedef struct tt_render_stats {
@@ -302,7 +306,7 @@ typedef struct tt_render_stats {
     int64_t samples;          /* width * height * spp */
     double seconds;           /* GPU time of the render (HIP events) */
     int32_t iterations;       /* wavefront iterations launched */
-    int32_t pad;
+    int32_t paths_in_flight;  /* paths per pass */
 } tt_render_stats;
 
 /* Renders one image. rgb_out (host, width*height*3 floats, row 0 = top) receives the
@@ -310,6 +314,18 @@ typedef struct tt_render_stats {
  * when accumulate_fp64 is set. Synchronous. */
 tt_status tt_render(tt_scene *s, const tt_camera *cam, const tt_render_params *p, float *rgb_out,
                     double *rgb64_out, tt_render_stats *stats);
+
+/* Batched rendering: up to 64 scenes in one wavefront (one node / triangle array,
+ * per-ray root nodes, per-scene camera and resolution), sharing one env map.
+ * tt_batch_render fills rgb_out[k] (width_k * height_k * 3 floats) for camera k;
+ * the RNG stream of each scene equals that of rendering it alone. */
+typedef struct tt_batch tt_batch;
+tt_status tt_batch_create(tt_ctx *ctx, int32_t n, const tt_mesh *const *meshes, const tt_bvh *const *bvhs,
+                          const tt_material *const *mats, const int32_t *nmats, const float *env_rgb,
+                          int32_t env_w, int32_t env_h, float env_scale, tt_batch **out);
+void tt_batch_destroy(tt_batch *b);
+tt_status tt_batch_render(tt_batch *b, const tt_camera *cams, const tt_render_params *p, float *const *rgb_out,
+                          tt_render_stats *stats);
 
 /* Reproducibility hook for the render math: applies op (0 sin, 1 cos, 2 pow(x, y)) of
  * the device sampler math to n device values; y is read for op 2. Synchronous. */

@@ -19,7 +19,7 @@ import numpy as np
 from . import _native as N
 from ._native import TTError, check, lib, tt_trace_cfg
 
-__all__ = ["Mesh", "BVH", "Context", "DeviceScene", "DeviceBuffer", "Timer", "TTError", "rays_primary", "rays_incoherent",
+__all__ = ["Mesh", "BVH", "Context", "DeviceScene", "DeviceBuffer", "Timer", "Batch", "TTError", "rays_primary", "rays_incoherent",
            "trace_cfg", "MAT_DIFFUSE", "MAT_CONDUCTOR", "MAT_DIELECTRIC", "MAT_EMITTER", "material"]
 
 MAT_DIFFUSE, MAT_CONDUCTOR, MAT_DIELECTRIC, MAT_EMITTER = 0, 1, 2, 3
@@ -220,8 +220,11 @@ def camera_from_orbit(center, yaw, pitch, radius=4.0, fovx=45.0, width=256, heig
 
 
 def render_params(spp=16, max_depth=8, rr_depth=4, seed=1, mode=RENDER_REFERENCE, guided=0, sample_offset=0,
-                  fp64=False, pi=None, two_pi=None, tmin=0.01):
+                  fp64=False, pi=None, two_pi=None, tmin=0.01, use_graph=False, paths_in_flight=0, lds_stack=0,
+                  block=0, variant=-1):
     p = N.tt_render_params()
+    p.use_graph, p.paths_in_flight, p.lds_stack, p.block, p.variant = (int(bool(use_graph)), int(paths_in_flight),
+                                                                       int(lds_stack), int(block), int(variant))
     p.spp, p.max_depth, p.rr_depth, p.seed = int(spp), int(max_depth), int(rr_depth), int(seed)
     p.mode, p.guided, p.sample_offset, p.accumulate_fp64 = int(mode), int(guided), int(sample_offset), int(bool(fp64))
     if pi is None:
@@ -230,6 +233,52 @@ def render_params(spp=16, max_depth=8, rr_depth=4, seed=1, mode=RENDER_REFERENCE
     p.two_pi = float(REF_TWO_PI if two_pi is None else two_pi)
     p.tmin = float(tmin)
     return p
+
+
+def _stats(st):
+    return {"rays_extension": st.rays_extension, "rays_shadow": st.rays_shadow, "samples": st.samples,
+            "seconds": st.seconds, "iterations": st.iterations, "paths_in_flight": st.paths_in_flight}
+
+
+class Batch:
+    """Up to 64 scenes rendered in one wavefront (tt_batch): one node array with
+    per-scene roots, per-scene cameras / resolutions, one shared env map."""
+
+    def __init__(self, ctx, meshes, bvhs, materials, env=None, env_scale=1.0):
+        n = len(meshes)
+        self.ctx = ctx
+        self._mats = [(N.tt_material * len(m))(*m) for m in materials]
+        mesh_arr = (C.c_void_p * n)(*[m._h.value for m in meshes])
+        bvh_arr = (C.c_void_p * n)(*[b._h.value for b in bvhs])
+        mat_arr = (C.c_void_p * n)(*[C.cast(m, C.c_void_p).value for m in self._mats])
+        nm = (C.c_int32 * n)(*[len(m) for m in materials])
+        env_arr = None
+        ew = eh = 0
+        if env is not None:
+            env_arr = np.ascontiguousarray(env, dtype=np.float32)
+            eh, ew = env_arr.shape[:2]
+        h = C.c_void_p()
+        check(lib.tt_batch_create(ctx._h, n, mesh_arr, bvh_arr, mat_arr, nm,
+                                  _vp(env_arr) if env_arr is not None else None, ew, eh, float(env_scale), C.byref(h)))
+        self._h = h
+        self.n = n
+        ctx._adopt(self)
+
+    def render(self, cams, params):
+        imgs = [np.empty((c.height, c.width, 3), np.float32) for c in cams]
+        cam_arr = (N.tt_camera * len(cams))(*cams)
+        ptrs = (C.c_void_p * len(cams))(*[im.ctypes.data for im in imgs])
+        st = N.tt_render_stats()
+        check(lib.tt_batch_render(self._h, C.cast(cam_arr, C.c_void_p), C.byref(params), ptrs, C.byref(st)))
+        return imgs, _stats(st)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib.tt_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
 
 
 class Context:
@@ -435,9 +484,7 @@ class DeviceScene:
         out64 = np.empty((cam.height, cam.width, 3), np.float64) if want_fp64 else None
         st = N.tt_render_stats()
         check(lib.tt_render(self._h, C.byref(cam), C.byref(params), _vp(out), _vp(out64), C.byref(st)))
-        stats = {"rays_extension": st.rays_extension, "rays_shadow": st.rays_shadow, "samples": st.samples,
-                 "seconds": st.seconds, "iterations": st.iterations}
-        return (out, out64, stats) if want_fp64 else (out, stats)
+        return (out, out64, _stats(st)) if want_fp64 else (out, _stats(st))
 
     def env_samples(self, n, seed=1):
         out = np.empty((int(n), 8), np.float32)

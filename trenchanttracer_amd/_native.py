@@ -60,12 +60,14 @@ class tt_render_params(C.Structure):
     _fields_ = [("spp", C.c_int32), ("max_depth", C.c_int32), ("rr_depth", C.c_int32),
                 ("seed", C.c_uint32), ("mode", C.c_int32), ("guided", C.c_int32),
                 ("sample_offset", C.c_int32), ("accumulate_fp64", C.c_int32),
-                ("pi", C.c_float), ("two_pi", C.c_float), ("tmin", C.c_float), ("reserved", C.c_float)]
+                ("pi", C.c_float), ("two_pi", C.c_float), ("tmin", C.c_float), ("use_graph", C.c_int32),
+                ("paths_in_flight", C.c_int32), ("lds_stack", C.c_int32), ("block", C.c_int32),
+                ("variant", C.c_int32)]
 
 
 class tt_render_stats(C.Structure):
     _fields_ = [("rays_extension", C.c_int64), ("rays_shadow", C.c_int64), ("samples", C.c_int64),
-                ("seconds", C.c_double), ("iterations", C.c_int32), ("pad", C.c_int32)]
+                ("seconds", C.c_double), ("iterations", C.c_int32), ("paths_in_flight", C.c_int32)]
 
 
 P = C.c_void_p
@@ -119,6 +121,9 @@ PROTOS = {
     "tt_camera_from_orbit": (C.c_int, [PF, F32, F32, F32, F32, I32, I32, F32, F32, C.POINTER(tt_camera)]),
     "tt_render": (C.c_int, [P, C.POINTER(tt_camera), C.POINTER(tt_render_params), P, P, C.POINTER(tt_render_stats)]),
     "tt_debug_math": (C.c_int, [P, I32, P, P, P, I64]),
+    "tt_batch_create": (C.c_int, [P, I32, P, P, P, P, P, I32, I32, F32, C.POINTER(P)]),
+    "tt_batch_destroy": (None, [P]),
+    "tt_batch_render": (C.c_int, [P, P, C.POINTER(tt_render_params), P, C.POINTER(tt_render_stats)]),
     "tt_debug_bsdf_sample": (C.c_int, [P, C.POINTER(tt_material), PF, I32, I64, U32, P, P]),
     "tt_debug_env_sample": (C.c_int, [P, I64, U32, P]),
     "tt_trace_closest": (C.c_int, [P, P, I64, P, C.POINTER(tt_trace_cfg), P]),

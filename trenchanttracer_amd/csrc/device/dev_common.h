@@ -19,8 +19,7 @@ struct TraceArgs {
     int32_t *flags;        // any-hit output (or nullptr)
     int64_t n;
     int32_t num_nodes;
-    int32_t *work_counter; // persistent variant: next ray batch
-    int32_t batch;         // persistent variant: rays per wave fetch
+    const int32_t *ray_root;  // if set: root node of each ray (several scenes in one node array)
     const int32_t *n_dev;  // if set: ray count read on the device (capped at n)
     int32_t *reset0;       // if set: zeroed by the first lane of block 0
     int32_t *reset1;

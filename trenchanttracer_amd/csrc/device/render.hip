@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -32,6 +33,10 @@ struct RenderState {
     int32_t cap = 0;
     tt_ray *rays[2] = {nullptr, nullptr};
     int32_t *path[2] = {nullptr, nullptr};
+    int32_t *root[2] = {nullptr, nullptr};   // root node of each queued ray's scene
+    int32_t *sroot = nullptr;
+    int32_t *sample_dev = nullptr;            // first sample index of the current pass
+    void *table = nullptr;                    // device SceneTable
     tt_hit *hits = nullptr;
     float4 *thr = nullptr;       // reference: mask | path: throughput, w = pdf of the last BSDF sample
     float4 *L = nullptr;         // reference: path radiance
@@ -104,13 +109,34 @@ struct CamArgs {
     float focal, aperture, two_pi;
 };
 
+// Up to kMaxScenes scenes rendered in one wavefront: per scene its pixel range
+// in the concatenated image, resolution, root node and camera.
+constexpr int kMaxScenes = 64;
+struct SceneTable {
+    int32_t n;
+    int32_t pix_off[kMaxScenes + 1];
+    int32_t w[kMaxScenes], h[kMaxScenes], root[kMaxScenes];
+    CamArgs cam[kMaxScenes];
+};
+
+__device__ __forceinline__ int scene_of(const SceneTable *t, int32_t pix) {
+    int lo = 0, hi = t->n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (t->pix_off[mid] <= pix) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 struct GenArgs {
-    CamArgs cam;
-    int32_t w, h, mode, k;   // k samples per pixel in flight: path id = j * w * h + pixel
-    uint32_t sample, seed;
+    const SceneTable *tab;
+    int32_t npix, mode, k;   // k samples per pixel in flight: path id = j * npix + pixel
+    const int32_t *sample_dev;
+    uint32_t seed;
     float tmin;
     tt_ray *rays;
-    int32_t *path;
+    int32_t *path, *root;
     float4 *thr, *L;
     int32_t *flags;
     int32_t *counts;
@@ -118,7 +144,7 @@ struct GenArgs {
 
 __global__ __launch_bounds__(256) void k_generate(GenArgs a) {
     const int32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    const int32_t npix = a.w * a.h;
+    const int32_t npix = a.npix;
     if (p == 0) {
         a.counts[0] = npix * a.k;
         a.counts[1] = 0;
@@ -126,14 +152,17 @@ __global__ __launch_bounds__(256) void k_generate(GenArgs a) {
     }
     if (p >= npix * a.k) return;
     const int32_t pix = p % npix;
-    const int32_t X = pix % a.w, Y = pix / a.w;
+    const int sid = scene_of(a.tab, pix);
+    const int32_t local = pix - a.tab->pix_off[sid];
+    const int32_t W = a.tab->w[sid], H = a.tab->h[sid];
+    const int32_t X = local % W, Y = local / W;
     float u[4];
-    draws((uint32_t)pix, a.sample + (uint32_t)(p / npix), 0u, 0u, a.seed, u);
+    draws((uint32_t)local, (uint32_t)*a.sample_dev + (uint32_t)(p / npix), 0u, 0u, a.seed, u);
     // camera ray, RenderKernel.cu:152-197
     const float jx = u[0] - 0.5f, jy = u[1] - 0.5f;
-    const float sx = (jx + (float)X) / (float)(a.w - 1);
-    const float sy = (jy + (float)Y) / (float)(a.h - 1);
-    const CamArgs &c = a.cam;
+    const float sx = (jx + (float)X) / (float)(W - 1);
+    const float sy = (jy + (float)Y) / (float)(H - 1);
+    const CamArgs &c = a.tab->cam[sid];
     f3 on_plane = (c.middle + c.horizontal * ((2 * sx) - 1)) + c.vertical * ((2 * sy) - 1);
     f3 on_image = c.pos + (on_plane - c.pos) * c.focal;
     f3 ap = c.pos;
@@ -146,10 +175,14 @@ __global__ __launch_bounds__(256) void k_generate(GenArgs a) {
     f3 d = vnorm(vnorm(on_image - ap));
     a.rays[p] = {ap.x, ap.y, ap.z, a.tmin, d.x, d.y, d.z, 1e20f};
     a.path[p] = p;
+    a.root[p] = a.tab->root[sid];
     a.thr[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     if (a.L) a.L[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (a.flags) a.flags[p] = 0;
 }
+
+__global__ void k_advance_sample(int32_t *sample_dev, int32_t k) { *sample_dev += k; }
+__global__ void k_set_sample(int32_t *sample_dev, int32_t v) { *sample_dev = v; }
 
 struct SceneView {
     const float4 *tri_normals;
@@ -172,8 +205,12 @@ struct SceneView {
 
 struct ShadeArgs {
     SceneView sv;
+    const SceneTable *tab;
+    const int32_t *sample_dev;
     tt_ray *rays_in, *rays_out;
     int32_t *path_in, *path_out;
+    const int32_t *root_in;
+    int32_t *root_out, *sroot;
     const tt_hit *hits;
     float4 *thr, *L;
     int32_t *flags;
@@ -185,8 +222,8 @@ struct ShadeArgs {
     double *acc64;
     unsigned long long *stats;
     int32_t bounce, max_depth, rr_depth;
-    int32_t npix;     // path id = j * npix + pixel, sample = sample + j
-    uint32_t sample, seed;
+    int32_t npix;     // path id = j * npix + pixel, sample = *sample_dev + j
+    uint32_t seed;
     float inv_spp, pi, two_pi;
 };
 
@@ -210,10 +247,14 @@ __global__ __launch_bounds__(256) void k_shade_reference(ShadeArgs a) {
     const int32_t n = a.counts[a.cur];
     if (i == 0) atomicAdd(&a.stats[0], (unsigned long long)n);
     bool cont = false;
-    int32_t p = 0;
+    int32_t p = 0, root = 0;
     tt_ray nr{};
     if (i < n) {
         p = a.path_in[i];
+        root = a.root_in[i];
+        const int32_t pix = p % a.npix;
+        const int32_t local = pix - a.tab->pix_off[scene_of(a.tab, pix)];
+        const uint32_t smp = (uint32_t)*a.sample_dev + (uint32_t)(p / a.npix);
         const tt_ray r = a.rays_in[i];
         const tt_hit h = a.hits[i];
         f3 mask = xyz(a.thr[p]);
@@ -237,7 +278,7 @@ __global__ __launch_bounds__(256) void k_shade_reference(ShadeArgs a) {
                 finish = true;
             } else {
                 float u[4];
-                draws((uint32_t)p, a.sample, (uint32_t)(a.bounce + 1), 0u, a.seed, u);
+                draws((uint32_t)local, smp, (uint32_t)(a.bounce + 1), 0u, a.seed, u);
                 f3 no = ro, nd = rd;
                 const f3 alb = F3(m.albedo[0], m.albedo[1], m.albedo[2]);
                 if (m.type == TT_MAT_DIFFUSE) {
@@ -305,7 +346,7 @@ __global__ __launch_bounds__(256) void k_shade_reference(ShadeArgs a) {
             }
         }
         if (finish) {
-            float *o = a.acc + 3 * (int64_t)p;
+            float *o = a.acc + 3 * (int64_t)pix;
             o[0] = o[0] + Lp.x * a.inv_spp;
             o[1] = o[1] + Lp.y * a.inv_spp;
             o[2] = o[2] + Lp.z * a.inv_spp;
@@ -315,6 +356,7 @@ __global__ __launch_bounds__(256) void k_shade_reference(ShadeArgs a) {
     if (cont) {
         a.rays_out[slot] = nr;
         a.path_out[slot] = p;
+        a.root_out[slot] = root;
     }
 }
 
@@ -496,7 +538,7 @@ __global__ __launch_bounds__(256) void k_shade_path(ShadeArgs a) {
     const int32_t n = a.counts[a.cur];
     if (i == 0) atomicAdd(&a.stats[0], (unsigned long long)n);
     bool cont = false, shadow = false;
-    int32_t p = 0;
+    int32_t p = 0, root = 0;
     tt_ray nr{}, sr{};
     f3 scontrib = F3(0, 0, 0);
     const SceneView &sv = a.sv;
@@ -504,8 +546,10 @@ __global__ __launch_bounds__(256) void k_shade_path(ShadeArgs a) {
     const float p_emit = sv.num_emit > 0 ? 1.0f - p_env : 0.0f;
     if (i < n) {
         p = a.path_in[i];
+        root = a.root_in[i];
         const int32_t pix = p % a.npix;
-        const uint32_t smp = a.sample + (uint32_t)(p / a.npix);
+        const int32_t local = pix - a.tab->pix_off[scene_of(a.tab, pix)];
+        const uint32_t smp = (uint32_t)*a.sample_dev + (uint32_t)(p / a.npix);
         const tt_ray r = a.rays_in[i];
         const tt_hit h = a.hits[i];
         const float4 tp = a.thr[p];
@@ -546,8 +590,8 @@ __global__ __launch_bounds__(256) void k_shade_path(ShadeArgs a) {
                 const Frame fr = make_frame(ns);
                 const f3 wol = to_local(fr, wo);
                 float ul[4], ub[4];
-                draws((uint32_t)pix, smp, (uint32_t)(a.bounce + 1), 1u, a.seed, ul);
-                draws((uint32_t)pix, smp, (uint32_t)(a.bounce + 1), 2u, a.seed, ub);
+                draws((uint32_t)local, smp, (uint32_t)(a.bounce + 1), 1u, a.seed, ul);
+                draws((uint32_t)local, smp, (uint32_t)(a.bounce + 1), 2u, a.seed, ub);
                 const bool smooth = m.type == TT_MAT_DIELECTRIC;
                 // next-event estimation
                 if (!smooth && (p_env > 0.0f || p_emit > 0.0f) && wol.z > 0.0f) {
@@ -634,10 +678,12 @@ __global__ __launch_bounds__(256) void k_shade_path(ShadeArgs a) {
     if (cont) {
         a.rays_out[slot] = nr;
         a.path_out[slot] = p;
+        a.root_out[slot] = root;
     }
     const int sslot = wave_append(shadow, &a.counts[2]);
     if (shadow) {
         a.srays[sslot] = sr;
+        a.sroot[sslot] = root;
         a.scontrib[sslot] = make_float4(scontrib.x, scontrib.y, scontrib.z, __int_as_float(p % a.npix));
     }
 }
@@ -761,6 +807,8 @@ HF3 hadd(HF3 a, HF3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
 HF3 hscale(HF3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
 f3 tof3(HF3 a) { return {a.x, a.y, a.z}; }
 
+// Host-side camera basis: the same float operations as the host restatement
+// (RenderKernel.cu:131-146, evaluated once per render instead of per sample).
 CamArgs camera_args(const tt_camera &c, float pi, float two_pi) {
     HF3 pos = {c.eye[0], c.eye[1], c.eye[2]};
     HF3 view = hnorm({c.view[0], c.view[1], c.view[2]});
@@ -799,16 +847,20 @@ tt_status ensure_state(tt_scene *s, int32_t cap, int32_t npix, bool fp64) {
     for (int k = 0; k < 2; ++k) {
         al((void **)&rs->rays[k], (size_t)cap * sizeof(tt_ray), "render rays");
         al((void **)&rs->path[k], (size_t)cap * 4, "render path ids");
+        al((void **)&rs->root[k], (size_t)cap * 4, "render ray roots");
     }
     al((void **)&rs->hits, (size_t)cap * sizeof(tt_hit), "render hits");
     al((void **)&rs->thr, (size_t)cap * 16, "render throughput");
     al((void **)&rs->L, (size_t)cap * 16, "render radiance");
     al((void **)&rs->flags, (size_t)cap * 4, "render flags");
     al((void **)&rs->srays, (size_t)cap * sizeof(tt_ray), "render shadow rays");
+    al((void **)&rs->sroot, (size_t)cap * 4, "render shadow roots");
     al((void **)&rs->scontrib, (size_t)cap * 16, "render shadow contributions");
     al((void **)&rs->socc, (size_t)cap * 4, "render shadow flags");
     al((void **)&rs->counts, 16, "render queue counts");
     al((void **)&rs->stats, 16, "render stats");
+    al((void **)&rs->sample_dev, 16, "render sample index");
+    al((void **)&rs->table, sizeof(SceneTable), "render scene table");
     al((void **)&rs->acc, (size_t)npix * 12, "render accumulation");
     if (fp64) {
         al((void **)&rs->acc64, (size_t)npix * 24, "render FP64 accumulation");
@@ -818,15 +870,280 @@ tt_status ensure_state(tt_scene *s, int32_t cap, int32_t npix, bool fp64) {
     return st;
 }
 
+// One bounce of the wavefront: extend -> shade (-> shadow -> apply).
+hipError_t launch_bounce(tt_scene *s, RenderState *rs, const tt_render_params *p, const SceneView &sv, int32_t cap,
+                         int32_t npix, int b, bool fp64, hipStream_t hs) {
+    const int cur = b & 1, nxt = cur ^ 1;
+    const int tri_mode = p->mode == TT_RENDER_REFERENCE ? kTriReference : kTriTwoSided;
+    // reference mode: the exact loop (hits identical to the host restatement);
+    // path mode: the speculative while-while loop unless told otherwise
+    const int variant = p->mode == TT_RENDER_REFERENCE ? 0 : (p->variant >= 0 ? p->variant : 1);
+    const int lds = p->lds_stack > 0 ? p->lds_stack : 16;
+    const int block = p->block > 0 ? p->block : 64;
+    const unsigned gblocks = (unsigned)((cap + 255) / 256);
+    TraceArgs ta{};
+    ta.nodes = (const float4 *)s->nodes;
+    ta.tris = (const float4 *)s->tris;
+    ta.rays = rs->rays[cur];
+    ta.hits = rs->hits;
+    ta.n = cap;
+    ta.num_nodes = (int32_t)s->num_nodes;
+    ta.n_dev = rs->counts + cur;
+    ta.reset0 = rs->counts + nxt;
+    ta.reset1 = rs->counts + 2;
+    ta.ray_root = rs->root[cur];
+    hipError_t e = launch_trace(ta, variant, false, lds, block, 0, s->ctx->num_cus, hs, tri_mode);
+    if (e != hipSuccess) return e;
+    ShadeArgs sa{};
+    sa.sv = sv;
+    sa.tab = (const SceneTable *)rs->table;
+    sa.sample_dev = rs->sample_dev;
+    sa.rays_in = rs->rays[cur];
+    sa.rays_out = rs->rays[nxt];
+    sa.path_in = rs->path[cur];
+    sa.path_out = rs->path[nxt];
+    sa.root_in = rs->root[cur];
+    sa.root_out = rs->root[nxt];
+    sa.sroot = rs->sroot;
+    sa.hits = rs->hits;
+    sa.thr = rs->thr;
+    sa.L = rs->L;
+    sa.flags = rs->flags;
+    sa.counts = rs->counts;
+    sa.cur = cur;
+    sa.nxt = nxt;
+    sa.srays = rs->srays;
+    sa.scontrib = rs->scontrib;
+    sa.acc = rs->acc;
+    sa.acc64 = fp64 ? rs->acc64 : nullptr;
+    sa.stats = rs->stats;
+    sa.bounce = b;
+    sa.max_depth = p->max_depth;
+    sa.rr_depth = p->rr_depth;
+    sa.npix = npix;
+    sa.seed = p->seed;
+    sa.inv_spp = 1.0f / (float)p->spp;
+    sa.pi = p->pi;
+    sa.two_pi = p->two_pi;
+    if (p->mode == TT_RENDER_REFERENCE) {
+        hipLaunchKernelGGL(k_shade_reference, dim3(gblocks), dim3(256), 0, hs, sa);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL(k_shade_path, dim3(gblocks), dim3(256), 0, hs, sa);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    TraceArgs sh{};
+    sh.nodes = (const float4 *)s->nodes;
+    sh.tris = (const float4 *)s->tris;
+    sh.rays = rs->srays;
+    sh.flags = rs->socc;
+    sh.n = cap;
+    sh.num_nodes = (int32_t)s->num_nodes;
+    sh.n_dev = rs->counts + 2;
+    sh.ray_root = rs->sroot;
+    e = launch_trace(sh, variant, true, lds, block, 0, s->ctx->num_cus, hs, kTriTwoSided);
+    if (e != hipSuccess) return e;
+    ApplyArgs aa{rs->counts, rs->socc, rs->scontrib, rs->acc, fp64 ? rs->acc64 : nullptr, rs->stats};
+    hipLaunchKernelGGL(k_apply_shadow, dim3(gblocks), dim3(256), 0, hs, aa);
+    return hipGetLastError();
+}
+
+// One sample pass (kk samples per pixel in flight) for all scenes of the table.
+hipError_t launch_pass(tt_scene *s, RenderState *rs, const tt_render_params *p, const SceneView &sv, int32_t npix,
+                       int32_t kk, bool fp64, hipStream_t hs) {
+    const int32_t cap = npix * kk;
+    GenArgs g{};
+    g.tab = (const SceneTable *)rs->table;
+    g.npix = npix;
+    g.mode = p->mode;
+    g.k = kk;
+    g.sample_dev = rs->sample_dev;
+    g.seed = p->seed;
+    g.tmin = p->mode == TT_RENDER_REFERENCE ? p->tmin : 0.0f;
+    g.rays = rs->rays[0];
+    g.path = rs->path[0];
+    g.root = rs->root[0];
+    g.thr = rs->thr;
+    g.L = rs->L;
+    g.flags = rs->flags;
+    g.counts = rs->counts;
+    hipLaunchKernelGGL(k_generate, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, hs, g);
+    hipError_t e = hipGetLastError();
+    for (int32_t b = 0; e == hipSuccess && b < p->max_depth; ++b) e = launch_bounce(s, rs, p, sv, cap, npix, b, fp64, hs);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_advance_sample, dim3(1), dim3(1), 0, hs, rs->sample_dev, kk);
+        e = hipGetLastError();
+    }
+    return e;
+}
+
+SceneView scene_view(const tt_scene *s, const tt_render_params *p) {
+    SceneView sv{};
+    sv.tri_normals = (const float4 *)s->tri_normals;
+    sv.tri_geo = (const float4 *)s->tri_geo;
+    sv.tri_mat = s->tri_mat;
+    sv.mats = s->mats;
+    sv.num_mats = s->num_mats;
+    sv.env = (const float4 *)s->env_rgb;
+    sv.env_marg = s->env_marg_cdf;
+    sv.env_cond = s->env_cond_cdf;
+    sv.env_w = s->env_w;
+    sv.env_h = s->env_h;
+    sv.env_scale = s->env_scale;
+    sv.env_inv_total = s->env_total > 0.0 ? (float)(1.0 / s->env_total) : 0.0f;
+    if (p->mode == TT_RENDER_PATH && s->env_total <= 0.0) sv.env = nullptr;  // black map: nothing to sample
+    sv.emit_tri = s->emit_tri;
+    sv.emit_cdf = s->emit_cdf;
+    sv.emit_verts = (const float4 *)s->emit_verts;
+    sv.num_emit = s->num_emit;
+    sv.emit_inv_area = s->emit_area > 0.0f ? 1.0f / s->emit_area : 0.0f;
+    return sv;
+}
+
+tt_status validate_params(const tt_render_params *p) {
+    if (p->spp <= 0 || p->max_depth <= 0 || p->max_depth > 1024 || p->sample_offset < 0)
+        return fail(TT_ERR_INVALID, "render: spp / max_depth out of range");
+    if (p->mode != TT_RENDER_REFERENCE && p->mode != TT_RENDER_PATH) return fail(TT_ERR_INVALID, "render: bad mode");
+    if (p->guided) return fail(TT_ERR_UNSUPPORTED, "render: guided rendering needs a guide (tt_guide_render)");
+    if (p->mode == TT_RENDER_REFERENCE && p->accumulate_fp64)
+        return fail(TT_ERR_INVALID, "render: FP64 accumulation is a path-mode option");
+    if (p->lds_stack != 0 && p->lds_stack != 8 && p->lds_stack != 16 && p->lds_stack != 24 && p->lds_stack != 32)
+        return fail(TT_ERR_INVALID, "render: lds_stack must be 0, 8, 16, 24 or 32");
+    if (p->block != 0 && (p->block < 64 || p->block > 256 || p->block % 64))
+        return fail(TT_ERR_INVALID, "render: block must be 0, 64, 128, 192 or 256");
+    if (p->variant < -1 || p->variant > 1) return fail(TT_ERR_INVALID, "render: variant must be -1, 0 or 1");
+    if (p->paths_in_flight < 0) return fail(TT_ERR_INVALID, "render: paths_in_flight must be >= 0");
+    return TT_OK;
+}
+
+// Renders nsc scenes sharing one node / triangle array (roots[k] = root node of
+// scene k) in one wavefront. out: host, concatenated images in scene order.
+tt_status render_impl(tt_scene *s, int32_t nsc, const tt_camera *cams, const int32_t *roots,
+                      const tt_render_params *p, float *out, double *out64, tt_render_stats *stats) {
+    tt_status st = validate_params(p);
+    if (st != TT_OK) return st;
+    if (nsc <= 0 || nsc > kMaxScenes) return fail(TT_ERR_INVALID, "render: 1..64 scenes");
+    if (!s->tri_normals || !s->tri_mat || !s->mats || !s->tri_geo)
+        return fail(TT_ERR_INVALID, "render: scene has no shading data");
+    SceneTable tab{};
+    tab.n = nsc;
+    int64_t npix64 = 0;
+    for (int32_t k = 0; k < nsc; ++k) {
+        const tt_camera &c = cams[k];
+        if (c.width < 2 || c.height < 2 || (int64_t)c.width * c.height > (1 << 26))
+            return fail(TT_ERR_INVALID, "render: resolution out of range");
+        tab.pix_off[k] = (int32_t)npix64;
+        tab.w[k] = c.width;
+        tab.h[k] = c.height;
+        tab.root[k] = roots[k];
+        tab.cam[k] = camera_args(c, p->pi, p->two_pi);
+        npix64 += (int64_t)c.width * c.height;
+    }
+    if (npix64 > (1 << 26)) return fail(TT_ERR_INVALID, "render: total resolution out of range");
+    const int32_t npix = (int32_t)npix64;
+    tab.pix_off[nsc] = npix;
+    const bool fp64 = p->accumulate_fp64 != 0;
+    tt_ctx *ctx = s->ctx;
+    if (hipSetDevice(ctx->device) != hipSuccess) return fail(TT_ERR_DEVICE, "hipSetDevice failed");
+    // path mode keeps k samples per pixel in flight (about 8M paths) so that deep
+    // bounces still fill the GPU; reference mode runs one sample per pass, which
+    // keeps the per-pixel float sums in sample order
+    int32_t k = 1;
+    if (p->mode == TT_RENDER_PATH) {
+        k = p->paths_in_flight > 0 ? std::max(1, p->paths_in_flight / npix) : std::max(1, (8 << 20) / npix);
+        k = std::min(k, p->spp);
+        while (k > 1 && (int64_t)k * npix > (1LL << 27)) --k;
+    }
+    st = ensure_state(s, npix * k, npix, fp64);
+    if (st != TT_OK) return st;
+    RenderState *rs = s->rs;
+    hipStream_t hs = ctx->stream;
+    const SceneView sv = scene_view(s, p);
+    const int64_t n3 = (int64_t)npix * 3;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    hipError_t e = hipMemcpyAsync(rs->table, &tab, sizeof(tab), hipMemcpyHostToDevice, hs);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipEventRecord(e0, hs);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_zero, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, hs, rs->acc,
+                           fp64 ? rs->acc64 : nullptr, rs->stats, n3);
+        hipLaunchKernelGGL(k_set_sample, dim3(1), dim3(1), 0, hs, rs->sample_dev, p->sample_offset);
+        e = hipGetLastError();
+    }
+    int32_t iters = 0;
+    const int32_t full = p->spp / k, rest = p->spp % k;
+    if (e == hipSuccess && p->use_graph && full > 1) {
+        // capture one pass (generate, max_depth x (extend, shade, shadow, apply),
+        // advance the sample index) and replay it
+        e = hipStreamBeginCapture(hs, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            hipError_t ce = launch_pass(s, rs, p, sv, npix, k, fp64, hs);
+            e = hipStreamEndCapture(hs, &graph);
+            if (e == hipSuccess) e = ce;
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+        for (int32_t i = 0; e == hipSuccess && i < full; ++i) e = hipGraphLaunch(gexec, hs);
+        iters += full * p->max_depth;
+    } else {
+        for (int32_t i = 0; e == hipSuccess && i < full; ++i) e = launch_pass(s, rs, p, sv, npix, k, fp64, hs);
+        iters += full * p->max_depth;
+    }
+    if (e == hipSuccess && rest > 0) {
+        e = launch_pass(s, rs, p, sv, npix, rest, fp64, hs);
+        iters += p->max_depth;
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, hs);
+    std::vector<float> acc((size_t)n3);
+    std::vector<double> acc64;
+    unsigned long long hstats[2] = {0, 0};
+    if (e == hipSuccess) e = hipMemcpyAsync(acc.data(), rs->acc, (size_t)n3 * 4, hipMemcpyDeviceToHost, hs);
+    if (e == hipSuccess && fp64) {
+        acc64.resize((size_t)n3);
+        e = hipMemcpyAsync(acc64.data(), rs->acc64, (size_t)n3 * 8, hipMemcpyDeviceToHost, hs);
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(hstats, rs->stats, 16, hipMemcpyDeviceToHost, hs);
+    if (e == hipSuccess) e = hipStreamSynchronize(hs);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (gexec) hipGraphExecDestroy(gexec);
+    if (graph) hipGraphDestroy(graph);
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (e != hipSuccess) return fail(TT_ERR_DEVICE, std::string("render: ") + hipGetErrorString(e));
+    if (p->mode == TT_RENDER_REFERENCE) {
+        std::memcpy(out, acc.data(), (size_t)n3 * 4);
+    } else if (fp64) {
+        for (int64_t q = 0; q < n3; ++q) out[q] = (float)(acc64[q] / (double)p->spp);
+        if (out64)
+            for (int64_t q = 0; q < n3; ++q) out64[q] = acc64[q] / (double)p->spp;
+    } else {
+        const float inv = 1.0f / (float)p->spp;
+        for (int64_t q = 0; q < n3; ++q) out[q] = acc[q] * inv;
+    }
+    if (stats) {
+        stats->rays_extension = (int64_t)hstats[0];
+        stats->rays_shadow = (int64_t)hstats[1];
+        stats->samples = (int64_t)npix * p->spp;
+        stats->seconds = ms * 1e-3;
+        stats->iterations = iters;
+        stats->paths_in_flight = npix * k;
+    }
+    return TT_OK;
+}
+
 }  // namespace
 
 void free_render_state(tt_scene *s) {
     RenderState *rs = s->rs;
     if (!rs) return;
-    void *ptrs[] = {rs->rays[0], rs->rays[1], rs->path[0], rs->path[1], rs->hits, rs->thr, rs->L, rs->flags,
-                    rs->srays, rs->scontrib, rs->socc, rs->counts, rs->stats, rs->acc, rs->acc64};
-    for (void *p : ptrs)
-        if (p) hipFree(p);
+    void *ptrs[] = {rs->rays[0], rs->rays[1], rs->path[0], rs->path[1], rs->root[0], rs->root[1], rs->hits, rs->thr,
+                    rs->L, rs->flags, rs->srays, rs->sroot, rs->scontrib, rs->socc, rs->counts, rs->stats,
+                    rs->sample_dev, rs->table, rs->acc, rs->acc64};
+    for (void *q : ptrs)
+        if (q) hipFree(q);
     delete rs;
     s->rs = nullptr;
 }
@@ -840,6 +1157,13 @@ using namespace tt;
         hipError_t e_ = (call);                                                                   \
         if (e_ != hipSuccess) return fail(TT_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e_)); \
     } while (0)
+
+struct tt_batch {
+    tt_scene *scene = nullptr;   // all scenes concatenated
+    tt_mesh *mesh = nullptr;
+    tt_bvh *bvh = nullptr;
+    std::vector<int32_t> roots;
+};
 
 extern "C" {
 
@@ -878,176 +1202,97 @@ tt_status tt_camera_from_orbit(const float center[3], float yaw, float pitch, fl
 tt_status tt_render(tt_scene *s, const tt_camera *cam, const tt_render_params *p, float *rgb_out, double *rgb64_out,
                     tt_render_stats *stats) {
     if (!s || !cam || !p || !rgb_out) return fail(TT_ERR_INVALID, "tt_render: null argument");
-    if (cam->width < 2 || cam->height < 2 || (int64_t)cam->width * cam->height > (1 << 26))
-        return fail(TT_ERR_INVALID, "tt_render: resolution out of range");
-    if (p->spp <= 0 || p->max_depth <= 0 || p->max_depth > 1024 || p->sample_offset < 0)
-        return fail(TT_ERR_INVALID, "tt_render: spp / max_depth out of range");
-    if (p->mode != TT_RENDER_REFERENCE && p->mode != TT_RENDER_PATH) return fail(TT_ERR_INVALID, "tt_render: bad mode");
-    if (p->guided) return fail(TT_ERR_UNSUPPORTED, "tt_render: guided rendering needs a guide (tt_guide_render)");
-    if (!s->tri_normals || !s->tri_mat || !s->mats) return fail(TT_ERR_INVALID, "tt_render: scene has no shading data");
-    const bool fp64 = p->accumulate_fp64 != 0;
-    if (p->mode == TT_RENDER_REFERENCE && fp64)
-        return fail(TT_ERR_INVALID, "tt_render: FP64 accumulation is a path-mode option");
-    tt_ctx *ctx = s->ctx;
-    TT_HIPR(hipSetDevice(ctx->device), "hipSetDevice");
-    const int32_t W = cam->width, H = cam->height, npix = W * H;
-    // path mode keeps k samples per pixel in flight (about 8M paths) so that
-    // deep bounces still fill the GPU; reference mode runs one sample per pass,
-    // which keeps the per-pixel float sums in sample order
-    int32_t k = 1;
-    if (p->mode == TT_RENDER_PATH) {
-        k = std::max(1, std::min(p->spp, (8 << 20) / npix));
-    }
-    const int32_t cap = npix * k;
-    tt_status st = ensure_state(s, cap, npix, fp64);
-    if (st != TT_OK) return st;
-    RenderState *rs = s->rs;
-    hipStream_t hs = ctx->stream;
+    const int32_t root = 0;
+    return render_impl(s, 1, cam, &root, p, rgb_out, rgb64_out, stats);
+}
 
-    const CamArgs cargs = camera_args(*cam, p->pi, p->two_pi);
-    SceneView sv{};
-    sv.tri_normals = (const float4 *)s->tri_normals;
-    sv.tri_geo = (const float4 *)s->tri_geo;
-    sv.tri_mat = s->tri_mat;
-    sv.mats = s->mats;
-    sv.num_mats = s->num_mats;
-    sv.env = (const float4 *)s->env_rgb;
-    sv.env_marg = s->env_marg_cdf;
-    sv.env_cond = s->env_cond_cdf;
-    sv.env_w = s->env_w;
-    sv.env_h = s->env_h;
-    sv.env_scale = s->env_scale;
-    sv.env_inv_total = s->env_total > 0.0 ? (float)(1.0 / s->env_total) : 0.0f;
-    if (p->mode == TT_RENDER_PATH && s->env_total <= 0.0) sv.env = nullptr;  // black map: nothing to sample
-    sv.emit_tri = s->emit_tri;
-    sv.emit_cdf = s->emit_cdf;
-    sv.emit_verts = (const float4 *)s->emit_verts;
-    sv.num_emit = s->num_emit;
-    sv.emit_inv_area = s->emit_area > 0.0f ? 1.0f / s->emit_area : 0.0f;
-
-    const int64_t n3 = (int64_t)npix * 3;
-    const unsigned gblocks = (unsigned)((cap + 255) / 256);
-    hipEvent_t e0, e1;
-    TT_HIPR(hipEventCreate(&e0), "hipEventCreate");
-    TT_HIPR(hipEventCreate(&e1), "hipEventCreate");
-    hipEventRecord(e0, hs);
-    hipLaunchKernelGGL(k_zero, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, hs, rs->acc, fp64 ? rs->acc64 : nullptr,
-                       rs->stats, n3);
-    const int tri_mode = p->mode == TT_RENDER_REFERENCE ? kTriReference : kTriTwoSided;
-    int32_t iters = 0;
-    for (int32_t smp = 0; smp < p->spp; smp += k) {
-        const int32_t kk = std::min(k, p->spp - smp);
-        GenArgs g{};
-        g.cam = cargs;
-        g.w = W;
-        g.h = H;
-        g.mode = p->mode;
-        g.k = kk;
-        g.sample = (uint32_t)(p->sample_offset + smp);
-        g.seed = p->seed;
-        g.tmin = p->mode == TT_RENDER_REFERENCE ? p->tmin : 0.0f;
-        g.rays = rs->rays[0];
-        g.path = rs->path[0];
-        g.thr = rs->thr;
-        g.L = rs->L;
-        g.flags = rs->flags;
-        g.counts = rs->counts;
-        hipLaunchKernelGGL(k_generate, dim3(gblocks), dim3(256), 0, hs, g);
-        for (int32_t b = 0; b < p->max_depth; ++b) {
-            const int cur = b & 1, nxt = cur ^ 1;
-            TraceArgs ta{};
-            ta.nodes = (const float4 *)s->nodes;
-            ta.tris = (const float4 *)s->tris;
-            ta.rays = rs->rays[cur];
-            ta.hits = rs->hits;
-            ta.n = cap;
-            ta.num_nodes = (int32_t)s->num_nodes;
-            ta.n_dev = rs->counts + cur;
-            ta.reset0 = rs->counts + nxt;
-            ta.reset1 = rs->counts + 2;
-            // reference mode: the exact loop (hits identical to the host restatement);
-            // path mode: the speculative while-while loop
-            const int variant = p->mode == TT_RENDER_REFERENCE ? 0 : 1;
-            TT_HIPR(launch_trace(ta, variant, false, 16, 64, 0, ctx->num_cus, hs, tri_mode), "render extend");
-            ShadeArgs sa{};
-            sa.sv = sv;
-            sa.rays_in = rs->rays[cur];
-            sa.rays_out = rs->rays[nxt];
-            sa.path_in = rs->path[cur];
-            sa.path_out = rs->path[nxt];
-            sa.hits = rs->hits;
-            sa.thr = rs->thr;
-            sa.L = rs->L;
-            sa.flags = rs->flags;
-            sa.counts = rs->counts;
-            sa.cur = cur;
-            sa.nxt = nxt;
-            sa.srays = rs->srays;
-            sa.scontrib = rs->scontrib;
-            sa.acc = rs->acc;
-            sa.acc64 = fp64 ? rs->acc64 : nullptr;
-            sa.stats = rs->stats;
-            sa.bounce = b;
-            sa.max_depth = p->max_depth;
-            sa.rr_depth = p->rr_depth;
-            sa.npix = npix;
-            sa.sample = (uint32_t)(p->sample_offset + smp);
-            sa.seed = p->seed;
-            sa.inv_spp = 1.0f / (float)p->spp;
-            sa.pi = p->pi;
-            sa.two_pi = p->two_pi;
-            if (p->mode == TT_RENDER_REFERENCE) {
-                hipLaunchKernelGGL(k_shade_reference, dim3(gblocks), dim3(256), 0, hs, sa);
-            } else {
-                hipLaunchKernelGGL(k_shade_path, dim3(gblocks), dim3(256), 0, hs, sa);
-                TraceArgs sh{};
-                sh.nodes = (const float4 *)s->nodes;
-                sh.tris = (const float4 *)s->tris;
-                sh.rays = rs->srays;
-                sh.flags = rs->socc;
-                sh.n = cap;
-                sh.num_nodes = (int32_t)s->num_nodes;
-                sh.n_dev = rs->counts + 2;
-                TT_HIPR(launch_trace(sh, 1, true, 16, 64, 0, ctx->num_cus, hs, kTriTwoSided), "render shadow");
-                ApplyArgs aa{rs->counts, rs->socc, rs->scontrib, rs->acc, fp64 ? rs->acc64 : nullptr, rs->stats};
-                hipLaunchKernelGGL(k_apply_shadow, dim3(gblocks), dim3(256), 0, hs, aa);
+tt_status tt_batch_create(tt_ctx *ctx, int32_t n, const tt_mesh *const *meshes, const tt_bvh *const *bvhs,
+                          const tt_material *const *mats, const int32_t *nmats, const float *env_rgb, int32_t env_w,
+                          int32_t env_h, float env_scale, tt_batch **out) {
+    if (!ctx || !out || n <= 0 || n > kMaxScenes || !meshes || !bvhs || !mats || !nmats)
+        return fail(TT_ERR_INVALID, "tt_batch_create: bad arguments (1..64 scenes)");
+    try {
+        auto b = std::make_unique<tt_batch>();
+        b->mesh = new tt_mesh();
+        b->bvh = new tt_bvh();
+        Mesh &M = b->mesh->m;
+        Flat &F = b->bvh->b.flat;
+        std::vector<tt_material> allm;
+        for (int32_t k = 0; k < n; ++k) {
+            if (!meshes[k] || !bvhs[k] || nmats[k] <= 0 || !mats[k]) return fail(TT_ERR_INVALID, "tt_batch_create: null scene");
+            const Mesh &m = meshes[k]->m;
+            const Flat &f = bvhs[k]->b.flat;
+            const int32_t vo = (int32_t)M.verts.size(), no = (int32_t)M.normals.size(), to = M.ntris();
+            const int32_t mo = (int32_t)allm.size();
+            const int32_t node_o = (int32_t)(F.dev_nodes.size() / 16), rec_o = (int32_t)(F.dev_tris.size() / 12);
+            if (f.dev_nodes.empty()) return fail(TT_ERR_INVALID, "tt_batch_create: empty tree");
+            M.verts.insert(M.verts.end(), m.verts.begin(), m.verts.end());
+            M.normals.insert(M.normals.end(), m.normals.begin(), m.normals.end());
+            for (int32_t x : m.tri_v) M.tri_v.push_back(x + vo);
+            for (int32_t x : m.tri_n) M.tri_n.push_back(x + no);
+            for (int32_t x : m.tri_mat) {
+                if (x < 0 || x >= nmats[k]) return fail(TT_ERR_INVALID, "tt_batch_create: material id out of range");
+                M.tri_mat.push_back(x + mo);
             }
-            TT_HIPR(hipGetLastError(), "render launch");
-            ++iters;
+            allm.insert(allm.end(), mats[k], mats[k] + nmats[k]);
+            b->roots.push_back(node_o);
+            for (size_t q = 0; q < f.dev_nodes.size(); q += 16) {
+                int32_t w[16];
+                std::memcpy(w, &f.dev_nodes[q], 64);
+                for (int c = 0; c < 2; ++c) w[12 + c] = w[12 + c] >= 0 ? w[12 + c] + node_o : ~(~w[12 + c] + rec_o);
+                F.dev_nodes.insert(F.dev_nodes.end(), w, w + 16);
+            }
+            for (size_t q = 0; q < f.dev_tris.size(); q += 12) {
+                float r[12];
+                std::memcpy(r, &f.dev_tris[q], 48);
+                int32_t id, mid;
+                std::memcpy(&id, &r[3], 4);
+                std::memcpy(&mid, &r[7], 4);
+                if (id >= 0) id += to;
+                mid += mo;
+                std::memcpy(&r[3], &id, 4);
+                std::memcpy(&r[7], &mid, 4);
+                F.dev_tris.insert(F.dev_tris.end(), r, r + 12);
+            }
         }
+        tt_status st = tt_scene_create(ctx, b->mesh, b->bvh, allm.data(), (int32_t)allm.size(), env_rgb, env_w, env_h,
+                                       env_scale, &b->scene);
+        if (st != TT_OK) {
+            tt_batch_destroy(b.release());
+            return st;
+        }
+        *out = b.release();
+        return TT_OK;
+    } catch (const std::bad_alloc &) {
+        return fail(TT_ERR_NOMEM, "tt_batch_create: host allocation failed");
     }
-    hipEventRecord(e1, hs);
-    std::vector<float> acc((size_t)n3);
-    std::vector<double> acc64;
-    unsigned long long hstats[2] = {0, 0};
-    hipError_t e = hipMemcpyAsync(acc.data(), rs->acc, (size_t)n3 * 4, hipMemcpyDeviceToHost, hs);
-    if (e == hipSuccess && fp64) {
-        acc64.resize((size_t)n3);
-        e = hipMemcpyAsync(acc64.data(), rs->acc64, (size_t)n3 * 8, hipMemcpyDeviceToHost, hs);
+}
+
+void tt_batch_destroy(tt_batch *b) {
+    if (!b) return;
+    if (b->scene) tt_scene_destroy(b->scene);
+    delete b->mesh;
+    delete b->bvh;
+    delete b;
+}
+
+tt_status tt_batch_render(tt_batch *b, const tt_camera *cams, const tt_render_params *p, float *const *rgb_out,
+                          tt_render_stats *stats) {
+    if (!b || !cams || !p || !rgb_out) return fail(TT_ERR_INVALID, "tt_batch_render: null argument");
+    const int32_t n = (int32_t)b->roots.size();
+    size_t total = 0;
+    for (int32_t k = 0; k < n; ++k) {
+        if (!rgb_out[k]) return fail(TT_ERR_INVALID, "tt_batch_render: null image");
+        total += (size_t)std::max(0, cams[k].width) * (size_t)std::max(0, cams[k].height) * 3;
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(hstats, rs->stats, 16, hipMemcpyDeviceToHost, hs);
-    if (e == hipSuccess) e = hipStreamSynchronize(hs);
-    float ms = 0.0f;
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    if (e != hipSuccess) return fail(TT_ERR_DEVICE, std::string("tt_render: ") + hipGetErrorString(e));
-    if (p->mode == TT_RENDER_REFERENCE) {
-        std::memcpy(rgb_out, acc.data(), (size_t)n3 * 4);
-    } else if (fp64) {
-        for (int64_t k = 0; k < n3; ++k) rgb_out[k] = (float)(acc64[k] / (double)p->spp);
-        if (rgb64_out)
-            for (int64_t k = 0; k < n3; ++k) rgb64_out[k] = acc64[k] / (double)p->spp;
-    } else {
-        const float inv = 1.0f / (float)p->spp;
-        for (int64_t k = 0; k < n3; ++k) rgb_out[k] = acc[k] * inv;
-    }
-    if (stats) {
-        stats->rays_extension = (int64_t)hstats[0];
-        stats->rays_shadow = (int64_t)hstats[1];
-        stats->samples = (int64_t)npix * p->spp;
-        stats->seconds = ms * 1e-3;
-        stats->iterations = iters;
+    std::vector<float> all(total);
+    tt_status st = render_impl(b->scene, n, cams, b->roots.data(), p, all.data(), nullptr, stats);
+    if (st != TT_OK) return st;
+    size_t off = 0;
+    for (int32_t k = 0; k < n; ++k) {
+        size_t cnt = (size_t)cams[k].width * cams[k].height * 3;
+        std::memcpy(rgb_out[k], all.data() + off, cnt * 4);
+        off += cnt;
     }
     return TT_OK;
 }

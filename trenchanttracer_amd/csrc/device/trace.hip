@@ -107,7 +107,7 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
     st.sp = 0;
 
     int leaf_addr = 0;
-    int node_addr = (valid && a.num_nodes > 0) ? 0 : kSentinel;
+    int node_addr = (valid && a.num_nodes > 0) ? (a.ray_root ? a.ray_root[rid] : 0) : kSentinel;
     float hit_t = tmax, hit_u = 0.0f, hit_v = 0.0f;
     int hit_id = -1;
 
