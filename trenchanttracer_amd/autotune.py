@@ -1,0 +1,80 @@
+"""Autotuner for the traversal and the wavefront renderer.
+
+Times short runs of candidate launch configurations on the actual workload and
+keeps the fastest: traversal loop variant x LDS stack entries x block size for
+`tune_trace`; paths in flight x LDS stack x block x hipGraph for `tune_render`.
+Results can be cached in a JSON file keyed by a workload signature."""
+import itertools
+import json
+import os
+
+from . import render_params, trace_cfg
+
+
+def _load(path):
+    if path and os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return {}
+
+
+def _save(path, data):
+    if path:
+        tmp = path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(data, f, indent=1)
+        os.replace(tmp, path)
+
+
+def tune_trace(ctx, scene, d_rays, n, d_hits, variants=(0, 1), lds=(8, 16, 24), blocks=(64, 128), reps=3,
+               exact_only=True, cache=None, key=None):
+    """Fastest trace configuration for this scene and ray buffer. With exact_only
+    the speculative variant is not considered (hit ids stay bit-exact)."""
+    from . import Timer
+    db = _load(cache)
+    if key and key in db:
+        c = db[key]
+        return trace_cfg(c["variant"], c["lds_stack"], c["block"]), c["ms"]
+    timer = Timer(ctx, 2)
+    best = None
+    for v, s, b in itertools.product(variants, lds, blocks):
+        if exact_only and v != 0:
+            continue
+        cfg = trace_cfg(v, s, b)
+        scene.trace_closest(d_rays, n, d_hits, cfg)
+        ms = []
+        for _ in range(reps):
+            timer.record(0)
+            scene.trace_closest(d_rays, n, d_hits, cfg)
+            timer.record(1)
+            ms.append(timer.elapsed(0, 1))
+        t = sorted(ms)[len(ms) // 2]
+        if best is None or t < best[0]:
+            best = (t, v, s, b)
+    timer.close()
+    if key:
+        db[key] = {"variant": best[1], "lds_stack": best[2], "block": best[3], "ms": best[0]}
+        _save(cache, db)
+    return trace_cfg(best[1], best[2], best[3]), best[0]
+
+
+def tune_render(render_fn, base, paths=(0, 4 << 20, 16 << 20), lds=(8, 16), blocks=(64, 128), graphs=(0, 1),
+                probe_spp=2, cache=None, key=None):
+    """Fastest render configuration. render_fn(params) -> stats; base: a
+    tt_render_params-like dict of the real render (spp is replaced by probe_spp)."""
+    db = _load(cache)
+    if key and key in db:
+        return db[key]
+    best = None
+    for pf, s, b, g in itertools.product(paths, lds, blocks, graphs):
+        prm = dict(base)
+        prm.update(spp=probe_spp, paths_in_flight=pf, lds_stack=s, block=b, use_graph=bool(g))
+        render_fn(render_params(**prm))           # warm (allocations, code objects)
+        st = render_fn(render_params(**prm))
+        t = st["seconds"]
+        if best is None or t < best[0]:
+            best = (t, {"paths_in_flight": pf, "lds_stack": s, "block": b, "use_graph": bool(g)})
+    if key:
+        db[key] = best[1]
+        _save(cache, db)
+    return best[1]

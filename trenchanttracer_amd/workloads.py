@@ -124,3 +124,39 @@ def pathtrace(subdiv=8, width=1920, height=1080, env_w=2048, env_h=1024, seed=1)
     cam = camera((0.0, 0.35, 3.3), (0.0, -0.1, -1.0), fov=(40.0, 40.0 * height / width), width=width,
                  height=height, focal=1.0)
     return mesh, mats, env, cam
+
+
+def batched(n=32, seed=1, tri_lo=1_000, tri_hi=2_000_000, resolutions=(256, 512, 1024), log=None):
+    """The batched case: n seeded scenes, alternately displaced icospheres and
+    triangle soups, triangle counts log-uniform in [tri_lo, tri_hi] (icospheres take
+    the subdivision level nearest in log), square resolutions drawn from
+    `resolutions`, path-mode materials, one shared sky. Returns
+    (meshes, materials, cameras, env, tri_counts)."""
+    from . import MAT_CONDUCTOR, MAT_DIELECTRIC, MAT_DIFFUSE, camera, material
+    rng = np.random.default_rng(seed)
+    meshes, mats, cams, counts = [], [], [], []
+    for k in range(n):
+        target = float(np.exp(rng.uniform(np.log(tri_lo), np.log(tri_hi))))
+        res = int(rng.choice(resolutions))
+        if k % 2 == 0:
+            sub = int(np.clip(np.round(np.log(target / 20.0) / np.log(4.0)), 2, 8))
+            m = Mesh.icosphere(sub, 1.0, (0, 0, 0), amp=0.05, octaves=6, seed=seed * 100 + k)
+            mm = [material(MAT_DIFFUSE, (0.8, 0.7, 0.6)),
+                  material(MAT_CONDUCTOR, (0.95, 0.78, 0.4), roughness=float(rng.uniform(0.05, 0.5))),
+                  material(MAT_DIELECTRIC, (1, 1, 1), ior=1.5)]
+            m.assign_materials([0.4, 0.4, 0.2], [0, 1, 2], seed=seed * 100 + k)
+            cam = camera((0.0, 0.35, 3.3), (0.0, -0.1, -1.0), fov=(40.0, 40.0), width=res, height=res)
+        else:
+            m = Mesh.soup(int(target), seed=seed * 100 + k, mu=float(np.log(0.01 * (1e6 / target) ** (1 / 3))),
+                          sigma=0.5)
+            mm = [material(MAT_DIFFUSE, (0.75, 0.75, 0.75)), material(MAT_CONDUCTOR, (0.9, 0.9, 0.9), roughness=0.3)]
+            m.assign_materials([0.7, 0.3], [0, 1], seed=seed * 100 + k)
+            cam = camera((0.5, 0.5, -1.2), (0.0, 0.0, 1.0), fov=(50.0, 50.0), width=res, height=res)
+        meshes.append(m)
+        mats.append(mm)
+        cams.append(cam)
+        counts.append(m.ntris)
+        if log:
+            log(f"scene {k}: {m.ntris} tris, {res}x{res}")
+    env = sky_env(1024, 512)
+    return meshes, mats, cams, env, counts
