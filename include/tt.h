@@ -327,6 +327,26 @@ void tt_batch_destroy(tt_batch *b);
 tt_status tt_batch_render(tt_batch *b, const tt_camera *cams, const tt_render_params *p, float *const *rgb_out,
                           tt_render_stats *stats);
 
+/* Learned path-guiding proposal: MLP (frequency encoding of position / normal /
+ * outgoing direction -> 3 x 64 -> 8-lobe vMF mixture), BF16 MFMA forward and
+ * backward kernels with FP32 accumulation, FP32 master weights, Adam.
+ * inputs: n x 9 floats in [0, 1]; raw outputs: n x 40 floats (lobe k: mean
+ * raw[3k..3k+2], log kappa raw[24+k], logit raw[32+k]). Training rows: direction
+ * (n x 3, unit) and weight (n, radiance / sampling pdf); the loss is
+ * sum_r w_r (-log q(d_r)) / sum_r w_r. Parameters: tt_guide_param_count() floats,
+ * per layer W[out][in] then b[out] (layers 224->64, 64->64, 64->64, 64->48). */
+typedef struct tt_guide tt_guide;
+tt_status tt_guide_create(tt_ctx *ctx, uint32_t seed, float lr, tt_guide **out);
+void tt_guide_destroy(tt_guide *g);
+int64_t tt_guide_param_count(void);
+tt_status tt_guide_get_params(tt_guide *g, float *out);
+tt_status tt_guide_set_params(tt_guide *g, const float *in);
+tt_status tt_guide_forward(tt_guide *g, const float *inputs, int64_t n, float *raw_out);
+tt_status tt_guide_grad(tt_guide *g, const float *inputs, const float *dirs, const float *wts, int64_t n,
+                        float *loss, float *grad_out);
+tt_status tt_guide_train_step(tt_guide *g, const float *inputs, const float *dirs, const float *wts, int64_t n,
+                              float *loss);
+
 /* Reproducibility hook for the render math: applies op (0 sin, 1 cos, 2 pow(x, y)) of
  * the device sampler math to n device values; y is read for op 2. Synchronous. */
 tt_status tt_debug_math(tt_ctx *ctx, int32_t op, const float *x, const float *y, float *out, int64_t n);

@@ -19,7 +19,7 @@ import numpy as np
 from . import _native as N
 from ._native import TTError, check, lib, tt_trace_cfg
 
-__all__ = ["Mesh", "BVH", "Context", "DeviceScene", "DeviceBuffer", "Timer", "Batch", "TTError", "rays_primary", "rays_incoherent",
+__all__ = ["Mesh", "BVH", "Context", "DeviceScene", "DeviceBuffer", "Timer", "Batch", "Guide", "TTError", "rays_primary", "rays_incoherent",
            "trace_cfg", "MAT_DIFFUSE", "MAT_CONDUCTOR", "MAT_DIELECTRIC", "MAT_EMITTER", "material"]
 
 MAT_DIFFUSE, MAT_CONDUCTOR, MAT_DIELECTRIC, MAT_EMITTER = 0, 1, 2, 3
@@ -275,6 +275,59 @@ class Batch:
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             lib.tt_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Guide:
+    """Path-guiding network (tt_guide): BF16 MFMA MLP -> 8-lobe vMF mixture."""
+
+    def __init__(self, ctx, seed=1, lr=1e-3):
+        h = C.c_void_p()
+        check(lib.tt_guide_create(ctx._h, int(seed), float(lr), C.byref(h)))
+        self._h = h
+        self.ctx = ctx
+        self.nparams = int(lib.tt_guide_param_count())
+        ctx._adopt(self)
+
+    def params(self):
+        out = np.empty(self.nparams, np.float32)
+        check(lib.tt_guide_get_params(self._h, _vp(out)))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.nparams
+        check(lib.tt_guide_set_params(self._h, _vp(p)))
+
+    def forward(self, inputs):
+        x = np.ascontiguousarray(inputs, np.float32).reshape(-1, 9)
+        out = np.empty((len(x), 40), np.float32)
+        check(lib.tt_guide_forward(self._h, _vp(x), len(x), _vp(out)))
+        return out
+
+    def grad(self, inputs, dirs, wts):
+        x = np.ascontiguousarray(inputs, np.float32).reshape(-1, 9)
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        w = np.ascontiguousarray(wts, np.float32).reshape(-1)
+        g = np.empty(self.nparams, np.float32)
+        loss = C.c_float()
+        check(lib.tt_guide_grad(self._h, _vp(x), _vp(d), _vp(w), len(x), C.byref(loss), _vp(g)))
+        return loss.value, g
+
+    def train_step(self, inputs, dirs, wts):
+        x = np.ascontiguousarray(inputs, np.float32).reshape(-1, 9)
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        w = np.ascontiguousarray(wts, np.float32).reshape(-1)
+        loss = C.c_float()
+        check(lib.tt_guide_train_step(self._h, _vp(x), _vp(d), _vp(w), len(x), C.byref(loss)))
+        return loss.value
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib.tt_guide_destroy(self._h)
             self._h = None
 
     def __del__(self):

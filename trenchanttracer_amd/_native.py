@@ -121,6 +121,14 @@ PROTOS = {
     "tt_camera_from_orbit": (C.c_int, [PF, F32, F32, F32, F32, I32, I32, F32, F32, C.POINTER(tt_camera)]),
     "tt_render": (C.c_int, [P, C.POINTER(tt_camera), C.POINTER(tt_render_params), P, P, C.POINTER(tt_render_stats)]),
     "tt_debug_math": (C.c_int, [P, I32, P, P, P, I64]),
+    "tt_guide_create": (C.c_int, [P, U32, F32, C.POINTER(P)]),
+    "tt_guide_destroy": (None, [P]),
+    "tt_guide_param_count": (I64, []),
+    "tt_guide_get_params": (C.c_int, [P, P]),
+    "tt_guide_set_params": (C.c_int, [P, P]),
+    "tt_guide_forward": (C.c_int, [P, P, I64, P]),
+    "tt_guide_grad": (C.c_int, [P, P, P, P, I64, PF, P]),
+    "tt_guide_train_step": (C.c_int, [P, P, P, P, I64, PF]),
     "tt_batch_create": (C.c_int, [P, I32, P, P, P, P, P, I32, I32, F32, C.POINTER(P)]),
     "tt_batch_destroy": (None, [P]),
     "tt_batch_render": (C.c_int, [P, P, C.POINTER(tt_render_params), P, C.POINTER(tt_render_stats)]),

@@ -12,7 +12,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtt.so")
 
 HOST_SRC = ["host/sbvh.cpp", "host/flatten.cpp", "host/gen.cpp", "host/api_host.cpp"]
-DEV_SRC = ["device/trace.hip", "device/api_dev.hip", "device/env.hip", "device/render.hip"]
+DEV_SRC = ["device/trace.hip", "device/api_dev.hip", "device/env.hip", "device/render.hip", "device/guide.hip"]
 
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")

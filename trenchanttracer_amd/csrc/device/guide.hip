@@ -1,0 +1,622 @@
+// Learned path-guiding proposal: a small MLP trained online on the tracer's own
+// radiance samples (no reference counterpart).
+//
+// Network: 9 inputs (position normalised to the scene box, shading normal and
+// outgoing direction mapped to [0, 1]) -> frequency encoding sin / cos(2^f pi x),
+// f = 0..11 (216 features, padded to 224) -> 3 hidden layers of 64 (ReLU) -> 40
+// outputs (padded to 48): an 8-lobe von Mises-Fisher mixture, per lobe a raw mean
+// direction (3), log concentration (1) and logit weight (1).
+// BF16 weights and activations on MFMA (v_mfma_f32_16x16x32_bf16) with FP32
+// accumulation; FP32 master weights and Adam moments.
+//
+// Operand layout of v_mfma_f32_16x16x32_bf16 on wave64 (lane l):
+//   A: A[l % 16][8 (l / 16) + j], B: B[8 (l / 16) + j][l % 16], j = 0..7
+//   D: D[4 (l / 16) + j][l % 16], j = 0..3
+// Matrices are kept so that each lane's 8 operand values are 16 contiguous bytes:
+// activations row-major [rows][K], weights as Bt[out][in] for the forward pass
+// and as W[in][out] for the backward pass (both copies refreshed by Adam).
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../common/philox.h"
+#include "../host/tt_host.h"
+#include "guide_dev.h"
+#include "scene_dev.h"
+
+namespace tt {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// ------------------------------------------------------------------ encoding
+__global__ __launch_bounds__(256) void k_encode(const float *__restrict__ in, __bf16 *__restrict__ x, int64_t n,
+                                                const int32_t *n_dev) {
+    int64_t rows = n_dev ? min((int64_t)*n_dev, n) : n;
+    const int64_t rows_pad = (rows + 15) & ~15LL;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < rows_pad * kGuideK1;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = t / kGuideK1;
+        const int k = (int)(t % kGuideK1);
+        float v = 0.0f;
+        if (r < rows && k < kGuideFeat) {
+            const int c = k / (2 * kGuideFreq), f = (k / 2) % kGuideFreq;
+            const float arg = in[r * kGuideIn + c] * (3.14159265358979f * (float)(1 << f));
+            v = (k & 1) ? cosf(arg) : sinf(arg);
+        }
+        x[t] = (__bf16)v;
+    }
+}
+
+// ------------------------------------------------------------------ GEMM
+// out[r][c] = act(sum_k A[r][k] Bt[c][k] + bias[c]), rows r < n (device count if
+// n_dev), K % 32 == 0, NC % 16 == 0 (<= 64). One wave per 16-row tile.
+// mask (optional, bf16 [rows][NC]): out *= (mask > 0) (ReLU derivative).
+template <int K, int NC, bool RELU, bool OUT_BF16>
+__global__ __launch_bounds__(256) void k_gemm(const __bf16 *__restrict__ A, const __bf16 *__restrict__ Bt,
+                                              const float *__restrict__ bias, void *__restrict__ out, int ldo,
+                                              const __bf16 *__restrict__ mask, int64_t n, const int32_t *n_dev) {
+    constexpr int T = NC / 16;
+    const int lane = threadIdx.x & 63;
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    int64_t rows = n_dev ? min((int64_t)*n_dev, n) : n;
+    const int64_t tiles = (rows + 15) >> 4;
+    const int lr = lane & 15, lq = lane >> 4;
+    for (int64_t tile = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); tile < tiles; tile += waves) {
+        f32x4 acc[T];
+#pragma unroll
+        for (int t = 0; t < T; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+        const __bf16 *arow = A + (tile * 16 + lr) * K + 8 * lq;
+#pragma unroll 2
+        for (int s = 0; s < K / 32; ++s) {
+            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(arow + 32 * s);
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+                const bf16x8 b = *reinterpret_cast<const bf16x8 *>(Bt + (int64_t)(16 * t + lr) * K + 32 * s + 8 * lq);
+                acc[t] = mfma(a, b, acc[t]);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            const int col = 16 * t + lr;
+            const float bcol = bias ? bias[col] : 0.0f;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int64_t row = tile * 16 + 4 * lq + j;
+                float v = acc[t][j] + bcol;
+                if (RELU) v = fmaxf(v, 0.0f);
+                if (mask && !((float)mask[row * NC + col] > 0.0f)) v = 0.0f;
+                if (OUT_BF16) reinterpret_cast<__bf16 *>(out)[row * ldo + col] = (__bf16)v;
+                else reinterpret_cast<float *>(out)[row * ldo + col] = v;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ vMF mixture
+struct Lobes {
+    float mu[kGuideLobes][3];
+    float kappa[kGuideLobes];
+    float w[kGuideLobes];
+    float inv_len[kGuideLobes];
+};
+
+__device__ __forceinline__ void decode_lobes(const float *raw, Lobes &L) {
+    float mx = -1e30f;
+    for (int k = 0; k < kGuideLobes; ++k) mx = fmaxf(mx, raw[kGuideLobes * 4 + k]);
+    float sum = 0.0f;
+    for (int k = 0; k < kGuideLobes; ++k) {
+        float m0 = raw[3 * k], m1 = raw[3 * k + 1], m2 = raw[3 * k + 2];
+        float len = sqrtf(m0 * m0 + m1 * m1 + m2 * m2) + 1e-6f;
+        L.inv_len[k] = 1.0f / len;
+        L.mu[k][0] = m0 * L.inv_len[k];
+        L.mu[k][1] = m1 * L.inv_len[k];
+        L.mu[k][2] = m2 * L.inv_len[k];
+        L.kappa[k] = expf(fminf(fmaxf(raw[kGuideLobes * 3 + k], kGuideLogKappaMin), kGuideLogKappaMax));
+        L.w[k] = expf(raw[kGuideLobes * 4 + k] - mx);
+        sum += L.w[k];
+    }
+    for (int k = 0; k < kGuideLobes; ++k) L.w[k] /= sum;
+}
+
+// log of the vMF density kappa / (2 pi (1 - e^-2k)) exp(k (mu.w - 1))
+__device__ __forceinline__ float vmf_log(float kappa, float cosang) {
+    return logf(kappa) - 1.83787706641f - log1pf(-expf(-2.0f * kappa)) + kappa * (cosang - 1.0f);
+}
+
+// loss = sum_r s_r * (-log q(w_r)) / sum_r s_r; writes d loss / d raw (bf16, 48 wide + pad to 64)
+__global__ __launch_bounds__(256) void k_loss_grad(const float *__restrict__ raw, const float *__restrict__ dirs,
+                                                   const float *__restrict__ wts, __bf16 *__restrict__ draw,
+                                                   float *__restrict__ loss_out, const float *__restrict__ wsum,
+                                                   int64_t n) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ float red[256];
+    float contrib = 0.0f;
+    const int64_t rows_pad = (n + 15) & ~15LL;
+    if (r < rows_pad) {
+        __bf16 *g = draw + r * kGuideNOutPad;
+        for (int c = 0; c < kGuideNOutPad; ++c) g[c] = (__bf16)0.0f;
+    }
+    if (r < n) {
+        const float *rr = raw + r * kGuideNOutPad16;
+        Lobes L;
+        decode_lobes(rr, L);
+        const float wx = dirs[3 * r], wy = dirs[3 * r + 1], wz = dirs[3 * r + 2];
+        const float scale = wts[r] / fmaxf(*wsum, 1e-30f);
+        float lv[kGuideLobes], mx = -1e30f;
+        for (int k = 0; k < kGuideLobes; ++k) {
+            float ca = L.mu[k][0] * wx + L.mu[k][1] * wy + L.mu[k][2] * wz;
+            lv[k] = logf(L.w[k]) + vmf_log(L.kappa[k], ca);
+            mx = fmaxf(mx, lv[k]);
+        }
+        float s = 0.0f;
+        for (int k = 0; k < kGuideLobes; ++k) s += expf(lv[k] - mx);
+        const float logq = mx + logf(s);
+        contrib = scale * (-logq);
+        __bf16 *g = draw + r * kGuideNOutPad;
+        const float lk_min = kGuideLogKappaMin, lk_max = kGuideLogKappaMax;
+        for (int k = 0; k < kGuideLobes; ++k) {
+            const float resp = expf(lv[k] - logq);
+            // logits: d(-log q)/d logit_k = w_k - resp_k
+            g[kGuideLobes * 4 + k] = (__bf16)(scale * (L.w[k] - resp));
+            // log kappa: d log v / d kappa = 1/k - 2/(e^{2k} - 1) + (mu.w - 1); d kappa / d lk = kappa
+            const float kap = L.kappa[k];
+            const float ca = L.mu[k][0] * wx + L.mu[k][1] * wy + L.mu[k][2] * wz;
+            const float dlv = 1.0f / kap - 2.0f / expm1f(2.0f * kap) + (ca - 1.0f);
+            const float lk = rr[kGuideLobes * 3 + k];
+            const float gk = (lk > lk_min && lk < lk_max) ? -scale * resp * dlv * kap : 0.0f;
+            g[kGuideLobes * 3 + k] = (__bf16)gk;
+            // raw mean m: d log v / d m = kappa (w - (mu.w) mu) / |m|
+            const float f = -scale * resp * kap * L.inv_len[k];
+            g[3 * k] = (__bf16)(f * (wx - ca * L.mu[k][0]));
+            g[3 * k + 1] = (__bf16)(f * (wy - ca * L.mu[k][1]));
+            g[3 * k + 2] = (__bf16)(f * (wz - ca * L.mu[k][2]));
+        }
+    }
+    red[threadIdx.x] = contrib;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(loss_out, red[0]);
+}
+
+__global__ __launch_bounds__(256) void k_sum(const float *__restrict__ v, int64_t n, float *__restrict__ out) {
+    __shared__ float red[256];
+    float s = 0.0f;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        s += v[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(out, red[0]);
+}
+
+// ------------------------------------------------------------------ weight gradients
+// partial[blk][i][o] = sum over the block's rows n of H[n][i] G[n][o]  (KI x KO)
+// Rows are staged 32 at a time in LDS, transposed so that each lane's 8 values
+// along the row dimension are contiguous.
+template <int KI, int KO>
+__global__ __launch_bounds__(256) void k_wgrad(const __bf16 *__restrict__ H, int ldh, const __bf16 *__restrict__ G,
+                                               int ldg, int64_t n, int64_t rows_per_block, float *__restrict__ partial) {
+    constexpr int TI = KI / 16, TO = KO / 16, TILES = TI * TO;
+    constexpr int PER_WAVE = (TILES + 3) / 4;
+    __shared__ __align__(16) __bf16 hs[KI][40];  // [feature][row], 32 rows + pad
+    __shared__ __align__(16) __bf16 gs[KO][40];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lr = lane & 15, lq = lane >> 4;
+    const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
+    const int64_t r1 = min(n, r0 + rows_per_block);
+    f32x4 acc[PER_WAVE];
+#pragma unroll
+    for (int q = 0; q < PER_WAVE; ++q) acc[q] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int64_t rb = r0; rb < r1; rb += 32) {
+        __syncthreads();
+        for (int t = threadIdx.x; t < 32 * KI; t += blockDim.x) {
+            const int rr = t / KI, i = t % KI;
+            hs[i][rr] = (rb + rr < r1) ? H[(rb + rr) * ldh + i] : (__bf16)0.0f;
+        }
+        for (int t = threadIdx.x; t < 32 * KO; t += blockDim.x) {
+            const int rr = t / KO, o = t % KO;
+            gs[o][rr] = (rb + rr < r1) ? G[(rb + rr) * ldg + o] : (__bf16)0.0f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < PER_WAVE; ++q) {
+            const int tile = wave * PER_WAVE + q;
+            if (tile < TILES) {
+                const int ti = tile / TO, to = tile % TO;
+                const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&hs[16 * ti + lr][8 * lq]);
+                const bf16x8 b = *reinterpret_cast<const bf16x8 *>(&gs[16 * to + lr][8 * lq]);
+                acc[q] = mfma(a, b, acc[q]);
+            }
+        }
+    }
+    float *out = partial + (int64_t)blockIdx.x * KI * KO;
+#pragma unroll
+    for (int q = 0; q < PER_WAVE; ++q) {
+        const int tile = wave * PER_WAVE + q;
+        if (tile < TILES) {
+            const int ti = tile / TO, to = tile % TO;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) out[(16 * ti + 4 * lq + j) * KO + 16 * to + lr] = acc[q][j];
+        }
+    }
+}
+
+// partial[blk][o] = sum over the block's rows of G[n][o] (bias gradients)
+__global__ __launch_bounds__(256) void k_colsum(const __bf16 *__restrict__ G, int ldg, int ko, int64_t n,
+                                                int64_t rows_per_block, float *__restrict__ partial) {
+    const int64_t r0 = (int64_t)blockIdx.x * rows_per_block, r1 = min(n, r0 + rows_per_block);
+    for (int o = threadIdx.x; o < ko; o += blockDim.x) {
+        float s = 0.0f;
+        for (int64_t r = r0; r < r1; ++r) s += (float)G[r * ldg + o];
+        partial[(int64_t)blockIdx.x * ko + o] = s;
+    }
+}
+
+// grad[offset + e] = sum_b partial[b][e] for e < count (deterministic order)
+__global__ __launch_bounds__(256) void k_reduce(const float *__restrict__ partial, int nb, int count,
+                                                float *__restrict__ grad, int offset, bool transpose_ko,
+                                                int ki, int ko) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= count) return;
+    float s = 0.0f;
+    for (int b = 0; b < nb; ++b) s += partial[(int64_t)b * count + e];
+    // partials are [in][out]; master weights are stored [out][in]
+    int dst = e;
+    if (transpose_ko) {
+        const int i = e / ko, o = e % ko;
+        dst = o * ki + i;
+    }
+    grad[offset + dst] = s;
+}
+
+// ------------------------------------------------------------------ Adam + BF16 copies
+__global__ __launch_bounds__(256) void k_adam(float *__restrict__ p, float *__restrict__ m, float *__restrict__ v,
+                                              const float *__restrict__ g, int n, float lr, float b1, float b2,
+                                              float eps, float c1, float c2) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const float gg = g[e];
+    const float mm = b1 * m[e] + (1.0f - b1) * gg;
+    const float vv = b2 * v[e] + (1.0f - b2) * gg * gg;
+    m[e] = mm;
+    v[e] = vv;
+    p[e] -= lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+}
+
+// Master layout per layer: W[out][in] then b[out]. Forward copy Bt = W[out][in]
+// (rows padded to the padded out width), backward copy W[in][out_pad].
+__global__ __launch_bounds__(256) void k_to_bf16(const float *__restrict__ p, __bf16 *__restrict__ fwd,
+                                                 __bf16 *__restrict__ bwd, int in, int out, int out_pad_bwd) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= in * out) return;
+    const int o = e / in, i = e % in;
+    const __bf16 b = (__bf16)p[e];
+    fwd[e] = b;
+    bwd[i * out_pad_bwd + o] = b;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ host side
+const GuideLayer kGuideLayers[4] = {
+    {kGuideK1, kGuideH}, {kGuideH, kGuideH}, {kGuideH, kGuideH}, {kGuideH, kGuideNOutPad16}};
+
+int64_t guide_param_count() {
+    int64_t n = 0;
+    for (const GuideLayer &l : kGuideLayers) n += (int64_t)l.in * l.out + l.out;
+    return n;
+}
+
+static int64_t layer_offset(int k) {
+    int64_t n = 0;
+    for (int q = 0; q < k; ++q) n += (int64_t)kGuideLayers[q].in * kGuideLayers[q].out + kGuideLayers[q].out;
+    return n;
+}
+
+tt_status guide_alloc_batch(tt_guide *g, int64_t rows) {
+    if (g->cap >= rows) return TT_OK;
+    GuideBatch &b = g->batch;
+    void *ptrs[] = {b.in, b.x, b.h[0], b.h[1], b.h[2], b.raw, b.draw, b.dh[0], b.dh[1], b.dh[2], b.dirs, b.wts};
+    for (void *q : ptrs)
+        if (q) hipFree(q);
+    b = GuideBatch();
+    g->cap = 0;
+    const int64_t R = (rows + 15) & ~15LL;
+    tt_status st = TT_OK;
+    auto al = [&](void **p, size_t bytes, const char *w) {
+        if (st == TT_OK) st = dev_alloc(g->ctx, p, bytes, w);
+    };
+    al((void **)&b.in, (size_t)R * kGuideIn * 4, "guide inputs");
+    al((void **)&b.x, (size_t)R * kGuideK1 * 2, "guide features");
+    for (int k = 0; k < 3; ++k) al((void **)&b.h[k], (size_t)R * kGuideH * 2, "guide activations");
+    al((void **)&b.raw, (size_t)R * kGuideNOutPad16 * 4, "guide outputs");
+    al((void **)&b.draw, (size_t)R * kGuideNOutPad * 2, "guide output gradients");
+    for (int k = 0; k < 3; ++k) al((void **)&b.dh[k], (size_t)R * kGuideH * 2, "guide activation gradients");
+    al((void **)&b.dirs, (size_t)R * 3 * 4, "guide directions");
+    al((void **)&b.wts, (size_t)R * 4, "guide weights");
+    if (st == TT_OK) g->cap = R;
+    return st;
+}
+
+static hipError_t refresh_bf16(tt_guide *g, hipStream_t s) {
+    for (int k = 0; k < 4; ++k) {
+        const GuideLayer &l = kGuideLayers[k];
+        const int n = l.in * l.out;
+        const int out_pad_bwd = k == 3 ? kGuideNOutPad : l.out;
+        hipLaunchKernelGGL(k_to_bf16, dim3((n + 255) / 256), dim3(256), 0, s, g->params + layer_offset(k), g->wf[k],
+                           g->wb[k], l.in, l.out, out_pad_bwd);
+    }
+    return hipGetLastError();
+}
+
+// Forward over the first n rows of g->batch.in (device count n_dev optional):
+// features, 3 hidden layers, raw outputs (fp32, 48 wide).
+hipError_t guide_forward(tt_guide *g, int64_t n, const int32_t *n_dev, hipStream_t s) {
+    GuideBatch &b = g->batch;
+    const int64_t R = (n + 15) & ~15LL;
+    const unsigned gb = (unsigned)std::min<int64_t>((R * kGuideK1 + 255) / 256, 65535);
+    hipLaunchKernelGGL(k_encode, dim3(gb), dim3(256), 0, s, b.in, b.x, n, n_dev);
+    const unsigned gw = (unsigned)std::min<int64_t>(((R / 16) + 3) / 4 + 1, 8192);
+    const float *p = g->params;
+    hipLaunchKernelGGL((k_gemm<kGuideK1, kGuideH, true, true>), dim3(gw), dim3(256), 0, s, b.x, g->wf[0],
+                       p + layer_offset(0) + kGuideK1 * kGuideH, (void *)b.h[0], kGuideH, (const __bf16 *)nullptr, n,
+                       n_dev);
+    hipLaunchKernelGGL((k_gemm<kGuideH, kGuideH, true, true>), dim3(gw), dim3(256), 0, s, b.h[0], g->wf[1],
+                       p + layer_offset(1) + kGuideH * kGuideH, (void *)b.h[1], kGuideH, (const __bf16 *)nullptr, n,
+                       n_dev);
+    hipLaunchKernelGGL((k_gemm<kGuideH, kGuideH, true, true>), dim3(gw), dim3(256), 0, s, b.h[1], g->wf[2],
+                       p + layer_offset(2) + kGuideH * kGuideH, (void *)b.h[2], kGuideH, (const __bf16 *)nullptr, n,
+                       n_dev);
+    hipLaunchKernelGGL((k_gemm<kGuideH, kGuideNOutPad16, false, false>), dim3(gw), dim3(256), 0, s, b.h[2], g->wf[3],
+                       p + layer_offset(3) + kGuideH * kGuideNOutPad16, (void *)b.raw, kGuideNOutPad16,
+                       (const __bf16 *)nullptr, n, n_dev);
+    return hipGetLastError();
+}
+
+// Gradient of the loss over rows [0, n) of the batch (inputs, dirs, wts staged).
+static hipError_t guide_backward(tt_guide *g, int64_t n, hipStream_t s) {
+    GuideBatch &b = g->batch;
+    const int64_t R = (n + 15) & ~15LL;
+    hipError_t e = guide_forward(g, n, nullptr, s);
+    if (e != hipSuccess) return e;
+    hipMemsetAsync(g->scalars, 0, 16, s);
+    hipLaunchKernelGGL(k_sum, dim3(256), dim3(256), 0, s, b.wts, n, g->scalars + 1);
+    hipLaunchKernelGGL(k_loss_grad, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, s, b.raw, b.dirs, b.wts, b.draw,
+                       g->scalars, (const float *)(g->scalars + 1), n);
+    const unsigned gw = (unsigned)std::min<int64_t>(((R / 16) + 3) / 4 + 1, 8192);
+    // dH3 = dRaw W4 (mask H3 > 0), dH2 = dH3 W3 (mask), dH1 = dH2 W2 (mask)
+    hipLaunchKernelGGL((k_gemm<kGuideNOutPad, kGuideH, false, true>), dim3(gw), dim3(256), 0, s, b.draw, g->wb[3],
+                       (const float *)nullptr, (void *)b.dh[2], kGuideH, (const __bf16 *)b.h[2], R, (const int32_t *)nullptr);
+    hipLaunchKernelGGL((k_gemm<kGuideH, kGuideH, false, true>), dim3(gw), dim3(256), 0, s, b.dh[2], g->wb[2],
+                       (const float *)nullptr, (void *)b.dh[1], kGuideH, (const __bf16 *)b.h[1], R, (const int32_t *)nullptr);
+    hipLaunchKernelGGL((k_gemm<kGuideH, kGuideH, false, true>), dim3(gw), dim3(256), 0, s, b.dh[1], g->wb[1],
+                       (const float *)nullptr, (void *)b.dh[0], kGuideH, (const __bf16 *)b.h[0], R, (const int32_t *)nullptr);
+    // weight gradients: dW_k = H_{k-1}^T dH_k, partial sums per block, reduced in order
+    const int64_t rpb = 2048;
+    const int nb = (int)((R + rpb - 1) / rpb);
+    if (g->partial_cap < (int64_t)nb * kGuideK1 * kGuideH) {
+        if (g->partial) hipFree(g->partial);
+        g->partial = nullptr;
+        g->partial_cap = 0;
+        if (dev_alloc(g->ctx, (void **)&g->partial, (size_t)nb * kGuideK1 * kGuideH * 4, "guide partials") != TT_OK)
+            return hipErrorOutOfMemory;
+        g->partial_cap = (int64_t)nb * kGuideK1 * kGuideH;
+    }
+    struct L {
+        const __bf16 *h;
+        int ldh;
+        const __bf16 *d;
+        int ldd;
+        int ki, ko;
+    } ls[4] = {{b.x, kGuideK1, b.dh[0], kGuideH, kGuideK1, kGuideH},
+               {b.h[0], kGuideH, b.dh[1], kGuideH, kGuideH, kGuideH},
+               {b.h[1], kGuideH, b.dh[2], kGuideH, kGuideH, kGuideH},
+               {b.h[2], kGuideH, b.draw, kGuideNOutPad, kGuideH, kGuideNOutPad16}};
+    for (int k = 0; k < 4; ++k) {
+        const L &l = ls[k];
+        if (k == 0)
+            hipLaunchKernelGGL((k_wgrad<kGuideK1, kGuideH>), dim3(nb), dim3(256), 0, s, l.h, l.ldh, l.d, l.ldd, R, rpb,
+                               g->partial);
+        else if (k < 3)
+            hipLaunchKernelGGL((k_wgrad<kGuideH, kGuideH>), dim3(nb), dim3(256), 0, s, l.h, l.ldh, l.d, l.ldd, R, rpb,
+                               g->partial);
+        else
+            hipLaunchKernelGGL((k_wgrad<kGuideH, kGuideNOutPad16>), dim3(nb), dim3(256), 0, s, l.h, l.ldh, l.d, l.ldd,
+                               R, rpb, g->partial);
+        const int cnt = l.ki * l.ko;
+        hipLaunchKernelGGL(k_reduce, dim3((cnt + 255) / 256), dim3(256), 0, s, g->partial, nb, cnt, g->grad,
+                           (int)layer_offset(k), true, l.ki, l.ko);
+        hipLaunchKernelGGL(k_colsum, dim3(nb), dim3(64), 0, s, l.d, l.ldd, l.ko, R, rpb, g->partial);
+        hipLaunchKernelGGL(k_reduce, dim3((l.ko + 255) / 256), dim3(256), 0, s, g->partial, nb, l.ko, g->grad,
+                           (int)(layer_offset(k) + cnt), false, l.ki, l.ko);
+    }
+    return hipGetLastError();
+}
+
+static hipError_t guide_adam(tt_guide *g, hipStream_t s) {
+    g->step++;
+    const float c1 = 1.0f - std::pow(g->beta1, (float)g->step), c2 = 1.0f - std::pow(g->beta2, (float)g->step);
+    const int n = (int)guide_param_count();
+    hipLaunchKernelGGL(k_adam, dim3((n + 255) / 256), dim3(256), 0, s, g->params, g->m1, g->m2, g->grad, n, g->lr,
+                       g->beta1, g->beta2, g->eps, c1, c2);
+    return refresh_bf16(g, s);
+}
+
+hipError_t guide_train_rows(tt_guide *g, int64_t n, hipStream_t s) {
+    hipError_t e = guide_backward(g, n, s);
+    if (e == hipSuccess) e = guide_adam(g, s);
+    return e;
+}
+
+}  // namespace tt
+
+using namespace tt;
+
+#define TT_HIPG(call, what)                                                                       \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess) return fail(TT_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+extern "C" {
+
+tt_status tt_guide_create(tt_ctx *ctx, uint32_t seed, float lr, tt_guide **out) {
+    if (!ctx || !out || !(lr > 0.0f)) return fail(TT_ERR_INVALID, "tt_guide_create: bad arguments");
+    TT_HIPG(hipSetDevice(ctx->device), "hipSetDevice");
+    auto *g = new (std::nothrow) tt_guide();
+    if (!g) return fail(TT_ERR_NOMEM, "tt_guide_create: host allocation failed");
+    g->ctx = ctx;
+    g->lr = lr;
+    const int64_t np = guide_param_count();
+    tt_status st = TT_OK;
+    auto al = [&](void **p, size_t bytes, const char *w) {
+        if (st == TT_OK) st = dev_alloc(ctx, p, bytes, w);
+    };
+    al((void **)&g->params, (size_t)np * 4, "guide params");
+    al((void **)&g->m1, (size_t)np * 4, "guide adam m");
+    al((void **)&g->m2, (size_t)np * 4, "guide adam v");
+    al((void **)&g->grad, (size_t)np * 4, "guide grad");
+    al((void **)&g->scalars, 16, "guide scalars");
+    for (int k = 0; k < 4; ++k) {
+        const GuideLayer &l = kGuideLayers[k];
+        al((void **)&g->wf[k], (size_t)l.in * l.out * 2, "guide bf16 weights");
+        const int out_pad_bwd = k == 3 ? kGuideNOutPad : l.out;
+        al((void **)&g->wb[k], (size_t)l.in * out_pad_bwd * 2, "guide bf16 weights (transposed)");
+    }
+    if (st != TT_OK) {
+        tt_guide_destroy(g);
+        return st;
+    }
+    // He-normal initialisation (Box-Muller over Philox), zero biases; the output
+    // layer is scaled down so the initial mixture is broad and evenly weighted.
+    std::vector<float> p((size_t)np, 0.0f);
+    uint32_t ctr = 0;
+    for (int k = 0; k < 4; ++k) {
+        const GuideLayer &l = kGuideLayers[k];
+        const double sd = (k == 3 ? 0.1 : 1.0) * std::sqrt(2.0 / l.in);
+        float *w = p.data() + layer_offset(k);
+        for (int e = 0; e < l.in * l.out; ++e) {
+            u32x4 r = philox4x32_10({ctr++, 0x6A1D, 0, 0}, seed, 0x7u);
+            double u1 = 1.0 - u01d(r.x, r.y), u2 = u01d(r.z, r.w);
+            double z = std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+            const int o = e / l.in;
+            w[e] = (k == 3 && o >= kGuideNOut) ? 0.0f : (float)(sd * z);
+        }
+    }
+    hipError_t e = hipMemcpy(g->params, p.data(), (size_t)np * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(g->m1, 0, (size_t)np * 4);
+    if (e == hipSuccess) e = hipMemset(g->m2, 0, (size_t)np * 4);
+    for (int k = 0; k < 4 && e == hipSuccess; ++k) {
+        const GuideLayer &l = kGuideLayers[k];
+        const int out_pad_bwd = k == 3 ? kGuideNOutPad : l.out;
+        e = hipMemset(g->wb[k], 0, (size_t)l.in * out_pad_bwd * 2);
+    }
+    if (e == hipSuccess) e = refresh_bf16(g, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        tt_guide_destroy(g);
+        return fail(TT_ERR_DEVICE, std::string("tt_guide_create: ") + hipGetErrorString(e));
+    }
+    *out = g;
+    return TT_OK;
+}
+
+void tt_guide_destroy(tt_guide *g) {
+    if (!g) return;
+    hipSetDevice(g->ctx->device);
+    hipStreamSynchronize(g->ctx->stream);
+    GuideBatch &b = g->batch;
+    void *ptrs[] = {g->params, g->m1, g->m2, g->grad, g->scalars, g->partial, b.in, b.x, b.h[0], b.h[1], b.h[2],
+                    b.raw, b.draw, b.dh[0], b.dh[1], b.dh[2], b.dirs, b.wts, g->wf[0], g->wf[1], g->wf[2], g->wf[3],
+                    g->wb[0], g->wb[1], g->wb[2], g->wb[3], g->train_buf, g->train_count, g->rec};
+    for (void *q : ptrs)
+        if (q) hipFree(q);
+    delete g;
+}
+
+int64_t tt_guide_param_count(void) { return guide_param_count(); }
+
+tt_status tt_guide_get_params(tt_guide *g, float *out) {
+    if (!g || !out) return fail(TT_ERR_INVALID, "tt_guide_get_params: bad arguments");
+    TT_HIPG(hipMemcpy(out, g->params, (size_t)guide_param_count() * 4, hipMemcpyDeviceToHost), "copy params");
+    return TT_OK;
+}
+
+tt_status tt_guide_set_params(tt_guide *g, const float *in) {
+    if (!g || !in) return fail(TT_ERR_INVALID, "tt_guide_set_params: bad arguments");
+    TT_HIPG(hipSetDevice(g->ctx->device), "hipSetDevice");
+    TT_HIPG(hipMemcpy(g->params, in, (size_t)guide_param_count() * 4, hipMemcpyHostToDevice), "copy params");
+    TT_HIPG(refresh_bf16(g, g->ctx->stream), "bf16 refresh");
+    TT_HIPG(hipStreamSynchronize(g->ctx->stream), "sync");
+    return TT_OK;
+}
+
+tt_status tt_guide_forward(tt_guide *g, const float *inputs, int64_t n, float *raw_out) {
+    if (!g || !inputs || !raw_out || n <= 0 || n > (1LL << 24)) return fail(TT_ERR_INVALID, "tt_guide_forward: bad arguments");
+    TT_HIPG(hipSetDevice(g->ctx->device), "hipSetDevice");
+    tt_status st = guide_alloc_batch(g, n);
+    if (st != TT_OK) return st;
+    hipStream_t s = g->ctx->stream;
+    TT_HIPG(hipMemcpyAsync(g->batch.in, inputs, (size_t)n * kGuideIn * 4, hipMemcpyHostToDevice, s), "copy inputs");
+    TT_HIPG(guide_forward(g, n, nullptr, s), "guide forward");
+    TT_HIPG(hipMemcpy2DAsync(raw_out, kGuideNOut * 4, g->batch.raw, kGuideNOutPad16 * 4, kGuideNOut * 4, (size_t)n,
+                             hipMemcpyDeviceToHost, s), "copy out");
+    TT_HIPG(hipStreamSynchronize(s), "sync");
+    return TT_OK;
+}
+
+static tt_status stage_rows(tt_guide *g, const float *inputs, const float *dirs, const float *wts, int64_t n) {
+    if (!g || !inputs || !dirs || !wts || n <= 0 || n > (1LL << 24)) return fail(TT_ERR_INVALID, "guide: bad arguments");
+    if (hipSetDevice(g->ctx->device) != hipSuccess) return fail(TT_ERR_DEVICE, "hipSetDevice failed");
+    tt_status st = guide_alloc_batch(g, n);
+    if (st != TT_OK) return st;
+    hipStream_t s = g->ctx->stream;
+    hipError_t e = hipMemcpyAsync(g->batch.in, inputs, (size_t)n * kGuideIn * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(g->batch.dirs, dirs, (size_t)n * 12, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(g->batch.wts, wts, (size_t)n * 4, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return fail(TT_ERR_DEVICE, std::string("guide staging: ") + hipGetErrorString(e));
+    return TT_OK;
+}
+
+tt_status tt_guide_grad(tt_guide *g, const float *inputs, const float *dirs, const float *wts, int64_t n, float *loss,
+                        float *grad_out) {
+    tt_status st = stage_rows(g, inputs, dirs, wts, n);
+    if (st != TT_OK) return st;
+    hipStream_t s = g->ctx->stream;
+    TT_HIPG(guide_backward(g, n, s), "guide backward");
+    float sc[2];
+    TT_HIPG(hipMemcpyAsync(sc, g->scalars, 8, hipMemcpyDeviceToHost, s), "copy loss");
+    if (grad_out)
+        TT_HIPG(hipMemcpyAsync(grad_out, g->grad, (size_t)guide_param_count() * 4, hipMemcpyDeviceToHost, s), "copy grad");
+    TT_HIPG(hipStreamSynchronize(s), "sync");
+    if (loss) *loss = sc[0];
+    return TT_OK;
+}
+
+tt_status tt_guide_train_step(tt_guide *g, const float *inputs, const float *dirs, const float *wts, int64_t n,
+                              float *loss) {
+    tt_status st = stage_rows(g, inputs, dirs, wts, n);
+    if (st != TT_OK) return st;
+    hipStream_t s = g->ctx->stream;
+    TT_HIPG(guide_train_rows(g, n, s), "guide train step");
+    float sc[2];
+    TT_HIPG(hipMemcpyAsync(sc, g->scalars, 8, hipMemcpyDeviceToHost, s), "copy loss");
+    TT_HIPG(hipStreamSynchronize(s), "sync");
+    if (loss) *loss = sc[0];
+    return TT_OK;
+}
+
+}  // extern "C"
