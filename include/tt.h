@@ -347,6 +347,37 @@ tt_status tt_guide_grad(tt_guide *g, const float *inputs, const float *dirs, con
 tt_status tt_guide_train_step(tt_guide *g, const float *inputs, const float *dirs, const float *wts, int64_t n,
                               float *loss);
 
+/* Guided path tracing (path mode only; no counterpart in the reference, whose
+ * renderer samples BSDFs only, RenderKernel.cu:40-60). On diffuse vertices and
+ * conductors with roughness >= min_roughness, the direction is drawn from the
+ * network's mixture with probability alpha and from the BSDF otherwise, weighted
+ * by the combined density alpha q + (1 - alpha) p_bsdf (one-sample MIS, so the
+ * image is unbiased for any network state); next-event MIS uses the same density.
+ * With train set, train_paths paths of every pass record their vertices (inputs,
+ * sampled direction, density, the radiance later found along the path) and the
+ * network takes Adam steps on batches of train_batch rows between passes, until
+ * train_spp samples per pixel are done (0: the whole render). */
+typedef struct tt_guide_params {
+    float alpha;          /* probability of sampling the network on guided vertices */
+    float min_roughness;  /* conductors at least this rough are guided */
+    int32_t train;        /* 1: train online between passes */
+    int32_t train_paths;  /* paths recorded per pass (0: 65536) */
+    int32_t train_batch;  /* rows per Adam step (0: 262144, at most 4194304) */
+    int32_t train_spp;    /* samples per pixel after which training stops (0: never) */
+} tt_guide_params;
+
+typedef struct tt_guide_stats {
+    int64_t train_steps;  /* Adam steps taken */
+    int64_t train_rows;   /* rows trained on */
+    double train_seconds; /* GPU time of the training between passes */
+    float last_loss;      /* loss of the last step */
+    int32_t pad;
+} tt_guide_stats;
+
+tt_status tt_guide_render(tt_scene *s, tt_guide *g, const tt_camera *cam, const tt_render_params *p,
+                          const tt_guide_params *gp, float *rgb_out, double *rgb64_out, tt_render_stats *stats,
+                          tt_guide_stats *gstats);
+
 /* Reproducibility hook for the render math: applies op (0 sin, 1 cos, 2 pow(x, y)) of
  * the device sampler math to n device values; y is read for op 2. Synchronous. */
 tt_status tt_debug_math(tt_ctx *ctx, int32_t op, const float *x, const float *y, float *out, int64_t n);

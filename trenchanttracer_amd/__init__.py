@@ -325,6 +325,22 @@ class Guide:
         check(lib.tt_guide_train_step(self._h, _vp(x), _vp(d), _vp(w), len(x), C.byref(loss)))
         return loss.value
 
+    def render(self, scene, cam, params, alpha=0.5, train=True, train_paths=0, train_batch=0, train_spp=0,
+               min_roughness=0.3, want_fp64=False):
+        """Guided path-traced image of `scene` (tt_guide_render): returns (image, stats,
+        guide stats) or (image, image64, stats, guide stats) with want_fp64."""
+        gp = N.tt_guide_params(float(alpha), float(min_roughness), int(bool(train)), int(train_paths),
+                               int(train_batch), int(train_spp))
+        out = np.empty((cam.height, cam.width, 3), np.float32)
+        out64 = np.empty((cam.height, cam.width, 3), np.float64) if want_fp64 else None
+        st = N.tt_render_stats()
+        gs = N.tt_guide_stats()
+        check(lib.tt_guide_render(scene._h, self._h, C.byref(cam), C.byref(params), C.byref(gp), _vp(out), _vp(out64),
+                                  C.byref(st), C.byref(gs)))
+        gst = dict(train_steps=gs.train_steps, train_rows=gs.train_rows, train_seconds=gs.train_seconds,
+                   last_loss=gs.last_loss)
+        return (out, out64, _stats(st), gst) if want_fp64 else (out, _stats(st), gst)
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             lib.tt_guide_destroy(self._h)

@@ -70,6 +70,16 @@ class tt_render_stats(C.Structure):
                 ("seconds", C.c_double), ("iterations", C.c_int32), ("paths_in_flight", C.c_int32)]
 
 
+class tt_guide_params(C.Structure):
+    _fields_ = [("alpha", C.c_float), ("min_roughness", C.c_float), ("train", C.c_int32), ("train_paths", C.c_int32),
+                ("train_batch", C.c_int32), ("train_spp", C.c_int32)]
+
+
+class tt_guide_stats(C.Structure):
+    _fields_ = [("train_steps", C.c_int64), ("train_rows", C.c_int64), ("train_seconds", C.c_double),
+                ("last_loss", C.c_float), ("pad", C.c_int32)]
+
+
 P = C.c_void_p
 I32, I64, U32, U64, F32, F64 = C.c_int32, C.c_int64, C.c_uint32, C.c_uint64, C.c_float, C.c_double
 PF = C.POINTER(C.c_float)
@@ -129,6 +139,8 @@ PROTOS = {
     "tt_guide_forward": (C.c_int, [P, P, I64, P]),
     "tt_guide_grad": (C.c_int, [P, P, P, P, I64, PF, P]),
     "tt_guide_train_step": (C.c_int, [P, P, P, P, I64, PF]),
+    "tt_guide_render": (C.c_int, [P, P, C.POINTER(tt_camera), C.POINTER(tt_render_params), C.POINTER(tt_guide_params),
+                                  P, P, C.POINTER(tt_render_stats), C.POINTER(tt_guide_stats)]),
     "tt_batch_create": (C.c_int, [P, I32, P, P, P, P, P, I32, I32, F32, C.POINTER(P)]),
     "tt_batch_destroy": (None, [P]),
     "tt_batch_render": (C.c_int, [P, P, C.POINTER(tt_render_params), P, C.POINTER(tt_render_stats)]),

@@ -1,5 +1,7 @@
 // C ABI, device half: context, scene upload, traversal entry points.
+#include <algorithm>
 #include <cmath>
+#include <initializer_list>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -298,9 +300,14 @@ tt_status tt_scene_create(tt_ctx *ctx, const tt_mesh *mesh, const tt_bvh *bvh, c
         std::vector<int32_t> etri;
         std::vector<float> ecdf(1, 0.0f), everts;
         double earea = 0.0;
+        float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
         for (int32_t i = 0; i < m.ntris(); ++i) {
             V3 v0 = m.verts[m.tri_v[3 * (size_t)i]], v1 = m.verts[m.tri_v[3 * (size_t)i + 1]],
                v2 = m.verts[m.tri_v[3 * (size_t)i + 2]];
+            for (const V3 &v : {v0, v1, v2}) {
+                lo[0] = std::min(lo[0], v.x), lo[1] = std::min(lo[1], v.y), lo[2] = std::min(lo[2], v.z);
+                hi[0] = std::max(hi[0], v.x), hi[1] = std::max(hi[1], v.y), hi[2] = std::max(hi[2], v.z);
+            }
             double ax = (double)v1.x - v0.x, ay = (double)v1.y - v0.y, az = (double)v1.z - v0.z;
             double bx = (double)v2.x - v0.x, by = (double)v2.y - v0.y, bz = (double)v2.z - v0.z;
             double cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
@@ -328,6 +335,11 @@ tt_status tt_scene_create(tt_ctx *ctx, const tt_mesh *mesh, const tt_bvh *bvh, c
                 }
             }
         }
+        if (m.ntris() > 0)
+            for (int k = 0; k < 3; ++k) {
+                s->bmin[k] = lo[k];
+                s->bmax[k] = hi[k];
+            }
         up((void **)&s->tri_geo, geo.data(), geo.size() * 4, "scene geometric normals");
         if (!etri.empty()) {
             for (float &c : ecdf) c = (float)(c / earea);

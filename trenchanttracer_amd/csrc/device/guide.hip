@@ -99,37 +99,6 @@ __global__ __launch_bounds__(256) void k_gemm(const __bf16 *__restrict__ A, cons
     }
 }
 
-// ------------------------------------------------------------------ vMF mixture
-struct Lobes {
-    float mu[kGuideLobes][3];
-    float kappa[kGuideLobes];
-    float w[kGuideLobes];
-    float inv_len[kGuideLobes];
-};
-
-__device__ __forceinline__ void decode_lobes(const float *raw, Lobes &L) {
-    float mx = -1e30f;
-    for (int k = 0; k < kGuideLobes; ++k) mx = fmaxf(mx, raw[kGuideLobes * 4 + k]);
-    float sum = 0.0f;
-    for (int k = 0; k < kGuideLobes; ++k) {
-        float m0 = raw[3 * k], m1 = raw[3 * k + 1], m2 = raw[3 * k + 2];
-        float len = sqrtf(m0 * m0 + m1 * m1 + m2 * m2) + 1e-6f;
-        L.inv_len[k] = 1.0f / len;
-        L.mu[k][0] = m0 * L.inv_len[k];
-        L.mu[k][1] = m1 * L.inv_len[k];
-        L.mu[k][2] = m2 * L.inv_len[k];
-        L.kappa[k] = expf(fminf(fmaxf(raw[kGuideLobes * 3 + k], kGuideLogKappaMin), kGuideLogKappaMax));
-        L.w[k] = expf(raw[kGuideLobes * 4 + k] - mx);
-        sum += L.w[k];
-    }
-    for (int k = 0; k < kGuideLobes; ++k) L.w[k] /= sum;
-}
-
-// log of the vMF density kappa / (2 pi (1 - e^-2k)) exp(k (mu.w - 1))
-__device__ __forceinline__ float vmf_log(float kappa, float cosang) {
-    return logf(kappa) - 1.83787706641f - log1pf(-expf(-2.0f * kappa)) + kappa * (cosang - 1.0f);
-}
-
 // loss = sum_r s_r * (-log q(w_r)) / sum_r s_r; writes d loss / d raw (bf16, 48 wide + pad to 64)
 __global__ __launch_bounds__(256) void k_loss_grad(const float *__restrict__ raw, const float *__restrict__ dirs,
                                                    const float *__restrict__ wts, __bf16 *__restrict__ draw,

@@ -36,6 +36,80 @@ struct GuideBatch {
     float *wts = nullptr;     // [R] training weights (radiance / pdf)
 };
 
+// ------------------------------------------------------------------ vMF mixture
+struct Lobes {
+    float mu[kGuideLobes][3];
+    float kappa[kGuideLobes];
+    float w[kGuideLobes];
+    float inv_len[kGuideLobes];
+};
+
+// raw: 40 outputs (mean 3k..3k+2, log kappa 24+k, logit 32+k)
+__device__ __forceinline__ void decode_lobes(const float *raw, Lobes &L) {
+    float mx = -1e30f;
+    for (int k = 0; k < kGuideLobes; ++k) mx = fmaxf(mx, raw[kGuideLobes * 4 + k]);
+    float sum = 0.0f;
+    for (int k = 0; k < kGuideLobes; ++k) {
+        float m0 = raw[3 * k], m1 = raw[3 * k + 1], m2 = raw[3 * k + 2];
+        float len = sqrtf(m0 * m0 + m1 * m1 + m2 * m2) + 1e-6f;
+        L.inv_len[k] = 1.0f / len;
+        L.mu[k][0] = m0 * L.inv_len[k];
+        L.mu[k][1] = m1 * L.inv_len[k];
+        L.mu[k][2] = m2 * L.inv_len[k];
+        L.kappa[k] = expf(fminf(fmaxf(raw[kGuideLobes * 3 + k], kGuideLogKappaMin), kGuideLogKappaMax));
+        L.w[k] = expf(raw[kGuideLobes * 4 + k] - mx);
+        sum += L.w[k];
+    }
+    for (int k = 0; k < kGuideLobes; ++k) L.w[k] /= sum;
+}
+
+// log of the vMF density kappa / (2 pi (1 - e^-2k)) exp(k (mu.w - 1))
+__device__ __forceinline__ float vmf_log(float kappa, float cosang) {
+    return logf(kappa) - 1.83787706641f - log1pf(-expf(-2.0f * kappa)) + kappa * (cosang - 1.0f);
+}
+
+// mixture density at unit direction (x, y, z)
+__device__ __forceinline__ float mixture_pdf(const Lobes &L, float x, float y, float z) {
+    float q = 0.0f;
+    for (int k = 0; k < kGuideLobes; ++k) {
+        const float c = L.mu[k][0] * x + L.mu[k][1] * y + L.mu[k][2] * z;
+        q += L.w[k] * expf(vmf_log(L.kappa[k], c));
+    }
+    return q;
+}
+
+// Samples the mixture: lobe by cumulative weight (u0), then the vMF lobe by
+// Jakob's stable inversion w = 1 + log(u1 + (1 - u1) e^{-2k}) / k, phi = 2 pi u2.
+__device__ __forceinline__ void mixture_sample(const Lobes &L, float u0, float u1, float u2, float *out) {
+    int k = kGuideLobes - 1;
+    float acc = 0.0f;
+    for (int j = 0; j < kGuideLobes; ++j) {
+        acc += L.w[j];
+        if (u0 <= acc) {
+            k = j;
+            break;
+        }
+    }
+    const float kap = L.kappa[k];
+    const float w = fminf(1.0f, fmaxf(-1.0f, 1.0f + logf(u1 + (1.0f - u1) * expf(-2.0f * kap)) / kap));
+    const float sr = sqrtf(fmaxf(0.0f, 1.0f - w * w));
+    float sp, cp;
+    __sincosf(6.28318531f * u2, &sp, &cp);
+    const float mx = L.mu[k][0], my = L.mu[k][1], mz = L.mu[k][2];
+    // frame around mu (Duff et al. 2017)
+    const float s = mz >= 0.0f ? 1.0f : -1.0f;
+    const float a = -1.0f / (s + mz);
+    const float b = mx * my * a;
+    const float tx = 1.0f + s * mx * mx * a, ty = s * b, tz = -s * mx;
+    const float bx = b, by = s + my * my * a, bz = -my;
+    const float lx = sr * cp, ly = sr * sp;
+    float dx = tx * lx + bx * ly + mx * w, dy = ty * lx + by * ly + my * w, dz = tz * lx + bz * ly + mz * w;
+    const float inv = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
+    out[0] = dx * inv;
+    out[1] = dy * inv;
+    out[2] = dz * inv;
+}
+
 hipError_t guide_forward(tt_guide *g, int64_t n, const int32_t *n_dev, hipStream_t s);
 hipError_t guide_train_rows(tt_guide *g, int64_t n, hipStream_t s);
 tt_status guide_alloc_batch(tt_guide *g, int64_t rows);

@@ -24,6 +24,7 @@
 #include "../common/philox.h"
 #include "../host/tt_host.h"
 #include "dev_common.h"
+#include "guide_dev.h"
 #include "scene_dev.h"
 #include "ttmath.h"
 
@@ -203,7 +204,20 @@ struct SceneView {
     float emit_inv_area;
 };
 
+// Guiding inputs of the shade stage (all null / zero when unguided).
+constexpr int kRecFloats = 20;  // inputs (9), wi (3), pdf, 1 / throughput after the vertex (3), radiance (3), pad
+struct GuideShade {
+    const float *raw;        // [slot][48] mixture of this bounce's queue slots (null: unguided)
+    float alpha;             // probability of sampling the guide (one-sample MIS with the BSDF)
+    float *rec;              // training records [ntrain][max_depth][kRecFloats] (null: no training)
+    int32_t stride, ntrain;  // training paths: (path id + toff) % stride == 0, (id + toff) / stride < ntrain
+    int32_t toff;            // per-pass offset, so that the recorded pixels change from pass to pass
+    float min_rough;         // conductors with at least this roughness are guided (diffuse always)
+    float bmin[3], binv[3];  // scene box, for the position input
+};
+
 struct ShadeArgs {
+    GuideShade gd;
     SceneView sv;
     const SceneTable *tab;
     const int32_t *sample_dev;
@@ -533,6 +547,78 @@ __device__ f3 env_sample(const SceneView &sv, const float u[4], f3 *dir, float *
     return env_eval(sv, *dir, pdf);
 }
 
+__device__ __forceinline__ int train_slot(const GuideShade &gd, int32_t p) {
+    if (!gd.rec || (p + gd.toff) % gd.stride) return -1;
+    const int t = (p + gd.toff) / gd.stride;
+    return t < gd.ntrain ? t : -1;
+}
+
+// Adds a radiance contribution found at segment `bounce` to the incident-radiance
+// estimates of the path's earlier recorded vertices: L_i += c / T_{i+1}.
+__device__ __forceinline__ void credit(float *rec, int t, int bounce, int max_depth, f3 c) {
+    for (int i = 0; i < bounce; ++i) {
+        float *r = rec + ((int64_t)t * max_depth + i) * kRecFloats;
+        if (r[12] > 0.0f) {
+            r[16] += c.x * r[13];
+            r[17] += c.y * r[14];
+            r[18] += c.z * r[15];
+        }
+    }
+}
+
+__device__ __forceinline__ void guide_inputs(const GuideShade &gd, f3 point, f3 ns, f3 wo, float *in) {
+    in[0] = fminf(1.0f, fmaxf(0.0f, (point.x - gd.bmin[0]) * gd.binv[0]));
+    in[1] = fminf(1.0f, fmaxf(0.0f, (point.y - gd.bmin[1]) * gd.binv[1]));
+    in[2] = fminf(1.0f, fmaxf(0.0f, (point.z - gd.bmin[2]) * gd.binv[2]));
+    in[3] = 0.5f * (ns.x + 1.0f);
+    in[4] = 0.5f * (ns.y + 1.0f);
+    in[5] = 0.5f * (ns.z + 1.0f);
+    in[6] = 0.5f * (wo.x + 1.0f);
+    in[7] = 0.5f * (wo.y + 1.0f);
+    in[8] = 0.5f * (wo.z + 1.0f);
+}
+
+// Shading normal oriented like the geometric normal towards wo (as the shade stage).
+__device__ __forceinline__ f3 oriented_normal(const SceneView &sv, const tt_hit &h, f3 wo, f3 *ngf_out, bool *front) {
+    const f3 ng = xyz(sv.tri_geo[h.tri_id]);
+    *front = dot(wo, ng) > 0.0f;
+    const f3 ngf = *front ? ng : ng * -1.0f;
+    const float4 *tn = sv.tri_normals + 3 * (int64_t)h.tri_id;
+    const float bw = 1.0f - h.u - h.v;
+    f3 ns = vnorm((xyz(tn[0]) * bw + xyz(tn[1]) * h.u) + xyz(tn[2]) * h.v);
+    if (dot(ns, ngf) < 0.0f) ns = ns * -1.0f;
+    *ngf_out = ngf;
+    return ns;
+}
+
+struct GuideInArgs {
+    GuideShade gd;
+    SceneView sv;
+    const tt_ray *rays;
+    const tt_hit *hits;
+    const int32_t *counts;
+    int32_t cur;
+    float *in;   // [slot][9]
+};
+
+__global__ __launch_bounds__(256) void k_guide_inputs(GuideInArgs a) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.counts[a.cur]) return;
+    float *in = a.in + (int64_t)i * kGuideIn;
+    const tt_hit h = a.hits[i];
+    if (h.tri_id < 0) {
+        for (int k = 0; k < kGuideIn; ++k) in[k] = 0.0f;
+        return;
+    }
+    const tt_ray r = a.rays[i];
+    const f3 rd = F3(r.dx, r.dy, r.dz), wo = rd * -1.0f;
+    f3 ngf;
+    bool front;
+    const f3 ns = oriented_normal(a.sv, h, wo, &ngf, &front);
+    const f3 point = F3(r.ox, r.oy, r.oz) + rd * h.t;
+    guide_inputs(a.gd, point, ns, wo, in);
+}
+
 __global__ __launch_bounds__(256) void k_shade_path(ShadeArgs a) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const int32_t n = a.counts[a.cur];
@@ -585,6 +671,9 @@ __global__ __launch_bounds__(256) void k_shade_path(ShadeArgs a) {
                 }
                 add = thr * em * w;
             }
+            const int ts = train_slot(a.gd, p);
+            float *rec_b = ts >= 0 ? a.gd.rec + ((int64_t)ts * a.max_depth + a.bounce) * kRecFloats : nullptr;
+            if (rec_b) rec_b[12] = 0.0f;
             if (a.bounce < a.max_depth - 1 && m.type != TT_MAT_EMITTER) {
                 const float eps = 1e-4f * fmaxf(1.0f, fmaxf(fabsf(point.x), fmaxf(fabsf(point.y), fabsf(point.z))));
                 const Frame fr = make_frame(ns);
@@ -593,6 +682,12 @@ __global__ __launch_bounds__(256) void k_shade_path(ShadeArgs a) {
                 draws((uint32_t)local, smp, (uint32_t)(a.bounce + 1), 1u, a.seed, ul);
                 draws((uint32_t)local, smp, (uint32_t)(a.bounce + 1), 2u, a.seed, ub);
                 const bool smooth = m.type == TT_MAT_DIELECTRIC;
+                // guiding: mixture of this slot (non-delta surfaces only)
+                const bool guided = a.gd.raw != nullptr &&
+                                    (m.type == TT_MAT_DIFFUSE || (m.type == TT_MAT_CONDUCTOR && m.roughness >= a.gd.min_rough));
+                Lobes lobes;
+                if (guided) decode_lobes(a.gd.raw + (int64_t)i * kGuideNOutPad16, lobes);
+                const float alpha = guided ? a.gd.alpha : 0.0f;
                 // next-event estimation
                 if (!smooth && (p_env > 0.0f || p_emit > 0.0f) && wol.z > 0.0f) {
                     f3 wi, le;
@@ -623,6 +718,7 @@ __global__ __launch_bounds__(256) void k_shade_path(ShadeArgs a) {
                         f3 f_cos;
                         float pdf_b;
                         bsdf_eval(m, wol, to_local(fr, wi), &f_cos, &pdf_b);
+                        if (guided) pdf_b = alpha * mixture_pdf(lobes, wi.x, wi.y, wi.z) + (1.0f - alpha) * pdf_b;
                         if (maxc(f_cos) > 0.0f) {
                             float w = power_heuristic(pdf_l, pdf_b);
                             scontrib = thr * f_cos * le * (w / pdf_l);
@@ -632,11 +728,34 @@ __global__ __launch_bounds__(256) void k_shade_path(ShadeArgs a) {
                         }
                     }
                 }
-                // BSDF sampling
+                // BSDF sampling, or one-sample MIS of guide and BSDF on guided vertices
                 f3 wil, weight;
                 float pdf_b;
                 bool delta;
-                if (wol.z > 0.0f && bsdf_sample(m, wol, front, ub, &wil, &weight, &pdf_b, &delta)) {
+                bool sampled;
+                if (guided) {
+                    float ug[4];
+                    draws((uint32_t)local, smp, (uint32_t)(a.bounce + 1), 3u, a.seed, ug);
+                    delta = false;
+                    f3 f_cos;
+                    if (ug[0] <= alpha) {
+                        float d[3];
+                        mixture_sample(lobes, ug[1], ug[2], ug[3], d);
+                        wil = to_local(fr, F3(d[0], d[1], d[2]));
+                        bsdf_eval(m, wol, wil, &f_cos, &pdf_b);
+                    } else {
+                        sampled = wol.z > 0.0f && bsdf_sample(m, wol, front, ub, &wil, &weight, &pdf_b, &delta);
+                        f_cos = sampled ? weight * pdf_b : F3(0, 0, 0);
+                    }
+                    const f3 wiw = to_world(fr, wil);
+                    const float pdf_c = alpha * mixture_pdf(lobes, wiw.x, wiw.y, wiw.z) + (1.0f - alpha) * pdf_b;
+                    sampled = pdf_c > 0.0f && maxc(f_cos) > 0.0f && wol.z > 0.0f;
+                    weight = sampled ? f_cos * (1.0f / pdf_c) : F3(0, 0, 0);
+                    pdf_b = pdf_c;
+                } else {
+                    sampled = wol.z > 0.0f && bsdf_sample(m, wol, front, ub, &wil, &weight, &pdf_b, &delta);
+                }
+                if (sampled) {
                     f3 wi = to_world(fr, wil);
                     float side = dot(wi, ngf);
                     bool transmit = delta && wil.z < 0.0f;
@@ -655,12 +774,25 @@ __global__ __launch_bounds__(256) void k_shade_path(ShadeArgs a) {
                             cont = true;
                             a.thr[p] = make_float4(thr.x, thr.y, thr.z, pdf_b);
                             a.flags[p] = delta ? 1 : 0;
+                            if (rec_b && !delta && pdf_b > 0.0f) {
+                                guide_inputs(a.gd, point, ns, wo, rec_b);
+                                rec_b[9] = wi.x;
+                                rec_b[10] = wi.y;
+                                rec_b[11] = wi.z;
+                                rec_b[13] = thr.x > 0.0f ? 1.0f / thr.x : 0.0f;
+                                rec_b[14] = thr.y > 0.0f ? 1.0f / thr.y : 0.0f;
+                                rec_b[15] = thr.z > 0.0f ? 1.0f / thr.z : 0.0f;
+                                rec_b[16] = rec_b[17] = rec_b[18] = 0.0f;
+                                rec_b[12] = pdf_b;
+                            }
                         }
                     }
                 }
             }
         }
         if (maxc(add) > 0.0f) {
+            const int tc = train_slot(a.gd, p);
+            if (tc >= 0) credit(a.gd.rec, tc, a.bounce, a.max_depth, add);
             if (a.acc64) {
                 double *o = a.acc64 + 3 * (int64_t)pix;
                 atomicAdd(o, (double)add.x);
@@ -684,7 +816,7 @@ __global__ __launch_bounds__(256) void k_shade_path(ShadeArgs a) {
     if (shadow) {
         a.srays[sslot] = sr;
         a.sroot[sslot] = root;
-        a.scontrib[sslot] = make_float4(scontrib.x, scontrib.y, scontrib.z, __int_as_float(p % a.npix));
+        a.scontrib[sslot] = make_float4(scontrib.x, scontrib.y, scontrib.z, __int_as_float(p));
     }
 }
 
@@ -695,6 +827,8 @@ struct ApplyArgs {
     float *acc;
     double *acc64;
     unsigned long long *stats;
+    int32_t npix, bounce, max_depth;
+    GuideShade gd;
 };
 
 __global__ __launch_bounds__(256) void k_apply_shadow(ApplyArgs a) {
@@ -703,7 +837,10 @@ __global__ __launch_bounds__(256) void k_apply_shadow(ApplyArgs a) {
     if (i == 0) atomicAdd(&a.stats[1], (unsigned long long)n);
     if (i >= n || a.occ[i]) return;
     const float4 c = a.contrib[i];
-    const int32_t p = __float_as_int(c.w);
+    const int32_t path = __float_as_int(c.w);
+    const int32_t p = path % a.npix;
+    const int tc = train_slot(a.gd, path);
+    if (tc >= 0) credit(a.gd.rec, tc, a.bounce, a.max_depth, F3(c.x, c.y, c.z));
     if (a.acc64) {
         double *o = a.acc64 + 3 * (int64_t)p;
         atomicAdd(o, (double)c.x);
@@ -715,6 +852,39 @@ __global__ __launch_bounds__(256) void k_apply_shadow(ApplyArgs a) {
         atomicAdd(o + 1, c.y);
         atomicAdd(o + 2, c.z);
     }
+}
+
+// Turns the recorded vertices of the training paths into training rows
+// {inputs (9), wi (3), luminance(L_i) / pdf_i} and clears the records.
+__global__ __launch_bounds__(256) void k_collect(float *rec, int32_t ntrain, int32_t max_depth, float *buf,
+                                                 int32_t *count, int32_t cap) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)ntrain * max_depth) return;
+    float *r = rec + e * kRecFloats;
+    const float pdf = r[12];
+    if (pdf > 0.0f) {
+        const float lum = 0.2126f * r[16] + 0.7152f * r[17] + 0.0722f * r[18];
+        if (lum > 0.0f && isfinite(lum)) {
+            const int slot = atomicAdd(count, 1);
+            if (slot < cap) {
+                float *o = buf + (int64_t)slot * 16;
+                for (int k = 0; k < 12; ++k) o[k] = r[k];
+                o[12] = lum / pdf;
+            }
+        }
+    }
+    r[12] = 0.0f;
+}
+
+__global__ __launch_bounds__(256) void k_stage_train(const float *buf, int32_t n, float *in, float *dirs, float *wts) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float *b = buf + (int64_t)i * 16;
+    for (int k = 0; k < 9; ++k) in[(int64_t)i * 9 + k] = b[k];
+    dirs[3 * (int64_t)i] = b[9];
+    dirs[3 * (int64_t)i + 1] = b[10];
+    dirs[3 * (int64_t)i + 2] = b[11];
+    wts[i] = b[12];
 }
 
 __global__ void k_zero(float *acc, double *acc64, unsigned long long *stats, int64_t n3) {
@@ -831,6 +1001,18 @@ CamArgs camera_args(const tt_camera &c, float pi, float two_pi) {
     return a;
 }
 
+// Guiding state of one render call: the network, its sampling share and the
+// online-training schedule (paths recorded per pass, rows per Adam step).
+struct GuideRun {
+    tt_guide *g = nullptr;
+    GuideShade gd{};          // raw / rec filled per launch
+    bool train = false;
+    int32_t batch_rows = 1 << 18;
+    int32_t train_spp = 0;    // train only while fewer samples than this are done (0: always)
+    int64_t steps = 0, rows = 0;
+    double train_ms = 0.0;
+};
+
 tt_status ensure_state(tt_scene *s, int32_t cap, int32_t npix, bool fp64) {
     RenderState *rs = s->rs;
     if (rs && rs->cap >= cap && rs->pix_cap >= npix && (!fp64 || rs->acc64_cap >= npix)) return TT_OK;
@@ -872,7 +1054,7 @@ tt_status ensure_state(tt_scene *s, int32_t cap, int32_t npix, bool fp64) {
 
 // One bounce of the wavefront: extend -> shade (-> shadow -> apply).
 hipError_t launch_bounce(tt_scene *s, RenderState *rs, const tt_render_params *p, const SceneView &sv, int32_t cap,
-                         int32_t npix, int b, bool fp64, hipStream_t hs) {
+                         int32_t npix, int b, bool fp64, hipStream_t hs, const GuideRun *gr) {
     const int cur = b & 1, nxt = cur ^ 1;
     const int tri_mode = p->mode == TT_RENDER_REFERENCE ? kTriReference : kTriTwoSided;
     // reference mode: the exact loop (hits identical to the host restatement);
@@ -895,6 +1077,16 @@ hipError_t launch_bounce(tt_scene *s, RenderState *rs, const tt_render_params *p
     hipError_t e = launch_trace(ta, variant, false, lds, block, 0, s->ctx->num_cus, hs, tri_mode);
     if (e != hipSuccess) return e;
     ShadeArgs sa{};
+    if (gr) {
+        // guide inference over the hit queue: inputs, then the MLP (rows up to the device count)
+        GuideInArgs ga{gr->gd, sv, rs->rays[cur], rs->hits, rs->counts, cur, gr->g->batch.in};
+        hipLaunchKernelGGL(k_guide_inputs, dim3(gblocks), dim3(256), 0, hs, ga);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = guide_forward(gr->g, cap, rs->counts + cur, hs);
+        if (e != hipSuccess) return e;
+        sa.gd = gr->gd;
+        sa.gd.raw = gr->g->batch.raw;
+    }
     sa.sv = sv;
     sa.tab = (const SceneTable *)rs->table;
     sa.sample_dev = rs->sample_dev;
@@ -943,14 +1135,15 @@ hipError_t launch_bounce(tt_scene *s, RenderState *rs, const tt_render_params *p
     sh.ray_root = rs->sroot;
     e = launch_trace(sh, variant, true, lds, block, 0, s->ctx->num_cus, hs, kTriTwoSided);
     if (e != hipSuccess) return e;
-    ApplyArgs aa{rs->counts, rs->socc, rs->scontrib, rs->acc, fp64 ? rs->acc64 : nullptr, rs->stats};
+    ApplyArgs aa{rs->counts, rs->socc, rs->scontrib, rs->acc, fp64 ? rs->acc64 : nullptr, rs->stats,
+                 npix, b, p->max_depth, gr ? gr->gd : GuideShade{}};
     hipLaunchKernelGGL(k_apply_shadow, dim3(gblocks), dim3(256), 0, hs, aa);
     return hipGetLastError();
 }
 
 // One sample pass (kk samples per pixel in flight) for all scenes of the table.
 hipError_t launch_pass(tt_scene *s, RenderState *rs, const tt_render_params *p, const SceneView &sv, int32_t npix,
-                       int32_t kk, bool fp64, hipStream_t hs) {
+                       int32_t kk, bool fp64, hipStream_t hs, const GuideRun *gr = nullptr) {
     const int32_t cap = npix * kk;
     GenArgs g{};
     g.tab = (const SceneTable *)rs->table;
@@ -969,7 +1162,7 @@ hipError_t launch_pass(tt_scene *s, RenderState *rs, const tt_render_params *p, 
     g.counts = rs->counts;
     hipLaunchKernelGGL(k_generate, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, hs, g);
     hipError_t e = hipGetLastError();
-    for (int32_t b = 0; e == hipSuccess && b < p->max_depth; ++b) e = launch_bounce(s, rs, p, sv, cap, npix, b, fp64, hs);
+    for (int32_t b = 0; e == hipSuccess && b < p->max_depth; ++b) e = launch_bounce(s, rs, p, sv, cap, npix, b, fp64, hs, gr);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_advance_sample, dim3(1), dim3(1), 0, hs, rs->sample_dev, kk);
         e = hipGetLastError();
@@ -1016,12 +1209,65 @@ tt_status validate_params(const tt_render_params *p) {
     return TT_OK;
 }
 
+// Sizes the guide's record / training buffers for ntrain paths of max_depth vertices.
+tt_status guide_train_buffers(tt_guide *g, int32_t ntrain, int32_t max_depth, hipStream_t hs) {
+    const int64_t ne = (int64_t)ntrain * max_depth;
+    if (g->rec_cap < ne * kRecFloats || g->train_cap < ne) {
+        if (g->rec) hipFree(g->rec);
+        if (g->train_buf) hipFree(g->train_buf);
+        g->rec = g->train_buf = nullptr;
+        g->rec_cap = g->train_cap = 0;
+        tt_status st = dev_alloc(g->ctx, (void **)&g->rec, (size_t)ne * kRecFloats * 4, "guide path records");
+        if (st == TT_OK) st = dev_alloc(g->ctx, (void **)&g->train_buf, (size_t)ne * 16 * 4, "guide training rows");
+        if (st != TT_OK) return st;
+        g->rec_cap = ne * kRecFloats;
+        g->train_cap = ne;
+    }
+    if (!g->train_count) {
+        tt_status st = dev_alloc(g->ctx, (void **)&g->train_count, 16, "guide training count");
+        if (st != TT_OK) return st;
+    }
+    hipError_t e = hipMemsetAsync(g->rec, 0, (size_t)ne * kRecFloats * 4, hs);
+    if (e == hipSuccess) e = hipMemsetAsync(g->train_count, 0, 16, hs);
+    return e == hipSuccess ? TT_OK : fail(TT_ERR_DEVICE, std::string("guide buffers: ") + hipGetErrorString(e));
+}
+
+// After a pass: the recorded vertices become training rows, and the network
+// takes Adam steps over them in batches of batch_rows (a remainder under a
+// quarter batch is dropped, except when it is the only batch).
+hipError_t guide_train_pass(GuideRun *gr, int32_t max_depth, hipStream_t hs) {
+    tt_guide *g = gr->g;
+    const int64_t ne = (int64_t)gr->gd.ntrain * max_depth;
+    hipLaunchKernelGGL(k_collect, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, hs, g->rec, gr->gd.ntrain,
+                       max_depth, g->train_buf, g->train_count, (int32_t)g->train_cap);
+    int32_t cnt = 0;
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(&cnt, g->train_count, 4, hipMemcpyDeviceToHost, hs);
+    if (e == hipSuccess) e = hipStreamSynchronize(hs);
+    if (e == hipSuccess) e = hipMemsetAsync(g->train_count, 0, 4, hs);
+    cnt = (int32_t)std::min<int64_t>(cnt, g->train_cap);
+    for (int32_t off = 0; e == hipSuccess && off < cnt; off += gr->batch_rows) {
+        const int32_t n = std::min(gr->batch_rows, cnt - off);
+        if (off > 0 && n < gr->batch_rows / 4) break;
+        hipLaunchKernelGGL(k_stage_train, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, hs,
+                           (const float *)(g->train_buf + (int64_t)off * 16), n, g->batch.in, g->batch.dirs,
+                           g->batch.wts);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = guide_train_rows(g, n, hs);
+        gr->steps++;
+        gr->rows += n;
+    }
+    return e;
+}
+
 // Renders nsc scenes sharing one node / triangle array (roots[k] = root node of
 // scene k) in one wavefront. out: host, concatenated images in scene order.
 tt_status render_impl(tt_scene *s, int32_t nsc, const tt_camera *cams, const int32_t *roots,
-                      const tt_render_params *p, float *out, double *out64, tt_render_stats *stats) {
+                      const tt_render_params *p, float *out, double *out64, tt_render_stats *stats,
+                      GuideRun *gr = nullptr) {
     tt_status st = validate_params(p);
     if (st != TT_OK) return st;
+    if (gr && p->mode != TT_RENDER_PATH) return fail(TT_ERR_INVALID, "render: guiding needs path mode");
     if (nsc <= 0 || nsc > kMaxScenes) return fail(TT_ERR_INVALID, "render: 1..64 scenes");
     if (!s->tri_normals || !s->tri_mat || !s->mats || !s->tri_geo)
         return fail(TT_ERR_INVALID, "render: scene has no shading data");
@@ -1058,6 +1304,22 @@ tt_status render_impl(tt_scene *s, int32_t nsc, const tt_camera *cams, const int
     if (st != TT_OK) return st;
     RenderState *rs = s->rs;
     hipStream_t hs = ctx->stream;
+    if (gr) {
+        // inference rows: one per queued path; training rows: batch_rows
+        st = guide_alloc_batch(gr->g, std::max<int64_t>((int64_t)npix * k, gr->batch_rows));
+        if (st == TT_OK && gr->train) {
+            const int32_t want = std::max(1, gr->gd.ntrain);
+            gr->gd.stride = std::max(1, (npix * k) / want);
+            gr->gd.ntrain = (npix * k) / gr->gd.stride + 1;
+            st = guide_train_buffers(gr->g, gr->gd.ntrain, p->max_depth, hs);
+        }
+        if (st != TT_OK) return st;
+        for (int q = 0; q < 3; ++q) {
+            const float ext = s->bmax[q] - s->bmin[q];
+            gr->gd.bmin[q] = s->bmin[q];
+            gr->gd.binv[q] = ext > 0.0f ? 1.0f / ext : 1.0f;
+        }
+    }
     const SceneView sv = scene_view(s, p);
     const int64_t n3 = (int64_t)npix * 3;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1075,7 +1337,33 @@ tt_status render_impl(tt_scene *s, int32_t nsc, const tt_camera *cams, const int
     }
     int32_t iters = 0;
     const int32_t full = p->spp / k, rest = p->spp % k;
-    if (e == hipSuccess && p->use_graph && full > 1) {
+    if (gr) {
+        // guided: host-driven passes, training between them (not graph-captured)
+        int32_t done = 0, pass = 0;
+        while (e == hipSuccess && done < p->spp) {
+            const int32_t kk = std::min(k, p->spp - done);
+            const bool train = gr->train && (gr->train_spp <= 0 || done < gr->train_spp);
+            gr->gd.rec = train ? gr->g->rec : nullptr;
+            gr->gd.toff = gr->gd.stride > 1 ? (int32_t)(((int64_t)pass * 7919) % gr->gd.stride) : 0;
+            e = launch_pass(s, rs, p, sv, npix, kk, fp64, hs, gr);
+            if (e == hipSuccess && train) {
+                hipEvent_t t0 = nullptr, t1 = nullptr;
+                hipEventCreate(&t0);
+                hipEventCreate(&t1);
+                hipEventRecord(t0, hs);
+                e = guide_train_pass(gr, p->max_depth, hs);
+                hipEventRecord(t1, hs);
+                if (e == hipSuccess) e = hipEventSynchronize(t1);
+                float tms = 0.0f;
+                if (e == hipSuccess && hipEventElapsedTime(&tms, t0, t1) == hipSuccess) gr->train_ms += tms;
+                hipEventDestroy(t0);
+                hipEventDestroy(t1);
+            }
+            done += kk;
+            ++pass;
+            iters += p->max_depth;
+        }
+    } else if (e == hipSuccess && p->use_graph && full > 1) {
         // capture one pass (generate, max_depth x (extend, shade, shadow, apply),
         // advance the sample index) and replay it
         e = hipStreamBeginCapture(hs, hipStreamCaptureModeThreadLocal);
@@ -1091,7 +1379,7 @@ tt_status render_impl(tt_scene *s, int32_t nsc, const tt_camera *cams, const int
         for (int32_t i = 0; e == hipSuccess && i < full; ++i) e = launch_pass(s, rs, p, sv, npix, k, fp64, hs);
         iters += full * p->max_depth;
     }
-    if (e == hipSuccess && rest > 0) {
+    if (!gr && e == hipSuccess && rest > 0) {
         e = launch_pass(s, rs, p, sv, npix, rest, fp64, hs);
         iters += p->max_depth;
     }
@@ -1204,6 +1492,38 @@ tt_status tt_render(tt_scene *s, const tt_camera *cam, const tt_render_params *p
     if (!s || !cam || !p || !rgb_out) return fail(TT_ERR_INVALID, "tt_render: null argument");
     const int32_t root = 0;
     return render_impl(s, 1, cam, &root, p, rgb_out, rgb64_out, stats);
+}
+
+tt_status tt_guide_render(tt_scene *s, tt_guide *g, const tt_camera *cam, const tt_render_params *p,
+                          const tt_guide_params *gp, float *rgb_out, double *rgb64_out, tt_render_stats *stats,
+                          tt_guide_stats *gstats) {
+    if (!s || !g || !cam || !p || !gp || !rgb_out) return fail(TT_ERR_INVALID, "tt_guide_render: null argument");
+    if (g->ctx != s->ctx) return fail(TT_ERR_INVALID, "tt_guide_render: guide and scene on different contexts");
+    if (!(gp->alpha >= 0.0f && gp->alpha <= 1.0f)) return fail(TT_ERR_INVALID, "tt_guide_render: alpha outside [0, 1]");
+    if (gp->train_paths < 0 || gp->train_batch < 0 || gp->train_spp < 0 || gp->train_batch > (1 << 22))
+        return fail(TT_ERR_INVALID, "tt_guide_render: bad training sizes");
+    tt_render_params q = *p;
+    q.guided = 0;
+    GuideRun gr;
+    gr.g = g;
+    gr.gd.alpha = gp->alpha;
+    gr.gd.min_rough = gp->min_roughness;
+    gr.train = gp->train != 0;
+    gr.gd.ntrain = gp->train_paths > 0 ? gp->train_paths : (1 << 16);
+    gr.batch_rows = gp->train_batch > 0 ? gp->train_batch : (1 << 18);
+    gr.train_spp = gp->train_spp;
+    const int32_t root = 0;
+    tt_status st = render_impl(s, 1, cam, &root, &q, rgb_out, rgb64_out, stats, &gr);
+    if (st == TT_OK && gstats) {
+        gstats->train_steps = gr.steps;
+        gstats->train_rows = gr.rows;
+        gstats->train_seconds = gr.train_ms * 1e-3;
+        float loss = 0.0f;
+        if (gr.steps > 0 && hipMemcpy(&loss, g->scalars, 4, hipMemcpyDeviceToHost) != hipSuccess)
+            return fail(TT_ERR_DEVICE, "tt_guide_render: reading the loss failed");
+        gstats->last_loss = loss;
+    }
+    return st;
 }
 
 tt_status tt_batch_create(tt_ctx *ctx, int32_t n, const tt_mesh *const *meshes, const tt_bvh *const *bvhs,

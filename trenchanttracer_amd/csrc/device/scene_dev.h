@@ -46,6 +46,7 @@ struct tt_scene {
     int32_t env_w = 0, env_h = 0;
     float env_scale = 1.0f;
     double env_total = 0.0;
+    float bmin[3] = {0.0f, 0.0f, 0.0f}, bmax[3] = {1.0f, 1.0f, 1.0f};  // mesh bounds (guide positions)
     size_t bytes = 0;
     struct RenderState *rs = nullptr;  // wavefront buffers, sized on first render
 };

@@ -160,3 +160,64 @@ def batched(n=32, seed=1, tri_lo=1_000, tri_hi=2_000_000, resolutions=(256, 512,
             log(f"scene {k}: {m.ntris} tris, {res}x{res}")
     env = sky_env(1024, 512)
     return meshes, mats, cams, env, counts
+
+
+def _rect(axis, c, a0, a1, b0, b1, sign, mat):
+    """Axis-aligned rectangle in the plane coord[axis] = c spanning [a0, a1] x [b0, b1]
+    along the next two axes in cyclic order, geometric normal sign * e_axis."""
+    u, v = (axis + 1) % 3, (axis + 2) % 3
+
+    def pt(a, b):
+        p = [0.0, 0.0, 0.0]
+        p[axis], p[u], p[v] = c, a, b
+        return p
+
+    p0, p1, p2 = pt(a0, b0), pt(a1, b0), pt(a0, b1)
+    if sign < 0:
+        p1, p2 = p2, p1
+    return Mesh.quad(p0, p1, p2, mat=mat)
+
+
+def door_room(width=256, height=256, emission=60.0):
+    """Indirect-lighting case for guiding: a closed room x in [-2, 2], y in [0, 2],
+    z in [-2, 2] (walls facing inwards) split by a partition at x = 1 with a door
+    (|z| < 0.2, y < 1.2); the only light is a ceiling panel (x in [1.2, 1.8],
+    z in [-1.6, -0.8], facing down) behind the partition, so the camera's part of
+    the room is lit through the door. Materials: 0 walls grey diffuse 0.7,
+    1 light (albedo 0, the given emission), 2 partition diffuse 0.6. Returns
+    (mesh, materials, camera)."""
+    from . import MAT_DIFFUSE, camera, material
+    parts = [
+        _rect(1, 0.0, -2, 2, -2, 2, +1, 0),    # floor (y = 0, up): axes z, x
+        _rect(1, 2.0, -2, 2, -2, 2, -1, 0),    # ceiling
+        _rect(0, -2.0, 0, 2, -2, 2, +1, 0),    # x = -2 wall: axes y, z
+        _rect(0, 2.0, 0, 2, -2, 2, -1, 0),     # x = +2 wall
+        _rect(2, -2.0, -2, 2, 0, 2, +1, 0),    # z = -2 wall: axes x, y
+        _rect(2, 2.0, -2, 2, 0, 2, -1, 0),     # z = +2 wall
+        _rect(0, 1.0, 0, 2, -2, -0.2, -1, 2),  # partition, left of the door
+        _rect(0, 1.0, 0, 2, 0.2, 2, -1, 2),    # partition, right of the door
+        _rect(0, 1.0, 1.2, 2, -0.2, 0.2, -1, 2),  # lintel above the door
+        _rect(1, 1.98, -1.6, -0.8, 1.2, 1.8, -1, 1),  # light: axes z, x
+    ]
+    mesh = parts[0]
+    for q in parts[1:]:
+        mesh.append(q)
+    mats = [material(MAT_DIFFUSE, (0.7, 0.7, 0.7)),
+            material(MAT_DIFFUSE, (0.0, 0.0, 0.0), emission=(emission, emission, emission)),
+            material(MAT_DIFFUSE, (0.6, 0.6, 0.6))]
+    cam = camera((-1.7, 1.0, 1.7), (2.7, -0.6, -2.0), fov=(70.0, 70.0 * height / width), width=width, height=height)
+    return mesh, mats, cam
+
+
+def emissive_box(albedo=0.5, emission=0.5, width=64, height=64):
+    """Closed box [-1, 1]^3, walls facing inwards, every wall diffuse with the given
+    albedo and emission: the radiance everywhere inside is emission / (1 - albedo)
+    (up to the path-depth truncation). Returns (mesh, materials, camera)."""
+    from . import MAT_DIFFUSE, camera, material
+    parts = [_rect(a, s * -1.0, -1, 1, -1, 1, s, 0) for a in range(3) for s in (+1, -1)]
+    mesh = parts[0]
+    for q in parts[1:]:
+        mesh.append(q)
+    mats = [material(MAT_DIFFUSE, (albedo,) * 3, emission=(emission,) * 3)]
+    cam = camera((0.1, 0.2, 0.3), (0.3, -0.2, -1.0), fov=(90.0, 90.0), width=width, height=height)
+    return mesh, mats, cam
