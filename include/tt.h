@@ -378,6 +378,11 @@ tt_status tt_guide_render(tt_scene *s, tt_guide *g, const tt_camera *cam, const 
                           const tt_guide_params *gp, float *rgb_out, double *rgb64_out, tt_render_stats *stats,
                           tt_guide_stats *gstats);
 
+/* Sampling hook for the guiding mixture: decodes one row of 40 raw network outputs
+ * (host), draws n directions with the device's mixture sampler and returns n x 4
+ * floats {direction.xyz, mixture density at the direction}. Synchronous. */
+tt_status tt_debug_guide_sample(tt_ctx *ctx, const float *raw, int64_t n, uint32_t seed, float *out);
+
 /* Reproducibility hook for the render math: applies op (0 sin, 1 cos, 2 pow(x, y)) of
  * the device sampler math to n device values; y is read for op 2. Synchronous. */
 tt_status tt_debug_math(tt_ctx *ctx, int32_t op, const float *x, const float *y, float *out, int64_t n);

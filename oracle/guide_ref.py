@@ -28,14 +28,46 @@ def unpack(params):
     return out
 
 
-def encode(inp):
+def sincos_f32(x):
+    """Sine and cosine in float32 arithmetic, step for step the device's
+    tm_sincos (and the C oracle's o_sincos): Cody-Waite reduction by pi/2 with a
+    three-part constant, Taylor polynomials of sin (to r^9) and cos (to r^10)."""
+    f = np.float32
+    x = np.asarray(x, np.float32)
+    k = np.rint(x * f(float.fromhex("0x1.45f306p-1"))).astype(np.float32)
+    r = x - k * f(float.fromhex("0x1.92p+0"))
+    r = r - k * f(float.fromhex("0x1.fb4p-12"))
+    r = r - k * f(float.fromhex("0x1.4442d2p-24"))
+    r2 = r * r
+    ps = f(-float.fromhex("0x1.555556p-3")) + r2 * (f(float.fromhex("0x1.111112p-7")) + r2 * (f(-float.fromhex("0x1.a01a02p-13")) + r2 * f(float.fromhex("0x1.71de3ap-19"))))
+    sn = r + r * r2 * ps
+    pc = f(float.fromhex("0x1.555556p-5")) + r2 * (f(-float.fromhex("0x1.6c16c2p-10")) + r2 * (f(float.fromhex("0x1.a01a02p-16")) + r2 * f(-float.fromhex("0x1.27e4fcp-22"))))
+    cs = f(1.0) - f(0.5) * r2 + r2 * r2 * pc
+    q = k.astype(np.int64) & 3
+    s_out = np.select([q == 0, q == 1, q == 2], [sn, cs, -sn], -cs).astype(np.float32)
+    c_out = np.select([q == 0, q == 1, q == 2], [cs, -sn, -cs], sn).astype(np.float32)
+    return s_out, c_out
+
+
+def features_f32(inp):
+    """Frequency features before BF16 rounding, [n][216] float32: the base pair
+    sincos(pi x) then the double-angle recurrences s' = (2 s) c, c' = (c - s)(c + s)
+    in float32 (guide.hip freq_base / freq_next)."""
+    f = np.float32
     x = np.asarray(inp, np.float32)
-    feats = np.zeros((len(x), K1))
+    out = np.zeros((len(x), FEAT), np.float32)
     for c in range(IN):
-        for f in range(FREQ):
-            arg = (x[:, c] * np.float32(np.float32(np.pi) * np.float32(1 << f))).astype(np.float64)
-            feats[:, c * 2 * FREQ + 2 * f] = np.sin(arg)
-            feats[:, c * 2 * FREQ + 2 * f + 1] = np.cos(arg)
+        s, co = sincos_f32(x[:, c] * f(3.14159265))
+        for q in range(FREQ):
+            out[:, c * 2 * FREQ + 2 * q] = s
+            out[:, c * 2 * FREQ + 2 * q + 1] = co
+            s, co = ((f(2.0) * s) * co).astype(np.float32), ((co - s) * (co + s)).astype(np.float32)
+    return out
+
+
+def encode(inp):
+    feats = np.zeros((len(inp), K1))
+    feats[:, :FEAT] = features_f32(inp)
     return bf16(feats)
 
 

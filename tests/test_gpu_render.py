@@ -140,3 +140,45 @@ def test_env_cdf_matches_fp64_numpy(tt, ctx):
     want_c = np.concatenate([np.zeros((h, 1)), np.cumsum(wgt, 1) / rows[:, None]], 1).astype(np.float32)
     assert np.max(np.abs(cond - want_c)) <= 1.2e-7
     s.close()
+
+
+def _ff_corner(a, b, d):
+    """Form factor from a differential area to a parallel rectangle [0, a] x [0, b]
+    whose corner lies on the area's normal at distance d (closed form)."""
+    A, B = a / d, b / d
+    sa, sb = np.sqrt(1 + A * A), np.sqrt(1 + B * B)
+    return (A / sa * np.arctan(B / sa) + B / sb * np.arctan(A / sb)) / (2 * np.pi)
+
+
+def _ff_rect(y0, y1, z0, z1, d):
+    """Parallel rectangle [y0, y1] x [z0, z1] in coordinates centred on the foot of
+    the normal, by superposition of corner rectangles (signed extents)."""
+    def F(y, z):
+        return np.sign(y) * np.sign(z) * _ff_corner(abs(y), abs(z), d)
+    return F(y1, z1) - F(y0, z1) - F(y1, z0) + F(y0, z0)
+
+
+@pytest.mark.parametrize("point", [(2.0, 1.0, -0.5), (2.0, 1.6, 0.3), (2.0, 0.4, 1.2)])
+def test_direct_light_matches_form_factor(tt, ctx, point):
+    """Direct lighting (path depth 2: next-event estimation + BSDF hits under MIS)
+    of a diffuse wall point by a parallel rectangular light 0.98 away equals
+    albedo * Le * F, F the closed-form differential-area-to-rectangle form factor.
+    The camera looks at the point through a 0.02 degree field of view, so every
+    pixel sees it. Coordinates reach 2, so the shadow-ray origin offset is twice
+    the minimum: a shadow ray measured from the unoffset point would stop past the
+    light and report it occluded."""
+    from trenchanttracer_amd.workloads import _rect
+    _say(f"direct light at {point}")
+    wall = _rect(0, 2.0, 0, 2, -2, 2, -1, 0)               # x = 2, facing -x
+    light = _rect(0, 1.02, 1.3, 1.9, -1.5, 1.5, +1, 1)     # x = 1.02, facing +x
+    mesh = wall.append(light)
+    rho, le = 0.7, 60.0
+    mats = [tt.material(tt.MAT_DIFFUSE, (rho,) * 3), tt.material(tt.MAT_DIFFUSE, (0, 0, 0), emission=(le,) * 3)]
+    sc = tt.DeviceScene(ctx, mesh, tt.BVH.build(mesh), mats)
+    P = np.array(point)
+    cam = tt.camera(P - [0.5, 0, 0], (1, 0, 0), fov=(0.02, 0.02), width=16, height=16)
+    img, st = sc.render(cam, tt.render_params(spp=2048, max_depth=2, rr_depth=8, seed=3, mode=tt.RENDER_PATH))
+    want = rho * le * _ff_rect(1.3 - P[1], 1.9 - P[1], -1.5 - P[2], 1.5 - P[2], 2.0 - 1.02)
+    got = img.mean()
+    _say(f"  radiance {got:.5f}, closed form {want:.5f}")
+    assert abs(got - want) < 0.01 * want

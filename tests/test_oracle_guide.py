@@ -64,3 +64,31 @@ def test_network_gradient_matches_finite_differences():
             assert abs(num - g[idx]) < 1e-7 + 1e-4 * abs(num)
     finally:
         G.bf16 = saved
+
+
+def test_sincos_f32_bit_identical_to_c_oracle(oracle):
+    """The numpy float32 restatement of the device sine / cosine equals the C
+    oracle's o_sinf / o_cosf bit for bit (the C functions are pinned to the device
+    in the GPU math test)."""
+    from oracle import guide_ref as G
+    rng = np.random.default_rng(3)
+    x = np.concatenate([np.linspace(0, 3.2, 20001, dtype=np.float32), rng.uniform(0, 3.2, 20000).astype(np.float32)])
+    s, c = G.sincos_f32(x)
+    want_s = np.array([oracle.lib.o_sinf(float(v)) for v in x], np.float32)
+    want_c = np.array([oracle.lib.o_cosf(float(v)) for v in x], np.float32)
+    assert np.array_equal(s.view(np.int32), want_s.view(np.int32))
+    assert np.array_equal(c.view(np.int32), want_c.view(np.int32))
+
+
+def test_frequency_features_close_to_exact():
+    """Features from the base sincos and the doubling recurrences stay within the
+    error that doubling allows of sin / cos(2^f pi x) evaluated in FP64 (the input
+    product x * pi is rounded to float32 first, which alone is worth up to
+    2^f * 2.4e-7 at f)."""
+    from oracle import guide_ref as G
+    x = np.random.default_rng(4).uniform(0, 1, (4000, 9)).astype(np.float32)
+    F = G.features_f32(x).reshape(4000, 9, 12, 2)
+    for f in range(12):
+        a = x.astype(np.float64) * np.pi * 2.0 ** f
+        err = max(np.abs(F[:, :, f, 0] - np.sin(a)).max(), np.abs(F[:, :, f, 1] - np.cos(a)).max())
+        assert err < 4e-7 * 2.0 ** f + 1e-6, (f, err)

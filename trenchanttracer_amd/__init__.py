@@ -341,6 +341,14 @@ class Guide:
                    last_loss=gs.last_loss)
         return (out, out64, _stats(st), gst) if want_fp64 else (out, _stats(st), gst)
 
+    def sample(self, raw, n, seed=1):
+        """n directions drawn from the mixture of one raw output row (40 floats) by the
+        device sampler, with the mixture density at each: (n, 4) float32."""
+        r = np.ascontiguousarray(raw, np.float32).reshape(40)
+        out = np.empty((int(n), 4), np.float32)
+        check(lib.tt_debug_guide_sample(self.ctx._h, _vp(r), int(n), int(seed), _vp(out)))
+        return out
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             lib.tt_guide_destroy(self._h)

@@ -139,6 +139,7 @@ PROTOS = {
     "tt_guide_forward": (C.c_int, [P, P, I64, P]),
     "tt_guide_grad": (C.c_int, [P, P, P, P, I64, PF, P]),
     "tt_guide_train_step": (C.c_int, [P, P, P, P, I64, PF]),
+    "tt_debug_guide_sample": (C.c_int, [P, P, I64, U32, P]),
     "tt_guide_render": (C.c_int, [P, P, C.POINTER(tt_camera), C.POINTER(tt_render_params), C.POINTER(tt_guide_params),
                                   P, P, C.POINTER(tt_render_stats), C.POINTER(tt_guide_stats)]),
     "tt_batch_create": (C.c_int, [P, I32, P, P, P, P, P, I32, I32, F32, C.POINTER(P)]),

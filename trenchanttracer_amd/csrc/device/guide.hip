@@ -24,6 +24,7 @@
 #include "../host/tt_host.h"
 #include "guide_dev.h"
 #include "scene_dev.h"
+#include "ttmath.h"
 
 namespace tt {
 namespace {
@@ -36,21 +37,49 @@ __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
 }
 
 // ------------------------------------------------------------------ encoding
+// Frequency features of one input x: sin / cos(2^f pi x), f = 0..11. The base
+// pair comes from the bit-reproducible tm_sincos(pi x); each next frequency from
+// the double-angle recurrences s' = (2 s) c, c' = (c - s)(c + s) in float (no
+// contraction), so the features are a fixed function of x that a host
+// restatement reproduces bit for bit. Each doubling at most doubles the error
+// (both maps have gradient norm <= 2), so after 11 doublings the absolute error
+// stays near 1e-4, well under the BF16 rounding that follows.
+struct FreqState {
+    float s, c;
+};
+__device__ __forceinline__ FreqState freq_base(float x) {
+    FreqState st;
+    st.s = tm_sincos(x * 3.14159265f, &st.c);
+    return st;
+}
+__device__ __forceinline__ void freq_next(FreqState &st) {
+    const float s2 = (2.0f * st.s) * st.c;
+    st.c = (st.c - st.s) * (st.c + st.s);
+    st.s = s2;
+}
+
+// x[r][k], k = 24 c + 2 f + {0: sin, 1: cos}, zero for k >= 216 and rows >= n:
+// one thread per (row, input); the 9th "input" writes the 8 padding features.
 __global__ __launch_bounds__(256) void k_encode(const float *__restrict__ in, __bf16 *__restrict__ x, int64_t n,
                                                 const int32_t *n_dev) {
     int64_t rows = n_dev ? min((int64_t)*n_dev, n) : n;
     const int64_t rows_pad = (rows + 15) & ~15LL;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < rows_pad * kGuideK1;
+    constexpr int kSlots = kGuideIn + 1;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < rows_pad * kSlots;
          t += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t r = t / kGuideK1;
-        const int k = (int)(t % kGuideK1);
-        float v = 0.0f;
-        if (r < rows && k < kGuideFeat) {
-            const int c = k / (2 * kGuideFreq), f = (k / 2) % kGuideFreq;
-            const float arg = in[r * kGuideIn + c] * (3.14159265358979f * (float)(1 << f));
-            v = (k & 1) ? cosf(arg) : sinf(arg);
+        const int64_t r = t / kSlots;
+        const int c = (int)(t % kSlots);
+        __bf16 *o = x + r * kGuideK1 + 2 * kGuideFreq * c;
+        if (c == kGuideIn) {
+            for (int k = 0; k < kGuideK1 - kGuideFeat; ++k) o[k] = (__bf16)0.0f;
+            continue;
         }
-        x[t] = (__bf16)v;
+        FreqState st = freq_base(r < rows ? in[r * kGuideIn + c] : 0.0f);
+        for (int f = 0; f < kGuideFreq; ++f) {
+            o[2 * f] = (__bf16)(r < rows ? st.s : 0.0f);
+            o[2 * f + 1] = (__bf16)(r < rows ? st.c : 0.0f);
+            freq_next(st);
+        }
     }
 }
 
@@ -96,6 +125,160 @@ __global__ __launch_bounds__(256) void k_gemm(const __bf16 *__restrict__ A, cons
                 else reinterpret_cast<float *>(out)[row * ldo + col] = v;
             }
         }
+    }
+}
+
+// ------------------------------------------------------------------ fused inference
+// raw[r] = MLP(encode(in[r])) for r < rows: encoding, 3 hidden layers and the
+// output layer in one kernel. The block stages all BF16 weights (Bt[out][in],
+// rows padded by 8 elements against LDS bank conflicts) and FP32 biases in LDS
+// once, then its 4 waves take 16-row tiles: features are computed straight into
+// the A operand registers, each layer's output goes through a per-wave LDS tile
+// (D layout -> A layout) and the last layer writes FP32. Arithmetic, rounding
+// points and accumulation order are those of k_encode + k_gemm, so the outputs
+// are identical to the unfused forward pass.
+constexpr int kLd1 = kGuideK1 + 8, kLdH = kGuideH + 8;
+constexpr int kInferLds = 2 * (kGuideH * kLd1 + 2 * kGuideH * kLdH + kGuideNOutPad16 * kLdH + 4 * 16 * kLdH) +
+                          4 * (3 * kGuideH + kGuideNOutPad16);
+
+struct InferArgs {
+    const float *in;
+    const __bf16 *w[4];
+    const float *bias[4];
+    float *raw;
+    int64_t n;
+    const int32_t *n_dev;
+};
+
+template <int K, int LD>
+__device__ __forceinline__ void stage_weights(__bf16 *dst, const __bf16 *src, int rows) {
+    // rows x K bf16 from global (row-major, K contiguous) into LDS rows of stride LD
+    const int vec = rows * K / 8;
+    for (int t = threadIdx.x; t < vec; t += blockDim.x) {
+        const int r = t / (K / 8), c = t % (K / 8);
+        *reinterpret_cast<bf16x8 *>(dst + r * LD + 8 * c) = *reinterpret_cast<const bf16x8 *>(src + (int64_t)r * K + 8 * c);
+    }
+}
+
+// hidden layer from the per-wave activation tile: acc[t] = act (16 x 64) . W^T
+template <int NT>
+__device__ __forceinline__ void hidden_mfma(const __bf16 *act, const __bf16 *w, int lr, int lq, f32x4 *acc) {
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < kGuideH / 32; ++s) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(act + lr * kLdH + 32 * s + 8 * lq);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const bf16x8 b = *reinterpret_cast<const bf16x8 *>(w + (16 * t + lr) * kLdH + 32 * s + 8 * lq);
+            acc[t] = mfma(a, b, acc[t]);
+        }
+    }
+}
+
+// bias + ReLU, BF16, into the per-wave activation tile (D layout: row 4 lq + j, column 16 t + lr)
+__device__ __forceinline__ void store_hidden(__bf16 *act, const f32x4 *acc, const float *bias, int lr, int lq) {
+#pragma unroll
+    for (int t = 0; t < kGuideH / 16; ++t)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            act[(4 * lq + j) * kLdH + 16 * t + lr] = (__bf16)fmaxf(acc[t][j] + bias[16 * t + lr], 0.0f);
+}
+
+__global__ __launch_bounds__(256) void k_mlp_infer(InferArgs a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __bf16 *w1 = reinterpret_cast<__bf16 *>(lds_raw);
+    __bf16 *w2 = w1 + kGuideH * kLd1;
+    __bf16 *w3 = w2 + kGuideH * kLdH;
+    __bf16 *w4 = w3 + kGuideH * kLdH;
+    __bf16 *acts = w4 + kGuideNOutPad16 * kLdH;
+    float *b1 = reinterpret_cast<float *>(acts + 4 * 16 * kLdH);
+    float *b2 = b1 + kGuideH, *b3 = b2 + kGuideH, *b4 = b3 + kGuideH;
+    stage_weights<kGuideK1, kLd1>(w1, a.w[0], kGuideH);
+    stage_weights<kGuideH, kLdH>(w2, a.w[1], kGuideH);
+    stage_weights<kGuideH, kLdH>(w3, a.w[2], kGuideH);
+    stage_weights<kGuideH, kLdH>(w4, a.w[3], kGuideNOutPad16);
+    for (int t = threadIdx.x; t < kGuideH; t += blockDim.x) {
+        b1[t] = a.bias[0][t];
+        b2[t] = a.bias[1][t];
+        b3[t] = a.bias[2][t];
+    }
+    for (int t = threadIdx.x; t < kGuideNOutPad16; t += blockDim.x) b4[t] = a.bias[3][t];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lr = lane & 15, lq = lane >> 4;
+    __bf16 *act = acts + wave * 16 * kLdH;
+    const int64_t rows = a.n_dev ? min((int64_t)*a.n_dev, a.n) : a.n;
+    const int64_t tiles = (rows + 15) >> 4;
+    // whole waves only ever touch their own activation tile, so no block barrier is
+    // needed inside the loop; the wave barrier orders the tile's LDS writes and reads
+    for (int64_t tile = (int64_t)blockIdx.x * 4 + wave; tile < tiles; tile += (int64_t)gridDim.x * 4) {
+        const int64_t r = tile * 16 + lr;
+        float x[kGuideIn];
+#pragma unroll
+        for (int c = 0; c < kGuideIn; ++c) x[c] = r < rows ? a.in[r * kGuideIn + c] : 0.0f;
+        f32x4 acc[kGuideH / 16];
+#pragma unroll
+        for (int t = 0; t < kGuideH / 16; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < kGuideK1 / 32; ++s) {
+            // features k = 32 s + 8 lq + j: one input c, frequencies f0 .. f0 + 3, sin / cos pairs
+            const int k0 = 32 * s + 8 * lq;
+            bf16x8 av;
+            if (k0 < kGuideFeat && r < rows) {
+                const int c = k0 / (2 * kGuideFreq), f0 = (k0 / 2) % kGuideFreq;
+                float xc = x[0];
+#pragma unroll
+                for (int q = 1; q < kGuideIn; ++q)
+                    if (q == c) xc = x[q];
+                FreqState st = freq_base(xc);
+                for (int f = 0; f < f0; ++f) freq_next(st);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    av[2 * j] = (__bf16)st.s;
+                    av[2 * j + 1] = (__bf16)st.c;
+                    freq_next(st);
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) av[j] = (__bf16)0.0f;
+            }
+#pragma unroll
+            for (int t = 0; t < kGuideH / 16; ++t) {
+                const bf16x8 b = *reinterpret_cast<const bf16x8 *>(w1 + (16 * t + lr) * kLd1 + 32 * s + 8 * lq);
+                acc[t] = mfma(av, b, acc[t]);
+            }
+        }
+        store_hidden(act, acc, b1, lr, lq);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        hidden_mfma<kGuideH / 16>(act, w2, lr, lq, acc);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        store_hidden(act, acc, b2, lr, lq);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        hidden_mfma<kGuideH / 16>(act, w3, lr, lq, acc);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        store_hidden(act, acc, b3, lr, lq);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        f32x4 out[kGuideNOutPad16 / 16];
+        hidden_mfma<kGuideNOutPad16 / 16>(act, w4, lr, lq, out);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int t = 0; t < kGuideNOutPad16 / 16; ++t)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                a.raw[(tile * 16 + 4 * lq + j) * kGuideNOutPad16 + 16 * t + lr] = out[t][j] + b4[16 * t + lr];
     }
 }
 
@@ -172,16 +355,49 @@ __global__ __launch_bounds__(256) void k_sum(const float *__restrict__ v, int64_
 }
 
 // ------------------------------------------------------------------ weight gradients
-// partial[blk][i][o] = sum over the block's rows n of H[n][i] G[n][o]  (KI x KO)
-// Rows are staged 32 at a time in LDS, transposed so that each lane's 8 values
-// along the row dimension are contiguous.
+// partial[blk][i][o] = sum over the block's rows r of H[r][i] G[r][o]  (KI x KO) and
+// pbias[blk][o] = sum over the block's rows of G[r][o]. Rows are staged kWgRows at
+// a time in LDS, transposed ([feature][row]) so that each lane's 8 operand values
+// along the row (reduction) dimension are 16 contiguous bytes: every thread loads
+// 8 rows x 8 features with 16-byte loads, transposes the 8 x 8 block in registers
+// and writes 8 16-byte rows.
+constexpr int kWgRows = 64;
+
+template <int KF>
+__device__ __forceinline__ void stage_transposed(__bf16 (*dst)[kWgRows + 8], const __bf16 *__restrict__ src, int ld,
+                                                 int64_t rb, int64_t r1) {
+    constexpr int FG = KF / 8, TASKS = (kWgRows / 8) * FG;
+    for (int task = threadIdx.x; task < TASKS; task += blockDim.x) {
+        const int rg = task / FG, fg = task % FG;
+        bf16x8 v[8];
+#pragma unroll
+        for (int rr = 0; rr < 8; ++rr) {
+            const int64_t row = rb + 8 * rg + rr;
+            if (row < r1) {
+                v[rr] = *reinterpret_cast<const bf16x8 *>(src + row * ld + 8 * fg);
+            } else {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) v[rr][q] = (__bf16)0.0f;
+            }
+        }
+#pragma unroll
+        for (int ff = 0; ff < 8; ++ff) {
+            bf16x8 w;
+#pragma unroll
+            for (int rr = 0; rr < 8; ++rr) w[rr] = v[rr][ff];
+            *reinterpret_cast<bf16x8 *>(&dst[8 * fg + ff][8 * rg]) = w;
+        }
+    }
+}
+
 template <int KI, int KO>
 __global__ __launch_bounds__(256) void k_wgrad(const __bf16 *__restrict__ H, int ldh, const __bf16 *__restrict__ G,
-                                               int ldg, int64_t n, int64_t rows_per_block, float *__restrict__ partial) {
+                                               int ldg, int64_t n, int64_t rows_per_block, float *__restrict__ partial,
+                                               float *__restrict__ pbias) {
     constexpr int TI = KI / 16, TO = KO / 16, TILES = TI * TO;
     constexpr int PER_WAVE = (TILES + 3) / 4;
-    __shared__ __align__(16) __bf16 hs[KI][40];  // [feature][row], 32 rows + pad
-    __shared__ __align__(16) __bf16 gs[KO][40];
+    __shared__ __align__(16) __bf16 hs[KI][kWgRows + 8];  // [feature][row]
+    __shared__ __align__(16) __bf16 gs[KO][kWgRows + 8];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lr = lane & 15, lq = lane >> 4;
     const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
@@ -189,25 +405,25 @@ __global__ __launch_bounds__(256) void k_wgrad(const __bf16 *__restrict__ H, int
     f32x4 acc[PER_WAVE];
 #pragma unroll
     for (int q = 0; q < PER_WAVE; ++q) acc[q] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int64_t rb = r0; rb < r1; rb += 32) {
+    float bsum = 0.0f;
+    for (int64_t rb = r0; rb < r1; rb += kWgRows) {
         __syncthreads();
-        for (int t = threadIdx.x; t < 32 * KI; t += blockDim.x) {
-            const int rr = t / KI, i = t % KI;
-            hs[i][rr] = (rb + rr < r1) ? H[(rb + rr) * ldh + i] : (__bf16)0.0f;
-        }
-        for (int t = threadIdx.x; t < 32 * KO; t += blockDim.x) {
-            const int rr = t / KO, o = t % KO;
-            gs[o][rr] = (rb + rr < r1) ? G[(rb + rr) * ldg + o] : (__bf16)0.0f;
-        }
+        stage_transposed<KI>(hs, H, ldh, rb, r1);
+        stage_transposed<KO>(gs, G, ldg, rb, r1);
         __syncthreads();
+        if (threadIdx.x < KO)
+            for (int rr = 0; rr < kWgRows; ++rr) bsum += (float)gs[threadIdx.x][rr];
 #pragma unroll
         for (int q = 0; q < PER_WAVE; ++q) {
             const int tile = wave * PER_WAVE + q;
             if (tile < TILES) {
                 const int ti = tile / TO, to = tile % TO;
-                const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&hs[16 * ti + lr][8 * lq]);
-                const bf16x8 b = *reinterpret_cast<const bf16x8 *>(&gs[16 * to + lr][8 * lq]);
-                acc[q] = mfma(a, b, acc[q]);
+#pragma unroll
+                for (int s2 = 0; s2 < kWgRows / 32; ++s2) {
+                    const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&hs[16 * ti + lr][32 * s2 + 8 * lq]);
+                    const bf16x8 b = *reinterpret_cast<const bf16x8 *>(&gs[16 * to + lr][32 * s2 + 8 * lq]);
+                    acc[q] = mfma(a, b, acc[q]);
+                }
             }
         }
     }
@@ -221,17 +437,7 @@ __global__ __launch_bounds__(256) void k_wgrad(const __bf16 *__restrict__ H, int
             for (int j = 0; j < 4; ++j) out[(16 * ti + 4 * lq + j) * KO + 16 * to + lr] = acc[q][j];
         }
     }
-}
-
-// partial[blk][o] = sum over the block's rows of G[n][o] (bias gradients)
-__global__ __launch_bounds__(256) void k_colsum(const __bf16 *__restrict__ G, int ldg, int ko, int64_t n,
-                                                int64_t rows_per_block, float *__restrict__ partial) {
-    const int64_t r0 = (int64_t)blockIdx.x * rows_per_block, r1 = min(n, r0 + rows_per_block);
-    for (int o = threadIdx.x; o < ko; o += blockDim.x) {
-        float s = 0.0f;
-        for (int64_t r = r0; r < r1; ++r) s += (float)G[r * ldg + o];
-        partial[(int64_t)blockIdx.x * ko + o] = s;
-    }
+    if (threadIdx.x < KO) pbias[(int64_t)blockIdx.x * KO + threadIdx.x] = bsum;
 }
 
 // grad[offset + e] = sum_b partial[b][e] for e < count (deterministic order)
@@ -355,6 +561,24 @@ hipError_t guide_forward(tt_guide *g, int64_t n, const int32_t *n_dev, hipStream
     return hipGetLastError();
 }
 
+// Inference over the first n rows of g->batch.in (device count n_dev optional):
+// the fused kernel, raw outputs (fp32, 48 wide) into g->batch.raw.
+hipError_t guide_infer(tt_guide *g, int64_t n, const int32_t *n_dev, hipStream_t s) {
+    InferArgs a{};
+    a.in = g->batch.in;
+    for (int k = 0; k < 4; ++k) {
+        a.w[k] = g->wf[k];
+        a.bias[k] = g->params + layer_offset(k) + (int64_t)kGuideLayers[k].in * kGuideLayers[k].out;
+    }
+    a.raw = g->batch.raw;
+    a.n = n;
+    a.n_dev = n_dev;
+    const int64_t tiles = (n + 15) / 16;
+    const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((tiles + 3) / 4, 2LL * std::max(1, g->ctx->num_cus)));
+    hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)grid), dim3(256), kInferLds, s, a);
+    return hipGetLastError();
+}
+
 // Gradient of the loss over rows [0, n) of the batch (inputs, dirs, wts staged).
 static hipError_t guide_backward(tt_guide *g, int64_t n, hipStream_t s) {
     GuideBatch &b = g->batch;
@@ -373,17 +597,21 @@ static hipError_t guide_backward(tt_guide *g, int64_t n, hipStream_t s) {
                        (const float *)nullptr, (void *)b.dh[1], kGuideH, (const __bf16 *)b.h[1], R, (const int32_t *)nullptr);
     hipLaunchKernelGGL((k_gemm<kGuideH, kGuideH, false, true>), dim3(gw), dim3(256), 0, s, b.dh[1], g->wb[1],
                        (const float *)nullptr, (void *)b.dh[0], kGuideH, (const __bf16 *)b.h[0], R, (const int32_t *)nullptr);
-    // weight gradients: dW_k = H_{k-1}^T dH_k, partial sums per block, reduced in order
-    const int64_t rpb = 2048;
+    // weight gradients: dW_k = H_{k-1}^T dH_k and bias gradients, partial sums per
+    // block (at most kWgBlocks blocks of whole kWgRows chunks), reduced in block order
+    constexpr int64_t kWgBlocks = 1024;
+    const int64_t rpb = std::max<int64_t>(kWgRows, ((R + kWgBlocks - 1) / kWgBlocks + kWgRows - 1) / kWgRows * kWgRows);
     const int nb = (int)((R + rpb - 1) / rpb);
-    if (g->partial_cap < (int64_t)nb * kGuideK1 * kGuideH) {
+    const int64_t need = (int64_t)nb * (kGuideK1 * kGuideH + kGuideNOutPad16 + kGuideH);
+    if (g->partial_cap < need) {
         if (g->partial) hipFree(g->partial);
         g->partial = nullptr;
         g->partial_cap = 0;
-        if (dev_alloc(g->ctx, (void **)&g->partial, (size_t)nb * kGuideK1 * kGuideH * 4, "guide partials") != TT_OK)
+        if (dev_alloc(g->ctx, (void **)&g->partial, (size_t)need * 4, "guide partials") != TT_OK)
             return hipErrorOutOfMemory;
-        g->partial_cap = (int64_t)nb * kGuideK1 * kGuideH;
+        g->partial_cap = need;
     }
+    float *pbias = g->partial + (int64_t)nb * kGuideK1 * kGuideH;
     struct L {
         const __bf16 *h;
         int ldh;
@@ -398,19 +626,18 @@ static hipError_t guide_backward(tt_guide *g, int64_t n, hipStream_t s) {
         const L &l = ls[k];
         if (k == 0)
             hipLaunchKernelGGL((k_wgrad<kGuideK1, kGuideH>), dim3(nb), dim3(256), 0, s, l.h, l.ldh, l.d, l.ldd, R, rpb,
-                               g->partial);
+                               g->partial, pbias);
         else if (k < 3)
             hipLaunchKernelGGL((k_wgrad<kGuideH, kGuideH>), dim3(nb), dim3(256), 0, s, l.h, l.ldh, l.d, l.ldd, R, rpb,
-                               g->partial);
+                               g->partial, pbias);
         else
             hipLaunchKernelGGL((k_wgrad<kGuideH, kGuideNOutPad16>), dim3(nb), dim3(256), 0, s, l.h, l.ldh, l.d, l.ldd,
-                               R, rpb, g->partial);
+                               R, rpb, g->partial, pbias);
         const int cnt = l.ki * l.ko;
-        hipLaunchKernelGGL(k_reduce, dim3((cnt + 255) / 256), dim3(256), 0, s, g->partial, nb, cnt, g->grad,
-                           (int)layer_offset(k), true, l.ki, l.ko);
-        hipLaunchKernelGGL(k_colsum, dim3(nb), dim3(64), 0, s, l.d, l.ldd, l.ko, R, rpb, g->partial);
-        hipLaunchKernelGGL(k_reduce, dim3((l.ko + 255) / 256), dim3(256), 0, s, g->partial, nb, l.ko, g->grad,
-                           (int)(layer_offset(k) + cnt), false, l.ki, l.ko);
+        hipLaunchKernelGGL(k_reduce, dim3((cnt + 255) / 256), dim3(256), 0, s, (const float *)g->partial, nb, cnt,
+                           g->grad, (int)layer_offset(k), true, l.ki, l.ko);
+        hipLaunchKernelGGL(k_reduce, dim3((l.ko + 255) / 256), dim3(256), 0, s, (const float *)pbias, nb, l.ko,
+                           g->grad, (int)(layer_offset(k) + cnt), false, l.ki, l.ko);
     }
     return hipGetLastError();
 }
@@ -540,7 +767,7 @@ tt_status tt_guide_forward(tt_guide *g, const float *inputs, int64_t n, float *r
     if (st != TT_OK) return st;
     hipStream_t s = g->ctx->stream;
     TT_HIPG(hipMemcpyAsync(g->batch.in, inputs, (size_t)n * kGuideIn * 4, hipMemcpyHostToDevice, s), "copy inputs");
-    TT_HIPG(guide_forward(g, n, nullptr, s), "guide forward");
+    TT_HIPG(guide_infer(g, n, nullptr, s), "guide inference");
     TT_HIPG(hipMemcpy2DAsync(raw_out, kGuideNOut * 4, g->batch.raw, kGuideNOutPad16 * 4, kGuideNOut * 4, (size_t)n,
                              hipMemcpyDeviceToHost, s), "copy out");
     TT_HIPG(hipStreamSynchronize(s), "sync");

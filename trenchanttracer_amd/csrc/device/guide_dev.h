@@ -111,6 +111,7 @@ __device__ __forceinline__ void mixture_sample(const Lobes &L, float u0, float u
 }
 
 hipError_t guide_forward(tt_guide *g, int64_t n, const int32_t *n_dev, hipStream_t s);
+hipError_t guide_infer(tt_guide *g, int64_t n, const int32_t *n_dev, hipStream_t s);
 hipError_t guide_train_rows(tt_guide *g, int64_t n, hipStream_t s);
 tt_status guide_alloc_batch(tt_guide *g, int64_t rows);
 

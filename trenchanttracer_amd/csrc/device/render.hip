@@ -178,7 +178,8 @@ __global__ __launch_bounds__(256) void k_generate(GenArgs a) {
     a.path[p] = p;
     a.root[p] = a.tab->root[sid];
     a.thr[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    if (a.L) a.L[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    // reference mode: path radiance; path mode: L.x = Russian-roulette weight
+    if (a.L) a.L[p] = make_float4(a.mode == TT_RENDER_PATH ? 1.0f : 0.0f, 0.0f, 0.0f, 0.0f);
     if (a.flags) a.flags[p] = 0;
 }
 
@@ -690,8 +691,8 @@ __global__ __launch_bounds__(256) void k_shade_path(ShadeArgs a) {
                 const float alpha = guided ? a.gd.alpha : 0.0f;
                 // next-event estimation
                 if (!smooth && (p_env > 0.0f || p_emit > 0.0f) && wol.z > 0.0f) {
-                    f3 wi, le;
-                    float pdf_l = 0.0f, dist = 1e20f;
+                    f3 wi, le, ylight = F3(0, 0, 0);
+                    float pdf_l = 0.0f;
                     bool pick_env = ul[0] <= p_env;
                     float uu[4] = {pick_env ? ul[0] / p_env : (ul[0] - p_env) / p_emit, ul[1], ul[2], ul[3]};
                     uu[0] = fminf(fmaxf(uu[0], 1e-7f), 1.0f);
@@ -707,8 +708,8 @@ __global__ __launch_bounds__(256) void k_shade_path(ShadeArgs a) {
                         f3 nl = xyz(sv.tri_geo[t]);
                         f3 dv = y - point;
                         float d2 = dot(dv, dv);
-                        dist = sqrtf(d2);
-                        wi = dv * (1.0f / dist);
+                        wi = dv * (1.0f / sqrtf(d2));
+                        ylight = y;
                         float cos_l = -dot(wi, nl);
                         const tt_material lm = sv.mats[sv.tri_mat[t]];
                         le = F3(lm.emission[0], lm.emission[1], lm.emission[2]);
@@ -722,8 +723,19 @@ __global__ __launch_bounds__(256) void k_shade_path(ShadeArgs a) {
                         if (maxc(f_cos) > 0.0f) {
                             float w = power_heuristic(pdf_l, pdf_b);
                             scontrib = thr * f_cos * le * (w / pdf_l);
+                            // shadow ray from the offset origin; towards an emitter it aims at
+                            // the sampled point and stops short of it by a relative 1e-3 of
+                            // its own length (measured from the offset origin, so that the
+                            // offset cannot carry the emitter inside the tested interval)
                             f3 so = point + ngf * eps;
-                            sr = {so.x, so.y, so.z, 0.0f, wi.x, wi.y, wi.z, pick_env ? 1e20f : dist * (1.0f - 2e-4f)};
+                            if (pick_env) {
+                                sr = {so.x, so.y, so.z, 0.0f, wi.x, wi.y, wi.z, 1e20f};
+                            } else {
+                                f3 ds = ylight - so;
+                                float ls = sqrtf(dot(ds, ds));
+                                ds = ds * (1.0f / ls);
+                                sr = {so.x, so.y, so.z, 0.0f, ds.x, ds.y, ds.z, ls * (1.0f - 1e-3f)};
+                            }
                             shadow = maxc(scontrib) > 0.0f;
                         }
                     }
@@ -733,6 +745,7 @@ __global__ __launch_bounds__(256) void k_shade_path(ShadeArgs a) {
                 float pdf_b;
                 bool delta;
                 bool sampled;
+                float w_rr;  // BSDF-sampling weight of the direction (max component), for roulette
                 if (guided) {
                     float ug[4];
                     draws((uint32_t)local, smp, (uint32_t)(a.bounce + 1), 3u, a.seed, ug);
@@ -751,9 +764,11 @@ __global__ __launch_bounds__(256) void k_shade_path(ShadeArgs a) {
                     const float pdf_c = alpha * mixture_pdf(lobes, wiw.x, wiw.y, wiw.z) + (1.0f - alpha) * pdf_b;
                     sampled = pdf_c > 0.0f && maxc(f_cos) > 0.0f && wol.z > 0.0f;
                     weight = sampled ? f_cos * (1.0f / pdf_c) : F3(0, 0, 0);
+                    w_rr = sampled && pdf_b > 0.0f ? maxc(f_cos) / pdf_b : 0.0f;
                     pdf_b = pdf_c;
                 } else {
                     sampled = wol.z > 0.0f && bsdf_sample(m, wol, front, ub, &wil, &weight, &pdf_b, &delta);
+                    w_rr = maxc(weight);
                 }
                 if (sampled) {
                     f3 wi = to_world(fr, wil);
@@ -763,16 +778,26 @@ __global__ __launch_bounds__(256) void k_shade_path(ShadeArgs a) {
                     if (ok) {
                         thr = thr * weight;
                         bool alive = maxc(thr) > 0.0f;
+                        // roulette: survival min(0.95, max throughput); guided renders use
+                        // the product of BSDF-sampling weights instead, so that a direction
+                        // the guide favours (small f cos / pdf, large expected radiance) is
+                        // not culled for its small throughput
+                        float rr = a.L[p].x * w_rr;
                         if (alive && a.bounce + 1 >= a.rr_depth) {
-                            float q = fminf(0.95f, maxc(thr));
-                            if (ub[3] > q) alive = false;
-                            else thr = thr * (1.0f / q);
+                            float q = fminf(0.95f, a.gd.raw ? rr : maxc(thr));
+                            if (ub[3] > q) {
+                                alive = false;
+                            } else {
+                                thr = thr * (1.0f / q);
+                                rr = rr / q;
+                            }
                         }
                         if (alive) {
                             f3 no = point + (side > 0.0f ? ngf : ngf * -1.0f) * eps;
                             nr = {no.x, no.y, no.z, 0.0f, wi.x, wi.y, wi.z, 1e20f};
                             cont = true;
                             a.thr[p] = make_float4(thr.x, thr.y, thr.z, pdf_b);
+                            a.L[p] = make_float4(rr, 0.0f, 0.0f, 0.0f);
                             a.flags[p] = delta ? 1 : 0;
                             if (rec_b && !delta && pdf_b > 0.0f) {
                                 guide_inputs(a.gd, point, ns, wo, rec_b);
@@ -855,7 +880,10 @@ __global__ __launch_bounds__(256) void k_apply_shadow(ApplyArgs a) {
 }
 
 // Turns the recorded vertices of the training paths into training rows
-// {inputs (9), wi (3), luminance(L_i) / pdf_i} and clears the records.
+// {inputs (9), wi (3), luminance(L_i) |cos theta_i| / pdf_i} and clears the
+// records: the network learns the cosine-weighted incident radiance, the
+// product that a diffuse vertex integrates (the shading normal is recovered
+// from the normal inputs).
 __global__ __launch_bounds__(256) void k_collect(float *rec, int32_t ntrain, int32_t max_depth, float *buf,
                                                  int32_t *count, int32_t cap) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -863,7 +891,9 @@ __global__ __launch_bounds__(256) void k_collect(float *rec, int32_t ntrain, int
     float *r = rec + e * kRecFloats;
     const float pdf = r[12];
     if (pdf > 0.0f) {
-        const float lum = 0.2126f * r[16] + 0.7152f * r[17] + 0.0722f * r[18];
+        const float nx = 2.0f * r[3] - 1.0f, ny = 2.0f * r[4] - 1.0f, nz = 2.0f * r[5] - 1.0f;
+        const float cosw = fabsf(nx * r[9] + ny * r[10] + nz * r[11]);
+        const float lum = (0.2126f * r[16] + 0.7152f * r[17] + 0.0722f * r[18]) * cosw;
         if (lum > 0.0f && isfinite(lum)) {
             const int slot = atomicAdd(count, 1);
             if (slot < cap) {
@@ -950,6 +980,21 @@ __global__ void k_debug_env(SceneView sv, int64_t n, uint32_t seed, float *out) 
     o[5] = le.y;
     o[6] = le.z;
     o[7] = pe;
+}
+
+__global__ void k_debug_guide(const float *raw, int64_t n, uint32_t seed, float *out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Lobes lobes;
+    decode_lobes(raw, lobes);
+    float u[4], d[3];
+    draws((uint32_t)i, (uint32_t)(i >> 32), 0x0F0Bu, 3u, seed, u);
+    mixture_sample(lobes, u[1], u[2], u[3], d);
+    float *o = out + 4 * i;
+    o[0] = d[0];
+    o[1] = d[1];
+    o[2] = d[2];
+    o[3] = mixture_pdf(lobes, d[0], d[1], d[2]);
 }
 
 __global__ void k_debug_math(int op, const float *x, const float *y, float *out, int64_t n) {
@@ -1082,7 +1127,7 @@ hipError_t launch_bounce(tt_scene *s, RenderState *rs, const tt_render_params *p
         GuideInArgs ga{gr->gd, sv, rs->rays[cur], rs->hits, rs->counts, cur, gr->g->batch.in};
         hipLaunchKernelGGL(k_guide_inputs, dim3(gblocks), dim3(256), 0, hs, ga);
         e = hipGetLastError();
-        if (e == hipSuccess) e = guide_forward(gr->g, cap, rs->counts + cur, hs);
+        if (e == hipSuccess) e = guide_infer(gr->g, cap, rs->counts + cur, hs);
         if (e != hipSuccess) return e;
         sa.gd = gr->gd;
         sa.gd.raw = gr->g->batch.raw;
@@ -1662,6 +1707,26 @@ tt_status tt_debug_env_sample(tt_scene *s, int64_t n, uint32_t seed, float *out)
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     hipFree(d_out);
     if (e != hipSuccess) return fail(TT_ERR_DEVICE, std::string("tt_debug_env_sample: ") + hipGetErrorString(e));
+    return TT_OK;
+}
+
+tt_status tt_debug_guide_sample(tt_ctx *ctx, const float *raw, int64_t n, uint32_t seed, float *out) {
+    if (!ctx || !raw || !out || n <= 0 || n > (1LL << 26)) return fail(TT_ERR_INVALID, "tt_debug_guide_sample: bad arguments");
+    TT_HIPR(hipSetDevice(ctx->device), "hipSetDevice");
+    float *d_buf = nullptr;
+    tt_status st = dev_alloc(ctx, (void **)&d_buf, (size_t)n * 16 + 256, "debug guide");
+    if (st != TT_OK) return st;
+    float *d_raw = d_buf + 4 * n;
+    hipError_t e = hipMemcpyAsync(d_raw, raw, kGuideNOut * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_debug_guide, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                           (const float *)d_raw, n, seed, d_buf);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d_buf, (size_t)n * 16, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    hipFree(d_buf);
+    if (e != hipSuccess) return fail(TT_ERR_DEVICE, std::string("tt_debug_guide_sample: ") + hipGetErrorString(e));
     return TT_OK;
 }
 

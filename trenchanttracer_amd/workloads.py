@@ -181,11 +181,12 @@ def _rect(axis, c, a0, a1, b0, b1, sign, mat):
 def door_room(width=256, height=256, emission=60.0):
     """Indirect-lighting case for guiding: a closed room x in [-2, 2], y in [0, 2],
     z in [-2, 2] (walls facing inwards) split by a partition at x = 1 with a door
-    (|z| < 0.2, y < 1.2); the only light is a ceiling panel (x in [1.2, 1.8],
-    z in [-1.6, -0.8], facing down) behind the partition, so the camera's part of
-    the room is lit through the door. Materials: 0 walls grey diffuse 0.7,
-    1 light (albedo 0, the given emission), 2 partition diffuse 0.6. Returns
-    (mesh, materials, camera)."""
+    (|z| < 0.2, y < 1.2). The only light is a panel just behind the partition
+    (x = 1.02, y in [1.3, 1.9], z in [-1.5, 1.5]) facing +x, away from the door:
+    no light passes the door directly, so the camera's part of the room sees only
+    what the lit far wall of the light's part reflects through the door.
+    Materials: 0 walls grey diffuse 0.7, 1 light (albedo 0, the given emission),
+    2 partition diffuse 0.6. Returns (mesh, materials, camera)."""
     from . import MAT_DIFFUSE, camera, material
     parts = [
         _rect(1, 0.0, -2, 2, -2, 2, +1, 0),    # floor (y = 0, up): axes z, x
@@ -197,7 +198,7 @@ def door_room(width=256, height=256, emission=60.0):
         _rect(0, 1.0, 0, 2, -2, -0.2, -1, 2),  # partition, left of the door
         _rect(0, 1.0, 0, 2, 0.2, 2, -1, 2),    # partition, right of the door
         _rect(0, 1.0, 1.2, 2, -0.2, 0.2, -1, 2),  # lintel above the door
-        _rect(1, 1.98, -1.6, -0.8, 1.2, 1.8, -1, 1),  # light: axes z, x
+        _rect(0, 1.02, 1.3, 1.9, -1.5, 1.5, +1, 1),  # light facing +x: axes y, z
     ]
     mesh = parts[0]
     for q in parts[1:]:
