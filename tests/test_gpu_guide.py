@@ -30,8 +30,10 @@ def test_forward_matches_oracle(tt, ctx):
     ref, _ = G.forward(p, inp)
     err = np.abs(raw - ref)
     _say(f"forward: max abs err {err.max():.3g}, mean {err.mean():.3g}, ref scale {np.abs(ref).mean():.3g}")
-    assert err.max() < 2e-2 * max(1.0, np.abs(ref).max())
-    assert err.mean() < 2e-3 * max(1.0, np.abs(ref).mean())
+    # features are bit-identical to the oracle's (same sincos and recurrence), so
+    # only the FP32 accumulation order of the MFMA products differs
+    assert err.max() < 1e-4 * max(1.0, np.abs(ref).max())
+    assert err.mean() < 1e-5 * max(1.0, np.abs(ref).mean())
     g.close()
 
 

@@ -1,0 +1,149 @@
+"""Guided path tracing on the GPU: the mixture sampler against its FP64 density,
+unbiasedness of the one-sample MIS estimator (closed-form scenes and agreement
+with the unguided tracer) and the variance the trained network removes."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(m):
+    print(m, file=sys.stderr, flush=True)
+
+
+def _raw(mu, logk, logits):
+    r = np.zeros(40, np.float32)
+    for k in range(8):
+        r[3 * k:3 * k + 3] = mu[k]
+        r[24 + k] = logk[k]
+        r[32 + k] = logits[k]
+    return r
+
+
+def _const_guide(g, raw):
+    """Network whose output is `raw` for every input (zero weights, last-layer bias)."""
+    p = np.zeros(g.nparams, np.float32)
+    p[-48:-8] = raw
+    g.set_params(p)
+
+
+def test_mixture_sampler_matches_density(tt, ctx):
+    """Directions drawn by the device's mixture sampler follow the FP64 mixture
+    density of the oracle: the device density at each sample equals the oracle's,
+    chi-square over 512 equal-area bins (bin probabilities by 16 x 16 sub-cell
+    quadrature), and E[1 / q] = 4 pi for a broad mixture."""
+    from oracle import guide_ref as G
+    from scipy.stats import chi2
+    rng = np.random.default_rng(7)
+    g = tt.Guide(ctx, seed=1)
+    cases = [_raw(rng.normal(size=(8, 3)), rng.uniform(-2, 3, 8), rng.normal(size=8)),
+             _raw(rng.normal(size=(8, 3)), rng.uniform(2, 6, 8), rng.normal(size=8)),
+             _raw([(0, 0, 1)] * 4 + [(1, 0, 0)] * 4, [np.log(0.5)] * 4 + [np.log(40)] * 4, [0] * 8)]
+    nb_t, nb_p, sub = 16, 32, 16
+    n = 400000
+    for ci, raw in enumerate(cases):
+        s = g.sample(raw, n, seed=11 + ci)
+        d = s[:, :3].astype(np.float64)
+        assert np.allclose(np.linalg.norm(d, axis=1), 1, atol=1e-5)
+        # device density at the samples equals the oracle's
+        lq = G.mixture_logpdf(np.repeat(raw[None].astype(np.float64), n, 0), d)[0]
+        assert np.allclose(s[:, 3], np.exp(lq), rtol=2e-4, atol=1e-7)
+        inv = np.mean(1.0 / np.exp(lq)) / (4 * np.pi)
+        # bins: z = cos(theta) in nb_t equal slices, phi in nb_p slices (equal area)
+        zi = np.clip(((d[:, 2] + 1) / 2 * nb_t).astype(int), 0, nb_t - 1)
+        ph = np.mod(np.arctan2(d[:, 1], d[:, 0]), 2 * np.pi)
+        pi_ = np.clip((ph / (2 * np.pi) * nb_p).astype(int), 0, nb_p - 1)
+        counts = np.bincount(zi * nb_p + pi_, minlength=nb_t * nb_p)
+        zc = -1 + (np.arange(nb_t * sub) + 0.5) * 2 / (nb_t * sub)
+        pc = (np.arange(nb_p * sub) + 0.5) * 2 * np.pi / (nb_p * sub)
+        Z, PH = np.meshgrid(zc, pc, indexing="ij")
+        R = np.sqrt(1 - Z ** 2)
+        D = np.stack([R * np.cos(PH), R * np.sin(PH), Z], -1).reshape(-1, 3)
+        q = np.exp(G.mixture_logpdf(np.repeat(raw[None].astype(np.float64), len(D), 0), D)[0])
+        cell = (2 / (nb_t * sub)) * (2 * np.pi / (nb_p * sub))
+        prob = (q * cell).reshape(nb_t, sub, nb_p, sub).sum(axis=(1, 3)).ravel()
+        prob /= prob.sum()
+        exp = prob * n
+        keep = exp > 5
+        stat = np.sum((counts[keep] - exp[keep]) ** 2 / exp[keep])
+        p = chi2.sf(stat, keep.sum() - 1)
+        _say(f"case {ci}: chi2 {stat:.1f} dof {keep.sum() - 1} p {p:.3g}, E[1/q]/4pi {inv:.4f}")
+        assert p > 1e-4
+        if ci == 0:   # 1 / q has a heavy tail for sharp lobes: the mean is checked on the broad case
+            assert abs(inv - 1) < 0.02
+    g.close()
+
+
+def test_guided_render_unbiased_in_emissive_box(tt, ctx):
+    """Inside a closed box whose walls have albedo 0.5 and emission 0.5 the radiance
+    is 1 everywhere. Guided renders (alpha 0.5) with constant guides that point one
+    fixed way, sharp (kappa 200) and broad (kappa 5), and with a network trained
+    online from scratch all average to 1. (With alpha near 1 and a sharp wrong
+    guide the estimator stays unbiased but its per-pixel spread approaches 1, too
+    wide for a mean test.)"""
+    from trenchanttracer_amd import workloads as W
+    mesh, mats, cam = W.emissive_box(width=64, height=64)
+    sc = tt.DeviceScene(ctx, mesh, tt.BVH.build(mesh), mats)
+    p = tt.render_params(spp=1024, max_depth=32, rr_depth=4, seed=5, mode=tt.RENDER_PATH, paths_in_flight=65536)
+    g = tt.Guide(ctx, seed=2)
+    for kappa in (200.0, 5.0):
+        _const_guide(g, _raw([(0.3, 0.8, 0.5)] * 8, [np.log(kappa)] * 8, [0] * 8))
+        img, st, gs = g.render(sc, cam, p, alpha=0.5, train=False)
+        _say(f"box, constant guide kappa {kappa}: mean {img.mean():.5f}, pixel std {img.mean(2).std():.3f}")
+        assert abs(img.mean() - 1) < 0.005
+    g2 = tt.Guide(ctx, seed=3)
+    p = tt.render_params(spp=256, max_depth=32, rr_depth=4, seed=3, mode=tt.RENDER_PATH, paths_in_flight=16384)
+    img, st, gs = g2.render(sc, cam, p, alpha=0.5, train=True, train_paths=4096, train_batch=8192)
+    _say(f"box, online-trained guide: mean {img.mean():.5f}, {gs}")
+    assert gs["train_steps"] > 0
+    assert abs(img.mean() - 1) < 0.01
+    g.close()
+    g2.close()
+
+
+@pytest.fixture(scope="module")
+def door(tt, ctx):
+    from trenchanttracer_amd import workloads as W
+    mesh, mats, cam = W.door_room(128, 128)
+    sc = tt.DeviceScene(ctx, mesh, tt.BVH.build(mesh), mats)
+    ref = sc.render(cam, tt.render_params(spp=4096, max_depth=12, rr_depth=4, seed=11, mode=tt.RENDER_PATH,
+                                          fp64=True), want_fp64=True)[1]
+    return sc, cam, ref
+
+
+def test_guided_door_room_agrees_with_plain(tt, ctx, door):
+    """A network trained online (256 spp) and then frozen renders the same image as
+    the unguided tracer: image means within 1.5% of the FP64 4096-spp plain render."""
+    sc, cam, ref = door
+    g = tt.Guide(ctx, seed=1, lr=1e-2)
+    pt = tt.render_params(spp=256, max_depth=12, rr_depth=4, seed=99, mode=tt.RENDER_PATH, paths_in_flight=32768)
+    img, st, gs = g.render(sc, cam, pt, alpha=0.5, train=True, train_paths=16384, train_batch=16384)
+    _say(f"online: mean {img.mean():.5f} ref {ref.mean():.5f} {gs}")
+    assert abs(img.mean() / ref.mean() - 1) < 0.015
+    img, st, gs = g.render(sc, cam, tt.render_params(spp=512, max_depth=12, rr_depth=4, seed=5, mode=tt.RENDER_PATH),
+                           alpha=0.75, train=False)
+    _say(f"frozen: mean {img.mean():.5f}")
+    assert abs(img.mean() / ref.mean() - 1) < 0.015
+    g.close()
+
+
+def test_trained_guide_reduces_error(tt, ctx, door):
+    """The gain of the learned part: at equal samples per pixel the guided render's
+    error against the FP64 4096-spp render is well below the unguided tracer's
+    (light reaches the camera's part of the room only through a door)."""
+    sc, cam, ref = door
+    p64 = tt.render_params(spp=64, max_depth=12, rr_depth=4, seed=5, mode=tt.RENDER_PATH)
+    plain, st = sc.render(cam, p64)
+    mse_plain = np.mean((plain - ref) ** 2)
+    g = tt.Guide(ctx, seed=1, lr=1e-2)
+    pt = tt.render_params(spp=1024, max_depth=12, rr_depth=4, seed=99, mode=tt.RENDER_PATH, paths_in_flight=32768)
+    _, _, gs = g.render(sc, cam, pt, alpha=0.5, train=True, train_paths=32768, train_batch=8192)
+    img, st, _ = g.render(sc, cam, p64, alpha=0.5, train=False)
+    mse_g = np.mean((img - ref) ** 2)
+    _say(f"64 spp MSE: plain {mse_plain:.5g}, guided {mse_g:.5g} (ratio {mse_g / mse_plain:.3f}); "
+         f"training {gs['train_steps']} steps, last loss {gs['last_loss']:.3f}")
+    assert gs["last_loss"] < 2.0          # uniform sphere density: log(4 pi) = 2.53
+    assert mse_g < 0.6 * mse_plain
+    g.close()
