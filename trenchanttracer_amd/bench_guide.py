@@ -1,0 +1,113 @@
+"""Guided-versus-plain measurement. Prints one JSON line.
+
+Equal-time RMSE on the full path-trace case (1,310,720-tri displaced icosphere,
+40/40/20 diffuse/GGX/dielectric, 2048x1024 sky + sun, 1920x1080, depth 16,
+roulette from depth 4) against an FP64-accumulated oracle render of the plain
+tracer (8192 spp by default, a different seed): plain renders at 16..128 spp,
+guided renders trained online from scratch (Adam lr 1e-3, 2^18-row batches) at
+32..128 spp, and renders with the network trained by the 128-spp run and then
+frozen. The time at which guided rendering reaches the plain 64-spp RMSE is read
+off a log-log fit of RMSE against time (RMSE ~ time^-1/2 locally).
+
+The same protocol on the door room (256x256, depth 12), where all light reaches
+the camera's part of the room through a door, with a 4096-spp oracle.
+
+    python -m trenchanttracer_amd.bench_guide [--oracle-spp 8192] [--skip-c5]
+"""
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+
+def log(m):
+    print(f"[guide] {m}", file=sys.stderr, flush=True)
+
+
+def _time_to_rmse(points, target):
+    """Time at which RMSE reaches `target`, from (time, rmse) pairs: log-log
+    interpolation between the bracketing points, or extrapolation with slope -1/2
+    from the nearest one."""
+    pts = sorted(points)
+    for (t0, r0), (t1, r1) in zip(pts, pts[1:]):
+        if r0 >= target >= r1 and r0 > r1:
+            a = (np.log(target) - np.log(r0)) / (np.log(r1) - np.log(r0))
+            return float(np.exp(np.log(t0) + a * (np.log(t1) - np.log(t0))))
+    t, r = min(pts, key=lambda p: abs(np.log(p[1] / target)))
+    return float(t * (r / target) ** 2)
+
+
+def compare(tt, ctx, sc, cam, ref, depth, plain_spp, guided_spp, paths, lr, batch, train_paths, seed=7):
+    def rmse(img):
+        return float(np.sqrt(np.mean((img.astype(np.float64) - ref) ** 2)))
+
+    out = {"plain": [], "guided_online": [], "guided_pretrained": []}
+    for spp in plain_spp:
+        img, st = sc.render(cam, tt.render_params(spp=spp, max_depth=depth, rr_depth=4, seed=seed, mode=tt.RENDER_PATH))
+        out["plain"].append({"spp": spp, "rmse": rmse(img), "seconds": st["seconds"]})
+        log(f"plain {spp}: {out['plain'][-1]}")
+    g = None
+    for spp in guided_spp:
+        if g is not None:
+            g.close()
+        g = tt.Guide(ctx, seed=1, lr=lr)
+        p = tt.render_params(spp=spp, max_depth=depth, rr_depth=4, seed=seed, mode=tt.RENDER_PATH, paths_in_flight=paths)
+        img, st, gs = g.render(sc, cam, p, alpha=0.5, train=True, train_paths=train_paths, train_batch=batch)
+        out["guided_online"].append({"spp": spp, "rmse": rmse(img), "seconds": st["seconds"],
+                                     "train_seconds": gs["train_seconds"], "train_steps": gs["train_steps"],
+                                     "last_loss": gs["last_loss"]})
+        log(f"guided online {spp}: {out['guided_online'][-1]}")
+    for spp in plain_spp:
+        img, st, _ = g.render(sc, cam, tt.render_params(spp=spp, max_depth=depth, rr_depth=4, seed=seed + 1,
+                                                        mode=tt.RENDER_PATH), alpha=0.5, train=False)
+        out["guided_pretrained"].append({"spp": spp, "rmse": rmse(img), "seconds": st["seconds"]})
+        log(f"guided pretrained {spp}: {out['guided_pretrained'][-1]}")
+    g.close()
+    p64 = next(r for r in out["plain"] if r["spp"] == 64)
+    for key in ("guided_online", "guided_pretrained"):
+        t = _time_to_rmse([(r["seconds"], r["rmse"]) for r in out[key]], p64["rmse"])
+        out[f"{key}_time_to_plain64_rmse"] = t
+        out[f"{key}_wall_ratio"] = t / p64["seconds"]
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--oracle-spp", type=int, default=8192)
+    ap.add_argument("--skip-c5", action="store_true")
+    args = ap.parse_args()
+    import trenchanttracer_amd as tt
+    from trenchanttracer_amd import workloads
+
+    ctx = tt.Context(0)
+    result = {"metric": "equal-time RMSE vs FP64 oracle, guided vs plain"}
+
+    mesh, mats, cam = workloads.door_room(256, 256)
+    sc = tt.DeviceScene(ctx, mesh, tt.BVH.build(mesh), mats)
+    _, ref, st = sc.render(cam, tt.render_params(spp=4096, max_depth=12, rr_depth=4, seed=1234, mode=tt.RENDER_PATH,
+                                                 fp64=True), want_fp64=True)
+    log(f"door oracle 4096 spp in {st['seconds']:.2f}s")
+    result["door_room_256"] = compare(tt, ctx, sc, cam, ref, 12, (16, 32, 64, 128), (64, 256, 1024), 131072, 1e-2,
+                                      16384, 65536)
+    result["door_room_256"]["oracle_spp"] = 4096
+    sc.close()
+
+    if not args.skip_c5:
+        mesh, mats, env, cam = workloads.pathtrace()
+        sc = tt.DeviceScene(ctx, mesh, tt.BVH.build(mesh), mats, env=env)
+        t0 = time.time()
+        _, ref, st = sc.render(cam, tt.render_params(spp=args.oracle_spp, max_depth=16, rr_depth=4, seed=1234,
+                                                     mode=tt.RENDER_PATH, fp64=True), want_fp64=True)
+        log(f"C5 oracle {args.oracle_spp} spp in {time.time() - t0:.1f}s")
+        result["pathtrace_1080p"] = compare(tt, ctx, sc, cam, ref, 16, (16, 32, 64, 128), (32, 64, 128),
+                                            2 * 1920 * 1080, 1e-3, 262144, 65536)
+        result["pathtrace_1080p"]["oracle_spp"] = args.oracle_spp
+        sc.close()
+    print(json.dumps(result), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()
