@@ -30,7 +30,7 @@ def main():
     ap.add_argument("--lds-stack", type=int, default=16)
     ap.add_argument("--block", type=int, default=64)
     ap.add_argument("--ntris", type=int, default=1_000_000)
-    ap.add_argument("--cpu-sample-stride", type=int, default=16)
+    ap.add_argument("--cpu-sample-stride", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-render", action="store_true", help="skip the render workloads (secondary metrics)")
     args = ap.parse_args()
@@ -222,7 +222,40 @@ def render_bench(tt, ctx, np, no_cpu):
     out["reference_cornell_256_16spp"] = c1
     log(f"reference cornell: {c1}")
     s.close()
+    out["guided_door_room_256"] = guided_bench(tt, ctx, np)
     return out
+
+
+def guided_bench(tt, ctx, np):
+    """Learned guiding where light reaches the camera through a door (door room,
+    256x256, depth 12): the network is trained online during a 1024-spp render,
+    then plain and guided 64-spp renders are scored against an FP64 4096-spp
+    render. Equal-time efficiency = (MSE x seconds) plain / (MSE x seconds) guided.
+    The full equal-time protocol on the 1080p path-trace case (8192-spp oracle) is
+    trenchanttracer_amd/bench_guide.py."""
+    from trenchanttracer_amd import workloads
+    mesh, mats, cam = workloads.door_room(256, 256)
+    s = tt.DeviceScene(ctx, mesh, tt.BVH.build(mesh), mats)
+    ref = s.render(cam, tt.render_params(spp=4096, max_depth=12, rr_depth=4, seed=1234, mode=tt.RENDER_PATH,
+                                         fp64=True), want_fp64=True)[1]
+    g = tt.Guide(ctx, seed=1, lr=1e-2)
+    pt = tt.render_params(spp=1024, max_depth=12, rr_depth=4, seed=99, mode=tt.RENDER_PATH, paths_in_flight=131072)
+    _, _, gs = g.render(s, cam, pt, alpha=0.5, train=True, train_paths=65536, train_batch=16384)
+    p64 = tt.render_params(spp=64, max_depth=12, rr_depth=4, seed=5, mode=tt.RENDER_PATH)
+    s.render(cam, p64)
+    plain, sp = s.render(cam, p64)
+    g.render(s, cam, p64, alpha=0.5, train=False)
+    guided, sg, _ = g.render(s, cam, p64, alpha=0.5, train=False)
+    mp, mg = float(np.mean((plain - ref) ** 2)), float(np.mean((guided - ref) ** 2))
+    r = {"mse_plain_64spp": mp, "mse_guided_64spp": mg, "seconds_plain": round(sp["seconds"], 5),
+         "seconds_guided": round(sg["seconds"], 5), "mse_ratio": round(mg / mp, 4),
+         "equal_time_efficiency": round((mp * sp["seconds"]) / (mg * sg["seconds"]), 3),
+         "training": {"steps": gs["train_steps"], "rows": gs["train_rows"],
+                      "seconds": round(gs["train_seconds"], 4), "last_loss": round(gs["last_loss"], 4)}}
+    log(f"guided door room: {r}")
+    g.close()
+    s.close()
+    return r
 
 
 if __name__ == "__main__":
