@@ -440,27 +440,43 @@ __global__ __launch_bounds__(256) void k_wgrad(const __bf16 *__restrict__ H, int
     if (threadIdx.x < KO) pbias[(int64_t)blockIdx.x * KO + threadIdx.x] = bsum;
 }
 
-// grad[offset + e] = sum_b partial[b][e] for e < count (deterministic order)
+// grad[offset + dst(e)] = sum_b partial[b][e] for e < count: 64 elements per
+// block, 4 threads per element each summing every 4th block's partial in order,
+// combined in a fixed order (deterministic). dst transposes [in][out] partials to
+// the [out][in] master layout when transpose_ko is set.
 __global__ __launch_bounds__(256) void k_reduce(const float *__restrict__ partial, int nb, int count,
                                                 float *__restrict__ grad, int offset, bool transpose_ko,
                                                 int ki, int ko) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= count) return;
+    __shared__ float red[4][64];
+    const int el = threadIdx.x & 63, part = threadIdx.x >> 6;
+    const int e = blockIdx.x * 64 + el;
     float s = 0.0f;
-    for (int b = 0; b < nb; ++b) s += partial[(int64_t)b * count + e];
-    // partials are [in][out]; master weights are stored [out][in]
-    int dst = e;
-    if (transpose_ko) {
-        const int i = e / ko, o = e % ko;
-        dst = o * ki + i;
+    if (e < count)
+        for (int b = part; b < nb; b += 4) s += partial[(int64_t)b * count + e];
+    red[part][el] = s;
+    __syncthreads();
+    if (part == 0 && e < count) {
+        const float t = ((red[0][el] + red[1][el]) + red[2][el]) + red[3][el];
+        int dst = e;
+        if (transpose_ko) {
+            const int i = e / ko, o = e % ko;
+            dst = o * ki + i;
+        }
+        grad[offset + dst] = t;
     }
-    grad[offset + dst] = s;
 }
 
 // ------------------------------------------------------------------ Adam + BF16 copies
+// Adam step on every parameter, and the BF16 copies of the updated weights:
+// forward Bt[out][in] and backward W[in][out_pad] of the layer the element is in.
+struct AdamLayers {
+    int off[4], in[4], out[4], out_pad_bwd[4];
+    __bf16 *fwd[4], *bwd[4];
+};
+
 __global__ __launch_bounds__(256) void k_adam(float *__restrict__ p, float *__restrict__ m, float *__restrict__ v,
                                               const float *__restrict__ g, int n, float lr, float b1, float b2,
-                                              float eps, float c1, float c2) {
+                                              float eps, float c1, float c2, AdamLayers L) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
     const float gg = g[e];
@@ -468,7 +484,19 @@ __global__ __launch_bounds__(256) void k_adam(float *__restrict__ p, float *__re
     const float vv = b2 * v[e] + (1.0f - b2) * gg * gg;
     m[e] = mm;
     v[e] = vv;
-    p[e] -= lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+    const float np = p[e] - lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+    p[e] = np;
+    int k = 0;
+#pragma unroll
+    for (int q = 1; q < 4; ++q)
+        if (e >= L.off[q]) k = q;
+    const int le = e - L.off[k];
+    if (le < L.in[k] * L.out[k]) {
+        const int o = le / L.in[k], i = le % L.in[k];
+        const __bf16 bv = (__bf16)np;
+        L.fwd[k][le] = bv;
+        L.bwd[k][i * L.out_pad_bwd[k] + o] = bv;
+    }
 }
 
 // Master layout per layer: W[out][in] then b[out]. Forward copy Bt = W[out][in]
@@ -599,8 +627,9 @@ static hipError_t guide_backward(tt_guide *g, int64_t n, hipStream_t s) {
                        (const float *)nullptr, (void *)b.dh[0], kGuideH, (const __bf16 *)b.h[0], R, (const int32_t *)nullptr);
     // weight gradients: dW_k = H_{k-1}^T dH_k and bias gradients, partial sums per
     // block (at most kWgBlocks blocks of whole kWgRows chunks), reduced in block order
-    constexpr int64_t kWgBlocks = 1024;
-    const int64_t rpb = std::max<int64_t>(kWgRows, ((R + kWgBlocks - 1) / kWgBlocks + kWgRows - 1) / kWgRows * kWgRows);
+    constexpr int64_t kWgBlocks = 1024, kWgMinRows = 256;
+    const int64_t rpb =
+        std::max<int64_t>(kWgMinRows, ((R + kWgBlocks - 1) / kWgBlocks + kWgRows - 1) / kWgRows * kWgRows);
     const int nb = (int)((R + rpb - 1) / rpb);
     const int64_t need = (int64_t)nb * (kGuideK1 * kGuideH + kGuideNOutPad16 + kGuideH);
     if (g->partial_cap < need) {
@@ -634,9 +663,9 @@ static hipError_t guide_backward(tt_guide *g, int64_t n, hipStream_t s) {
             hipLaunchKernelGGL((k_wgrad<kGuideH, kGuideNOutPad16>), dim3(nb), dim3(256), 0, s, l.h, l.ldh, l.d, l.ldd,
                                R, rpb, g->partial, pbias);
         const int cnt = l.ki * l.ko;
-        hipLaunchKernelGGL(k_reduce, dim3((cnt + 255) / 256), dim3(256), 0, s, (const float *)g->partial, nb, cnt,
+        hipLaunchKernelGGL(k_reduce, dim3((cnt + 63) / 64), dim3(256), 0, s, (const float *)g->partial, nb, cnt,
                            g->grad, (int)layer_offset(k), true, l.ki, l.ko);
-        hipLaunchKernelGGL(k_reduce, dim3((l.ko + 255) / 256), dim3(256), 0, s, (const float *)pbias, nb, l.ko,
+        hipLaunchKernelGGL(k_reduce, dim3((l.ko + 63) / 64), dim3(256), 0, s, (const float *)pbias, nb, l.ko,
                            g->grad, (int)(layer_offset(k) + cnt), false, l.ki, l.ko);
     }
     return hipGetLastError();
@@ -646,9 +675,18 @@ static hipError_t guide_adam(tt_guide *g, hipStream_t s) {
     g->step++;
     const float c1 = 1.0f - std::pow(g->beta1, (float)g->step), c2 = 1.0f - std::pow(g->beta2, (float)g->step);
     const int n = (int)guide_param_count();
+    AdamLayers L{};
+    for (int k = 0; k < 4; ++k) {
+        L.off[k] = (int)layer_offset(k);
+        L.in[k] = kGuideLayers[k].in;
+        L.out[k] = kGuideLayers[k].out;
+        L.out_pad_bwd[k] = k == 3 ? kGuideNOutPad : kGuideLayers[k].out;
+        L.fwd[k] = g->wf[k];
+        L.bwd[k] = g->wb[k];
+    }
     hipLaunchKernelGGL(k_adam, dim3((n + 255) / 256), dim3(256), 0, s, g->params, g->m1, g->m2, g->grad, n, g->lr,
-                       g->beta1, g->beta2, g->eps, c1, c2);
-    return refresh_bf16(g, s);
+                       g->beta1, g->beta2, g->eps, c1, c2, L);
+    return hipGetLastError();
 }
 
 hipError_t guide_train_rows(tt_guide *g, int64_t n, hipStream_t s) {

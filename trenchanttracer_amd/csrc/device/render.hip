@@ -887,23 +887,22 @@ __global__ __launch_bounds__(256) void k_apply_shadow(ApplyArgs a) {
 __global__ __launch_bounds__(256) void k_collect(float *rec, int32_t ntrain, int32_t max_depth, float *buf,
                                                  int32_t *count, int32_t cap) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (int64_t)ntrain * max_depth) return;
-    float *r = rec + e * kRecFloats;
-    const float pdf = r[12];
-    if (pdf > 0.0f) {
+    const bool live = e < (int64_t)ntrain * max_depth;
+    float *r = rec + (live ? e : 0) * kRecFloats;
+    float w = 0.0f;
+    if (live && r[12] > 0.0f) {
         const float nx = 2.0f * r[3] - 1.0f, ny = 2.0f * r[4] - 1.0f, nz = 2.0f * r[5] - 1.0f;
         const float cosw = fabsf(nx * r[9] + ny * r[10] + nz * r[11]);
-        const float lum = (0.2126f * r[16] + 0.7152f * r[17] + 0.0722f * r[18]) * cosw;
-        if (lum > 0.0f && isfinite(lum)) {
-            const int slot = atomicAdd(count, 1);
-            if (slot < cap) {
-                float *o = buf + (int64_t)slot * 16;
-                for (int k = 0; k < 12; ++k) o[k] = r[k];
-                o[12] = lum / pdf;
-            }
-        }
+        w = (0.2126f * r[16] + 0.7152f * r[17] + 0.0722f * r[18]) * cosw / r[12];
     }
-    r[12] = 0.0f;
+    const bool valid = w > 0.0f && isfinite(w);
+    const int slot = wave_append(valid, count);   // one counter update per wave
+    if (valid && slot < cap) {
+        float *o = buf + (int64_t)slot * 16;
+        for (int k = 0; k < 12; ++k) o[k] = r[k];
+        o[12] = w;
+    }
+    if (live) r[12] = 0.0f;
 }
 
 __global__ __launch_bounds__(256) void k_stage_train(const float *buf, int32_t n, float *in, float *dirs, float *wts) {
