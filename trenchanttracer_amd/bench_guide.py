@@ -12,7 +12,7 @@ off a log-log fit of RMSE against time (RMSE ~ time^-1/2 locally).
 The same protocol on the door room (256x256, depth 12), where all light reaches
 the camera's part of the room through a door, with a 4096-spp oracle.
 
-    python -m trenchanttracer_amd.bench_guide [--oracle-spp 8192] [--skip-c5]
+    python -m trenchanttracer_amd.bench_guide [--oracle-spp 8192] [--skip-c5] [--quick]
 """
 import argparse
 import json
@@ -77,6 +77,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--oracle-spp", type=int, default=8192)
     ap.add_argument("--skip-c5", action="store_true")
+    ap.add_argument("--quick", action="store_true", help="door room only, one training run (for profiling)")
     args = ap.parse_args()
     import trenchanttracer_amd as tt
     from trenchanttracer_amd import workloads
@@ -89,12 +90,15 @@ def main():
     _, ref, st = sc.render(cam, tt.render_params(spp=4096, max_depth=12, rr_depth=4, seed=1234, mode=tt.RENDER_PATH,
                                                  fp64=True), want_fp64=True)
     log(f"door oracle 4096 spp in {st['seconds']:.2f}s")
-    result["door_room_256"] = compare(tt, ctx, sc, cam, ref, 12, (16, 32, 64, 128), (64, 256, 1024), 131072, 1e-2,
-                                      16384, 65536)
+    if args.quick:
+        result["door_room_256"] = compare(tt, ctx, sc, cam, ref, 12, (64,), (256,), 131072, 1e-2, 16384, 65536)
+    else:
+        result["door_room_256"] = compare(tt, ctx, sc, cam, ref, 12, (16, 32, 64, 128), (64, 256, 1024), 131072,
+                                          1e-2, 16384, 65536)
     result["door_room_256"]["oracle_spp"] = 4096
     sc.close()
 
-    if not args.skip_c5:
+    if not (args.skip_c5 or args.quick):
         mesh, mats, env, cam = workloads.pathtrace()
         sc = tt.DeviceScene(ctx, mesh, tt.BVH.build(mesh), mats, env=env)
         t0 = time.time()
