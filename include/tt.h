@@ -206,17 +206,38 @@ typedef enum {
     TT_MAT_DIFFUSE = 0,     /* Lambert; ref: diffSample (include/material/DiffImpl.cuh:12-36) */
     TT_MAT_CONDUCTOR = 1,   /* GGX conductor (VNDF sampling); ref mode: metalSample Phong lobe */
     TT_MAT_DIELECTRIC = 2,  /* smooth dielectric, Fresnel; ref mode: refrSample (IOR 1.4) */
-    TT_MAT_EMITTER = 3      /* emission only (black BSDF) */
+    TT_MAT_EMITTER = 3,     /* emission only (black BSDF) */
+    TT_MAT_SPECULAR = 4,    /* perfect mirror; ref: specSample (include/material/SpecImpl.cuh:12-27) */
+    TT_MAT_COAT = 5         /* Fresnel-chosen mirror or Lambert layer (n 1.000293 / 1.291);
+                               ref: coatSample (include/material/CoatImpl.cuh:75-125) */
 } tt_mat_type;
 
 typedef struct tt_material {
     int32_t type;           /* tt_mat_type */
-    float albedo[3];        /* diffuse colour / conductor F0 colour / dielectric tint */
+    float albedo[3];        /* diffuse colour / conductor F0 / dielectric tint / mirror colour */
     float emission[3];
     float roughness;        /* GGX alpha = roughness^2 (conductor) */
     float ior;              /* dielectric */
-    float pad[3];
+    float specular[3];      /* coat: colour of the mirror reflection (albedo = its diffuse colour) */
 } tt_material;              /* 48 B */
+
+/* Analytic primitives tested after the triangle traversal, as the reference's
+ * per-ray loop over its geometry list (src/core/RenderKernel.cu:38-61):
+ *   sphere (include/geometry/SphereImpl.cuh:10-32): centre a, radius b[0]; nearest
+ *     root above 0.01, accepted below the ray's tmax;
+ *   plane (include/geometry/PlaneImpl.cuh:10-52): the parallelogram p0 = a,
+ *     p1 = b, p2 = c (u, v in [0, 1] along p1 - p0, p2 - p0), both sides, |det| >=
+ *     1e-5, t > 1e-5 and below tmax; normal (p1 - p0) x (p2 - p0).
+ * The nearest primitive replaces the triangle hit when strictly closer. Hits report
+ * tri_id = -2 - (geometry index). */
+typedef enum { TT_GEOM_SPHERE = 0, TT_GEOM_PLANE = 1 } tt_geom_type;
+
+typedef struct tt_geometry {
+    int32_t type;           /* tt_geom_type */
+    int32_t material;       /* index into the scene's materials */
+    float a[3], b[3], c[3];
+    float pad;
+} tt_geometry;              /* 48 B */
 
 /* Uploads the SBVH device layout, per-triangle shading normals and materials.
  * The env map (RGB float, width x height, equirectangular, may be NULL = black)
@@ -227,6 +248,10 @@ tt_status tt_scene_create(tt_ctx *ctx, const tt_mesh *mesh, const tt_bvh *bvh,
                           float env_scale, tt_scene **out);
 void tt_scene_destroy(tt_scene *s);
 tt_status tt_scene_info(const tt_scene *s, int64_t *num_nodes, int64_t *num_tris, int64_t *device_bytes);
+/* Replaces the scene's analytic primitives (n <= 4096; n = 0 removes them). tt_render
+ * tests them in both modes (extension and shadow rays); tt_trace_closest /
+ * tt_trace_any test triangles only, as intersect() does. */
+tt_status tt_scene_set_geometries(tt_scene *s, const tt_geometry *g, int32_t n);
 /* Copies the env-map sampling distribution built on the GPU: marginal CDF over
  * rows (env_h + 1 floats), conditional CDFs (env_h x (env_w + 1) floats) and the
  * FP64 total weight (sum of luminance * sin(theta) over texels). */

@@ -29,7 +29,12 @@ class o_camera(C.Structure):
 
 class o_material(C.Structure):
     _fields_ = [("type", C.c_int32), ("albedo", C.c_float * 3), ("emission", C.c_float * 3),
-                ("roughness", C.c_float), ("ior", C.c_float), ("pad", C.c_float * 3)]
+                ("roughness", C.c_float), ("ior", C.c_float), ("specular", C.c_float * 3)]
+
+
+class o_geometry(C.Structure):
+    _fields_ = [("type", C.c_int32), ("material", C.c_int32), ("a", C.c_float * 3), ("b", C.c_float * 3),
+                ("c", C.c_float * 3), ("pad", C.c_float)]
 
 
 class o_ref_params(C.Structure):
@@ -54,9 +59,11 @@ def _load():
     lib.o_trace_batch.restype = C.c_double
     lib.o_tri_test.argtypes = [P, P, P, P, C.c_float, P, P, P]
     lib.o_tri_test.restype = C.c_int
-    lib.o_render_ref.argtypes = [C.POINTER(o_bvh), P, P, P, C.POINTER(o_camera), C.POINTER(o_ref_params), C.c_int32,
-                                 C.c_int32, P, C.c_int]
+    lib.o_render_ref.argtypes = [C.POINTER(o_bvh), P, P, P, P, C.c_int32, C.POINTER(o_camera),
+                                 C.POINTER(o_ref_params), C.c_int32, C.c_int32, P, C.c_int]
     lib.o_render_ref.restype = C.c_int
+    lib.o_geom_t.argtypes = [C.POINTER(o_geometry), P]
+    lib.o_geom_t.restype = C.c_float
     for name in ("o_sinf", "o_cosf", "o_logf", "o_expf"):
         if hasattr(lib, name):
             getattr(lib, name).argtypes = [C.c_float]
@@ -136,9 +143,28 @@ def tri_normal_table(mesh_arrays):
     return out.reshape(-1, 12)
 
 
+def geometries(geoms):
+    """ctypes array of o_geometry from objects with type, material, a, b, c fields."""
+    arr = (o_geometry * max(1, len(geoms)))()
+    for k, g in enumerate(geoms):
+        arr[k].type, arr[k].material = int(g.type), int(g.material)
+        arr[k].a[:] = list(g.a)
+        arr[k].b[:] = list(g.b)
+        arr[k].c[:] = list(g.c)
+    return arr
+
+
+def geom_t(geom, ray):
+    """Distance at which `ray` (8 floats) hits one analytic primitive (+inf: none)."""
+    g = geometries([geom])
+    r = np.ascontiguousarray(ray, np.float32).reshape(8)
+    return float(lib.o_geom_t(C.byref(g[0]), _p(r)))
+
+
 def render_ref(compact_bvh, mesh_arrays, materials, cam, spp, max_depth, seed, pi=3.14156265,
-               two_pi=6.2831853071795864769, tmin=0.01, rows=None, nthreads=None, sample_offset=0):
-    """Reference-semantics render on the CPU: (H, W, 3) float32 image (rows outside `rows` stay 0)."""
+               two_pi=6.2831853071795864769, tmin=0.01, rows=None, nthreads=None, sample_offset=0, geoms=()):
+    """Reference-semantics render on the CPU: (H, W, 3) float32 image (rows outside `rows` stay 0);
+    `geoms` are the analytic primitives tested after the triangles."""
     oc = o_camera()
     for f in ("eye", "view", "up", "fov"):
         getattr(oc, f)[:] = list(getattr(cam, f))
@@ -149,6 +175,7 @@ def render_ref(compact_bvh, mesh_arrays, materials, cam, spp, max_depth, seed, p
         mats[k].albedo[:] = list(m.albedo)
         mats[k].emission[:] = list(m.emission)
         mats[k].roughness, mats[k].ior = m.roughness, m.ior
+        mats[k].specular[:] = list(m.specular)
     prm = o_ref_params(int(spp), int(max_depth), int(seed), int(sample_offset), float(pi), float(two_pi),
                        float(tmin), 1e20)
     tn = tri_normal_table(mesh_arrays)
@@ -156,8 +183,9 @@ def render_ref(compact_bvh, mesh_arrays, materials, cam, spp, max_depth, seed, p
     out = np.zeros((cam.height, cam.width, 3), np.float32)
     y0, y1 = rows if rows is not None else (0, cam.height)
     nt = nthreads or min(8, os.cpu_count() or 1)
-    rc = lib.o_render_ref(C.byref(compact_bvh.c), _p(tn), _p(tm), C.cast(mats, C.c_void_p), C.byref(oc),
-                          C.byref(prm), int(y0), int(y1), _p(out), nt)
+    ga = geometries(list(geoms))
+    rc = lib.o_render_ref(C.byref(compact_bvh.c), _p(tn), _p(tm), C.cast(mats, C.c_void_p), C.cast(ga, C.c_void_p),
+                          len(geoms), C.byref(oc), C.byref(prm), int(y0), int(y1), _p(out), nt)
     if rc != 0:
         raise RuntimeError("o_render_ref failed")
     return out

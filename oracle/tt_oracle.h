@@ -74,14 +74,20 @@ double o_trace_batch(const o_bvh *b, const o_ray *rays, int64_t n, int any, o_hi
 
 /* Reference render restatement (tt_oracle_render.c). */
 typedef struct { float eye[3], view[3], up[3]; float fov[2]; float aperture, focal; int32_t width, height; } o_camera;
-typedef struct { int32_t type; float albedo[3]; float emission[3]; float roughness, ior; float pad[3]; } o_material;
+typedef struct { int32_t type; float albedo[3]; float emission[3]; float roughness, ior; float specular[3]; } o_material;
+/* analytic primitive: type 0 sphere (centre a, radius b[0]), 1 parallelogram (p0 a, p1 b, p2 c) */
+typedef struct { int32_t type, material; float a[3], b[3], c[3]; float pad; } o_geometry;
 typedef struct { int32_t spp, max_depth; uint32_t seed; int32_t sample_offset; float pi, two_pi, tmin, tmax; } o_ref_params;
 
 /* Renders rows [y0, y1) of the image (row 0 = top) into out_rgb (W*H*3 floats,
  * the average over spp samples). tri_normals: 12 floats (3 float4) per original
  * triangle; tri_mat: material per original triangle. */
 int o_render_ref(const o_bvh *b, const float *tri_normals, const int32_t *tri_mat, const o_material *mats,
-                 const o_camera *cam, const o_ref_params *p, int32_t y0, int32_t y1, float *out_rgb, int nthreads);
+                 const o_geometry *geoms, int32_t num_geoms, const o_camera *cam, const o_ref_params *p, int32_t y0,
+                 int32_t y1, float *out_rgb, int nthreads);
+/* distance at which ray r hits primitive g under the reference's rules
+ * (SphereImpl.cuh:10-32, PlaneImpl.cuh:10-52), +inf when it does not */
+float o_geom_t(const o_geometry *g, const o_ray *r);
 
 /* Float math used by the restated samplers (Cody–Waite reduction + Taylor
  * polynomials; |error| <= 2 ulp on the documented ranges). */

@@ -186,12 +186,107 @@ static ray3 refr_sample(const o_material *m, const float u[4], ray3 r, v3 point,
     return r;
 }
 
+/* specSample, SpecImpl.cuh:12-27: mirror about n, origin off nl */
+static ray3 spec_sample(const o_material *m, ray3 r, v3 point, v3 n, v3 nl, v3 *mask) {
+    r.d = vnorm(vsub3(r.d, vscale(vscale(n, vdot3(n, r.d)), 2.0f)));
+    r.o = vadd(point, vscale(nl, 0.001f));
+    *mask = vmul(*mask, V(m->albedo[0], m->albedo[1], m->albedo[2]));
+    return r;
+}
+
+/* coatSample, CoatImpl.cuh:12-125: reflect(), refract() and fresnel() with air
+ * 1.000293 over coat 1.291 (powf(x, 2) as the rounded square), then the Fresnel
+ * coin picks the mirror (specular colour) or a cosine-weighted Lambert bounce
+ * (diffuse colour). Uniforms: coin u[0], then u[1], u[2] for the Lambert bounce. */
+static ray3 coat_sample(const o_material *m, const float u[4], ray3 r, v3 point, v3 n, v3 nl, v3 *mask) {
+    const float n_air = 1.000293f, n_coat = 1.291f;
+    v3 reflect_dir = vsub3(r.d, vscale(vscale(n, 2.0f), vdot3(n, r.d)));
+    float cos1 = vdot3(n, vscale(r.d, -1.0f));
+    float ratio = n_air / n_coat;
+    float radicand = 1 - (ratio * ratio) * (1 - cos1 * cos1);
+    v3 transmit = V(0, 0, 0);
+    if (!(radicand < 0)) {
+        float cos2 = sqrtf(radicand);
+        if (cos1 > 0) transmit = vadd(vscale(r.d, ratio), vscale(n, cos1 * ratio - cos2));
+        else transmit = vadd(vscale(r.d, ratio), vscale(n, cos1 * ratio + cos2));
+    }
+    float threshold;
+    float tlen = sqrtf(vdot3(transmit, transmit));
+    if ((double)tlen <= 0.12345 || vdot3(n, transmit) > 0) {
+        threshold = 1.0f; /* total internal reflection */
+    } else {
+        float ci = vdot3(n, vscale(r.d, -1.0f));
+        float ct = vdot3(vscale(n, -1.0f), transmit);
+        float rs = (n_air * ci - n_coat * ct) / (n_air * ci + n_coat * ct);
+        float rp = (n_air * ct - n_coat * ci) / (n_air * ct + n_coat * ci);
+        threshold = (rs * rs + rp * rp) / 2.0f;
+    }
+    if (u[0] < threshold) {
+        *mask = vmul(*mask, V(m->specular[0], m->specular[1], m->specular[2]));
+        r.d = vnorm(reflect_dir);
+    } else {
+        float r1 = (float)(2.0 * 3.14159265358979323846 * (double)u[1]);
+        float r2 = u[2];
+        float r2s = sqrtf(r2);
+        v3 w = vnorm(nl);
+        v3 a = fabsf(w.x) >= 0.1f ? V(0, 1, 0) : V(1, 0, 0);
+        v3 uu = vnorm(vcross3(a, w));
+        v3 vv = vcross3(w, uu);
+        float c, s = o_sincos(r1, &c);
+        v3 d = vadd(vadd(vscale(vscale(uu, c), r2s), vscale(vscale(vv, s), r2s)), vscale(w, sqrtf(1 - r2)));
+        r.d = vnorm(d);
+        *mask = vmul(*mask, V(m->albedo[0], m->albedo[1], m->albedo[2]));
+    }
+    r.o = vadd(point, vscale(nl, 0.001f));
+    return r;
+}
+
+/* ---------------------------------------------------------------- analytic primitives */
+/* planeIntersect, PlaneImpl.cuh:10-52 (not culling; double literals) */
+static float plane_t(const o_geometry *g, v3 o, v3 d) {
+    v3 p0 = V(g->a[0], g->a[1], g->a[2]);
+    v3 e1 = vsub3(V(g->b[0], g->b[1], g->b[2]), p0);
+    v3 e2 = vsub3(V(g->c[0], g->c[1], g->c[2]), p0);
+    v3 P = vcross3(d, e2);
+    float det = vdot3(e1, P);
+    if ((double)det > -0.00001 && (double)det < 0.00001) return INFINITY;
+    float inv_det = 1.f / det;
+    v3 T = vsub3(o, p0);
+    v3 Q = vcross3(T, e1);
+    float u = vdot3(T, P) * inv_det;
+    float v = vdot3(d, Q) * inv_det;
+    if (u < 0.f || u > 1.f || v < 0.f || v > 1.f) return INFINITY;
+    float t = vdot3(e2, Q) * inv_det;
+    return (double)t > 0.00001 ? t : INFINITY;
+}
+
+/* sphereIntersect, SphereImpl.cuh:10-32 */
+static float sphere_t(const o_geometry *g, v3 o, v3 d, float tmax) {
+    v3 op = vsub3(V(g->a[0], g->a[1], g->a[2]), o);
+    float eps = 0.01f;
+    float b = vdot3(op, d);
+    float disc = b * b - vdot3(op, op) + g->b[0] * g->b[0];
+    if (disc >= 0) {
+        disc = sqrtf(disc);
+        float t, dist = (t = b - disc) > eps ? t : ((t = b + disc) > eps ? t : 0.0f);
+        if (0.01f < dist && dist < tmax) return dist;
+    }
+    return INFINITY;
+}
+
+float o_geom_t(const o_geometry *g, const o_ray *r) {
+    v3 o = V(r->ox, r->oy, r->oz), d = V(r->dx, r->dy, r->dz);
+    return g->type == 0 ? sphere_t(g, o, d, r->tmax) : plane_t(g, o, d);
+}
+
 /* ---------------------------------------------------------------- render */
 typedef struct {
     const o_bvh *b;
     const float *tri_normals;
     const int32_t *tri_mat;
     const o_material *mats;
+    const o_geometry *geoms;
+    int32_t num_geoms;
     const o_camera *cam;
     const o_ref_params *p;
     float *out;
@@ -245,20 +340,44 @@ static void render_rows(const o_rjob *j) {
                     o_ray q = {r.o.x, r.o.y, r.o.z, r.tmin, r.d.x, r.d.y, r.d.z, r.tmax};
                     o_hit h;
                     o_intersect(j->b, &q, 0, &h, NULL, NULL);
-                    if (h.tri < 0 || h.t > 1e19f) break; /* miss: black environment */
-                    const float *tn = j->tri_normals + 12 * (int64_t)h.tri;
-                    float w = 1 - h.u - h.v;
-                    v3 nrm = V(tn[0] * w + tn[4] * h.u + tn[8] * h.v, tn[1] * w + tn[5] * h.u + tn[9] * h.v,
-                               tn[2] * w + tn[6] * h.u + tn[10] * h.v);
+                    /* geometry list after the triangles, RenderKernel.cu:38-61 */
+                    int gsel = -1;
+                    for (int32_t k = 0; k < j->num_geoms; ++k) {
+                        float t = o_geom_t(&j->geoms[k], &q);
+                        if (t < q.tmax && t < h.t) {
+                            h.t = t;
+                            gsel = k;
+                        }
+                    }
+                    if ((gsel < 0 && h.tri < 0) || h.t > 1e19f) break; /* miss: black environment */
+                    v3 point = vadd(r.o, vscale(r.d, h.t));
+                    v3 nrm;
+                    const o_material *m;
+                    if (gsel >= 0) {
+                        const o_geometry *g = &j->geoms[gsel];
+                        if (g->type == 0) {
+                            nrm = V(point.x - g->a[0], point.y - g->a[1], point.z - g->a[2]);
+                        } else {
+                            v3 p0 = V(g->a[0], g->a[1], g->a[2]);
+                            nrm = vcross3(vsub3(V(g->b[0], g->b[1], g->b[2]), p0), vsub3(V(g->c[0], g->c[1], g->c[2]), p0));
+                        }
+                        m = &j->mats[g->material];
+                    } else {
+                        const float *tn = j->tri_normals + 12 * (int64_t)h.tri;
+                        float w = 1 - h.u - h.v;
+                        nrm = V(tn[0] * w + tn[4] * h.u + tn[8] * h.v, tn[1] * w + tn[5] * h.u + tn[9] * h.v,
+                                tn[2] * w + tn[6] * h.u + tn[10] * h.v);
+                        m = &j->mats[j->tri_mat[h.tri]];
+                    }
                     v3 n = vnorm(nrm);
                     v3 nl = vdot3(n, r.d) < 0 ? n : vscale(n, -1);
-                    v3 point = vadd(r.o, vscale(r.d, h.t));
-                    const o_material *m = &j->mats[j->tri_mat[h.tri]];
                     acc = vadd(acc, vmul(mask, V(m->emission[0], m->emission[1], m->emission[2])));
                     draws(pix, sg, (uint32_t)(bounce + 1), p->seed, u);
                     if (m->type == 0) r = diff_sample(m, u, r, point, nl, &mask);
                     else if (m->type == 1) r = metal_sample(m, u, r, point, n, nl, &mask);
                     else if (m->type == 2) r = refr_sample(m, u, r, point, n, nl, &mask);
+                    else if (m->type == 4) r = spec_sample(m, r, point, n, nl, &mask);
+                    else if (m->type == 5) r = coat_sample(m, u, r, point, n, nl, &mask);
                     else break; /* emitter: emission only */
                 }
                 final = vadd(final, vscale(acc, inv_spp));
@@ -276,7 +395,8 @@ static void *render_worker(void *arg) {
 }
 
 int o_render_ref(const o_bvh *b, const float *tri_normals, const int32_t *tri_mat, const o_material *mats,
-                 const o_camera *cam, const o_ref_params *p, int32_t y0, int32_t y1, float *out_rgb, int nthreads) {
+                 const o_geometry *geoms, int32_t num_geoms, const o_camera *cam, const o_ref_params *p, int32_t y0,
+                 int32_t y1, float *out_rgb, int nthreads) {
     if (nthreads < 1) nthreads = 1;
     if (nthreads > 256) nthreads = 256;
     if (y0 < 0 || y1 > cam->height || y0 > y1) return -1;
@@ -284,7 +404,7 @@ int o_render_ref(const o_bvh *b, const float *tri_normals, const int32_t *tri_ma
     pthread_t th[256];
     int32_t rows = y1 - y0;
     for (int k = 0; k < nthreads; ++k) {
-        o_rjob jb = {b, tri_normals, tri_mat, mats, cam, p, out_rgb, y0 + rows * k / nthreads,
+        o_rjob jb = {b, tri_normals, tri_mat, mats, geoms, num_geoms, cam, p, out_rgb, y0 + rows * k / nthreads,
                      y0 + rows * (k + 1) / nthreads};
         jobs[k] = jb;
         pthread_create(&th[k], NULL, render_worker, &jobs[k]);

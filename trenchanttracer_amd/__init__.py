@@ -22,7 +22,7 @@ from ._native import TTError, check, lib, tt_trace_cfg
 __all__ = ["Mesh", "BVH", "Context", "DeviceScene", "DeviceBuffer", "Timer", "Batch", "Guide", "TTError", "rays_primary", "rays_incoherent",
            "trace_cfg", "MAT_DIFFUSE", "MAT_CONDUCTOR", "MAT_DIELECTRIC", "MAT_EMITTER", "material"]
 
-MAT_DIFFUSE, MAT_CONDUCTOR, MAT_DIELECTRIC, MAT_EMITTER = 0, 1, 2, 3
+MAT_DIFFUSE, MAT_CONDUCTOR, MAT_DIELECTRIC, MAT_EMITTER, MAT_SPECULAR, MAT_COAT = 0, 1, 2, 3, 4, 5
 
 
 def _fptr(a):
@@ -184,14 +184,39 @@ def trace_cfg(variant=0, lds_stack=16, block=128, persistent=0):
     return tt_trace_cfg(int(variant), int(lds_stack), int(block), int(persistent))
 
 
-def material(type, albedo=(0.8, 0.8, 0.8), emission=(0, 0, 0), roughness=0.0, ior=1.5):
+def material(type, albedo=(0.8, 0.8, 0.8), emission=(0, 0, 0), roughness=0.0, ior=1.5, specular=(1.0, 1.0, 1.0)):
+    """tt_material; `specular` is the mirror colour of a coat (albedo its diffuse colour)."""
     m = N.tt_material()
     m.type = int(type)
     m.albedo[:] = [float(x) for x in albedo]
     m.emission[:] = [float(x) for x in emission]
     m.roughness = float(roughness)
     m.ior = float(ior)
+    m.specular[:] = [float(x) for x in specular]
     return m
+
+
+GEOM_SPHERE, GEOM_PLANE = 0, 1
+
+
+def sphere(center, radius, mat):
+    """Analytic sphere (tt_geometry), as the reference's Sphere object."""
+    g = N.tt_geometry()
+    g.type, g.material = GEOM_SPHERE, int(mat)
+    g.a[:] = [float(x) for x in center]
+    g.b[:] = [float(radius), 0.0, 0.0]
+    return g
+
+
+def plane(p0, p1, p2, mat):
+    """Analytic parallelogram p0 + u (p1 - p0) + v (p2 - p0), u, v in [0, 1] (the
+    reference's Plane object, tested on both sides)."""
+    g = N.tt_geometry()
+    g.type, g.material = GEOM_PLANE, int(mat)
+    g.a[:] = [float(x) for x in p0]
+    g.b[:] = [float(x) for x in p1]
+    g.c[:] = [float(x) for x in p2]
+    return g
 
 
 REF_PI = 3.14156265        # the reference's PI (src/core/Renderer.cpp:5, Camera.cpp:5)
@@ -562,6 +587,11 @@ class DeviceScene:
         st = N.tt_render_stats()
         check(lib.tt_render(self._h, C.byref(cam), C.byref(params), _vp(out), _vp(out64), C.byref(st)))
         return (out, out64, _stats(st)) if want_fp64 else (out, _stats(st))
+
+    def set_geometries(self, geoms):
+        """Analytic primitives (list of `sphere` / `plane`) tested after the triangles."""
+        arr = (N.tt_geometry * max(1, len(geoms)))(*geoms)
+        check(lib.tt_scene_set_geometries(self._h, C.cast(arr, C.c_void_p), len(geoms)))
 
     def env_samples(self, n, seed=1):
         out = np.empty((int(n), 8), np.float32)

@@ -42,7 +42,12 @@ class tt_compact_sizes(C.Structure):
 
 class tt_material(C.Structure):
     _fields_ = [("type", C.c_int32), ("albedo", C.c_float * 3), ("emission", C.c_float * 3),
-                ("roughness", C.c_float), ("ior", C.c_float), ("pad", C.c_float * 3)]
+                ("roughness", C.c_float), ("ior", C.c_float), ("specular", C.c_float * 3)]
+
+
+class tt_geometry(C.Structure):
+    _fields_ = [("type", C.c_int32), ("material", C.c_int32), ("a", C.c_float * 3), ("b", C.c_float * 3),
+                ("c", C.c_float * 3), ("pad", C.c_float)]
 
 
 class tt_trace_cfg(C.Structure):
@@ -140,6 +145,7 @@ PROTOS = {
     "tt_guide_grad": (C.c_int, [P, P, P, P, I64, PF, P]),
     "tt_guide_train_step": (C.c_int, [P, P, P, P, I64, PF]),
     "tt_debug_guide_sample": (C.c_int, [P, P, I64, U32, P]),
+    "tt_scene_set_geometries": (C.c_int, [P, P, I32]),
     "tt_guide_render": (C.c_int, [P, P, C.POINTER(tt_camera), C.POINTER(tt_render_params), C.POINTER(tt_guide_params),
                                   P, P, C.POINTER(tt_render_stats), C.POINTER(tt_guide_stats)]),
     "tt_batch_create": (C.c_int, [P, I32, P, P, P, P, P, I32, I32, F32, C.POINTER(P)]),

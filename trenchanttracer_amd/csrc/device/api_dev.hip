@@ -373,10 +373,33 @@ void tt_scene_destroy(tt_scene *s) {
     hipStreamSynchronize(s->ctx->stream);
     free_render_state(s);
     void *ptrs[] = {s->nodes, s->tris, s->tri_normals, s->tri_mat, s->mats, s->tri_geo, s->emit_tri, s->emit_cdf,
-                    s->emit_verts, s->env_rgb, s->env_marg_cdf, s->env_cond_cdf, s->env_pdf_scale};
+                    s->emit_verts, s->env_rgb, s->env_marg_cdf, s->env_cond_cdf, s->env_pdf_scale, s->geoms};
     for (void *p : ptrs)
         if (p) hipFree(p);
     delete s;
+}
+
+tt_status tt_scene_set_geometries(tt_scene *s, const tt_geometry *g, int32_t n) {
+    if (!s || n < 0 || n > 4096 || (n > 0 && !g)) return fail(TT_ERR_INVALID, "tt_scene_set_geometries: bad arguments");
+    for (int32_t k = 0; k < n; ++k) {
+        if (g[k].type != TT_GEOM_SPHERE && g[k].type != TT_GEOM_PLANE)
+            return fail(TT_ERR_INVALID, "tt_scene_set_geometries: unknown geometry type");
+        if (g[k].material < 0 || g[k].material >= s->num_mats)
+            return fail(TT_ERR_INVALID, "tt_scene_set_geometries: material index out of range");
+        if (g[k].type == TT_GEOM_SPHERE && !(g[k].b[0] > 0.0f))
+            return fail(TT_ERR_INVALID, "tt_scene_set_geometries: sphere radius must be > 0");
+    }
+    TT_HIP(hipSetDevice(s->ctx->device), "hipSetDevice");
+    TT_HIP(hipStreamSynchronize(s->ctx->stream), "sync");
+    if (s->geoms) hipFree(s->geoms);
+    s->geoms = nullptr;
+    s->num_geoms = 0;
+    if (n == 0) return TT_OK;
+    tt_status st = dev_alloc(s->ctx, (void **)&s->geoms, sizeof(tt_geometry) * (size_t)n, "scene geometries");
+    if (st != TT_OK) return st;
+    TT_HIP(hipMemcpy(s->geoms, g, sizeof(tt_geometry) * (size_t)n, hipMemcpyHostToDevice), "copy geometries");
+    s->num_geoms = n;
+    return TT_OK;
 }
 
 tt_status tt_scene_info(const tt_scene *s, int64_t *num_nodes, int64_t *num_tris, int64_t *device_bytes) {

@@ -203,7 +203,125 @@ struct SceneView {
     const float4 *emit_verts;
     int32_t num_emit;
     float emit_inv_area;
+    // analytic primitives
+    const tt_geometry *geoms;
+    int32_t num_geoms;
 };
+
+// ------------------------------------------------------------------ analytic primitives
+// Parallelogram test, planeIntersect (include/geometry/PlaneImpl.cuh:10-52):
+// two-sided Moller-Trumbore with |det| >= 1e-5, u and v each in [0, 1], t > 1e-5
+// (double comparisons, as the literals are double). +inf when not hit.
+__device__ __forceinline__ float geom_plane_t(const tt_geometry &g, f3 o, f3 d) {
+    const f3 p0 = F3(g.a[0], g.a[1], g.a[2]);
+    const f3 e1 = F3(g.b[0], g.b[1], g.b[2]) - p0;
+    const f3 e2 = F3(g.c[0], g.c[1], g.c[2]) - p0;
+    const f3 P = cross(d, e2);
+    const float det = dot(e1, P);
+    if ((double)det > -0.00001 && (double)det < 0.00001) return INFINITY;
+    const float inv_det = 1.f / det;
+    const f3 T = o - p0;
+    const f3 Q = cross(T, e1);
+    const float u = dot(T, P) * inv_det;
+    const float v = dot(d, Q) * inv_det;
+    if (u < 0.f || u > 1.f || v < 0.f || v > 1.f) return INFINITY;
+    const float t = dot(e2, Q) * inv_det;
+    return (double)t > 0.00001 ? t : INFINITY;
+}
+
+// Sphere test, sphereIntersect (include/geometry/SphereImpl.cuh:10-32): the nearer
+// root above 0.01 (else the farther), accepted when 0.01 < t < tmax.
+__device__ __forceinline__ float geom_sphere_t(const tt_geometry &g, f3 o, f3 d, float tmax) {
+    const f3 op = F3(g.a[0], g.a[1], g.a[2]) - o;
+    const float eps = 0.01f;
+    const float b = dot(op, d);
+    float disc = b * b - dot(op, op) + g.b[0] * g.b[0];
+    if (disc >= 0.0f) {
+        disc = sqrtf(disc);
+        float t;
+        const float dist = (t = b - disc) > eps ? t : ((t = b + disc) > eps ? t : 0.0f);
+        if (0.01f < dist && dist < tmax) return dist;
+    }
+    return INFINITY;
+}
+
+__device__ __forceinline__ float geom_t(const tt_geometry &g, f3 o, f3 d, float tmax) {
+    return g.type == TT_GEOM_SPHERE ? geom_sphere_t(g, o, d, tmax) : geom_plane_t(g, o, d);
+}
+
+struct GeomArgs {
+    const tt_ray *rays;
+    tt_hit *hits;        // closest: replaced when a primitive is strictly nearer
+    int32_t *occ;        // any: set when a primitive lies inside the ray interval
+    const int32_t *n_dev;
+    const tt_geometry *geoms;
+    int32_t num_geoms;
+};
+
+// After the triangle traversal: the per-ray loop over the geometry list
+// (src/core/RenderKernel.cu:38-61); the nearest primitive below tmax wins when
+// strictly nearer than the triangle hit (a miss carries t = tmax).
+__global__ __launch_bounds__(256) void k_geom_hits(GeomArgs a) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *a.n_dev) return;
+    const tt_ray r = a.rays[i];
+    const tt_hit h = a.hits[i];
+    const f3 o = F3(r.ox, r.oy, r.oz), d = F3(r.dx, r.dy, r.dz);
+    float best = h.t;
+    int32_t id = h.tri_id;
+    for (int32_t k = 0; k < a.num_geoms; ++k) {
+        const float t = geom_t(a.geoms[k], o, d, r.tmax);
+        if (t < r.tmax && t < best) {
+            best = t;
+            id = -2 - k;
+        }
+    }
+    if (id != h.tri_id) a.hits[i] = tt_hit{id, best, 0.0f, 0.0f};
+}
+
+__global__ __launch_bounds__(256) void k_geom_occluded(GeomArgs a) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *a.n_dev || a.occ[i]) return;
+    const tt_ray r = a.rays[i];
+    const f3 o = F3(r.ox, r.oy, r.oz), d = F3(r.dx, r.dy, r.dz);
+    for (int32_t k = 0; k < a.num_geoms; ++k)
+        if (geom_t(a.geoms[k], o, d, r.tmax) < r.tmax) {
+            a.occ[i] = 1;
+            return;
+        }
+}
+
+// Surface at a hit: unnormalised normal (interpolated vertex normals of a
+// triangle; the primitive's own normal, as the reference computes it, for analytic
+// geometry), unit geometric normal and material id.
+struct Surf {
+    f3 nrm, ng;
+    int32_t mat;
+    bool analytic;
+};
+__device__ __forceinline__ Surf surface(const SceneView &sv, const tt_hit &h, f3 point) {
+    Surf s;
+    if (h.tri_id >= 0) {
+        const float4 *tn = sv.tri_normals + 3 * (int64_t)h.tri_id;
+        const float w = 1.0f - h.u - h.v;
+        s.nrm = (xyz(tn[0]) * w + xyz(tn[1]) * h.u) + xyz(tn[2]) * h.v;
+        s.ng = xyz(sv.tri_geo[h.tri_id]);
+        s.mat = sv.tri_mat[h.tri_id];
+        s.analytic = false;
+    } else {
+        const tt_geometry &g = sv.geoms[-2 - h.tri_id];
+        if (g.type == TT_GEOM_SPHERE) {
+            s.nrm = F3(point.x - g.a[0], point.y - g.a[1], point.z - g.a[2]);
+        } else {
+            const f3 p0 = F3(g.a[0], g.a[1], g.a[2]);
+            s.nrm = cross(F3(g.b[0], g.b[1], g.b[2]) - p0, F3(g.c[0], g.c[1], g.c[2]) - p0);
+        }
+        s.ng = vnorm(s.nrm);
+        s.mat = g.material;
+        s.analytic = true;
+    }
+    return s;
+}
 
 // Guiding inputs of the shade stage (all null / zero when unguided).
 constexpr int kRecFloats = 20;  // inputs (9), wi (3), pdf, 1 / throughput after the vertex (3), radiance (3), pad
@@ -276,18 +394,15 @@ __global__ __launch_bounds__(256) void k_shade_reference(ShadeArgs a) {
         f3 Lp = xyz(a.L[p]);
         const f3 ro = F3(r.ox, r.oy, r.oz), rd = F3(r.dx, r.dy, r.dz);
         bool finish = false;
-        if (h.tri_id < 0 || h.t > 1e19f) {
+        if (h.tri_id == -1 || h.t > 1e19f) {
             Lp = Lp + mask * env_reference(a.sv, rd, a.pi, a.two_pi);
             finish = true;
         } else {
-            const float4 *tn = a.sv.tri_normals + 3 * (int64_t)h.tri_id;
-            const float w = 1 - h.u - h.v;
-            const f3 n0 = xyz(tn[0]), n1 = xyz(tn[1]), n2 = xyz(tn[2]);
-            f3 nrm = (n0 * w + n1 * h.u) + n2 * h.v;
-            f3 nn = vnorm(nrm);
-            f3 nl = dot(nn, rd) < 0 ? nn : nn * -1.0f;
             f3 point = ro + rd * h.t;
-            const tt_material m = a.sv.mats[a.sv.tri_mat[h.tri_id]];
+            const Surf sf = surface(a.sv, h, point);
+            f3 nn = vnorm(sf.nrm);
+            f3 nl = dot(nn, rd) < 0 ? nn : nn * -1.0f;
+            const tt_material m = a.sv.mats[sf.mat];
             Lp = Lp + mask * F3(m.emission[0], m.emission[1], m.emission[2]);
             if (a.bounce == a.max_depth - 1 || m.type == TT_MAT_EMITTER) {
                 finish = true;
@@ -323,6 +438,50 @@ __global__ __launch_bounds__(256) void k_shade_reference(ShadeArgs a) {
                     nd = vnorm(((uu * c) * sin_t + (vv * s) * sin_t) + ww * cos_t);
                     no = point + nl * 0.0001f;
                     mask = mask * alb;
+                } else if (m.type == TT_MAT_SPECULAR) {
+                    // specSample: mirror about the unoriented normal, origin off the oriented one
+                    nd = vnorm(rd - (nn * dot(nn, rd)) * 2.0f);
+                    no = point + nl * 0.001f;
+                    mask = mask * alb;
+                } else if (m.type == TT_MAT_COAT) {
+                    // coatSample: reflect with the Fresnel probability of air (1.000293) over
+                    // the coat (1.291), else Lambert; reflect / refract / fresnel follow
+                    // CoatImpl.cuh:12-69 (powf(x, 2) taken as the rounded square x * x)
+                    const float n1 = 1.000293f, n2 = 1.291f;
+                    const f3 refl = rd - (nn * 2.0f) * dot(nn, rd);
+                    const float cos1 = dot(nn, rd * -1.0f);
+                    const float n12 = n1 / n2;
+                    const float radicand = 1 - (n12 * n12) * (1 - cos1 * cos1);
+                    f3 tdir = F3(0, 0, 0);
+                    if (!(radicand < 0)) {
+                        const float cos2 = sqrtf(radicand);
+                        tdir = cos1 > 0 ? rd * n12 + nn * (cos1 * n12 - cos2) : rd * n12 + nn * (cos1 * n12 + cos2);
+                    }
+                    float thr_f;
+                    if ((double)sqrtf(dot(tdir, tdir)) <= 0.12345 || dot(nn, tdir) > 0) {
+                        thr_f = 1.0f;
+                    } else {
+                        const float ci = dot(nn, rd * -1.0f), ct = dot(nn * -1.0f, tdir);
+                        const float rs = (n1 * ci - n2 * ct) / (n1 * ci + n2 * ct);
+                        const float rp = (n1 * ct - n2 * ci) / (n1 * ct + n2 * ci);
+                        thr_f = (rs * rs + rp * rp) / 2.0f;
+                    }
+                    if (u[0] < thr_f) {
+                        mask = mask * F3(m.specular[0], m.specular[1], m.specular[2]);
+                        nd = vnorm(refl);
+                    } else {
+                        float phi = (float)(2.0 * 3.14159265358979323846 * (double)u[1]);
+                        float r2 = u[2];
+                        float r2s = sqrtf(r2);
+                        f3 ww = vnorm(nl);
+                        f3 ax = fabsf(ww.x) >= 0.1f ? F3(0, 1, 0) : F3(1, 0, 0);
+                        f3 uu = vnorm(cross(ax, ww));
+                        f3 vv = cross(ww, uu);
+                        float c, s = tm_sincos(phi, &c);
+                        nd = vnorm(((uu * c) * r2s + (vv * s) * r2s) + ww * sqrtf(1 - r2));
+                        mask = mask * alb;
+                    }
+                    no = point + nl * 0.001f;
                 } else {
                     // refrSample
                     bool into = dot(nn, nl) > 0;
@@ -426,6 +585,12 @@ __device__ __forceinline__ float fresnel_dielectric(float cosi, float eta, float
     return 0.5f * (rs * rs + rp * rp);
 }
 
+// Fresnel reflectance of the coat layer (air 1.000293 over 1.291) at cos(wo).
+__device__ __forceinline__ float coat_fresnel(float cos_o) {
+    float cost;
+    return fresnel_dielectric(fminf(cos_o, 1.0f), 1.000293f / 1.291f, &cost);
+}
+
 // BSDF value times |cos(wi)| and the sampling pdf for a non-delta material;
 // wo, wi in the local frame of the shading normal (z up, wo.z > 0).
 __device__ void bsdf_eval(const tt_material &m, f3 wo, f3 wi, f3 *f_cos, float *pdf) {
@@ -436,6 +601,11 @@ __device__ void bsdf_eval(const tt_material &m, f3 wo, f3 wi, f3 *f_cos, float *
     if (m.type == TT_MAT_DIFFUSE) {
         *f_cos = alb * (wi.z * 0.318309886f);
         *pdf = wi.z * 0.318309886f;
+    } else if (m.type == TT_MAT_COAT) {
+        // Lambert layer under a smooth coat: the share 1 - F(wo) that enters it
+        const float keep = 1.0f - coat_fresnel(wo.z);
+        *f_cos = alb * (keep * wi.z * 0.318309886f);
+        *pdf = keep * wi.z * 0.318309886f;
     } else if (m.type == TT_MAT_CONDUCTOR) {
         float al = ggx_alpha(m.roughness);
         f3 h = vnorm(wo + wi);
@@ -487,6 +657,31 @@ __device__ bool bsdf_sample(const tt_material &m, f3 wo, bool front, const float
         *weight = schlick(alb, woh) * (G2 / G1);
         *pdf = G1 * D / (4.0f * wo.z);
         return true;
+    }
+    if (m.type == TT_MAT_SPECULAR) {
+        *delta = true;
+        *pdf = 0.0f;
+        *wi = F3(-wo.x, -wo.y, wo.z);
+        *weight = alb;
+        return true;
+    }
+    if (m.type == TT_MAT_COAT) {
+        // mirror with the coat's Fresnel probability, else the Lambert layer
+        const float F = coat_fresnel(wo.z);
+        if (u[2] <= F) {
+            *delta = true;
+            *pdf = 0.0f;
+            *wi = F3(-wo.x, -wo.y, wo.z);
+            *weight = F3(m.specular[0], m.specular[1], m.specular[2]);
+            return true;
+        }
+        float r = sqrtf(u[0]), phi = 6.28318531f * u[1];
+        float c, s = tm_sincos(phi, &c);
+        f3 d = F3(r * c, r * s, sqrtf(fmaxf(0.0f, 1.0f - u[0])));
+        *wi = d;
+        *weight = alb;
+        *pdf = (1.0f - F) * d.z * 0.318309886f;
+        return d.z > 0.0f;
     }
     if (m.type == TT_MAT_DIELECTRIC) {
         float eta = front ? 1.0f / m.ior : m.ior;
@@ -580,13 +775,13 @@ __device__ __forceinline__ void guide_inputs(const GuideShade &gd, f3 point, f3 
 }
 
 // Shading normal oriented like the geometric normal towards wo (as the shade stage).
-__device__ __forceinline__ f3 oriented_normal(const SceneView &sv, const tt_hit &h, f3 wo, f3 *ngf_out, bool *front) {
-    const f3 ng = xyz(sv.tri_geo[h.tri_id]);
+__device__ __forceinline__ f3 oriented_normal(const SceneView &sv, const tt_hit &h, f3 point, f3 wo, f3 *ngf_out,
+                                              bool *front) {
+    const Surf sf = surface(sv, h, point);
+    const f3 ng = sf.ng;
     *front = dot(wo, ng) > 0.0f;
     const f3 ngf = *front ? ng : ng * -1.0f;
-    const float4 *tn = sv.tri_normals + 3 * (int64_t)h.tri_id;
-    const float bw = 1.0f - h.u - h.v;
-    f3 ns = vnorm((xyz(tn[0]) * bw + xyz(tn[1]) * h.u) + xyz(tn[2]) * h.v);
+    f3 ns = sf.analytic ? ngf : vnorm(sf.nrm);
     if (dot(ns, ngf) < 0.0f) ns = ns * -1.0f;
     *ngf_out = ngf;
     return ns;
@@ -607,16 +802,16 @@ __global__ __launch_bounds__(256) void k_guide_inputs(GuideInArgs a) {
     if (i >= a.counts[a.cur]) return;
     float *in = a.in + (int64_t)i * kGuideIn;
     const tt_hit h = a.hits[i];
-    if (h.tri_id < 0) {
+    if (h.tri_id == -1) {
         for (int k = 0; k < kGuideIn; ++k) in[k] = 0.0f;
         return;
     }
     const tt_ray r = a.rays[i];
     const f3 rd = F3(r.dx, r.dy, r.dz), wo = rd * -1.0f;
+    const f3 point = F3(r.ox, r.oy, r.oz) + rd * h.t;
     f3 ngf;
     bool front;
-    const f3 ns = oriented_normal(a.sv, h, wo, &ngf, &front);
-    const f3 point = F3(r.ox, r.oy, r.oz) + rd * h.t;
+    const f3 ns = oriented_normal(a.sv, h, point, wo, &ngf, &front);
     guide_inputs(a.gd, point, ns, wo, in);
 }
 
@@ -645,27 +840,26 @@ __global__ __launch_bounds__(256) void k_shade_path(ShadeArgs a) {
         const bool last_delta = (a.flags[p] & 1) != 0;
         const f3 ro = F3(r.ox, r.oy, r.oz), rd = F3(r.dx, r.dy, r.dz);
         f3 add = F3(0, 0, 0);
-        if (h.tri_id < 0) {
+        if (h.tri_id == -1) {
             float pdf_env;
             f3 le = env_eval(sv, rd, &pdf_env);
             float w = (a.bounce == 0 || last_delta) ? 1.0f : power_heuristic(last_pdf, pdf_env * p_env);
             add = thr * le * w;
         } else {
-            const float4 g = sv.tri_geo[h.tri_id];
-            const f3 ng = xyz(g);
+            const f3 point = ro + rd * h.t;
+            const Surf sf = surface(sv, h, point);
+            const f3 ng = sf.ng;
             const f3 wo = rd * -1.0f;
             const bool front = dot(wo, ng) > 0.0f;
             const f3 ngf = front ? ng : ng * -1.0f;
-            const float4 *tn = sv.tri_normals + 3 * (int64_t)h.tri_id;
-            const float bw = 1.0f - h.u - h.v;
-            f3 ns = vnorm((xyz(tn[0]) * bw + xyz(tn[1]) * h.u) + xyz(tn[2]) * h.v);
+            f3 ns = sf.analytic ? ngf : vnorm(sf.nrm);
             if (dot(ns, ngf) < 0.0f) ns = ns * -1.0f;
-            const f3 point = ro + rd * h.t;
-            const tt_material m = sv.mats[sv.tri_mat[h.tri_id]];
+            const tt_material m = sv.mats[sf.mat];
             const f3 em = F3(m.emission[0], m.emission[1], m.emission[2]);
             if (front && maxc(em) > 0.0f) {
+                // analytic emitters are not in the next-event list: full weight
                 float w = 1.0f;
-                if (a.bounce > 0 && !last_delta) {
+                if (a.bounce > 0 && !last_delta && !sf.analytic) {
                     float cos_l = dot(wo, ng);
                     float pdf_l = p_emit * sv.emit_inv_area * h.t * h.t / fmaxf(cos_l, 1e-12f);
                     w = power_heuristic(last_pdf, pdf_l);
@@ -682,7 +876,7 @@ __global__ __launch_bounds__(256) void k_shade_path(ShadeArgs a) {
                 float ul[4], ub[4];
                 draws((uint32_t)local, smp, (uint32_t)(a.bounce + 1), 1u, a.seed, ul);
                 draws((uint32_t)local, smp, (uint32_t)(a.bounce + 1), 2u, a.seed, ub);
-                const bool smooth = m.type == TT_MAT_DIELECTRIC;
+                const bool smooth = m.type == TT_MAT_DIELECTRIC || m.type == TT_MAT_SPECULAR;
                 // guiding: mixture of this slot (non-delta surfaces only)
                 const bool guided = a.gd.raw != nullptr &&
                                     (m.type == TT_MAT_DIFFUSE || (m.type == TT_MAT_CONDUCTOR && m.roughness >= a.gd.min_rough));
@@ -1120,6 +1314,12 @@ hipError_t launch_bounce(tt_scene *s, RenderState *rs, const tt_render_params *p
     ta.ray_root = rs->root[cur];
     hipError_t e = launch_trace(ta, variant, false, lds, block, 0, s->ctx->num_cus, hs, tri_mode);
     if (e != hipSuccess) return e;
+    if (sv.num_geoms > 0) {
+        GeomArgs ga{rs->rays[cur], rs->hits, nullptr, rs->counts + cur, sv.geoms, sv.num_geoms};
+        hipLaunchKernelGGL(k_geom_hits, dim3(gblocks), dim3(256), 0, hs, ga);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
     ShadeArgs sa{};
     if (gr) {
         // guide inference over the hit queue: inputs, then the MLP (rows up to the device count)
@@ -1179,6 +1379,12 @@ hipError_t launch_bounce(tt_scene *s, RenderState *rs, const tt_render_params *p
     sh.ray_root = rs->sroot;
     e = launch_trace(sh, variant, true, lds, block, 0, s->ctx->num_cus, hs, kTriTwoSided);
     if (e != hipSuccess) return e;
+    if (sv.num_geoms > 0) {
+        GeomArgs go{rs->srays, nullptr, rs->socc, rs->counts + 2, sv.geoms, sv.num_geoms};
+        hipLaunchKernelGGL(k_geom_occluded, dim3(gblocks), dim3(256), 0, hs, go);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
     ApplyArgs aa{rs->counts, rs->socc, rs->scontrib, rs->acc, fp64 ? rs->acc64 : nullptr, rs->stats,
                  npix, b, p->max_depth, gr ? gr->gd : GuideShade{}};
     hipLaunchKernelGGL(k_apply_shadow, dim3(gblocks), dim3(256), 0, hs, aa);
@@ -1234,6 +1440,8 @@ SceneView scene_view(const tt_scene *s, const tt_render_params *p) {
     sv.emit_verts = (const float4 *)s->emit_verts;
     sv.num_emit = s->num_emit;
     sv.emit_inv_area = s->emit_area > 0.0f ? 1.0f / s->emit_area : 0.0f;
+    sv.geoms = s->geoms;
+    sv.num_geoms = s->num_geoms;
     return sv;
 }
 

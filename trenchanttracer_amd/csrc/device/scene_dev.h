@@ -47,6 +47,8 @@ struct tt_scene {
     float env_scale = 1.0f;
     double env_total = 0.0;
     float bmin[3] = {0.0f, 0.0f, 0.0f}, bmax[3] = {1.0f, 1.0f, 1.0f};  // mesh bounds (guide positions)
+    tt_geometry *geoms = nullptr;  // analytic primitives (device)
+    int32_t num_geoms = 0;
     size_t bytes = 0;
     struct RenderState *rs = nullptr;  // wavefront buffers, sized on first render
 };

@@ -44,6 +44,8 @@ using namespace tt;
 
 static_assert(sizeof(tt_ray) == 32, "tt_ray must be 32 bytes");
 static_assert(sizeof(tt_hit) == 16, "tt_hit must be 16 bytes");
+static_assert(sizeof(tt_material) == 48, "tt_material must be 48 bytes");
+static_assert(sizeof(tt_geometry) == 48, "tt_geometry must be 48 bytes");
 
 extern "C" {
 

@@ -222,3 +222,31 @@ def emissive_box(albedo=0.5, emission=0.5, width=64, height=64):
     mats = [material(MAT_DIFFUSE, (albedo,) * 3, emission=(emission,) * 3)]
     cam = camera((0.1, 0.2, 0.3), (0.3, -0.2, -1.0), fov=(90.0, 90.0), width=width, height=height)
     return mesh, mats, cam
+
+
+def cornell_analytic(width=128, height=128, subdiv=3):
+    """The reference's own Cornell configuration (config/cornell_dragon.json:19-62) as
+    it renders it: the five walls and the light as analytic planes (both sides), a
+    mesh in the box (an icosphere of `subdiv`, standing in for data/dragon.obj,
+    which does not ship) and four analytic spheres with the remaining materials:
+    mirror, coat, Phong metal and glass. Camera as in the config (orbit centre
+    (0, 0.2, 3), pitch 0.1, radius 4, focal 1, no aperture). Materials: 0 grey
+    diffuse 0.75, 1 light (albedo 0, emission 15), 2 sphere diffuse 0.8, 3 mirror
+    0.95, 4 coat (diffuse (0.6, 0.2, 0.2), mirror 0.9), 5 metal (0.9, 0.8, 0.6),
+    6 glass IOR 1.4. Returns (mesh, materials, geometries, camera)."""
+    from . import (MAT_COAT, MAT_CONDUCTOR, MAT_DIELECTRIC, MAT_DIFFUSE, MAT_SPECULAR, camera_from_orbit, material,
+                   plane, sphere)
+    geoms = [plane(p0, p1, p2, 1 if name == "light" else 0) for name, p0, p1, p2 in CORNELL_QUADS]
+    geoms += [sphere((-0.8, -0.7, 0.6), 0.3, 3), sphere((0.8, -0.7, 0.7), 0.3, 4),
+              sphere((-0.6, -0.75, 2.0), 0.25, 5), sphere((0.6, -0.75, 2.1), 0.25, 6)]
+    mesh = Mesh.icosphere(subdiv, 0.3, (0.0, -0.7, 1.3), mat=2)
+    mats = [material(MAT_DIFFUSE, (0.75, 0.75, 0.75)),
+            material(MAT_DIFFUSE, (0.0, 0.0, 0.0), emission=(15.0, 15.0, 15.0)),
+            material(MAT_DIFFUSE, (0.8, 0.8, 0.8)),
+            material(MAT_SPECULAR, (0.95, 0.95, 0.95)),
+            material(MAT_COAT, (0.6, 0.2, 0.2), specular=(0.9, 0.9, 0.9)),
+            material(MAT_CONDUCTOR, (0.9, 0.8, 0.6)),
+            material(MAT_DIELECTRIC, (1.0, 1.0, 1.0), ior=1.4)]
+    cam = camera_from_orbit((0.0, 0.2, 3.0), 0.0, 0.1, radius=4.0, fovx=45.0, width=width, height=height,
+                            focal=1.0, aperture=0.0)
+    return mesh, mats, geoms, cam
