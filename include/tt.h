@@ -372,6 +372,49 @@ tt_status tt_guide_grad(tt_guide *g, const float *inputs, const float *dirs, con
 tt_status tt_guide_train_step(tt_guide *g, const float *inputs, const float *dirs, const float *wts, int64_t n,
                               float *loss);
 
+/* ------------------------------------------------------------------ loaders */
+/* Radiance RGBE (.hdr), as HDRLoader::load (src/loader/HDRLoader.cpp:5-153): header
+ * "#?RADIANCE", lines up to an empty line, resolution "-Y h +X w"; flat, old-style
+ * and adaptive run-length scanlines; pixel (r, g, b, e) -> (m / 256) * 2^(e - 128);
+ * rows in file order (row 0 = top, the row the env lookup reads for +y).
+ * tt_hdr_read always returns the size; pass rgb = NULL to query it, else rgb needs
+ * capacity >= width * height * 3 floats. tt_hdr_write writes adaptive-RLE RGBE. */
+tt_status tt_hdr_read(const char *path, int32_t *width, int32_t *height, float *rgb, int64_t capacity);
+tt_status tt_hdr_write(const char *path, int32_t width, int32_t height, const float *rgb);
+
+/* Wavefront OBJ, as OBJLoader::load (src/loader/OBJLoader.cpp:3-105): v / vn (vt,
+ * vp ignored), faces v, v/t, v//n, v/t/n (1-based, fan-triangulated), groups (g),
+ * usemtl per group, mtllib = a JSON material library beside the file
+ * (MaterialLoader, src/loader/MaterialLoader.cpp:9-36: Diff / Metal / Spec / Refr /
+ * Coat with optional emission; materials in name order). Vertices are centred on
+ * their bounding box and scaled to a largest coordinate of 1 (Object::postProcess,
+ * src/core/Object.cpp:40-87). Without vn, each triangle gets its face normal.
+ * Without a library, material 0 is a grey diffuse. *num_mats receives the count;
+ * mats (may be NULL) needs mats_capacity >= it. */
+tt_status tt_obj_load(const char *path, tt_mesh **mesh, tt_material *mats, int32_t mats_capacity, int32_t *num_mats);
+
+/* JSON scene configuration, as Config (src/util/Config.cpp:3-22), SceneLoader
+ * (src/loader/SceneLoader.cpp:8-44) and Camera (src/core/Camera.cpp:12-44): Mesh
+ * objects (OBJ files, then "transform" {scale, translate, orientation in degrees}
+ * applied as S * T * (Ry * Rx * Rz), Transform.cpp:13-34) become one mesh; Plane and
+ * Sphere objects become tt_geometry records with materials of the master library.
+ * Materials: the master library ("material-file") in name order, then each mesh's
+ * library in object order. Relative paths resolve against base_dir (may be NULL). */
+typedef struct tt_config tt_config;
+typedef struct tt_config_info {
+    int32_t width, height, samples, bvh_reload, material_reload;
+    int32_t num_materials, num_geometries, num_tris;
+    float position[3];      /* orbit centre */
+    float yaw, pitch, radius, focal, aperture, fovx;
+    char work_dir[256], hdr_file[256], material_file[256];
+} tt_config_info;
+tt_status tt_config_load(const char *path, const char *base_dir, tt_config **out);
+void tt_config_destroy(tt_config *c);
+/* Any output may be NULL; mats / geoms need num_materials / num_geometries entries;
+ * *mesh receives a new mesh (NULL when the scene has no triangles). */
+tt_status tt_config_get(const tt_config *c, tt_config_info *info, tt_material *mats, tt_geometry *geoms,
+                        tt_mesh **mesh);
+
 /* Guided path tracing (path mode only; no counterpart in the reference, whose
  * renderer samples BSDFs only, RenderKernel.cu:40-60). On diffuse vertices and
  * conductors with roughness >= min_roughness, the direction is drawn from the

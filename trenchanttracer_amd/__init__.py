@@ -12,6 +12,7 @@ Device entry points run the HIP kernels of libtt.so; they raise if the library
 or a GPU is missing (there is no CPU fallback).
 """
 import ctypes as C
+import os
 import weakref
 
 import numpy as np
@@ -611,3 +612,62 @@ class DeviceScene:
 
     def __del__(self):
         self.close()
+
+
+# ---------------------------------------------------------------- loaders
+def hdr_read(path):
+    """Radiance RGBE file -> (height, width, 3) float32, row 0 = top (tt_hdr_read)."""
+    w, h = C.c_int32(), C.c_int32()
+    check(lib.tt_hdr_read(os.fsencode(path), C.byref(w), C.byref(h), None, 0))
+    out = np.empty((h.value, w.value, 3), np.float32)
+    check(lib.tt_hdr_read(os.fsencode(path), C.byref(w), C.byref(h), _vp(out), out.size))
+    return out
+
+
+def hdr_write(path, rgb):
+    """(height, width, 3) float image -> adaptive-RLE Radiance RGBE file."""
+    a = np.ascontiguousarray(rgb, np.float32)
+    check(lib.tt_hdr_write(os.fsencode(path), a.shape[1], a.shape[0], _vp(a)))
+
+
+def obj_load(path):
+    """OBJ file (+ its JSON material library) -> (Mesh, [tt_material])."""
+    h = C.c_void_p()
+    n = C.c_int32()
+    check(lib.tt_obj_load(os.fsencode(path), C.byref(h), None, 0, C.byref(n)))
+    lib.tt_mesh_destroy(h)
+    mats = (N.tt_material * n.value)()
+    check(lib.tt_obj_load(os.fsencode(path), C.byref(h), C.cast(mats, C.c_void_p), n.value, C.byref(n)))
+    return Mesh(h), list(mats)
+
+
+class SceneConfig:
+    """A reference scene configuration (tt_config): camera, materials, analytic
+    geometries and the mesh of its OBJ objects."""
+
+    def __init__(self, path, base_dir=None):
+        h = C.c_void_p()
+        check(lib.tt_config_load(os.fsencode(path), os.fsencode(base_dir) if base_dir else None, C.byref(h)))
+        try:
+            info = N.tt_config_info()
+            check(lib.tt_config_get(h, C.byref(info), None, None, None))
+            mats = (N.tt_material * max(1, info.num_materials))()
+            geoms = (N.tt_geometry * max(1, info.num_geometries))()
+            mh = C.c_void_p()
+            check(lib.tt_config_get(h, None, C.cast(mats, C.c_void_p), C.cast(geoms, C.c_void_p), C.byref(mh)))
+        finally:
+            lib.tt_config_destroy(h)
+        self.info = info
+        self.materials = list(mats)[:info.num_materials]
+        self.geometries = list(geoms)[:info.num_geometries]
+        self.mesh = Mesh(mh) if mh.value else None
+        self.width, self.height, self.samples = info.width, info.height, info.samples
+        self.hdr_file = info.hdr_file.decode()
+        self.work_dir = info.work_dir.decode()
+
+    def camera(self, width=None, height=None):
+        """Orbit camera of the configuration (Camera.cpp defaults, fovx 45)."""
+        i = self.info
+        return camera_from_orbit(tuple(i.position), i.yaw, i.pitch, radius=i.radius, fovx=i.fovx,
+                                 width=width or i.width, height=height or i.height, focal=i.focal,
+                                 aperture=i.aperture)

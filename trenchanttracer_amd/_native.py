@@ -75,6 +75,14 @@ class tt_render_stats(C.Structure):
                 ("seconds", C.c_double), ("iterations", C.c_int32), ("paths_in_flight", C.c_int32)]
 
 
+class tt_config_info(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("samples", C.c_int32), ("bvh_reload", C.c_int32),
+                ("material_reload", C.c_int32), ("num_materials", C.c_int32), ("num_geometries", C.c_int32),
+                ("num_tris", C.c_int32), ("position", C.c_float * 3), ("yaw", C.c_float), ("pitch", C.c_float),
+                ("radius", C.c_float), ("focal", C.c_float), ("aperture", C.c_float), ("fovx", C.c_float),
+                ("work_dir", C.c_char * 256), ("hdr_file", C.c_char * 256), ("material_file", C.c_char * 256)]
+
+
 class tt_guide_params(C.Structure):
     _fields_ = [("alpha", C.c_float), ("min_roughness", C.c_float), ("train", C.c_int32), ("train_paths", C.c_int32),
                 ("train_batch", C.c_int32), ("train_spp", C.c_int32)]
@@ -146,6 +154,12 @@ PROTOS = {
     "tt_guide_train_step": (C.c_int, [P, P, P, P, I64, PF]),
     "tt_debug_guide_sample": (C.c_int, [P, P, I64, U32, P]),
     "tt_scene_set_geometries": (C.c_int, [P, P, I32]),
+    "tt_hdr_read": (C.c_int, [C.c_char_p, PI32, PI32, P, I64]),
+    "tt_hdr_write": (C.c_int, [C.c_char_p, I32, I32, P]),
+    "tt_obj_load": (C.c_int, [C.c_char_p, C.POINTER(P), P, I32, PI32]),
+    "tt_config_load": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(P)]),
+    "tt_config_destroy": (None, [P]),
+    "tt_config_get": (C.c_int, [P, C.POINTER(tt_config_info), P, P, C.POINTER(P)]),
     "tt_guide_render": (C.c_int, [P, P, C.POINTER(tt_camera), C.POINTER(tt_render_params), C.POINTER(tt_guide_params),
                                   P, P, C.POINTER(tt_render_stats), C.POINTER(tt_guide_stats)]),
     "tt_batch_create": (C.c_int, [P, I32, P, P, P, P, P, I32, I32, F32, C.POINTER(P)]),

@@ -11,7 +11,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtt.so")
 
-HOST_SRC = ["host/sbvh.cpp", "host/flatten.cpp", "host/gen.cpp", "host/api_host.cpp"]
+HOST_SRC = ["host/sbvh.cpp", "host/flatten.cpp", "host/gen.cpp", "host/api_host.cpp", "host/loaders.cpp"]
 DEV_SRC = ["device/trace.hip", "device/api_dev.hip", "device/env.hip", "device/render.hip", "device/guide.hip"]
 
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
