@@ -1,6 +1,7 @@
 """Benchmark of the rendering hot path on one MI355X.
 
-Step = one closest-hit traversal launch over the whole ray set of the traversal
+Step = one closest-hit traversal launch (the speculative while-while kernel; the
+exact per-lane kernel is timed beside it) over the whole ray set of the traversal
 workload (1M-triangle soup SBVH, 8,294,400 rays: 4,147,200 coherent primary rays
 from a 1920x1080 pinhole at 2 spp + 4,147,200 incoherent cosine-hemisphere rays),
 rays and hits resident in HBM. Prints one JSON line.
@@ -26,7 +27,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--variant", type=int, default=0)
+    ap.add_argument("--variant", type=int, default=1,
+                    help="1: speculative while-while (headline); 0: exact per-lane order")
     ap.add_argument("--lds-stack", type=int, default=16)
     ap.add_argument("--block", type=int, default=64)
     ap.add_argument("--ntris", type=int, default=1_000_000)
@@ -81,22 +83,25 @@ def main():
     hits = d_hits.download((n, 4), np.int32)
     hit_frac = float((hits[:, 0] >= 0).mean())
 
-    # the speculative while-while variant (wave-ballot leaf postponement), timed
-    # beside the exact one; its hits are compared with the exact variant's
-    spec_cfg = tt.trace_cfg(variant=1, lds_stack=args.lds_stack, block=args.block)
+    # the other traversal variant timed beside the headline one (exact per-lane
+    # order if the headline is speculative, and vice versa); its hits are compared
+    other = 1 - args.variant
+    other_cfg = tt.trace_cfg(variant=other, lds_stack=args.lds_stack, block=args.block)
     d_hits2 = tt.DeviceBuffer(ctx, n * 16)
-    scene.trace_closest(d_rays, n, d_hits2, spec_cfg)
-    spec_ms = []
+    scene.trace_closest(d_rays, n, d_hits2, other_cfg)
+    other_ms = []
     for i in range(min(args.steps, 5)):
         timer.record(0)
-        scene.trace_closest(d_rays, n, d_hits2, spec_cfg)
+        scene.trace_closest(d_rays, n, d_hits2, other_cfg)
         timer.record(1)
-        spec_ms.append(timer.elapsed(0, 1))
+        other_ms.append(timer.elapsed(0, 1))
     hits2 = d_hits2.download((n, 4), np.int32)
-    spec = {"mrays_kernel": round(n / float(np.median(spec_ms)) / 1e3, 1),
-            "hit_id_diffs_vs_exact": int((hits2[:, 0] != hits[:, 0]).sum())}
+    spec = {"variant": "exact" if other == 0 else "speculative",
+            "mrays_kernel": round(n / float(np.median(other_ms)) / 1e3, 1),
+            "hit_id_diffs_vs_headline": int((hits2[:, 0] != hits[:, 0]).sum()),
+            "hits_bit_identical_to_headline": bool(np.array_equal(hits2, hits))}
     d_hits2.free()
-    log(f"speculative variant: {spec}")
+    log(f"other variant: {spec}")
 
     # measured node-fetch roofline: random 64-byte gathers over a buffer the size
     # of the device BVH (nodes + triangle records)
@@ -126,7 +131,7 @@ def main():
             "block": args.block, "hit_fraction": round(hit_frac, 4),
         },
         "kernel_ms": round(kmean, 4),
-        "speculative_variant": spec,
+        "other_variant": spec,
         "roofline": None,
         "cpu_baseline": None,
     }
@@ -151,9 +156,15 @@ def main():
         mism = int((gpu_ids != o_ids).sum())
         exact = bool(np.array_equal(hits[idx].view(np.int32), oh.view(np.int32)))
         achieved = bpr * n / (kmean * 1e-3) / 1e9
+        # traffic per launch of the default configuration (speculative, LDS 16, block
+        # 64), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of
+        # trenchanttracer_amd.profile_trace scaled by a 256 MiB copy's counts (reads x
+        # 1.99986, writes x 1.0): 11,859,322 KiB fetched, 1,561,815 KiB written
+        default_cfg = (args.variant, args.lds_stack, args.block, args.ntris) == (1, 16, 64, 1_000_000)
+        traffic = round((11_859_322.1 * 1.99986 + 1_561_814.8) * 1024 / 1e9, 2) if default_cfg else None
         result["roofline"] = {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
-            "frac": round(achieved / 8000.0, 4), "traffic": None,
+            "frac": round(achieved / 8000.0, 4), "traffic": traffic, "traffic_unit": "GB per launch",
             "bytes_per_ray": round(bpr, 1),
             "node_fetch_roofline_gbps": round(gather_gbps, 1),
             "node_fetch_frac": round(achieved / gather_gbps, 4),

@@ -260,12 +260,15 @@ tt_status tt_env_export_cdf(const tt_scene *s, float *marg, float *cond, double 
 /* Traversal launch tuning. variant: 0 = per-lane exact (single-lane order; hit ids
  * bit-identical to the host reference traversal), 1 = speculative while-while
  * (Aila-Laine postponed leaf + wave ballot). lds_stack: LDS stack entries per lane
- * (8, 16, 24 or 32); block: threads per block (64..1024, multiple of 64). */
+ * (8, 16, 24 or 32); block: threads per block (64..256, multiple of 64).
+ * xcd_order: 0 = block b traces rays [b * block, (b + 1) * block); 1 = blocks are
+ * renumbered so that each of the 8 XCDs (which take blocks round-robin) traces one
+ * contiguous eighth of the rays, keeping neighbouring rays in one L2. */
 typedef struct tt_trace_cfg {
     int32_t variant;
     int32_t lds_stack;
     int32_t block;
-    int32_t persistent;      /* 0 = one ray per lane; >0 = persistent waves fetching this many rays per wave batch */
+    int32_t xcd_order;
 } tt_trace_cfg;
 
 /* Closest hit for n rays. rays / hits are DEVICE pointers; stream is a hipStream_t

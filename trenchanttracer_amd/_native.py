@@ -52,7 +52,7 @@ class tt_geometry(C.Structure):
 
 class tt_trace_cfg(C.Structure):
     _fields_ = [("variant", C.c_int32), ("lds_stack", C.c_int32), ("block", C.c_int32),
-                ("persistent", C.c_int32)]
+                ("xcd_order", C.c_int32)]
 
 
 class tt_camera(C.Structure):

@@ -23,6 +23,7 @@ struct TraceArgs {
     const int32_t *n_dev;  // if set: ray count read on the device (capped at n)
     int32_t *reset0;       // if set: zeroed by the first lane of block 0
     int32_t *reset1;
+    int32_t xcd_order;     // 1: each XCD (blocks b = x mod 8) traces one contiguous eighth
 };
 
 // Triangle acceptance: the reference's test (front faces only, det > 1e-5) or
@@ -30,7 +31,7 @@ struct TraceArgs {
 enum TriMode { kTriReference = 0, kTriTwoSided = 1 };
 
 // Launchers (trace.hip).
-hipError_t launch_trace(const TraceArgs &a, int variant, bool any, int lds_stack, int block, int persistent,
+hipError_t launch_trace(const TraceArgs &a, int variant, bool any, int lds_stack, int block, int xcd_order,
                         int num_cus, hipStream_t s, int tri_mode = kTriReference);
 
 }  // namespace tt

@@ -77,7 +77,16 @@ template <int VARIANT, bool ANY, int S, int TRI>
 __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
     extern __shared__ int lds_stack[];
     const int tid = threadIdx.x;
-    const int64_t rid = (int64_t)blockIdx.x * blockDim.x + tid;
+    // XCD-contiguous order: the hardware hands blocks to the 8 XCDs round-robin, so
+    // block b runs on XCD b % 8; renumbering b -> (b % 8) * q + b / 8 (for the first
+    // 8q blocks, q = blocks / 8; the tail keeps its number) is a bijection that gives
+    // each XCD's L2 one contiguous range of rays
+    unsigned vb = blockIdx.x;
+    if (a.xcd_order) {
+        const unsigned q = gridDim.x >> 3;
+        if (vb < (q << 3)) vb = (vb & 7u) * q + (vb >> 3);
+    }
+    const int64_t rid = (int64_t)vb * blockDim.x + tid;
     int64_t n = a.n;
     if (a.n_dev) n = min((int64_t)*a.n_dev, n);
     if (blockIdx.x == 0 && tid == 0) {
@@ -254,15 +263,16 @@ hipError_t launch_v(const TraceArgs &a, int variant, bool any, int S, int block,
 
 }  // namespace
 
-hipError_t launch_trace(const TraceArgs &a, int variant, bool any, int lds_stack, int block, int persistent,
+hipError_t launch_trace(const TraceArgs &a, int variant, bool any, int lds_stack, int block, int xcd_order,
                         int num_cus, hipStream_t s, int tri_mode) {
-    (void)persistent;
     (void)num_cus;
     if (a.n <= 0) return hipSuccess;
     if (block < 64 || block > 256 || block % 64 != 0) return hipErrorInvalidValue;
     if ((a.n + block - 1) / block > 0x7FFFFFFF) return hipErrorInvalidValue;
-    if (tri_mode == kTriReference) return launch_v<kTriReference>(a, variant, any, lds_stack, block, s);
-    if (tri_mode == kTriTwoSided) return launch_v<kTriTwoSided>(a, variant, any, lds_stack, block, s);
+    TraceArgs b = a;
+    b.xcd_order = xcd_order ? 1 : 0;
+    if (tri_mode == kTriReference) return launch_v<kTriReference>(b, variant, any, lds_stack, block, s);
+    if (tri_mode == kTriTwoSided) return launch_v<kTriTwoSided>(b, variant, any, lds_stack, block, s);
     return hipErrorInvalidValue;
 }
 

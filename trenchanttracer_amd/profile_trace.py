@@ -1,0 +1,45 @@
+"""One closest-hit traversal launch over the traversal workload (1M-triangle soup
+SBVH, 8,294,400 rays), then one 256 MiB device-to-device copy whose known byte
+counts calibrate the FETCH_SIZE / WRITE_SIZE hardware counters. For profiler runs,
+one counter set per run:
+
+    rocprofv3 --pmc FETCH_SIZE --output-format csv -d OUT -o run -- \\
+        python3 -m trenchanttracer_amd.profile_trace [--variant 1] [--lds-stack 16] [--block 64]
+"""
+import argparse
+import sys
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--variant", type=int, default=1)
+    ap.add_argument("--lds-stack", type=int, default=16)
+    ap.add_argument("--block", type=int, default=64)
+    args = ap.parse_args()
+    import trenchanttracer_amd as tt
+    from trenchanttracer_amd import workloads
+
+    mesh, bvh, rays, n_prim = workloads.traversal()
+    n = len(rays)
+    ctx = tt.Context(0)
+    scene = tt.DeviceScene(ctx, mesh, bvh)
+    d_rays = tt.DeviceBuffer.from_array(ctx, rays)
+    d_hits = tt.DeviceBuffer(ctx, n * 16)
+    cfg = tt.trace_cfg(variant=args.variant, lds_stack=args.lds_stack, block=args.block)
+    scene.trace_closest(d_rays, n, d_hits, cfg)
+    ctx.sync()
+    nbytes = 256 << 20
+    src, dst = tt.DeviceBuffer(ctx, nbytes), tt.DeviceBuffer(ctx, nbytes)
+    src.zero()
+    ctx.sync()
+    dst.copy_from(src)
+    ctx.sync()
+    print(f"traced {n} rays (variant {args.variant}); copied {nbytes} bytes", file=sys.stderr, flush=True)
+    for b in (src, dst, d_rays, d_hits):
+        b.free()
+    scene.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()
