@@ -1543,12 +1543,16 @@ tt_status render_impl(tt_scene *s, int32_t nsc, const tt_camera *cams, const int
     const bool fp64 = p->accumulate_fp64 != 0;
     tt_ctx *ctx = s->ctx;
     if (hipSetDevice(ctx->device) != hipSuccess) return fail(TT_ERR_DEVICE, "hipSetDevice failed");
-    // path mode keeps k samples per pixel in flight (about 8M paths) so that deep
-    // bounces still fill the GPU; reference mode runs one sample per pass, which
-    // keeps the per-pixel float sums in sample order
+    // path mode keeps k samples per pixel in flight; reference mode runs one sample
+    // per pass, which keeps the per-pixel float sums in sample order. Each bounce's
+    // launches last as long as their slowest waves, so the fewer the passes the
+    // less that tail costs: by default up to 2^27 paths (188 B of state each, 25 GB)
+    // are in flight. Guided renders default to about 8M paths: the network's
+    // per-row buffers (about 1.6 KB) scale with them, and training needs passes.
     int32_t k = 1;
     if (p->mode == TT_RENDER_PATH) {
-        k = p->paths_in_flight > 0 ? std::max(1, p->paths_in_flight / npix) : std::max(1, (8 << 20) / npix);
+        const int32_t target = p->paths_in_flight > 0 ? p->paths_in_flight : gr ? (8 << 20) : (1 << 27);
+        k = std::max(1, target / npix);
         k = std::min(k, p->spp);
         while (k > 1 && (int64_t)k * npix > (1LL << 27)) --k;
     }

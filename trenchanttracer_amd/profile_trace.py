@@ -1,10 +1,14 @@
-"""One closest-hit traversal launch over the traversal workload (1M-triangle soup
-SBVH, 8,294,400 rays), then one 256 MiB device-to-device copy whose known byte
-counts calibrate the FETCH_SIZE / WRITE_SIZE hardware counters. For profiler runs,
-one counter set per run:
+"""Profiling cases.
+
+traversal (default): one closest-hit traversal launch over the traversal workload
+(1M-triangle soup SBVH, 8,294,400 rays), then one 256 MiB device-to-device copy
+whose known byte counts calibrate the FETCH_SIZE / WRITE_SIZE hardware counters.
+pathtrace: the path-trace case (1,310,720-tri displaced icosphere, sky + sun,
+1920x1080, depth 16) rendered once at 4 spp (warm-up) and once at 64 spp.
+For profiler runs, one counter set per run:
 
     rocprofv3 --pmc FETCH_SIZE --output-format csv -d OUT -o run -- \\
-        python3 -m trenchanttracer_amd.profile_trace [--variant 1] [--lds-stack 16] [--block 64]
+        python3 -m trenchanttracer_amd.profile_trace [--case traversal|pathtrace] [--variant 1] [--lds-stack 16] [--block 64]
 """
 import argparse
 import sys
@@ -15,9 +19,21 @@ def main():
     ap.add_argument("--variant", type=int, default=1)
     ap.add_argument("--lds-stack", type=int, default=16)
     ap.add_argument("--block", type=int, default=64)
+    ap.add_argument("--case", choices=("traversal", "pathtrace"), default="traversal")
     args = ap.parse_args()
     import trenchanttracer_amd as tt
     from trenchanttracer_amd import workloads
+
+    if args.case == "pathtrace":
+        mesh, mats, env, cam = workloads.pathtrace()
+        ctx = tt.Context(0)
+        s = tt.DeviceScene(ctx, mesh, tt.BVH.build(mesh), mats, env=env)
+        s.render(cam, tt.render_params(spp=4, max_depth=16, rr_depth=4, seed=7, mode=tt.RENDER_PATH))
+        img, st = s.render(cam, tt.render_params(spp=64, max_depth=16, rr_depth=4, seed=1, mode=tt.RENDER_PATH))
+        print(f"path trace 64 spp: {st}", file=sys.stderr, flush=True)
+        s.close()
+        ctx.close()
+        return
 
     mesh, bvh, rays, n_prim = workloads.traversal()
     n = len(rays)
