@@ -8,7 +8,8 @@ pathtrace: the path-trace case (1,310,720-tri displaced icosphere, sky + sun,
 For profiler runs, one counter set per run:
 
     rocprofv3 --pmc FETCH_SIZE --output-format csv -d OUT -o run -- \\
-        python3 -m trenchanttracer_amd.profile_trace [--case traversal|pathtrace] [--variant 1] [--lds-stack 16] [--block 64]
+        python3 -m trenchanttracer_amd.profile_trace [--case traversal|pathtrace] [--variant 1] [--lds-stack 16]
+            [--block 64] [--paths-in-flight N]
 """
 import argparse
 import sys
@@ -20,6 +21,7 @@ def main():
     ap.add_argument("--lds-stack", type=int, default=16)
     ap.add_argument("--block", type=int, default=64)
     ap.add_argument("--case", choices=("traversal", "pathtrace"), default="traversal")
+    ap.add_argument("--paths-in-flight", type=int, default=0, help="pathtrace: paths per pass (0: the default)")
     args = ap.parse_args()
     import trenchanttracer_amd as tt
     from trenchanttracer_amd import workloads
@@ -29,7 +31,8 @@ def main():
         ctx = tt.Context(0)
         s = tt.DeviceScene(ctx, mesh, tt.BVH.build(mesh), mats, env=env)
         s.render(cam, tt.render_params(spp=4, max_depth=16, rr_depth=4, seed=7, mode=tt.RENDER_PATH))
-        img, st = s.render(cam, tt.render_params(spp=64, max_depth=16, rr_depth=4, seed=1, mode=tt.RENDER_PATH))
+        img, st = s.render(cam, tt.render_params(spp=64, max_depth=16, rr_depth=4, seed=1, mode=tt.RENDER_PATH,
+                                                 paths_in_flight=args.paths_in_flight))
         print(f"path trace 64 spp: {st}", file=sys.stderr, flush=True)
         s.close()
         ctx.close()
