@@ -47,8 +47,10 @@ struct Lobes {
 // raw: 40 outputs (mean 3k..3k+2, log kappa 24+k, logit 32+k)
 __device__ __forceinline__ void decode_lobes(const float *raw, Lobes &L) {
     float mx = -1e30f;
+#pragma unroll
     for (int k = 0; k < kGuideLobes; ++k) mx = fmaxf(mx, raw[kGuideLobes * 4 + k]);
     float sum = 0.0f;
+#pragma unroll
     for (int k = 0; k < kGuideLobes; ++k) {
         float m0 = raw[3 * k], m1 = raw[3 * k + 1], m2 = raw[3 * k + 2];
         float len = sqrtf(m0 * m0 + m1 * m1 + m2 * m2) + 1e-6f;
@@ -60,6 +62,7 @@ __device__ __forceinline__ void decode_lobes(const float *raw, Lobes &L) {
         L.w[k] = expf(raw[kGuideLobes * 4 + k] - mx);
         sum += L.w[k];
     }
+#pragma unroll
     for (int k = 0; k < kGuideLobes; ++k) L.w[k] /= sum;
 }
 
@@ -71,6 +74,7 @@ __device__ __forceinline__ float vmf_log(float kappa, float cosang) {
 // mixture density at unit direction (x, y, z)
 __device__ __forceinline__ float mixture_pdf(const Lobes &L, float x, float y, float z) {
     float q = 0.0f;
+#pragma unroll
     for (int k = 0; k < kGuideLobes; ++k) {
         const float c = L.mu[k][0] * x + L.mu[k][1] * y + L.mu[k][2] * z;
         q += L.w[k] * expf(vmf_log(L.kappa[k], c));
@@ -81,21 +85,26 @@ __device__ __forceinline__ float mixture_pdf(const Lobes &L, float x, float y, f
 // Samples the mixture: lobe by cumulative weight (u0), then the vMF lobe by
 // Jakob's stable inversion w = 1 + log(u1 + (1 - u1) e^{-2k}) / k, phi = 2 pi u2.
 __device__ __forceinline__ void mixture_sample(const Lobes &L, float u0, float u1, float u2, float *out) {
-    int k = kGuideLobes - 1;
+    // the lobe is picked with constant indices only (selects, no break), so the
+    // mixture stays in registers
+    float kap = L.kappa[kGuideLobes - 1], mx = L.mu[kGuideLobes - 1][0], my = L.mu[kGuideLobes - 1][1],
+          mz = L.mu[kGuideLobes - 1][2];
     float acc = 0.0f;
+    bool found = false;
+#pragma unroll
     for (int j = 0; j < kGuideLobes; ++j) {
         acc += L.w[j];
-        if (u0 <= acc) {
-            k = j;
-            break;
-        }
+        const bool take = !found && u0 <= acc;
+        kap = take ? L.kappa[j] : kap;
+        mx = take ? L.mu[j][0] : mx;
+        my = take ? L.mu[j][1] : my;
+        mz = take ? L.mu[j][2] : mz;
+        found = found || take;
     }
-    const float kap = L.kappa[k];
     const float w = fminf(1.0f, fmaxf(-1.0f, 1.0f + logf(u1 + (1.0f - u1) * expf(-2.0f * kap)) / kap));
     const float sr = sqrtf(fmaxf(0.0f, 1.0f - w * w));
     float sp, cp;
     __sincosf(6.28318531f * u2, &sp, &cp);
-    const float mx = L.mu[k][0], my = L.mu[k][1], mz = L.mu[k][2];
     // frame around mu (Duff et al. 2017)
     const float s = mz >= 0.0f ? 1.0f : -1.0f;
     const float a = -1.0f / (s + mz);

@@ -815,6 +815,9 @@ __global__ __launch_bounds__(256) void k_guide_inputs(GuideInArgs a) {
     guide_inputs(a.gd, point, ns, wo, in);
 }
 
+// GUIDED: the guide's mixture, training records and roulette rule are compiled in
+// only for guided renders, so the plain kernel keeps its registers for occupancy.
+template <bool GUIDED>
 __global__ __launch_bounds__(256) void k_shade_path(ShadeArgs a) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const int32_t n = a.counts[a.cur];
@@ -838,6 +841,7 @@ __global__ __launch_bounds__(256) void k_shade_path(ShadeArgs a) {
         f3 thr = xyz(tp);
         const float last_pdf = tp.w;
         const bool last_delta = (a.flags[p] & 1) != 0;
+        float4 Ls = a.L[p];   // x: roulette weight, yzw: the path's radiance so far
         const f3 ro = F3(r.ox, r.oy, r.oz), rd = F3(r.dx, r.dy, r.dz);
         f3 add = F3(0, 0, 0);
         if (h.tri_id == -1) {
@@ -866,7 +870,7 @@ __global__ __launch_bounds__(256) void k_shade_path(ShadeArgs a) {
                 }
                 add = thr * em * w;
             }
-            const int ts = train_slot(a.gd, p);
+            const int ts = GUIDED ? train_slot(a.gd, p) : -1;
             float *rec_b = ts >= 0 ? a.gd.rec + ((int64_t)ts * a.max_depth + a.bounce) * kRecFloats : nullptr;
             if (rec_b) rec_b[12] = 0.0f;
             if (a.bounce < a.max_depth - 1 && m.type != TT_MAT_EMITTER) {
@@ -878,7 +882,7 @@ __global__ __launch_bounds__(256) void k_shade_path(ShadeArgs a) {
                 draws((uint32_t)local, smp, (uint32_t)(a.bounce + 1), 2u, a.seed, ub);
                 const bool smooth = m.type == TT_MAT_DIELECTRIC || m.type == TT_MAT_SPECULAR;
                 // guiding: mixture of this slot (non-delta surfaces only)
-                const bool guided = a.gd.raw != nullptr &&
+                const bool guided = GUIDED && a.gd.raw != nullptr &&
                                     (m.type == TT_MAT_DIFFUSE || (m.type == TT_MAT_CONDUCTOR && m.roughness >= a.gd.min_rough));
                 Lobes lobes;
                 if (guided) decode_lobes(a.gd.raw + (int64_t)i * kGuideNOutPad16, lobes);
@@ -976,9 +980,9 @@ __global__ __launch_bounds__(256) void k_shade_path(ShadeArgs a) {
                         // the product of BSDF-sampling weights instead, so that a direction
                         // the guide favours (small f cos / pdf, large expected radiance) is
                         // not culled for its small throughput
-                        float rr = a.L[p].x * w_rr;
+                        float rr = Ls.x * w_rr;
                         if (alive && a.bounce + 1 >= a.rr_depth) {
-                            float q = fminf(0.95f, a.gd.raw ? rr : maxc(thr));
+                            float q = fminf(0.95f, GUIDED ? rr : maxc(thr));
                             if (ub[3] > q) {
                                 alive = false;
                             } else {
@@ -991,7 +995,7 @@ __global__ __launch_bounds__(256) void k_shade_path(ShadeArgs a) {
                             nr = {no.x, no.y, no.z, 0.0f, wi.x, wi.y, wi.z, 1e20f};
                             cont = true;
                             a.thr[p] = make_float4(thr.x, thr.y, thr.z, pdf_b);
-                            a.L[p] = make_float4(rr, 0.0f, 0.0f, 0.0f);
+                            Ls.x = rr;
                             a.flags[p] = delta ? 1 : 0;
                             if (rec_b && !delta && pdf_b > 0.0f) {
                                 guide_inputs(a.gd, point, ns, wo, rec_b);
@@ -1010,20 +1014,13 @@ __global__ __launch_bounds__(256) void k_shade_path(ShadeArgs a) {
             }
         }
         if (maxc(add) > 0.0f) {
-            const int tc = train_slot(a.gd, p);
+            const int tc = GUIDED ? train_slot(a.gd, p) : -1;
             if (tc >= 0) credit(a.gd.rec, tc, a.bounce, a.max_depth, add);
-            if (a.acc64) {
-                double *o = a.acc64 + 3 * (int64_t)pix;
-                atomicAdd(o, (double)add.x);
-                atomicAdd(o + 1, (double)add.y);
-                atomicAdd(o + 2, (double)add.z);
-            } else {
-                float *o = a.acc + 3 * (int64_t)pix;
-                atomicAdd(o, add.x);
-                atomicAdd(o + 1, add.y);
-                atomicAdd(o + 2, add.z);
-            }
+            Ls.y += add.x;
+            Ls.z += add.y;
+            Ls.w += add.z;
         }
+        a.L[p] = Ls;   // the path's own slot: no atomics; k_resolve sums a pixel's paths
     }
     const int slot = wave_append(cont, &a.counts[a.nxt]);
     if (cont) {
@@ -1043,10 +1040,9 @@ struct ApplyArgs {
     const int32_t *counts;
     const int32_t *occ;
     const float4 *contrib;
-    float *acc;
-    double *acc64;
+    float4 *L;           // per-path radiance in yzw
     unsigned long long *stats;
-    int32_t npix, bounce, max_depth;
+    int32_t bounce, max_depth;
     GuideShade gd;
 };
 
@@ -1057,19 +1053,47 @@ __global__ __launch_bounds__(256) void k_apply_shadow(ApplyArgs a) {
     if (i >= n || a.occ[i]) return;
     const float4 c = a.contrib[i];
     const int32_t path = __float_as_int(c.w);
-    const int32_t p = path % a.npix;
     const int tc = train_slot(a.gd, path);
     if (tc >= 0) credit(a.gd.rec, tc, a.bounce, a.max_depth, F3(c.x, c.y, c.z));
-    if (a.acc64) {
-        double *o = a.acc64 + 3 * (int64_t)p;
-        atomicAdd(o, (double)c.x);
-        atomicAdd(o + 1, (double)c.y);
-        atomicAdd(o + 2, (double)c.z);
+    // a path has at most one next-event ray per bounce: plain read-modify-write
+    float4 l = a.L[path];
+    l.y += c.x;
+    l.z += c.y;
+    l.w += c.z;
+    a.L[path] = l;
+}
+
+// End of a path-mode pass: each pixel's radiance is the sum of its kk paths'
+// radiance (paths s * npix + pixel), added in sample order to the float or FP64
+// accumulation, so images do not depend on the order threads finish.
+__global__ __launch_bounds__(256) void k_resolve(const float4 *L, int32_t npix, int32_t kk, float *acc,
+                                                 double *acc64) {
+    const int32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= npix) return;
+    if (acc64) {
+        double r = 0.0, g = 0.0, b = 0.0;
+        for (int32_t s = 0; s < kk; ++s) {
+            const float4 v = L[(int64_t)s * npix + pix];
+            r += (double)v.y;
+            g += (double)v.z;
+            b += (double)v.w;
+        }
+        double *o = acc64 + 3 * (int64_t)pix;
+        o[0] += r;
+        o[1] += g;
+        o[2] += b;
     } else {
-        float *o = a.acc + 3 * (int64_t)p;
-        atomicAdd(o, c.x);
-        atomicAdd(o + 1, c.y);
-        atomicAdd(o + 2, c.z);
+        float r = 0.0f, g = 0.0f, b = 0.0f;
+        for (int32_t s = 0; s < kk; ++s) {
+            const float4 v = L[(int64_t)s * npix + pix];
+            r += v.y;
+            g += v.z;
+            b += v.w;
+        }
+        float *o = acc + 3 * (int64_t)pix;
+        o[0] += r;
+        o[1] += g;
+        o[2] += b;
     }
 }
 
@@ -1365,7 +1389,8 @@ hipError_t launch_bounce(tt_scene *s, RenderState *rs, const tt_render_params *p
         hipLaunchKernelGGL(k_shade_reference, dim3(gblocks), dim3(256), 0, hs, sa);
         return hipGetLastError();
     }
-    hipLaunchKernelGGL(k_shade_path, dim3(gblocks), dim3(256), 0, hs, sa);
+    if (gr) hipLaunchKernelGGL(k_shade_path<true>, dim3(gblocks), dim3(256), 0, hs, sa);
+    else hipLaunchKernelGGL(k_shade_path<false>, dim3(gblocks), dim3(256), 0, hs, sa);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     TraceArgs sh{};
@@ -1385,8 +1410,7 @@ hipError_t launch_bounce(tt_scene *s, RenderState *rs, const tt_render_params *p
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    ApplyArgs aa{rs->counts, rs->socc, rs->scontrib, rs->acc, fp64 ? rs->acc64 : nullptr, rs->stats,
-                 npix, b, p->max_depth, gr ? gr->gd : GuideShade{}};
+    ApplyArgs aa{rs->counts, rs->socc, rs->scontrib, rs->L, rs->stats, b, p->max_depth, gr ? gr->gd : GuideShade{}};
     hipLaunchKernelGGL(k_apply_shadow, dim3(gblocks), dim3(256), 0, hs, aa);
     return hipGetLastError();
 }
@@ -1413,6 +1437,11 @@ hipError_t launch_pass(tt_scene *s, RenderState *rs, const tt_render_params *p, 
     hipLaunchKernelGGL(k_generate, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, hs, g);
     hipError_t e = hipGetLastError();
     for (int32_t b = 0; e == hipSuccess && b < p->max_depth; ++b) e = launch_bounce(s, rs, p, sv, cap, npix, b, fp64, hs, gr);
+    if (e == hipSuccess && p->mode == TT_RENDER_PATH) {
+        hipLaunchKernelGGL(k_resolve, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, hs, (const float4 *)rs->L,
+                           npix, kk, fp64 ? nullptr : rs->acc, fp64 ? rs->acc64 : nullptr);
+        e = hipGetLastError();
+    }
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_advance_sample, dim3(1), dim3(1), 0, hs, rs->sample_dev, kk);
         e = hipGetLastError();
