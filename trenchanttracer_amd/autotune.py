@@ -58,17 +58,19 @@ def tune_trace(ctx, scene, d_rays, n, d_hits, variants=(0, 1), lds=(8, 16, 24), 
     return trace_cfg(best[1], best[2], best[3]), best[0]
 
 
-def tune_render(render_fn, base, paths=(0, 4 << 20, 16 << 20), lds=(8, 16), blocks=(64, 128), graphs=(0, 1),
-                probe_spp=2, cache=None, key=None):
+def tune_render(render_fn, base, paths=(0, 16 << 20), lds=(16,), blocks=(64,), graphs=(0, 1),
+                probe_spp=None, cache=None, key=None):
     """Fastest render configuration. render_fn(params) -> stats; base: a
-    tt_render_params-like dict of the real render (spp is replaced by probe_spp)."""
+    tt_render_params-like dict of the real render. Probes run at the real spp
+    unless probe_spp is given: the number of passes, which paths in flight decide,
+    only shows at the real sample count."""
     db = _load(cache)
     if key and key in db:
         return db[key]
     best = None
     for pf, s, b, g in itertools.product(paths, lds, blocks, graphs):
         prm = dict(base)
-        prm.update(spp=probe_spp, paths_in_flight=pf, lds_stack=s, block=b, use_graph=bool(g))
+        prm.update(spp=probe_spp or base["spp"], paths_in_flight=pf, lds_stack=s, block=b, use_graph=bool(g))
         render_fn(render_params(**prm))           # warm (allocations, code objects)
         st = render_fn(render_params(**prm))
         t = st["seconds"]
