@@ -323,7 +323,8 @@ typedef struct tt_render_params {
     float tmin;               /* reference mode ray tmin (0.01, RenderKernel.cu:110) */
     int32_t use_graph;        /* capture one sample pass in a hipGraph and replay it */
     int32_t paths_in_flight;  /* path mode: target paths per pass, rounded to whole samples (0 = up to
-                                 2^27 paths, 188 B of state each; guided renders: about 8M) */
+                                 2^27 paths, 188 B of state each, 416 B with a guide;
+                                 guided renders that train the network: about 8M) */
     int32_t lds_stack;        /* traversal LDS stack entries per lane (0 = 16) */
     int32_t block;            /* traversal block size (0 = 64) */
     int32_t variant;          /* path-mode traversal loop: -1 speculative (default), 0 exact, 1 speculative */

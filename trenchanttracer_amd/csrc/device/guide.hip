@@ -554,6 +554,22 @@ tt_status guide_alloc_batch(tt_guide *g, int64_t rows) {
     return st;
 }
 
+// Inference rows of the renderer: 228 B each (inputs and mixture), so a render can
+// keep as many paths in flight as the tracer itself; the training batch above
+// (about 1.6 KB a row) is sized by the Adam batch alone.
+tt_status guide_alloc_infer(tt_guide *g, int64_t rows) {
+    if (g->icap >= rows) return TT_OK;
+    if (g->iin) hipFree(g->iin);
+    if (g->iraw) hipFree(g->iraw);
+    g->iin = g->iraw = nullptr;
+    g->icap = 0;
+    const int64_t R = (rows + 15) & ~15LL;
+    tt_status st = dev_alloc(g->ctx, (void **)&g->iin, (size_t)R * kGuideIn * 4, "guide inference inputs");
+    if (st == TT_OK) st = dev_alloc(g->ctx, (void **)&g->iraw, (size_t)R * kGuideNOutPad16 * 4, "guide inference outputs");
+    if (st == TT_OK) g->icap = R;
+    return st;
+}
+
 static hipError_t refresh_bf16(tt_guide *g, hipStream_t s) {
     for (int k = 0; k < 4; ++k) {
         const GuideLayer &l = kGuideLayers[k];
@@ -591,14 +607,14 @@ hipError_t guide_forward(tt_guide *g, int64_t n, const int32_t *n_dev, hipStream
 
 // Inference over the first n rows of g->batch.in (device count n_dev optional):
 // the fused kernel, raw outputs (fp32, 48 wide) into g->batch.raw.
-hipError_t guide_infer(tt_guide *g, int64_t n, const int32_t *n_dev, hipStream_t s) {
+hipError_t guide_infer(tt_guide *g, const float *in, float *raw, int64_t n, const int32_t *n_dev, hipStream_t s) {
     InferArgs a{};
-    a.in = g->batch.in;
+    a.in = in;
     for (int k = 0; k < 4; ++k) {
         a.w[k] = g->wf[k];
         a.bias[k] = g->params + layer_offset(k) + (int64_t)kGuideLayers[k].in * kGuideLayers[k].out;
     }
-    a.raw = g->batch.raw;
+    a.raw = raw;
     a.n = n;
     a.n_dev = n_dev;
     const int64_t tiles = (n + 15) / 16;
@@ -775,7 +791,7 @@ void tt_guide_destroy(tt_guide *g) {
     GuideBatch &b = g->batch;
     void *ptrs[] = {g->params, g->m1, g->m2, g->grad, g->scalars, g->partial, b.in, b.x, b.h[0], b.h[1], b.h[2],
                     b.raw, b.draw, b.dh[0], b.dh[1], b.dh[2], b.dirs, b.wts, g->wf[0], g->wf[1], g->wf[2], g->wf[3],
-                    g->wb[0], g->wb[1], g->wb[2], g->wb[3], g->train_buf, g->train_count, g->rec};
+                    g->wb[0], g->wb[1], g->wb[2], g->wb[3], g->train_buf, g->train_count, g->rec, g->iin, g->iraw};
     for (void *q : ptrs)
         if (q) hipFree(q);
     delete g;
@@ -805,7 +821,7 @@ tt_status tt_guide_forward(tt_guide *g, const float *inputs, int64_t n, float *r
     if (st != TT_OK) return st;
     hipStream_t s = g->ctx->stream;
     TT_HIPG(hipMemcpyAsync(g->batch.in, inputs, (size_t)n * kGuideIn * 4, hipMemcpyHostToDevice, s), "copy inputs");
-    TT_HIPG(guide_infer(g, n, nullptr, s), "guide inference");
+    TT_HIPG(guide_infer(g, g->batch.in, g->batch.raw, n, nullptr, s), "guide inference");
     TT_HIPG(hipMemcpy2DAsync(raw_out, kGuideNOut * 4, g->batch.raw, kGuideNOutPad16 * 4, kGuideNOut * 4, (size_t)n,
                              hipMemcpyDeviceToHost, s), "copy out");
     TT_HIPG(hipStreamSynchronize(s), "sync");

@@ -1347,13 +1347,13 @@ hipError_t launch_bounce(tt_scene *s, RenderState *rs, const tt_render_params *p
     ShadeArgs sa{};
     if (gr) {
         // guide inference over the hit queue: inputs, then the MLP (rows up to the device count)
-        GuideInArgs ga{gr->gd, sv, rs->rays[cur], rs->hits, rs->counts, cur, gr->g->batch.in};
+        GuideInArgs ga{gr->gd, sv, rs->rays[cur], rs->hits, rs->counts, cur, gr->g->iin};
         hipLaunchKernelGGL(k_guide_inputs, dim3(gblocks), dim3(256), 0, hs, ga);
         e = hipGetLastError();
-        if (e == hipSuccess) e = guide_infer(gr->g, cap, rs->counts + cur, hs);
+        if (e == hipSuccess) e = guide_infer(gr->g, gr->g->iin, gr->g->iraw, cap, rs->counts + cur, hs);
         if (e != hipSuccess) return e;
         sa.gd = gr->gd;
-        sa.gd.raw = gr->g->batch.raw;
+        sa.gd.raw = gr->g->iraw;
     }
     sa.sv = sv;
     sa.tab = (const SceneTable *)rs->table;
@@ -1576,11 +1576,12 @@ tt_status render_impl(tt_scene *s, int32_t nsc, const tt_camera *cams, const int
     // per pass, which keeps the per-pixel float sums in sample order. Each bounce's
     // launches last as long as their slowest waves, so the fewer the passes the
     // less that tail costs: by default up to 2^27 paths (188 B of state each, 25 GB)
-    // are in flight. Guided renders default to about 8M paths: the network's
-    // per-row buffers (about 1.6 KB) scale with them, and training needs passes.
+    // are in flight, guided renders with a frozen network included (228 B more per
+    // path for the network's inputs and mixture). Renders that train the network
+    // default to about 8M paths: it learns between passes.
     int32_t k = 1;
     if (p->mode == TT_RENDER_PATH) {
-        const int32_t target = p->paths_in_flight > 0 ? p->paths_in_flight : gr ? (8 << 20) : (1 << 27);
+        const int32_t target = p->paths_in_flight > 0 ? p->paths_in_flight : (gr && gr->train) ? (8 << 20) : (1 << 27);
         k = std::max(1, target / npix);
         k = std::min(k, p->spp);
         while (k > 1 && (int64_t)k * npix > (1LL << 27)) --k;
@@ -1591,7 +1592,8 @@ tt_status render_impl(tt_scene *s, int32_t nsc, const tt_camera *cams, const int
     hipStream_t hs = ctx->stream;
     if (gr) {
         // inference rows: one per queued path; training rows: batch_rows
-        st = guide_alloc_batch(gr->g, std::max<int64_t>((int64_t)npix * k, gr->batch_rows));
+        st = guide_alloc_infer(gr->g, (int64_t)npix * k);
+        if (st == TT_OK && gr->train) st = guide_alloc_batch(gr->g, gr->batch_rows);
         if (st == TT_OK && gr->train) {
             const int32_t want = std::max(1, gr->gd.ntrain);
             gr->gd.stride = std::max(1, (npix * k) / want);
