@@ -105,6 +105,42 @@ __device__ __forceinline__ int wave_append(bool want, int32_t *counter) {
     return want ? base + off : -1;
 }
 
+// Slots in up to two queues for the lanes of a block of kAppendBlock threads that
+// want them, with one counter update per block and queue instead of one per wave:
+// a wavefront bounce appends to the same two counters from millions of waves, and
+// updates of one address are serialised in L2. Every thread of the block must call
+// it (it synchronises the block). Slot order follows the lanes within a block;
+// blocks take their ranges in the order their updates reach L2.
+constexpr int kAppendBlock = 256;
+__device__ __forceinline__ void block_append2(bool w0, int32_t *c0, bool w1, int32_t *c1, int *s0, int *s1) {
+    constexpr int kWaves = kAppendBlock / 64;
+    __shared__ int cnt[2][kWaves];
+    __shared__ int base[2];
+    const unsigned long long m0 = __ballot(w0), m1 = __ballot(w1);
+    const int lane = __lane_id(), wv = (int)threadIdx.x >> 6;
+    if (lane == 0) {
+        cnt[0][wv] = __popcll(m0);
+        cnt[1][wv] = __popcll(m1);
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        const int q = (int)threadIdx.x;
+        int t = 0;
+        for (int k = 0; k < kWaves; ++k) t += cnt[q][k];
+        int32_t *c = q ? c1 : c0;
+        base[q] = (t > 0 && c) ? atomicAdd(c, t) : 0;
+    }
+    __syncthreads();
+    int p0 = base[0], p1 = base[1];
+    for (int k = 0; k < wv; ++k) {
+        p0 += cnt[0][k];
+        p1 += cnt[1][k];
+    }
+    const unsigned long long below = (1ull << lane) - 1ull;
+    *s0 = w0 ? p0 + __popcll(m0 & below) : -1;
+    *s1 = w1 ? p1 + __popcll(m1 & below) : -1;
+}
+
 struct CamArgs {
     f3 pos, hax, vax, middle, horizontal, vertical;
     float focal, aperture, two_pi;
@@ -375,7 +411,7 @@ __device__ f3 env_reference(const SceneView &sv, f3 d, float pi, float two_pi) {
 }
 
 // ------------------------------------------------------------------ reference shade
-__global__ __launch_bounds__(256) void k_shade_reference(ShadeArgs a) {
+__global__ __launch_bounds__(kAppendBlock) void k_shade_reference(ShadeArgs a) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const int32_t n = a.counts[a.cur];
     if (i == 0) atomicAdd(&a.stats[0], (unsigned long long)n);
@@ -526,7 +562,8 @@ __global__ __launch_bounds__(256) void k_shade_reference(ShadeArgs a) {
             o[2] = o[2] + Lp.z * a.inv_spp;
         }
     }
-    const int slot = wave_append(cont, &a.counts[a.nxt]);
+    int slot, unused;
+    block_append2(cont, &a.counts[a.nxt], false, nullptr, &slot, &unused);
     if (cont) {
         a.rays_out[slot] = nr;
         a.path_out[slot] = p;
@@ -818,7 +855,7 @@ __global__ __launch_bounds__(256) void k_guide_inputs(GuideInArgs a) {
 // GUIDED: the guide's mixture, training records and roulette rule are compiled in
 // only for guided renders, so the plain kernel keeps its registers for occupancy.
 template <bool GUIDED>
-__global__ __launch_bounds__(256) void k_shade_path(ShadeArgs a) {
+__global__ __launch_bounds__(kAppendBlock) void k_shade_path(ShadeArgs a) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const int32_t n = a.counts[a.cur];
     if (i == 0) atomicAdd(&a.stats[0], (unsigned long long)n);
@@ -1022,13 +1059,13 @@ __global__ __launch_bounds__(256) void k_shade_path(ShadeArgs a) {
         }
         a.L[p] = Ls;   // the path's own slot: no atomics; k_resolve sums a pixel's paths
     }
-    const int slot = wave_append(cont, &a.counts[a.nxt]);
+    int slot, sslot;
+    block_append2(cont, &a.counts[a.nxt], shadow, &a.counts[2], &slot, &sslot);
     if (cont) {
         a.rays_out[slot] = nr;
         a.path_out[slot] = p;
         a.root_out[slot] = root;
     }
-    const int sslot = wave_append(shadow, &a.counts[2]);
     if (shadow) {
         a.srays[sslot] = sr;
         a.sroot[sslot] = root;
@@ -1324,6 +1361,7 @@ hipError_t launch_bounce(tt_scene *s, RenderState *rs, const tt_render_params *p
     const int variant = p->mode == TT_RENDER_REFERENCE ? 0 : (p->variant >= 0 ? p->variant : 1);
     const int lds = p->lds_stack > 0 ? p->lds_stack : 16;
     const int block = p->block > 0 ? p->block : 64;
+    static_assert(kAppendBlock == 256, "shade launches use the 256-thread grid");
     const unsigned gblocks = (unsigned)((cap + 255) / 256);
     TraceArgs ta{};
     ta.nodes = (const float4 *)s->nodes;
@@ -1386,11 +1424,11 @@ hipError_t launch_bounce(tt_scene *s, RenderState *rs, const tt_render_params *p
     sa.pi = p->pi;
     sa.two_pi = p->two_pi;
     if (p->mode == TT_RENDER_REFERENCE) {
-        hipLaunchKernelGGL(k_shade_reference, dim3(gblocks), dim3(256), 0, hs, sa);
+        hipLaunchKernelGGL(k_shade_reference, dim3(gblocks), dim3(kAppendBlock), 0, hs, sa);
         return hipGetLastError();
     }
-    if (gr) hipLaunchKernelGGL(k_shade_path<true>, dim3(gblocks), dim3(256), 0, hs, sa);
-    else hipLaunchKernelGGL(k_shade_path<false>, dim3(gblocks), dim3(256), 0, hs, sa);
+    if (gr) hipLaunchKernelGGL(k_shade_path<true>, dim3(gblocks), dim3(kAppendBlock), 0, hs, sa);
+    else hipLaunchKernelGGL(k_shade_path<false>, dim3(gblocks), dim3(kAppendBlock), 0, hs, sa);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     TraceArgs sh{};
