@@ -159,9 +159,9 @@ def main():
         # traffic per launch of the default configuration (speculative, LDS 16, block
         # 64), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of
         # trenchanttracer_amd.profile_trace scaled by a 256 MiB copy's counts (reads x
-        # 1.99986, writes x 1.0): 11,859,322 KiB fetched, 1,561,815 KiB written
+        # 1.99989, writes x 1.0): 11,283,264 KiB fetched, 129,600 KiB written (the hits)
         default_cfg = (args.variant, args.lds_stack, args.block, args.ntris) == (1, 16, 64, 1_000_000)
-        traffic = round((11_859_322.1 * 1.99986 + 1_561_814.8) * 1024 / 1e9, 2) if default_cfg else None
+        traffic = round((11_283_264.4 * 1.99989 + 129_600.0) * 1024 / 1e9, 2) if default_cfg else None
         result["roofline"] = {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
             "frac": round(achieved / 8000.0, 4), "traffic": traffic, "traffic_unit": "GB per launch",

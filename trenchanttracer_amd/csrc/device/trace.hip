@@ -49,12 +49,15 @@ __device__ __forceinline__ float span_end(float a0, float a1, float b0, float b1
     return bf(min(min(fb(fmx(a0, a1)), fb(fmx(b0, b1))), z));
 }
 
+// The overflow array lives outside the struct: a struct that holds a dynamically
+// indexed array is kept in scratch as a whole, and then every push and pop would
+// load and store sp and the LDS pointer through scratch as well.
 template <int S>
 struct Stack {
     int *lds;       // this lane's column: lds[k * stride]
     int stride;
     int sp;         // entries held (the sentinel lies below entry 0)
-    int spill[kStackCap - S];
+    int *spill;     // this lane's private overflow, kStackCap - S entries
 
     __device__ __forceinline__ void push(int v) {
         if (sp < S) {
@@ -110,10 +113,12 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
     const float idirz = 1.0f / (fabsf(dz) > ooeps ? dz : copysignf(ooeps, dz));
     const float oodx = ox * idirx, oody = oy * idiry, oodz = oz * idirz;
 
+    int spill[kStackCap - S];
     Stack<S> st;
     st.lds = lds_stack + tid;
     st.stride = blockDim.x;
     st.sp = 0;
+    st.spill = spill;
 
     int leaf_addr = 0;
     int node_addr = (valid && a.num_nodes > 0) ? (a.ray_root ? a.ray_root[rid] : 0) : kSentinel;
