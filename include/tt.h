@@ -456,6 +456,11 @@ tt_status tt_guide_render(tt_scene *s, tt_guide *g, const tt_camera *cam, const 
  * floats {direction.xyz, mixture density at the direction}. Synchronous. */
 tt_status tt_debug_guide_sample(tt_ctx *ctx, const float *raw, int64_t n, uint32_t seed, float *out);
 
+/* Check hook for the fused inference kernel: the same n x 40 outputs as
+ * tt_guide_forward, computed by the training path's separate encoding and layer
+ * kernels. The two agree bit for bit. Synchronous. */
+tt_status tt_debug_guide_forward_unfused(tt_guide *g, const float *inputs, int64_t n, float *raw_out);
+
 /* Reproducibility hook for the render math: applies op (0 sin, 1 cos, 2 pow(x, y)) of
  * the device sampler math to n device values; y is read for op 2. Synchronous. */
 tt_status tt_debug_math(tt_ctx *ctx, int32_t op, const float *x, const float *y, float *out, int64_t n);

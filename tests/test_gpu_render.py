@@ -53,6 +53,20 @@ def test_reference_render_cornell_matches_oracle(tt, oracle, ctx):
     assert img.mean() > 0.01
 
 
+def test_reference_render_large_frame_matches_oracle(tt, oracle, ctx):
+    """1024x640 (655,360 paths in one queue): more rays than the traversal kernel's
+    grid holds at once (32 one-wave blocks per CU), so its blocks stride over the
+    queue more than once. Bit-identical to the oracle at 1 spp."""
+    from trenchanttracer_amd import workloads
+    _say("reference render cornell 1024x640")
+    mesh, mats, cam = workloads.cornell(width=1024, height=640)
+    img, ref, st = _ref_case(tt, oracle, ctx, mesh, mats, cam, 1, 8, 2)
+    same = np.mean(img.view(np.int32) == ref.view(np.int32))
+    _say(f"cornell 1024x640 reference render: {same * 100:.4f}% of channels bit-identical, stats {st}")
+    assert same == 1.0
+    assert img.mean() > 0.01
+
+
 def test_reference_render_metal_and_glass_match_oracle(tt, oracle, ctx):
     _say("reference render metal/glass")
     from trenchanttracer_amd import workloads
@@ -282,4 +296,27 @@ def test_path_mode_is_deterministic(tt, ctx):
                           paths_in_flight=64 * 48 * 4)
     _, c64, _ = s.render(cam, p4, want_fp64=True)
     assert np.allclose(c64, a64, rtol=1e-12, atol=1e-12)
+    s.close()
+
+
+def test_path_mode_large_queues_match_small_passes(tt, ctx):
+    """256x256 at 32 spp in one pass is 2,097,152 paths: the queue kernels' blocks
+    stride over each bounce's queue several times (traversal grid 32 one-wave blocks
+    per CU, shade grid 16 blocks of 256 per CU). The same samples rendered as 32
+    passes of 65,536 paths (one stride each) give the same FP64 image up to the
+    grouping of the per-pixel sums."""
+    from trenchanttracer_amd import workloads
+    _say("path-mode large queues")
+    mesh, mats, cam = workloads.cornell(subdiv=3, width=256, height=256)
+    s = tt.DeviceScene(ctx, mesh, tt.BVH.build(mesh), mats)
+    big = tt.render_params(spp=32, max_depth=8, rr_depth=3, seed=9, mode=tt.RENDER_PATH, fp64=True)
+    small = tt.render_params(spp=32, max_depth=8, rr_depth=3, seed=9, mode=tt.RENDER_PATH, fp64=True,
+                             paths_in_flight=256 * 256)
+    _, a64, sa = s.render(cam, big, want_fp64=True)
+    _, b64, sb = s.render(cam, small, want_fp64=True)
+    _say(f"one pass: {sa}; 32 passes: {sb}")
+    assert sa["paths_in_flight"] == 256 * 256 * 32 and sb["paths_in_flight"] == 256 * 256
+    assert sa["rays_extension"] == sb["rays_extension"] and sa["rays_shadow"] == sb["rays_shadow"]
+    assert np.allclose(a64, b64, rtol=1e-12, atol=1e-12)
+    assert a64.mean() > 0.01
     s.close()

@@ -334,6 +334,14 @@ class Guide:
         check(lib.tt_guide_forward(self._h, _vp(x), len(x), _vp(out)))
         return out
 
+    def forward_unfused(self, inputs):
+        """Same outputs as forward() from the training path's separate encoding and
+        layer kernels (tt_debug_guide_forward_unfused)."""
+        x = np.ascontiguousarray(inputs, np.float32).reshape(-1, 9)
+        out = np.empty((len(x), 40), np.float32)
+        check(lib.tt_debug_guide_forward_unfused(self._h, _vp(x), len(x), _vp(out)))
+        return out
+
     def grad(self, inputs, dirs, wts):
         x = np.ascontiguousarray(inputs, np.float32).reshape(-1, 9)
         d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)

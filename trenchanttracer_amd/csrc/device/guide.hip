@@ -130,16 +130,22 @@ __global__ __launch_bounds__(256) void k_gemm(const __bf16 *__restrict__ A, cons
 
 // ------------------------------------------------------------------ fused inference
 // raw[r] = MLP(encode(in[r])) for r < rows: encoding, 3 hidden layers and the
-// output layer in one kernel. The block stages all BF16 weights (Bt[out][in],
-// rows padded by 8 elements against LDS bank conflicts) and FP32 biases in LDS
-// once, then its 4 waves take 16-row tiles: features are computed straight into
-// the A operand registers, each layer's output goes through a per-wave LDS tile
-// (D layout -> A layout) and the last layer writes FP32. Arithmetic, rounding
-// points and accumulation order are those of k_encode + k_gemm, so the outputs
-// are identical to the unfused forward pass.
+// output layer in one kernel. One block of kInferWaves waves per CU stages all
+// BF16 weights (Bt[out][in], rows padded by 8 elements against LDS bank conflicts)
+// and FP32 biases in LDS once; its waves then take 16-row tiles. A tile's features
+// are computed once per (row, input) frequency chain (144 chains over 64 lanes)
+// into the wave's LDS tile, read back in the MFMA A layout, and the same tile then
+// carries each hidden layer's output (D layout -> A layout); the last layer writes
+// FP32. Arithmetic, rounding points and accumulation order are those of k_encode +
+// k_gemm, so the outputs are identical to the unfused forward pass.
+constexpr int kInferWaves = 12;
 constexpr int kLd1 = kGuideK1 + 8, kLdH = kGuideH + 8;
-constexpr int kInferLds = 2 * (kGuideH * kLd1 + 2 * kGuideH * kLdH + kGuideNOutPad16 * kLdH + 4 * 16 * kLdH) +
+constexpr int kInferTile = 16 * kLd1;   // bf16 elements per wave tile (features, then activations)
+constexpr int kInferLds = 2 * (kGuideH * kLd1 + 2 * kGuideH * kLdH + kGuideNOutPad16 * kLdH + kInferWaves * kInferTile) +
                           4 * (3 * kGuideH + kGuideNOutPad16);
+static_assert(kInferLds <= 160 * 1024, "inference block exceeds the CU's LDS");
+static_assert(16 * kLdH <= kInferTile, "activation tile must fit in the feature tile");
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 
 struct InferArgs {
     const float *in;
@@ -160,17 +166,17 @@ __device__ __forceinline__ void stage_weights(__bf16 *dst, const __bf16 *src, in
     }
 }
 
-// hidden layer from the per-wave activation tile: acc[t] = act (16 x 64) . W^T
-template <int NT>
-__device__ __forceinline__ void hidden_mfma(const __bf16 *act, const __bf16 *w, int lr, int lq, f32x4 *acc) {
+// A (16 x 32 s.. of a row-major LDS tile with row stride LD) times W^T over K
+template <int NT, int K, int LD>
+__device__ __forceinline__ void tile_mfma(const __bf16 *act, const __bf16 *w, int lr, int lq, f32x4 *acc) {
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int s = 0; s < kGuideH / 32; ++s) {
-        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(act + lr * kLdH + 32 * s + 8 * lq);
+    for (int s = 0; s < K / 32; ++s) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(act + lr * LD + 32 * s + 8 * lq);
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-            const bf16x8 b = *reinterpret_cast<const bf16x8 *>(w + (16 * t + lr) * kLdH + 32 * s + 8 * lq);
+            const bf16x8 b = *reinterpret_cast<const bf16x8 *>(w + (16 * t + lr) * LD + 32 * s + 8 * lq);
             acc[t] = mfma(a, b, acc[t]);
         }
     }
@@ -185,14 +191,21 @@ __device__ __forceinline__ void store_hidden(__bf16 *act, const f32x4 *acc, cons
             act[(4 * lq + j) * kLdH + 16 * t + lr] = (__bf16)fmaxf(acc[t][j] + bias[16 * t + lr], 0.0f);
 }
 
-__global__ __launch_bounds__(256) void k_mlp_infer(InferArgs a) {
+// orders a wave's LDS writes before its later reads (the tile is private to the wave)
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(InferArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     __bf16 *w1 = reinterpret_cast<__bf16 *>(lds_raw);
     __bf16 *w2 = w1 + kGuideH * kLd1;
     __bf16 *w3 = w2 + kGuideH * kLdH;
     __bf16 *w4 = w3 + kGuideH * kLdH;
-    __bf16 *acts = w4 + kGuideNOutPad16 * kLdH;
-    float *b1 = reinterpret_cast<float *>(acts + 4 * 16 * kLdH);
+    __bf16 *tiles = w4 + kGuideNOutPad16 * kLdH;
+    float *b1 = reinterpret_cast<float *>(tiles + kInferWaves * kInferTile);
     float *b2 = b1 + kGuideH, *b3 = b2 + kGuideH, *b4 = b3 + kGuideH;
     stage_weights<kGuideK1, kLd1>(w1, a.w[0], kGuideH);
     stage_weights<kGuideH, kLdH>(w2, a.w[1], kGuideH);
@@ -207,73 +220,54 @@ __global__ __launch_bounds__(256) void k_mlp_infer(InferArgs a) {
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lr = lane & 15, lq = lane >> 4;
-    __bf16 *act = acts + wave * 16 * kLdH;
+    __bf16 *ft = tiles + wave * kInferTile;
     const int64_t rows = a.n_dev ? min((int64_t)*a.n_dev, a.n) : a.n;
-    const int64_t tiles = (rows + 15) >> 4;
-    // whole waves only ever touch their own activation tile, so no block barrier is
-    // needed inside the loop; the wave barrier orders the tile's LDS writes and reads
-    for (int64_t tile = (int64_t)blockIdx.x * 4 + wave; tile < tiles; tile += (int64_t)gridDim.x * 4) {
-        const int64_t r = tile * 16 + lr;
-        float x[kGuideIn];
+    const int64_t ntiles = (rows + 15) >> 4;
+    // whole waves only ever touch their own tile, so no block barrier is needed
+    // inside the loop; wave_sync orders the tile's LDS writes and reads
+    for (int64_t tile = (int64_t)blockIdx.x * kInferWaves + wave; tile < ntiles;
+         tile += (int64_t)gridDim.x * kInferWaves) {
+        // features x[rr][24 c + 2 f + {sin, cos}]: chain q = (row rr, input c), row-major
+        // so that the input loads coalesce; rows past the end get zero features
+        for (int q = lane; q < 16 * kGuideIn; q += 64) {
+            const int rr = q / kGuideIn, c = q - rr * kGuideIn;
+            const int64_t r = tile * 16 + rr;
+            bf16x2 *o = reinterpret_cast<bf16x2 *>(ft + rr * kLd1 + 2 * kGuideFreq * c);
+            if (r < rows) {
+                FreqState st = freq_base(a.in[r * kGuideIn + c]);
 #pragma unroll
-        for (int c = 0; c < kGuideIn; ++c) x[c] = r < rows ? a.in[r * kGuideIn + c] : 0.0f;
-        f32x4 acc[kGuideH / 16];
-#pragma unroll
-        for (int t = 0; t < kGuideH / 16; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int s = 0; s < kGuideK1 / 32; ++s) {
-            // features k = 32 s + 8 lq + j: one input c, frequencies f0 .. f0 + 3, sin / cos pairs
-            const int k0 = 32 * s + 8 * lq;
-            bf16x8 av;
-            if (k0 < kGuideFeat && r < rows) {
-                const int c = k0 / (2 * kGuideFreq), f0 = (k0 / 2) % kGuideFreq;
-                float xc = x[0];
-#pragma unroll
-                for (int q = 1; q < kGuideIn; ++q)
-                    if (q == c) xc = x[q];
-                FreqState st = freq_base(xc);
-                for (int f = 0; f < f0; ++f) freq_next(st);
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    av[2 * j] = (__bf16)st.s;
-                    av[2 * j + 1] = (__bf16)st.c;
+                for (int f = 0; f < kGuideFreq; ++f) {
+                    o[f] = bf16x2{(__bf16)st.s, (__bf16)st.c};
                     freq_next(st);
                 }
             } else {
 #pragma unroll
-                for (int j = 0; j < 8; ++j) av[j] = (__bf16)0.0f;
-            }
-#pragma unroll
-            for (int t = 0; t < kGuideH / 16; ++t) {
-                const bf16x8 b = *reinterpret_cast<const bf16x8 *>(w1 + (16 * t + lr) * kLd1 + 32 * s + 8 * lq);
-                acc[t] = mfma(av, b, acc[t]);
+                for (int f = 0; f < kGuideFreq; ++f) o[f] = bf16x2{(__bf16)0.0f, (__bf16)0.0f};
             }
         }
-        store_hidden(act, acc, b1, lr, lq);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        hidden_mfma<kGuideH / 16>(act, w2, lr, lq, acc);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        store_hidden(act, acc, b2, lr, lq);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        hidden_mfma<kGuideH / 16>(act, w3, lr, lq, acc);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        store_hidden(act, acc, b3, lr, lq);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (lane < 16) {
+            bf16x8 z;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) z[j] = (__bf16)0.0f;
+            *reinterpret_cast<bf16x8 *>(ft + lane * kLd1 + kGuideFeat) = z;   // features 216..223
+        }
+        wave_sync();
+        f32x4 acc[kGuideH / 16];
+        tile_mfma<kGuideH / 16, kGuideK1, kLd1>(ft, w1, lr, lq, acc);
+        wave_sync();
+        store_hidden(ft, acc, b1, lr, lq);
+        wave_sync();
+        tile_mfma<kGuideH / 16, kGuideH, kLdH>(ft, w2, lr, lq, acc);
+        wave_sync();
+        store_hidden(ft, acc, b2, lr, lq);
+        wave_sync();
+        tile_mfma<kGuideH / 16, kGuideH, kLdH>(ft, w3, lr, lq, acc);
+        wave_sync();
+        store_hidden(ft, acc, b3, lr, lq);
+        wave_sync();
         f32x4 out[kGuideNOutPad16 / 16];
-        hidden_mfma<kGuideNOutPad16 / 16>(act, w4, lr, lq, out);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        tile_mfma<kGuideNOutPad16 / 16, kGuideH, kLdH>(ft, w4, lr, lq, out);
+        wave_sync();
 #pragma unroll
         for (int t = 0; t < kGuideNOutPad16 / 16; ++t)
 #pragma unroll
@@ -617,9 +611,18 @@ hipError_t guide_infer(tt_guide *g, const float *in, float *raw, int64_t n, cons
     a.raw = raw;
     a.n = n;
     a.n_dev = n_dev;
+    // one block per CU (its LDS holds the weights and kInferWaves tiles)
+    static bool lds_set = false;
+    if (!lds_set) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_mlp_infer),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, kInferLds);
+        if (e != hipSuccess) return e;
+        lds_set = true;
+    }
     const int64_t tiles = (n + 15) / 16;
-    const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((tiles + 3) / 4, 2LL * std::max(1, g->ctx->num_cus)));
-    hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)grid), dim3(256), kInferLds, s, a);
+    const int64_t grid = std::max<int64_t>(
+        1, std::min<int64_t>((tiles + kInferWaves - 1) / kInferWaves, std::max(1, g->ctx->num_cus)));
+    hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)grid), dim3(64 * kInferWaves), kInferLds, s, a);
     return hipGetLastError();
 }
 
@@ -822,6 +825,21 @@ tt_status tt_guide_forward(tt_guide *g, const float *inputs, int64_t n, float *r
     hipStream_t s = g->ctx->stream;
     TT_HIPG(hipMemcpyAsync(g->batch.in, inputs, (size_t)n * kGuideIn * 4, hipMemcpyHostToDevice, s), "copy inputs");
     TT_HIPG(guide_infer(g, g->batch.in, g->batch.raw, n, nullptr, s), "guide inference");
+    TT_HIPG(hipMemcpy2DAsync(raw_out, kGuideNOut * 4, g->batch.raw, kGuideNOutPad16 * 4, kGuideNOut * 4, (size_t)n,
+                             hipMemcpyDeviceToHost, s), "copy out");
+    TT_HIPG(hipStreamSynchronize(s), "sync");
+    return TT_OK;
+}
+
+tt_status tt_debug_guide_forward_unfused(tt_guide *g, const float *inputs, int64_t n, float *raw_out) {
+    if (!g || !inputs || !raw_out || n <= 0 || n > (1LL << 24))
+        return fail(TT_ERR_INVALID, "tt_debug_guide_forward_unfused: bad arguments");
+    TT_HIPG(hipSetDevice(g->ctx->device), "hipSetDevice");
+    tt_status st = guide_alloc_batch(g, n);
+    if (st != TT_OK) return st;
+    hipStream_t s = g->ctx->stream;
+    TT_HIPG(hipMemcpyAsync(g->batch.in, inputs, (size_t)n * kGuideIn * 4, hipMemcpyHostToDevice, s), "copy inputs");
+    TT_HIPG(guide_forward(g, n, nullptr, s), "guide forward");
     TT_HIPG(hipMemcpy2DAsync(raw_out, kGuideNOut * 4, g->batch.raw, kGuideNOutPad16 * 4, kGuideNOut * 4, (size_t)n,
                              hipMemcpyDeviceToHost, s), "copy out");
     TT_HIPG(hipStreamSynchronize(s), "sync");

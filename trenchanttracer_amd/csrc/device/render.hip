@@ -110,7 +110,10 @@ __device__ __forceinline__ int wave_append(bool want, int32_t *counter) {
 // a wavefront bounce appends to the same two counters from millions of waves, and
 // updates of one address are serialised in L2. Every thread of the block must call
 // it (it synchronises the block). Slot order follows the lanes within a block;
-// blocks take their ranges in the order their updates reach L2.
+// blocks take their ranges in the order their updates reach L2. Back-to-back
+// calls are safe: cnt is read only between the two barriers, and base is written
+// only after the next call's first barrier, which every wave reaches after its
+// last read of base.
 constexpr int kAppendBlock = 256;
 __device__ __forceinline__ void block_append2(bool w0, int32_t *c0, bool w1, int32_t *c1, int *s0, int *s1) {
     constexpr int kWaves = kAppendBlock / 64;
@@ -123,6 +126,11 @@ __device__ __forceinline__ void block_append2(bool w0, int32_t *c0, bool w1, int
         cnt[1][wv] = __popcll(m1);
     }
     __syncthreads();
+    int p0 = 0, p1 = 0;
+    for (int k = 0; k < wv; ++k) {
+        p0 += cnt[0][k];
+        p1 += cnt[1][k];
+    }
     if (threadIdx.x < 2) {
         const int q = (int)threadIdx.x;
         int t = 0;
@@ -131,11 +139,8 @@ __device__ __forceinline__ void block_append2(bool w0, int32_t *c0, bool w1, int
         base[q] = (t > 0 && c) ? atomicAdd(c, t) : 0;
     }
     __syncthreads();
-    int p0 = base[0], p1 = base[1];
-    for (int k = 0; k < wv; ++k) {
-        p0 += cnt[0][k];
-        p1 += cnt[1][k];
-    }
+    p0 += base[0];
+    p1 += base[1];
     const unsigned long long below = (1ull << lane) - 1ull;
     *s0 = w0 ? p0 + __popcll(m0 & below) : -1;
     *s1 = w1 ? p1 + __popcll(m1 & below) : -1;
@@ -298,33 +303,36 @@ struct GeomArgs {
 // (src/core/RenderKernel.cu:38-61); the nearest primitive below tmax wins when
 // strictly nearer than the triangle hit (a miss carries t = tmax).
 __global__ __launch_bounds__(256) void k_geom_hits(GeomArgs a) {
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *a.n_dev) return;
-    const tt_ray r = a.rays[i];
-    const tt_hit h = a.hits[i];
-    const f3 o = F3(r.ox, r.oy, r.oz), d = F3(r.dx, r.dy, r.dz);
-    float best = h.t;
-    int32_t id = h.tri_id;
-    for (int32_t k = 0; k < a.num_geoms; ++k) {
-        const float t = geom_t(a.geoms[k], o, d, r.tmax);
-        if (t < r.tmax && t < best) {
-            best = t;
-            id = -2 - k;
+    const int32_t n = *a.n_dev;
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const tt_ray r = a.rays[i];
+        const tt_hit h = a.hits[i];
+        const f3 o = F3(r.ox, r.oy, r.oz), d = F3(r.dx, r.dy, r.dz);
+        float best = h.t;
+        int32_t id = h.tri_id;
+        for (int32_t k = 0; k < a.num_geoms; ++k) {
+            const float t = geom_t(a.geoms[k], o, d, r.tmax);
+            if (t < r.tmax && t < best) {
+                best = t;
+                id = -2 - k;
+            }
         }
+        if (id != h.tri_id) a.hits[i] = tt_hit{id, best, 0.0f, 0.0f};
     }
-    if (id != h.tri_id) a.hits[i] = tt_hit{id, best, 0.0f, 0.0f};
 }
 
 __global__ __launch_bounds__(256) void k_geom_occluded(GeomArgs a) {
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *a.n_dev || a.occ[i]) return;
-    const tt_ray r = a.rays[i];
-    const f3 o = F3(r.ox, r.oy, r.oz), d = F3(r.dx, r.dy, r.dz);
-    for (int32_t k = 0; k < a.num_geoms; ++k)
-        if (geom_t(a.geoms[k], o, d, r.tmax) < r.tmax) {
-            a.occ[i] = 1;
-            return;
-        }
+    const int32_t n = *a.n_dev;
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (a.occ[i]) continue;
+        const tt_ray r = a.rays[i];
+        const f3 o = F3(r.ox, r.oy, r.oz), d = F3(r.dx, r.dy, r.dz);
+        for (int32_t k = 0; k < a.num_geoms; ++k)
+            if (geom_t(a.geoms[k], o, d, r.tmax) < r.tmax) {
+                a.occ[i] = 1;
+                break;
+            }
+    }
 }
 
 // Surface at a hit: unnormalised normal (interpolated vertex normals of a
@@ -411,10 +419,9 @@ __device__ f3 env_reference(const SceneView &sv, f3 d, float pi, float two_pi) {
 }
 
 // ------------------------------------------------------------------ reference shade
-__global__ __launch_bounds__(kAppendBlock) void k_shade_reference(ShadeArgs a) {
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int32_t n = a.counts[a.cur];
-    if (i == 0) atomicAdd(&a.stats[0], (unsigned long long)n);
+// One slot of the hit queue (i >= n: no path; the thread still joins the block's
+// queue append).
+__device__ __forceinline__ void shade_reference_one(const ShadeArgs &a, const int32_t i, const int32_t n) {
     bool cont = false;
     int32_t p = 0, root = 0;
     tt_ray nr{};
@@ -569,6 +576,17 @@ __global__ __launch_bounds__(kAppendBlock) void k_shade_reference(ShadeArgs a) {
         a.path_out[slot] = p;
         a.root_out[slot] = root;
     }
+}
+
+// One thread per queue slot over the whole capacity; blocks wholly past the queue's
+// end leave at once (block-uniform, before the append's barriers). A loop over the
+// queue would save those blocks' launch but takes the shade kernels from 73 to 97
+// VGPRs (6 to 4 waves per SIMD), which costs more at the full early bounces.
+__global__ __launch_bounds__(kAppendBlock) void k_shade_reference(ShadeArgs a) {
+    const int32_t n = a.counts[a.cur];
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&a.stats[0], (unsigned long long)n);
+    if ((int64_t)blockIdx.x * kAppendBlock >= n) return;
+    shade_reference_one(a, (int32_t)(blockIdx.x * kAppendBlock + threadIdx.x), n);
 }
 
 // ------------------------------------------------------------------ path mode
@@ -835,30 +853,28 @@ struct GuideInArgs {
 };
 
 __global__ __launch_bounds__(256) void k_guide_inputs(GuideInArgs a) {
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.counts[a.cur]) return;
-    float *in = a.in + (int64_t)i * kGuideIn;
-    const tt_hit h = a.hits[i];
-    if (h.tri_id == -1) {
-        for (int k = 0; k < kGuideIn; ++k) in[k] = 0.0f;
-        return;
+    const int32_t n = a.counts[a.cur];
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float *in = a.in + (int64_t)i * kGuideIn;
+        const tt_hit h = a.hits[i];
+        if (h.tri_id == -1) {
+            for (int k = 0; k < kGuideIn; ++k) in[k] = 0.0f;
+            continue;
+        }
+        const tt_ray r = a.rays[i];
+        const f3 rd = F3(r.dx, r.dy, r.dz), wo = rd * -1.0f;
+        const f3 point = F3(r.ox, r.oy, r.oz) + rd * h.t;
+        f3 ngf;
+        bool front;
+        const f3 ns = oriented_normal(a.sv, h, point, wo, &ngf, &front);
+        guide_inputs(a.gd, point, ns, wo, in);
     }
-    const tt_ray r = a.rays[i];
-    const f3 rd = F3(r.dx, r.dy, r.dz), wo = rd * -1.0f;
-    const f3 point = F3(r.ox, r.oy, r.oz) + rd * h.t;
-    f3 ngf;
-    bool front;
-    const f3 ns = oriented_normal(a.sv, h, point, wo, &ngf, &front);
-    guide_inputs(a.gd, point, ns, wo, in);
 }
 
 // GUIDED: the guide's mixture, training records and roulette rule are compiled in
 // only for guided renders, so the plain kernel keeps its registers for occupancy.
 template <bool GUIDED>
-__global__ __launch_bounds__(kAppendBlock) void k_shade_path(ShadeArgs a) {
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int32_t n = a.counts[a.cur];
-    if (i == 0) atomicAdd(&a.stats[0], (unsigned long long)n);
+__device__ __forceinline__ void shade_path_one(const ShadeArgs &a, const int32_t i, const int32_t n) {
     bool cont = false, shadow = false;
     int32_t p = 0, root = 0;
     tt_ray nr{}, sr{};
@@ -1073,6 +1089,15 @@ __global__ __launch_bounds__(kAppendBlock) void k_shade_path(ShadeArgs a) {
     }
 }
 
+// as k_shade_reference: one thread per slot, empty blocks leave at once
+template <bool GUIDED>
+__global__ __launch_bounds__(kAppendBlock) void k_shade_path(ShadeArgs a) {
+    const int32_t n = a.counts[a.cur];
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&a.stats[0], (unsigned long long)n);
+    if ((int64_t)blockIdx.x * kAppendBlock >= n) return;
+    shade_path_one<GUIDED>(a, (int32_t)(blockIdx.x * kAppendBlock + threadIdx.x), n);
+}
+
 struct ApplyArgs {
     const int32_t *counts;
     const int32_t *occ;
@@ -1084,20 +1109,21 @@ struct ApplyArgs {
 };
 
 __global__ __launch_bounds__(256) void k_apply_shadow(ApplyArgs a) {
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const int32_t n = a.counts[2];
-    if (i == 0) atomicAdd(&a.stats[1], (unsigned long long)n);
-    if (i >= n || a.occ[i]) return;
-    const float4 c = a.contrib[i];
-    const int32_t path = __float_as_int(c.w);
-    const int tc = train_slot(a.gd, path);
-    if (tc >= 0) credit(a.gd.rec, tc, a.bounce, a.max_depth, F3(c.x, c.y, c.z));
-    // a path has at most one next-event ray per bounce: plain read-modify-write
-    float4 l = a.L[path];
-    l.y += c.x;
-    l.z += c.y;
-    l.w += c.z;
-    a.L[path] = l;
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&a.stats[1], (unsigned long long)n);
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (a.occ[i]) continue;
+        const float4 c = a.contrib[i];
+        const int32_t path = __float_as_int(c.w);
+        const int tc = train_slot(a.gd, path);
+        if (tc >= 0) credit(a.gd.rec, tc, a.bounce, a.max_depth, F3(c.x, c.y, c.z));
+        // a path has at most one next-event ray per bounce: plain read-modify-write
+        float4 l = a.L[path];
+        l.y += c.x;
+        l.z += c.y;
+        l.w += c.z;
+        a.L[path] = l;
+    }
 }
 
 // End of a path-mode pass: each pixel's radiance is the sum of its kk paths'
@@ -1361,8 +1387,12 @@ hipError_t launch_bounce(tt_scene *s, RenderState *rs, const tt_render_params *p
     const int variant = p->mode == TT_RENDER_REFERENCE ? 0 : (p->variant >= 0 ? p->variant : 1);
     const int lds = p->lds_stack > 0 ? p->lds_stack : 16;
     const int block = p->block > 0 ? p->block : 64;
+    // the small queue kernels stride over the device-side count with 16 blocks of 256
+    // threads per CU; the shade kernels take one thread per slot of the capacity
     static_assert(kAppendBlock == 256, "shade launches use the 256-thread grid");
-    const unsigned gblocks = (unsigned)((cap + 255) / 256);
+    const unsigned gblocks =
+        (unsigned)std::min<int64_t>(((int64_t)cap + 255) / 256, 16LL * std::max(1, s->ctx->num_cus));
+    const unsigned sblocks = (unsigned)(((int64_t)cap + kAppendBlock - 1) / kAppendBlock);
     TraceArgs ta{};
     ta.nodes = (const float4 *)s->nodes;
     ta.tris = (const float4 *)s->tris;
@@ -1424,11 +1454,11 @@ hipError_t launch_bounce(tt_scene *s, RenderState *rs, const tt_render_params *p
     sa.pi = p->pi;
     sa.two_pi = p->two_pi;
     if (p->mode == TT_RENDER_REFERENCE) {
-        hipLaunchKernelGGL(k_shade_reference, dim3(gblocks), dim3(kAppendBlock), 0, hs, sa);
+        hipLaunchKernelGGL(k_shade_reference, dim3(sblocks), dim3(kAppendBlock), 0, hs, sa);
         return hipGetLastError();
     }
-    if (gr) hipLaunchKernelGGL(k_shade_path<true>, dim3(gblocks), dim3(kAppendBlock), 0, hs, sa);
-    else hipLaunchKernelGGL(k_shade_path<false>, dim3(gblocks), dim3(kAppendBlock), 0, hs, sa);
+    if (gr) hipLaunchKernelGGL(k_shade_path<true>, dim3(sblocks), dim3(kAppendBlock), 0, hs, sa);
+    else hipLaunchKernelGGL(k_shade_path<false>, dim3(sblocks), dim3(kAppendBlock), 0, hs, sa);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     TraceArgs sh{};

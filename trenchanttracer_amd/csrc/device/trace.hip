@@ -57,20 +57,23 @@ struct Stack {
     int *lds;       // this lane's column: lds[k * stride]
     int stride;
     int sp;         // entries held (the sentinel lies below entry 0)
+    int off;        // sp * stride, kept by additions (no multiply per push / pop)
     int *spill;     // this lane's private overflow, kStackCap - S entries
 
     __device__ __forceinline__ void push(int v) {
         if (sp < S) {
-            lds[sp * stride] = v;
+            lds[off] = v;
         } else if (sp < kStackCap) {
             spill[sp - S] = v;
         }
         sp++;
+        off += stride;
     }
     __device__ __forceinline__ int pop() {
         if (sp <= 0) return kSentinel;
         sp--;
-        if (sp < S) return lds[sp * stride];
+        off -= stride;
+        if (sp < S) return lds[off];
         if (sp < kStackCap) return spill[sp - S];
         return kSentinel;
     }
@@ -89,13 +92,18 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
         const unsigned q = gridDim.x >> 3;
         if (vb < (q << 3)) vb = (vb & 7u) * q + (vb >> 3);
     }
-    const int64_t rid = (int64_t)vb * blockDim.x + tid;
     int64_t n = a.n;
     if (a.n_dev) n = min((int64_t)*a.n_dev, n);
     if (blockIdx.x == 0 && tid == 0) {
         if (a.reset0) *a.reset0 = 0;
         if (a.reset1) *a.reset1 = 0;
     }
+    // blocks stride over the rays: a queue launch (count on the device) runs a
+    // bounded grid instead of one block per queue slot, so a near-empty deep bounce
+    // costs a few microseconds; a launch over a known count has one block per block
+    // of rays and makes one pass
+    const int64_t step = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t rid = (int64_t)vb * blockDim.x + tid; rid - tid < n; rid += step) {
     const bool valid = rid < n;
 
     float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = make_float4(0.f, 0.f, 1.f, 0.f);
@@ -118,6 +126,7 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
     st.lds = lds_stack + tid;
     st.stride = blockDim.x;
     st.sp = 0;
+    st.off = 0;
     st.spill = spill;
 
     int leaf_addr = 0;
@@ -131,7 +140,7 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
             const float4 n0xy = np[0];
             const float4 n1xy = np[1];
             const float4 nz = np[2];
-            const int4 ch = reinterpret_cast<const int4 *>(np)[3];
+            const int2 ch = reinterpret_cast<const int2 *>(np + 3)[0];
 
             const float c0lox = n0xy.x * idirx - oodx;
             const float c0hix = n0xy.y * idirx - oodx;
@@ -153,6 +162,11 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
             const float ray_tmax = 1e20f;
             const bool t0 = (c0min <= c0max) && (c0min >= tmin) && (c0min <= ray_tmax);
             const bool t1 = (c1min <= c1max) && (c1min >= tmin) && (c1min <= ray_tmax);
+            // the child indices are used only when a child is entered; without a use on
+            // every path the compiler issues their load there, after the slab tests,
+            // which makes it a second dependent fetch per visited node. This empty
+            // statement uses them here, so all 64 bytes of the node are fetched at once.
+            asm volatile("" : : "v"(ch.x), "v"(ch.y));
 
             if (!t0 && !t1) {
                 node_addr = st.pop();
@@ -232,22 +246,28 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
         }
     }
 
-    if (!valid) return;
-    if (ANY) {
-        a.flags[rid] = hit_id != -1 ? 1 : 0;
-    } else {
-        if (hit_id == -1) {
-            hit_u = 0.0f;
-            hit_v = 0.0f;
+    if (valid) {
+        if (ANY) {
+            a.flags[rid] = hit_id != -1 ? 1 : 0;
+        } else {
+            if (hit_id == -1) {
+                hit_u = 0.0f;
+                hit_v = 0.0f;
+            }
+            float4 out = make_float4(bf(hit_id), hit_t, hit_u, hit_v);
+            *reinterpret_cast<float4 *>(a.hits + rid) = out;
         }
-        float4 out = make_float4(bf(hit_id), hit_t, hit_u, hit_v);
-        *reinterpret_cast<float4 *>(a.hits + rid) = out;
+    }
     }
 }
 
 template <int V, bool ANY, int TRI>
-hipError_t launch_s(const TraceArgs &a, int S, int block, hipStream_t s) {
-    const int64_t grid = (a.n + block - 1) / block;
+hipError_t launch_s(const TraceArgs &a, int S, int block, int num_cus, hipStream_t s) {
+    int64_t grid = (a.n + block - 1) / block;
+    // queue launches: 8 times as many blocks as the chip holds (32 waves per CU), so
+    // the dispatcher still balances uneven traversals while a near-empty queue costs
+    // a few microseconds instead of one wave per slot of the capacity
+    if (a.n_dev) grid = std::min<int64_t>(grid, 8LL * std::max(1, num_cus) * (2048 / block));
     const size_t lds = (size_t)S * block * sizeof(int);
     switch (S) {
         case 8: hipLaunchKernelGGL((k_trace<V, ANY, 8, TRI>), dim3((unsigned)grid), dim3(block), lds, s, a); break;
@@ -260,9 +280,11 @@ hipError_t launch_s(const TraceArgs &a, int S, int block, hipStream_t s) {
 }
 
 template <int TRI>
-hipError_t launch_v(const TraceArgs &a, int variant, bool any, int S, int block, hipStream_t s) {
-    if (variant == 0) return any ? launch_s<0, true, TRI>(a, S, block, s) : launch_s<0, false, TRI>(a, S, block, s);
-    if (variant == 1) return any ? launch_s<1, true, TRI>(a, S, block, s) : launch_s<1, false, TRI>(a, S, block, s);
+hipError_t launch_v(const TraceArgs &a, int variant, bool any, int S, int block, int num_cus, hipStream_t s) {
+    if (variant == 0)
+        return any ? launch_s<0, true, TRI>(a, S, block, num_cus, s) : launch_s<0, false, TRI>(a, S, block, num_cus, s);
+    if (variant == 1)
+        return any ? launch_s<1, true, TRI>(a, S, block, num_cus, s) : launch_s<1, false, TRI>(a, S, block, num_cus, s);
     return hipErrorInvalidValue;
 }
 
@@ -270,14 +292,13 @@ hipError_t launch_v(const TraceArgs &a, int variant, bool any, int S, int block,
 
 hipError_t launch_trace(const TraceArgs &a, int variant, bool any, int lds_stack, int block, int xcd_order,
                         int num_cus, hipStream_t s, int tri_mode) {
-    (void)num_cus;
     if (a.n <= 0) return hipSuccess;
     if (block < 64 || block > 256 || block % 64 != 0) return hipErrorInvalidValue;
     if ((a.n + block - 1) / block > 0x7FFFFFFF) return hipErrorInvalidValue;
     TraceArgs b = a;
     b.xcd_order = xcd_order ? 1 : 0;
-    if (tri_mode == kTriReference) return launch_v<kTriReference>(b, variant, any, lds_stack, block, s);
-    if (tri_mode == kTriTwoSided) return launch_v<kTriTwoSided>(b, variant, any, lds_stack, block, s);
+    if (tri_mode == kTriReference) return launch_v<kTriReference>(b, variant, any, lds_stack, block, num_cus, s);
+    if (tri_mode == kTriTwoSided) return launch_v<kTriTwoSided>(b, variant, any, lds_stack, block, num_cus, s);
     return hipErrorInvalidValue;
 }
 
