@@ -159,9 +159,9 @@ def main():
         # traffic per launch of the default configuration (speculative, LDS 16, block
         # 64), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of
         # trenchanttracer_amd.profile_trace scaled by a 256 MiB copy's counts (reads x
-        # 1.99989, writes x 1.0): 11,283,264 KiB fetched, 129,600 KiB written (the hits)
+        # 1.99987, writes x 1.0): 11,385,820 KiB fetched, 129,600 KiB written (the hits)
         default_cfg = (args.variant, args.lds_stack, args.block, args.ntris) == (1, 16, 64, 1_000_000)
-        traffic = round((11_283_264.4 * 1.99989 + 129_600.0) * 1024 / 1e9, 2) if default_cfg else None
+        traffic = round((11_385_819.7 * 1.99987 + 129_600.0) * 1024 / 1e9, 2) if default_cfg else None
         result["roofline"] = {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
             "frac": round(achieved / 8000.0, 4), "traffic": traffic, "traffic_unit": "GB per launch",
