@@ -261,14 +261,19 @@ tt_status tt_env_export_cdf(const tt_scene *s, float *marg, float *cond, double 
  * bit-identical to the host reference traversal), 1 = speculative while-while
  * (Aila-Laine postponed leaf + wave ballot). lds_stack: LDS stack entries per lane
  * (8, 16, 24 or 32); block: threads per block (64..256, multiple of 64).
- * xcd_order: 0 = block b traces rays [b * block, (b + 1) * block); 1 = blocks are
- * renumbered so that each of the 8 XCDs (which take blocks round-robin) traces one
- * contiguous eighth of the rays, keeping neighbouring rays in one L2. */
+ * refill: 0 (default) = one wave per 64-ray group; 1..64 = persistent waves: the
+ * launch holds as many waves as the GPU keeps resident, wave w of W traces the
+ * 64-ray groups w, w + W, ..., and a lane whose ray is done takes the wave's next ray
+ * once at least `refill` of its 64 lanes are idle (keeps incoherent waves full; on
+ * the 1M-soup traversal workload, half coherent rays, one wave per group is faster).
+ * xcd_order: 0 = waves in launch order; 1 = blocks are renumbered so that each of the
+ * 8 XCDs (which take blocks round-robin) holds one contiguous range of waves. */
 typedef struct tt_trace_cfg {
     int32_t variant;
     int32_t lds_stack;
     int32_t block;
     int32_t xcd_order;
+    int32_t refill;
 } tt_trace_cfg;
 
 /* Closest hit for n rays. rays / hits are DEVICE pointers; stream is a hipStream_t
@@ -328,6 +333,8 @@ typedef struct tt_render_params {
     int32_t lds_stack;        /* traversal LDS stack entries per lane (0 = 16) */
     int32_t block;            /* traversal block size (0 = 64) */
     int32_t variant;          /* path-mode traversal loop: -1 speculative (default), 0 exact, 1 speculative */
+    int32_t refill_camera;    /* traversal refill (see tt_trace_cfg) for the camera rays (0 = one wave per group) */
+    int32_t refill;           /* ... for every later extension and shadow launch (0 = persistent, 16) */
 } tt_render_params;
 
 typedef struct tt_render_stats {

@@ -181,8 +181,8 @@ def rays_incoherent(mesh, n, tmin=1e-4, tmax=1e20, seed=2):
     return out
 
 
-def trace_cfg(variant=0, lds_stack=16, block=128, xcd_order=0):
-    return tt_trace_cfg(int(variant), int(lds_stack), int(block), int(xcd_order))
+def trace_cfg(variant=0, lds_stack=16, block=128, xcd_order=0, refill=0):
+    return tt_trace_cfg(int(variant), int(lds_stack), int(block), int(xcd_order), int(refill))
 
 
 def material(type, albedo=(0.8, 0.8, 0.8), emission=(0, 0, 0), roughness=0.0, ior=1.5, specular=(1.0, 1.0, 1.0)):
@@ -247,8 +247,9 @@ def camera_from_orbit(center, yaw, pitch, radius=4.0, fovx=45.0, width=256, heig
 
 def render_params(spp=16, max_depth=8, rr_depth=4, seed=1, mode=RENDER_REFERENCE, guided=0, sample_offset=0,
                   fp64=False, pi=None, two_pi=None, tmin=0.01, use_graph=False, paths_in_flight=0, lds_stack=0,
-                  block=0, variant=-1):
+                  block=0, variant=-1, refill_camera=0, refill=0):
     p = N.tt_render_params()
+    p.refill_camera, p.refill = int(refill_camera), int(refill)
     p.use_graph, p.paths_in_flight, p.lds_stack, p.block, p.variant = (int(bool(use_graph)), int(paths_in_flight),
                                                                        int(lds_stack), int(block), int(variant))
     p.spp, p.max_depth, p.rr_depth, p.seed = int(spp), int(max_depth), int(rr_depth), int(seed)

@@ -52,7 +52,7 @@ class tt_geometry(C.Structure):
 
 class tt_trace_cfg(C.Structure):
     _fields_ = [("variant", C.c_int32), ("lds_stack", C.c_int32), ("block", C.c_int32),
-                ("xcd_order", C.c_int32)]
+                ("xcd_order", C.c_int32), ("refill", C.c_int32)]
 
 
 class tt_camera(C.Structure):
@@ -67,7 +67,7 @@ class tt_render_params(C.Structure):
                 ("sample_offset", C.c_int32), ("accumulate_fp64", C.c_int32),
                 ("pi", C.c_float), ("two_pi", C.c_float), ("tmin", C.c_float), ("use_graph", C.c_int32),
                 ("paths_in_flight", C.c_int32), ("lds_stack", C.c_int32), ("block", C.c_int32),
-                ("variant", C.c_int32)]
+                ("variant", C.c_int32), ("refill_camera", C.c_int32), ("refill", C.c_int32)]
 
 
 class tt_render_stats(C.Structure):

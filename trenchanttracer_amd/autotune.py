@@ -1,8 +1,9 @@
 """Autotuner for the traversal and the wavefront renderer.
 
 Times short runs of candidate launch configurations on the actual workload and
-keeps the fastest: traversal loop variant x LDS stack entries x block size for
-`tune_trace`; paths in flight x LDS stack x block x hipGraph for `tune_render`.
+keeps the fastest: traversal loop variant x LDS stack entries x block size x
+launch form (one wave per 64-ray group, or persistent waves with lane refill) for
+`tune_trace`; paths in flight x hipGraph replay for `tune_render`.
 Results can be cached in a JSON file keyed by a workload signature."""
 import itertools
 import json
@@ -26,21 +27,23 @@ def _save(path, data):
         os.replace(tmp, path)
 
 
-def tune_trace(ctx, scene, d_rays, n, d_hits, variants=(0, 1), lds=(8, 16, 24), blocks=(64, 128), reps=3,
-               exact_only=True, cache=None, key=None):
-    """Fastest trace configuration for this scene and ray buffer. With exact_only
-    the speculative variant is not considered (hit ids stay bit-exact)."""
+def tune_trace(ctx, scene, d_rays, n, d_hits, variants=(0, 1), lds=(8, 16, 24), blocks=(64, 128), refills=(0, 16),
+               reps=3, exact_only=True, cache=None, key=None):
+    """Fastest trace configuration for this scene and ray buffer: loop variant, LDS
+    stack depth, block size and launch form (refill 0: one wave per 64-ray group;
+    > 0: persistent waves that take new rays once that many lanes are idle). With
+    exact_only the speculative variant is not considered."""
     from . import Timer
     db = _load(cache)
     if key and key in db:
         c = db[key]
-        return trace_cfg(c["variant"], c["lds_stack"], c["block"]), c["ms"]
+        return trace_cfg(c["variant"], c["lds_stack"], c["block"], refill=c.get("refill", 0)), c["ms"]
     timer = Timer(ctx, 2)
     best = None
-    for v, s, b in itertools.product(variants, lds, blocks):
+    for v, s, b, r in itertools.product(variants, lds, blocks, refills):
         if exact_only and v != 0:
             continue
-        cfg = trace_cfg(v, s, b)
+        cfg = trace_cfg(v, s, b, refill=r)
         scene.trace_closest(d_rays, n, d_hits, cfg)
         ms = []
         for _ in range(reps):
@@ -50,12 +53,12 @@ def tune_trace(ctx, scene, d_rays, n, d_hits, variants=(0, 1), lds=(8, 16, 24), 
             ms.append(timer.elapsed(0, 1))
         t = sorted(ms)[len(ms) // 2]
         if best is None or t < best[0]:
-            best = (t, v, s, b)
+            best = (t, v, s, b, r)
     timer.close()
     if key:
-        db[key] = {"variant": best[1], "lds_stack": best[2], "block": best[3], "ms": best[0]}
+        db[key] = {"variant": best[1], "lds_stack": best[2], "block": best[3], "refill": best[4], "ms": best[0]}
         _save(cache, db)
-    return trace_cfg(best[1], best[2], best[3]), best[0]
+    return trace_cfg(best[1], best[2], best[3], refill=best[4]), best[0]
 
 
 def tune_render(render_fn, base, paths=(0, 16 << 20), lds=(16,), blocks=(64,), graphs=(0, 1),

@@ -38,7 +38,9 @@ tt_status resolve_cfg(const tt_trace_cfg *cfg, tt_trace_cfg &out) {
     out.lds_stack = 16;
     out.block = 128;
     out.xcd_order = 0;
+    out.refill = 0;
     if (cfg) out = *cfg;
+    if (out.refill < 0 || out.refill > 64) return fail(TT_ERR_INVALID, "trace cfg: refill must be 0..64");
     if (out.variant < 0 || out.variant > 1) return fail(TT_ERR_INVALID, "trace cfg: variant must be 0 or 1");
     if (out.lds_stack != 8 && out.lds_stack != 16 && out.lds_stack != 24 && out.lds_stack != 32)
         return fail(TT_ERR_INVALID, "trace cfg: lds_stack must be 8, 16, 24 or 32");
@@ -429,7 +431,7 @@ static tt_status trace_impl(tt_scene *s, const tt_ray *rays, int64_t n, tt_hit *
     a.num_nodes = (int32_t)s->num_nodes;
     hipStream_t hs = stream ? (hipStream_t)stream : s->ctx->stream;
     TT_HIP(hipSetDevice(s->ctx->device), "hipSetDevice");
-    TT_HIP(launch_trace(a, c.variant, any, c.lds_stack, c.block, c.xcd_order, s->ctx->num_cus, hs), "trace launch");
+    TT_HIP(launch_trace(a, c.variant, any, c.lds_stack, c.block, c.xcd_order, c.refill, s->ctx->num_cus, hs), "trace launch");
     return TT_OK;
 }
 

@@ -23,7 +23,8 @@ struct TraceArgs {
     const int32_t *n_dev;  // if set: ray count read on the device (capped at n)
     int32_t *reset0;       // if set: zeroed by the first lane of block 0
     int32_t *reset1;
-    int32_t xcd_order;     // 1: each XCD (blocks b = x mod 8) traces one contiguous eighth
+    int32_t xcd_order;     // 1: each XCD (blocks b = x mod 8) holds one contiguous range of waves
+    int32_t refill;        // a wave takes new rays once this many of its lanes are idle (1..64)
 };
 
 // Triangle acceptance: the reference's test (front faces only, det > 1e-5) or
@@ -32,6 +33,6 @@ enum TriMode { kTriReference = 0, kTriTwoSided = 1 };
 
 // Launchers (trace.hip).
 hipError_t launch_trace(const TraceArgs &a, int variant, bool any, int lds_stack, int block, int xcd_order,
-                        int num_cus, hipStream_t s, int tri_mode = kTriReference);
+                        int refill, int num_cus, hipStream_t s, int tri_mode = kTriReference);
 
 }  // namespace tt

@@ -817,6 +817,18 @@ tt_status tt_guide_set_params(tt_guide *g, const float *in) {
     return TT_OK;
 }
 
+// The n x 40 outputs from the padded n x 48 rows: one contiguous copy, then the
+// rows are packed on the host (a strided 2-D copy into pageable memory moves one
+// row at a time and took seconds for millions of rows).
+static tt_status copy_raw_out(tt_guide *g, int64_t n, float *raw_out, hipStream_t s) {
+    std::vector<float> pad((size_t)n * kGuideNOutPad16);
+    TT_HIPG(hipMemcpyAsync(pad.data(), g->batch.raw, pad.size() * 4, hipMemcpyDeviceToHost, s), "copy out");
+    TT_HIPG(hipStreamSynchronize(s), "sync");
+    for (int64_t r = 0; r < n; ++r)
+        std::memcpy(raw_out + r * kGuideNOut, pad.data() + r * kGuideNOutPad16, kGuideNOut * 4);
+    return TT_OK;
+}
+
 tt_status tt_guide_forward(tt_guide *g, const float *inputs, int64_t n, float *raw_out) {
     if (!g || !inputs || !raw_out || n <= 0 || n > (1LL << 24)) return fail(TT_ERR_INVALID, "tt_guide_forward: bad arguments");
     TT_HIPG(hipSetDevice(g->ctx->device), "hipSetDevice");
@@ -825,10 +837,7 @@ tt_status tt_guide_forward(tt_guide *g, const float *inputs, int64_t n, float *r
     hipStream_t s = g->ctx->stream;
     TT_HIPG(hipMemcpyAsync(g->batch.in, inputs, (size_t)n * kGuideIn * 4, hipMemcpyHostToDevice, s), "copy inputs");
     TT_HIPG(guide_infer(g, g->batch.in, g->batch.raw, n, nullptr, s), "guide inference");
-    TT_HIPG(hipMemcpy2DAsync(raw_out, kGuideNOut * 4, g->batch.raw, kGuideNOutPad16 * 4, kGuideNOut * 4, (size_t)n,
-                             hipMemcpyDeviceToHost, s), "copy out");
-    TT_HIPG(hipStreamSynchronize(s), "sync");
-    return TT_OK;
+    return copy_raw_out(g, n, raw_out, s);
 }
 
 tt_status tt_debug_guide_forward_unfused(tt_guide *g, const float *inputs, int64_t n, float *raw_out) {
@@ -840,10 +849,7 @@ tt_status tt_debug_guide_forward_unfused(tt_guide *g, const float *inputs, int64
     hipStream_t s = g->ctx->stream;
     TT_HIPG(hipMemcpyAsync(g->batch.in, inputs, (size_t)n * kGuideIn * 4, hipMemcpyHostToDevice, s), "copy inputs");
     TT_HIPG(guide_forward(g, n, nullptr, s), "guide forward");
-    TT_HIPG(hipMemcpy2DAsync(raw_out, kGuideNOut * 4, g->batch.raw, kGuideNOutPad16 * 4, kGuideNOut * 4, (size_t)n,
-                             hipMemcpyDeviceToHost, s), "copy out");
-    TT_HIPG(hipStreamSynchronize(s), "sync");
-    return TT_OK;
+    return copy_raw_out(g, n, raw_out, s);
 }
 
 static tt_status stage_rows(tt_guide *g, const float *inputs, const float *dirs, const float *wts, int64_t n) {

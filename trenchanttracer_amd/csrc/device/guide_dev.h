@@ -71,20 +71,61 @@ __device__ __forceinline__ float vmf_log(float kappa, float cosang) {
     return logf(kappa) - 1.83787706641f - log1pf(-expf(-2.0f * kappa)) + kappa * (cosang - 1.0f);
 }
 
-// mixture density at unit direction (x, y, z)
-__device__ __forceinline__ float mixture_pdf(const Lobes &L, float x, float y, float z) {
+// Render-side form of the mixture: per lobe the unit mean, kappa, the weight (for
+// picking a lobe) and lc = log(weight) + log of the vMF normaliser, so that the
+// density at a unit direction is sum_k exp(lc_k + kappa_k (mu_k . w - 1)) — one exp
+// per lobe per evaluation instead of a log, a log1p and two exps. Decoded once per
+// vertex; the density is evaluated for the next-event direction and the sampled one.
+struct Mixture {
+    float mu[kGuideLobes][3];
+    float kappa[kGuideLobes];
+    float w[kGuideLobes];
+    float lc[kGuideLobes];
+};
+
+__device__ __forceinline__ void decode_mixture(const float *raw, Mixture &M) {
+    float mx = -1e30f;
+#pragma unroll
+    for (int k = 0; k < kGuideLobes; ++k) mx = fmaxf(mx, raw[kGuideLobes * 4 + k]);
+    float sum = 0.0f;
+#pragma unroll
+    for (int k = 0; k < kGuideLobes; ++k) {
+        const float m0 = raw[3 * k], m1 = raw[3 * k + 1], m2 = raw[3 * k + 2];
+        const float inv = 1.0f / (sqrtf(m0 * m0 + m1 * m1 + m2 * m2) + 1e-6f);
+        M.mu[k][0] = m0 * inv;
+        M.mu[k][1] = m1 * inv;
+        M.mu[k][2] = m2 * inv;
+        const float lk = fminf(fmaxf(raw[kGuideLobes * 3 + k], kGuideLogKappaMin), kGuideLogKappaMax);
+        M.kappa[k] = expf(lk);
+        // log of kappa / (2 pi (1 - e^-2k)), with log kappa taken from the output itself
+        M.lc[k] = lk - 1.83787706641f - log1pf(-expf(-2.0f * M.kappa[k]));
+        const float d = raw[kGuideLobes * 4 + k] - mx;
+        M.w[k] = expf(d);
+        M.lc[k] += d;
+        sum += M.w[k];
+    }
+    const float lsum = logf(sum), isum = 1.0f / sum;
+#pragma unroll
+    for (int k = 0; k < kGuideLobes; ++k) {
+        M.w[k] *= isum;
+        M.lc[k] -= lsum;
+    }
+}
+
+__device__ __forceinline__ float mixture_density(const Mixture &M, float x, float y, float z) {
     float q = 0.0f;
 #pragma unroll
     for (int k = 0; k < kGuideLobes; ++k) {
-        const float c = L.mu[k][0] * x + L.mu[k][1] * y + L.mu[k][2] * z;
-        q += L.w[k] * expf(vmf_log(L.kappa[k], c));
+        const float c = M.mu[k][0] * x + M.mu[k][1] * y + M.mu[k][2] * z;
+        q += expf(M.lc[k] + M.kappa[k] * (c - 1.0f));
     }
     return q;
 }
 
 // Samples the mixture: lobe by cumulative weight (u0), then the vMF lobe by
 // Jakob's stable inversion w = 1 + log(u1 + (1 - u1) e^{-2k}) / k, phi = 2 pi u2.
-__device__ __forceinline__ void mixture_sample(const Lobes &L, float u0, float u1, float u2, float *out) {
+template <class Mix>
+__device__ __forceinline__ void mixture_sample(const Mix &L, float u0, float u1, float u2, float *out) {
     // the lobe is picked with constant indices only (selects, no break), so the
     // mixture stays in registers
     float kap = L.kappa[kGuideLobes - 1], mx = L.mu[kGuideLobes - 1][0], my = L.mu[kGuideLobes - 1][1],

@@ -937,8 +937,8 @@ __device__ __forceinline__ void shade_path_one(const ShadeArgs &a, const int32_t
                 // guiding: mixture of this slot (non-delta surfaces only)
                 const bool guided = GUIDED && a.gd.raw != nullptr &&
                                     (m.type == TT_MAT_DIFFUSE || (m.type == TT_MAT_CONDUCTOR && m.roughness >= a.gd.min_rough));
-                Lobes lobes;
-                if (guided) decode_lobes(a.gd.raw + (int64_t)i * kGuideNOutPad16, lobes);
+                Mixture lobes;
+                if (guided) decode_mixture(a.gd.raw + (int64_t)i * kGuideNOutPad16, lobes);
                 const float alpha = guided ? a.gd.alpha : 0.0f;
                 // next-event estimation
                 if (!smooth && (p_env > 0.0f || p_emit > 0.0f) && wol.z > 0.0f) {
@@ -970,7 +970,7 @@ __device__ __forceinline__ void shade_path_one(const ShadeArgs &a, const int32_t
                         f3 f_cos;
                         float pdf_b;
                         bsdf_eval(m, wol, to_local(fr, wi), &f_cos, &pdf_b);
-                        if (guided) pdf_b = alpha * mixture_pdf(lobes, wi.x, wi.y, wi.z) + (1.0f - alpha) * pdf_b;
+                        if (guided) pdf_b = alpha * mixture_density(lobes, wi.x, wi.y, wi.z) + (1.0f - alpha) * pdf_b;
                         if (maxc(f_cos) > 0.0f) {
                             float w = power_heuristic(pdf_l, pdf_b);
                             scontrib = thr * f_cos * le * (w / pdf_l);
@@ -1012,7 +1012,7 @@ __device__ __forceinline__ void shade_path_one(const ShadeArgs &a, const int32_t
                         f_cos = sampled ? weight * pdf_b : F3(0, 0, 0);
                     }
                     const f3 wiw = to_world(fr, wil);
-                    const float pdf_c = alpha * mixture_pdf(lobes, wiw.x, wiw.y, wiw.z) + (1.0f - alpha) * pdf_b;
+                    const float pdf_c = alpha * mixture_density(lobes, wiw.x, wiw.y, wiw.z) + (1.0f - alpha) * pdf_b;
                     sampled = pdf_c > 0.0f && maxc(f_cos) > 0.0f && wol.z > 0.0f;
                     weight = sampled ? f_cos * (1.0f / pdf_c) : F3(0, 0, 0);
                     w_rr = sampled && pdf_b > 0.0f ? maxc(f_cos) / pdf_b : 0.0f;
@@ -1265,8 +1265,8 @@ __global__ void k_debug_env(SceneView sv, int64_t n, uint32_t seed, float *out) 
 __global__ void k_debug_guide(const float *raw, int64_t n, uint32_t seed, float *out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    Lobes lobes;
-    decode_lobes(raw, lobes);
+    Mixture lobes;
+    decode_mixture(raw, lobes);
     float u[4], d[3];
     draws((uint32_t)i, (uint32_t)(i >> 32), 0x0F0Bu, 3u, seed, u);
     mixture_sample(lobes, u[1], u[2], u[3], d);
@@ -1274,7 +1274,7 @@ __global__ void k_debug_guide(const float *raw, int64_t n, uint32_t seed, float 
     o[0] = d[0];
     o[1] = d[1];
     o[2] = d[2];
-    o[3] = mixture_pdf(lobes, d[0], d[1], d[2]);
+    o[3] = mixture_density(lobes, d[0], d[1], d[2]);
 }
 
 __global__ void k_debug_math(int op, const float *x, const float *y, float *out, int64_t n) {
@@ -1404,7 +1404,12 @@ hipError_t launch_bounce(tt_scene *s, RenderState *rs, const tt_render_params *p
     ta.reset0 = rs->counts + nxt;
     ta.reset1 = rs->counts + 2;
     ta.ray_root = rs->root[cur];
-    hipError_t e = launch_trace(ta, variant, false, lds, block, 0, s->ctx->num_cus, hs, tri_mode);
+    // camera rays (bounce 0) are coherent and their queue is full: one wave per
+    // 64-ray group; later extension and shadow queues: persistent waves that refill
+    // once 16 lanes are idle
+    const int refill_later = p->refill > 0 ? p->refill : 16;
+    const int refill_ext = b == 0 ? p->refill_camera : refill_later;
+    hipError_t e = launch_trace(ta, variant, false, lds, block, 0, refill_ext, s->ctx->num_cus, hs, tri_mode);
     if (e != hipSuccess) return e;
     if (sv.num_geoms > 0) {
         GeomArgs ga{rs->rays[cur], rs->hits, nullptr, rs->counts + cur, sv.geoms, sv.num_geoms};
@@ -1470,7 +1475,7 @@ hipError_t launch_bounce(tt_scene *s, RenderState *rs, const tt_render_params *p
     sh.num_nodes = (int32_t)s->num_nodes;
     sh.n_dev = rs->counts + 2;
     sh.ray_root = rs->sroot;
-    e = launch_trace(sh, variant, true, lds, block, 0, s->ctx->num_cus, hs, kTriTwoSided);
+    e = launch_trace(sh, variant, true, lds, block, 0, refill_later, s->ctx->num_cus, hs, kTriTwoSided);
     if (e != hipSuccess) return e;
     if (sv.num_geoms > 0) {
         GeomArgs go{rs->srays, nullptr, rs->socc, rs->counts + 2, sv.geoms, sv.num_geoms};
@@ -1555,6 +1560,8 @@ tt_status validate_params(const tt_render_params *p) {
         return fail(TT_ERR_INVALID, "render: block must be 0, 64, 128, 192 or 256");
     if (p->variant < -1 || p->variant > 1) return fail(TT_ERR_INVALID, "render: variant must be -1, 0 or 1");
     if (p->paths_in_flight < 0) return fail(TT_ERR_INVALID, "render: paths_in_flight must be >= 0");
+    if (p->refill < 0 || p->refill > 64 || p->refill_camera < 0 || p->refill_camera > 64)
+        return fail(TT_ERR_INVALID, "render: refill thresholds must be 0..64");
     return TT_OK;
 }
 

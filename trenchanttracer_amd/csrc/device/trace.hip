@@ -83,10 +83,11 @@ template <int VARIANT, bool ANY, int S, int TRI>
 __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
     extern __shared__ int lds_stack[];
     const int tid = threadIdx.x;
+    const int lane = tid & 63;
     // XCD-contiguous order: the hardware hands blocks to the 8 XCDs round-robin, so
     // block b runs on XCD b % 8; renumbering b -> (b % 8) * q + b / 8 (for the first
     // 8q blocks, q = blocks / 8; the tail keeps its number) is a bijection that gives
-    // each XCD's L2 one contiguous range of rays
+    // each XCD's L2 one contiguous range of waves
     unsigned vb = blockIdx.x;
     if (a.xcd_order) {
         const unsigned q = gridDim.x >> 3;
@@ -98,28 +99,22 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
         if (a.reset0) *a.reset0 = 0;
         if (a.reset1) *a.reset1 = 0;
     }
-    // blocks stride over the rays: a queue launch (count on the device) runs a
-    // bounded grid instead of one block per queue slot, so a near-empty deep bounce
-    // costs a few microseconds; a launch over a known count has one block per block
-    // of rays and makes one pass
-    const int64_t step = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t rid = (int64_t)vb * blockDim.x + tid; rid - tid < n; rid += step) {
-    const bool valid = rid < n;
-
-    float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = make_float4(0.f, 0.f, 1.f, 0.f);
-    if (valid) {
-        const float4 *rp = reinterpret_cast<const float4 *>(a.rays + rid);
-        r0 = rp[0];
-        r1 = rp[1];
-    }
-    const float ox = r0.x, oy = r0.y, oz = r0.z, tmin = r0.w;
-    const float dx = r1.x, dy = r1.y, dz = r1.z, tmax = r1.w;
-
-    const float ooeps = 0x1p-80f;
-    const float idirx = 1.0f / (fabsf(dx) > ooeps ? dx : copysignf(ooeps, dx));
-    const float idiry = 1.0f / (fabsf(dy) > ooeps ? dy : copysignf(ooeps, dy));
-    const float idirz = 1.0f / (fabsf(dz) > ooeps ? dz : copysignf(ooeps, dz));
-    const float oodx = ox * idirx, oody = oy * idiry, oodz = oz * idirz;
+    // Persistent waves (Aila & Laine's persistent threads with ray replacement):
+    // the grid holds about as many waves as the chip keeps resident, and wave w of W
+    // owns the 64-ray groups w, w + W, w + 2W, ... in that order, so that all waves
+    // sweep the ray range together (the rays in flight at any moment lie in a window
+    // of about 2W groups, as with one block per group, which keeps coherent rays'
+    // nodes in cache). A lane whose ray is done writes its result and takes the next
+    // ray of its wave's sequence (ballot and prefix count, no atomics): the lanes
+    // stay busy until the sequence runs out, instead of idling behind the wave's
+    // longest ray. Each ray's traversal is the same as with one ray per lane.
+    const int64_t groups = (n + 63) >> 6;
+    const int64_t waves_all = (int64_t)gridDim.x * (blockDim.x >> 6);
+    const int64_t waves = max((int64_t)1, min(waves_all, groups));
+    const int64_t w = (int64_t)vb * (blockDim.x >> 6) + (tid >> 6);
+    if (w >= waves) return;   // whole waves only: nothing below synchronises the block
+    const int64_t total = ((groups - w + waves - 1) / waves) * 64;   // the wave's sequence positions
+    const unsigned long long below = (1ull << lane) - 1ull;
 
     int spill[kStackCap - S];
     Stack<S> st;
@@ -129,12 +124,71 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
     st.off = 0;
     st.spill = spill;
 
+    int64_t rid = -1;          // the lane's ray (-1: none)
+    int64_t cursor = 0;        // next sequence position (uniform)
+    float ox = 0.0f, oy = 0.0f, oz = 0.0f, tmin = 0.0f, dx = 0.0f, dy = 0.0f, dz = 1.0f;
+    float idirx = 0.0f, idiry = 0.0f, idirz = 0.0f, oodx = 0.0f, oody = 0.0f, oodz = 0.0f;
     int leaf_addr = 0;
-    int node_addr = (valid && a.num_nodes > 0) ? (a.ray_root ? a.ray_root[rid] : 0) : kSentinel;
-    float hit_t = tmax, hit_u = 0.0f, hit_v = 0.0f;
+    int node_addr = kSentinel;
+    float hit_t = 0.0f, hit_u = 0.0f, hit_v = 0.0f;
     int hit_id = -1;
 
-    while (node_addr != kSentinel) {
+    for (;;) {
+        const bool done = node_addr == kSentinel;
+        if (done && rid >= 0) {
+            if (ANY) {
+                a.flags[rid] = hit_id != -1 ? 1 : 0;
+            } else {
+                if (hit_id == -1) {
+                    hit_u = 0.0f;
+                    hit_v = 0.0f;
+                }
+                *reinterpret_cast<float4 *>(a.hits + rid) = make_float4(bf(hit_id), hit_t, hit_u, hit_v);
+            }
+            rid = -1;
+        }
+        const unsigned long long m = __ballot(done);
+        // new rays once enough lanes are idle (all idle always qualifies, so the
+        // wave cannot wait on itself)
+        const bool take = __popcll(m) >= a.refill;
+        if (take && done) {
+            const int64_t q = cursor + __popcll(m & below);
+            if (q < total) {
+                const int64_t r = ((q >> 6) * waves + w) * 64 + (q & 63);
+                if (r < n) rid = r;
+            }
+            if (rid >= 0) {
+                const float4 *rp = reinterpret_cast<const float4 *>(a.rays + rid);
+                const float4 r0 = rp[0], r1 = rp[1];
+                ox = r0.x;
+                oy = r0.y;
+                oz = r0.z;
+                tmin = r0.w;
+                dx = r1.x;
+                dy = r1.y;
+                dz = r1.z;
+                const float ooeps = 0x1p-80f;
+                idirx = 1.0f / (fabsf(dx) > ooeps ? dx : copysignf(ooeps, dx));
+                idiry = 1.0f / (fabsf(dy) > ooeps ? dy : copysignf(ooeps, dy));
+                idirz = 1.0f / (fabsf(dz) > ooeps ? dz : copysignf(ooeps, dz));
+                oodx = ox * idirx;
+                oody = oy * idiry;
+                oodz = oz * idirz;
+                st.sp = 0;
+                st.off = 0;
+                leaf_addr = 0;
+                node_addr = a.num_nodes > 0 ? (a.ray_root ? a.ray_root[rid] : 0) : kSentinel;
+                hit_t = r1.w;
+                hit_u = 0.0f;
+                hit_v = 0.0f;
+                hit_id = -1;
+            }
+        }
+        if (take) cursor += __popcll(m);
+        // ends once the sequence is used up and no lane holds a ray (only the last
+        // group of the range can be partial)
+        if (cursor >= total && !__any(rid >= 0)) break;
+
         while (node_addr >= 0 && node_addr != kSentinel) {
             const float4 *np = a.nodes + (int64_t)node_addr * 4;
             const float4 n0xy = np[0];
@@ -245,60 +299,66 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
             if (node_addr < 0) node_addr = st.pop();
         }
     }
-
-    if (valid) {
-        if (ANY) {
-            a.flags[rid] = hit_id != -1 ? 1 : 0;
-        } else {
-            if (hit_id == -1) {
-                hit_u = 0.0f;
-                hit_v = 0.0f;
-            }
-            float4 out = make_float4(bf(hit_id), hit_t, hit_u, hit_v);
-            *reinterpret_cast<float4 *>(a.hits + rid) = out;
-        }
-    }
-    }
 }
 
-template <int V, bool ANY, int TRI>
-hipError_t launch_s(const TraceArgs &a, int S, int block, int num_cus, hipStream_t s) {
+// Persistent launch: as many blocks as the CUs hold at once (occupancy of this
+// instantiation at this block size and LDS), and for a known ray count no more
+// blocks than 64-ray groups. Otherwise one wave per 64-ray group (every wave's
+// sequence is its one group): the hardware dispatcher then balances the waves,
+// which wins where rays are coherent or the count is known.
+template <class K>
+hipError_t launch_k(K kernel, const TraceArgs &a, bool persistent, int block, size_t lds, int num_cus, hipStream_t s) {
     int64_t grid = (a.n + block - 1) / block;
-    // queue launches: 8 times as many blocks as the chip holds (32 waves per CU), so
-    // the dispatcher still balances uneven traversals while a near-empty queue costs
-    // a few microseconds instead of one wave per slot of the capacity
-    if (a.n_dev) grid = std::min<int64_t>(grid, 8LL * std::max(1, num_cus) * (2048 / block));
-    const size_t lds = (size_t)S * block * sizeof(int);
-    switch (S) {
-        case 8: hipLaunchKernelGGL((k_trace<V, ANY, 8, TRI>), dim3((unsigned)grid), dim3(block), lds, s, a); break;
-        case 16: hipLaunchKernelGGL((k_trace<V, ANY, 16, TRI>), dim3((unsigned)grid), dim3(block), lds, s, a); break;
-        case 24: hipLaunchKernelGGL((k_trace<V, ANY, 24, TRI>), dim3((unsigned)grid), dim3(block), lds, s, a); break;
-        case 32: hipLaunchKernelGGL((k_trace<V, ANY, 32, TRI>), dim3((unsigned)grid), dim3(block), lds, s, a); break;
-        default: return hipErrorInvalidValue;
+    if (persistent) {
+        int per_cu = 0;
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, lds);
+        if (e != hipSuccess) return e;
+        int64_t resident = (int64_t)std::max(1, per_cu) * std::max(1, num_cus);
+        grid = a.n_dev ? resident : std::min<int64_t>(resident, (a.n + 63) / 64);
     }
+    grid = std::max<int64_t>(1, grid);
+    hipLaunchKernelGGL(kernel, dim3((unsigned)grid), dim3(block), lds, s, a);
     return hipGetLastError();
 }
 
+template <int V, bool ANY, int TRI>
+hipError_t launch_s(const TraceArgs &a, bool persistent, int S, int block, int num_cus, hipStream_t s) {
+    const size_t lds = (size_t)S * block * sizeof(int);
+    switch (S) {
+        case 8: return launch_k(k_trace<V, ANY, 8, TRI>, a, persistent, block, lds, num_cus, s);
+        case 16: return launch_k(k_trace<V, ANY, 16, TRI>, a, persistent, block, lds, num_cus, s);
+        case 24: return launch_k(k_trace<V, ANY, 24, TRI>, a, persistent, block, lds, num_cus, s);
+        case 32: return launch_k(k_trace<V, ANY, 32, TRI>, a, persistent, block, lds, num_cus, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 template <int TRI>
-hipError_t launch_v(const TraceArgs &a, int variant, bool any, int S, int block, int num_cus, hipStream_t s) {
+hipError_t launch_v(const TraceArgs &a, bool persistent, int variant, bool any, int S, int block, int num_cus,
+                    hipStream_t s) {
     if (variant == 0)
-        return any ? launch_s<0, true, TRI>(a, S, block, num_cus, s) : launch_s<0, false, TRI>(a, S, block, num_cus, s);
+        return any ? launch_s<0, true, TRI>(a, persistent, S, block, num_cus, s)
+                   : launch_s<0, false, TRI>(a, persistent, S, block, num_cus, s);
     if (variant == 1)
-        return any ? launch_s<1, true, TRI>(a, S, block, num_cus, s) : launch_s<1, false, TRI>(a, S, block, num_cus, s);
+        return any ? launch_s<1, true, TRI>(a, persistent, S, block, num_cus, s)
+                   : launch_s<1, false, TRI>(a, persistent, S, block, num_cus, s);
     return hipErrorInvalidValue;
 }
 
 }  // namespace
 
 hipError_t launch_trace(const TraceArgs &a, int variant, bool any, int lds_stack, int block, int xcd_order,
-                        int num_cus, hipStream_t s, int tri_mode) {
+                        int refill, int num_cus, hipStream_t s, int tri_mode) {
     if (a.n <= 0) return hipSuccess;
     if (block < 64 || block > 256 || block % 64 != 0) return hipErrorInvalidValue;
     if ((a.n + block - 1) / block > 0x7FFFFFFF) return hipErrorInvalidValue;
     TraceArgs b = a;
     b.xcd_order = xcd_order ? 1 : 0;
-    if (tri_mode == kTriReference) return launch_v<kTriReference>(b, variant, any, lds_stack, block, num_cus, s);
-    if (tri_mode == kTriTwoSided) return launch_v<kTriTwoSided>(b, variant, any, lds_stack, block, num_cus, s);
+    // refill 0: one wave per 64-ray group; 1..64: persistent waves with that threshold
+    const bool persistent = refill > 0;
+    b.refill = persistent ? (refill > 64 ? 64 : refill) : 64;
+    if (tri_mode == kTriReference) return launch_v<kTriReference>(b, persistent, variant, any, lds_stack, block, num_cus, s);
+    if (tri_mode == kTriTwoSided) return launch_v<kTriTwoSided>(b, persistent, variant, any, lds_stack, block, num_cus, s);
     return hipErrorInvalidValue;
 }
 
