@@ -75,6 +75,37 @@ def test_any_hit_matches_closest(tt, oracle, ctx):
     s.close()
 
 
+def test_persistent_refill_matches_oracle(tt, oracle, ctx):
+    """Persistent waves (refill 1..64: wave w of the resident waves takes the 64-ray
+    groups w, w + W, ...; idle lanes take the wave's next ray) trace each ray as one
+    ray per lane does: hit records bit-identical to the oracle for the exact loop and
+    for every refill threshold, closest and any hit, at a ragged ray count and at
+    counts below one group per resident wave; the empty scene returns misses."""
+    _say("trace persistent refill")
+    m = tt.Mesh.soup(40000, seed=11)
+    b, s = _scene(tt, ctx, m)
+    rays = _rays(tt, m, 97, 61, 30001, seed=11)          # 41,835 rays: not a multiple of 64
+    cb = oracle.CompactBVH(b.export_compact())
+    oh, _ = cb.trace(rays)
+    for refill in (0, 1, 16, 33, 64):
+        ids, t, u, v = s.intersect(rays, tt.trace_cfg(0, 16, 64, refill=refill))
+        _check_exact(oracle, oh, ids, t, u, v)
+        flags = s.occluded(rays, tt.trace_cfg(1, 16, 128, refill=refill))
+        assert np.array_equal(flags, (oracle.hit_ids(oh) >= 0).astype(np.int32))
+    for n in (1, 65, 4097):
+        ohn, _ = cb.trace(rays[:n])
+        ids, t, u, v = s.intersect(rays[:n], tt.trace_cfg(0, 16, 64, refill=8))
+        _check_exact(oracle, ohn, ids, t, u, v)
+    with pytest.raises(tt.TTError):
+        s.intersect(rays[:10], tt.trace_cfg(0, 16, 64, refill=65))
+    s.close()
+    e = tt.Mesh.from_arrays(np.zeros((0, 3)), np.zeros((0, 3)))
+    b, s = _scene(tt, ctx, e)
+    ids, t, u, v = s.intersect(rays[:100], tt.trace_cfg(0, 16, 64, refill=16))
+    assert (ids == -1).all() and np.array_equal(t, rays[:100, 7])
+    s.close()
+
+
 def test_ragged_and_empty_inputs(tt, oracle, ctx):
     _say("trace ragged/empty")
     m = tt.Mesh.soup(3000, seed=9)
