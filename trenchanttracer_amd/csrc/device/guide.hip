@@ -611,14 +611,12 @@ hipError_t guide_infer(tt_guide *g, const float *in, float *raw, int64_t n, cons
     a.raw = raw;
     a.n = n;
     a.n_dev = n_dev;
-    // one block per CU (its LDS holds the weights and kInferWaves tiles)
-    static bool lds_set = false;
-    if (!lds_set) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_mlp_infer),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, kInferLds);
-        if (e != hipSuccess) return e;
-        lds_set = true;
-    }
+    // one block per CU (its LDS holds the weights and kInferWaves tiles); the limit is
+    // set on every call (a host-side call of microseconds), so it holds on whichever
+    // device and thread launches
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_mlp_infer),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, kInferLds);
+    if (e != hipSuccess) return e;
     const int64_t tiles = (n + 15) / 16;
     const int64_t grid = std::max<int64_t>(
         1, std::min<int64_t>((tiles + kInferWaves - 1) / kInferWaves, std::max(1, g->ctx->num_cus)));
