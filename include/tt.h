@@ -15,7 +15,7 @@
  *   - BVH::BVH + BVH::createHolder          src/bvh/BVH.cpp:3-63,358      -> tt_bvh_build
  *   - BVHCompact::createCompact             src/bvh/BVHCompact.cpp:88-250 -> tt_bvh_export_compact
  *   - BVHCompact(FILE*) / BVHCompact::save  src/bvh/BVHCompact.cpp:23-79,319-375 -> tt_bvh_save / tt_bvh_load
- *   - Renderer::init / Renderer::Renderer   src/core/Renderer.cpp:8-57    -> tt_ctx_create + tt_scene_create
+ *   - Renderer::Renderer                    src/core/Renderer.cpp:8-57    -> tt_ctx_create + tt_scene_create
  *   - intersect(ray, bvhCompact, true)      include/geometry/IntersectKernel.cuh:78-296 -> tt_trace_closest
  *   - intersect(ray, bvhCompact, false)     include/geometry/IntersectKernel.cuh:265-269 -> tt_trace_any
  *   - Renderer::render / pathTracingKernel  src/core/RenderKernel.cu:97-233 -> tt_render
@@ -106,7 +106,8 @@ tt_status tt_mesh_get(const tt_mesh *m, float *verts, int32_t *tris, float *norm
 /* Appends src (vertex/normal indices rebased, material ids offset by mat_offset). */
 tt_status tt_mesh_append(tt_mesh *dst, const tt_mesh *src, int32_t mat_offset);
 
-/* Procedural generators (deterministic for a given seed).
+/* Procedural generators (deterministic for a given seed) of the measured workloads;
+ * no reference counterpart (the reference loads its scenes from files).
  * Soup: ntris equilateral triangles, centroid ~ U[0,1]^3, side ~ LogNormal(mu, sigma),
  *       uniformly random orientation; face normals. */
 tt_status tt_mesh_gen_soup(int32_t ntris, uint64_t seed, float mu, float sigma, tt_mesh **out);
@@ -124,7 +125,8 @@ tt_status tt_mesh_gen_quad(const float p0[3], const float p1[3], const float p2[
 tt_status tt_mesh_assign_materials(tt_mesh *m, int32_t k, const float *fractions,
                                    const int32_t *mats, uint64_t seed);
 
-/* Ray sets of the traversal workload.
+/* Ray sets of the traversal workload (no reference counterpart: the reference
+ * generates its camera rays inside pathTracingKernel, src/core/RenderKernel.cu:128-197).
  * Primary: width*height*spp pinhole rays (eye, look-at target, fovx degrees),
  *   jittered in-pixel, ordered in 8x8 pixel tiles (spp-major inside a tile).
  * Incoherent: n rays from uniform points on uniformly chosen triangles, cosine
@@ -171,13 +173,16 @@ tt_status tt_bvh_load(const char *path, tt_bvh **out);
 typedef struct tt_ctx tt_ctx;
 typedef struct tt_scene tt_scene;
 
+/* Device, stream and per-frame device state of the renderer: replaces the buffers
+ * Renderer::Renderer allocates for a frame (accumulation buffer, camera and render
+ * metadata, src/core/Renderer.cpp:25-41); destroy scenes before their context. */
 tt_status tt_ctx_create(int32_t device, tt_ctx **out);
 void tt_ctx_destroy(tt_ctx *ctx);
 /* Synchronises the context's stream. */
 tt_status tt_ctx_sync(tt_ctx *ctx);
 
 /* Device memory and timing on the context's device / stream (plumbing for callers
- * that hold no other HIP runtime). Copies are synchronous. Timer: `n` HIP event
+ * that hold no other HIP runtime; no reference counterpart). Copies are synchronous. Timer: `n` HIP event
  * slots recorded on the context stream; elapsed = ms between two recorded slots. */
 tt_status tt_dev_alloc(tt_ctx *ctx, size_t bytes, void **out);
 tt_status tt_dev_free(tt_ctx *ctx, void *p);
@@ -239,7 +244,8 @@ typedef struct tt_geometry {
     float pad;
 } tt_geometry;              /* 48 B */
 
-/* Uploads the SBVH device layout, per-triangle shading normals and materials.
+/* Uploads the SBVH device layout, per-triangle shading normals and materials
+ * (replaces the uploads of Renderer::Renderer, src/core/Renderer.cpp:12-23).
  * The env map (RGB float, width x height, equirectangular, may be NULL = black)
  * gets its marginal/conditional CDF built on the GPU (FP64 prefix sums). */
 tt_status tt_scene_create(tt_ctx *ctx, const tt_mesh *mesh, const tt_bvh *bvh,
@@ -276,12 +282,14 @@ typedef struct tt_trace_cfg {
     int32_t refill;
 } tt_trace_cfg;
 
-/* Closest hit for n rays. rays / hits are DEVICE pointers; stream is a hipStream_t
+/* Closest hit for n rays (replaces intersect(ray, bvh, true),
+ * include/geometry/IntersectKernel.cuh:78-296). rays / hits are DEVICE pointers; stream is a hipStream_t
  * (NULL = the context stream). Asynchronous: synchronise before reading hits.
  * cfg may be NULL (autotuned default). */
 tt_status tt_trace_closest(tt_scene *s, const tt_ray *rays, int64_t n, tt_hit *hits,
                            const tt_trace_cfg *cfg, void *stream);
-/* Any hit (shadow rays): hit_flags[i] = 1 if some triangle is hit in (tmin, tmax). */
+/* Any hit (shadow rays; intersect(ray, bvh, false), IntersectKernel.cuh:265-269):
+ * hit_flags[i] = 1 if some triangle is hit in (tmin, tmax). */
 tt_status tt_trace_any(tt_scene *s, const tt_ray *rays, int64_t n, int32_t *hit_flags,
                        const tt_trace_cfg *cfg, void *stream);
 /* Host-buffer convenience wrappers (copy in, trace, copy out, synchronise). */
@@ -346,13 +354,15 @@ typedef struct tt_render_stats {
     int32_t paths_in_flight;  /* paths per pass */
 } tt_render_stats;
 
-/* Renders one image. rgb_out (host, width*height*3 floats, row 0 = top) receives the
+/* Renders one image (replaces Renderer::render and renderKernel,
+ * src/core/RenderKernel.cu:97-233, src/core/Renderer.cpp:82-134). rgb_out (host, width*height*3 floats, row 0 = top) receives the
  * per-pixel mean over the spp samples; rgb64_out (host, may be NULL) the FP64 mean
  * when accumulate_fp64 is set. Synchronous. */
 tt_status tt_render(tt_scene *s, const tt_camera *cam, const tt_render_params *p, float *rgb_out,
                     double *rgb64_out, tt_render_stats *stats);
 
-/* Batched rendering: up to 64 scenes in one wavefront (one node / triangle array,
+/* Batched rendering (no reference counterpart; the reference renders one scene):
+ * up to 64 scenes in one wavefront (one node / triangle array,
  * per-ray root nodes, per-scene camera and resolution), sharing one env map.
  * tt_batch_render fills rgb_out[k] (width_k * height_k * 3 floats) for camera k;
  * the RNG stream of each scene equals that of rendering it alone. */
@@ -364,7 +374,7 @@ void tt_batch_destroy(tt_batch *b);
 tt_status tt_batch_render(tt_batch *b, const tt_camera *cams, const tt_render_params *p, float *const *rgb_out,
                           tt_render_stats *stats);
 
-/* Learned path-guiding proposal: MLP (frequency encoding of position / normal /
+/* Learned path-guiding proposal (no reference counterpart): MLP (frequency encoding of position / normal /
  * outgoing direction -> 3 x 64 -> 8-lobe vMF mixture), BF16 MFMA forward and
  * backward kernels with FP32 accumulation, FP32 master weights, Adam.
  * inputs: n x 9 floats in [0, 1]; raw outputs: n x 40 floats (lobe k: mean
@@ -458,7 +468,8 @@ tt_status tt_guide_render(tt_scene *s, tt_guide *g, const tt_camera *cam, const 
                           const tt_guide_params *gp, float *rgb_out, double *rgb64_out, tt_render_stats *stats,
                           tt_guide_stats *gstats);
 
-/* Sampling hook for the guiding mixture: decodes one row of 40 raw network outputs
+/* Test hooks (no reference counterpart).
+ * Sampling hook for the guiding mixture: decodes one row of 40 raw network outputs
  * (host), draws n directions with the device's mixture sampler and returns n x 4
  * floats {direction.xyz, mixture density at the direction}. Synchronous. */
 tt_status tt_debug_guide_sample(tt_ctx *ctx, const float *raw, int64_t n, uint32_t seed, float *out);

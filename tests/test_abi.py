@@ -31,6 +31,19 @@ def test_struct_sizes(tt):
     assert ctypes.sizeof(N.tt_ray) == 32
     assert ctypes.sizeof(N.tt_hit) == 16
     assert ctypes.sizeof(N.tt_material) == 48
+    # the same sizes the library asserts at compile time (api_host.cpp)
+    assert ctypes.sizeof(N.tt_geometry) == 48
+    assert ctypes.sizeof(N.tt_trace_cfg) == 20
+    assert ctypes.sizeof(N.tt_camera) == 60
+    assert ctypes.sizeof(N.tt_render_params) == 72
+    assert ctypes.sizeof(N.tt_render_stats) == 40
+    assert ctypes.sizeof(N.tt_guide_params) == 24
+    assert ctypes.sizeof(N.tt_guide_stats) == 32
+
+
+def test_abi_version_matches_binding(tt):
+    from trenchanttracer_amd import _native as N
+    assert N.lib.tt_abi_version() == N.ABI_VERSION
 
 
 def test_no_gpu_entry_points_fail_cleanly_without_device(tt):

@@ -8,6 +8,7 @@ import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtt.so")
+ABI_VERSION = 2   # tt_abi_version() of the library this binding's structs describe
 
 TT_OK = 0
 STATUS_NAMES = {0: "TT_OK", 1: "TT_ERR_INVALID", 2: "TT_ERR_NOMEM", 3: "TT_ERR_DEVICE",
@@ -182,6 +183,10 @@ def _load():
         raise ImportError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`"
                           " (the product has no fallback path)")
     lib = C.CDLL(LIB_PATH)
+    lib.tt_abi_version.restype = C.c_int
+    if lib.tt_abi_version() != ABI_VERSION:
+        raise ImportError(f"{LIB_PATH} has ABI version {lib.tt_abi_version()}, this binding expects {ABI_VERSION}:"
+                          " rebuild it")
     for name, (res, args) in PROTOS.items():
         try:
             fn = getattr(lib, name)
