@@ -324,7 +324,7 @@ typedef enum {
 } tt_render_mode;
 
 typedef struct tt_render_params {
-    int32_t spp;              /* samples per pixel (one wavefront pass each) */
+    int32_t spp;              /* samples per pixel */
     int32_t max_depth;        /* path segments (the reference hard-codes 10 bounces, RenderKernel.cu:35) */
     int32_t rr_depth;         /* Russian roulette from this segment on (path mode) */
     uint32_t seed;            /* Philox key */
@@ -335,7 +335,7 @@ typedef struct tt_render_params {
     float pi, two_pi;         /* camera / env constants (reference: 3.14156265f and 6.2831853f) */
     float tmin;               /* reference mode ray tmin (0.01, RenderKernel.cu:110) */
     int32_t use_graph;        /* capture one sample pass in a hipGraph and replay it */
-    int32_t paths_in_flight;  /* path mode: target paths per pass, rounded to whole samples (0 = up to
+    int32_t paths_in_flight;  /* target paths per pass, rounded to whole samples (0 = up to
                                  2^27 paths, 188 B of state each, 416 B with a guide;
                                  guided renders that train the network: about 8M) */
     int32_t lds_stack;        /* traversal LDS stack entries per lane (0 = 16) */

@@ -53,6 +53,27 @@ def test_reference_render_cornell_matches_oracle(tt, oracle, ctx):
     assert img.mean() > 0.01
 
 
+@pytest.mark.parametrize("paths", [0, 256 * 256, 3 * 256 * 256])
+def test_reference_render_any_pass_structure_matches_oracle(tt, oracle, ctx, paths):
+    """Reference mode keeps each path's radiance in its slot and ends a pass by adding
+    a pixel's samples in sample order (acc + L * 1/spp, as the reference accumulates):
+    16 spp in one pass, in 16 passes of one sample, or in passes of 3 samples (the
+    last pass partial) give the oracle's image bit for bit."""
+    from trenchanttracer_amd import workloads
+    _say(f"reference render pass structure, paths in flight {paths}")
+    mesh, mats, cam = workloads.cornell(subdiv=3)
+    b = tt.BVH.build(mesh)
+    s = tt.DeviceScene(ctx, mesh, b, mats)
+    img, st = s.render(cam, tt.render_params(spp=16, max_depth=8, seed=5, mode=tt.RENDER_REFERENCE,
+                                             paths_in_flight=paths))
+    ref = oracle.render_ref(oracle.CompactBVH(b.export_compact()), mesh.arrays(), mats, cam, spp=16, max_depth=8,
+                            seed=5)
+    s.close()
+    _say(f"  {st}")
+    assert st["paths_in_flight"] == (256 * 256 * 16 if paths == 0 else paths)
+    assert np.array_equal(img.view(np.int32), ref.view(np.int32))
+
+
 def test_reference_render_large_frame_matches_oracle(tt, oracle, ctx):
     """1024x640 (655,360 paths in one queue): more rays than the traversal kernel's
     grid holds at once (32 one-wave blocks per CU), so its blocks stride over the

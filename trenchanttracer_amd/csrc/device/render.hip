@@ -562,12 +562,9 @@ __device__ __forceinline__ void shade_reference_one(const ShadeArgs &a, const in
                 a.L[p] = make_float4(Lp.x, Lp.y, Lp.z, 0.0f);
             }
         }
-        if (finish) {
-            float *o = a.acc + 3 * (int64_t)pix;
-            o[0] = o[0] + Lp.x * a.inv_spp;
-            o[1] = o[1] + Lp.y * a.inv_spp;
-            o[2] = o[2] + Lp.z * a.inv_spp;
-        }
+        // the finished path's radiance stays in its slot; k_resolve_reference adds
+        // it to the pixel in sample order
+        if (finish) a.L[p] = make_float4(Lp.x, Lp.y, Lp.z, 0.0f);
     }
     int slot, unused;
     block_append2(cont, &a.counts[a.nxt], false, nullptr, &slot, &unused);
@@ -1129,6 +1126,27 @@ __global__ __launch_bounds__(256) void k_apply_shadow(ApplyArgs a) {
 // End of a path-mode pass: each pixel's radiance is the sum of its kk paths'
 // radiance (paths s * npix + pixel), added in sample order to the float or FP64
 // accumulation, so images do not depend on the order threads finish.
+// Reference mode: acc += L * (1 / spp) for the pass's kk samples of each pixel in
+// sample order, one multiply and one add per sample and channel as the reference
+// accumulates (src/core/RenderKernel.cu:198-199), so a pass of many samples gives
+// the float sums of one sample per pass bit for bit.
+__global__ __launch_bounds__(256) void k_resolve_reference(const float4 *L, int32_t npix, int32_t kk, float inv_spp,
+                                                           float *acc) {
+    const int32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= npix) return;
+    float *o = acc + 3 * (int64_t)pix;
+    float r = o[0], g = o[1], b = o[2];
+    for (int32_t s = 0; s < kk; ++s) {
+        const float4 v = L[(int64_t)s * npix + pix];
+        r = r + v.x * inv_spp;
+        g = g + v.y * inv_spp;
+        b = b + v.z * inv_spp;
+    }
+    o[0] = r;
+    o[1] = g;
+    o[2] = b;
+}
+
 __global__ __launch_bounds__(256) void k_resolve(const float4 *L, int32_t npix, int32_t kk, float *acc,
                                                  double *acc64) {
     const int32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1514,6 +1532,10 @@ hipError_t launch_pass(tt_scene *s, RenderState *rs, const tt_render_params *p, 
         hipLaunchKernelGGL(k_resolve, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, hs, (const float4 *)rs->L,
                            npix, kk, fp64 ? nullptr : rs->acc, fp64 ? rs->acc64 : nullptr);
         e = hipGetLastError();
+    } else if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_resolve_reference, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, hs,
+                           (const float4 *)rs->L, npix, kk, 1.0f / (float)p->spp, rs->acc);
+        e = hipGetLastError();
     }
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_advance_sample, dim3(1), dim3(1), 0, hs, rs->sample_dev, kk);
@@ -1647,15 +1669,15 @@ tt_status render_impl(tt_scene *s, int32_t nsc, const tt_camera *cams, const int
     const bool fp64 = p->accumulate_fp64 != 0;
     tt_ctx *ctx = s->ctx;
     if (hipSetDevice(ctx->device) != hipSuccess) return fail(TT_ERR_DEVICE, "hipSetDevice failed");
-    // path mode keeps k samples per pixel in flight; reference mode runs one sample
-    // per pass, which keeps the per-pixel float sums in sample order. Each bounce's
-    // launches last as long as their slowest waves, so the fewer the passes the
-    // less that tail costs: by default up to 2^27 paths (188 B of state each, 25 GB)
+    // k samples per pixel are in flight in one pass (each path's radiance is kept in
+    // its own slot and the pass ends by adding a pixel's samples in sample order, in
+    // both modes). Each bounce's launches last as long as their slowest waves, so the
+    // fewer the passes the less that tail costs: by default up to 2^27 paths (188 B of state each, 25 GB)
     // are in flight, guided renders with a frozen network included (228 B more per
     // path for the network's inputs and mixture). Renders that train the network
     // default to about 8M paths: it learns between passes.
     int32_t k = 1;
-    if (p->mode == TT_RENDER_PATH) {
+    {
         const int32_t target = p->paths_in_flight > 0 ? p->paths_in_flight : (gr && gr->train) ? (8 << 20) : (1 << 27);
         k = std::max(1, target / npix);
         k = std::min(k, p->spp);
