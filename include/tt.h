@@ -285,7 +285,8 @@ typedef struct tt_trace_cfg {
 /* Closest hit for n rays (replaces intersect(ray, bvh, true),
  * include/geometry/IntersectKernel.cuh:78-296). rays / hits are DEVICE pointers; stream is a hipStream_t
  * (NULL = the context stream). Asynchronous: synchronise before reading hits.
- * cfg may be NULL (autotuned default). */
+ * cfg may be NULL: exact loop, 16 LDS entries, 128-thread blocks, one wave per 64-ray
+ * group (trenchanttracer_amd.autotune.tune_trace picks a configuration per workload). */
 tt_status tt_trace_closest(tt_scene *s, const tt_ray *rays, int64_t n, tt_hit *hits,
                            const tt_trace_cfg *cfg, void *stream);
 /* Any hit (shadow rays; intersect(ray, bvh, false), IntersectKernel.cuh:265-269):
