@@ -39,7 +39,7 @@ def main():
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(scenes are replicated per GPU and rays shard without exchange; see DESIGN.md). "
+              "(scenes are replicated per GPU and rays shard without exchange). "
               "Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
