@@ -434,23 +434,29 @@ __global__ __launch_bounds__(256) void k_wgrad(const __bf16 *__restrict__ H, int
     if (threadIdx.x < KO) pbias[(int64_t)blockIdx.x * KO + threadIdx.x] = bsum;
 }
 
-// grad[offset + dst(e)] = sum_b partial[b][e] for e < count: 64 elements per
-// block, 4 threads per element each summing every 4th block's partial in order,
-// combined in a fixed order (deterministic). dst transposes [in][out] partials to
+// Sum of the nb per-block partial gradients of each element, in a fixed order
+// (deterministic): a block takes 64 elements, its kRedParts waves each sum every
+// kRedParts-th partial (unrolled, so several loads are in flight per lane), and the
+// first wave adds the part sums in part order. dst transposes [in][out] partials to
 // the [out][in] master layout when transpose_ko is set.
-__global__ __launch_bounds__(256) void k_reduce(const float *__restrict__ partial, int nb, int count,
-                                                float *__restrict__ grad, int offset, bool transpose_ko,
-                                                int ki, int ko) {
-    __shared__ float red[4][64];
+constexpr int kRedParts = 16;
+__global__ __launch_bounds__(64 * kRedParts) void k_reduce(const float *__restrict__ partial, int nb, int count,
+                                                           float *__restrict__ grad, int offset, bool transpose_ko,
+                                                           int ki, int ko) {
+    __shared__ float red[kRedParts][64];
     const int el = threadIdx.x & 63, part = threadIdx.x >> 6;
     const int e = blockIdx.x * 64 + el;
     float s = 0.0f;
-    if (e < count)
-        for (int b = part; b < nb; b += 4) s += partial[(int64_t)b * count + e];
+    if (e < count) {
+#pragma unroll 8
+        for (int b = part; b < nb; b += kRedParts) s += partial[(int64_t)b * count + e];
+    }
     red[part][el] = s;
     __syncthreads();
     if (part == 0 && e < count) {
-        const float t = ((red[0][el] + red[1][el]) + red[2][el]) + red[3][el];
+        float t = red[0][el];
+#pragma unroll
+        for (int q = 1; q < kRedParts; ++q) t += red[q][el];
         int dst = e;
         if (transpose_ko) {
             const int i = e / ko, o = e % ko;
@@ -680,10 +686,10 @@ static hipError_t guide_backward(tt_guide *g, int64_t n, hipStream_t s) {
             hipLaunchKernelGGL((k_wgrad<kGuideH, kGuideNOutPad16>), dim3(nb), dim3(256), 0, s, l.h, l.ldh, l.d, l.ldd,
                                R, rpb, g->partial, pbias);
         const int cnt = l.ki * l.ko;
-        hipLaunchKernelGGL(k_reduce, dim3((cnt + 63) / 64), dim3(256), 0, s, (const float *)g->partial, nb, cnt,
-                           g->grad, (int)layer_offset(k), true, l.ki, l.ko);
-        hipLaunchKernelGGL(k_reduce, dim3((l.ko + 63) / 64), dim3(256), 0, s, (const float *)pbias, nb, l.ko,
-                           g->grad, (int)(layer_offset(k) + cnt), false, l.ki, l.ko);
+        hipLaunchKernelGGL(k_reduce, dim3((cnt + 63) / 64), dim3(64 * kRedParts), 0, s, (const float *)g->partial, nb,
+                           cnt, g->grad, (int)layer_offset(k), true, l.ki, l.ko);
+        hipLaunchKernelGGL(k_reduce, dim3((l.ko + 63) / 64), dim3(64 * kRedParts), 0, s, (const float *)pbias, nb,
+                           l.ko, g->grad, (int)(layer_offset(k) + cnt), false, l.ki, l.ko);
     }
     return hipGetLastError();
 }
