@@ -455,6 +455,8 @@ typedef struct tt_guide_params {
     int32_t train_paths;  /* paths recorded per pass (0: 65536) */
     int32_t train_batch;  /* rows per Adam step (0: 262144, at most 4194304) */
     int32_t train_spp;    /* samples per pixel after which training stops (0: never) */
+    int32_t max_bounce;   /* the network guides bounces 0 .. max_bounce - 1 (0: every bounce);
+                             later vertices sample the BSDF alone */
 } tt_guide_params;
 
 typedef struct tt_guide_stats {

@@ -79,8 +79,8 @@ def test_mixture_sampler_matches_density(tt, ctx):
 def test_guided_render_unbiased_in_emissive_box(tt, ctx):
     """Inside a closed box whose walls have albedo 0.5 and emission 0.5 the radiance
     is 1 everywhere. Guided renders (alpha 0.5) with constant guides that point one
-    fixed way, sharp (kappa 200) and broad (kappa 5), and with a network trained
-    online from scratch all average to 1. (With alpha near 1 and a sharp wrong
+    fixed way, sharp (kappa 200) and broad (kappa 5), the broad one on the first two
+    bounces only, and with a network trained online from scratch all average to 1. (With alpha near 1 and a sharp wrong
     guide the estimator stays unbiased but its per-pixel spread approaches 1, too
     wide for a mean test.)"""
     from trenchanttracer_amd import workloads as W
@@ -93,6 +93,10 @@ def test_guided_render_unbiased_in_emissive_box(tt, ctx):
         img, st, gs = g.render(sc, cam, p, alpha=0.5, train=False)
         _say(f"box, constant guide kappa {kappa}: mean {img.mean():.5f}, pixel std {img.mean(2).std():.3f}")
         assert abs(img.mean() - 1) < 0.005
+    # guiding only the first two bounces (later vertices sample the BSDF alone)
+    img, st, gs = g.render(sc, cam, p, alpha=0.5, train=False, max_bounce=2)
+    _say(f"box, constant guide kappa 5, bounces 0-1 guided: mean {img.mean():.5f}")
+    assert abs(img.mean() - 1) < 0.005
     g2 = tt.Guide(ctx, seed=3)
     p = tt.render_params(spp=256, max_depth=32, rr_depth=4, seed=3, mode=tt.RENDER_PATH, paths_in_flight=16384)
     img, st, gs = g2.render(sc, cam, p, alpha=0.5, train=True, train_paths=4096, train_batch=8192)

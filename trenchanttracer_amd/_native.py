@@ -8,7 +8,7 @@ import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtt.so")
-ABI_VERSION = 2   # tt_abi_version() of the library this binding's structs describe
+ABI_VERSION = 3   # tt_abi_version() of the library this binding's structs describe
 
 TT_OK = 0
 STATUS_NAMES = {0: "TT_OK", 1: "TT_ERR_INVALID", 2: "TT_ERR_NOMEM", 3: "TT_ERR_DEVICE",
@@ -86,7 +86,7 @@ class tt_config_info(C.Structure):
 
 class tt_guide_params(C.Structure):
     _fields_ = [("alpha", C.c_float), ("min_roughness", C.c_float), ("train", C.c_int32), ("train_paths", C.c_int32),
-                ("train_batch", C.c_int32), ("train_spp", C.c_int32)]
+                ("train_batch", C.c_int32), ("train_spp", C.c_int32), ("max_bounce", C.c_int32)]
 
 
 class tt_guide_stats(C.Structure):

@@ -1352,6 +1352,7 @@ struct GuideRun {
     bool train = false;
     int32_t batch_rows = 1 << 18;
     int32_t train_spp = 0;    // train only while fewer samples than this are done (0: always)
+    int32_t max_bounce = 0;   // the network guides bounces below this (0: every bounce)
     int64_t steps = 0, rows = 0;
     double train_ms = 0.0;
 };
@@ -1436,7 +1437,11 @@ hipError_t launch_bounce(tt_scene *s, RenderState *rs, const tt_render_params *p
         if (e != hipSuccess) return e;
     }
     ShadeArgs sa{};
-    if (gr) {
+    // the network guides bounces below max_bounce; a training pass records every
+    // bounce (the guided shade kernel with no mixture samples the BSDF alone)
+    const bool guide_here = gr && (gr->max_bounce <= 0 || b < gr->max_bounce);
+    const bool records = gr && gr->gd.rec != nullptr;
+    if (guide_here) {
         // guide inference over the hit queue: inputs, then the MLP (rows up to the device count)
         GuideInArgs ga{gr->gd, sv, rs->rays[cur], rs->hits, rs->counts, cur, gr->g->iin};
         hipLaunchKernelGGL(k_guide_inputs, dim3(gblocks), dim3(256), 0, hs, ga);
@@ -1445,6 +1450,9 @@ hipError_t launch_bounce(tt_scene *s, RenderState *rs, const tt_render_params *p
         if (e != hipSuccess) return e;
         sa.gd = gr->gd;
         sa.gd.raw = gr->g->iraw;
+    } else if (records) {
+        sa.gd = gr->gd;
+        sa.gd.raw = nullptr;
     }
     sa.sv = sv;
     sa.tab = (const SceneTable *)rs->table;
@@ -1480,7 +1488,7 @@ hipError_t launch_bounce(tt_scene *s, RenderState *rs, const tt_render_params *p
         hipLaunchKernelGGL(k_shade_reference, dim3(sblocks), dim3(kAppendBlock), 0, hs, sa);
         return hipGetLastError();
     }
-    if (gr) hipLaunchKernelGGL(k_shade_path<true>, dim3(sblocks), dim3(kAppendBlock), 0, hs, sa);
+    if (guide_here || records) hipLaunchKernelGGL(k_shade_path<true>, dim3(sblocks), dim3(kAppendBlock), 0, hs, sa);
     else hipLaunchKernelGGL(k_shade_path<false>, dim3(sblocks), dim3(kAppendBlock), 0, hs, sa);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -1884,7 +1892,8 @@ tt_status tt_guide_render(tt_scene *s, tt_guide *g, const tt_camera *cam, const 
     if (!s || !g || !cam || !p || !gp || !rgb_out) return fail(TT_ERR_INVALID, "tt_guide_render: null argument");
     if (g->ctx != s->ctx) return fail(TT_ERR_INVALID, "tt_guide_render: guide and scene on different contexts");
     if (!(gp->alpha >= 0.0f && gp->alpha <= 1.0f)) return fail(TT_ERR_INVALID, "tt_guide_render: alpha outside [0, 1]");
-    if (gp->train_paths < 0 || gp->train_batch < 0 || gp->train_spp < 0 || gp->train_batch > (1 << 22))
+    if (gp->train_paths < 0 || gp->train_batch < 0 || gp->train_spp < 0 || gp->train_batch > (1 << 22) ||
+        gp->max_bounce < 0)
         return fail(TT_ERR_INVALID, "tt_guide_render: bad training sizes");
     tt_render_params q = *p;
     q.guided = 0;
@@ -1896,6 +1905,7 @@ tt_status tt_guide_render(tt_scene *s, tt_guide *g, const tt_camera *cam, const 
     gr.gd.ntrain = gp->train_paths > 0 ? gp->train_paths : (1 << 16);
     gr.batch_rows = gp->train_batch > 0 ? gp->train_batch : (1 << 18);
     gr.train_spp = gp->train_spp;
+    gr.max_bounce = gp->max_bounce;
     const int32_t root = 0;
     tt_status st = render_impl(s, 1, cam, &root, &q, rgb_out, rgb64_out, stats, &gr);
     if (st == TT_OK && gstats) {
