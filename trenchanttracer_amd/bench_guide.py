@@ -18,6 +18,12 @@ The same protocol on the door room (256x256, depth 12), where all light reaches
 the camera's part of the room through a door, with a 4096-spp oracle.
 
     python -m trenchanttracer_amd.bench_guide [--oracle-spp 8192] [--seeds 4] [--skip-c5] [--quick]
+    python -m trenchanttracer_amd.bench_guide --error-classes
+
+--error-classes: where the path-trace case's error sits. Plain 64-spp renders (seeds
+100, 101) against a 4096-spp FP64 oracle, pixels classed by the material the centre
+ray of the pixel hits first (diffuse, GGX conductor, glass, sky): share of pixels, share
+of the squared error, and the classes of the 400 worst pixels.
 """
 import argparse
 import json
@@ -97,17 +103,70 @@ def compare(tt, ctx, sc, cam, ref, depth, plain_spp, guided_spp, paths, lr, batc
     return out
 
 
+def error_classes(tt, ctx, log):
+    from trenchanttracer_amd import workloads
+    mesh, mats, env, cam = workloads.pathtrace()
+    sc = tt.DeviceScene(ctx, mesh, tt.BVH.build(mesh), mats, env=env)
+    _, ref, _ = sc.render(cam, tt.render_params(spp=4096, max_depth=16, rr_depth=4, seed=1234, mode=tt.RENDER_PATH,
+                                                fp64=True), want_fp64=True)
+    ref = ref.astype(np.float64)
+    W, H = cam.width, cam.height
+    # centre ray of each pixel (the render's pinhole camera: the image plane one unit
+    # along the view, spanned by tan(fov / 2) along the side and up axes)
+    eye, view, up = (np.array(v, np.float64) for v in (cam.eye, cam.view, cam.up))
+    view /= np.linalg.norm(view)
+    up /= np.linalg.norm(up)
+    hax = np.cross(view, up)
+    hax /= np.linalg.norm(hax)
+    vax = np.cross(hax, view)
+    hor, ver = hax * np.tan(np.radians(cam.fov[0]) / 2), vax * np.tan(-np.radians(cam.fov[1]) / 2)
+    Y, X = np.mgrid[0:H, 0:W]
+    d = view + hor * (2 * ((X + 0.5) / (W - 1))[..., None] - 1) + ver * (2 * ((Y + 0.5) / (H - 1))[..., None] - 1)
+    d /= np.linalg.norm(d, axis=-1, keepdims=True)
+    rays = np.zeros((H * W, 8), np.float32)
+    rays[:, 0:3] = eye
+    rays[:, 4:7] = d.reshape(-1, 3)
+    rays[:, 7] = 1e20
+    tid = sc.intersect(rays)[0]
+    tm = mesh.arrays()["tri_mats"]
+    cls = np.full(H * W, 3)
+    hit = tid >= 0
+    mt = tm[tid[hit]]
+    cls[hit] = np.where(mt == 0, 0, np.where(mt == len(mats) - 1, 2, 1))
+    names = ["diffuse", "conductor", "glass", "sky"]
+    out = {"pixel_share": {names[c]: float(np.mean(cls == c)) for c in range(4)}, "seeds": {}}
+    for seed in (100, 101):
+        img, _ = sc.render(cam, tt.render_params(spp=64, max_depth=16, rr_depth=4, seed=seed, mode=tt.RENDER_PATH))
+        e = ((img.astype(np.float64) - ref) ** 2).sum(-1).ravel()
+        worst = np.argsort(e)[::-1][:400]
+        out["seeds"][seed] = {
+            "mse": float(e.mean() / 3),
+            "mse_share": {names[c]: float(e[cls == c].sum() / e.sum()) for c in range(4)},
+            "worst400": {names[c]: int(np.sum(cls[worst] == c)) for c in range(4)},
+            "worst_0.01%_share": float(np.sort(e)[::-1][: e.size // 10000].sum() / e.sum()),
+        }
+        log(f"error classes seed {seed}: {out['seeds'][seed]}")
+    sc.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--oracle-spp", type=int, default=8192)
     ap.add_argument("--seeds", type=int, default=4, help="independent renders averaged per point")
     ap.add_argument("--skip-c5", action="store_true")
     ap.add_argument("--quick", action="store_true", help="door room only, one training run (for profiling)")
+    ap.add_argument("--error-classes", action="store_true", help="where the path-trace case's error sits")
     args = ap.parse_args()
     import trenchanttracer_amd as tt
     from trenchanttracer_amd import workloads
 
     ctx = tt.Context(0)
+    if args.error_classes:
+        print(json.dumps({"metric": "path-trace case: squared error by pixel class",
+                          "pathtrace_1080p": error_classes(tt, ctx, log)}), flush=True)
+        ctx.close()
+        return
     result = {"metric": "equal-time RMSE vs FP64 oracle, guided vs plain"}
 
     mesh, mats, cam = workloads.door_room(256, 256)

@@ -5,10 +5,12 @@ traversal (default): one closest-hit traversal launch over the traversal workloa
 whose known byte counts calibrate the FETCH_SIZE / WRITE_SIZE hardware counters.
 pathtrace: the path-trace case (1,310,720-tri displaced icosphere, sky + sun,
 1920x1080, depth 16) rendered once at 4 spp (warm-up) and once at 64 spp.
+train: 13 Adam steps of the guiding network on 262,144 seeded random rows (inputs in
+[0, 1], unit directions, Gamma(1) weights), the batch of the online training.
 For profiler runs, one counter set per run:
 
     rocprofv3 --pmc FETCH_SIZE --output-format csv -d OUT -o run -- \\
-        python3 -m trenchanttracer_amd.profile_trace [--case traversal|pathtrace] [--variant 1] [--lds-stack 16]
+        python3 -m trenchanttracer_amd.profile_trace [--case traversal|pathtrace|train] [--variant 1] [--lds-stack 16]
             [--block 64] [--paths-in-flight N]
 """
 import argparse
@@ -20,11 +22,29 @@ def main():
     ap.add_argument("--variant", type=int, default=1)
     ap.add_argument("--lds-stack", type=int, default=16)
     ap.add_argument("--block", type=int, default=64)
-    ap.add_argument("--case", choices=("traversal", "pathtrace"), default="traversal")
+    ap.add_argument("--case", choices=("traversal", "pathtrace", "train"), default="traversal")
     ap.add_argument("--paths-in-flight", type=int, default=0, help="pathtrace: paths per pass (0: the default)")
     args = ap.parse_args()
     import trenchanttracer_amd as tt
     from trenchanttracer_amd import workloads
+
+    if args.case == "train":
+        import numpy as np
+        ctx = tt.Context(0)
+        g = tt.Guide(ctx, seed=1, lr=1e-3)
+        rng = np.random.default_rng(1)
+        n = 1 << 18
+        x = rng.random((n, 9), dtype=np.float32)
+        d = rng.normal(size=(n, 3)).astype(np.float32)
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        w = rng.gamma(1.0, 1.0, n).astype(np.float32)
+        for _ in range(13):
+            loss = g.train_step(x, d, w)
+        ctx.sync()
+        print(f"13 training steps of {n} rows, last loss {loss:.4f}", file=sys.stderr, flush=True)
+        g.close()
+        ctx.close()
+        return
 
     if args.case == "pathtrace":
         mesh, mats, env, cam = workloads.pathtrace()
