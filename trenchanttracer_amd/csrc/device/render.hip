@@ -1,10 +1,11 @@
-// Wavefront path tracing on gfx950: one pass per sample over all pixels, each
-// bounce = extend (closest-hit traversal of the ray queue) -> shade (materials,
-// emission, next-event rays, continuation rays appended to the next queue with
-// a wave64 ballot + prefix popcount and one atomic per wave) -> shadow (any-hit
-// traversal of the next-event queue) -> apply (unoccluded contributions).
-// Queue lengths stay on the device; kernels read them, so the host never waits
-// between stages.
+// Wavefront path tracing on gfx950: a pass keeps k samples of every pixel in
+// flight (up to 2^27 paths); each bounce = extend (closest-hit traversal of the ray
+// queue) -> shade (materials, emission, next-event rays, continuation rays appended
+// to the next queue with a wave64 ballot + prefix popcount and one atomic per
+// 256-thread block) -> shadow (any-hit traversal of the next-event queue) -> apply
+// (unoccluded contributions); the pass ends by adding each pixel's paths in sample
+// order. Queue lengths stay on the device; kernels read them, so the host never
+// waits between stages.
 //
 // Reference mode restates the reference megakernel (src/core/RenderKernel.cu:27-218)
 // and its samplers (include/material/DiffImpl.cuh:12-36, MetalImpl.cuh:13-41,

@@ -1,6 +1,7 @@
 // Closest-hit / any-hit traversal of the SBVH on gfx950 (wave64).
 //
-// Per lane: one ray, the while-while loop of Aila & Laine ("Understanding the
+// Per lane: one ray at a time (a lane whose ray is done takes the next one of its
+// wave's sequence), the while-while loop of Aila & Laine ("Understanding the
 // Efficiency of Ray Traversal on GPUs", HPG 2009) in the form of the reference
 // intersect() (include/geometry/IntersectKernel.cuh:78-296):
 //   - ray precompute idir / ood with the 2^-80 clamp                (:92-101)
