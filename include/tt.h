@@ -447,7 +447,9 @@ tt_status tt_config_get(const tt_config *c, tt_config_info *info, tt_material *m
  * With train set, train_paths paths of every pass record their vertices (inputs,
  * sampled direction, density, the radiance later found along the path) and the
  * network takes Adam steps on batches of train_batch rows between passes, until
- * train_spp samples per pixel are done (0: the whole render). */
+ * train_spp samples per pixel are done (0: the whole render); the remaining samples
+ * then run in passes as wide as a render without training (paths_in_flight, if
+ * set, sizes the training passes only). */
 typedef struct tt_guide_params {
     float alpha;          /* probability of sampling the network on guided vertices */
     float min_roughness;  /* conductors at least this rough are guided */

@@ -103,6 +103,15 @@ def test_guided_render_unbiased_in_emissive_box(tt, ctx):
     _say(f"box, online-trained guide: mean {img.mean():.5f}, {gs}")
     assert gs["train_steps"] > 0
     assert abs(img.mean() - 1) < 0.01
+    # training during the first 64 of 256 spp in passes of 16384 paths, the rest frozen
+    # in one pass as wide as an untrained render (64 x 64 pixels x 192 spp)
+    g3 = tt.Guide(ctx, seed=4)
+    img, st, gs = g3.render(sc, cam, p, alpha=0.5, train=True, train_paths=4096, train_batch=8192, train_spp=64)
+    _say(f"box, trained for the first 64 spp: mean {img.mean():.5f}, {st}, {gs}")
+    assert gs["train_steps"] > 0
+    assert st["paths_in_flight"] == 64 * 64 * 256
+    assert abs(img.mean() - 1) < 0.01
+    g3.close()
     g.close()
     g2.close()
 

@@ -5,8 +5,9 @@ Equal-time RMSE on the full path-trace case (1,310,720-tri displaced icosphere,
 roulette from depth 4) against an FP64-accumulated oracle render of the plain
 tracer (8192 spp by default, a different seed): plain renders at 16..128 spp,
 guided renders trained online from scratch (Adam lr 1e-3, 2^18-row batches) at
-32..128 spp, and renders with the network trained by the 128-spp run and then
-frozen. Every point is averaged over independent seeds (MSE mean over `--seeds`
+32..128 spp, the same trained during the first half of the samples and frozen for the
+rest (rendered in wide passes), and renders with the network trained by the 128-spp
+run and then frozen. Every point is averaged over independent seeds (MSE mean over `--seeds`
 renders, each seed a fresh sample set): the error of this scene is carried by rare
 sun caustics through glass faces (most of it in 0.1% of the pixels), so one render's
 RMSE varies by tens of percent from seed to seed. The time at which guided rendering
@@ -65,7 +66,7 @@ def compare(tt, ctx, sc, cam, ref, depth, plain_spp, guided_spp, paths, lr, batc
     def rp(spp, seed, **kw):
         return tt.render_params(spp=spp, max_depth=depth, rr_depth=4, seed=seed, mode=tt.RENDER_PATH, **kw)
 
-    out = {"seeds": seeds, "plain": [], "guided_online": [], "guided_pretrained": []}
+    out = {"seeds": seeds, "plain": [], "guided_online": [], "guided_online_half": [], "guided_pretrained": []}
     for spp in plain_spp:
         runs = []
         for j in range(seeds):
@@ -87,6 +88,18 @@ def compare(tt, ctx, sc, cam, ref, depth, plain_spp, guided_spp, paths, lr, batc
         out["guided_online"].append(dict(point(spp, runs), train_seconds=float(np.mean([t["train_seconds"] for t in train])),
                                          train_steps=train[-1]["train_steps"], last_loss=train[-1]["last_loss"]))
         log(f"guided online {spp}: {out['guided_online'][-1]}")
+    # trained from scratch during the first half of the samples (small passes), the
+    # second half rendered with the network frozen in passes of the default width
+    for spp in guided_spp:
+        runs = []
+        for j in range(seeds):
+            gh = tt.Guide(ctx, seed=1 + j, lr=lr)
+            img, st, gs = gh.render(sc, cam, rp(spp, 4000 * spp + j, paths_in_flight=paths), alpha=0.5, train=True,
+                                    train_paths=train_paths, train_batch=batch, train_spp=spp // 2)
+            runs.append(err(img) + (st["seconds"],))
+            gh.close()
+        out["guided_online_half"].append(point(spp, runs))
+        log(f"guided online, first half trained {spp}: {out['guided_online_half'][-1]}")
     for spp in plain_spp:
         runs = []
         for j in range(seeds):
@@ -96,7 +109,7 @@ def compare(tt, ctx, sc, cam, ref, depth, plain_spp, guided_spp, paths, lr, batc
         log(f"guided pretrained {spp}: {out['guided_pretrained'][-1]}")
     g.close()
     p64 = next(r for r in out["plain"] if r["spp"] == 64)
-    for key in ("guided_online", "guided_pretrained"):
+    for key in ("guided_online", "guided_online_half", "guided_pretrained"):
         t = _time_to_rmse([(r["seconds"], r["rmse"]) for r in out[key]], p64["rmse"])
         out[f"{key}_time_to_plain64_rmse"] = t
         out[f"{key}_wall_ratio"] = t / p64["seconds"]

@@ -1692,13 +1692,21 @@ tt_status render_impl(tt_scene *s, int32_t nsc, const tt_camera *cams, const int
         k = std::min(k, p->spp);
         while (k > 1 && (int64_t)k * npix > (1LL << 27)) --k;
     }
-    st = ensure_state(s, npix * k, npix, fp64);
+    // a render that stops training part-way finishes with passes as wide as an
+    // untrained one (paths_in_flight then sizes the training passes only)
+    int32_t k_after = k;
+    if (gr && gr->train && gr->train_spp > 0 && gr->train_spp < p->spp) {
+        k_after = std::max(1, std::min(p->spp, (1 << 27) / npix));
+        while (k_after > 1 && (int64_t)k_after * npix > (1LL << 27)) --k_after;
+        k_after = std::max(k_after, k);
+    }
+    st = ensure_state(s, npix * k_after, npix, fp64);
     if (st != TT_OK) return st;
     RenderState *rs = s->rs;
     hipStream_t hs = ctx->stream;
     if (gr) {
         // inference rows: one per queued path; training rows: batch_rows
-        st = guide_alloc_infer(gr->g, (int64_t)npix * k);
+        st = guide_alloc_infer(gr->g, (int64_t)npix * k_after);
         if (st == TT_OK && gr->train) st = guide_alloc_batch(gr->g, gr->batch_rows);
         if (st == TT_OK && gr->train) {
             const int32_t want = std::max(1, gr->gd.ntrain);
@@ -1734,8 +1742,8 @@ tt_status render_impl(tt_scene *s, int32_t nsc, const tt_camera *cams, const int
         // guided: host-driven passes, training between them (not graph-captured)
         int32_t done = 0, pass = 0;
         while (e == hipSuccess && done < p->spp) {
-            const int32_t kk = std::min(k, p->spp - done);
             const bool train = gr->train && (gr->train_spp <= 0 || done < gr->train_spp);
+            const int32_t kk = std::min(train ? k : k_after, p->spp - done);
             gr->gd.rec = train ? gr->g->rec : nullptr;
             gr->gd.toff = gr->gd.stride > 1 ? (int32_t)(((int64_t)pass * 7919) % gr->gd.stride) : 0;
             e = launch_pass(s, rs, p, sv, npix, kk, fp64, hs, gr);
@@ -1810,7 +1818,7 @@ tt_status render_impl(tt_scene *s, int32_t nsc, const tt_camera *cams, const int
         stats->samples = (int64_t)npix * p->spp;
         stats->seconds = ms * 1e-3;
         stats->iterations = iters;
-        stats->paths_in_flight = npix * k;
+        stats->paths_in_flight = npix * k_after;
     }
     return TT_OK;
 }
