@@ -30,7 +30,10 @@ def main():
     ap.add_argument("--variant", type=int, default=1,
                     help="1: speculative while-while (headline); 0: exact per-lane order")
     ap.add_argument("--lds-stack", type=int, default=16)
-    ap.add_argument("--block", type=int, default=64)
+    ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--refill", type=int, default=32,
+                    help="persistent waves take new rays once this many lanes are idle (0: one wave per group)")
+    ap.add_argument("--reorder", type=int, default=1, help="1: rays traced in Morton order of their origins")
     ap.add_argument("--ntris", type=int, default=1_000_000)
     ap.add_argument("--cpu-sample-stride", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -57,7 +60,8 @@ def main():
 
     d_rays = tt.DeviceBuffer.from_array(ctx, rays)
     d_hits = tt.DeviceBuffer(ctx, n * 16)
-    cfg = tt.trace_cfg(variant=args.variant, lds_stack=args.lds_stack, block=args.block)
+    cfg = tt.trace_cfg(variant=args.variant, lds_stack=args.lds_stack, block=args.block, refill=args.refill,
+                       reorder=args.reorder)
 
     for _ in range(args.warmup):
         scene.trace_closest(d_rays, n, d_hits, cfg)
@@ -86,7 +90,8 @@ def main():
     # the other traversal variant timed beside the headline one (exact per-lane
     # order if the headline is speculative, and vice versa); its hits are compared
     other = 1 - args.variant
-    other_cfg = tt.trace_cfg(variant=other, lds_stack=args.lds_stack, block=args.block)
+    other_cfg = tt.trace_cfg(variant=other, lds_stack=args.lds_stack, block=args.block, refill=args.refill,
+                             reorder=args.reorder)
     d_hits2 = tt.DeviceBuffer(ctx, n * 16)
     scene.trace_closest(d_rays, n, d_hits2, other_cfg)
     other_ms = []
@@ -128,7 +133,7 @@ def main():
                         f"({n_prim} primary 1920x1080x2spp + {n - n_prim} incoherent), closest hit",
             "rays": n, "triangles": args.ntris, "tri_refs": info["num_tris"], "nodes": info["num_nodes"],
             "variant": "exact" if args.variant == 0 else "speculative", "lds_stack": args.lds_stack,
-            "block": args.block, "hit_fraction": round(hit_frac, 4),
+            "block": args.block, "refill": args.refill, "reorder": args.reorder, "hit_fraction": round(hit_frac, 4),
         },
         "kernel_ms": round(kmean, 4),
         "other_variant": spec,
@@ -156,15 +161,19 @@ def main():
         mism = int((gpu_ids != o_ids).sum())
         exact = bool(np.array_equal(hits[idx].view(np.int32), oh.view(np.int32)))
         achieved = bpr * n / (kmean * 1e-3) / 1e9
-        # traffic per launch of the default configuration (speculative, LDS 16, block
-        # 64), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of
-        # trenchanttracer_amd.profile_trace scaled by a 256 MiB copy's counts (reads x
-        # 1.99987, writes x 1.0): 11,385,820 KiB fetched, 129,600 KiB written (the hits)
-        default_cfg = (args.variant, args.lds_stack, args.block, args.ntris) == (1, 16, 64, 1_000_000)
-        traffic = round((11_385_819.7 * 1.99987 + 129_600.0) * 1024 / 1e9, 2) if default_cfg else None
+        # traffic per launch of the traversal kernel in the default configuration
+        # (speculative, LDS 16, block 256, refill 32, reorder 1), from rocprofv3 --pmc
+        # FETCH_SIZE and WRITE_SIZE runs of trenchanttracer_amd.profile_trace scaled by
+        # a 256 MiB copy's counts (reads x 1.99976, writes x 1.0): 6,597,741 KiB
+        # fetched, 214,109 KiB written (the hits, scattered to the rays' own indices)
+        default_cfg = (args.variant, args.lds_stack, args.block, args.refill, args.reorder,
+                       args.ntris) == (1, 16, 256, 32, 1, 1_000_000)
+        traffic = round((6_597_740.5 * 1.99976 + 214_108.7) * 1024 / 1e9, 2) if default_cfg else None
         result["roofline"] = {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
             "frac": round(achieved / 8000.0, 4), "traffic": traffic, "traffic_unit": "GB per launch",
+            "timed_span": "whole tt_trace_closest call (Morton keys + radix sort + traversal)" if args.reorder
+                          else "traversal launch",
             "bytes_per_ray": round(bpr, 1),
             "node_fetch_roofline_gbps": round(gather_gbps, 1),
             "node_fetch_frac": round(achieved / gather_gbps, 4),

@@ -273,13 +273,23 @@ tt_status tt_env_export_cdf(const tt_scene *s, float *marg, float *cond, double 
  * once at least `refill` of its 64 lanes are idle (keeps incoherent waves full; on
  * the 1M-soup traversal workload, half coherent rays, one wave per group is faster).
  * xcd_order: 0 = waves in launch order; 1 = blocks are renumbered so that each of the
- * 8 XCDs (which take blocks round-robin) holds one contiguous range of waves. */
+ * 8 XCDs (which take blocks round-robin) holds one contiguous range of waves.
+ * reorder: 0 (default) = rays are traced in the caller's order; 1 = the call first
+ * stable-sorts the rays by a 24-bit Morton code of their origins (8 bits per axis over
+ * the scene's mesh bounds), traces them in that order and writes each result at the
+ * ray's own index. Results are identical; waves of rays that start near each other
+ * share node fetches (on the 1M-soup traversal workload the 4.1M incoherent rays trace
+ * 18% faster; camera rays, which share one origin, keep their order). Needs n < 2^31
+ * and a context-owned workspace of 16 bytes per ray plus the sort's temporary storage,
+ * allocated on first use and grown when a larger n arrives; calls with reorder = 1 on
+ * different streams of one context must not overlap. */
 typedef struct tt_trace_cfg {
     int32_t variant;
     int32_t lds_stack;
     int32_t block;
     int32_t xcd_order;
     int32_t refill;
+    int32_t reorder;
 } tt_trace_cfg;
 
 /* Closest hit for n rays (replaces intersect(ray, bvh, true),

@@ -33,7 +33,7 @@ def test_struct_sizes(tt):
     assert ctypes.sizeof(N.tt_material) == 48
     # the same sizes the library asserts at compile time (api_host.cpp)
     assert ctypes.sizeof(N.tt_geometry) == 48
-    assert ctypes.sizeof(N.tt_trace_cfg) == 20
+    assert ctypes.sizeof(N.tt_trace_cfg) == 24
     assert ctypes.sizeof(N.tt_camera) == 60
     assert ctypes.sizeof(N.tt_render_params) == 72
     assert ctypes.sizeof(N.tt_render_stats) == 40

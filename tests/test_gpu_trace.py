@@ -106,6 +106,39 @@ def test_persistent_refill_matches_oracle(tt, oracle, ctx):
     s.close()
 
 
+def test_reorder_matches_oracle(tt, oracle, ctx):
+    """Rays traced in Morton order of their origins (reorder = 1) get, at their own
+    index, the record the unsorted launch gives them: bit-identical to the oracle for
+    the exact loop (one wave per group and persistent waves), identical to the unsorted
+    speculative launch, any hit equal to the oracle's; single rays and ragged counts,
+    a workspace that grows between calls, origins far outside the scene box."""
+    _say("trace reorder")
+    m = tt.Mesh.soup(40000, seed=12)
+    b, s = _scene(tt, ctx, m)
+    rays = _rays(tt, m, 97, 61, 30001, seed=12)          # 41,835 rays
+    rays = rays[np.random.default_rng(12).permutation(len(rays))]   # camera and incoherent rays mixed
+    rays[::7, 2] -= 1e6                                   # origins far outside the box (clamped cells)
+    cb = oracle.CompactBVH(b.export_compact())
+    for n in (1, 65, 4097, len(rays)):                    # the workspace grows with n
+        ohn, _ = cb.trace(rays[:n])
+        ids, t, u, v = s.intersect(rays[:n], tt.trace_cfg(0, 16, 64, reorder=1))
+        _check_exact(oracle, ohn, ids, t, u, v)
+    oh, _ = cb.trace(rays)
+    ids, t, u, v = s.intersect(rays[:999], tt.trace_cfg(0, 16, 128, refill=16, reorder=1))
+    _check_exact(oracle, oh[:999], ids, t, u, v)
+    ids, t, u, v = s.intersect(rays, tt.trace_cfg(0, 16, 128, refill=16, reorder=1))
+    _check_exact(oracle, oh, ids, t, u, v)
+    plain = s.intersect(rays, tt.trace_cfg(1, 16, 64))
+    sorted_ = s.intersect(rays, tt.trace_cfg(1, 16, 64, reorder=1))
+    for x, y in zip(plain, sorted_):
+        assert np.array_equal(x.view(np.int32), y.view(np.int32))
+    flags = s.occluded(rays, tt.trace_cfg(1, 16, 128, reorder=1))
+    assert np.array_equal(flags, (oracle.hit_ids(oh) >= 0).astype(np.int32))
+    with pytest.raises(tt.TTError):
+        s.intersect(rays[:10], tt.trace_cfg(0, 16, 64, reorder=2))
+    s.close()
+
+
 def test_ragged_and_empty_inputs(tt, oracle, ctx):
     _say("trace ragged/empty")
     m = tt.Mesh.soup(3000, seed=9)

@@ -181,8 +181,8 @@ def rays_incoherent(mesh, n, tmin=1e-4, tmax=1e20, seed=2):
     return out
 
 
-def trace_cfg(variant=0, lds_stack=16, block=128, xcd_order=0, refill=0):
-    return tt_trace_cfg(int(variant), int(lds_stack), int(block), int(xcd_order), int(refill))
+def trace_cfg(variant=0, lds_stack=16, block=128, xcd_order=0, refill=0, reorder=0):
+    return tt_trace_cfg(int(variant), int(lds_stack), int(block), int(xcd_order), int(refill), int(reorder))
 
 
 def material(type, albedo=(0.8, 0.8, 0.8), emission=(0, 0, 0), roughness=0.0, ior=1.5, specular=(1.0, 1.0, 1.0)):

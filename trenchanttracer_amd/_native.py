@@ -8,7 +8,7 @@ import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtt.so")
-ABI_VERSION = 3   # tt_abi_version() of the library this binding's structs describe
+ABI_VERSION = 4   # tt_abi_version() of the library this binding's structs describe
 
 TT_OK = 0
 STATUS_NAMES = {0: "TT_OK", 1: "TT_ERR_INVALID", 2: "TT_ERR_NOMEM", 3: "TT_ERR_DEVICE",
@@ -53,7 +53,7 @@ class tt_geometry(C.Structure):
 
 class tt_trace_cfg(C.Structure):
     _fields_ = [("variant", C.c_int32), ("lds_stack", C.c_int32), ("block", C.c_int32),
-                ("xcd_order", C.c_int32), ("refill", C.c_int32)]
+                ("xcd_order", C.c_int32), ("refill", C.c_int32), ("reorder", C.c_int32)]
 
 
 class tt_camera(C.Structure):

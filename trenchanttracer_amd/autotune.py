@@ -27,23 +27,25 @@ def _save(path, data):
         os.replace(tmp, path)
 
 
-def tune_trace(ctx, scene, d_rays, n, d_hits, variants=(0, 1), lds=(8, 16, 24), blocks=(64, 128), refills=(0, 16),
-               reps=3, exact_only=True, cache=None, key=None):
+def tune_trace(ctx, scene, d_rays, n, d_hits, variants=(0, 1), lds=(8, 16, 24), blocks=(64, 128, 256),
+               refills=(0, 16, 32), reorders=(0, 1), reps=3, exact_only=True, cache=None, key=None):
     """Fastest trace configuration for this scene and ray buffer: loop variant, LDS
-    stack depth, block size and launch form (refill 0: one wave per 64-ray group;
-    > 0: persistent waves that take new rays once that many lanes are idle). With
-    exact_only the speculative variant is not considered."""
+    stack depth, block size, launch form (refill 0: one wave per 64-ray group;
+    > 0: persistent waves that take new rays once that many lanes are idle) and ray
+    order (reorder 1: traced in Morton order of the origins). With exact_only the
+    speculative variant is not considered."""
     from . import Timer
     db = _load(cache)
     if key and key in db:
         c = db[key]
-        return trace_cfg(c["variant"], c["lds_stack"], c["block"], refill=c.get("refill", 0)), c["ms"]
+        return trace_cfg(c["variant"], c["lds_stack"], c["block"], refill=c.get("refill", 0),
+                         reorder=c.get("reorder", 0)), c["ms"]
     timer = Timer(ctx, 2)
     best = None
-    for v, s, b, r in itertools.product(variants, lds, blocks, refills):
+    for v, s, b, r, o in itertools.product(variants, lds, blocks, refills, reorders):
         if exact_only and v != 0:
             continue
-        cfg = trace_cfg(v, s, b, refill=r)
+        cfg = trace_cfg(v, s, b, refill=r, reorder=o)
         scene.trace_closest(d_rays, n, d_hits, cfg)
         ms = []
         for _ in range(reps):
@@ -53,12 +55,13 @@ def tune_trace(ctx, scene, d_rays, n, d_hits, variants=(0, 1), lds=(8, 16, 24), 
             ms.append(timer.elapsed(0, 1))
         t = sorted(ms)[len(ms) // 2]
         if best is None or t < best[0]:
-            best = (t, v, s, b, r)
+            best = (t, v, s, b, r, o)
     timer.close()
     if key:
-        db[key] = {"variant": best[1], "lds_stack": best[2], "block": best[3], "refill": best[4], "ms": best[0]}
+        db[key] = {"variant": best[1], "lds_stack": best[2], "block": best[3], "refill": best[4],
+                   "reorder": best[5], "ms": best[0]}
         _save(cache, db)
-    return trace_cfg(best[1], best[2], best[3], refill=best[4]), best[0]
+    return trace_cfg(best[1], best[2], best[3], refill=best[4], reorder=best[5]), best[0]
 
 
 def tune_render(render_fn, base, paths=(0, 16 << 20), lds=(16,), blocks=(64,), graphs=(0, 1),

@@ -12,7 +12,8 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtt.so")
 
 HOST_SRC = ["host/sbvh.cpp", "host/flatten.cpp", "host/gen.cpp", "host/api_host.cpp", "host/loaders.cpp"]
-DEV_SRC = ["device/trace.hip", "device/api_dev.hip", "device/env.hip", "device/render.hip", "device/guide.hip"]
+DEV_SRC = ["device/trace.hip", "device/api_dev.hip", "device/env.hip", "device/render.hip", "device/guide.hip",
+           "device/reorder.hip"]
 
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")

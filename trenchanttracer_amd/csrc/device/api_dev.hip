@@ -39,7 +39,9 @@ tt_status resolve_cfg(const tt_trace_cfg *cfg, tt_trace_cfg &out) {
     out.block = 128;
     out.xcd_order = 0;
     out.refill = 0;
+    out.reorder = 0;
     if (cfg) out = *cfg;
+    if (out.reorder < 0 || out.reorder > 1) return fail(TT_ERR_INVALID, "trace cfg: reorder must be 0 or 1");
     if (out.refill < 0 || out.refill > 64) return fail(TT_ERR_INVALID, "trace cfg: refill must be 0..64");
     if (out.variant < 0 || out.variant > 1) return fail(TT_ERR_INVALID, "trace cfg: variant must be 0 or 1");
     if (out.lds_stack != 8 && out.lds_stack != 16 && out.lds_stack != 24 && out.lds_stack != 32)
@@ -82,6 +84,7 @@ void tt_ctx_destroy(tt_ctx *ctx) {
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
     if (ctx->scratch) hipFree(ctx->scratch);
+    if (ctx->reorder_ws) hipFree(ctx->reorder_ws);
     hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -431,6 +434,23 @@ static tt_status trace_impl(tt_scene *s, const tt_ray *rays, int64_t n, tt_hit *
     a.num_nodes = (int32_t)s->num_nodes;
     hipStream_t hs = stream ? (hipStream_t)stream : s->ctx->stream;
     TT_HIP(hipSetDevice(s->ctx->device), "hipSetDevice");
+    if (c.reorder) {
+        if (n >= ((int64_t)1 << 31)) return fail(TT_ERR_INVALID, "trace: reorder supports at most 2^31 - 1 rays per call");
+        tt_ctx *ctx = s->ctx;
+        size_t need = 0, sort_bytes = 0;
+        TT_HIP(reorder_workspace_bytes(n, &need, &sort_bytes), "trace: reorder workspace size");
+        if (ctx->reorder_ws_bytes < need) {
+            // launches on any stream may still read the old workspace
+            TT_HIP(hipDeviceSynchronize(), "hipDeviceSynchronize");
+            if (ctx->reorder_ws) hipFree(ctx->reorder_ws);
+            ctx->reorder_ws = nullptr;
+            ctx->reorder_ws_bytes = 0;
+            st = dev_alloc(ctx, &ctx->reorder_ws, need, "trace: reorder workspace");
+            if (st != TT_OK) return st;
+            ctx->reorder_ws_bytes = need;
+        }
+        TT_HIP(reorder_rays(rays, n, s->bmin, s->bmax, ctx->reorder_ws, sort_bytes, hs, &a.order), "trace: reorder");
+    }
     TT_HIP(launch_trace(a, c.variant, any, c.lds_stack, c.block, c.xcd_order, c.refill, s->ctx->num_cus, hs), "trace launch");
     return TT_OK;
 }

@@ -25,11 +25,19 @@ struct TraceArgs {
     int32_t *reset1;
     int32_t xcd_order;     // 1: each XCD (blocks b = x mod 8) holds one contiguous range of waves
     int32_t refill;        // a wave takes new rays once this many of its lanes are idle (1..64)
+    const uint32_t *order; // if set: the r-th ray traced is ray order[r] (a permutation of 0..n-1)
 };
 
 // Triangle acceptance: the reference's test (front faces only, det > 1e-5) or
 // a two-sided test with no epsilon (accept any det != 0) for the path tracer.
 enum TriMode { kTriReference = 0, kTriTwoSided = 1 };
+
+// Ray reordering (reorder.hip): workspace size for n rays (sort_bytes: the part the
+// radix sort uses), and the permutation that sorts the rays by a Morton code of
+// their origins within the box lo..hi.
+hipError_t reorder_workspace_bytes(int64_t n, size_t *bytes, size_t *sort_bytes);
+hipError_t reorder_rays(const tt_ray *rays, int64_t n, const float lo[3], const float hi[3], void *ws,
+                        size_t sort_bytes, hipStream_t s, const uint32_t **order);
 
 // Launchers (trace.hip).
 hipError_t launch_trace(const TraceArgs &a, int variant, bool any, int lds_stack, int block, int xcd_order,

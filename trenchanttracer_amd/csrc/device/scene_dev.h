@@ -16,6 +16,8 @@ struct tt_ctx {
     hipStream_t stream = nullptr;
     void *scratch = nullptr;  // staging for the host-buffer wrappers
     size_t scratch_bytes = 0;
+    void *reorder_ws = nullptr;  // ray-reordering keys, permutation and sort storage
+    size_t reorder_ws_bytes = 0;
 };
 
 struct tt_scene {

@@ -156,7 +156,7 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
             const int64_t q = cursor + __popcll(m & below);
             if (q < total) {
                 const int64_t r = ((q >> 6) * waves + w) * 64 + (q & 63);
-                if (r < n) rid = r;
+                if (r < n) rid = a.order ? (int64_t)a.order[r] : r;
             }
             if (rid >= 0) {
                 const float4 *rp = reinterpret_cast<const float4 *>(a.rays + rid);

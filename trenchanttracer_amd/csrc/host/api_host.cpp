@@ -46,7 +46,7 @@ static_assert(sizeof(tt_ray) == 32, "tt_ray must be 32 bytes");
 static_assert(sizeof(tt_hit) == 16, "tt_hit must be 16 bytes");
 static_assert(sizeof(tt_material) == 48, "tt_material must be 48 bytes");
 static_assert(sizeof(tt_geometry) == 48, "tt_geometry must be 48 bytes");
-static_assert(sizeof(tt_trace_cfg) == 20, "tt_trace_cfg must be 20 bytes");
+static_assert(sizeof(tt_trace_cfg) == 24, "tt_trace_cfg must be 24 bytes");
 static_assert(sizeof(tt_camera) == 60, "tt_camera must be 60 bytes");
 static_assert(sizeof(tt_render_params) == 72, "tt_render_params must be 72 bytes");
 static_assert(sizeof(tt_render_stats) == 40, "tt_render_stats must be 40 bytes");
@@ -56,7 +56,7 @@ static_assert(sizeof(tt_guide_stats) == 32, "tt_guide_stats must be 32 bytes");
 extern "C" {
 
 const char *tt_last_error(void) { return tt::g_err.c_str(); }
-int tt_abi_version(void) { return 3; }   // 2: refill fields in tt_trace_cfg / tt_render_params; 3: max_bounce
+int tt_abi_version(void) { return 4; }   // 2: refill fields in tt_trace_cfg / tt_render_params; 3: max_bounce; 4: reorder
 
 tt_status tt_mesh_create(const float *verts, int32_t nverts, const int32_t *tris, int32_t ntris,
                          const float *normals, int32_t nnormals, const int32_t *tri_normals,

@@ -11,7 +11,7 @@ For profiler runs, one counter set per run:
 
     rocprofv3 --pmc FETCH_SIZE --output-format csv -d OUT -o run -- \\
         python3 -m trenchanttracer_amd.profile_trace [--case traversal|pathtrace|train] [--variant 1] [--lds-stack 16]
-            [--block 64] [--paths-in-flight N]
+            [--block 64] [--refill 0] [--reorder 0] [--paths-in-flight N]
 """
 import argparse
 import sys
@@ -22,6 +22,8 @@ def main():
     ap.add_argument("--variant", type=int, default=1)
     ap.add_argument("--lds-stack", type=int, default=16)
     ap.add_argument("--block", type=int, default=64)
+    ap.add_argument("--refill", type=int, default=0, help="traversal: refill threshold (0: one wave per group)")
+    ap.add_argument("--reorder", type=int, default=0, help="traversal: 1 = trace in Morton order of the origins")
     ap.add_argument("--case", choices=("traversal", "pathtrace", "train"), default="traversal")
     ap.add_argument("--paths-in-flight", type=int, default=0, help="pathtrace: paths per pass (0: the default)")
     args = ap.parse_args()
@@ -64,7 +66,8 @@ def main():
     scene = tt.DeviceScene(ctx, mesh, bvh)
     d_rays = tt.DeviceBuffer.from_array(ctx, rays)
     d_hits = tt.DeviceBuffer(ctx, n * 16)
-    cfg = tt.trace_cfg(variant=args.variant, lds_stack=args.lds_stack, block=args.block)
+    cfg = tt.trace_cfg(variant=args.variant, lds_stack=args.lds_stack, block=args.block, refill=args.refill,
+                       reorder=args.reorder)
     scene.trace_closest(d_rays, n, d_hits, cfg)
     ctx.sync()
     nbytes = 256 << 20
@@ -73,7 +76,8 @@ def main():
     ctx.sync()
     dst.copy_from(src)
     ctx.sync()
-    print(f"traced {n} rays (variant {args.variant}); copied {nbytes} bytes", file=sys.stderr, flush=True)
+    print(f"traced {n} rays (variant {args.variant}, refill {args.refill}, reorder {args.reorder}); "
+          f"copied {nbytes} bytes", file=sys.stderr, flush=True)
     for b in (src, dst, d_rays, d_hits):
         b.free()
     scene.close()
