@@ -282,7 +282,8 @@ tt_status tt_env_export_cdf(const tt_scene *s, float *marg, float *cond, double 
  * 18% faster; camera rays, which share one origin, keep their order). Needs n < 2^31
  * and a context-owned workspace of 16 bytes per ray plus the sort's temporary storage,
  * allocated on first use and grown when a larger n arrives; calls with reorder = 1 on
- * different streams of one context must not overlap. */
+ * different streams of one context must not overlap, and are not meant for HIP graph
+ * capture (the sort clears its state with memset operations). */
 typedef struct tt_trace_cfg {
     int32_t variant;
     int32_t lds_stack;

@@ -43,7 +43,7 @@ def test_batch_equals_individual_renders(tt, ctx):
     scenes.append((m1, mats1, cam1))
     m2 = tt.Mesh.soup(3000, seed=5, mu=float(np.log(0.05)))
     mats2 = [tt.material(tt.MAT_DIFFUSE, (0.7, 0.7, 0.7), emission=(0.5, 0.4, 0.3))]
-    cam2 = tt.camera((0.5, 0.5, -1.2), (0, 0, 1), fov=(50, 40), width=40, height=32)
+    cam2 = tt.camera((0.5, 0.5, -1.2), (0, 0, 1), fov=(50, 40), width=41, height=29)   # partial edge tiles
     scenes.append((m2, mats2, cam2))
     m3 = tt.Mesh.icosphere(4, 0.6, (0, 0, 0), amp=0.05, octaves=4, seed=2)
     mats3 = [tt.material(tt.MAT_DIFFUSE, (0.2, 0.6, 0.9), emission=(1, 1, 1))]

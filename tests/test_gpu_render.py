@@ -53,6 +53,18 @@ def test_reference_render_cornell_matches_oracle(tt, oracle, ctx):
     assert img.mean() > 0.01
 
 
+@pytest.mark.parametrize("w,h", [(37, 23), (2, 2), (9, 66), (130, 7)])
+def test_reference_render_ragged_sizes_match_oracle(tt, oracle, ctx, w, h):
+    """Paths walk the image in 8x8 tiles (narrower or shorter tiles at the right and
+    bottom edges) while each pixel keeps its own random numbers and accumulation slot:
+    at sizes that are not multiples of 8 the image equals the oracle's bit for bit."""
+    from trenchanttracer_amd import workloads
+    _say(f"reference render ragged {w}x{h}")
+    mesh, mats, cam = workloads.cornell(subdiv=2, width=w, height=h)
+    img, ref, st = _ref_case(tt, oracle, ctx, mesh, mats, cam, 4, 6, 3)
+    assert np.array_equal(img.view(np.int32), ref.view(np.int32))
+
+
 @pytest.mark.parametrize("paths", [0, 256 * 256, 3 * 256 * 256])
 def test_reference_render_any_pass_structure_matches_oracle(tt, oracle, ctx, paths):
     """Reference mode keeps each path's radiance in its slot and ends a pass by adding
