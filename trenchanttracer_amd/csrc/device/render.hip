@@ -272,7 +272,7 @@ struct SceneView {
     int32_t num_mats;
     // environment
     const float4 *env;
-    const float *env_marg, *env_cond;
+    const float2 *env_amarg, *env_acond;   // alias tables: rows, then columns of each row
     int32_t env_w, env_h;
     float env_scale, env_inv_total;
     // emitters
@@ -818,15 +818,29 @@ __device__ __forceinline__ int cdf_search(const float *cdf, int n, float u) {
     return lo;
 }
 
+// Walker's alias method, one draw from n entries {probability, alias}: slot k =
+// floor(u n) is kept with its probability (decided by the independent uniform d),
+// else its alias is taken. *frac returns u n - k, uniform in [0, 1) and independent
+// of the choice.
+__device__ __forceinline__ int alias_pick(const float2 *tab, int n, float u, float d, float *frac) {
+    const float x = u * (float)n;
+    const int k = min(n - 1, (int)x);
+    *frac = fminf(x - (float)k, 0x1.fffffep-1f);
+    const float2 e = tab[k];
+    return d < e.x ? k : __float_as_int(e.y);
+}
+
 __device__ f3 env_sample(const SceneView &sv, const float u[4], f3 *dir, float *pdf) {
-    // texel from the marginal / conditional CDFs, uniform (u, v) inside it; the
-    // density and radiance are those of the texel the direction looks up, so
-    // sampling and env_eval agree even where rounding moves a direction across a
-    // texel edge
-    int j = cdf_search(sv.env_marg, sv.env_h, u[0]);
-    int i = cdf_search(sv.env_cond + (int64_t)j * (sv.env_w + 1), sv.env_w, u[1]);
-    float theta = 3.14159265f * ((float)j + (1.0f - u[2])) / (float)sv.env_h;
-    float phi = 6.28318531f * ((float)i + (1.0f - u[3])) / (float)sv.env_w;
+    // texel by the alias tables (row, then column within the row: two dependent
+    // loads where binary searches of the marginal and conditional CDFs take 21),
+    // uniform (u, v) inside it; the density and radiance are those of the texel
+    // the direction looks up, so sampling and env_eval agree even where rounding
+    // moves a direction across a texel edge
+    float fj, fi;
+    const int j = alias_pick(sv.env_amarg, sv.env_h, u[0], u[2], &fj);
+    const int i = alias_pick(sv.env_acond + (int64_t)j * sv.env_w, sv.env_w, u[1], u[3], &fi);
+    float theta = 3.14159265f * ((float)j + (1.0f - fj)) / (float)sv.env_h;
+    float phi = 6.28318531f * ((float)i + (1.0f - fi)) / (float)sv.env_w;
     float ct, st = tm_sincos(theta, &ct);
     float cp, sp = tm_sincos(phi, &cp);
     *dir = F3(st * sp, ct, st * cp);
@@ -1607,8 +1621,8 @@ SceneView scene_view(const tt_scene *s, const tt_render_params *p) {
     sv.mats = s->mats;
     sv.num_mats = s->num_mats;
     sv.env = (const float4 *)s->env_rgb;
-    sv.env_marg = s->env_marg_cdf;
-    sv.env_cond = s->env_cond_cdf;
+    sv.env_amarg = s->env_alias_marg;
+    sv.env_acond = s->env_alias_cond;
     sv.env_w = s->env_w;
     sv.env_h = s->env_h;
     sv.env_scale = s->env_scale;
@@ -2097,8 +2111,8 @@ tt_status tt_debug_env_sample(tt_scene *s, int64_t n, uint32_t seed, float *out)
     TT_HIPR(hipSetDevice(ctx->device), "hipSetDevice");
     SceneView sv{};
     sv.env = (const float4 *)s->env_rgb;
-    sv.env_marg = s->env_marg_cdf;
-    sv.env_cond = s->env_cond_cdf;
+    sv.env_amarg = s->env_alias_marg;
+    sv.env_acond = s->env_alias_cond;
     sv.env_w = s->env_w;
     sv.env_h = s->env_h;
     sv.env_scale = s->env_scale;
