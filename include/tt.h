@@ -511,11 +511,11 @@ tt_status tt_debug_math(tt_ctx *ctx, int32_t op, const float *x, const float *y,
 tt_status tt_debug_bsdf_sample(tt_ctx *ctx, const tt_material *m, const float wo[3], int32_t front, int64_t n,
                                uint32_t seed, float *out, float *out_eval);
 tt_status tt_debug_env_sample(tt_scene *s, int64_t n, uint32_t seed, float *out);
-/* Host only (no device needed): the alias table the renderer samples the env map with,
- * for n weights (>= 0, FP64): out[2k] = probability of keeping entry k, out[2k + 1] =
- * its alias index (as a float). Entry k is drawn with probability
- * (out[2k] + sum over m with alias k of (1 - out[2m])) / n. */
-tt_status tt_debug_alias_table(const double *weights, int32_t n, float *out);
+/* The env texel (row, column) the sampler picks for given uniforms: u = n x 2 floats
+ * in [0, 1) (marginal, conditional), out = n x 2 int32 {row, column}. The inversion
+ * of the marginal and conditional CDFs (tt_env_export_cdf) through their guide
+ * tables: the largest index k with cdf[k] < u. Host buffers; synchronous. */
+tt_status tt_debug_env_invert(tt_scene *s, const float *u, int64_t n, int32_t *out);
 
 #ifdef __cplusplus
 } /* extern "C" */

@@ -603,6 +603,13 @@ class DeviceScene:
         arr = (N.tt_geometry * max(1, len(geoms)))(*geoms)
         check(lib.tt_scene_set_geometries(self._h, C.cast(arr, C.c_void_p), len(geoms)))
 
+    def env_invert(self, u):
+        """Env texels (row, column) the sampler picks for uniforms u: (n, 2) float32 in [0, 1)."""
+        u = np.ascontiguousarray(u, dtype=np.float32).reshape(-1, 2)
+        out = np.empty((len(u), 2), np.int32)
+        check(lib.tt_debug_env_invert(self._h, _vp(u), len(u), _vp(out)))
+        return out
+
     def env_samples(self, n, seed=1):
         out = np.empty((int(n), 8), np.float32)
         check(lib.tt_debug_env_sample(self._h, int(n), int(seed), _vp(out)))

@@ -169,7 +169,7 @@ PROTOS = {
     "tt_batch_render": (C.c_int, [P, P, C.POINTER(tt_render_params), P, C.POINTER(tt_render_stats)]),
     "tt_debug_bsdf_sample": (C.c_int, [P, C.POINTER(tt_material), PF, I32, I64, U32, P, P]),
     "tt_debug_env_sample": (C.c_int, [P, I64, U32, P]),
-    "tt_debug_alias_table": (C.c_int, [P, I32, P]),
+    "tt_debug_env_invert": (C.c_int, [P, P, I64, P]),
     "tt_trace_closest": (C.c_int, [P, P, I64, P, C.POINTER(tt_trace_cfg), P]),
     "tt_trace_any": (C.c_int, [P, P, I64, P, C.POINTER(tt_trace_cfg), P]),
     "tt_trace_closest_host": (C.c_int, [P, P, I64, P, C.POINTER(tt_trace_cfg)]),

@@ -379,7 +379,7 @@ void tt_scene_destroy(tt_scene *s) {
     free_render_state(s);
     void *ptrs[] = {s->nodes, s->tris, s->tri_normals, s->tri_mat, s->mats, s->tri_geo, s->emit_tri, s->emit_cdf,
                     s->emit_verts, s->env_rgb, s->env_marg_cdf, s->env_cond_cdf, s->env_pdf_scale, s->geoms,
-                    s->env_alias_marg, s->env_alias_cond};
+                    s->env_marg_guide, s->env_cond_guide};
     for (void *p : ptrs)
         if (p) hipFree(p);
     delete s;

@@ -39,6 +39,31 @@ hipError_t reorder_workspace_bytes(int64_t n, size_t *bytes, size_t *sort_bytes)
 hipError_t reorder_rays(const tt_ray *rays, int64_t n, const float lo[3], const float hi[3], void *ws,
                         size_t sort_bytes, hipStream_t s, const uint32_t **order);
 
+// Inversion of a float CDF (cdf[0] = 0, nondecreasing): the largest k in [lo, hi]
+// with cdf[k] < u, or lo when there is none. Over [0, n - 1] this is the sampled
+// entry: cdf[k] < u <= cdf[k + 1].
+__device__ __forceinline__ int cdf_search_range(const float *cdf, int lo, int hi, float u) {
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cdf[mid] < u) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Guide-table cell of u in [0, 1) for a table of m cells; the build and the
+// lookup use these same float operations.
+__device__ __forceinline__ int guide_cell(float u, int m) { return min(m - 1, max(0, (int)(u * (float)m))); }
+
+// CDF inversion through a guide table of n cells (Chen & Asau's index table):
+// guide[c] holds the smallest and largest index the full search returns for any
+// float u of cell c, so the search runs inside that range (usually 0-2 steps
+// instead of log2 n) and returns the full search's index for every u.
+__device__ __forceinline__ int cdf_search_guided(const float *cdf, const int2 *guide, int n, float u) {
+    const int2 r = guide[guide_cell(u, n)];
+    return cdf_search_range(cdf, r.x, r.y, u);
+}
+
 // Launchers (trace.hip).
 hipError_t launch_trace(const TraceArgs &a, int variant, bool any, int lds_stack, int block, int xcd_order,
                         int refill, int num_cus, hipStream_t s, int tri_mode = kTriReference);

@@ -1,18 +1,16 @@
-// Environment map upload and its importance-sampling distribution: per-texel
-// weight = luminance * sin(theta) (equal-area correction of the equirectangular
-// map). The GPU builds the conditional CDF per row and the marginal CDF over rows
-// from FP64 block prefix sums (stored normalised as float, exported by
-// tt_env_export_cdf); the host builds, from the same FP64 weights, the alias
-// tables the renderer samples with (Vose's construction of Walker's method: a row
-// from the row sums, then a column of that row, each one table load).
+// Environment map upload and its importance-sampling distribution, built on the
+// GPU: per-texel weight = luminance * sin(theta) (equal-area correction of the
+// equirectangular map), conditional CDF per row and marginal CDF over rows from
+// FP64 block prefix sums, stored normalised as float, and a guide table per CDF
+// (one cell per entry) that confines each inversion to a short index range.
 // Mapping (the reference's hdrSample, include/hdr/HDRKernel.cuh:16-27):
 //   u = atan2(d.x, d.z) / 2pi wrapped to [0, 1), v = acos(d.y) / pi.
-#include <cmath>
-#include <cstring>
+#include <algorithm>
 #include <string>
 #include <vector>
 
 #include "../host/tt_host.h"
+#include "dev_common.h"
 #include "scene_dev.h"
 
 namespace tt {
@@ -84,92 +82,38 @@ __global__ __launch_bounds__(kScanThreads) void k_env_marginal(const double *__r
     if (threadIdx.x == 0) total_out[0] = total;
 }
 
+// Smallest float u in (0, 1] whose cell (guide_cell's arithmetic, unclamped) is
+// >= c, for c >= 1: bisection over the float bit patterns, which order
+// nonnegative floats.
+__device__ float cell_start(int c, int m) {
+    uint32_t lo = 0u, hi = 0x3f800000u;   // cell(lo) < c <= cell(hi)
+    while (hi - lo > 1u) {
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        if ((int)(__uint_as_float(mid) * (float)m) >= c) hi = mid;
+        else lo = mid;
+    }
+    return __uint_as_float(hi);
+}
+
+// Guide tables of `rows` CDFs of n entries each (row r at cdf + r * cdf_stride):
+// cell c of a row gets the full search's index at the first and the last float u
+// that guide_cell maps to c (the search is monotone in u, so every u of the cell
+// lands between them).
+__global__ __launch_bounds__(256) void k_env_guide(const float *__restrict__ cdf, int64_t cdf_stride, int n, int rows,
+                                                   int2 *__restrict__ guide) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const float u_lo = c == 0 ? 0.0f : cell_start(c, n);
+    const float u_hi = c == n - 1 ? 0x1.fffffep-1f : __uint_as_float(__float_as_uint(cell_start(c + 1, n)) - 1u);
+    for (int r = blockIdx.y; r < rows; r += gridDim.y) {
+        const float *cd = cdf + (int64_t)r * cdf_stride;
+        const int lo = cdf_search_range(cd, 0, n - 1, u_lo);
+        const int hi = u_hi < u_lo ? lo : cdf_search_range(cd, 0, n - 1, u_hi);
+        guide[(int64_t)r * n + c] = make_int2(lo, hi);
+    }
+}
+
 }  // namespace
-
-// Alias table of n weights (all >= 0) into tab[k] = {probability of keeping k,
-// alias index as int bits}: Vose, "A linear algorithm for generating random
-// numbers with a given distribution" (1991). Entry k is drawn with probability
-// (prob[k] + sum of (1 - prob[m]) over the m aliased to k) / n = w[k] / sum(w).
-// All-zero weights give the uniform table.
-static void build_alias(const double *w, int n, float2 *tab, std::vector<double> &q, std::vector<int> &small,
-                        std::vector<int> &large) {
-    double sum = 0.0;
-    for (int k = 0; k < n; ++k) sum += w[k];
-    q.resize(n);
-    small.clear();
-    large.clear();
-    for (int k = 0; k < n; ++k) {
-        q[k] = sum > 0.0 ? w[k] * (double)n / sum : 1.0;
-        (q[k] < 1.0 ? small : large).push_back(k);
-    }
-    auto put = [&](int k, double p, int a) {
-        tab[k].x = (float)p;
-        std::memcpy(&tab[k].y, &a, sizeof(int));
-    };
-    while (!small.empty() && !large.empty()) {
-        const int l = small.back();
-        small.pop_back();
-        const int g = large.back();
-        put(l, q[l], g);
-        q[g] = (q[g] + q[l]) - 1.0;
-        if (q[g] < 1.0) {
-            large.pop_back();
-            small.push_back(g);
-        }
-    }
-    // what is left holds probability 1 up to rounding
-    for (int k : large) put(k, 1.0, k);
-    for (int k : small) put(k, 1.0, k);
-}
-
-static tt_status build_env_alias(tt_scene *s, const float *rgb, int32_t w, int32_t h) {
-    std::vector<float2> cond((size_t)w * h), marg(h);
-    std::vector<double> wt(w), rows(h), q;
-    std::vector<int> small, large;
-    for (int32_t row = 0; row < h; ++row) {
-        const double sin_t = std::sin(3.14159265358979323846 * ((double)row + 0.5) / (double)h);
-        double sum = 0.0;
-        for (int32_t i = 0; i < w; ++i) {
-            const float *p = rgb + 3 * ((size_t)row * w + i);
-            const double lum = 0.2126 * (double)p[0] + 0.7152 * (double)p[1] + 0.0722 * (double)p[2];
-            wt[i] = lum > 0.0 ? lum * sin_t : 0.0;
-            sum += wt[i];
-        }
-        rows[row] = sum;
-        build_alias(wt.data(), w, cond.data() + (size_t)row * w, q, small, large);
-    }
-    build_alias(rows.data(), h, marg.data(), q, small, large);
-    tt_status st = dev_alloc(s->ctx, (void **)&s->env_alias_cond, cond.size() * sizeof(float2), "env alias rows");
-    if (st == TT_OK) st = dev_alloc(s->ctx, (void **)&s->env_alias_marg, marg.size() * sizeof(float2), "env alias marginal");
-    if (st != TT_OK) return st;
-    hipError_t e = hipMemcpy(s->env_alias_cond, cond.data(), cond.size() * sizeof(float2), hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = hipMemcpy(s->env_alias_marg, marg.data(), marg.size() * sizeof(float2), hipMemcpyHostToDevice);
-    if (e != hipSuccess) return fail(TT_ERR_DEVICE, std::string("env alias upload: ") + hipGetErrorString(e));
-    s->bytes += (cond.size() + marg.size()) * sizeof(float2);
-    return TT_OK;
-}
-
-}  // namespace tt
-
-extern "C" tt_status tt_debug_alias_table(const double *weights, int32_t n, float *out) {
-    if (!weights || !out || n <= 0) return tt::fail(TT_ERR_INVALID, "tt_debug_alias_table: bad arguments");
-    for (int32_t k = 0; k < n; ++k)
-        if (!(weights[k] >= 0.0)) return tt::fail(TT_ERR_INVALID, "tt_debug_alias_table: weights must be >= 0");
-    std::vector<float2> tab(n);
-    std::vector<double> q;
-    std::vector<int> small, large;
-    tt::build_alias(weights, n, tab.data(), q, small, large);
-    for (int32_t k = 0; k < n; ++k) {
-        int a;
-        std::memcpy(&a, &tab[k].y, sizeof(int));
-        out[2 * k] = tab[k].x;
-        out[2 * k + 1] = (float)a;
-    }
-    return TT_OK;
-}
-
-namespace tt {
 
 tt_status build_env(tt_scene *s, const float *rgb, int32_t w, int32_t h, float scale) {
     tt_ctx *ctx = s->ctx;
@@ -180,6 +124,8 @@ tt_status build_env(tt_scene *s, const float *rgb, int32_t w, int32_t h, float s
     if (st == TT_OK) st = dev_alloc(ctx, (void **)&s->env_cond_cdf, (size_t)h * (w + 1) * sizeof(float), "env conditional cdf");
     if (st == TT_OK) st = dev_alloc(ctx, (void **)&s->env_marg_cdf, (size_t)(h + 1) * sizeof(float), "env marginal cdf");
     if (st == TT_OK) st = dev_alloc(ctx, (void **)&s->env_pdf_scale, sizeof(double), "env total");
+    if (st == TT_OK) st = dev_alloc(ctx, (void **)&s->env_cond_guide, (size_t)h * w * sizeof(int2), "env conditional guide");
+    if (st == TT_OK) st = dev_alloc(ctx, (void **)&s->env_marg_guide, (size_t)h * sizeof(int2), "env marginal guide");
     hipError_t e = hipSuccess;
     if (st == TT_OK) e = hipMemcpy(d_rgb, rgb, (size_t)w * h * 3 * sizeof(float), hipMemcpyHostToDevice);
     if (st == TT_OK && e == hipSuccess) {
@@ -192,6 +138,16 @@ tt_status build_env(tt_scene *s, const float *rgb, int32_t w, int32_t h, float s
                            s->env_marg_cdf, s->env_pdf_scale, h);
         e = hipGetLastError();
     }
+    if (st == TT_OK && e == hipSuccess) {
+        hipLaunchKernelGGL(k_env_guide, dim3((unsigned)((w + 255) / 256), (unsigned)std::min(h, 65535)), dim3(256), 0,
+                           ctx->stream, (const float *)s->env_cond_cdf, (int64_t)(w + 1), w, h, s->env_cond_guide);
+        e = hipGetLastError();
+    }
+    if (st == TT_OK && e == hipSuccess) {
+        hipLaunchKernelGGL(k_env_guide, dim3((unsigned)((h + 255) / 256), 1), dim3(256), 0, ctx->stream,
+                           (const float *)s->env_marg_cdf, (int64_t)(h + 1), h, 1, s->env_marg_guide);
+        e = hipGetLastError();
+    }
     double total = 0.0;
     if (st == TT_OK && e == hipSuccess)
         e = hipMemcpyAsync(&total, s->env_pdf_scale, sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
@@ -200,13 +156,12 @@ tt_status build_env(tt_scene *s, const float *rgb, int32_t w, int32_t h, float s
     if (d_rows) hipFree(d_rows);
     if (st != TT_OK) return st;
     if (e != hipSuccess) return fail(TT_ERR_DEVICE, std::string("env build: ") + hipGetErrorString(e));
-    st = build_env_alias(s, rgb, w, h);
-    if (st != TT_OK) return st;
     s->env_w = w;
     s->env_h = h;
     s->env_scale = scale;
     s->env_total = total;
-    s->bytes += (size_t)w * h * sizeof(float4) + (size_t)h * (w + 1) * 4 + (size_t)(h + 1) * 4 + 8;
+    s->bytes += (size_t)w * h * sizeof(float4) + (size_t)h * (w + 1) * 4 + (size_t)(h + 1) * 4 + 8 +
+                (size_t)(h * (int64_t)w + h) * sizeof(int2);
     return TT_OK;
 }
 

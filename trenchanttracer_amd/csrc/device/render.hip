@@ -272,7 +272,8 @@ struct SceneView {
     int32_t num_mats;
     // environment
     const float4 *env;
-    const float2 *env_amarg, *env_acond;   // alias tables: rows, then columns of each row
+    const float *env_marg, *env_cond;
+    const int2 *env_gmarg, *env_gcond;   // guide tables of the two CDFs
     int32_t env_w, env_h;
     float env_scale, env_inv_total;
     // emitters
@@ -808,39 +809,27 @@ __device__ f3 env_eval(const SceneView &sv, f3 d, float *pdf) {
 }
 
 // Largest index k in [0, n) with cdf[k] < u (cdf[0] = 0 < u).
-__device__ __forceinline__ int cdf_search(const float *cdf, int n, float u) {
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (cdf[mid] < u) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
+__device__ __forceinline__ int cdf_search(const float *cdf, int n, float u) { return cdf_search_range(cdf, 0, n - 1, u); }
 
-// Walker's alias method, one draw from n entries {probability, alias}: slot k =
-// floor(u n) is kept with its probability (decided by the independent uniform d),
-// else its alias is taken. *frac returns u n - k, uniform in [0, 1) and independent
-// of the choice.
-__device__ __forceinline__ int alias_pick(const float2 *tab, int n, float u, float d, float *frac) {
-    const float x = u * (float)n;
-    const int k = min(n - 1, (int)x);
-    *frac = fminf(x - (float)k, 0x1.fffffep-1f);
-    const float2 e = tab[k];
-    return d < e.x ? k : __float_as_int(e.y);
+// Texel (row j, column i) the env distribution gives for (u0, u1): inversion of the
+// marginal CDF, then of row j's conditional CDF, each through its guide table (a
+// few dependent loads where the two binary searches take 21 at 2048 x 1024; the
+// same texel for every u).
+__device__ __forceinline__ void env_texel(const SceneView &sv, float u0, float u1, int *j, int *i) {
+    *j = cdf_search_guided(sv.env_marg, sv.env_gmarg, sv.env_h, u0);
+    *i = cdf_search_guided(sv.env_cond + (int64_t)*j * (sv.env_w + 1), sv.env_gcond + (int64_t)*j * sv.env_w,
+                           sv.env_w, u1);
 }
 
 __device__ f3 env_sample(const SceneView &sv, const float u[4], f3 *dir, float *pdf) {
-    // texel by the alias tables (row, then column within the row: two dependent
-    // loads where binary searches of the marginal and conditional CDFs take 21),
-    // uniform (u, v) inside it; the density and radiance are those of the texel
-    // the direction looks up, so sampling and env_eval agree even where rounding
-    // moves a direction across a texel edge
-    float fj, fi;
-    const int j = alias_pick(sv.env_amarg, sv.env_h, u[0], u[2], &fj);
-    const int i = alias_pick(sv.env_acond + (int64_t)j * sv.env_w, sv.env_w, u[1], u[3], &fi);
-    float theta = 3.14159265f * ((float)j + (1.0f - fj)) / (float)sv.env_h;
-    float phi = 6.28318531f * ((float)i + (1.0f - fi)) / (float)sv.env_w;
+    // texel from the marginal / conditional CDFs, uniform (u, v) inside it; the
+    // density and radiance are those of the texel the direction looks up, so
+    // sampling and env_eval agree even where rounding moves a direction across a
+    // texel edge
+    int j, i;
+    env_texel(sv, u[0], u[1], &j, &i);
+    float theta = 3.14159265f * ((float)j + (1.0f - u[2])) / (float)sv.env_h;
+    float phi = 6.28318531f * ((float)i + (1.0f - u[3])) / (float)sv.env_w;
     float ct, st = tm_sincos(theta, &ct);
     float cp, sp = tm_sincos(phi, &cp);
     *dir = F3(st * sp, ct, st * cp);
@@ -1320,6 +1309,15 @@ __global__ void k_debug_bsdf(tt_material m, f3 wo, bool front, int64_t n, uint32
     }
 }
 
+__global__ void k_debug_env_invert(SceneView sv, int64_t n, const float *u, int32_t *out) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int j, i;
+    env_texel(sv, u[2 * k], u[2 * k + 1], &j, &i);
+    out[2 * k] = j;
+    out[2 * k + 1] = i;
+}
+
 __global__ void k_debug_env(SceneView sv, int64_t n, uint32_t seed, float *out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -1621,8 +1619,10 @@ SceneView scene_view(const tt_scene *s, const tt_render_params *p) {
     sv.mats = s->mats;
     sv.num_mats = s->num_mats;
     sv.env = (const float4 *)s->env_rgb;
-    sv.env_amarg = s->env_alias_marg;
-    sv.env_acond = s->env_alias_cond;
+    sv.env_marg = s->env_marg_cdf;
+    sv.env_cond = s->env_cond_cdf;
+    sv.env_gmarg = s->env_marg_guide;
+    sv.env_gcond = s->env_cond_guide;
     sv.env_w = s->env_w;
     sv.env_h = s->env_h;
     sv.env_scale = s->env_scale;
@@ -2111,8 +2111,10 @@ tt_status tt_debug_env_sample(tt_scene *s, int64_t n, uint32_t seed, float *out)
     TT_HIPR(hipSetDevice(ctx->device), "hipSetDevice");
     SceneView sv{};
     sv.env = (const float4 *)s->env_rgb;
-    sv.env_amarg = s->env_alias_marg;
-    sv.env_acond = s->env_alias_cond;
+    sv.env_marg = s->env_marg_cdf;
+    sv.env_cond = s->env_cond_cdf;
+    sv.env_gmarg = s->env_marg_guide;
+    sv.env_gcond = s->env_cond_guide;
     sv.env_w = s->env_w;
     sv.env_h = s->env_h;
     sv.env_scale = s->env_scale;
@@ -2126,6 +2128,38 @@ tt_status tt_debug_env_sample(tt_scene *s, int64_t n, uint32_t seed, float *out)
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     hipFree(d_out);
     if (e != hipSuccess) return fail(TT_ERR_DEVICE, std::string("tt_debug_env_sample: ") + hipGetErrorString(e));
+    return TT_OK;
+}
+
+tt_status tt_debug_env_invert(tt_scene *s, const float *u, int64_t n, int32_t *out) {
+    if (!s || !u || !out || n <= 0 || n > (1LL << 26)) return fail(TT_ERR_INVALID, "tt_debug_env_invert: bad arguments");
+    if (!s->env_rgb || s->env_total <= 0.0) return fail(TT_ERR_INVALID, "tt_debug_env_invert: scene has no env map");
+    for (int64_t k = 0; k < 2 * n; ++k)
+        if (!(u[k] >= 0.0f && u[k] < 1.0f)) return fail(TT_ERR_INVALID, "tt_debug_env_invert: u must be in [0, 1)");
+    tt_ctx *ctx = s->ctx;
+    TT_HIPR(hipSetDevice(ctx->device), "hipSetDevice");
+    SceneView sv{};
+    sv.env_marg = s->env_marg_cdf;
+    sv.env_cond = s->env_cond_cdf;
+    sv.env_gmarg = s->env_marg_guide;
+    sv.env_gcond = s->env_cond_guide;
+    sv.env_w = s->env_w;
+    sv.env_h = s->env_h;
+    void *d = nullptr;
+    tt_status st = dev_alloc(ctx, &d, (size_t)n * 16, "debug env invert");
+    if (st != TT_OK) return st;
+    float *d_u = (float *)d;
+    int32_t *d_out = (int32_t *)((char *)d + (size_t)n * 8);
+    hipError_t e = hipMemcpyAsync(d_u, u, (size_t)n * 8, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_debug_env_invert, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, sv, n,
+                           (const float *)d_u, d_out);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    hipFree(d);
+    if (e != hipSuccess) return fail(TT_ERR_DEVICE, std::string("tt_debug_env_invert: ") + hipGetErrorString(e));
     return TT_OK;
 }
 

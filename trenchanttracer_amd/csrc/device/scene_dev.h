@@ -44,8 +44,8 @@ struct tt_scene {
     float *env_rgb = nullptr;        // float4 per texel (rgb, luminance weight)
     float *env_marg_cdf = nullptr;   // H + 1 floats
     float *env_cond_cdf = nullptr;   // H * (W + 1) floats
-    float2 *env_alias_marg = nullptr;   // H entries {probability, alias row}: sampling of rows
-    float2 *env_alias_cond = nullptr;   // H * W entries {probability, alias column}: per row
+    int2 *env_marg_guide = nullptr;  // H cells: index range of the marginal CDF search
+    int2 *env_cond_guide = nullptr;  // H * W cells: the same for each row's conditional CDF
     double *env_pdf_scale = nullptr; // [0] = total weight (FP64)
     int32_t env_w = 0, env_h = 0;
     float env_scale = 1.0f;
