@@ -44,3 +44,40 @@ def test_env_probs_constant_map_uniform_solid_angle(oracle):
     th = np.pi * (np.arange(h) + 0.5) / h
     dens = p * w * h / (2 * np.pi ** 2 * np.sin(th)[:, None])   # solid-angle density
     assert np.allclose(dens, dens.mean(), rtol=0.02) and abs(dens.mean() - 1 / (4 * np.pi)) < 2e-3
+
+
+@pytest.mark.parametrize("rough", [0.1, 0.4, 0.9])
+def test_conductor_reciprocity_energy_and_pdf(oracle, rough):
+    """The FP64 GGX conductor the GPU sampler is checked against: f(wo, wi) = f(wi, wo)
+    (reciprocity), hemispherical albedo <= 1 and close to 1 for a smooth white
+    conductor (single scattering loses energy only at high roughness), and the VNDF
+    density integrating to at most 1 (the part below the horizon is lost)."""
+    from oracle import bsdf_ref as B
+    rng = np.random.default_rng(11)
+
+    def hemi(k):
+        v = rng.normal(size=(k, 3))
+        v /= np.linalg.norm(v, axis=1, keepdims=True)
+        v[:, 2] = np.abs(v[:, 2]) + 1e-3
+        return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+    f0 = (0.9, 0.7, 0.4)
+    a, b = hemi(300), hemi(300)
+    for wo, wi in zip(a, b):
+        f_ab = B.conductor_eval(wo, wi[None], rough, f0)[0][0] / wi[2]
+        f_ba = B.conductor_eval(wi, wo[None], rough, f0)[0][0] / wo[2]
+        assert np.allclose(f_ab, f_ba, rtol=1e-12, atol=0)
+    # albedo and density integrals by uniform hemisphere sampling (pdf 1 / 2 pi)
+    n = 400_000
+    wi = hemi(n)
+    for theta in (0.0, 45.0, 75.0):
+        t = np.radians(theta)
+        wo = np.array([np.sin(t), 0.0, np.cos(t)])
+        fcos, pdf = B.conductor_eval(wo, wi, rough, (1.0, 1.0, 1.0))
+        est = 2 * np.pi * fcos[:, 0]
+        albedo, se = est.mean(), est.std() / np.sqrt(n)
+        assert albedo <= 1 + 4 * se
+        if rough <= 0.1 and theta <= 45:
+            assert albedo > 0.97
+        dens = 2 * np.pi * pdf
+        assert dens.mean() <= 1 + 4 * dens.std() / np.sqrt(n)
