@@ -155,9 +155,9 @@ struct CamArgs {
 // Up to kMaxScenes scenes rendered in one wavefront: per scene its pixel range
 // in the concatenated image, resolution, root node and camera.
 constexpr int kMaxScenes = 64;
-// Path slots walk 8x8 pixel tiles: on the 1080p path-trace case (64 spp, depth 16)
-// the render takes 0.1605 s against 0.1650 s in pixel-row order (4x4: 0.1606,
-// 16x16: 0.1607), with bit-identical images.
+// Path slots walk 8x8 pixel tiles: on the 1080p path-trace case (64 spp, depth 16,
+// sample-major path ids) the render took 0.1605 s against 0.1650 s in pixel-row
+// order (4x4: 0.1606, 16x16: 0.1607), with bit-identical images.
 constexpr int kSlotTile = 8;
 struct SceneTable {
     int32_t n;
@@ -177,13 +177,13 @@ __device__ __forceinline__ int scene_of(const SceneTable *t, int32_t pix) {
     return lo;
 }
 
-// Pixel rendered by path slot q (a path's index within its sample, 0..npix-1):
-// scene sid, local pixel index Y * W + X within that scene. Slots walk bands of
-// `tile` rows, and each band tile by tile (tile x band-height, the last tile of a
-// band narrower), so the 64 paths of a wave cover one 8x8 block of the image and
-// their later rays start close together. The pixel's random numbers and its
-// accumulation slot follow the pixel, not the slot, so images do not depend on
-// the order.
+// Pixel rendered by path slot q (0..npix-1; the slot's k samples are consecutive
+// path ids): scene sid, local pixel index Y * W + X within that scene. Slots walk
+// bands of `tile` rows, and each band tile by tile (tile x band-height, the last
+// tile of a band narrower), so consecutive slots are neighbouring pixels of one
+// 8x8 block (a wave spans several pixels when k < 64) and their later rays start
+// close together. The pixel's random numbers and its accumulation slot follow the
+// pixel, not the slot, so images do not depend on the order.
 __device__ __forceinline__ int32_t pixel_of_slot(const SceneTable *t, int32_t q, int *sid_out, int32_t *x_out,
                                                  int32_t *y_out) {
     const int sid = scene_of(t, q);
@@ -209,9 +209,19 @@ __device__ __forceinline__ int32_t pixel_of_slot(const SceneTable *t, int32_t q,
     return Y * W + X;
 }
 
+// Path ids of a pass of k samples per pixel are pixel-major: path p = q * k + s is
+// sample s of slot q. The k samples of one pixel are neighbours in the queues, so a
+// wave's camera rays pass through one pixel (or a few, when k < 64) and its later
+// rays start from nearly one point: on the 1080p path-trace case (64 spp in one pass)
+// the render takes 0.1450 s against 0.1566 s with sample-major ids (s * npix + q),
+// with bit-identical images.
+__device__ __forceinline__ int32_t path_slot(int32_t p, int32_t k) { return p / k; }
+__device__ __forceinline__ int32_t path_sample(int32_t p, int32_t k) { return p % k; }
+__device__ __forceinline__ int64_t path_id(int32_t q, int32_t s, int32_t k) { return (int64_t)q * k + s; }
+
 struct GenArgs {
     const SceneTable *tab;
-    int32_t npix, mode, k;   // k samples per pixel in flight: path id = j * npix + slot (pixel_of_slot)
+    int32_t npix, mode, k;   // k samples per pixel in flight: path id = slot * k + j (pixel_of_slot)
     const int32_t *sample_dev;
     uint32_t seed;
     float tmin;
@@ -233,10 +243,10 @@ __global__ __launch_bounds__(256) void k_generate(GenArgs a) {
     if (p >= npix * a.k) return;
     int sid;
     int32_t X, Y;
-    const int32_t local = pixel_of_slot(a.tab, p % npix, &sid, &X, &Y);
+    const int32_t local = pixel_of_slot(a.tab, path_slot(p, a.k), &sid, &X, &Y);
     const int32_t W = a.tab->w[sid], H = a.tab->h[sid];
     float u[4];
-    draws((uint32_t)local, (uint32_t)*a.sample_dev + (uint32_t)(p / npix), 0u, 0u, a.seed, u);
+    draws((uint32_t)local, (uint32_t)*a.sample_dev + (uint32_t)path_sample(p, a.k), 0u, 0u, a.seed, u);
     // camera ray, RenderKernel.cu:152-197
     const float jx = u[0] - 0.5f, jy = u[1] - 0.5f;
     const float sx = (jx + (float)X) / (float)(W - 1);
@@ -437,7 +447,8 @@ struct ShadeArgs {
     double *acc64;
     unsigned long long *stats;
     int32_t bounce, max_depth, rr_depth;
-    int32_t npix;     // path id = j * npix + slot (pixel_of_slot), sample = *sample_dev + j
+    int32_t npix;     // pixel slots
+    int32_t k;        // samples per pixel in this pass: path id = slot * k + j, sample = *sample_dev + j
     uint32_t seed;
     float inv_spp, pi, two_pi;
 };
@@ -468,8 +479,8 @@ __device__ __forceinline__ void shade_reference_one(const ShadeArgs &a, const in
         root = a.root_in[i];
         int sid_;
         int32_t x_, y_;
-        const int32_t local = pixel_of_slot(a.tab, p % a.npix, &sid_, &x_, &y_);
-        const uint32_t smp = (uint32_t)*a.sample_dev + (uint32_t)(p / a.npix);
+        const int32_t local = pixel_of_slot(a.tab, path_slot(p, a.k), &sid_, &x_, &y_);
+        const uint32_t smp = (uint32_t)*a.sample_dev + (uint32_t)path_sample(p, a.k);
         const tt_ray r = a.rays_in[i];
         const tt_hit h = a.hits[i];
         f3 mask = xyz(a.thr[p]);
@@ -925,8 +936,8 @@ __device__ __forceinline__ void shade_path_one(const ShadeArgs &a, const int32_t
         root = a.root_in[i];
         int sid_;
         int32_t x_, y_;
-        const int32_t local = pixel_of_slot(a.tab, p % a.npix, &sid_, &x_, &y_);
-        const uint32_t smp = (uint32_t)*a.sample_dev + (uint32_t)(p / a.npix);
+        const int32_t local = pixel_of_slot(a.tab, path_slot(p, a.k), &sid_, &x_, &y_);
+        const uint32_t smp = (uint32_t)*a.sample_dev + (uint32_t)path_sample(p, a.k);
         const tt_ray r = a.rays_in[i];
         const tt_hit h = a.hits[i];
         const float4 tp = a.thr[p];
@@ -1166,7 +1177,7 @@ __global__ __launch_bounds__(256) void k_apply_shadow(ApplyArgs a) {
 }
 
 // End of a path-mode pass: each pixel's radiance is the sum of its kk paths'
-// radiance (paths s * npix + q for the pixel's slot q), added in sample order to the
+// radiance (paths q * kk + s for the pixel's slot q), added in sample order to the
 // float or FP64 accumulation, so images do not depend on the order threads finish.
 // Reference mode: acc += L * (1 / spp) for the pass's kk samples of each pixel in
 // sample order, one multiply and one add per sample and channel as the reference
@@ -1179,52 +1190,93 @@ __device__ __forceinline__ int32_t slot_pixel(const SceneTable *t, int32_t q) {
     return t->pix_off[sid] + local;
 }
 
+// A block's 256 slots have their samples side by side (path ids q * kk + s); the
+// block stages them in LDS kResolveChunk samples at a time, each pixel's chunk read
+// as one 128-byte segment (a thread per slot reading its own samples would put the
+// wave's 64 loads 16 * kk bytes apart). Rows are padded to kResolveChunk + 1 float4
+// so that both the staging writes and each thread's row reads hit distinct banks.
+constexpr int kResolveChunk = 8;
+struct ResolveTile {
+    float4 v[256][kResolveChunk + 1];
+};
+
+__device__ __forceinline__ void stage_chunk(ResolveTile &t, const float4 *L, int32_t q0, int32_t npix, int32_t kk,
+                                            int32_t s0) {
+    for (int i = threadIdx.x; i < 256 * kResolveChunk; i += 256) {
+        const int pl = i / kResolveChunk, j = i - pl * kResolveChunk;
+        const int32_t q = q0 + pl, s = s0 + j;
+        if (q < npix && s < kk) t.v[pl][j] = L[path_id(q, s, kk)];
+    }
+}
+
 __global__ __launch_bounds__(256) void k_resolve_reference(const SceneTable *tab, const float4 *L, int32_t npix,
                                                            int32_t kk, float inv_spp, float *acc) {
-    const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= npix) return;
-    float *o = acc + 3 * (int64_t)slot_pixel(tab, q);
-    float r = o[0], g = o[1], b = o[2];
-    for (int32_t s = 0; s < kk; ++s) {
-        const float4 v = L[(int64_t)s * npix + q];
-        r = r + v.x * inv_spp;
-        g = g + v.y * inv_spp;
-        b = b + v.z * inv_spp;
+    __shared__ ResolveTile t;
+    const int32_t q0 = blockIdx.x * 256, q = q0 + threadIdx.x;
+    const bool live = q < npix;
+    float *o = acc + 3 * (int64_t)(live ? slot_pixel(tab, q) : 0);
+    float r = 0.0f, g = 0.0f, b = 0.0f;
+    if (live) {
+        r = o[0];
+        g = o[1];
+        b = o[2];
     }
-    o[0] = r;
-    o[1] = g;
-    o[2] = b;
+    for (int32_t s0 = 0; s0 < kk; s0 += kResolveChunk) {
+        __syncthreads();
+        stage_chunk(t, L, q0, npix, kk, s0);
+        __syncthreads();
+        const int32_t m = min(kResolveChunk, kk - s0);
+        for (int j = 0; j < m; ++j) {
+            const float4 v = t.v[threadIdx.x][j];
+            r = r + v.x * inv_spp;
+            g = g + v.y * inv_spp;
+            b = b + v.z * inv_spp;
+        }
+    }
+    if (live) {
+        o[0] = r;
+        o[1] = g;
+        o[2] = b;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_resolve(const SceneTable *tab, const float4 *L, int32_t npix, int32_t kk,
                                                  float *acc, double *acc64) {
-    const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= npix) return;
+    __shared__ ResolveTile t;
+    const int32_t q0 = blockIdx.x * 256, q = q0 + threadIdx.x;
+    const bool live = q < npix;
+    double rd = 0.0, gd = 0.0, bd = 0.0;
+    float rf = 0.0f, gf = 0.0f, bf = 0.0f;
+    for (int32_t s0 = 0; s0 < kk; s0 += kResolveChunk) {
+        __syncthreads();
+        stage_chunk(t, L, q0, npix, kk, s0);
+        __syncthreads();
+        const int32_t m = min(kResolveChunk, kk - s0);
+        for (int j = 0; j < m; ++j) {
+            const float4 v = t.v[threadIdx.x][j];
+            if (acc64) {
+                rd += (double)v.y;
+                gd += (double)v.z;
+                bd += (double)v.w;
+            } else {
+                rf += v.y;
+                gf += v.z;
+                bf += v.w;
+            }
+        }
+    }
+    if (!live) return;
     const int32_t pix = slot_pixel(tab, q);
     if (acc64) {
-        double r = 0.0, g = 0.0, b = 0.0;
-        for (int32_t s = 0; s < kk; ++s) {
-            const float4 v = L[(int64_t)s * npix + q];
-            r += (double)v.y;
-            g += (double)v.z;
-            b += (double)v.w;
-        }
         double *o = acc64 + 3 * (int64_t)pix;
-        o[0] += r;
-        o[1] += g;
-        o[2] += b;
+        o[0] += rd;
+        o[1] += gd;
+        o[2] += bd;
     } else {
-        float r = 0.0f, g = 0.0f, b = 0.0f;
-        for (int32_t s = 0; s < kk; ++s) {
-            const float4 v = L[(int64_t)s * npix + q];
-            r += v.y;
-            g += v.z;
-            b += v.w;
-        }
         float *o = acc + 3 * (int64_t)pix;
-        o[0] += r;
-        o[1] += g;
-        o[2] += b;
+        o[0] += rf;
+        o[1] += gf;
+        o[2] += bf;
     }
 }
 
@@ -1539,6 +1591,7 @@ hipError_t launch_bounce(tt_scene *s, RenderState *rs, const tt_render_params *p
     sa.max_depth = p->max_depth;
     sa.rr_depth = p->rr_depth;
     sa.npix = npix;
+    sa.k = cap / npix;
     sa.seed = p->seed;
     sa.inv_spp = 1.0f / (float)p->spp;
     sa.pi = p->pi;
@@ -1721,6 +1774,7 @@ tt_status render_impl(tt_scene *s, int32_t nsc, const tt_camera *cams, const int
     SceneTable tab{};
     tab.n = nsc;
     tab.tile = kSlotTile;
+
     int64_t npix64 = 0;
     for (int32_t k = 0; k < nsc; ++k) {
         const tt_camera &c = cams[k];
