@@ -208,4 +208,13 @@ def test_full_traversal_workload_bit_exact(tt, oracle, ctx):
     _say(f"speculative variant: {len(bad)} differing ids of {len(rays)}")
     for i in bad[:1000]:
         assert abs(t1[i] - t[i]) <= 1e-6 * max(1.0, abs(t[i]))
+    # the headline configuration (speculative loop, 256-thread blocks, persistent waves
+    # refilling at 32 idle lanes, rays traced in Morton order of their origins) and the
+    # exact loop in the same launch: every record at its own index, bit-identical
+    for variant in (1, 0):
+        ids2, t2, u2, v2 = s.intersect(rays, tt.trace_cfg(variant, 16, 256, refill=32, reorder=1))
+        if variant == 0:
+            _check_exact(oracle, oh, ids2, t2, u2, v2)
+        else:
+            assert np.array_equal(ids2, ids1) and np.array_equal(t2.view(np.int32), t1.view(np.int32))
     s.close()
