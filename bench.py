@@ -108,6 +108,25 @@ def main():
     d_hits2.free()
     log(f"other variant: {spec}")
 
+    # each half of the ray set (coherent camera rays, incoherent surface rays) as a
+    # call of its own in the same configuration (each call sorts its own rays)
+    halves = {}
+    for name, lo, hi in (("coherent", 0, n_prim), ("incoherent", n_prim, n)):
+        d_half = tt.DeviceBuffer.from_array(ctx, np.ascontiguousarray(rays[lo:hi]))
+        d_hh = tt.DeviceBuffer(ctx, (hi - lo) * 16)
+        scene.trace_closest(d_half, hi - lo, d_hh, cfg)
+        ms_h = []
+        for i in range(min(args.steps, 10)):
+            timer.record(0)
+            scene.trace_closest(d_half, hi - lo, d_hh, cfg)
+            timer.record(1)
+            ms_h.append(timer.elapsed(0, 1))
+        mh = float(np.median(ms_h))
+        halves[name] = {"rays": hi - lo, "ms": round(mh, 4), "mrays_per_s": round((hi - lo) / mh / 1e3, 1)}
+        d_half.free()
+        d_hh.free()
+    log(f"halves: {halves}")
+
     # measured node-fetch roofline: random 64-byte gathers over a buffer the size
     # of the device BVH (nodes + triangle records)
     probe = tt.DeviceBuffer(ctx, info["device_bytes"])
@@ -149,11 +168,27 @@ def main():
         from oracle import oracle as O
 
         stride = args.cpu_sample_stride
-        idx = np.arange(0, n, stride)
-        sample = rays[idx]
         cb = O.CompactBVH(bvh.export_compact())
         cores = min(16, os.cpu_count() or 1)
-        oh, st = cb.trace(sample, nthreads=cores)
+        # the two halves traced separately, so that each half's visit counts give its
+        # own algorithmic bytes per ray
+        parts, idxs, st = [], [], {"seconds": 0.0, "inner": 0, "tris": 0}
+        for name, lo, hi in (("coherent", 0, n_prim), ("incoherent", n_prim, n)):
+            ih = np.arange(lo, hi, stride)
+            oh_h, st_h = cb.trace(rays[ih], nthreads=cores)
+            parts.append(oh_h)
+            idxs.append(ih)
+            for key in st:
+                st[key] += st_h[key]
+            bh = workloads.bytes_per_ray(st_h["inner"], st_h["tris"], len(ih))
+            gb = bh * halves[name]["rays"] / (halves[name]["ms"] * 1e-3) / 1e9
+            halves[name].update({"bytes_per_ray": round(bh, 1), "achieved_gbps": round(gb, 1),
+                                 "frac": round(gb / 8000.0, 4),
+                                 "node_visits_per_ray": round(st_h["inner"] / len(ih), 2),
+                                 "tri_tests_per_ray": round(st_h["tris"] / len(ih), 2)})
+        idx = np.concatenate(idxs)
+        sample = rays[idx]
+        oh = np.concatenate(parts)
         cpu_mrays = len(sample) / st["seconds"] / 1e6
         bpr = workloads.bytes_per_ray(st["inner"], st["tris"], len(sample))
         gpu_ids = hits[idx, 0]
@@ -179,6 +214,7 @@ def main():
             "node_fetch_frac": round(achieved / gather_gbps, 4),
             "node_visits_per_ray": round(st["inner"] / len(sample), 2),
             "tri_tests_per_ray": round(st["tris"] / len(sample), 2),
+            "per_half": halves,
         }
         result["cpu_baseline"] = {
             "value": round(cpu_mrays, 3), "unit": "Mrays/s", "cores": cores, "kind": "port",
