@@ -183,18 +183,20 @@ tt_status tt_bench_gather64(tt_ctx *ctx, const void *buf, size_t bytes, int64_t 
     void *sink = nullptr;
     tt_status st = dev_alloc(ctx, &sink, (size_t)lanes * 4, "gather sink");
     if (st != TT_OK) return st;
-    hipEvent_t a, b;
-    hipEventCreate(&a);
-    hipEventCreate(&b);
-    hipEventRecord(a, ctx->stream);
-    hipLaunchKernelGGL(k_gather64, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, ctx->stream,
-                       (const float4 *)buf, (uint64_t)(bytes / 64), seed, lanes, (uint32_t *)sink);
-    hipError_t e = hipGetLastError();
-    hipEventRecord(b, ctx->stream);
+    hipEvent_t a = nullptr, b = nullptr;
+    hipError_t e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    if (e == hipSuccess) e = hipEventRecord(a, ctx->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_gather64, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, ctx->stream,
+                           (const float4 *)buf, (uint64_t)(bytes / 64), seed, lanes, (uint32_t *)sink);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(b, ctx->stream);
     if (e == hipSuccess) e = hipEventSynchronize(b);
     if (e == hipSuccess) e = hipEventElapsedTime(ms, a, b);
-    hipEventDestroy(a);
-    hipEventDestroy(b);
+    if (a) hipEventDestroy(a);
+    if (b) hipEventDestroy(b);
     hipFree(sink);
     if (e != hipSuccess) return fail(TT_ERR_DEVICE, std::string("gather probe: ") + hipGetErrorString(e));
     return TT_OK;

@@ -636,7 +636,8 @@ static hipError_t guide_backward(tt_guide *g, int64_t n, hipStream_t s) {
     const int64_t R = (n + 15) & ~15LL;
     hipError_t e = guide_forward(g, n, nullptr, s);
     if (e != hipSuccess) return e;
-    hipMemsetAsync(g->scalars, 0, 16, s);
+    e = hipMemsetAsync(g->scalars, 0, 16, s);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_sum, dim3(256), dim3(256), 0, s, b.wts, n, g->scalars + 1);
     hipLaunchKernelGGL(k_loss_grad, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, s, b.raw, b.dirs, b.wts, b.draw,
                        g->scalars, (const float *)(g->scalars + 1), n);

@@ -1042,10 +1042,11 @@ __device__ __forceinline__ void shade_path_one(const ShadeArgs &a, const int32_t
                     }
                 }
                 // BSDF sampling, or one-sample MIS of guide and BSDF on guided vertices
-                f3 wil, weight;
-                float pdf_b;
-                bool delta;
-                bool sampled;
+                // (defined even where no direction is sampled: such paths end)
+                f3 wil = F3(0.0f, 0.0f, 1.0f), weight = F3(0.0f, 0.0f, 0.0f);
+                float pdf_b = 0.0f;
+                bool delta = false;
+                bool sampled = false;
                 float w_rr;  // BSDF-sampling weight of the direction (max component), for roulette
                 if (guided) {
                     float ug[4];
@@ -1864,16 +1865,16 @@ tt_status render_impl(tt_scene *s, int32_t nsc, const tt_camera *cams, const int
             e = launch_pass(s, rs, p, sv, npix, kk, fp64, hs, gr);
             if (e == hipSuccess && train) {
                 hipEvent_t t0 = nullptr, t1 = nullptr;
-                hipEventCreate(&t0);
-                hipEventCreate(&t1);
-                hipEventRecord(t0, hs);
-                e = guide_train_pass(gr, p->max_depth, hs);
-                hipEventRecord(t1, hs);
+                e = hipEventCreate(&t0);
+                if (e == hipSuccess) e = hipEventCreate(&t1);
+                if (e == hipSuccess) e = hipEventRecord(t0, hs);
+                if (e == hipSuccess) e = guide_train_pass(gr, p->max_depth, hs);
+                if (e == hipSuccess) e = hipEventRecord(t1, hs);
                 if (e == hipSuccess) e = hipEventSynchronize(t1);
                 float tms = 0.0f;
                 if (e == hipSuccess && hipEventElapsedTime(&tms, t0, t1) == hipSuccess) gr->train_ms += tms;
-                hipEventDestroy(t0);
-                hipEventDestroy(t1);
+                if (t0) hipEventDestroy(t0);
+                if (t1) hipEventDestroy(t1);
             }
             done += kk;
             ++pass;

@@ -273,6 +273,9 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
                     if (det > EPS) {
                         ok = !(u < 0.0f || u > 1.0f) && !(v < 0.0f || (u + v) > 1.0f);
                     } else if (det < -EPS) {
+                        // the reference's back-face bounds (IntersectKernel.cuh:248-250)
+                        // have their comparisons reversed: u > 0 || u < 1 always holds,
+                        // so back faces are never accepted; kept as written
                         ok = !(u > 0.0f || u < 1.0f) && !(v > 0.0f || (u + v) < 1.0f);
                     } else {
                         ok = false;
