@@ -218,3 +218,14 @@ def test_full_traversal_workload_bit_exact(tt, oracle, ctx):
         else:
             assert np.array_equal(ids2, ids1) and np.array_equal(t2.view(np.int32), t1.view(np.int32))
     s.close()
+
+
+def test_scene_refuses_triangle_ids_outside_its_mesh(tt, ctx):
+    """Shading reads the mesh's per-triangle arrays by the ids in the tree's records:
+    a tree built for a larger mesh is refused when paired with a smaller one."""
+    big, small = tt.Mesh.soup(3000, seed=2), tt.Mesh.soup(100, seed=2)
+    b = tt.BVH.build(big)
+    with pytest.raises(tt.TTError):
+        tt.DeviceScene(ctx, small, b)
+    s = tt.DeviceScene(ctx, big, b)
+    s.close()

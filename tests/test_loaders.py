@@ -218,3 +218,53 @@ def test_scene_config_errors(tt, tmp_path):
     (tmp_path / "s.json").write_text("{ not json")
     with pytest.raises(RuntimeError):
         tt.SceneConfig(str(tmp_path / "s.json"), base_dir=str(tmp_path))
+
+
+def _trailer_layout(data):
+    """Offsets in a saved cache: start of the device node words and tri floats."""
+    import ctypes
+    import struct
+    from trenchanttracer_amd import _native as N
+    at = data.index(b"TTDL")
+    dn, dt = struct.unpack_from("<qq", data, at + 8)
+    nodes = at + 24 + ctypes.sizeof(N.tt_bvh_stats)
+    return dn, dt, nodes, nodes + dn * 64
+
+
+def test_cache_with_broken_tree_is_refused(tt, tmp_path):
+    """The traversal kernel follows the cache's device layout on the GPU, so a loaded
+    layout must be a tree from node 0 (no cycle, every child in range), with leaves
+    inside the triangle array and the last record ending a leaf; each broken form is
+    refused with TT_ERR_IO while the intact file loads."""
+    import struct
+    m = tt.Mesh.soup(2000, seed=5)
+    b = tt.BVH.build(m)
+    good = tmp_path / "good.cache"
+    b.save(good)
+    data = good.read_bytes()
+    dn, dt, nodes, tris = _trailer_layout(data)
+    assert dn > 2 and dt > 2
+    tt.BVH.load(good)
+
+    def child_word(i, c):
+        return nodes + (i * 16 + 12 + c) * 4
+
+    # find an inner child of the root and a leaf slot anywhere
+    c0 = struct.unpack_from("<i", data, child_word(0, 0))[0]
+    assert c0 > 0
+    leaf_at = next(child_word(i, c) for i in range(dn) for c in (0, 1)
+                   if struct.unpack_from("<i", data, child_word(i, c))[0] < 0)
+    cases = {
+        "cycle to the root": (child_word(c0, 0), struct.pack("<i", 0)),
+        "child past the end": (child_word(0, 0), struct.pack("<i", dn)),
+        "node reached twice": (child_word(0, 1), struct.pack("<i", c0)),
+        "leaf past the triangles": (leaf_at, struct.pack("<i", ~(dt + 5))),
+        "last record not ending a leaf": (tris + ((dt - 1) * 12 + 11) * 4, struct.pack("<i", 0)),
+    }
+    for name, (off, word) in cases.items():
+        bad = bytearray(data)
+        bad[off:off + 4] = word
+        p = tmp_path / "bad.cache"
+        p.write_bytes(bytes(bad))
+        with pytest.raises(tt.TTError):
+            tt.BVH.load(p)

@@ -265,6 +265,10 @@ tt_status tt_scene_create(tt_ctx *ctx, const tt_mesh *mesh, const tt_bvh *bvh, c
     const Flat &f = bvh->b.flat;
     if (f.dev_nodes.size() % 16 != 0 || f.dev_tris.size() % 12 != 0)
         return fail(TT_ERR_INVALID, "tt_scene_create: corrupt device layout");
+    if (mesh) {   // shading reads the mesh's per-triangle arrays by the records' ids
+        const tt_status cs = check_tri_ids(f, mesh->m.ntris());
+        if (cs != TT_OK) return cs;
+    }
     TT_HIP(hipSetDevice(ctx->device), "hipSetDevice");
     auto s = new (std::nothrow) tt_scene();
     if (!s) return fail(TT_ERR_NOMEM, "tt_scene_create: host allocation failed");

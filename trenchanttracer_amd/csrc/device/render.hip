@@ -2065,6 +2065,8 @@ tt_status tt_batch_create(tt_ctx *ctx, int32_t n, const tt_mesh *const *meshes, 
             const int32_t mo = (int32_t)allm.size();
             const int32_t node_o = (int32_t)(F.dev_nodes.size() / 16), rec_o = (int32_t)(F.dev_tris.size() / 12);
             if (f.dev_nodes.empty()) return fail(TT_ERR_INVALID, "tt_batch_create: empty tree");
+            const tt_status cs = check_tri_ids(f, m.ntris());
+            if (cs != TT_OK) return cs;
             M.verts.insert(M.verts.end(), m.verts.begin(), m.verts.end());
             M.normals.insert(M.normals.end(), m.normals.begin(), m.normals.end());
             for (int32_t x : m.tri_v) M.tri_v.push_back(x + vo);

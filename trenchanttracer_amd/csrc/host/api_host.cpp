@@ -53,6 +53,63 @@ static_assert(sizeof(tt_render_stats) == 40, "tt_render_stats must be 40 bytes")
 static_assert(sizeof(tt_guide_params) == 28, "tt_guide_params must be 28 bytes");
 static_assert(sizeof(tt_guide_stats) == 32, "tt_guide_stats must be 32 bytes");
 
+namespace tt {
+
+// The traversal kernel follows child indices from node 0 with a stack of 72 entries
+// and walks a leaf's triangle records until one carries the end flag, so a layout
+// from a file must be a tree (each node reached once: the walk ends) no deeper than
+// 64 inner nodes, with every leaf starting inside the triangle array and the last
+// record flagged (a leaf walk then stops inside the array).
+tt_status check_device_layout(const Flat &f) {
+    if (f.dev_nodes.size() % 16 != 0 || f.dev_tris.size() % 12 != 0)
+        return fail(TT_ERR_IO, "bvh: device layout has a partial node or triangle record");
+    const int64_t dn = (int64_t)(f.dev_nodes.size() / 16), dt = (int64_t)(f.dev_tris.size() / 12);
+    if (dn == 0) return TT_OK;
+    if (dt > 0) {
+        int32_t last;
+        std::memcpy(&last, &f.dev_tris[(size_t)(dt - 1) * 12 + 11], 4);
+        if (last == 0) return fail(TT_ERR_IO, "bvh: the last triangle record does not end a leaf");
+    }
+    std::vector<uint8_t> seen((size_t)dn, 0);
+    std::vector<std::pair<int64_t, int32_t>> stack{{0, 0}};
+    seen[0] = 1;
+    while (!stack.empty()) {
+        const int64_t i = stack.back().first;
+        const int32_t depth = stack.back().second;
+        stack.pop_back();
+        for (int c = 0; c < 2; ++c) {
+            const int32_t ch = f.dev_nodes[(size_t)i * 16 + 12 + c];
+            if (ch >= 0) {
+                if (ch >= dn || seen[(size_t)ch]) return fail(TT_ERR_IO, "bvh: node graph is not a tree");
+                if (depth + 1 > 64) return fail(TT_ERR_IO, "bvh: tree deeper than 64 levels");
+                seen[(size_t)ch] = 1;
+                stack.push_back({ch, depth + 1});
+            } else if (~(int64_t)ch >= dt) {
+                return fail(TT_ERR_IO, "bvh: leaf starts past the triangle array");
+            }
+        }
+    }
+    return TT_OK;
+}
+
+tt_status check_tri_ids(const Flat &f, int64_t ntris) {
+    for (size_t q = 3; q < f.dev_tris.size(); q += 12) {
+        int32_t id;
+        std::memcpy(&id, &f.dev_tris[q], 4);
+        if (id < -1 || id >= ntris) return fail(TT_ERR_INVALID, "bvh: triangle id outside the mesh");
+    }
+    return TT_OK;
+}
+
+}  // namespace tt
+
+// Copies a host array out to a caller buffer (nothing for an empty array, whose
+// data() may be null).
+template <class V>
+static void copy_out(void *dst, const V &v, size_t elem_bytes) {
+    if (dst && !v.empty()) std::memcpy(dst, v.data(), v.size() * elem_bytes);
+}
+
 extern "C" {
 
 const char *tt_last_error(void) { return tt::g_err.c_str(); }
@@ -103,11 +160,11 @@ tt_status tt_mesh_get(const tt_mesh *m, float *verts, int32_t *tris, float *norm
                       int32_t *tri_mats) {
     if (!m) return fail(TT_ERR_INVALID, "tt_mesh_get: null mesh");
     const Mesh &M = m->m;
-    if (verts) std::memcpy(verts, M.verts.data(), M.verts.size() * 12);
-    if (tris) std::memcpy(tris, M.tri_v.data(), M.tri_v.size() * 4);
-    if (normals) std::memcpy(normals, M.normals.data(), M.normals.size() * 12);
-    if (tri_normals) std::memcpy(tri_normals, M.tri_n.data(), M.tri_n.size() * 4);
-    if (tri_mats) std::memcpy(tri_mats, M.tri_mat.data(), M.tri_mat.size() * 4);
+    copy_out(verts, M.verts, 12);
+    copy_out(tris, M.tri_v, 4);
+    copy_out(normals, M.normals, 12);
+    copy_out(tri_normals, M.tri_n, 4);
+    copy_out(tri_mats, M.tri_mat, 4);
     return TT_OK;
 }
 
@@ -210,11 +267,11 @@ tt_status tt_bvh_export_compact(const tt_bvh *b, tt_compact_sizes *sizes, int32_
     sizes->mat_indices = (uint32_t)f.ref_mat_idx.size();
     sizes->leaf_count = f.leaf_count;
     sizes->tri_count = f.tri_count;
-    if (nodes) std::memcpy(nodes, f.ref_nodes.data(), f.ref_nodes.size() * 4);
-    if (vertices) std::memcpy(vertices, f.ref_vertices.data(), f.ref_vertices.size() * 4);
-    if (normals) std::memcpy(normals, f.ref_normals.data(), f.ref_normals.size() * 4);
-    if (tri_indices) std::memcpy(tri_indices, f.ref_tri_idx.data(), f.ref_tri_idx.size() * 4);
-    if (mat_indices) std::memcpy(mat_indices, f.ref_mat_idx.data(), f.ref_mat_idx.size() * 4);
+    copy_out(nodes, f.ref_nodes, 4);
+    copy_out(vertices, f.ref_vertices, 4);
+    copy_out(normals, f.ref_normals, 4);
+    copy_out(tri_indices, f.ref_tri_idx, 4);
+    copy_out(mat_indices, f.ref_mat_idx, 4);
     return TT_OK;
 }
 
@@ -224,8 +281,8 @@ tt_status tt_bvh_export_device_layout(const tt_bvh *b, int64_t *num_nodes, int64
     const Flat &f = b->b.flat;
     if (num_nodes) *num_nodes = (int64_t)(f.dev_nodes.size() / 16);
     if (num_tris) *num_tris = (int64_t)(f.dev_tris.size() / 12);
-    if (nodes) std::memcpy(nodes, f.dev_nodes.data(), f.dev_nodes.size() * 4);
-    if (tris) std::memcpy(tris, f.dev_tris.data(), f.dev_tris.size() * 4);
+    copy_out(nodes, f.dev_nodes, 4);
+    copy_out(tris, f.dev_tris, 4);
     return TT_OK;
 }
 
@@ -377,6 +434,8 @@ tt_status tt_bvh_load(const char *path, tt_bvh **out) {
         b->b.stats.num_leaves = (int32_t)f.leaf_count;
         b->b.stats.num_refs = (int32_t)(f.dev_tris.size() / 12);
     }
+    tt_status vs = check_device_layout(f);
+    if (vs != TT_OK) return vs;
     *out = b.release();
     return TT_OK;
     TT_GUARD_END

@@ -112,6 +112,11 @@ void build_sbvh(const Mesh &m, float split_alpha, int nthreads, BVH &out);
 // flatten.cpp
 void flatten(const Mesh &m, BVH &b);
 void pair_align(std::vector<int32_t> &dev_nodes);
+// api_host.cpp: structural check of a device layout (a tree from node 0 with every
+// index in range, depth <= 64, leaves inside the triangle array, the last record
+// ending a leaf) and of its triangle ids against a mesh of ntris triangles
+tt_status check_device_layout(const Flat &f);
+tt_status check_tri_ids(const Flat &f, int64_t ntris);
 // gen.cpp helpers
 void face_normal_fill(Mesh &m);
 
