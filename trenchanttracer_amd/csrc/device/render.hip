@@ -1327,6 +1327,21 @@ __global__ void k_zero(float *acc, double *acc64, unsigned long long *stats, int
     if (i < 2 && stats) stats[i] = 0ull;
 }
 
+// End of a path-mode render: the pixel sums become means in place, with the
+// arithmetic the host would use (a float multiply by 1 / spp, or an FP64 division
+// rounded to float), so the image can be copied straight into the caller's buffers.
+__global__ void k_finish(float *acc, double *acc64, int64_t n3, int32_t spp) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n3) return;
+    if (acc64) {
+        const double m = acc64[i] / (double)spp;
+        acc64[i] = m;
+        acc[i] = (float)m;
+    } else {
+        acc[i] = acc[i] * (1.0f / (float)spp);
+    }
+}
+
 __global__ void k_debug_bsdf(tt_material m, f3 wo, bool front, int64_t n, uint32_t seed, float *out, float *out_eval) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -1763,8 +1778,16 @@ hipError_t guide_train_pass(GuideRun *gr, int32_t max_depth, hipStream_t hs) {
 
 // Renders nsc scenes sharing one node / triangle array (roots[k] = root node of
 // scene k) in one wavefront. out: host, concatenated images in scene order.
+// Where a render's image goes: `count` floats (and FP64 values, if rgb64 is set) of
+// the concatenated image starting at `offset`.
+struct OutSeg {
+    float *rgb;
+    double *rgb64;
+    int64_t offset, count;
+};
+
 tt_status render_impl(tt_scene *s, int32_t nsc, const tt_camera *cams, const int32_t *roots,
-                      const tt_render_params *p, float *out, double *out64, tt_render_stats *stats,
+                      const tt_render_params *p, const OutSeg *segs, int32_t nseg, tt_render_stats *stats,
                       GuideRun *gr = nullptr) {
     tt_status st = validate_params(p);
     if (st != TT_OK) return st;
@@ -1901,13 +1924,18 @@ tt_status render_impl(tt_scene *s, int32_t nsc, const tt_camera *cams, const int
         iters += p->max_depth;
     }
     if (e == hipSuccess) e = hipEventRecord(e1, hs);
-    std::vector<float> acc((size_t)n3);
-    std::vector<double> acc64;
+    // reference mode's sums are already the image (acc + L / spp per sample)
+    if (e == hipSuccess && p->mode != TT_RENDER_REFERENCE) {
+        hipLaunchKernelGGL(k_finish, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, hs, rs->acc,
+                           fp64 ? rs->acc64 : nullptr, n3, p->spp);
+        e = hipGetLastError();
+    }
     unsigned long long hstats[2] = {0, 0};
-    if (e == hipSuccess) e = hipMemcpyAsync(acc.data(), rs->acc, (size_t)n3 * 4, hipMemcpyDeviceToHost, hs);
-    if (e == hipSuccess && fp64) {
-        acc64.resize((size_t)n3);
-        e = hipMemcpyAsync(acc64.data(), rs->acc64, (size_t)n3 * 8, hipMemcpyDeviceToHost, hs);
+    for (int32_t k = 0; e == hipSuccess && k < nseg; ++k) {
+        e = hipMemcpyAsync(segs[k].rgb, rs->acc + segs[k].offset, (size_t)segs[k].count * 4, hipMemcpyDeviceToHost, hs);
+        if (e == hipSuccess && fp64 && segs[k].rgb64)
+            e = hipMemcpyAsync(segs[k].rgb64, rs->acc64 + segs[k].offset, (size_t)segs[k].count * 8,
+                               hipMemcpyDeviceToHost, hs);
     }
     if (e == hipSuccess) e = hipMemcpyAsync(hstats, rs->stats, 16, hipMemcpyDeviceToHost, hs);
     if (e == hipSuccess) e = hipStreamSynchronize(hs);
@@ -1918,16 +1946,6 @@ tt_status render_impl(tt_scene *s, int32_t nsc, const tt_camera *cams, const int
     if (e0) hipEventDestroy(e0);
     if (e1) hipEventDestroy(e1);
     if (e != hipSuccess) return fail(TT_ERR_DEVICE, std::string("render: ") + hipGetErrorString(e));
-    if (p->mode == TT_RENDER_REFERENCE) {
-        std::memcpy(out, acc.data(), (size_t)n3 * 4);
-    } else if (fp64) {
-        for (int64_t q = 0; q < n3; ++q) out[q] = (float)(acc64[q] / (double)p->spp);
-        if (out64)
-            for (int64_t q = 0; q < n3; ++q) out64[q] = acc64[q] / (double)p->spp;
-    } else {
-        const float inv = 1.0f / (float)p->spp;
-        for (int64_t q = 0; q < n3; ++q) out[q] = acc[q] * inv;
-    }
     if (stats) {
         stats->rays_extension = (int64_t)hstats[0];
         stats->rays_shadow = (int64_t)hstats[1];
@@ -2008,7 +2026,9 @@ tt_status tt_render(tt_scene *s, const tt_camera *cam, const tt_render_params *p
                     tt_render_stats *stats) {
     if (!s || !cam || !p || !rgb_out) return fail(TT_ERR_INVALID, "tt_render: null argument");
     const int32_t root = 0;
-    return render_impl(s, 1, cam, &root, p, rgb_out, rgb64_out, stats);
+    const int64_t n3 = (int64_t)std::max(0, cam->width) * std::max(0, cam->height) * 3;
+    const OutSeg seg{rgb_out, rgb64_out, 0, n3};
+    return render_impl(s, 1, cam, &root, p, &seg, 1, stats);
 }
 
 tt_status tt_guide_render(tt_scene *s, tt_guide *g, const tt_camera *cam, const tt_render_params *p,
@@ -2032,7 +2052,9 @@ tt_status tt_guide_render(tt_scene *s, tt_guide *g, const tt_camera *cam, const 
     gr.train_spp = gp->train_spp;
     gr.max_bounce = gp->max_bounce;
     const int32_t root = 0;
-    tt_status st = render_impl(s, 1, cam, &root, &q, rgb_out, rgb64_out, stats, &gr);
+    const int64_t n3 = (int64_t)std::max(0, cam->width) * std::max(0, cam->height) * 3;
+    const OutSeg seg{rgb_out, rgb64_out, 0, n3};
+    tt_status st = render_impl(s, 1, cam, &root, &q, &seg, 1, stats, &gr);
     if (st == TT_OK && gstats) {
         gstats->train_steps = gr.steps;
         gstats->train_rows = gr.rows;
@@ -2121,21 +2143,16 @@ tt_status tt_batch_render(tt_batch *b, const tt_camera *cams, const tt_render_pa
                           tt_render_stats *stats) {
     if (!b || !cams || !p || !rgb_out) return fail(TT_ERR_INVALID, "tt_batch_render: null argument");
     const int32_t n = (int32_t)b->roots.size();
-    size_t total = 0;
+    // each scene's part of the concatenated image goes straight to its buffer
+    std::vector<OutSeg> segs((size_t)n);
+    int64_t off = 0;
     for (int32_t k = 0; k < n; ++k) {
         if (!rgb_out[k]) return fail(TT_ERR_INVALID, "tt_batch_render: null image");
-        total += (size_t)std::max(0, cams[k].width) * (size_t)std::max(0, cams[k].height) * 3;
-    }
-    std::vector<float> all(total);
-    tt_status st = render_impl(b->scene, n, cams, b->roots.data(), p, all.data(), nullptr, stats);
-    if (st != TT_OK) return st;
-    size_t off = 0;
-    for (int32_t k = 0; k < n; ++k) {
-        size_t cnt = (size_t)cams[k].width * cams[k].height * 3;
-        std::memcpy(rgb_out[k], all.data() + off, cnt * 4);
+        const int64_t cnt = (int64_t)std::max(0, cams[k].width) * std::max(0, cams[k].height) * 3;
+        segs[(size_t)k] = OutSeg{rgb_out[k], nullptr, off, cnt};
         off += cnt;
     }
-    return TT_OK;
+    return render_impl(b->scene, n, cams, b->roots.data(), p, segs.data(), n, stats);
 }
 
 tt_status tt_debug_bsdf_sample(tt_ctx *ctx, const tt_material *m, const float wo[3], int32_t front, int64_t n,
