@@ -154,6 +154,7 @@ struct InferArgs {
     float *raw;
     int64_t n;
     const int32_t *n_dev;
+    const int32_t *live;   // if set: rows with live[r] == 0 are not needed; tiles of none are skipped
 };
 
 template <int K, int LD>
@@ -227,6 +228,10 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(InferArgs a) {
     // inside the loop; wave_sync orders the tile's LDS writes and reads
     for (int64_t tile = (int64_t)blockIdx.x * kInferWaves + wave; tile < ntiles;
          tile += (int64_t)gridDim.x * kInferWaves) {
+        if (a.live) {   // the wave skips a tile none of whose rows is needed (its outputs stay unwritten)
+            const int64_t r = tile * 16 + (lane & 15);
+            if (!__any(lane < 16 && r < rows && a.live[r] != 0)) continue;
+        }
         // features x[rr][24 c + 2 f + {sin, cos}]: chain q = (row rr, input c), row-major
         // so that the input loads coalesce; rows past the end get zero features
         for (int q = lane; q < 16 * kGuideIn; q += 64) {
@@ -561,11 +566,14 @@ tt_status guide_alloc_infer(tt_guide *g, int64_t rows) {
     if (g->icap >= rows) return TT_OK;
     if (g->iin) hipFree(g->iin);
     if (g->iraw) hipFree(g->iraw);
+    if (g->irow) hipFree(g->irow);
     g->iin = g->iraw = nullptr;
+    g->irow = nullptr;
     g->icap = 0;
     const int64_t R = (rows + 15) & ~15LL;
     tt_status st = dev_alloc(g->ctx, (void **)&g->iin, (size_t)R * kGuideIn * 4, "guide inference inputs");
     if (st == TT_OK) st = dev_alloc(g->ctx, (void **)&g->iraw, (size_t)R * kGuideNOutPad16 * 4, "guide inference outputs");
+    if (st == TT_OK) st = dev_alloc(g->ctx, (void **)&g->irow, (size_t)R * 4, "guide inference row flags");
     if (st == TT_OK) g->icap = R;
     return st;
 }
@@ -607,9 +615,11 @@ hipError_t guide_forward(tt_guide *g, int64_t n, const int32_t *n_dev, hipStream
 
 // Inference over the first n rows of g->batch.in (device count n_dev optional):
 // the fused kernel, raw outputs (fp32, 48 wide) into g->batch.raw.
-hipError_t guide_infer(tt_guide *g, const float *in, float *raw, int64_t n, const int32_t *n_dev, hipStream_t s) {
+hipError_t guide_infer(tt_guide *g, const float *in, float *raw, int64_t n, const int32_t *n_dev, hipStream_t s,
+                       const int32_t *live) {
     InferArgs a{};
     a.in = in;
+    a.live = live;
     for (int k = 0; k < 4; ++k) {
         a.w[k] = g->wf[k];
         a.bias[k] = g->params + layer_offset(k) + (int64_t)kGuideLayers[k].in * kGuideLayers[k].out;
@@ -799,7 +809,8 @@ void tt_guide_destroy(tt_guide *g) {
     GuideBatch &b = g->batch;
     void *ptrs[] = {g->params, g->m1, g->m2, g->grad, g->scalars, g->partial, b.in, b.x, b.h[0], b.h[1], b.h[2],
                     b.raw, b.draw, b.dh[0], b.dh[1], b.dh[2], b.dirs, b.wts, g->wf[0], g->wf[1], g->wf[2], g->wf[3],
-                    g->wb[0], g->wb[1], g->wb[2], g->wb[3], g->train_buf, g->train_count, g->rec, g->iin, g->iraw};
+                    g->wb[0], g->wb[1], g->wb[2], g->wb[3], g->train_buf, g->train_count, g->rec, g->iin, g->iraw,
+                    g->irow};
     for (void *q : ptrs)
         if (q) hipFree(q);
     delete g;

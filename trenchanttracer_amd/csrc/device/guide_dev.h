@@ -167,7 +167,9 @@ __device__ __forceinline__ void mixture_sample(const Mix &L, float u0, float u1,
 }
 
 hipError_t guide_forward(tt_guide *g, int64_t n, const int32_t *n_dev, hipStream_t s);
-hipError_t guide_infer(tt_guide *g, const float *in, float *raw, int64_t n, const int32_t *n_dev, hipStream_t s);
+// live (may be null): per-row flags; 16-row tiles with no flagged row are skipped
+hipError_t guide_infer(tt_guide *g, const float *in, float *raw, int64_t n, const int32_t *n_dev, hipStream_t s,
+                       const int32_t *live = nullptr);
 hipError_t guide_train_rows(tt_guide *g, int64_t n, hipStream_t s);
 tt_status guide_alloc_batch(tt_guide *g, int64_t rows);
 tt_status guide_alloc_infer(tt_guide *g, int64_t rows);
@@ -186,6 +188,7 @@ struct tt_guide {
     int64_t cap = 0;
     // renderer inference rows (one per queued path): inputs [icap][9], mixtures [icap][48]
     float *iin = nullptr, *iraw = nullptr;
+    int32_t *irow = nullptr;    // renderer: 1 where a queue slot's vertex is guided (its row is inferred)
     int64_t icap = 0;
     float lr = 1e-3f, beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
     int64_t step = 0;

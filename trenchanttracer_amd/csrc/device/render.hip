@@ -879,18 +879,6 @@ __device__ __forceinline__ void guide_inputs(const GuideShade &gd, f3 point, f3 
 }
 
 // Shading normal oriented like the geometric normal towards wo (as the shade stage).
-__device__ __forceinline__ f3 oriented_normal(const SceneView &sv, const tt_hit &h, f3 point, f3 wo, f3 *ngf_out,
-                                              bool *front) {
-    const Surf sf = surface(sv, h, point);
-    const f3 ng = sf.ng;
-    *front = dot(wo, ng) > 0.0f;
-    const f3 ngf = *front ? ng : ng * -1.0f;
-    f3 ns = sf.analytic ? ngf : vnorm(sf.nrm);
-    if (dot(ns, ngf) < 0.0f) ns = ns * -1.0f;
-    *ngf_out = ngf;
-    return ns;
-}
-
 struct GuideInArgs {
     GuideShade gd;
     SceneView sv;
@@ -898,25 +886,43 @@ struct GuideInArgs {
     const tt_hit *hits;
     const int32_t *counts;
     int32_t cur;
-    float *in;   // [slot][9]
+    float *in;        // [slot][9]
+    int32_t *live;    // [slot]: 1 if the vertex is guided (its mixture is needed)
 };
 
+// The vertices the guide samples at: diffuse surfaces and conductors at least
+// min_rough rough (smooth conductors and dielectrics are delta-like: BSDF sampling).
+__device__ __forceinline__ bool guide_eligible(const tt_material &m, float min_rough) {
+    return m.type == TT_MAT_DIFFUSE || (m.type == TT_MAT_CONDUCTOR && m.roughness >= min_rough);
+}
+
+// Network inputs of the hit queue, and a flag per slot for the vertices the guide
+// serves: inference skips every 16-row tile without a flagged row. Paths are
+// pixel-major, so a pixel's samples (all reaching the sky, or all one glass facet)
+// share their flag and whole tiles drop out: on the 1080p path-trace case most of the
+// queue (sky, glass, smooth conductors).
 __global__ __launch_bounds__(256) void k_guide_inputs(GuideInArgs a) {
     const int32_t n = a.counts[a.cur];
     for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        float *in = a.in + (int64_t)i * kGuideIn;
         const tt_hit h = a.hits[i];
-        if (h.tri_id == -1) {
-            for (int k = 0; k < kGuideIn; ++k) in[k] = 0.0f;
-            continue;
+        bool take = false;
+        if (h.tri_id != -1) {
+            const tt_ray r = a.rays[i];
+            const f3 rd = F3(r.dx, r.dy, r.dz), wo = rd * -1.0f;
+            const f3 point = F3(r.ox, r.oy, r.oz) + rd * h.t;
+            const Surf sf = surface(a.sv, h, point);
+            take = guide_eligible(a.sv.mats[sf.mat], a.gd.min_rough);
+            if (take) {
+                const f3 ng = sf.ng;
+                const f3 ngf = dot(wo, ng) > 0.0f ? ng : ng * -1.0f;
+                f3 ns = sf.analytic ? ngf : vnorm(sf.nrm);
+                if (dot(ns, ngf) < 0.0f) ns = ns * -1.0f;
+                guide_inputs(a.gd, point, ns, wo, a.in + (int64_t)i * kGuideIn);
+            }
         }
-        const tt_ray r = a.rays[i];
-        const f3 rd = F3(r.dx, r.dy, r.dz), wo = rd * -1.0f;
-        const f3 point = F3(r.ox, r.oy, r.oz) + rd * h.t;
-        f3 ngf;
-        bool front;
-        const f3 ns = oriented_normal(a.sv, h, point, wo, &ngf, &front);
-        guide_inputs(a.gd, point, ns, wo, in);
+        if (!take)   // unflagged rows of a tile that is inferred still read defined inputs
+            for (int k = 0; k < kGuideIn; ++k) a.in[(int64_t)i * kGuideIn + k] = 0.0f;
+        a.live[i] = take ? 1 : 0;
     }
 }
 
@@ -984,9 +990,9 @@ __device__ __forceinline__ void shade_path_one(const ShadeArgs &a, const int32_t
                 draws((uint32_t)local, smp, (uint32_t)(a.bounce + 1), 1u, a.seed, ul);
                 draws((uint32_t)local, smp, (uint32_t)(a.bounce + 1), 2u, a.seed, ub);
                 const bool smooth = m.type == TT_MAT_DIELECTRIC || m.type == TT_MAT_SPECULAR;
-                // guiding: mixture of this slot (non-delta surfaces only)
-                const bool guided = GUIDED && a.gd.raw != nullptr &&
-                                    (m.type == TT_MAT_DIFFUSE || (m.type == TT_MAT_CONDUCTOR && m.roughness >= a.gd.min_rough));
+                // guiding: mixture of this slot (the vertices k_guide_inputs flagged, by
+                // the same rule, so their rows were inferred)
+                const bool guided = GUIDED && a.gd.raw != nullptr && guide_eligible(m, a.gd.min_rough);
                 Mixture lobes;
                 if (guided) decode_mixture(a.gd.raw + (int64_t)i * kGuideNOutPad16, lobes);
                 const float alpha = guided ? a.gd.alpha : 0.0f;
@@ -1569,11 +1575,12 @@ hipError_t launch_bounce(tt_scene *s, RenderState *rs, const tt_render_params *p
     const bool guide_here = gr && (gr->max_bounce <= 0 || b < gr->max_bounce);
     const bool records = gr && gr->gd.rec != nullptr;
     if (guide_here) {
-        // guide inference over the hit queue: inputs, then the MLP (rows up to the device count)
-        GuideInArgs ga{gr->gd, sv, rs->rays[cur], rs->hits, rs->counts, cur, gr->g->iin};
+        // guide inference over the hit queue: inputs and flags, then the MLP (rows up
+        // to the device count, tiles of unflagged rows skipped)
+        GuideInArgs ga{gr->gd, sv, rs->rays[cur], rs->hits, rs->counts, cur, gr->g->iin, gr->g->irow};
         hipLaunchKernelGGL(k_guide_inputs, dim3(gblocks), dim3(256), 0, hs, ga);
         e = hipGetLastError();
-        if (e == hipSuccess) e = guide_infer(gr->g, gr->g->iin, gr->g->iraw, cap, rs->counts + cur, hs);
+        if (e == hipSuccess) e = guide_infer(gr->g, gr->g->iin, gr->g->iraw, cap, rs->counts + cur, hs, gr->g->irow);
         if (e != hipSuccess) return e;
         sa.gd = gr->gd;
         sa.gd.raw = gr->g->iraw;
