@@ -4,7 +4,8 @@ traversal (default): one closest-hit traversal launch over the traversal workloa
 (1M-triangle soup SBVH, 8,294,400 rays), then one 256 MiB device-to-device copy
 whose known byte counts calibrate the FETCH_SIZE / WRITE_SIZE hardware counters.
 pathtrace: the path-trace case (1,310,720-tri displaced icosphere, sky + sun,
-1920x1080, depth 16) rendered once at 4 spp (warm-up) and once at 64 spp.
+1920x1080, depth 16) rendered once at 4 spp (warm-up) and once at 64 spp, then the
+same calibration copy.
 train: 13 Adam steps of the guiding network on 262,144 seeded random rows (inputs in
 [0, 1], unit directions, Gamma(1) weights), the batch of the online training.
 For profiler runs, one counter set per run:
@@ -15,6 +16,20 @@ For profiler runs, one counter set per run:
 """
 import argparse
 import sys
+
+
+def _copy_calibration(ctx, nbytes=256 << 20):
+    """One device-to-device copy of known size (k_copy16): its counter readings scale
+    the HBM counters of the kernels profiled in the same process."""
+    import trenchanttracer_amd as tt
+    src, dst = tt.DeviceBuffer(ctx, nbytes), tt.DeviceBuffer(ctx, nbytes)
+    src.zero()
+    ctx.sync()
+    dst.copy_from(src)
+    ctx.sync()
+    src.free()
+    dst.free()
+    return nbytes
 
 
 def main():
@@ -56,6 +71,7 @@ def main():
         img, st = s.render(cam, tt.render_params(spp=64, max_depth=16, rr_depth=4, seed=1, mode=tt.RENDER_PATH,
                                                  paths_in_flight=args.paths_in_flight))
         print(f"path trace 64 spp: {st}", file=sys.stderr, flush=True)
+        print(f"copied {_copy_calibration(ctx)} bytes", file=sys.stderr, flush=True)
         s.close()
         ctx.close()
         return
@@ -70,15 +86,10 @@ def main():
                        reorder=args.reorder)
     scene.trace_closest(d_rays, n, d_hits, cfg)
     ctx.sync()
-    nbytes = 256 << 20
-    src, dst = tt.DeviceBuffer(ctx, nbytes), tt.DeviceBuffer(ctx, nbytes)
-    src.zero()
-    ctx.sync()
-    dst.copy_from(src)
-    ctx.sync()
+    nbytes = _copy_calibration(ctx)
     print(f"traced {n} rays (variant {args.variant}, refill {args.refill}, reorder {args.reorder}); "
           f"copied {nbytes} bytes", file=sys.stderr, flush=True)
-    for b in (src, dst, d_rays, d_hits):
+    for b in (d_rays, d_hits):
         b.free()
     scene.close()
     ctx.close()
