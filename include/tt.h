@@ -495,8 +495,9 @@ tt_status tt_debug_guide_sample(tt_ctx *ctx, const float *raw, int64_t n, uint32
  * kernels. The two agree bit for bit. Synchronous. */
 tt_status tt_debug_guide_forward_unfused(tt_guide *g, const float *inputs, int64_t n, float *raw_out);
 
-/* Reproducibility hook for the render math: applies op (0 sin, 1 cos, 2 pow(x, y)) of
- * the device sampler math to n device values; y is read for op 2. Synchronous. */
+/* Reproducibility hook for the render math: applies op (0 sin, 1 cos, 2 pow(x, y),
+ * 3 atan2(x, y) with x the ordinate, 4 acos) of the device sampler and env-lookup math
+ * to n device values; y is read for ops 2 and 3. Synchronous. */
 tt_status tt_debug_math(tt_ctx *ctx, int32_t op, const float *x, const float *y, float *out, int64_t n);
 
 /* Sampling hooks for the path-mode BSDFs and the env-map distribution (the same

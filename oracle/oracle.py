@@ -42,10 +42,15 @@ class o_ref_params(C.Structure):
                 ("pi", C.c_float), ("two_pi", C.c_float), ("tmin", C.c_float), ("tmax", C.c_float)]
 
 
-def _load():
-    if not os.path.exists(LIB_PATH):
-        from . import build as _b  # noqa
-    lib = C.CDLL(LIB_PATH)
+class o_env(C.Structure):
+    _fields_ = [("rgb", C.c_void_p), ("width", C.c_int32), ("height", C.c_int32), ("scale", C.c_float)]
+
+
+def _load(path=LIB_PATH):
+    if not os.path.exists(path):
+        from . import build as _b
+        _b.build()
+    lib = C.CDLL(path)
     P = C.c_void_p
     lib.o_philox4x32_10.argtypes = [P, P, P]
     lib.o_philox4x32_10.restype = None
@@ -59,8 +64,18 @@ def _load():
     lib.o_trace_batch.restype = C.c_double
     lib.o_tri_test.argtypes = [P, P, P, P, C.c_float, P, P, P]
     lib.o_tri_test.restype = C.c_int
-    lib.o_render_ref.argtypes = [C.POINTER(o_bvh), P, P, P, P, C.c_int32, C.POINTER(o_camera),
+    lib.o_render_ref.argtypes = [C.POINTER(o_bvh), P, P, P, P, C.c_int32, C.POINTER(o_env), C.POINTER(o_camera),
                                  C.POINTER(o_ref_params), C.c_int32, C.c_int32, P, C.c_int]
+    lib.o_ref_bounce.argtypes = [C.POINTER(o_material), P, P, P, P, P]
+    lib.o_ref_bounce.restype = C.c_int
+    lib.o_env_lookup.argtypes = [C.POINTER(o_env), P, C.c_float, C.c_float, P]
+    lib.o_env_lookup.restype = None
+    lib.o_atan2f.argtypes = [C.c_float, C.c_float]
+    lib.o_atan2f.restype = C.c_float
+    lib.o_acosf.argtypes = [C.c_float]
+    lib.o_acosf.restype = C.c_float
+    lib.o_math_batch.argtypes = [C.c_int, P, P, P, C.c_int64]
+    lib.o_math_batch.restype = None
     lib.o_render_ref.restype = C.c_int
     lib.o_geom_t.argtypes = [C.POINTER(o_geometry), P]
     lib.o_geom_t.restype = C.c_float
@@ -161,10 +176,69 @@ def geom_t(geom, ray):
     return float(lib.o_geom_t(C.byref(g[0]), _p(r)))
 
 
+def materials(mats):
+    """ctypes array of o_material from objects with type, albedo, emission, roughness, ior, specular."""
+    arr = (o_material * max(1, len(mats)))()
+    for k, m in enumerate(mats):
+        arr[k].type = m.type
+        arr[k].albedo[:] = list(m.albedo)
+        arr[k].emission[:] = list(m.emission)
+        arr[k].roughness, arr[k].ior = m.roughness, m.ior
+        arr[k].specular[:] = list(m.specular)
+    return arr
+
+
+class Env:
+    """An env map (H, W, 3) float32 held for the oracle's hdrSample restatement."""
+
+    def __init__(self, rgb, scale=1.0):
+        self.rgb = np.ascontiguousarray(rgb, np.float32)
+        assert self.rgb.ndim == 3 and self.rgb.shape[2] == 3
+        self.c = o_env(self.rgb.ctypes.data, self.rgb.shape[1], self.rgb.shape[0], float(scale))
+
+    def lookup(self, dirs, pi=3.14156265, two_pi=6.2831853071795864769, lib_=None):
+        L = lib_ or lib
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        out = np.empty_like(d)
+        for k in range(len(d)):
+            L.o_env_lookup(C.byref(self.c), C.c_void_p(d[k:].ctypes.data), pi, two_pi,
+                           C.c_void_p(out[k:].ctypes.data))
+        return out
+
+
+def ref_bounce(mat, u, d, point, nrm, lib_=None):
+    """One reference-mode surface interaction: (continues, next origin, next direction, mask)."""
+    L = lib_ or lib
+    m = materials([mat])
+    uu = np.ascontiguousarray(u, np.float32).reshape(4)
+    dd, pp, nn = (np.ascontiguousarray(x, np.float32).reshape(3) for x in (d, point, nrm))
+    out = np.empty(9, np.float32)
+    c = L.o_ref_bounce(C.byref(m[0]), _p(uu), _p(dd), _p(pp), _p(nn), _p(out))
+    return bool(c), out[0:3].copy(), out[3:6].copy(), out[6:9].copy()
+
+
+def math(op, x, y=None, lib_=None):
+    """Oracle float math over arrays: op 0 sin, 1 cos, 2 pow(x, y), 3 atan2(x, y), 4 acos."""
+    L = lib_ or lib
+    xa = np.ascontiguousarray(x, np.float32).ravel()
+    ya = np.ascontiguousarray(y if y is not None else np.zeros_like(xa), np.float32).ravel()
+    out = np.empty_like(xa)
+    L.o_math_batch(int(op), _p(xa), _p(ya), _p(out), len(xa))
+    return out
+
+
+def load_planted(k):
+    """The oracle built with planted mistake k (-DO_PLANT=k), for checks that must fail on it."""
+    from . import build as _b
+    return _load(_b.build(plant=k))
+
+
 def render_ref(compact_bvh, mesh_arrays, materials, cam, spp, max_depth, seed, pi=3.14156265,
-               two_pi=6.2831853071795864769, tmin=0.01, rows=None, nthreads=None, sample_offset=0, geoms=()):
+               two_pi=6.2831853071795864769, tmin=0.01, rows=None, nthreads=None, sample_offset=0, geoms=(),
+               env=None):
     """Reference-semantics render on the CPU: (H, W, 3) float32 image (rows outside `rows` stay 0);
-    `geoms` are the analytic primitives tested after the triangles."""
+    `geoms` are the analytic primitives tested after the triangles; `env` (an Env, or None = black)
+    is looked up on a miss."""
     oc = o_camera()
     for f in ("eye", "view", "up", "fov"):
         getattr(oc, f)[:] = list(getattr(cam, f))
@@ -184,8 +258,9 @@ def render_ref(compact_bvh, mesh_arrays, materials, cam, spp, max_depth, seed, p
     y0, y1 = rows if rows is not None else (0, cam.height)
     nt = nthreads or min(8, os.cpu_count() or 1)
     ga = geometries(list(geoms))
+    envp = C.byref(env.c) if env is not None else None
     rc = lib.o_render_ref(C.byref(compact_bvh.c), _p(tn), _p(tm), C.cast(mats, C.c_void_p), C.cast(ga, C.c_void_p),
-                          len(geoms), C.byref(oc), C.byref(prm), int(y0), int(y1), _p(out), nt)
+                          len(geoms), envp, C.byref(oc), C.byref(prm), int(y0), int(y1), _p(out), nt)
     if rc != 0:
         raise RuntimeError("o_render_ref failed")
     return out

@@ -20,6 +20,18 @@
  *   o_sinf/o_cosf/...    float64 numpy within stated ulp bounds
  *   o_render_ref         restated megakernel; checked by white furnace and
  *                        closed-form cases (tests/test_render_oracle.py)
+ *   o_ref_bounce         each reference sampler's law: direction cosines and
+ *                        azimuth against the uniforms, Phong lobe exponent,
+ *                        Snell / total internal reflection, the dielectric's
+ *                        0.2 reflection share and Re/P, Tr/(1-P) weights with
+ *                        the reference's R0, origin offsets along nl, chi-square
+ *                        of drawn directions (tests/test_oracle_samplers.py);
+ *                        every check is also run on builds with a planted
+ *                        mistake (-DO_PLANT=k) and must fail there
+ *   o_env_lookup         texel selection against closed-form longitude /
+ *                        latitude of directions, the x2 scale, the linear-index
+ *                        wrap and the end clamp (tests/test_oracle_env.py)
+ *   o_atan2f/o_acosf     float64 numpy within 2 ulp, C99 special values
  */
 #ifndef TT_ORACLE_H
 #define TT_ORACLE_H
@@ -78,13 +90,24 @@ typedef struct { int32_t type; float albedo[3]; float emission[3]; float roughne
 /* analytic primitive: type 0 sphere (centre a, radius b[0]), 1 parallelogram (p0 a, p1 b, p2 c) */
 typedef struct { int32_t type, material; float a[3], b[3], c[3]; float pad; } o_geometry;
 typedef struct { int32_t spp, max_depth; uint32_t seed; int32_t sample_offset; float pi, two_pi, tmin, tmax; } o_ref_params;
+/* equirectangular env map: width x height RGB floats, row 0 = +y (latitude 0) */
+typedef struct { const float *rgb; int32_t width, height; float scale; } o_env;
 
 /* Renders rows [y0, y1) of the image (row 0 = top) into out_rgb (W*H*3 floats,
  * the average over spp samples). tri_normals: 12 floats (3 float4) per original
- * triangle; tri_mat: material per original triangle. */
+ * triangle; tri_mat: material per original triangle; env (may be NULL = black)
+ * is looked up on a miss as hdrSample does. */
 int o_render_ref(const o_bvh *b, const float *tri_normals, const int32_t *tri_mat, const o_material *mats,
-                 const o_geometry *geoms, int32_t num_geoms, const o_camera *cam, const o_ref_params *p, int32_t y0,
-                 int32_t y1, float *out_rgb, int nthreads);
+                 const o_geometry *geoms, int32_t num_geoms, const o_env *env, const o_camera *cam,
+                 const o_ref_params *p, int32_t y0, int32_t y1, float *out_rgb, int nthreads);
+/* One reference-mode surface interaction (RenderKernel.cu:72-92 and the material's
+ * sampler): incoming direction d, hit point, unnormalised normal nrm, uniforms u.
+ * out: next origin (3), next direction (3), mask multiplier (3, from 1). Returns 0
+ * when the material ends the path. */
+int o_ref_bounce(const o_material *m, const float u[4], const float d[3], const float point[3], const float nrm[3],
+                 float out[9]);
+/* hdrSample (HDRKernel.cuh:6-42) for direction d (out: RGB). */
+void o_env_lookup(const o_env *e, const float d[3], float pi, float two_pi, float out[3]);
 /* distance at which ray r hits primitive g under the reference's rules
  * (SphereImpl.cuh:10-32, PlaneImpl.cuh:10-52), +inf when it does not */
 float o_geom_t(const o_geometry *g, const o_ray *r);
@@ -96,6 +119,10 @@ float o_cosf(float x);
 float o_logf(float x);
 float o_expf(float x);
 float o_powf(float x, float y);
+float o_atan2f(float y, float x);
+float o_acosf(float x);
+/* op over n values: 0 sin x, 1 cos x, 2 pow(x, y), 3 atan2(x, y) (x the ordinate), 4 acos x */
+void o_math_batch(int op, const float *x, const float *y, float *out, int64_t n);
 
 #ifdef __cplusplus
 }

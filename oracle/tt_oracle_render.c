@@ -1,16 +1,24 @@
 /*
  * tt_oracle_render.c — CPU restatement of the reference path tracer
  * (src/core/RenderKernel.cu:27-218 with DiffImpl.cuh, MetalImpl.cuh, RefrImpl.cuh,
- * HDRKernel.cuh) for image parity of the reference render mode. TEST
- * INFRASTRUCTURE: the checker, never part of the product.
+ * SpecImpl.cuh, CoatImpl.cuh, HDRKernel.cuh) for image parity of the reference
+ * render mode. TEST INFRASTRUCTURE: the checker, never part of the product.
  *
  * Differences from the reference that the GPU reference mode shares:
  *   - random numbers: Philox-4x32-10 addressed by (pixel, sample, bounce), uniform
  *     in (0, 1] like curand_uniform; cuRAND's XORWOW stream is not reproduced;
- *   - sinf / cosf / powf: the functions below (the reference calls CUDA's);
- *   - the miss colour is black (no environment image in this restatement);
+ *   - sinf / cosf / powf / atan2f / acosf: the functions below (the reference calls
+ *     CUDA's);
+ *   - the env lookup clamps its texel index to the last texel: the reference reads
+ *     past the end of its texture there (u2 + v2 * width >= width * height, reached
+ *     when acosf(d.y) / PI exceeds 1 with PI = 3.14156265f < pi, or when the wrapped
+ *     longitude rounds to TWO_PI in the last row);
  *   - the accumulation buffer starts at 0 (the reference memsets it to bytes of 1,
  *     i.e. 2.4e-38 per channel, Renderer.cpp:28).
+ *
+ * Planted mistakes: building with -DO_PLANT=<k> (k > 0) compiles one deliberate
+ * error into this file (see the PLANT() sites below). Only the tests build such a
+ * library, to show that their checks of the sampler and env laws fail on it.
  */
 #include <math.h>
 #include <pthread.h>
@@ -18,6 +26,11 @@
 #include <string.h>
 
 #include "tt_oracle.h"
+
+#ifndef O_PLANT
+#define O_PLANT 0
+#endif
+#define PLANT(k) (O_PLANT == (k))
 
 /* ---------------------------------------------------------------- float math */
 /* Cody–Waite reduction by pi/2 with a three-part constant (12 + 12 + 24 bits),
@@ -85,6 +98,51 @@ float o_powf(float x, float y) {
     return o_exp2f(y * o_log2f(x));
 }
 
+/* atan2 by octant reduction: a = min(|x|, |y|) / max(|x|, |y|) in [0, 1]; above
+ * tan(pi/8), atan a = pi/4 + atan((a - 1) / (a + 1)); atan t by its Taylor series
+ * t - t^3/3 + ... - t^19/19 on |t| <= tan(pi/8); then pi/2 - r when |y| > |x|,
+ * pi - r when x is negative, and the sign of y. pi/4, pi/2 and pi are
+ * each a float plus a float correction, the correction added first. Zeros follow C99 atan2
+ * (atan2(+-0, +0) = +-0, atan2(+-0, -0) = +-pi); NaN propagates. */
+float o_atan2f(float y, float x) {
+    float ax = fabsf(x), ay = fabsf(y);
+    int swap = ay > ax;
+    float num = swap ? ax : ay, den = swap ? ay : ax;
+    float r;
+    if (den == 0.0f) {
+        r = 0.0f;
+    } else {
+        float a = num / den;
+        float hi = 0.0f, lo = 0.0f, t = a;
+        if (a > 0x1.a8279ap-2f) {
+            t = (a - 1.0f) / (a + 1.0f);
+            hi = 0x1.921fb6p-1f;
+            lo = -0x1.777a5cp-26f;
+        }
+        float t2 = t * t;
+        float p = t2 * (-0x1.555556p-2f + t2 * (0x1.99999ap-3f + t2 * (-0x1.24924ap-3f + t2 * (0x1.c71c72p-4f +
+                  t2 * (-0x1.745d18p-4f + t2 * (0x1.3b13b2p-4f + t2 * (-0x1.111112p-4f + t2 * (0x1.e1e1e2p-5f +
+                  t2 * -0x1.af286cp-5f))))))));
+        r = hi + (lo + (t + t * p));
+    }
+    if (swap) r = 0x1.921fb6p+0f + (-0x1.777a5cp-25f - r);
+    if (x < 0.0f || (den == 0.0f && signbit(x))) r = 0x1.921fb6p+1f + (-0x1.777a5cp-24f - r);
+    return signbit(y) ? -r : r;
+}
+
+/* acos by the half-angle identity acos x = 2 atan2(sqrt(1 - x), sqrt(1 + x)) */
+float o_acosf(float x) { return 2.0f * o_atan2f(sqrtf(1.0f - x), sqrtf(1.0f + x)); }
+
+/* op over n values, as the device math hook: 0 sin x, 1 cos x, 2 pow(x, y),
+ * 3 atan2(x, y) (x the ordinate), 4 acos x */
+void o_math_batch(int op, const float *x, const float *y, float *out, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) {
+        float v = x[i];
+        out[i] = op == 0 ? o_sinf(v) : op == 1 ? o_cosf(v) : op == 2 ? o_powf(v, y[i]) : op == 3 ? o_atan2f(v, y[i])
+                                                                                                 : o_acosf(v);
+    }
+}
+
 /* ---------------------------------------------------------------- Vec3f ops
  * operand order of include/math/LinearMath.h */
 typedef struct { float x, y, z; } v3;
@@ -118,7 +176,7 @@ typedef struct {
 } ray3;
 
 /* diffSample, DiffImpl.cuh:12-36 */
-static ray3 diff_sample(const o_material *m, const float u[4], ray3 r, v3 point, v3 nl, v3 *mask) {
+static ray3 diff_sample(const o_material *m, const float u[4], ray3 r, v3 point, v3 n, v3 nl, v3 *mask) {
     float phi = (float)(2.0 * 3.14159265358979323846 * (double)u[0]);
     float r2 = u[1];
     float r2s = sqrtf(r2);
@@ -127,9 +185,12 @@ static ray3 diff_sample(const o_material *m, const float u[4], ray3 r, v3 point,
     v3 uu = vnorm(vcross3(a, w));
     v3 vv = vcross3(w, uu);
     float c, s = o_sincos(phi, &c);
-    v3 d = vadd(vadd(vscale(vscale(uu, c), r2s), vscale(vscale(vv, s), r2s)), vscale(w, sqrtf(1 - r2)));
+    if (PLANT(1)) { float t = c; c = s; s = t; }
+    float wz = sqrtf(1 - r2);
+    if (PLANT(2)) { wz = r2s; r2s = sqrtf(1 - r2); }
+    v3 d = vadd(vadd(vscale(vscale(uu, c), r2s), vscale(vscale(vv, s), r2s)), vscale(w, wz));
     r.d = vnorm(d);
-    r.o = vadd(point, vscale(nl, 0.001f));
+    r.o = vadd(point, vscale(PLANT(15) ? n : nl, 0.001f));
     *mask = vmul(*mask, V(m->albedo[0], m->albedo[1], m->albedo[2]));
     return r;
 }
@@ -138,7 +199,7 @@ static ray3 diff_sample(const o_material *m, const float u[4], ray3 r, v3 point,
 static ray3 metal_sample(const o_material *m, const float u[4], ray3 r, v3 point, v3 n, v3 nl, v3 *mask) {
     float phi = (float)(2.0 * 3.14159265358979323846 * (double)u[0]);
     float r2 = u[1];
-    float cos_t = o_powf(1 - r2, 1.0f / (30.0f + 1));
+    float cos_t = o_powf(1 - r2, 1.0f / (30.0f + (PLANT(3) ? 0 : 1)));
     float sin_t = sqrtf(1 - cos_t * cos_t);
     v3 w = vnorm(vsub3(r.d, vscale(vscale(n, 2.0f), vdot3(n, r.d))));
     v3 a = fabsf(w.x) >= 0.1f ? V(0, 1, 0) : V(1, 0, 0);
@@ -147,7 +208,7 @@ static ray3 metal_sample(const o_material *m, const float u[4], ray3 r, v3 point
     float c, s = o_sincos(phi, &c);
     v3 d = vadd(vadd(vscale(vscale(uu, c), sin_t), vscale(vscale(vv, s), sin_t)), vscale(w, cos_t));
     r.d = vnorm(d);
-    r.o = vadd(point, vscale(nl, 0.0001f));
+    r.o = vadd(point, vscale(PLANT(4) ? n : nl, 0.0001f));
     *mask = vmul(*mask, V(m->albedo[0], m->albedo[1], m->albedo[2]));
     return r;
 }
@@ -164,16 +225,17 @@ static ray3 refr_sample(const o_material *m, const float u[4], ray3 r, v3 point,
         r.o = vadd(point, vscale(nl, 0.001f));
     } else {
         v3 tdir = vscale(r.d, nnt);
-        tdir = vsub3(tdir, vscale(n, (float)(into ? 1 : -1) * (ddn * nnt + sqrtf(cos2t))));
+        tdir = vsub3(tdir, vscale(PLANT(5) ? nl : n, (float)(into ? 1 : -1) * (ddn * nnt + sqrtf(cos2t))));
         tdir = vnorm(tdir);
         float R0 = (nt - nc) * (nt - nc) / (nt + nc) * (nt + nc);
+        if (PLANT(6)) R0 = ((nt - nc) / (nt + nc)) * ((nt - nc) / (nt + nc));
         float c = 1.f - (into ? -ddn : vdot3(tdir, n));
         float Re = R0 + (1.f - R0) * c * c * c * c * c;
         float Tr = 1 - Re;
         float P = .25f + .5f * Re;
-        float RP = Re / P;
+        float RP = PLANT(8) ? Re : Re / P;
         float TP = Tr / (1.f - P);
-        if (u[0] < 0.2f) { /* (double)u < 0.2 */
+        if (u[0] < (PLANT(7) ? 0.25f : 0.2f)) { /* (double)u < 0.2 */
             *mask = vscale(*mask, RP);
             r.d = vnorm(vsub3(r.d, vscale(vscale(n, 2.0f), vdot3(n, r.d))));
             r.o = vadd(point, vscale(nl, 0.001f));
@@ -190,7 +252,7 @@ static ray3 refr_sample(const o_material *m, const float u[4], ray3 r, v3 point,
 static ray3 spec_sample(const o_material *m, ray3 r, v3 point, v3 n, v3 nl, v3 *mask) {
     r.d = vnorm(vsub3(r.d, vscale(vscale(n, vdot3(n, r.d)), 2.0f)));
     r.o = vadd(point, vscale(nl, 0.001f));
-    *mask = vmul(*mask, V(m->albedo[0], m->albedo[1], m->albedo[2]));
+    if (!PLANT(14)) *mask = vmul(*mask, V(m->albedo[0], m->albedo[1], m->albedo[2]));
     return r;
 }
 
@@ -199,7 +261,7 @@ static ray3 spec_sample(const o_material *m, ray3 r, v3 point, v3 n, v3 nl, v3 *
  * coin picks the mirror (specular colour) or a cosine-weighted Lambert bounce
  * (diffuse colour). Uniforms: coin u[0], then u[1], u[2] for the Lambert bounce. */
 static ray3 coat_sample(const o_material *m, const float u[4], ray3 r, v3 point, v3 n, v3 nl, v3 *mask) {
-    const float n_air = 1.000293f, n_coat = 1.291f;
+    const float n_air = PLANT(13) ? 1.291f : 1.000293f, n_coat = PLANT(13) ? 1.000293f : 1.291f;
     v3 reflect_dir = vsub3(r.d, vscale(vscale(n, 2.0f), vdot3(n, r.d)));
     float cos1 = vdot3(n, vscale(r.d, -1.0f));
     float ratio = n_air / n_coat;
@@ -239,6 +301,66 @@ static ray3 coat_sample(const o_material *m, const float u[4], ray3 r, v3 point,
     }
     r.o = vadd(point, vscale(nl, 0.001f));
     return r;
+}
+
+/* The surface interaction of renderKernel (RenderKernel.cu:72-92) after the
+ * emission is added: n = normalize(hit.normal), nl faces the incoming ray
+ * (IntersectKernel.cuh:68-72), then the material's sampler. Returns 0 when the
+ * material ends the path (an emitter: emission only). */
+static int ref_bounce(const o_material *m, const float u[4], ray3 *r, v3 point, v3 nrm, v3 *mask) {
+    v3 n = vnorm(nrm);
+    v3 nl = vdot3(n, r->d) < 0 ? n : vscale(n, -1);
+    if (m->type == 0) *r = diff_sample(m, u, *r, point, n, nl, mask);
+    else if (m->type == 1) *r = metal_sample(m, u, *r, point, n, nl, mask);
+    else if (m->type == 2) *r = refr_sample(m, u, *r, point, n, nl, mask);
+    else if (m->type == 4) *r = spec_sample(m, *r, point, n, nl, mask);
+    else if (m->type == 5) *r = coat_sample(m, u, *r, point, n, nl, mask);
+    else return 0;
+    return 1;
+}
+
+int o_ref_bounce(const o_material *m, const float u[4], const float d[3], const float point[3], const float nrm[3],
+                 float out[9]) {
+    ray3 r;
+    r.o = V(0, 0, 0);
+    r.d = V(d[0], d[1], d[2]);
+    r.tmin = 0.0f;
+    r.tmax = 1e20f;
+    v3 mask = V(1, 1, 1);
+    int cont = ref_bounce(m, u, &r, V(point[0], point[1], point[2]), V(nrm[0], nrm[1], nrm[2]), &mask);
+    out[0] = r.o.x; out[1] = r.o.y; out[2] = r.o.z;
+    out[3] = r.d.x; out[4] = r.d.y; out[5] = r.d.z;
+    out[6] = mask.x; out[7] = mask.y; out[8] = mask.z;
+    return cont;
+}
+
+/* ---------------------------------------------------------------- environment */
+/* hdrSample, HDRKernel.cuh:6-42: longitude atan2f(d.x, d.z) wrapped into [0, TWO_PI),
+ * latitude acosf(d.y); u = lon / TWO_PI, v = lat / PI (the reference's PI, slightly
+ * below pi); nearest texel (int)(u * width) + (int)(v * height) * width, read as a
+ * linear index (u * width reaching width reads the next row's first texel), times
+ * 2 (and the scene's env scale). An index past the last texel is clamped to it. */
+static v3 env_ref(const o_env *e, v3 d, float pi, float two_pi) {
+    if (!e || !e->rgb || e->width <= 0 || e->height <= 0) return V(0, 0, 0);
+    float lx = PLANT(9) ? o_atan2f(d.z, d.x) : o_atan2f(d.x, d.z);
+    lx = lx < 0.f ? lx + two_pi : lx;
+    float ly = o_acosf(d.y);
+    float u = lx / two_pi, v = ly / (PLANT(11) ? two_pi : pi);
+    float fu = u * (float)e->width, fv = v * (float)e->height;
+    int64_t u2 = fu >= 0.0f ? (int64_t)fu : 0, v2 = fv >= 0.0f ? (int64_t)fv : 0;
+    int64_t idx = PLANT(12) ? v2 + u2 * e->height : u2 + v2 * e->width;
+    const int64_t last = (int64_t)e->width * e->height - 1;
+    if (idx > last) idx = last;
+    const float *t = e->rgb + 3 * idx;
+    float s = (PLANT(10) ? 1.0f : 2.0f) * e->scale;
+    return V(t[0] * s, t[1] * s, t[2] * s);
+}
+
+void o_env_lookup(const o_env *e, const float d[3], float pi, float two_pi, float out[3]) {
+    v3 c = env_ref(e, V(d[0], d[1], d[2]), pi, two_pi);
+    out[0] = c.x;
+    out[1] = c.y;
+    out[2] = c.z;
 }
 
 /* ---------------------------------------------------------------- analytic primitives */
@@ -289,6 +411,7 @@ typedef struct {
     int32_t num_geoms;
     const o_camera *cam;
     const o_ref_params *p;
+    const o_env *env;
     float *out;
     int32_t y0, y1;
 } o_rjob;
@@ -349,7 +472,10 @@ static void render_rows(const o_rjob *j) {
                             gsel = k;
                         }
                     }
-                    if ((gsel < 0 && h.tri < 0) || h.t > 1e19f) break; /* miss: black environment */
+                    if ((gsel < 0 && h.tri < 0) || h.t > 1e19f) { /* miss: hdrSample, RenderKernel.cu:65-69 */
+                        acc = vadd(acc, vmul(mask, env_ref(j->env, r.d, p->pi, p->two_pi)));
+                        break;
+                    }
                     v3 point = vadd(r.o, vscale(r.d, h.t));
                     v3 nrm;
                     const o_material *m;
@@ -369,16 +495,9 @@ static void render_rows(const o_rjob *j) {
                                 tn[2] * w + tn[6] * h.u + tn[10] * h.v);
                         m = &j->mats[j->tri_mat[h.tri]];
                     }
-                    v3 n = vnorm(nrm);
-                    v3 nl = vdot3(n, r.d) < 0 ? n : vscale(n, -1);
                     acc = vadd(acc, vmul(mask, V(m->emission[0], m->emission[1], m->emission[2])));
                     draws(pix, sg, (uint32_t)(bounce + 1), p->seed, u);
-                    if (m->type == 0) r = diff_sample(m, u, r, point, nl, &mask);
-                    else if (m->type == 1) r = metal_sample(m, u, r, point, n, nl, &mask);
-                    else if (m->type == 2) r = refr_sample(m, u, r, point, n, nl, &mask);
-                    else if (m->type == 4) r = spec_sample(m, r, point, n, nl, &mask);
-                    else if (m->type == 5) r = coat_sample(m, u, r, point, n, nl, &mask);
-                    else break; /* emitter: emission only */
+                    if (!ref_bounce(m, u, &r, point, nrm, &mask)) break; /* emitter: emission only */
                 }
                 final = vadd(final, vscale(acc, inv_spp));
             }
@@ -395,8 +514,8 @@ static void *render_worker(void *arg) {
 }
 
 int o_render_ref(const o_bvh *b, const float *tri_normals, const int32_t *tri_mat, const o_material *mats,
-                 const o_geometry *geoms, int32_t num_geoms, const o_camera *cam, const o_ref_params *p, int32_t y0,
-                 int32_t y1, float *out_rgb, int nthreads) {
+                 const o_geometry *geoms, int32_t num_geoms, const o_env *env, const o_camera *cam,
+                 const o_ref_params *p, int32_t y0, int32_t y1, float *out_rgb, int nthreads) {
     if (nthreads < 1) nthreads = 1;
     if (nthreads > 256) nthreads = 256;
     if (y0 < 0 || y1 > cam->height || y0 > y1) return -1;
@@ -404,7 +523,7 @@ int o_render_ref(const o_bvh *b, const float *tri_normals, const int32_t *tri_ma
     pthread_t th[256];
     int32_t rows = y1 - y0;
     for (int k = 0; k < nthreads; ++k) {
-        o_rjob jb = {b, tri_normals, tri_mat, mats, geoms, num_geoms, cam, p, out_rgb, y0 + rows * k / nthreads,
+        o_rjob jb = {b, tri_normals, tri_mat, mats, geoms, num_geoms, cam, p, env, out_rgb, y0 + rows * k / nthreads,
                      y0 + rows * (k + 1) / nthreads};
         jobs[k] = jb;
         pthread_create(&th[k], NULL, render_worker, &jobs[k]);

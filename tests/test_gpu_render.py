@@ -26,16 +26,63 @@ def test_device_math_bit_identical_to_oracle(tt, oracle, ctx):
     got = tt.debug_math(ctx, 2, px, py)
     want = np.array([oracle.lib.o_powf(float(a), float(b)) for a, b in zip(px, py)], np.float32)
     assert np.array_equal(got.view(np.int32), want.view(np.int32))
+    # atan2 / acos of the env lookup: unit directions, axes, signed zeros, tiny and huge ratios
+    d = rng.standard_normal((60000, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    ay = np.concatenate([d[:, 0], [0.0, -0.0, 0.0, -0.0, 1.0, -1.0, 1e-30, 3e30, np.nan]]).astype(np.float32)
+    ax = np.concatenate([d[:, 2], [0.0, 0.0, -0.0, -0.0, -0.0, 0.0, -1.0, 1e-20, 1.0]]).astype(np.float32)
+    got = tt.debug_math(ctx, 3, ay, ax)
+    assert np.array_equal(got.view(np.int32), oracle.math(3, ay, ax).view(np.int32))
+    ac = np.concatenate([d[:, 1], np.linspace(-1, 1, 20001), [1.0, -1.0, 1 - 2 ** -24, -1 + 2 ** -24]]).astype(np.float32)
+    got = tt.debug_math(ctx, 4, ac)
+    assert np.array_equal(got.view(np.int32), oracle.math(4, ac).view(np.int32))
 
 
-def _ref_case(tt, oracle, ctx, mesh, mats, cam, spp, depth, seed):
+def _ref_case(tt, oracle, ctx, mesh, mats, cam, spp, depth, seed, env=None, env_scale=1.0):
     b = tt.BVH.build(mesh)
-    s = tt.DeviceScene(ctx, mesh, b, mats)
+    s = tt.DeviceScene(ctx, mesh, b, mats, env=env, env_scale=env_scale)
     img, st = s.render(cam, tt.render_params(spp=spp, max_depth=depth, seed=seed, mode=tt.RENDER_REFERENCE))
     ref = oracle.render_ref(oracle.CompactBVH(b.export_compact()), mesh.arrays(), mats, cam, spp=spp,
-                            max_depth=depth, seed=seed)
+                            max_depth=depth, seed=seed,
+                            env=oracle.Env(env, env_scale) if env is not None else None)
     s.close()
     return img, ref, st
+
+
+def test_reference_render_cornell_with_sky_matches_oracle(tt, oracle, ctx):
+    """The reference's miss path, hdrSample (HDRKernel.cuh:6-42): Cornell box + icosphere
+    (open at the front) under the procedural sky with its sun, 128x128, 16 spp, depth 8:
+    bit-identical to the oracle's restatement of the lookup."""
+    from trenchanttracer_amd import workloads
+    _say("reference render cornell + sky")
+    mesh, mats, cam = workloads.cornell(subdiv=3, width=128, height=128)
+    env = workloads.sky_env(512, 256, sun_radiance=50.0)
+    img, ref, st = _ref_case(tt, oracle, ctx, mesh, mats, cam, 16, 8, 11, env=env, env_scale=0.75)
+    same = np.mean(img.view(np.int32) == ref.view(np.int32))
+    _say(f"  {same * 100:.4f}% bit-identical, mean {img.mean():.4f}")
+    assert same == 1.0
+
+
+def test_reference_render_open_scene_in_sky_matches_oracle(tt, oracle, ctx):
+    """Diffuse, Phong-metal and glass icospheres over a diffuse floor quad in the open sky
+    (most paths end on the env map, many after a glass or metal bounce; directions cover
+    the whole sphere, the poles and the longitude seam): bit-identical to the oracle.
+    The map is 37 x 23 so texel boundaries fall at odd positions."""
+    from trenchanttracer_amd import workloads
+    _say("reference render open scene + sky")
+    mesh = tt.Mesh.icosphere(3, 0.45, (-0.9, 0.0, 0.0), mat=0)
+    mesh.append(tt.Mesh.icosphere(3, 0.45, (0.0, 0.0, 0.3), mat=1))
+    mesh.append(tt.Mesh.icosphere(3, 0.45, (0.9, 0.0, 0.0), mat=2))
+    mesh.append(tt.Mesh.quad((-4, -0.46, 4), (4, -0.46, 4), (-4, -0.46, -4), mat=3))
+    mats = [tt.material(tt.MAT_DIFFUSE, (0.8, 0.3, 0.2)), tt.material(tt.MAT_CONDUCTOR, (0.9, 0.9, 0.7)),
+            tt.material(tt.MAT_DIELECTRIC, (1, 1, 1), ior=1.4), tt.material(tt.MAT_DIFFUSE, (0.5, 0.5, 0.5))]
+    cam = tt.camera((0.0, 0.6, 3.2), (0.0, -0.2, -1.0), fov=(70, 55), width=160, height=120)
+    env = workloads.sky_env(37, 23, sun_radiance=20.0, sun_diam_deg=12.0)
+    img, ref, st = _ref_case(tt, oracle, ctx, mesh, mats, cam, 8, 10, 4, env=env)
+    same = np.mean(img.view(np.int32) == ref.view(np.int32))
+    _say(f"  {same * 100:.4f}% bit-identical, mean {img.mean():.4f}")
+    assert same == 1.0
+    assert img.mean() > 0.1
 
 
 def test_reference_render_cornell_matches_oracle(tt, oracle, ctx):

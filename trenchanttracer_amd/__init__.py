@@ -483,7 +483,8 @@ def gather64_gbps(ctx, buf, nbytes, fetches=1 << 26, seed=1):
 
 
 def debug_math(ctx, op, x, y=None):
-    """Device sampler math (op 0 sin, 1 cos, 2 pow) on host arrays, through device buffers."""
+    """Device sampler math (op 0 sin, 1 cos, 2 pow(x, y), 3 atan2(x, y), 4 acos) on host arrays,
+    through device buffers."""
     x = np.ascontiguousarray(x, dtype=np.float32)
     y = np.ascontiguousarray(x if y is None else y, dtype=np.float32)
     bx, by = DeviceBuffer.from_array(ctx, x), DeviceBuffer.from_array(ctx, y)

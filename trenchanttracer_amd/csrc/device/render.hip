@@ -455,15 +455,24 @@ struct ShadeArgs {
 
 // hdrSample (HDRKernel.cuh:6-42): nearest texel, x2; the row index is clamped
 // (the reference reads one row past the end for d.y near -1 with its PI).
+// hdrSample (HDRKernel.cuh:6-42): longitude atan2(d.x, d.z) wrapped into [0, TWO_PI),
+// latitude acos(d.y); u = lon / TWO_PI, v = lat / PI (the reference's PI, slightly
+// below pi); nearest texel (int)(u * width) + (int)(v * height) * width read as a
+// linear index, so u * width reaching width reads the next row's first texel as the
+// reference does; times 2 and the scene's env scale. Where the reference reads past
+// the end of its texture (v * height reaching height for d.y near -1, or the wrap in
+// the last row) the index is clamped to the last texel.
 __device__ f3 env_reference(const SceneView &sv, f3 d, float pi, float two_pi) {
     if (!sv.env) return F3(0, 0, 0);
-    float lx = atan2f(d.x, d.z);
+    float lx = tm_atan2(d.x, d.z);
     lx = lx < 0.f ? lx + two_pi : lx;
-    float ly = acosf(d.y);
-    float u = lx / two_pi, v = ly / pi;
-    int u2 = min(max((int)(u * sv.env_w), 0), sv.env_w - 1);
-    int v2 = min(max((int)(v * sv.env_h), 0), sv.env_h - 1);
-    float4 t = sv.env[u2 + v2 * sv.env_w];
+    const float ly = tm_acos(d.y);
+    const float u = lx / two_pi, v = ly / pi;
+    const float fu = u * (float)sv.env_w, fv = v * (float)sv.env_h;
+    const int64_t u2 = fu >= 0.0f ? (int64_t)fu : 0, v2 = fv >= 0.0f ? (int64_t)fv : 0;
+    const int64_t last = (int64_t)sv.env_w * sv.env_h - 1;
+    const int64_t idx = min(u2 + v2 * (int64_t)sv.env_w, last);
+    const float4 t = sv.env[idx];
     return F3(t.x, t.y, t.z) * (2.0f * sv.env_scale);
 }
 
@@ -1436,7 +1445,9 @@ __global__ void k_debug_math(int op, const float *x, const float *y, float *out,
     else if (op == 1) {
         tm_sincos(x[i], &c);
         out[i] = c;
-    } else out[i] = tm_pow(x[i], y[i]);
+    } else if (op == 2) out[i] = tm_pow(x[i], y[i]);
+    else if (op == 3) out[i] = tm_atan2(x[i], y[i]);
+    else out[i] = tm_acos(x[i]);
 }
 
 // Host-side camera basis: the same float operations as the device code and the
@@ -2265,7 +2276,7 @@ tt_status tt_debug_guide_sample(tt_ctx *ctx, const float *raw, int64_t n, uint32
 }
 
 tt_status tt_debug_math(tt_ctx *ctx, int32_t op, const float *x, const float *y, float *out, int64_t n) {
-    if (!ctx || op < 0 || op > 2 || n < 0 || (n > 0 && (!x || !out || (op == 2 && !y))))
+    if (!ctx || op < 0 || op > 4 || n < 0 || (n > 0 && (!x || !out || ((op == 2 || op == 3) && !y))))
         return fail(TT_ERR_INVALID, "tt_debug_math: bad arguments");
     if (n == 0) return TT_OK;
     TT_HIPR(hipSetDevice(ctx->device), "hipSetDevice");

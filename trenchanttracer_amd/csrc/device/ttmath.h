@@ -4,6 +4,10 @@
 //           Taylor polynomials of sin to r^9 and cos to r^10 on |r| <= pi/4
 //   log2:   x = m 2^e, m in [sqrt(1/2), sqrt(2)), ln m = 2 atanh((m-1)/(m+1)) to f^11
 //   exp2:   2^n e^(g ln 2), |g| <= 1/2, Taylor to t^8
+//   atan2:  octant reduction a = min(|x|,|y|) / max(|x|,|y|); above tan(pi/8)
+//           atan a = pi/4 + atan((a-1)/(a+1)); Taylor series of atan to t^19 on
+//           |t| <= tan(pi/8); pi/4, pi/2, pi as a float plus a float correction
+//   acos:   half-angle identity acos x = 2 atan2(sqrt(1-x), sqrt(1+x))
 // Coefficients are 1/n! and 1/(2k+1) rounded to float.
 #pragma once
 
@@ -84,5 +88,36 @@ __device__ __forceinline__ float tm_pow(float x, float y) {
     if (x == 0.0f) return 0.0f;
     return tm_exp2(y * tm_log2(x));
 }
+
+// C99 atan2 special values for zeros (atan2(+-0, +0) = +-0, atan2(+-0, -0) = +-pi);
+// NaN propagates. Within 2.5 ulp of the exact value.
+__device__ __forceinline__ float tm_atan2(float y, float x) {
+    const float ax = fabsf(x), ay = fabsf(y);
+    const bool swap = ay > ax;
+    const float num = swap ? ax : ay, den = swap ? ay : ax;
+    float r;
+    if (den == 0.0f) {
+        r = 0.0f;
+    } else {
+        const float a = num / den;
+        float hi = 0.0f, lo = 0.0f, t = a;
+        if (a > 0x1.a8279ap-2f) {
+            t = (a - 1.0f) / (a + 1.0f);
+            hi = 0x1.921fb6p-1f;
+            lo = -0x1.777a5cp-26f;
+        }
+        const float t2 = t * t;
+        const float p = t2 * (-0x1.555556p-2f + t2 * (0x1.99999ap-3f + t2 * (-0x1.24924ap-3f + t2 * (0x1.c71c72p-4f +
+                        t2 * (-0x1.745d18p-4f + t2 * (0x1.3b13b2p-4f + t2 * (-0x1.111112p-4f + t2 * (0x1.e1e1e2p-5f +
+                        t2 * -0x1.af286cp-5f))))))));
+        r = hi + (lo + (t + t * p));
+    }
+    if (swap) r = 0x1.921fb6p+0f + (-0x1.777a5cp-25f - r);
+    if (x < 0.0f || (den == 0.0f && (__float_as_uint(x) >> 31))) r = 0x1.921fb6p+1f + (-0x1.777a5cp-24f - r);
+    return (__float_as_uint(y) >> 31) ? -r : r;
+}
+
+// Within 3.5 ulp of the exact value on [-1, 1]; NaN outside.
+__device__ __forceinline__ float tm_acos(float x) { return 2.0f * tm_atan2(sqrtf(1.0f - x), sqrtf(1.0f + x)); }
 
 }  // namespace tt
