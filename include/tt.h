@@ -281,9 +281,11 @@ tt_status tt_env_export_cdf(const tt_scene *s, float *marg, float *cond, double 
  * share node fetches (on the 1M-soup traversal workload the 4.1M incoherent rays trace
  * 18% faster; camera rays, which share one origin, keep their order). Needs n < 2^31
  * and a context-owned workspace of 16 bytes per ray plus the sort's temporary storage,
- * allocated on first use and grown when a larger n arrives; calls with reorder = 1 on
- * different streams of one context must not overlap, and are not meant for HIP graph
- * capture (the sort clears its state with memset operations). */
+ * allocated on first use and grown when a larger n arrives; a call with reorder = 1
+ * waits (on its stream) for the previous such call of the context to finish with that
+ * workspace, whatever stream it ran on. Not meant for HIP graph capture (the sort
+ * clears its state with memset operations). The Morton cells span the scene's mesh
+ * bounds, or the root node's box when the scene was created without a mesh. */
 typedef struct tt_trace_cfg {
     int32_t variant;
     int32_t lds_stack;

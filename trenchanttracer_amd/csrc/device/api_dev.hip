@@ -75,6 +75,12 @@ tt_status tt_ctx_create(int32_t device, tt_ctx **out) {
         delete c;
         return fail(TT_ERR_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
     }
+    e = hipEventCreateWithFlags(&c->reorder_done, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        hipStreamDestroy(c->stream);
+        delete c;
+        return fail(TT_ERR_DEVICE, std::string("hipEventCreate: ") + hipGetErrorString(e));
+    }
     *out = c;
     return TT_OK;
 }
@@ -85,6 +91,7 @@ void tt_ctx_destroy(tt_ctx *ctx) {
     hipStreamSynchronize(ctx->stream);
     if (ctx->scratch) hipFree(ctx->scratch);
     if (ctx->reorder_ws) hipFree(ctx->reorder_ws);
+    if (ctx->reorder_done) hipEventDestroy(ctx->reorder_done);
     hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -290,6 +297,28 @@ tt_status tt_scene_create(tt_ctx *ctx, const tt_mesh *mesh, const tt_bvh *bvh, c
     };
     up((void **)&s->nodes, f.dev_nodes.data(), f.dev_nodes.size() * 4, "scene nodes");
     up((void **)&s->tris, f.dev_tris.data(), f.dev_tris.size() * 4, "scene triangles");
+    if (!mesh && s->num_nodes > 0) {
+        // no mesh: the scene box (ray-reordering keys) is the union of the root's child
+        // boxes, words {c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y}, {c1 ...}, {c0.lo.z, c0.hi.z,
+        // c1.lo.z, c1.hi.z}; an empty child box (lo > hi) adds nothing
+        float w[12];
+        std::memcpy(w, f.dev_nodes.data(), sizeof(w));
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int ch = 0; ch < 2; ++ch) {
+            const float cl[3] = {w[4 * ch + 0], w[4 * ch + 2], w[8 + 2 * ch]};
+            const float chh[3] = {w[4 * ch + 1], w[4 * ch + 3], w[9 + 2 * ch]};
+            if (!(cl[0] <= chh[0] && cl[1] <= chh[1] && cl[2] <= chh[2])) continue;
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = std::min(lo[k], cl[k]);
+                hi[k] = std::max(hi[k], chh[k]);
+            }
+        }
+        if (lo[0] <= hi[0])
+            for (int k = 0; k < 3; ++k) {
+                s->bmin[k] = lo[k];
+                s->bmax[k] = hi[k];
+            }
+    }
     if (mesh && st == TT_OK) {
         // per-triangle shading normals (3 float4) and material ids, by original triangle id
         const Mesh &m = mesh->m;
@@ -456,9 +485,16 @@ static tt_status trace_impl(tt_scene *s, const tt_ray *rays, int64_t n, tt_hit *
             if (st != TT_OK) return st;
             ctx->reorder_ws_bytes = need;
         }
+        // the workspace is the context's: a call on another stream waits for the last
+        // launch that read it
+        if (ctx->reorder_pending) TT_HIP(hipStreamWaitEvent(hs, ctx->reorder_done, 0), "hipStreamWaitEvent");
         TT_HIP(reorder_rays(rays, n, s->bmin, s->bmax, ctx->reorder_ws, sort_bytes, hs, &a.order), "trace: reorder");
     }
     TT_HIP(launch_trace(a, c.variant, any, c.lds_stack, c.block, c.xcd_order, c.refill, s->ctx->num_cus, hs), "trace launch");
+    if (c.reorder) {
+        TT_HIP(hipEventRecord(s->ctx->reorder_done, hs), "hipEventRecord");
+        s->ctx->reorder_pending = true;
+    }
     return TT_OK;
 }
 

@@ -18,6 +18,8 @@ struct tt_ctx {
     size_t scratch_bytes = 0;
     void *reorder_ws = nullptr;  // ray-reordering keys, permutation and sort storage
     size_t reorder_ws_bytes = 0;
+    hipEvent_t reorder_done = nullptr;  // recorded after the last launch that used reorder_ws
+    bool reorder_pending = false;       // reorder_done has been recorded at least once
 };
 
 struct tt_scene {
