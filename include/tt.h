@@ -306,6 +306,16 @@ tt_status tt_trace_closest(tt_scene *s, const tt_ray *rays, int64_t n, tt_hit *h
  * hit_flags[i] = 1 if some triangle is hit in (tmin, tmax). */
 tt_status tt_trace_any(tt_scene *s, const tt_ray *rays, int64_t n, int32_t *hit_flags,
                        const tt_trace_cfg *cfg, void *stream);
+/* No counterpart in the reference: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same results and arguments as tt_trace_closest, but with cfg->reorder = 1 the
+ * Morton keys, permutation and sort storage live in a fixed 72-MiB device array of the
+ * library (one copy per GPU, held until the process ends) instead of the context's
+ * workspace, and the call allocates nothing. At most 4,194,304 rays per call (larger n
+ * returns TT_ERR_INVALID). Every call on one GPU shares that array: calls that overlap
+ * on one GPU (any context, any stream) race, so callers order them. */
+tt_status tt_trace_closest_noalloc(tt_scene *s, const tt_ray *rays, int64_t n, tt_hit *hits,
+                                   const tt_trace_cfg *cfg, void *stream);
 /* Host-buffer convenience wrappers (copy in, trace, copy out, synchronise). */
 tt_status tt_trace_closest_host(tt_scene *s, const tt_ray *rays, int64_t n, tt_hit *hits,
                                 const tt_trace_cfg *cfg);

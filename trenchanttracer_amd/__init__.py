@@ -579,6 +579,13 @@ class DeviceScene:
                                    C.byref(cfg) if cfg is not None else None,
                                    C.c_void_p(stream) if stream else None))
 
+    def trace_closest_noalloc(self, rays, n, hits, cfg=None, stream=None):
+        """As trace_closest; with cfg.reorder = 1 the reorder workspace is the library's fixed
+        device array (no allocation; at most 4,194,304 rays per call)."""
+        check(lib.tt_trace_closest_noalloc(self._h, _dev_ptr(rays), int(n), _dev_ptr(hits),
+                                           C.byref(cfg) if cfg is not None else None,
+                                           C.c_void_p(stream) if stream else None))
+
     def trace_any(self, rays, n, flags, cfg=None, stream=None):
         check(lib.tt_trace_any(self._h, _dev_ptr(rays), int(n), _dev_ptr(flags),
                                C.byref(cfg) if cfg is not None else None,

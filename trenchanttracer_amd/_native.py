@@ -171,6 +171,7 @@ PROTOS = {
     "tt_debug_env_sample": (C.c_int, [P, I64, U32, P]),
     "tt_debug_env_invert": (C.c_int, [P, P, I64, P]),
     "tt_trace_closest": (C.c_int, [P, P, I64, P, C.POINTER(tt_trace_cfg), P]),
+    "tt_trace_closest_noalloc": (C.c_int, [P, P, I64, P, C.POINTER(tt_trace_cfg), P]),
     "tt_trace_any": (C.c_int, [P, P, I64, P, C.POINTER(tt_trace_cfg), P]),
     "tt_trace_closest_host": (C.c_int, [P, P, I64, P, C.POINTER(tt_trace_cfg)]),
     "tt_trace_any_host": (C.c_int, [P, P, I64, P, C.POINTER(tt_trace_cfg)]),

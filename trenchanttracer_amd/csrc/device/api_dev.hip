@@ -3,6 +3,7 @@
 #include <cmath>
 #include <initializer_list>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -51,6 +52,30 @@ tt_status resolve_cfg(const tt_trace_cfg *cfg, tt_trace_cfg &out) {
     return TT_OK;
 }
 
+// Workspace of tt_trace_closest_noalloc: a file-scope device array, so that the entry
+// point allocates nothing. Each GPU the library runs on holds its own copy at its own
+// address until the process ends. In host code the array's name is not a device
+// address: the address of the current device's copy comes from hipGetSymbolAddress
+// (once per device, when a context is created on it) and only that address is used.
+constexpr int64_t kNoallocMaxRays = (int64_t)1 << 22;
+constexpr size_t kNoallocScratchBytes = (size_t)72 << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+static void *g_scratch_ptr[kMaxDevices];
+static hipError_t g_scratch_err[kMaxDevices];
+static std::once_flag g_scratch_once[kMaxDevices];
+
+// Called with `device` current.
+static void scratch_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return;
+    std::call_once(g_scratch_once[device], [device] {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        g_scratch_err[device] = e;
+        g_scratch_ptr[device] = e == hipSuccess ? ptr : nullptr;
+    });
+}
+
 }  // namespace tt
 
 extern "C" {
@@ -81,6 +106,7 @@ tt_status tt_ctx_create(int32_t device, tt_ctx **out) {
         delete c;
         return fail(TT_ERR_DEVICE, std::string("hipEventCreate: ") + hipGetErrorString(e));
     }
+    scratch_setup(device);
     *out = c;
     return TT_OK;
 }
@@ -452,7 +478,7 @@ tt_status tt_scene_info(const tt_scene *s, int64_t *num_nodes, int64_t *num_tris
 }
 
 static tt_status trace_impl(tt_scene *s, const tt_ray *rays, int64_t n, tt_hit *hits, int32_t *flags,
-                            const tt_trace_cfg *cfg, void *stream, bool any) {
+                            const tt_trace_cfg *cfg, void *stream, bool any, bool noalloc = false) {
     if (!s) return fail(TT_ERR_INVALID, "trace: null scene");
     if (n < 0 || (n > 0 && (!rays || (any ? !flags : !hits)))) return fail(TT_ERR_INVALID, "trace: bad buffers");
     if (n >= (int64_t)1 << 40) return fail(TT_ERR_INVALID, "trace: too many rays");
@@ -470,7 +496,20 @@ static tt_status trace_impl(tt_scene *s, const tt_ray *rays, int64_t n, tt_hit *
     a.num_nodes = (int32_t)s->num_nodes;
     hipStream_t hs = stream ? (hipStream_t)stream : s->ctx->stream;
     TT_HIP(hipSetDevice(s->ctx->device), "hipSetDevice");
-    if (c.reorder) {
+    if (c.reorder && noalloc) {
+        // the device array's copy on this device; calls that overlap on one GPU share it
+        const int dev = s->ctx->device;
+        if (dev >= kMaxDevices || !g_scratch_ptr[dev])
+            return fail(TT_ERR_DEVICE, std::string("trace (no-alloc): workspace address unavailable: ") +
+                                           hipGetErrorString(dev < kMaxDevices ? g_scratch_err[dev] : hipErrorInvalidDevice));
+        size_t need = 0, sort_bytes = 0;
+        TT_HIP(reorder_workspace_bytes(n, &need, &sort_bytes), "trace: reorder workspace size");
+        if (need > kNoallocScratchBytes)
+            return fail(TT_ERR_INVALID, "trace (no-alloc): workspace of " + std::to_string(need) +
+                                            " bytes exceeds the fixed " + std::to_string(kNoallocScratchBytes));
+        TT_HIP(reorder_rays(rays, n, s->bmin, s->bmax, g_scratch_ptr[dev], sort_bytes, hs, &a.order),
+               "trace: reorder");
+    } else if (c.reorder) {
         if (n >= ((int64_t)1 << 31)) return fail(TT_ERR_INVALID, "trace: reorder supports at most 2^31 - 1 rays per call");
         tt_ctx *ctx = s->ctx;
         size_t need = 0, sort_bytes = 0;
@@ -491,7 +530,7 @@ static tt_status trace_impl(tt_scene *s, const tt_ray *rays, int64_t n, tt_hit *
         TT_HIP(reorder_rays(rays, n, s->bmin, s->bmax, ctx->reorder_ws, sort_bytes, hs, &a.order), "trace: reorder");
     }
     TT_HIP(launch_trace(a, c.variant, any, c.lds_stack, c.block, c.xcd_order, c.refill, s->ctx->num_cus, hs), "trace launch");
-    if (c.reorder) {
+    if (c.reorder && !noalloc) {
         TT_HIP(hipEventRecord(s->ctx->reorder_done, hs), "hipEventRecord");
         s->ctx->reorder_pending = true;
     }
@@ -506,6 +545,14 @@ tt_status tt_trace_closest(tt_scene *s, const tt_ray *rays, int64_t n, tt_hit *h
 tt_status tt_trace_any(tt_scene *s, const tt_ray *rays, int64_t n, int32_t *hit_flags, const tt_trace_cfg *cfg,
                        void *stream) {
     return trace_impl(s, rays, n, nullptr, hit_flags, cfg, stream, true);
+}
+
+tt_status tt_trace_closest_noalloc(tt_scene *s, const tt_ray *rays, int64_t n, tt_hit *hits, const tt_trace_cfg *cfg,
+                                   void *stream) {
+    if (n > kNoallocMaxRays)
+        return fail(TT_ERR_INVALID, "tt_trace_closest_noalloc: at most " + std::to_string(kNoallocMaxRays) +
+                                        " rays per call (got " + std::to_string(n) + ")");
+    return trace_impl(s, rays, n, hits, nullptr, cfg, stream, false, true);
 }
 
 static tt_status host_trace(tt_scene *s, const tt_ray *rays, int64_t n, void *out, size_t out_elem,
