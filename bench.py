@@ -127,12 +127,86 @@ def main():
         d_hh.free()
     log(f"halves: {halves}")
 
-    # measured node-fetch roofline: random 64-byte gathers over a buffer the size
-    # of the device BVH (nodes + triangle records)
+    # the traversal's own loop replayed without its box and triangle arithmetic, in
+    # the headline's launch form, after the same key pass and sort
+    # (tt_bench_fetch_replay); the whole set and each half
+    # (the exact loop's counting launch gives the algorithmic bytes: its node visits
+    # and triangle tests are the single-lane traversal's)
+    exact_cfg = tt.trace_cfg(variant=0, lds_stack=args.lds_stack, block=args.block, refill=args.refill,
+                             reorder=args.reorder)
+    replay = scene.fetch_replay(d_rays, n, cfg, reps=5)
+    algo = scene.fetch_replay(d_rays, n, exact_cfg, reps=1)
+    log(f"fetch replay of the whole set: {replay['ms']:.3f} ms (call {kmean:.3f} ms)")
+    for name, lo, hi in (("coherent", 0, n_prim), ("incoherent", n_prim, n)):
+        d_half = tt.DeviceBuffer.from_array(ctx, np.ascontiguousarray(rays[lo:hi]))
+        rh = scene.fetch_replay(d_half, hi - lo, cfg, reps=5)
+        ah = scene.fetch_replay(d_half, hi - lo, exact_cfg, reps=1)
+        d_half.free()
+        halves[name].update({
+            "algorithmic_bytes_per_ray": round(ah["bytes"] / (hi - lo), 1),
+            "effective_gbps": round(ah["bytes"] / (halves[name]["ms"] * 1e-3) / 1e9, 1),
+            "node_visits_per_ray": round(ah["node_fetches"] / (hi - lo), 2),
+            "tri_tests_per_ray": round(ah["tri_fetches"] / (hi - lo), 2),
+            "replay_ms": round(rh["ms"], 4),
+            "replay_over_call": round(rh["ms"] / halves[name]["ms"], 4)})
+    log(f"halves with their replays: {halves}")
+
+    # fetch-throughput ceiling: independent random 64-byte reads over a buffer the size
+    # of the device tree (the same probe the counter runs measure)
+    from trenchanttracer_amd.profile_trace import GATHER_FETCHES
     probe = tt.DeviceBuffer(ctx, info["device_bytes"])
-    gather_gbps = max(tt.gather64_gbps(ctx, probe, info["device_bytes"], seed=s) for s in (1, 2, 3))
+    probe.zero()
+    gather_ms = []
+    for _ in range(3):
+        gbps = tt.gather64_gbps(ctx, probe, info["device_bytes"], fetches=GATHER_FETCHES, seed=1)
+        gather_ms.append(GATHER_FETCHES * 64 / (gbps * 1e9) * 1e3)
     probe.free()
-    log(f"random 64-byte gather over {info['device_bytes'] / 1e6:.0f} MB: {gather_gbps:.0f} GB/s")
+    gather_ms = float(np.median(gather_ms))
+    log(f"random 64-byte gather over {info['device_bytes'] / 1e6:.0f} MB: {gather_ms:.3f} ms for 2^26 reads")
+
+    traffic, traffic_src = pmc_call_traffic(args)
+    if traffic is None:
+        log(f"live counter runs unavailable ({traffic_src}); using the committed counter profile")
+        traffic, traffic_src = committed_traffic(args)
+    bpr = algo["bytes"] / n
+    algo_gbps = algo["bytes"] / (kmean * 1e-3) / 1e9
+    # node-fetch roofline: the call's beyond-L2 bytes per second against those of the
+    # random-gather probe (both from the same counter runs); and the call's own loop
+    # replayed without its arithmetic (tt_bench_fetch_replay)
+    node_fetch = {"replay_ms": round(replay["ms"], 4), "call_ms": round(kmean, 4),
+                  "replay_over_call": round(replay["ms"] / kmean, 4),
+                  "gather_ms": round(gather_ms, 4)}
+    if traffic is not None and traffic.get("gather_bytes"):
+        g_rate = traffic["gather_bytes"] / (gather_ms * 1e-3) / 1e9
+        c_rate = traffic["call_bytes"] / (kmean * 1e-3) / 1e9
+        node_fetch.update({"gather_traffic_gbps": round(g_rate, 1), "call_traffic_gbps": round(c_rate, 1),
+                           "frac": round(c_rate / g_rate, 4)})
+    roofline = None
+    if traffic is not None:
+        achieved = traffic["call_bytes"] / (kmean * 1e-3) / 1e9
+        roofline = {
+            "bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
+            "frac": round(achieved / 8000.0, 4),
+            "traffic": round(traffic["call_bytes"] / 1e9, 3),
+            "traffic_unit": "GB per tt_trace_closest call (HBM side of L2: FETCH_SIZE + WRITE_SIZE)",
+            "traffic_source": traffic_src,
+            "traffic_traversal_kernel": round(traffic["trace_bytes"] / 1e9, 3),
+            "timed_span": "whole tt_trace_closest call (Morton keys + radix sort + traversal), HIP events"
+                          if args.reorder else "traversal launch, HIP events",
+        }
+    else:
+        roofline = {"bound": "hbm", "achieved": None, "peak": 8000.0, "unit": "GB/s", "frac": None,
+                    "traffic": None, "traffic_source": traffic_src}
+    roofline.update({
+        "algorithmic_bytes_per_ray": round(bpr, 1),
+        "effective_gbps": round(algo_gbps, 1),
+        "algorithmic_over_traffic": round(algo["bytes"] / traffic["call_bytes"], 3) if traffic else None,
+        "node_visits_per_ray": round(algo["node_fetches"] / n, 2),
+        "tri_tests_per_ray": round(algo["tri_fetches"] / n, 2),
+        "speculative_node_visits_per_ray": round(replay["node_fetches"] / n, 2),
+        "node_fetch": node_fetch,
+        "per_half": halves,
+    })
 
     result = {
         "metric": "Mrays/s closest-hit",
@@ -156,22 +230,22 @@ def main():
         },
         "kernel_ms": round(kmean, 4),
         "other_variant": spec,
-        "roofline": None,
+        "roofline": roofline,
         "cpu_baseline": None,
     }
 
     if not args.no_cpu_baseline:
         # cpu_baseline leg: the oracle (single-lane restatement of the reference
         # traversal) timed on host cores over a strided sample of the same rays;
-        # its per-ray visit counts give the algorithmic bytes, and its hits check
-        # the GPU's on the sample.
+        # its hits check the GPU's on the sample, and its visit counts the
+        # algorithmic bytes of the node-fetch probe.
         from oracle import oracle as O
 
         stride = args.cpu_sample_stride
         cb = O.CompactBVH(bvh.export_compact())
         cores = min(16, os.cpu_count() or 1)
-        # the two halves traced separately, so that each half's visit counts give its
-        # own algorithmic bytes per ray
+        # the two halves traced separately (the oracle's visit counts cross-check the
+        # recorded fetch counts of the node-fetch probe)
         parts, idxs, st = [], [], {"seconds": 0.0, "inner": 0, "tris": 0}
         for name, lo, hi in (("coherent", 0, n_prim), ("incoherent", n_prim, n)):
             ih = np.arange(lo, hi, stride)
@@ -180,50 +254,24 @@ def main():
             idxs.append(ih)
             for key in st:
                 st[key] += st_h[key]
-            bh = workloads.bytes_per_ray(st_h["inner"], st_h["tris"], len(ih))
-            gb = bh * halves[name]["rays"] / (halves[name]["ms"] * 1e-3) / 1e9
-            halves[name].update({"bytes_per_ray": round(bh, 1), "achieved_gbps": round(gb, 1),
-                                 "frac": round(gb / 8000.0, 4),
-                                 "node_visits_per_ray": round(st_h["inner"] / len(ih), 2),
-                                 "tri_tests_per_ray": round(st_h["tris"] / len(ih), 2)})
         idx = np.concatenate(idxs)
         sample = rays[idx]
         oh = np.concatenate(parts)
         cpu_mrays = len(sample) / st["seconds"] / 1e6
-        bpr = workloads.bytes_per_ray(st["inner"], st["tris"], len(sample))
+        bpr_o = workloads.bytes_per_ray(st["inner"], st["tris"], len(sample))
         gpu_ids = hits[idx, 0]
         o_ids = O.hit_ids(oh)
         mism = int((gpu_ids != o_ids).sum())
         exact = bool(np.array_equal(hits[idx].view(np.int32), oh.view(np.int32)))
-        achieved = bpr * n / (kmean * 1e-3) / 1e9
-        # traffic per launch of the traversal kernel in the default configuration
-        # (speculative, LDS 16, block 256, refill 32, reorder 1), from rocprofv3 --pmc
-        # FETCH_SIZE and WRITE_SIZE runs of trenchanttracer_amd.profile_trace scaled by
-        # a 256 MiB copy's counts (reads x 1.99976, writes x 1.0): 6,597,741 KiB
-        # fetched, 214,109 KiB written (the hits, scattered to the rays' own indices)
-        default_cfg = (args.variant, args.lds_stack, args.block, args.refill, args.reorder,
-                       args.ntris) == (1, 16, 256, 32, 1, 1_000_000)
-        traffic = round((6_597_740.5 * 1.99976 + 214_108.7) * 1024 / 1e9, 2) if default_cfg else None
-        result["roofline"] = {
-            "bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
-            "frac": round(achieved / 8000.0, 4), "traffic": traffic, "traffic_unit": "GB per launch",
-            "timed_span": "whole tt_trace_closest call (Morton keys + radix sort + traversal)" if args.reorder
-                          else "traversal launch",
-            "bytes_per_ray": round(bpr, 1),
-            "node_fetch_roofline_gbps": round(gather_gbps, 1),
-            "node_fetch_frac": round(achieved / gather_gbps, 4),
-            "node_visits_per_ray": round(st["inner"] / len(sample), 2),
-            "tri_tests_per_ray": round(st["tris"] / len(sample), 2),
-            "per_half": halves,
-        }
         result["cpu_baseline"] = {
             "value": round(cpu_mrays, 3), "unit": "Mrays/s", "cores": cores, "kind": "port",
             "sample": f"every {stride}th ray of the same set ({len(sample)} rays), oracle single-lane traversal, "
                       f"{st['seconds']:.2f}s wall on {cores} threads",
         }
-        result["parity_sample"] = {"rays": int(len(sample)), "hit_id_mismatches": mism, "bit_exact": exact}
+        result["parity_sample"] = {"rays": int(len(sample)), "hit_id_mismatches": mism, "bit_exact": exact,
+                                   "oracle_algorithmic_bytes_per_ray": round(bpr_o, 1)}
         log(f"cpu baseline {cpu_mrays:.2f} Mrays/s on {cores} threads; sample mismatches {mism}; "
-            f"bytes/ray {bpr:.1f}; achieved {achieved:.0f} GB/s")
+            f"oracle bytes/ray {bpr_o:.1f} (probe {bpr:.1f})")
 
     scene.close()
     d_rays.free()
@@ -235,12 +283,94 @@ def main():
     ctx.close()
 
 
+PMC_COPY_BYTES = 256 << 20   # the calibration copy of trenchanttracer_amd.profile_trace
+
+
+def pmc_call_traffic(args, timeout=240):
+    """HBM-side bytes of one tt_trace_closest call, measured in two child runs of
+    trenchanttracer_amd.profile_trace (the same workload and configuration; one call,
+    then a 256 MiB device copy) under rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE.
+    Each counter is scaled by what it reads for the copy (known 256 MiB read and
+    written). Returns ({call_bytes, trace_bytes}, source) or (None, reason)."""
+    import csv
+    import glob
+    import shutil
+    import signal
+    import subprocess
+    import tempfile
+    rocprof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(rocprof):
+        return None, "rocprofv3 not found"
+    prog = [sys.executable, "-m", "trenchanttracer_amd.profile_trace", "--variant", str(args.variant),
+            "--lds-stack", str(args.lds_stack), "--block", str(args.block), "--refill", str(args.refill),
+            "--reorder", str(args.reorder), "--ntris", str(args.ntris)]
+    env = dict(os.environ, TMPDIR="/tmp", PYTHONPATH=ROOT + os.pathsep + os.environ.get("PYTHONPATH", ""))
+    per = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        out = tempfile.mkdtemp(prefix="tt_pmc_", dir="/tmp")
+        cmd = [rocprof, "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "run", "--"] + prog
+        log(f"counter run: {counter}")
+        p = subprocess.Popen(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                             start_new_session=True)
+        try:
+            text, _ = p.communicate(timeout=timeout)
+        except subprocess.TimeoutExpired:
+            os.killpg(p.pid, signal.SIGKILL)
+            p.communicate()
+            shutil.rmtree(out, ignore_errors=True)
+            return None, f"{counter} run exceeded {timeout} s"
+        files = glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True)
+        rows = []
+        for f in files:
+            with open(f) as fh:
+                rows += [r for r in csv.DictReader(fh) if r.get("Counter_Name") == counter]
+        shutil.rmtree(out, ignore_errors=True)
+        if not rows:
+            return None, f"{counter}: no counter rows (rc {p.returncode}): {text[-300:]!r}"
+        per[counter] = [(r["Kernel_Name"], float(r["Counter_Value"])) for r in rows]
+    kib = {}
+    for counter, rows in per.items():
+        copy = [v for k, v in rows if "k_copy16" in k]
+        gather = [v for k, v in rows if "k_gather64" in k]
+        call = [(k, v) for k, v in rows if "k_copy16" not in k and "k_gather64" not in k]
+        if len(copy) != 1 or len(gather) != 1 or not call:
+            return None, f"{counter}: unexpected dispatches {[k for k, _ in rows]}"
+        kib[counter] = (copy[0], call, gather[0])
+    sr = PMC_COPY_BYTES / 1024 / kib["FETCH_SIZE"][0]
+    sw = PMC_COPY_BYTES / 1024 / kib["WRITE_SIZE"][0]
+    call = sum(v for _, v in kib["FETCH_SIZE"][1]) * sr + sum(v for _, v in kib["WRITE_SIZE"][1]) * sw
+    trace = (sum(v for k, v in kib["FETCH_SIZE"][1] if "k_trace" in k) * sr +
+             sum(v for k, v in kib["WRITE_SIZE"][1] if "k_trace" in k) * sw)
+    gather = kib["FETCH_SIZE"][2] * sr + kib["WRITE_SIZE"][2] * sw
+    src = (f"rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE child runs of trenchanttracer_amd.profile_trace in this "
+           f"bench run ({len(kib['FETCH_SIZE'][1])} dispatches per call), scaled by a 256 MiB copy: "
+           f"reads x {sr:.5f}, writes x {sw:.5f}")
+    return {"call_bytes": call * 1024, "trace_bytes": trace * 1024, "gather_bytes": gather * 1024}, src
+
+
+def committed_traffic(args):
+    """The counter profile committed for the headline configuration
+    (profiles/k_trace_speculative_soup1M_pmc.md), used only when the live counter runs
+    cannot run; None for any other configuration."""
+    if (args.variant, args.lds_stack, args.block, args.refill, args.reorder, args.ntris) != (1, 16, 256, 32, 1,
+                                                                                          1_000_000):
+        return None, "no live counter runs and no committed profile for this configuration"
+    sr = 262_144 / 131_089.0
+    fetch = 6_597_740.5 + 129_627.1 + 16_251.6 + 148_430.5
+    write = 214_108.7 + 64_800.0 + 1_524.0 + 297_800.3
+    return ({"call_bytes": (fetch * sr + write) * 1024, "trace_bytes": (6_597_740.5 * sr + 214_108.7) * 1024,
+             "gather_bytes": None},
+            "profiles/k_trace_speculative_soup1M_pmc.md (committed counter runs)")
+
+
 def render_bench(tt, ctx, np, no_cpu):
     """Secondary metrics: samples/s of the path tracer on the full path-trace case
     (1,310,720-tri displaced icosphere, 40/40/20 diffuse/GGX/dielectric, 2048x1024
-    sky + sun, 1920x1080, 64 spp, max depth 16, Russian roulette from depth 4), and
-    of the reference-semantics render of the Cornell case (256x256, 16 spp, depth 8)
-    beside the oracle's CPU restatement of it."""
+    sky + sun, 1920x1080, 64 spp, max depth 16, Russian roulette from depth 4); of
+    the reference-semantics render of the Cornell case (256x256, 16 spp, depth 8)
+    beside the oracle's CPU restatement of it; of the batched case (32 scenes, 16 spp)
+    against 32 sequential renders; and guided against plain path tracing on the
+    path-trace case (equal-time RMSE against an FP64 8192-spp render)."""
     from trenchanttracer_amd import workloads
     out = {}
     mesh, mats, env, cam = workloads.pathtrace()
@@ -278,40 +408,57 @@ def render_bench(tt, ctx, np, no_cpu):
     out["reference_cornell_256_16spp"] = c1
     log(f"reference cornell: {c1}")
     s.close()
-    out["guided_door_room_256"] = guided_bench(tt, ctx, np)
+    from trenchanttracer_amd import bench_batch
+    out["batched_32_scenes_16spp"] = bench_batch.measure(tt, ctx, log=log)
+    log(f"batched: {out['batched_32_scenes_16spp']}")
+    out["guided_pathtrace_1080p"] = guided_bench(tt, ctx, np)
     return out
 
 
 def guided_bench(tt, ctx, np):
-    """Learned guiding where light reaches the camera through a door (door room,
-    256x256, depth 12): the network is trained online during a 1024-spp render,
-    then plain and guided 64-spp renders are scored against an FP64 4096-spp
-    render. Equal-time efficiency = (MSE x seconds) plain / (MSE x seconds) guided.
-    The full equal-time protocol on the 1080p path-trace case (8192-spp oracle) is
-    trenchanttracer_amd/bench_guide.py."""
-    from trenchanttracer_amd import workloads
-    mesh, mats, cam = workloads.door_room(256, 256)
-    s = tt.DeviceScene(ctx, mesh, tt.BVH.build(mesh), mats)
-    ref = s.render(cam, tt.render_params(spp=4096, max_depth=12, rr_depth=4, seed=1234, mode=tt.RENDER_PATH,
-                                         fp64=True), want_fp64=True)[1]
-    g = tt.Guide(ctx, seed=1, lr=1e-2)
-    pt = tt.render_params(spp=1024, max_depth=12, rr_depth=4, seed=99, mode=tt.RENDER_PATH, paths_in_flight=131072)
-    _, _, gs = g.render(s, cam, pt, alpha=0.5, train=True, train_paths=65536, train_batch=16384)
-    p64 = tt.render_params(spp=64, max_depth=12, rr_depth=4, seed=5, mode=tt.RENDER_PATH)
-    s.render(cam, p64)
-    plain, sp = s.render(cam, p64)
-    g.render(s, cam, p64, alpha=0.5, train=False)
-    guided, sg, _ = g.render(s, cam, p64, alpha=0.5, train=False)
-    mp, mg = float(np.mean((plain - ref) ** 2)), float(np.mean((guided - ref) ** 2))
-    r = {"mse_plain_64spp": mp, "mse_guided_64spp": mg, "seconds_plain": round(sp["seconds"], 5),
-         "seconds_guided": round(sg["seconds"], 5), "mse_ratio": round(mg / mp, 4),
-         "equal_time_efficiency": round((mp * sp["seconds"]) / (mg * sg["seconds"]), 3),
-         "training": {"steps": gs["train_steps"], "rows": gs["train_rows"],
-                      "seconds": round(gs["train_seconds"], 4), "last_loss": round(gs["last_loss"], 4)}}
-    log(f"guided door room: {r}")
-    g.close()
-    s.close()
-    return r
+    """Learned guiding on the path-trace case (1,310,720 tris, 1920x1080, depth 16):
+    equal-time RMSE against an FP64-accumulated 8192-spp render of the plain tracer.
+    Plain renders at 64 and 128 spp; guided renders (8-lobe vMF MLP, BF16 MFMA, Adam
+    lr 1e-3 on 2^18-row batches, one-sample MIS with the BSDF) trained online from
+    scratch throughout, trained during the first half of the samples, and frozen after
+    the 128-spp online run; two seeds per point. The time at which each reaches the
+    plain 64-spp RMSE comes from a log-log fit of RMSE against time
+    (trenchanttracer_amd/bench_guide.py, which also runs the longer sweep)."""
+    from trenchanttracer_amd import bench_guide, workloads
+    mesh, mats, env, cam = workloads.pathtrace()
+    sc = tt.DeviceScene(ctx, mesh, tt.BVH.build(mesh, nthreads=16), mats, env=env)
+    t0 = time.time()
+    _, ref, st = sc.render(cam, tt.render_params(spp=8192, max_depth=16, rr_depth=4, seed=1234, mode=tt.RENDER_PATH,
+                                                 fp64=True), want_fp64=True)
+    log(f"guided leg: FP64 8192-spp oracle render in {time.time() - t0:.1f}s")
+    r = bench_guide.compare(tt, ctx, sc, cam, ref, 16, (64, 128), (64, 128), 2 * 1920 * 1080, 1e-3, 262144, 65536,
+                            seeds=2)
+    sc.close()
+    p64 = next(x for x in r["plain"] if x["spp"] == 64)
+    g64 = next(x for x in r["guided_online"] if x["spp"] == 64)
+    f64 = next(x for x in r["guided_pretrained"] if x["spp"] == 64)
+    out = {
+        "oracle_spp": 8192, "seeds_per_point": 2,
+        "plain_64spp": {"rmse": round(p64["rmse"], 5), "seconds": round(p64["seconds"], 4)},
+        "wall_ratio_to_plain_64spp_rmse": {
+            "online": round(r["guided_online_wall_ratio"], 3),
+            "online_first_half": round(r["guided_online_half_wall_ratio"], 3),
+            "frozen_after_training": round(r["guided_pretrained_wall_ratio"], 3)},
+        "target_wall_ratio": 0.5,
+        "breakdown_64spp_s": {
+            "plain": round(p64["seconds"], 4),
+            "frozen_network_render": round(f64["seconds"], 4),
+            "inference_and_guided_shading": round(f64["seconds"] - p64["seconds"], 4),
+            "online_render": round(g64["seconds"], 4),
+            "online_training": round(g64["train_seconds"], 4),
+            "online_rendering_between_steps": round(g64["seconds"] - g64["train_seconds"], 4)},
+        "rmse_64spp": {"plain": round(p64["rmse"], 5), "online": round(g64["rmse"], 5),
+                       "frozen": round(f64["rmse"], 5)},
+        "training": {"steps": g64["train_steps"], "last_loss": round(g64["last_loss"], 4)},
+        "points": r,
+    }
+    log(f"guided path trace: { {k: v for k, v in out.items() if k != 'points'} }")
+    return out
 
 
 if __name__ == "__main__":

@@ -316,6 +316,23 @@ tt_status tt_trace_any(tt_scene *s, const tt_ray *rays, int64_t n, int32_t *hit_
  * on one GPU (any context, any stream) race, so callers order them. */
 tt_status tt_trace_closest_noalloc(tt_scene *s, const tt_ray *rays, int64_t n, tt_hit *hits,
                                    const tt_trace_cfg *cfg, void *stream);
+/* Node-fetch roofline probe for a ray set (no reference counterpart): the time of
+ * the traversal's own loop without its arithmetic. A recording launch of the
+ * closest-hit loop (cfg->variant, LDS stack 16, cfg's block / refill / reorder) stores
+ * the outcome of every node's slab test (4 bits: which children are entered, in which
+ * order); the replay launch then runs the same loop over the same rays in the same
+ * launch form (same waves, stack, leaf postponement, ballots and refills) with each
+ * outcome read from that record, fetching every node (64 B) and every triangle
+ * record (48 B) and folding their words into one 16-B result per ray, but with no
+ * slab or triangle arithmetic. With reorder = 1 every replay is preceded by the
+ * call's own key pass and sort, so the time bounds a whole tt_trace_closest call.
+ * *ms = average over reps (HIP events, after one warm-up). node_fetches /
+ * tri_fetches (may be NULL): the loop's node visits and triangle tests (bytes: 64
+ * per node + 48 per triangle record + 48 per ray). counts_out (host, may be NULL):
+ * n x 2 {node visits, triangle tests} per ray. Allocates and frees its record (half a
+ * byte per node visit); synchronous. */
+tt_status tt_bench_fetch_replay(tt_scene *s, const tt_ray *rays, int64_t n, const tt_trace_cfg *cfg, int32_t reps,
+                                float *ms, int64_t *node_fetches, int64_t *tri_fetches, int32_t *counts_out);
 /* Host-buffer convenience wrappers (copy in, trace, copy out, synchronise). */
 tt_status tt_trace_closest_host(tt_scene *s, const tt_ray *rays, int64_t n, tt_hit *hits,
                                 const tt_trace_cfg *cfg);
@@ -367,6 +384,8 @@ typedef struct tt_render_params {
     int32_t variant;          /* path-mode traversal loop: -1 speculative (default), 0 exact, 1 speculative */
     int32_t refill_camera;    /* traversal refill (see tt_trace_cfg) for the camera rays (0 = one wave per group) */
     int32_t refill;           /* ... for every later extension and shadow launch (0 = persistent, 16) */
+    int32_t tile;             /* path slots walk tile x tile pixel blocks (0 = 8; 1 = pixel rows; 2..32);
+                                 the image does not depend on it */
 } tt_render_params;
 
 typedef struct tt_render_stats {
@@ -489,7 +508,7 @@ typedef struct tt_guide_stats {
     int64_t train_rows;   /* rows trained on */
     double train_seconds; /* GPU time of the training between passes */
     float last_loss;      /* loss of the last step */
-    int32_t pad;
+    float first_loss;     /* loss of the first step */
 } tt_guide_stats;
 
 tt_status tt_guide_render(tt_scene *s, tt_guide *g, const tt_camera *cam, const tt_render_params *p,

@@ -35,7 +35,7 @@ def test_struct_sizes(tt):
     assert ctypes.sizeof(N.tt_geometry) == 48
     assert ctypes.sizeof(N.tt_trace_cfg) == 24
     assert ctypes.sizeof(N.tt_camera) == 60
-    assert ctypes.sizeof(N.tt_render_params) == 72
+    assert ctypes.sizeof(N.tt_render_params) == 76
     assert ctypes.sizeof(N.tt_render_stats) == 40
     assert ctypes.sizeof(N.tt_guide_params) == 28
     assert ctypes.sizeof(N.tt_guide_stats) == 32

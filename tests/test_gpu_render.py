@@ -383,8 +383,8 @@ def test_path_mode_large_queues_match_small_passes(tt, ctx):
     """256x256 at 32 spp in one pass is 2,097,152 paths: the queue kernels' blocks
     stride over each bounce's queue several times (traversal grid 32 one-wave blocks
     per CU, shade grid 16 blocks of 256 per CU). The same samples rendered as 32
-    passes of 65,536 paths (one stride each) give the same FP64 image up to the
-    grouping of the per-pixel sums."""
+    passes of 65,536 paths (one stride each) give the same image bit for bit: each
+    pixel's sum takes its samples in sample order across passes."""
     from trenchanttracer_amd import workloads
     _say("path-mode large queues")
     mesh, mats, cam = workloads.cornell(subdiv=3, width=256, height=256)
@@ -397,8 +397,12 @@ def test_path_mode_large_queues_match_small_passes(tt, ctx):
     _say(f"one pass: {sa}; 32 passes: {sb}")
     assert sa["paths_in_flight"] == 256 * 256 * 32 and sb["paths_in_flight"] == 256 * 256
     assert sa["rays_extension"] == sb["rays_extension"] and sa["rays_shadow"] == sb["rays_shadow"]
-    assert np.allclose(a64, b64, rtol=1e-12, atol=1e-12)
+    assert np.array_equal(a64, b64)
     assert a64.mean() > 0.01
+    fa, _ = s.render(cam, tt.render_params(spp=32, max_depth=8, rr_depth=3, seed=9, mode=tt.RENDER_PATH))
+    fb, _ = s.render(cam, tt.render_params(spp=32, max_depth=8, rr_depth=3, seed=9, mode=tt.RENDER_PATH,
+                                           paths_in_flight=3 * 256 * 256))
+    assert np.array_equal(fa.view(np.int32), fb.view(np.int32))
     s.close()
 
 
@@ -409,7 +413,8 @@ def test_pathtrace_case_full_size_invariants(tt, ctx):
     ray counts are bit-identical for the exact and the speculative loop, for other
     block sizes, refill thresholds and LDS stack depths, and for a pass replayed from
     a hipGraph; FP64 accumulation of the same samples agrees with the float sums to
-    float rounding."""
+    float rounding. The pixel tile the path slots walk (16 x 16, pixel rows) changes
+    only the order of the paths, not the image."""
     from trenchanttracer_amd import workloads
     _say("path-trace case full size")
     mesh, mats, env, cam = workloads.pathtrace()
@@ -421,7 +426,7 @@ def test_pathtrace_case_full_size_invariants(tt, ctx):
     assert st["rays_extension"] > 1920 * 1080 * 64 and st["rays_shadow"] > 0
     assert np.isfinite(img).all() and 0.1 < img.mean() < 10.0
     for extra in (dict(variant=0), dict(block=256, refill=8), dict(lds_stack=8, refill=32, refill_camera=16),
-                  dict(use_graph=True)):
+                  dict(use_graph=True), dict(tile=16), dict(tile=1)):
         other, so = s.render(cam, tt.render_params(**base, **extra))
         _say(f"{extra}: {so['seconds']:.4f} s")
         assert so["rays_extension"] == st["rays_extension"] and so["rays_shadow"] == st["rays_shadow"], extra

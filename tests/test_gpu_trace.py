@@ -229,3 +229,34 @@ def test_scene_refuses_triangle_ids_outside_its_mesh(tt, ctx):
         tt.DeviceScene(ctx, small, b)
     s = tt.DeviceScene(ctx, big, b)
     s.close()
+
+
+@pytest.mark.parametrize("cfg_args", [dict(variant=0, lds_stack=16, block=64),
+                                      dict(variant=0, lds_stack=16, block=256, refill=32, reorder=1),
+                                      dict(variant=1, lds_stack=16, block=256, refill=32, reorder=1)])
+def test_fetch_replay_records_the_traversal_loop(tt, oracle, ctx, cfg_args):
+    """The node-fetch probe's counting launch of the exact loop sees, per ray, the node
+    visits and triangle tests the oracle's single-lane traversal counts; the
+    speculative loop (leaves postponed while the wave descends) visits at least as
+    many. The replay's bytes are 64 per node, 48 per triangle record and 48 per ray."""
+    print(f"fetch replay {cfg_args}", file=sys.stderr, flush=True)
+    mesh = tt.Mesh.soup(20000, seed=9)
+    bvh = tt.BVH.build(mesh)
+    scene = tt.DeviceScene(ctx, mesh, bvh)
+    rays = np.concatenate([tt.rays_primary(128, 96, 1, (0.5, 0.5, -1.0), (0.5, 0.5, 0.0)),
+                           tt.rays_incoherent(mesh, 20000, seed=4)])
+    n = len(rays)
+    d = tt.DeviceBuffer.from_array(ctx, rays)
+    r = scene.fetch_replay(d, n, tt.trace_cfg(**cfg_args), reps=2, want_counts=True)
+    _, oc = oracle.CompactBVH(bvh.export_compact()).trace_counts(rays)
+    if cfg_args["variant"] == 0:
+        assert np.array_equal(r["counts"][:, 0], oc[:, 0])
+        assert np.array_equal(r["counts"][:, 1], oc[:, 1])
+    else:
+        assert r["node_fetches"] >= int(oc[:, 0].sum()) and r["tri_fetches"] >= int(oc[:, 1].sum())
+        assert r["node_fetches"] <= 1.5 * int(oc[:, 0].sum())
+    assert r["node_fetches"] == int(r["counts"][:, 0].sum()) and r["tri_fetches"] == int(r["counts"][:, 1].sum())
+    assert r["bytes"] == 64 * r["node_fetches"] + 48 * r["tri_fetches"] + 48 * n
+    assert r["ms"] > 0
+    d.free()
+    scene.close()

@@ -247,9 +247,9 @@ def camera_from_orbit(center, yaw, pitch, radius=4.0, fovx=45.0, width=256, heig
 
 def render_params(spp=16, max_depth=8, rr_depth=4, seed=1, mode=RENDER_REFERENCE, guided=0, sample_offset=0,
                   fp64=False, pi=None, two_pi=None, tmin=0.01, use_graph=False, paths_in_flight=0, lds_stack=0,
-                  block=0, variant=-1, refill_camera=0, refill=0):
+                  block=0, variant=-1, refill_camera=0, refill=0, tile=0):
     p = N.tt_render_params()
-    p.refill_camera, p.refill = int(refill_camera), int(refill)
+    p.refill_camera, p.refill, p.tile = int(refill_camera), int(refill), int(tile)
     p.use_graph, p.paths_in_flight, p.lds_stack, p.block, p.variant = (int(bool(use_graph)), int(paths_in_flight),
                                                                        int(lds_stack), int(block), int(variant))
     p.spp, p.max_depth, p.rr_depth, p.seed = int(spp), int(max_depth), int(rr_depth), int(seed)
@@ -274,7 +274,7 @@ class Batch:
     def __init__(self, ctx, meshes, bvhs, materials, env=None, env_scale=1.0):
         n = len(meshes)
         self.ctx = ctx
-        self._mats = [(N.tt_material * len(m))(*m) for m in materials]
+        self._mats = [m if isinstance(m, C.Array) else (N.tt_material * len(m))(*m) for m in materials]
         mesh_arr = (C.c_void_p * n)(*[m._h.value for m in meshes])
         bvh_arr = (C.c_void_p * n)(*[b._h.value for b in bvhs])
         mat_arr = (C.c_void_p * n)(*[C.cast(m, C.c_void_p).value for m in self._mats])
@@ -373,7 +373,7 @@ class Guide:
         check(lib.tt_guide_render(scene._h, self._h, C.byref(cam), C.byref(params), C.byref(gp), _vp(out), _vp(out64),
                                   C.byref(st), C.byref(gs)))
         gst = dict(train_steps=gs.train_steps, train_rows=gs.train_rows, train_seconds=gs.train_seconds,
-                   last_loss=gs.last_loss)
+                   last_loss=gs.last_loss, first_loss=gs.first_loss)
         return (out, out64, _stats(st), gst) if want_fp64 else (out, _stats(st), gst)
 
     def sample(self, raw, n, seed=1):
@@ -545,7 +545,9 @@ class DeviceScene:
         self.ctx = ctx
         mats = None
         nm = 0
-        if materials:
+        if isinstance(materials, C.Array) and len(materials):
+            nm, mats = len(materials), materials
+        elif materials:
             nm = len(materials)
             mats = (N.tt_material * nm)(*materials)
         env_arr = None
@@ -585,6 +587,22 @@ class DeviceScene:
         check(lib.tt_trace_closest_noalloc(self._h, _dev_ptr(rays), int(n), _dev_ptr(hits),
                                            C.byref(cfg) if cfg is not None else None,
                                            C.c_void_p(stream) if stream else None))
+
+    def fetch_replay(self, rays, n, cfg=None, reps=3, want_counts=False):
+        """Node-fetch roofline probe (tt_bench_fetch_replay) over device rays: dict with the
+        replay's ms, node / triangle-record fetch totals, its bytes and GB/s; with
+        want_counts also the per-ray (n, 2) fetch counts."""
+        ms = C.c_float()
+        nf, tf = C.c_int64(), C.c_int64()
+        counts = np.empty((int(n), 2), np.int32) if want_counts else None
+        check(lib.tt_bench_fetch_replay(self._h, _dev_ptr(rays), int(n), C.byref(cfg) if cfg is not None else None,
+                                        int(reps), C.byref(ms), C.byref(nf), C.byref(tf), _vp(counts)))
+        nbytes = 64 * nf.value + 48 * tf.value + 48 * int(n)
+        out = {"ms": ms.value, "node_fetches": nf.value, "tri_fetches": tf.value, "bytes": nbytes,
+               "gbps": nbytes / (ms.value * 1e-3) / 1e9 if ms.value > 0 else 0.0}
+        if want_counts:
+            out["counts"] = counts
+        return out
 
     def trace_any(self, rays, n, flags, cfg=None, stream=None):
         check(lib.tt_trace_any(self._h, _dev_ptr(rays), int(n), _dev_ptr(flags),

@@ -8,7 +8,7 @@ import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtt.so")
-ABI_VERSION = 4   # tt_abi_version() of the library this binding's structs describe
+ABI_VERSION = 5   # tt_abi_version() of the library this binding's structs describe
 
 TT_OK = 0
 STATUS_NAMES = {0: "TT_OK", 1: "TT_ERR_INVALID", 2: "TT_ERR_NOMEM", 3: "TT_ERR_DEVICE",
@@ -68,7 +68,7 @@ class tt_render_params(C.Structure):
                 ("sample_offset", C.c_int32), ("accumulate_fp64", C.c_int32),
                 ("pi", C.c_float), ("two_pi", C.c_float), ("tmin", C.c_float), ("use_graph", C.c_int32),
                 ("paths_in_flight", C.c_int32), ("lds_stack", C.c_int32), ("block", C.c_int32),
-                ("variant", C.c_int32), ("refill_camera", C.c_int32), ("refill", C.c_int32)]
+                ("variant", C.c_int32), ("refill_camera", C.c_int32), ("refill", C.c_int32), ("tile", C.c_int32)]
 
 
 class tt_render_stats(C.Structure):
@@ -91,7 +91,7 @@ class tt_guide_params(C.Structure):
 
 class tt_guide_stats(C.Structure):
     _fields_ = [("train_steps", C.c_int64), ("train_rows", C.c_int64), ("train_seconds", C.c_double),
-                ("last_loss", C.c_float), ("pad", C.c_int32)]
+                ("last_loss", C.c_float), ("first_loss", C.c_float)]
 
 
 P = C.c_void_p
@@ -172,6 +172,8 @@ PROTOS = {
     "tt_debug_env_invert": (C.c_int, [P, P, I64, P]),
     "tt_trace_closest": (C.c_int, [P, P, I64, P, C.POINTER(tt_trace_cfg), P]),
     "tt_trace_closest_noalloc": (C.c_int, [P, P, I64, P, C.POINTER(tt_trace_cfg), P]),
+    "tt_bench_fetch_replay": (C.c_int, [P, P, I64, C.POINTER(tt_trace_cfg), C.c_int32, C.POINTER(C.c_float),
+                                        C.POINTER(I64), C.POINTER(I64), P]),
     "tt_trace_any": (C.c_int, [P, P, I64, P, C.POINTER(tt_trace_cfg), P]),
     "tt_trace_closest_host": (C.c_int, [P, P, I64, P, C.POINTER(tt_trace_cfg)]),
     "tt_trace_any_host": (C.c_int, [P, P, I64, P, C.POINTER(tt_trace_cfg)]),

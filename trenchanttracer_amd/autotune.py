@@ -3,7 +3,9 @@
 Times short runs of candidate launch configurations on the actual workload and
 keeps the fastest: traversal loop variant x LDS stack entries x block size x
 launch form (one wave per 64-ray group, or persistent waves with lane refill) for
-`tune_trace`; paths in flight x hipGraph replay for `tune_render`.
+`tune_trace`; paths in flight, hipGraph replay, traversal LDS stack, block size,
+refill and the pixel tile the path slots walk for `tune_render` (coordinate
+descent).
 Results can be cached in a JSON file keyed by a workload signature."""
 import itertools
 import json
@@ -64,25 +66,59 @@ def tune_trace(ctx, scene, d_rays, n, d_hits, variants=(0, 1), lds=(8, 16, 24), 
     return trace_cfg(best[1], best[2], best[3], refill=best[4], reorder=best[5]), best[0]
 
 
-def tune_render(render_fn, base, paths=(0, 16 << 20), lds=(16,), blocks=(64,), graphs=(0, 1),
-                probe_spp=None, cache=None, key=None):
-    """Fastest render configuration. render_fn(params) -> stats; base: a
-    tt_render_params-like dict of the real render. Probes run at the real spp
-    unless probe_spp is given: the number of passes, which paths in flight decide,
-    only shows at the real sample count."""
+RENDER_SPACE = {
+    "paths_in_flight": (0, 16 << 20),   # 0: up to 2^27 paths per pass
+    "use_graph": (False, True),
+    "lds_stack": (16, 8, 24),           # traversal LDS stack entries per lane
+    "block": (64, 128, 256),            # traversal block size
+    "refill": (16, 8, 32),              # persistent-wave refill of the later bounces
+    "tile": (8, 4, 16),                 # path slots walk tile x tile pixel blocks
+}
+
+
+def tune_render(render_fn, base, space=None, probe_spp=None, rounds=2, cache=None, key=None):
+    """Fastest render configuration by coordinate descent: starting from the first
+    value of every parameter, each round tries every value of one parameter at a time
+    with the others held at their best so far (paths in flight, hipGraph replay,
+    traversal LDS stack, block size, refill of the later bounces, slot tile size),
+    `rounds` sweeps over all of them. render_fn(params) -> stats; base: a
+    tt_render_params-like dict of the real render. Probes run at the real spp unless
+    probe_spp is given: the number of passes, which paths in flight decide, only
+    shows at the real sample count. Returns (settings, info): the chosen values of the
+    tuned parameters, and {"seconds": the best probe's GPU time, "trials": probes run,
+    "default_seconds": the probe at the first values}."""
+    space = dict(RENDER_SPACE if space is None else space)
     db = _load(cache)
     if key and key in db:
-        return db[key]
-    best = None
-    for pf, s, b, g in itertools.product(paths, lds, blocks, graphs):
-        prm = dict(base)
-        prm.update(spp=probe_spp or base["spp"], paths_in_flight=pf, lds_stack=s, block=b, use_graph=bool(g))
-        render_fn(render_params(**prm))           # warm (allocations, code objects)
-        st = render_fn(render_params(**prm))
-        t = st["seconds"]
-        if best is None or t < best[0]:
-            best = (t, {"paths_in_flight": pf, "lds_stack": s, "block": b, "use_graph": bool(g)})
+        return db[key]["settings"], db[key]["info"]
+    cur = {k: v[0] for k, v in space.items()}
+    seen = {}
+
+    def measure(cfg):
+        sig = tuple(sorted(cfg.items()))
+        if sig not in seen:
+            prm = dict(base)
+            prm.update(cfg)
+            prm["spp"] = probe_spp or base["spp"]
+            render_fn(render_params(**prm))       # warm (allocations, code objects, graph capture)
+            seen[sig] = render_fn(render_params(**prm))["seconds"]
+        return seen[sig]
+
+    best_t = default_t = measure(cur)
+    for _ in range(rounds):
+        changed = False
+        for name, values in space.items():
+            for v in values:
+                if v == cur[name]:
+                    continue
+                cand = dict(cur, **{name: v})
+                t = measure(cand)
+                if t < best_t:
+                    best_t, cur, changed = t, cand, True
+        if not changed:
+            break
+    info = {"seconds": best_t, "trials": len(seen), "default_seconds": default_t}
     if key:
-        db[key] = best[1]
+        db[key] = {"settings": cur, "info": info}
         _save(cache, db)
-    return best[1]
+    return cur, info

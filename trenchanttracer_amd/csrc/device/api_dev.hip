@@ -555,6 +555,122 @@ tt_status tt_trace_closest_noalloc(tt_scene *s, const tt_ray *rays, int64_t n, t
     return trace_impl(s, rays, n, hits, nullptr, cfg, stream, false, true);
 }
 
+tt_status tt_bench_fetch_replay(tt_scene *s, const tt_ray *rays, int64_t n, const tt_trace_cfg *cfg, int32_t reps,
+                                float *ms, int64_t *node_fetches, int64_t *tri_fetches, int32_t *counts_out) {
+    if (!s || !rays || n <= 0 || n >= ((int64_t)1 << 31) || reps < 1 || !ms)
+        return fail(TT_ERR_INVALID, "tt_bench_fetch_replay: bad arguments");
+    tt_trace_cfg c;
+    tt_status st = resolve_cfg(cfg, c);
+    if (st != TT_OK) return st;
+    tt_ctx *ctx = s->ctx;
+    hipStream_t hs = ctx->stream;
+    TT_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    TraceArgs a{};
+    a.nodes = reinterpret_cast<const float4 *>(s->nodes);
+    a.tris = reinterpret_cast<const float4 *>(s->tris);
+    a.rays = rays;
+    a.n = n;
+    a.num_nodes = (int32_t)s->num_nodes;
+    if (c.reorder) {
+        size_t need = 0, sort_bytes = 0;
+        TT_HIP(reorder_workspace_bytes(n, &need, &sort_bytes), "fetch replay: reorder workspace size");
+        if (ctx->reorder_ws_bytes < need) {
+            TT_HIP(hipDeviceSynchronize(), "hipDeviceSynchronize");
+            if (ctx->reorder_ws) hipFree(ctx->reorder_ws);
+            ctx->reorder_ws = nullptr;
+            ctx->reorder_ws_bytes = 0;
+            st = dev_alloc(ctx, &ctx->reorder_ws, need, "fetch replay: reorder workspace");
+            if (st != TT_OK) return st;
+            ctx->reorder_ws_bytes = need;
+        }
+        if (ctx->reorder_pending) TT_HIP(hipStreamWaitEvent(hs, ctx->reorder_done, 0), "hipStreamWaitEvent");
+        TT_HIP(reorder_rays(rays, n, s->bmin, s->bmax, ctx->reorder_ws, sort_bytes, hs, &a.order),
+               "fetch replay: reorder");
+    }
+    // per-ray counts (node visits, triangle tests, leaf visits), then the outcome of
+    // every node visit, 4 bits each, at the counts' prefix sums (in 32-bit words)
+    void *d_counts = nullptr, *d_off = nullptr, *d_log = nullptr, *d_out = nullptr;
+    auto cleanup = [&] {
+        hipStreamSynchronize(hs);
+        for (void *p : {d_counts, d_off, d_log, d_out})
+            if (p) hipFree(p);
+    };
+    st = dev_alloc(ctx, &d_counts, (size_t)n * 12, "fetch replay: counts");
+    if (st != TT_OK) return st;
+    a.rec_nodes = static_cast<int32_t *>(d_counts);
+    a.rec_tris = a.rec_nodes + n;
+    a.rec_leaves = a.rec_nodes + 2 * n;
+    hipError_t e = launch_trace_record(a, c.variant, 1, c.block, c.refill, ctx->num_cus, hs);
+    std::vector<int32_t> cnt((size_t)n * 3);
+    if (e == hipSuccess) e = hipMemcpyAsync(cnt.data(), d_counts, (size_t)n * 12, hipMemcpyDeviceToHost, hs);
+    if (e == hipSuccess) e = hipStreamSynchronize(hs);
+    if (e != hipSuccess) {
+        cleanup();
+        return fail(TT_ERR_DEVICE, std::string("fetch replay: counting pass: ") + hipGetErrorString(e));
+    }
+    std::vector<int64_t> off((size_t)n + 1);
+    int64_t nodes_total = 0, tris_total = 0;
+    off[0] = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        nodes_total += cnt[(size_t)i];
+        tris_total += cnt[(size_t)(n + i)];
+        off[(size_t)i + 1] = off[(size_t)i] + (cnt[(size_t)i] + 7) / 8;
+    }
+    if (counts_out)
+        for (int64_t i = 0; i < n; ++i) {
+            counts_out[2 * i] = cnt[(size_t)i];
+            counts_out[2 * i + 1] = cnt[(size_t)(n + i)];
+        }
+    const size_t log_words = (size_t)std::max<int64_t>(1, off[(size_t)n]);
+    st = dev_alloc(ctx, &d_off, off.size() * 8, "fetch replay: offsets");
+    if (st == TT_OK) st = dev_alloc(ctx, &d_log, log_words * 4, "fetch replay: log");
+    if (st == TT_OK) st = dev_alloc(ctx, &d_out, (size_t)n * 16, "fetch replay: output");
+    if (st != TT_OK) {
+        cleanup();
+        return st;
+    }
+    e = hipMemcpyAsync(d_off, off.data(), off.size() * 8, hipMemcpyHostToDevice, hs);
+    if (e == hipSuccess) e = hipMemsetAsync(d_log, 0, log_words * 4, hs);
+    a.rec_off = static_cast<const int64_t *>(d_off);
+    a.rec_log = static_cast<int32_t *>(d_log);
+    if (e == hipSuccess) e = launch_trace_record(a, c.variant, 2, c.block, c.refill, ctx->num_cus, hs);
+    TraceArgs r = a;
+    r.hits = static_cast<tt_hit *>(d_out);
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    if (e == hipSuccess) e = hipEventCreate(&ev[0]);
+    if (e == hipSuccess) e = hipEventCreate(&ev[1]);
+    // with reorder each timed replay is preceded by the same key pass and sort as a
+    // traversal call, so the time bounds the whole call
+    size_t sort_bytes = 0, ws_need = 0;
+    if (e == hipSuccess && c.reorder) e = reorder_workspace_bytes(n, &ws_need, &sort_bytes);
+    auto one = [&]() -> hipError_t {
+        if (c.reorder) {
+            const hipError_t er = reorder_rays(rays, n, s->bmin, s->bmax, ctx->reorder_ws, sort_bytes, hs, &r.order);
+            if (er != hipSuccess) return er;
+        }
+        return launch_trace_record(r, c.variant, 3, c.block, c.refill, ctx->num_cus, hs);
+    };
+    if (e == hipSuccess) e = one();   // warm-up
+    if (e == hipSuccess) e = hipEventRecord(ev[0], hs);
+    for (int k = 0; k < reps && e == hipSuccess; ++k) e = one();
+    if (e == hipSuccess) e = hipEventRecord(ev[1], hs);
+    if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+    float total_ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&total_ms, ev[0], ev[1]);
+    for (hipEvent_t x : ev)
+        if (x) hipEventDestroy(x);
+    cleanup();
+    if (e != hipSuccess) return fail(TT_ERR_DEVICE, std::string("fetch replay: ") + hipGetErrorString(e));
+    if (c.reorder) {
+        TT_HIP(hipEventRecord(ctx->reorder_done, hs), "hipEventRecord");
+        ctx->reorder_pending = true;
+    }
+    *ms = total_ms / (float)reps;
+    if (node_fetches) *node_fetches = nodes_total;
+    if (tri_fetches) *tri_fetches = tris_total;
+    return TT_OK;
+}
+
 static tt_status host_trace(tt_scene *s, const tt_ray *rays, int64_t n, void *out, size_t out_elem,
                             const tt_trace_cfg *cfg, bool any) {
     if (!s) return fail(TT_ERR_INVALID, "trace_host: null scene");

@@ -26,6 +26,14 @@ struct TraceArgs {
     int32_t xcd_order;     // 1: each XCD (blocks b = x mod 8) holds one contiguous range of waves
     int32_t refill;        // a wave takes new rays once this many of its lanes are idle (1..64)
     const uint32_t *order; // if set: the r-th ray traced is ray order[r] (a permutation of 0..n-1)
+    // node-fetch probe (closest-hit loop): per ray the counts of node visits,
+    // triangle tests and leaf visits, and from rec_off[ray] (32-bit words) the 4-bit
+    // slab-test outcome of every node visit, 8 per word
+    int32_t *rec_nodes;
+    int32_t *rec_tris;
+    int32_t *rec_leaves;
+    int32_t *rec_log;
+    const int64_t *rec_off;
 };
 
 // Triangle acceptance: the reference's test (front faces only, det > 1e-5) or
@@ -67,5 +75,11 @@ __device__ __forceinline__ int cdf_search_guided(const float *cdf, const int2 *g
 // Launchers (trace.hip).
 hipError_t launch_trace(const TraceArgs &a, int variant, bool any, int lds_stack, int block, int xcd_order,
                         int refill, int num_cus, hipStream_t s, int tri_mode = kTriReference);
+// The closest-hit loop (variant 0 or 1, LDS stack 16, reference triangle test) for
+// the node-fetch probe: pass 1 counts (rec_nodes / rec_tris / rec_leaves), pass 2
+// records the node visits' outcomes (rec_off, rec_log), pass 3 replays them,
+// writing one 16-byte record per ray to hits.
+hipError_t launch_trace_record(const TraceArgs &a, int variant, int pass, int block, int refill, int num_cus,
+                               hipStream_t s);
 
 }  // namespace tt

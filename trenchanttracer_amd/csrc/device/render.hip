@@ -1256,13 +1256,32 @@ __global__ __launch_bounds__(256) void k_resolve_reference(const SceneTable *tab
     }
 }
 
+// Path mode: each pixel's running sum (float, or FP64 when acc64 is set) takes the
+// pass's samples one at a time in sample order, continuing from the previous pass, so
+// the sum is the same left fold over the pixel's samples for any pass structure
+// (a batch of scenes, whose passes split the samples differently, gives each scene
+// the bits of its own render).
 __global__ __launch_bounds__(256) void k_resolve(const SceneTable *tab, const float4 *L, int32_t npix, int32_t kk,
                                                  float *acc, double *acc64) {
     __shared__ ResolveTile t;
     const int32_t q0 = blockIdx.x * 256, q = q0 + threadIdx.x;
     const bool live = q < npix;
+    const int32_t pix = live ? slot_pixel(tab, q) : 0;
     double rd = 0.0, gd = 0.0, bd = 0.0;
     float rf = 0.0f, gf = 0.0f, bf = 0.0f;
+    if (live) {
+        if (acc64) {
+            const double *o = acc64 + 3 * (int64_t)pix;
+            rd = o[0];
+            gd = o[1];
+            bd = o[2];
+        } else {
+            const float *o = acc + 3 * (int64_t)pix;
+            rf = o[0];
+            gf = o[1];
+            bf = o[2];
+        }
+    }
     for (int32_t s0 = 0; s0 < kk; s0 += kResolveChunk) {
         __syncthreads();
         stage_chunk(t, L, q0, npix, kk, s0);
@@ -1282,17 +1301,16 @@ __global__ __launch_bounds__(256) void k_resolve(const SceneTable *tab, const fl
         }
     }
     if (!live) return;
-    const int32_t pix = slot_pixel(tab, q);
     if (acc64) {
         double *o = acc64 + 3 * (int64_t)pix;
-        o[0] += rd;
-        o[1] += gd;
-        o[2] += bd;
+        o[0] = rd;
+        o[1] = gd;
+        o[2] = bd;
     } else {
         float *o = acc + 3 * (int64_t)pix;
-        o[0] += rf;
-        o[1] += gf;
-        o[2] += bf;
+        o[0] = rf;
+        o[1] = gf;
+        o[2] = bf;
     }
 }
 
@@ -1499,6 +1517,7 @@ struct GuideRun {
     int32_t max_bounce = 0;   // the network guides bounces below this (0: every bounce)
     int64_t steps = 0, rows = 0;
     double train_ms = 0.0;
+    float first_loss = 0.0f;  // loss of the first Adam step
 };
 
 tt_status ensure_state(tt_scene *s, int32_t cap, int32_t npix, bool fp64) {
@@ -1738,6 +1757,7 @@ tt_status validate_params(const tt_render_params *p) {
         return fail(TT_ERR_INVALID, "render: block must be 0, 64, 128, 192 or 256");
     if (p->variant < -1 || p->variant > 1) return fail(TT_ERR_INVALID, "render: variant must be -1, 0 or 1");
     if (p->paths_in_flight < 0) return fail(TT_ERR_INVALID, "render: paths_in_flight must be >= 0");
+    if (p->tile < 0 || p->tile > 32) return fail(TT_ERR_INVALID, "render: tile must be 0..32");
     if (p->refill < 0 || p->refill > 64 || p->refill_camera < 0 || p->refill_camera > 64)
         return fail(TT_ERR_INVALID, "render: refill thresholds must be 0..64");
     return TT_OK;
@@ -1790,6 +1810,10 @@ hipError_t guide_train_pass(GuideRun *gr, int32_t max_depth, hipStream_t hs) {
         if (e == hipSuccess) e = guide_train_rows(g, n, hs);
         gr->steps++;
         gr->rows += n;
+        if (e == hipSuccess && gr->steps == 1) {   // the first step's loss (one wait per render)
+            e = hipMemcpyAsync(&gr->first_loss, g->scalars, 4, hipMemcpyDeviceToHost, hs);
+            if (e == hipSuccess) e = hipStreamSynchronize(hs);
+        }
     }
     return e;
 }
@@ -1815,7 +1839,7 @@ tt_status render_impl(tt_scene *s, int32_t nsc, const tt_camera *cams, const int
         return fail(TT_ERR_INVALID, "render: scene has no shading data");
     SceneTable tab{};
     tab.n = nsc;
-    tab.tile = kSlotTile;
+    tab.tile = p->tile > 0 ? p->tile : kSlotTile;
 
     int64_t npix64 = 0;
     for (int32_t k = 0; k < nsc; ++k) {
@@ -2081,6 +2105,7 @@ tt_status tt_guide_render(tt_scene *s, tt_guide *g, const tt_camera *cam, const 
         if (gr.steps > 0 && hipMemcpy(&loss, g->scalars, 4, hipMemcpyDeviceToHost) != hipSuccess)
             return fail(TT_ERR_DEVICE, "tt_guide_render: reading the loss failed");
         gstats->last_loss = loss;
+        gstats->first_loss = gr.first_loss;
     }
     return st;
 }

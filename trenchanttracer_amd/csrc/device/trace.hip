@@ -80,8 +80,15 @@ struct Stack {
     }
 };
 
-template <int VARIANT, bool ANY, int S, int TRI>
+// REC (closest hit only; the node-fetch probe): 1 counts each ray's node visits,
+// triangle tests and leaf visits; 2 records at every node visit the slab test's
+// outcome as a 4-bit code (bit 0: child 0 entered, bit 1: child 1 entered, bit 2:
+// both, child 1 first); 3 replays: the same loop, stack, leaf postponement, ballots
+// and refills, every node and triangle record fetched, but each node's outcome read
+// from the record instead of computed and no triangle arithmetic.
+template <int VARIANT, bool ANY, int S, int TRI, int REC = 0>
 __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
+    static_assert(REC == 0 || !ANY, "fetch recording follows the closest-hit loop");
     extern __shared__ int lds_stack[];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -133,11 +140,24 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
     int node_addr = kSentinel;
     float hit_t = 0.0f, hit_u = 0.0f, hit_v = 0.0f;
     int hit_id = -1;
+    int rec_n = 0, rec_t = 0, rec_l = 0;   // node visits, triangle tests, leaf visits of the lane's ray (REC)
+    int64_t rec_base = 0, rec_cap = 0;     // the ray's words of outcome codes (REC 2, 3)
+    uint32_t rec_word = 0;                 // codes of the current 8 visits
+    uint32_t rec_next = 0;                 // codes of the next 8 (REC 3: loaded a word ahead)
+    uint32_t tok = 0;                      // every fetched word folded in (REC 3)
 
     for (;;) {
         const bool done = node_addr == kSentinel;
         if (done && rid >= 0) {
-            if (ANY) {
+            if (REC == 1) {
+                a.rec_nodes[rid] = rec_n;
+                a.rec_tris[rid] = rec_t;
+                a.rec_leaves[rid] = rec_l;
+            } else if (REC == 2) {
+                if ((rec_n & 7) != 0 && (rec_n >> 3) < rec_cap) a.rec_log[rec_base + (rec_n >> 3)] = (int32_t)rec_word;
+            } else if (REC == 3) {
+                *reinterpret_cast<float4 *>(a.hits + rid) = make_float4(bf((int)tok), 0.0f, 0.0f, 0.0f);
+            } else if (ANY) {
                 a.flags[rid] = hit_id != -1 ? 1 : 0;
             } else {
                 if (hit_id == -1) {
@@ -183,6 +203,18 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
                 hit_u = 0.0f;
                 hit_v = 0.0f;
                 hit_id = -1;
+                rec_n = 0;
+                rec_t = 0;
+                rec_l = 0;
+                rec_word = 0;
+                if (REC >= 2) {
+                    rec_base = a.rec_off[rid];
+                    rec_cap = a.rec_off[rid + 1] - rec_base;
+                }
+                if (REC == 3) {
+                    tok = __float_as_uint(r0.x) ^ __float_as_uint(r1.w);
+                    rec_next = rec_cap > 0 ? (uint32_t)a.rec_log[rec_base] : 0u;
+                }
             }
         }
         if (take) cursor += __popcll(m);
@@ -196,6 +228,26 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
             const float4 n1xy = np[1];
             const float4 nz = np[2];
             const int2 ch = reinterpret_cast<const int2 *>(np + 3)[0];
+            bool t0, t1, c1_first;
+            if (REC == 3) {
+                // the recorded outcome; the box words are still fetched and used
+                if ((rec_n & 7) == 0) {
+                    // the next word is requested 8 visits before it is needed
+                    rec_word = rec_next;
+                    const int64_t wn = (rec_n >> 3) + 1;
+                    rec_next = wn < rec_cap ? (uint32_t)a.rec_log[rec_base + wn] : 0u;
+                }
+                const uint32_t code = (rec_word >> ((rec_n & 7) * 4)) & 15u;
+                ++rec_n;
+                t0 = (code & 1u) != 0;
+                t1 = (code & 2u) != 0;
+                c1_first = (code & 4u) != 0;
+                tok ^= (__float_as_uint(n0xy.x) ^ __float_as_uint(n0xy.y) ^ __float_as_uint(n0xy.z) ^
+                        __float_as_uint(n0xy.w)) ^
+                       (__float_as_uint(n1xy.x) ^ __float_as_uint(n1xy.y) ^ __float_as_uint(n1xy.z) ^
+                        __float_as_uint(n1xy.w)) ^
+                       (__float_as_uint(nz.x) ^ __float_as_uint(nz.y) ^ __float_as_uint(nz.z) ^ __float_as_uint(nz.w));
+            } else {
 
             const float c0lox = n0xy.x * idirx - oodx;
             const float c0hix = n0xy.y * idirx - oodx;
@@ -215,8 +267,19 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
             const float c1max = span_end(c1lox, c1hix, c1loy, c1hiy, c1loz, c1hiz, hit_t);
 
             const float ray_tmax = 1e20f;
-            const bool t0 = (c0min <= c0max) && (c0min >= tmin) && (c0min <= ray_tmax);
-            const bool t1 = (c1min <= c1max) && (c1min >= tmin) && (c1min <= ray_tmax);
+            t0 = (c0min <= c0max) && (c0min >= tmin) && (c0min <= ray_tmax);
+            t1 = (c1min <= c1max) && (c1min >= tmin) && (c1min <= ray_tmax);
+            c1_first = c1min < c0min;
+            if (REC == 2) {
+                const uint32_t code = (t0 ? 1u : 0u) | (t1 ? 2u : 0u) | ((t0 && t1 && c1_first) ? 4u : 0u);
+                rec_word |= code << ((rec_n & 7) * 4);
+                if ((rec_n & 7) == 7) {
+                    if ((rec_n >> 3) < rec_cap) a.rec_log[rec_base + (rec_n >> 3)] = (int32_t)rec_word;
+                    rec_word = 0;
+                }
+            }
+            if (REC) ++rec_n;
+            }
             // the child indices are used only when a child is entered; without a use on
             // every path the compiler issues their load there, after the slab tests,
             // which makes it a second dependent fetch per visited node. This empty
@@ -229,7 +292,7 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
                 int far = ch.y;
                 node_addr = t0 ? ch.x : ch.y;
                 if (t0 && t1) {
-                    if (c1min < c0min) {
+                    if (c1_first) {
                         int tmp = node_addr;
                         node_addr = far;
                         far = tmp;
@@ -250,11 +313,22 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
 
         while (leaf_addr < 0) {
             int tri = ~leaf_addr;
+            if (REC) ++rec_l;
             for (;;) {
+                if (REC) ++rec_t;
                 const float4 *tp = a.tris + (int64_t)tri * 3;
                 const float4 A = tp[0];
                 const float4 B = tp[1];
                 const float4 C = tp[2];
+                if (REC == 3) {
+                    // the record is fetched and its words used; no test
+                    tok ^= (__float_as_uint(A.x) ^ __float_as_uint(A.y) ^ __float_as_uint(A.z) ^ __float_as_uint(A.w)) ^
+                           (__float_as_uint(B.x) ^ __float_as_uint(B.y) ^ __float_as_uint(B.z) ^ __float_as_uint(B.w)) ^
+                           (__float_as_uint(C.x) ^ __float_as_uint(C.y) ^ __float_as_uint(C.z));
+                    if (fb(C.w) != 0) break;
+                    ++tri;
+                    continue;
+                }
                 // Möller–Trumbore, IntersectKernel.cuh:230-254 (edges precomputed: same bits)
                 const float tvx = ox - A.x, tvy = oy - A.y, tvz = oz - A.z;
                 const float pvx = dy * C.z - dz * C.y;
@@ -350,6 +424,22 @@ hipError_t launch_v(const TraceArgs &a, bool persistent, int variant, bool any, 
 }
 
 }  // namespace
+
+hipError_t launch_trace_record(const TraceArgs &a, int variant, int pass, int block, int refill, int num_cus,
+                               hipStream_t s) {
+    if (a.n <= 0) return hipSuccess;
+    if (block < 64 || block > 256 || block % 64 != 0) return hipErrorInvalidValue;
+    TraceArgs b = a;
+    b.xcd_order = 0;
+    const bool persistent = refill > 0;
+    b.refill = persistent ? (refill > 64 ? 64 : refill) : 64;
+    const size_t lds = (size_t)16 * block * sizeof(int);
+#define TT_REC(V, P) \
+    if (variant == V && pass == P) return launch_k(k_trace<V, false, 16, kTriReference, P>, b, persistent, block, lds, num_cus, s);
+    TT_REC(0, 1) TT_REC(0, 2) TT_REC(0, 3) TT_REC(1, 1) TT_REC(1, 2) TT_REC(1, 3)
+#undef TT_REC
+    return hipErrorInvalidValue;
+}
 
 hipError_t launch_trace(const TraceArgs &a, int variant, bool any, int lds_stack, int block, int xcd_order,
                         int refill, int num_cus, hipStream_t s, int tri_mode) {

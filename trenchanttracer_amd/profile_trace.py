@@ -1,8 +1,10 @@
 """Profiling cases.
 
-traversal (default): one closest-hit traversal launch over the traversal workload
-(1M-triangle soup SBVH, 8,294,400 rays), then one 256 MiB device-to-device copy
-whose known byte counts calibrate the FETCH_SIZE / WRITE_SIZE hardware counters.
+traversal (default): one closest-hit traversal call over the traversal workload
+(1M-triangle soup SBVH, 8,294,400 rays), one random 64-byte gather probe (2^26
+independent reads over a buffer the size of the device tree), then one 256 MiB
+device-to-device copy whose known byte counts calibrate the FETCH_SIZE / WRITE_SIZE
+hardware counters.
 pathtrace: the path-trace case (1,310,720-tri displaced icosphere, sky + sun,
 1920x1080, depth 16) rendered once at 4 spp (warm-up) and once at 64 spp, then the
 same calibration copy.
@@ -16,6 +18,8 @@ For profiler runs, one counter set per run:
 """
 import argparse
 import sys
+
+GATHER_FETCHES = 1 << 26
 
 
 def _copy_calibration(ctx, nbytes=256 << 20):
@@ -41,6 +45,7 @@ def main():
     ap.add_argument("--reorder", type=int, default=0, help="traversal: 1 = trace in Morton order of the origins")
     ap.add_argument("--case", choices=("traversal", "pathtrace", "train"), default="traversal")
     ap.add_argument("--paths-in-flight", type=int, default=0, help="pathtrace: paths per pass (0: the default)")
+    ap.add_argument("--ntris", type=int, default=1_000_000, help="traversal: triangles of the soup")
     args = ap.parse_args()
     import trenchanttracer_amd as tt
     from trenchanttracer_amd import workloads
@@ -76,7 +81,7 @@ def main():
         ctx.close()
         return
 
-    mesh, bvh, rays, n_prim = workloads.traversal()
+    mesh, bvh, rays, n_prim = workloads.traversal(ntris=args.ntris)
     n = len(rays)
     ctx = tt.Context(0)
     scene = tt.DeviceScene(ctx, mesh, bvh)
@@ -86,6 +91,14 @@ def main():
                        reorder=args.reorder)
     scene.trace_closest(d_rays, n, d_hits, cfg)
     ctx.sync()
+    # independent random 64-byte reads over a buffer the size of the device tree
+    # (2^26 reads, seed 1): the fetch-throughput ceiling the traversal's traffic is
+    # compared with
+    info = scene.info()
+    probe = tt.DeviceBuffer(ctx, info["device_bytes"])
+    probe.zero()
+    tt.gather64_gbps(ctx, probe, info["device_bytes"], fetches=GATHER_FETCHES, seed=1)
+    probe.free()
     nbytes = _copy_calibration(ctx)
     print(f"traced {n} rays (variant {args.variant}, refill {args.refill}, reorder {args.reorder}); "
           f"copied {nbytes} bytes", file=sys.stderr, flush=True)

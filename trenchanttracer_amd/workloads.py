@@ -107,19 +107,35 @@ def sky_env(width=2048, height=1024, sun_elev_deg=35.0, sun_azim_deg=40.0, sun_d
 def pathtrace(subdiv=8, width=1920, height=1080, env_w=2048, env_h=1024, seed=1):
     """Full path-trace case: icosphere subdivided `subdiv` times (1,310,720 tris at 8)
     with seeded fBm radial displacement (6 octaves, amplitude 0.05); per-face
-    materials 40% diffuse / 40% GGX conductor (roughness in 16 bins spanning
-    [0.05, 0.5]) / 20% dielectric IOR 1.5; the procedural sky; pinhole camera.
-    Returns (mesh, materials, env, camera)."""
-    from . import MAT_CONDUCTOR, MAT_DIELECTRIC, MAT_DIFFUSE, camera, material
+    materials 40% diffuse / 40% GGX conductor (each face its own roughness drawn from
+    U[0.05, 0.5], gold or silver F0 alternating) / 20% dielectric IOR 1.5; the
+    procedural sky; pinhole camera. Returns (mesh, materials, env, camera); the
+    materials are a ctypes array of tt_material (one per conductor face plus the
+    diffuse and dielectric ones)."""
+    from . import camera
+    from ._native import tt_material
     mesh = Mesh.icosphere(subdiv, 1.0, (0.0, 0.0, 0.0), amp=0.05, octaves=6, seed=seed, mat=0)
-    mats = [material(MAT_DIFFUSE, (0.8, 0.7, 0.6))]
-    nbins = 16
-    for k in range(nbins):
-        rough = 0.05 + 0.45 * (k + 0.5) / nbins
-        mats.append(material(MAT_CONDUCTOR, (0.95, 0.78, 0.4) if k % 2 else (0.92, 0.92, 0.9), roughness=rough))
-    mats.append(material(MAT_DIELECTRIC, (1.0, 1.0, 1.0), ior=1.5))
-    fr = [0.4] + [0.4 / nbins] * nbins + [0.2]
-    mesh.assign_materials(fr, list(range(nbins + 2)), seed=seed)
+    mesh.assign_materials([0.4, 0.4, 0.2], [0, 1, 2], seed=seed)
+    a = mesh.arrays()
+    tm = a["tri_mats"]
+    cond = np.flatnonzero(tm == 1)
+    k = len(cond)
+    rng = np.random.default_rng(seed + 7)
+    rec = np.zeros(k + 2, dtype=[("type", "<i4"), ("albedo", "<f4", 3), ("emission", "<f4", 3),
+                                 ("roughness", "<f4"), ("ior", "<f4"), ("specular", "<f4", 3)])
+    assert rec.dtype.itemsize == 48
+    rec["ior"] = 1.5
+    rec["specular"] = 1.0
+    rec[0]["type"], rec[0]["albedo"] = 0, (0.8, 0.7, 0.6)
+    rec[1:k + 1]["type"] = 1
+    gold = (np.arange(k) % 2 == 1)[:, None]
+    rec["albedo"][1:k + 1] = np.where(gold, np.float32([0.95, 0.78, 0.4]), np.float32([0.92, 0.92, 0.9]))
+    rec["roughness"][1:k + 1] = rng.uniform(0.05, 0.5, k).astype(np.float32)
+    rec[k + 1]["type"], rec[k + 1]["albedo"] = 2, (1.0, 1.0, 1.0)
+    new_tm = np.where(tm == 0, 0, k + 1).astype(np.int32)
+    new_tm[cond] = 1 + np.arange(k, dtype=np.int32)
+    mesh = Mesh.from_arrays(a["verts"], a["tris"], a["normals"], a["tri_normals"], new_tm)
+    mats = (tt_material * (k + 2)).from_buffer_copy(rec.tobytes())
     env = sky_env(env_w, env_h)
     cam = camera((0.0, 0.35, 3.3), (0.0, -0.1, -1.0), fov=(40.0, 40.0 * height / width), width=width,
                  height=height, focal=1.0)
