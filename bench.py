@@ -419,11 +419,13 @@ def guided_bench(tt, ctx, np):
     """Learned guiding on the path-trace case (1,310,720 tris, 1920x1080, depth 16):
     equal-time RMSE against an FP64-accumulated 8192-spp render of the plain tracer.
     Plain renders at 64 and 128 spp; guided renders (8-lobe vMF MLP, BF16 MFMA, Adam
-    lr 1e-3 on 2^18-row batches, one-sample MIS with the BSDF) trained online from
-    scratch throughout, trained during the first half of the samples, and frozen after
-    the 128-spp online run; two seeds per point. The time at which each reaches the
-    plain 64-spp RMSE comes from a log-log fit of RMSE against time
-    (trenchanttracer_amd/bench_guide.py, which also runs the longer sweep)."""
+    lr 1e-3 on 2^18-row batches, one-sample MIS with the BSDF, the selection
+    probability learned per vertex) trained online from scratch throughout (passes
+    growing 4x from 1 spp, up to 2M recorded paths each), trained during the first
+    half of the samples, and frozen after the 128-spp online run; two seeds per point.
+    The time at which each reaches the plain 64-spp RMSE comes from a log-log fit of
+    RMSE against time (trenchanttracer_amd/bench_guide.py, which also runs the longer
+    sweep)."""
     from trenchanttracer_amd import bench_guide, workloads
     mesh, mats, env, cam = workloads.pathtrace()
     sc = tt.DeviceScene(ctx, mesh, tt.BVH.build(mesh, nthreads=16), mats, env=env)
@@ -431,8 +433,8 @@ def guided_bench(tt, ctx, np):
     _, ref, st = sc.render(cam, tt.render_params(spp=8192, max_depth=16, rr_depth=4, seed=1234, mode=tt.RENDER_PATH,
                                                  fp64=True), want_fp64=True)
     log(f"guided leg: FP64 8192-spp oracle render in {time.time() - t0:.1f}s")
-    r = bench_guide.compare(tt, ctx, sc, cam, ref, 16, (64, 128), (64, 128), 2 * 1920 * 1080, 1e-3, 262144, 65536,
-                            seeds=2)
+    r = bench_guide.compare(tt, ctx, sc, cam, ref, 16, (64, 128), (64, 128), 1920 * 1080, 1e-3, 262144, 2 << 20,
+                            seeds=2, pass_growth=4, learn_alpha=True)
     sc.close()
     p64 = next(x for x in r["plain"] if x["spp"] == 64)
     g64 = next(x for x in r["guided_online"] if x["spp"] == 64)

@@ -501,6 +501,11 @@ typedef struct tt_guide_params {
     int32_t train_spp;    /* samples per pixel after which training stops (0: never) */
     int32_t max_bounce;   /* the network guides bounces 0 .. max_bounce - 1 (0: every bounce);
                              later vertices sample the BSDF alone */
+    int32_t pass_growth;  /* 0 or 1: training passes of constant width; g >= 2: each training pass g times
+                             wider than the one before, up to the width of an untrained render */
+    int32_t learn_alpha;  /* 1: the selection probability is the network's own per-vertex output
+                             (0.02 .. 0.95), trained on the cross-entropy of the combined density
+                             alpha q + (1 - alpha) p_bsdf; alpha is then ignored */
 } tt_guide_params;
 
 typedef struct tt_guide_stats {

@@ -37,7 +37,7 @@ def test_struct_sizes(tt):
     assert ctypes.sizeof(N.tt_camera) == 60
     assert ctypes.sizeof(N.tt_render_params) == 76
     assert ctypes.sizeof(N.tt_render_stats) == 40
-    assert ctypes.sizeof(N.tt_guide_params) == 28
+    assert ctypes.sizeof(N.tt_guide_params) == 36
     assert ctypes.sizeof(N.tt_guide_stats) == 32
 
 

@@ -8,7 +8,8 @@ equals eager launches, and the autotuner's choice gives the same images.
 
 Guided: the path-trace case (1,310,720-tri displaced icosphere, 1920x1080, depth 16)
 rendered with the network trained online from scratch (Adam lr 1e-3, 2^18-row
-batches, one-sample MIS with the BSDF): the image is finite, the training loss falls,
+batches, one-sample MIS with the BSDF, learned selection probability, training passes
+growing 4x from 1 spp): the image is finite, the training loss falls,
 and the image agrees in the mean with an FP64-accumulated plain render once the
 worst 0.1% of pixels of either image (the sun caustics' fireflies) are set aside.
 """
@@ -61,11 +62,11 @@ def test_guided_pathtrace_case_trains_and_agrees_with_plain(tt, ctx):
                                                fp64=True), want_fp64=True)
     g = tt.Guide(ctx, seed=1, lr=1e-3)
     img, st, gs = g.render(s, cam, tt.render_params(spp=64, max_depth=16, rr_depth=4, seed=77, mode=tt.RENDER_PATH,
-                                                    paths_in_flight=2 * 1920 * 1080),
-                           alpha=0.5, train=True, train_paths=65536, train_batch=1 << 18)
+                                                    paths_in_flight=1920 * 1080),
+                           train=True, train_paths=2 << 20, train_batch=1 << 18, pass_growth=4, learn_alpha=True)
     _say(f"  render {st}; training {gs}")
     assert np.isfinite(img).all()
-    assert gs["train_steps"] >= 8
+    assert gs["train_steps"] >= 4
     assert gs["last_loss"] < gs["first_loss"], gs
     lum_g, lum_r = img.mean(-1), ref.mean(-1)
     cut = max(np.quantile(lum_g, 0.999), np.quantile(lum_r, 0.999))

@@ -113,3 +113,41 @@ def test_training_learns_conditional_lobes(tt, ctx):
     assert np.median(lq_t) > np.log(1 / (4 * np.pi)) + 2.0
     assert np.median(lq_t) > np.median(lq_o) + 2.0
     g.close()
+
+
+def test_adam_step_matches_fp32_restatement(tt, ctx):
+    """Two Adam steps (lr 1e-3, beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected) of the
+    device update equal a numpy float32 restatement of
+    m = b1 m + (1 - b1) g, v = b2 v + (1 - b2) g^2, p -= lr (m / c1) / (sqrt(v / c2) + eps)
+    with c_t = 1 - b^t, using the gradients the device computes for the same rows
+    (tt_guide_grad). A missing bias correction or a wrong moment update moves the
+    parameters by a different amount (the update is lr at the first step)."""
+    f = np.float32
+    rng = np.random.default_rng(17)
+    g = tt.Guide(ctx, seed=3, lr=1e-3)
+    m = np.zeros(g.nparams, f)
+    v = np.zeros(g.nparams, f)
+    b1, b2, lr, eps = f(0.9), f(0.999), f(1e-3), f(1e-8)
+    for step in (1, 2):
+        n = 3000
+        inp = rng.random((n, 9)).astype(f)
+        d = rng.normal(size=(n, 3))
+        d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(f)
+        w = rng.gamma(1.0, 1.0, n).astype(f)
+        p0 = g.params()
+        _, gr = g.grad(inp, d, w)
+        g.train_step(inp, d, w)
+        p1 = g.params()
+        m = b1 * m + (f(1) - b1) * gr
+        v = b2 * v + (f(1) - b2) * gr * gr
+        c1 = f(1) - b1 ** f(step)
+        c2 = f(1) - b2 ** f(step)
+        want = p0 - lr * (m / c1) / (np.sqrt(v / c2) + eps)
+        upd, upd_want = p1 - p0, want - p0
+        moved = np.abs(upd_want) > 1e-6
+        assert moved.mean() > 0.5
+        rel = np.abs(upd[moved] - upd_want[moved]) / np.abs(upd_want[moved])
+        _say(f"adam step {step}: max relative difference of the update {rel.max():.2e}")
+        assert rel.max() < 1e-3
+        assert np.abs(p1 - want).max() < 1e-6
+    g.close()

@@ -111,6 +111,20 @@ def test_guided_render_unbiased_in_emissive_box(tt, ctx):
     assert gs["train_steps"] > 0
     assert st["paths_in_flight"] == 64 * 64 * 256
     assert abs(img.mean() - 1) < 0.01
+    # the selection probability learned per vertex (network output 40), training
+    # passes growing 4x from 1 spp: still unbiased, with a constant guide (alpha from
+    # the network's own output) and online from scratch
+    g4 = tt.Guide(ctx, seed=5)
+    pg = tt.render_params(spp=256, max_depth=32, rr_depth=4, seed=6, mode=tt.RENDER_PATH, paths_in_flight=64 * 64)
+    img, st, gs = g4.render(sc, cam, pg, train=True, train_paths=4096, train_batch=8192, pass_growth=4,
+                            learn_alpha=True)
+    _say(f"box, learned alpha, growing passes: mean {img.mean():.5f}, {st}, {gs}")
+    assert gs["train_steps"] > 0
+    assert abs(img.mean() - 1) < 0.01
+    _const_guide(g, _raw([(0.3, 0.8, 0.5)] * 8, [np.log(5.0)] * 8, [0] * 8))
+    img, _, _ = g.render(sc, cam, p, train=False, learn_alpha=True)
+    assert abs(img.mean() - 1) < 0.005
+    g4.close()
     g3.close()
     g.close()
     g2.close()

@@ -206,8 +206,8 @@ def test_full_traversal_workload_bit_exact(tt, oracle, ctx):
     ids1, t1, *_ = s.intersect(rays, tt.trace_cfg(1, 16, 128))
     bad = np.nonzero(ids1 != ids)[0]
     _say(f"speculative variant: {len(bad)} differing ids of {len(rays)}")
-    for i in bad[:1000]:
-        assert abs(t1[i] - t[i]) <= 1e-6 * max(1.0, abs(t[i]))
+    # every differing id is a near tie (both hits at the same distance within rounding)
+    assert np.all(np.abs(t1[bad] - t[bad]) <= 1e-6 * np.maximum(1.0, np.abs(t[bad])))
     # the headline configuration (speculative loop, 256-thread blocks, persistent waves
     # refilling at 32 idle lanes, rays traced in Morton order of their origins) and the
     # exact loop in the same launch: every record at its own index, bit-identical

@@ -361,11 +361,12 @@ class Guide:
         return loss.value
 
     def render(self, scene, cam, params, alpha=0.5, train=True, train_paths=0, train_batch=0, train_spp=0,
-               min_roughness=0.3, want_fp64=False, max_bounce=0):
+               min_roughness=0.3, want_fp64=False, max_bounce=0, pass_growth=0, learn_alpha=False):
         """Guided path-traced image of `scene` (tt_guide_render): returns (image, stats,
         guide stats) or (image, image64, stats, guide stats) with want_fp64."""
         gp = N.tt_guide_params(float(alpha), float(min_roughness), int(bool(train)), int(train_paths),
-                               int(train_batch), int(train_spp), int(max_bounce))
+                               int(train_batch), int(train_spp), int(max_bounce), int(pass_growth),
+                               int(bool(learn_alpha)))
         out = np.empty((cam.height, cam.width, 3), np.float32)
         out64 = np.empty((cam.height, cam.width, 3), np.float64) if want_fp64 else None
         st = N.tt_render_stats()

@@ -86,7 +86,8 @@ class tt_config_info(C.Structure):
 
 class tt_guide_params(C.Structure):
     _fields_ = [("alpha", C.c_float), ("min_roughness", C.c_float), ("train", C.c_int32), ("train_paths", C.c_int32),
-                ("train_batch", C.c_int32), ("train_spp", C.c_int32), ("max_bounce", C.c_int32)]
+                ("train_batch", C.c_int32), ("train_spp", C.c_int32), ("max_bounce", C.c_int32),
+                ("pass_growth", C.c_int32), ("learn_alpha", C.c_int32)]
 
 
 class tt_guide_stats(C.Structure):

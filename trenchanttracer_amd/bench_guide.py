@@ -51,7 +51,7 @@ def _time_to_rmse(points, target):
     return float(t * (r / target) ** 2)
 
 
-def compare(tt, ctx, sc, cam, ref, depth, plain_spp, guided_spp, paths, lr, batch, train_paths, seeds=1):
+def compare(tt, ctx, sc, cam, ref, depth, plain_spp, guided_spp, paths, lr, batch, train_paths, seeds=1, **guide_kw):
     def err(img):
         e = ((img.astype(np.float64) - ref) ** 2).sum(-1).ravel()
         k = max(1, e.size // 1000)
@@ -82,7 +82,7 @@ def compare(tt, ctx, sc, cam, ref, depth, plain_spp, guided_spp, paths, lr, batc
                 g.close()
             g = tt.Guide(ctx, seed=1 + j, lr=lr)
             img, st, gs = g.render(sc, cam, rp(spp, 2000 * spp + j, paths_in_flight=paths), alpha=0.5, train=True,
-                                   train_paths=train_paths, train_batch=batch)
+                                   train_paths=train_paths, train_batch=batch, **guide_kw)
             runs.append(err(img) + (st["seconds"],))
             train.append(gs)
         out["guided_online"].append(dict(point(spp, runs), train_seconds=float(np.mean([t["train_seconds"] for t in train])),
@@ -95,7 +95,7 @@ def compare(tt, ctx, sc, cam, ref, depth, plain_spp, guided_spp, paths, lr, batc
         for j in range(seeds):
             gh = tt.Guide(ctx, seed=1 + j, lr=lr)
             img, st, gs = gh.render(sc, cam, rp(spp, 4000 * spp + j, paths_in_flight=paths), alpha=0.5, train=True,
-                                    train_paths=train_paths, train_batch=batch, train_spp=spp // 2)
+                                    train_paths=train_paths, train_batch=batch, train_spp=spp // 2, **guide_kw)
             runs.append(err(img) + (st["seconds"],))
             gh.close()
         out["guided_online_half"].append(point(spp, runs))
@@ -103,7 +103,8 @@ def compare(tt, ctx, sc, cam, ref, depth, plain_spp, guided_spp, paths, lr, batc
     for spp in plain_spp:
         runs = []
         for j in range(seeds):
-            img, st, _ = g.render(sc, cam, rp(spp, 3000 * spp + j), alpha=0.5, train=False)
+            img, st, _ = g.render(sc, cam, rp(spp, 3000 * spp + j), alpha=0.5, train=False,
+                                  learn_alpha=guide_kw.get("learn_alpha", False))
             runs.append(err(img) + (st["seconds"],))
         out["guided_pretrained"].append(point(spp, runs))
         log(f"guided pretrained {spp}: {out['guided_pretrained'][-1]}")

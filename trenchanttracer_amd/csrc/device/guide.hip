@@ -281,11 +281,16 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(InferArgs a) {
     }
 }
 
-// loss = sum_r s_r * (-log q(w_r)) / sum_r s_r; writes d loss / d raw (bf16, 48 wide + pad to 64)
+// loss = sum_r s_r * (-log q(w_r)) / sum_r s_r; writes d loss / d raw (bf16, 48 wide + pad to 64).
+// With pb (the BSDF density of each row's direction), output kGuideAlphaOut also gets
+// the gradient of sum_r s_r * (-log(alpha q + (1 - alpha) p_b)) / sum_r s_r, the
+// cross-entropy of the one-sample MIS density, through alpha = learned_alpha(z): the
+// selection probability falls where the mixture describes the incident radiance worse
+// than the BSDF density does (Mueller et al. 2019, "Neural Importance Sampling").
 __global__ __launch_bounds__(256) void k_loss_grad(const float *__restrict__ raw, const float *__restrict__ dirs,
-                                                   const float *__restrict__ wts, __bf16 *__restrict__ draw,
-                                                   float *__restrict__ loss_out, const float *__restrict__ wsum,
-                                                   int64_t n) {
+                                                   const float *__restrict__ wts, const float *__restrict__ pb,
+                                                   __bf16 *__restrict__ draw, float *__restrict__ loss_out,
+                                                   const float *__restrict__ wsum, int64_t n) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     __shared__ float red[256];
     float contrib = 0.0f;
@@ -328,6 +333,14 @@ __global__ __launch_bounds__(256) void k_loss_grad(const float *__restrict__ raw
             g[3 * k] = (__bf16)(f * (wx - ca * L.mu[k][0]));
             g[3 * k + 1] = (__bf16)(f * (wy - ca * L.mu[k][1]));
             g[3 * k + 2] = (__bf16)(f * (wz - ca * L.mu[k][2]));
+        }
+        if (pb) {
+            const float q = expf(logq), p_b = pb[r];
+            const float sg = sigmoidf_(rr[kGuideAlphaOut]);
+            const float al = kAlphaLo + (kAlphaHi - kAlphaLo) * sg;
+            const float pc = fmaxf(al * q + (1.0f - al) * p_b, 1e-30f);
+            const float ga = -scale * (q - p_b) / pc * (kAlphaHi - kAlphaLo) * sg * (1.0f - sg);
+            g[kGuideAlphaOut] = (__bf16)(isfinite(ga) ? ga : 0.0f);
         }
     }
     red[threadIdx.x] = contrib;
@@ -537,7 +550,7 @@ static int64_t layer_offset(int k) {
 tt_status guide_alloc_batch(tt_guide *g, int64_t rows) {
     if (g->cap >= rows) return TT_OK;
     GuideBatch &b = g->batch;
-    void *ptrs[] = {b.in, b.x, b.h[0], b.h[1], b.h[2], b.raw, b.draw, b.dh[0], b.dh[1], b.dh[2], b.dirs, b.wts};
+    void *ptrs[] = {b.in, b.x, b.h[0], b.h[1], b.h[2], b.raw, b.draw, b.dh[0], b.dh[1], b.dh[2], b.dirs, b.wts, b.pb};
     for (void *q : ptrs)
         if (q) hipFree(q);
     b = GuideBatch();
@@ -555,6 +568,7 @@ tt_status guide_alloc_batch(tt_guide *g, int64_t rows) {
     for (int k = 0; k < 3; ++k) al((void **)&b.dh[k], (size_t)R * kGuideH * 2, "guide activation gradients");
     al((void **)&b.dirs, (size_t)R * 3 * 4, "guide directions");
     al((void **)&b.wts, (size_t)R * 4, "guide weights");
+    al((void **)&b.pb, (size_t)R * 4, "guide BSDF densities");
     if (st == TT_OK) g->cap = R;
     return st;
 }
@@ -649,8 +663,9 @@ static hipError_t guide_backward(tt_guide *g, int64_t n, hipStream_t s) {
     e = hipMemsetAsync(g->scalars, 0, 16, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_sum, dim3(256), dim3(256), 0, s, b.wts, n, g->scalars + 1);
-    hipLaunchKernelGGL(k_loss_grad, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, s, b.raw, b.dirs, b.wts, b.draw,
-                       g->scalars, (const float *)(g->scalars + 1), n);
+    hipLaunchKernelGGL(k_loss_grad, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, s, b.raw, b.dirs, b.wts,
+                       g->train_alpha ? (const float *)b.pb : (const float *)nullptr, b.draw, g->scalars,
+                       (const float *)(g->scalars + 1), n);
     const unsigned gw = (unsigned)std::min<int64_t>(((R / 16) + 3) / 4 + 1, 8192);
     // dH3 = dRaw W4 (mask H3 > 0), dH2 = dH3 W3 (mask), dH1 = dH2 W2 (mask)
     hipLaunchKernelGGL((k_gemm<kGuideNOutPad, kGuideH, false, true>), dim3(gw), dim3(256), 0, s, b.draw, g->wb[3],
@@ -784,6 +799,9 @@ tt_status tt_guide_create(tt_ctx *ctx, uint32_t seed, float lr, tt_guide **out) 
             w[e] = (k == 3 && o >= kGuideNOut) ? 0.0f : (float)(sd * z);
         }
     }
+    // the learned selection probability starts low (alpha = learned_alpha(-3) = 0.064):
+    // an untrained mixture is sampled rarely until training raises it where it helps
+    p[(size_t)layer_offset(3) + (size_t)kGuideLayers[3].in * kGuideLayers[3].out + kGuideAlphaOut] = -3.0f;
     hipError_t e = hipMemcpy(g->params, p.data(), (size_t)np * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(g->m1, 0, (size_t)np * 4);
     if (e == hipSuccess) e = hipMemset(g->m2, 0, (size_t)np * 4);
@@ -808,7 +826,7 @@ void tt_guide_destroy(tt_guide *g) {
     hipStreamSynchronize(g->ctx->stream);
     GuideBatch &b = g->batch;
     void *ptrs[] = {g->params, g->m1, g->m2, g->grad, g->scalars, g->partial, b.in, b.x, b.h[0], b.h[1], b.h[2],
-                    b.raw, b.draw, b.dh[0], b.dh[1], b.dh[2], b.dirs, b.wts, g->wf[0], g->wf[1], g->wf[2], g->wf[3],
+                    b.raw, b.draw, b.dh[0], b.dh[1], b.dh[2], b.dirs, b.wts, b.pb, g->wf[0], g->wf[1], g->wf[2], g->wf[3],
                     g->wb[0], g->wb[1], g->wb[2], g->wb[3], g->train_buf, g->train_count, g->rec, g->iin, g->iraw,
                     g->irow};
     for (void *q : ptrs)

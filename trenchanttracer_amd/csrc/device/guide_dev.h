@@ -15,6 +15,10 @@ constexpr int kGuideK1 = 224;        // features padded to a multiple of 32
 constexpr int kGuideH = 64;          // hidden width (3 hidden layers)
 constexpr int kGuideLobes = 8;
 constexpr int kGuideNOut = kGuideLobes * 5;  // 40: mean (3), log kappa (1), logit (1) per lobe
+// output 40 (in the padding of the last layer): the logit z of the learned selection
+// probability of the mixture, alpha = kAlphaLo + (kAlphaHi - kAlphaLo) sigmoid(z)
+constexpr int kGuideAlphaOut = kGuideNOut;
+constexpr float kAlphaLo = 0.02f, kAlphaHi = 0.95f;
 constexpr int kGuideNOutPad16 = 48;  // output columns computed (multiple of 16); raw row stride
 constexpr int kGuideNOutPad = 64;    // output-gradient width (multiple of 32, the backward K)
 constexpr float kGuideLogKappaMin = -5.0f, kGuideLogKappaMax = 9.0f;
@@ -34,7 +38,11 @@ struct GuideBatch {
     __bf16 *dh[3] = {nullptr, nullptr, nullptr};  // [R][64]
     float *dirs = nullptr;    // [R][3] training directions
     float *wts = nullptr;     // [R] training weights (radiance / pdf)
+    float *pb = nullptr;      // [R] BSDF density of each direction (learned alpha), or unset
 };
+
+__device__ __forceinline__ float sigmoidf_(float z) { return 1.0f / (1.0f + __expf(-z)); }
+__device__ __forceinline__ float learned_alpha(float z) { return kAlphaLo + (kAlphaHi - kAlphaLo) * sigmoidf_(z); }
 
 // ------------------------------------------------------------------ vMF mixture
 struct Lobes {
@@ -193,9 +201,10 @@ struct tt_guide {
     float lr = 1e-3f, beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
     int64_t step = 0;
     // online training from the renderer
-    float *train_buf = nullptr;   // [cap][16] samples: inputs (9), dir (3), weight (1)
+    float *train_buf = nullptr;   // [cap][16] samples: inputs (9), dir (3), weight (1), BSDF density (1)
     int32_t *train_count = nullptr;
     int64_t train_cap = 0;
     float *rec = nullptr;         // per training path vertex records
     int64_t rec_cap = 0;
+    bool train_alpha = false;     // the staged batch carries BSDF densities: train the alpha output
 };

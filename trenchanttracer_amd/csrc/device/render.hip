@@ -416,10 +416,12 @@ __device__ __forceinline__ Surf surface(const SceneView &sv, const tt_hit &h, f3
 }
 
 // Guiding inputs of the shade stage (all null / zero when unguided).
-constexpr int kRecFloats = 20;  // inputs (9), wi (3), pdf, 1 / throughput after the vertex (3), radiance (3), pad
+constexpr int kRecFloats = 20;  // inputs (9), wi (3), pdf, 1 / throughput after the vertex (3), radiance (3),
+                                // BSDF density of wi
 struct GuideShade {
     const float *raw;        // [slot][48] mixture of this bounce's queue slots (null: unguided)
     float alpha;             // probability of sampling the guide (one-sample MIS with the BSDF)
+    int32_t learn_alpha;     // 1: alpha from the network's output kGuideAlphaOut per vertex
     float *rec;              // training records [ntrain][max_depth][kRecFloats] (null: no training)
     int32_t stride, ntrain;  // training paths: (path id + toff) % stride == 0, (id + toff) / stride < ntrain
     int32_t toff;            // per-pass offset, so that the recorded pixels change from pass to pass
@@ -1004,7 +1006,10 @@ __device__ __forceinline__ void shade_path_one(const ShadeArgs &a, const int32_t
                 const bool guided = GUIDED && a.gd.raw != nullptr && guide_eligible(m, a.gd.min_rough);
                 Mixture lobes;
                 if (guided) decode_mixture(a.gd.raw + (int64_t)i * kGuideNOutPad16, lobes);
-                const float alpha = guided ? a.gd.alpha : 0.0f;
+                const float alpha = !guided ? 0.0f
+                                            : a.gd.learn_alpha
+                                                  ? learned_alpha(a.gd.raw[(int64_t)i * kGuideNOutPad16 + kGuideAlphaOut])
+                                                  : a.gd.alpha;
                 // next-event estimation
                 if (!smooth && (p_env > 0.0f || p_emit > 0.0f) && wol.z > 0.0f) {
                     f3 wi, le, ylight = F3(0, 0, 0);
@@ -1060,6 +1065,7 @@ __device__ __forceinline__ void shade_path_one(const ShadeArgs &a, const int32_t
                 // (defined even where no direction is sampled: such paths end)
                 f3 wil = F3(0.0f, 0.0f, 1.0f), weight = F3(0.0f, 0.0f, 0.0f);
                 float pdf_b = 0.0f;
+                float pdf_bsdf = 0.0f;   // the BSDF's own density of the direction (guided vertices)
                 bool delta = false;
                 bool sampled = false;
                 float w_rr;  // BSDF-sampling weight of the direction (max component), for roulette
@@ -1078,6 +1084,7 @@ __device__ __forceinline__ void shade_path_one(const ShadeArgs &a, const int32_t
                         f_cos = sampled ? weight * pdf_b : F3(0, 0, 0);
                     }
                     const f3 wiw = to_world(fr, wil);
+                    pdf_bsdf = pdf_b;
                     const float pdf_c = alpha * mixture_density(lobes, wiw.x, wiw.y, wiw.z) + (1.0f - alpha) * pdf_b;
                     sampled = pdf_c > 0.0f && maxc(f_cos) > 0.0f && wol.z > 0.0f;
                     weight = sampled ? f_cos * (1.0f / pdf_c) : F3(0, 0, 0);
@@ -1125,6 +1132,7 @@ __device__ __forceinline__ void shade_path_one(const ShadeArgs &a, const int32_t
                                 rec_b[14] = thr.y > 0.0f ? 1.0f / thr.y : 0.0f;
                                 rec_b[15] = thr.z > 0.0f ? 1.0f / thr.z : 0.0f;
                                 rec_b[16] = rec_b[17] = rec_b[18] = 0.0f;
+                                rec_b[19] = pdf_bsdf;
                                 rec_b[12] = pdf_b;
                             }
                         }
@@ -1336,11 +1344,13 @@ __global__ __launch_bounds__(256) void k_collect(float *rec, int32_t ntrain, int
         float *o = buf + (int64_t)slot * 16;
         for (int k = 0; k < 12; ++k) o[k] = r[k];
         o[12] = w;
+        o[13] = r[19];
     }
     if (live) r[12] = 0.0f;
 }
 
-__global__ __launch_bounds__(256) void k_stage_train(const float *buf, int32_t n, float *in, float *dirs, float *wts) {
+__global__ __launch_bounds__(256) void k_stage_train(const float *buf, int32_t n, float *in, float *dirs, float *wts,
+                                                     float *pb) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float *b = buf + (int64_t)i * 16;
@@ -1349,6 +1359,7 @@ __global__ __launch_bounds__(256) void k_stage_train(const float *buf, int32_t n
     dirs[3 * (int64_t)i + 1] = b[10];
     dirs[3 * (int64_t)i + 2] = b[11];
     wts[i] = b[12];
+    pb[i] = b[13];
 }
 
 __global__ void k_zero(float *acc, double *acc64, unsigned long long *stats, int64_t n3) {
@@ -1515,6 +1526,7 @@ struct GuideRun {
     int32_t batch_rows = 1 << 18;
     int32_t train_spp = 0;    // train only while fewer samples than this are done (0: always)
     int32_t max_bounce = 0;   // the network guides bounces below this (0: every bounce)
+    int32_t growth = 1;       // each training pass this many times wider than the one before (up to the widest)
     int64_t steps = 0, rows = 0;
     double train_ms = 0.0;
     float first_loss = 0.0f;  // loss of the first Adam step
@@ -1805,9 +1817,11 @@ hipError_t guide_train_pass(GuideRun *gr, int32_t max_depth, hipStream_t hs) {
         if (off > 0 && n < gr->batch_rows / 4) break;
         hipLaunchKernelGGL(k_stage_train, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, hs,
                            (const float *)(g->train_buf + (int64_t)off * 16), n, g->batch.in, g->batch.dirs,
-                           g->batch.wts);
+                           g->batch.wts, g->batch.pb);
         e = hipGetLastError();
+        g->train_alpha = gr->gd.learn_alpha != 0;
         if (e == hipSuccess) e = guide_train_rows(g, n, hs);
+        g->train_alpha = false;
         gr->steps++;
         gr->rows += n;
         if (e == hipSuccess && gr->steps == 1) {   // the first step's loss (one wait per render)
@@ -1876,7 +1890,7 @@ tt_status render_impl(tt_scene *s, int32_t nsc, const tt_camera *cams, const int
     // a render that stops training part-way finishes with passes as wide as an
     // untrained one (paths_in_flight then sizes the training passes only)
     int32_t k_after = k;
-    if (gr && gr->train && gr->train_spp > 0 && gr->train_spp < p->spp) {
+    if (gr && gr->train && ((gr->train_spp > 0 && gr->train_spp < p->spp) || gr->growth > 1)) {
         k_after = std::max(1, std::min(p->spp, (1 << 27) / npix));
         while (k_after > 1 && (int64_t)k_after * npix > (1LL << 27)) --k_after;
         k_after = std::max(k_after, k);
@@ -1921,11 +1935,15 @@ tt_status render_impl(tt_scene *s, int32_t nsc, const tt_camera *cams, const int
     const int32_t full = p->spp / k, rest = p->spp % k;
     if (gr) {
         // guided: host-driven passes, training between them (not graph-captured)
-        int32_t done = 0, pass = 0;
+        int32_t done = 0, pass = 0, k_train = k;
         while (e == hipSuccess && done < p->spp) {
             const bool train = gr->train && (gr->train_spp <= 0 || done < gr->train_spp);
-            const int32_t kk = std::min(train ? k : k_after, p->spp - done);
+            const int32_t kk = std::min(train ? k_train : k_after, p->spp - done);
+            if (train && gr->growth > 1) k_train = (int32_t)std::min<int64_t>((int64_t)k_train * gr->growth, k_after);
             gr->gd.rec = train ? gr->g->rec : nullptr;
+            // the recorded paths (every stride-th id, at most ntrain) spread over this
+            // pass's whole width
+            if (train) gr->gd.stride = std::max<int32_t>(1, (int32_t)(((int64_t)npix * kk) / std::max(1, gr->gd.ntrain)));
             gr->gd.toff = gr->gd.stride > 1 ? (int32_t)(((int64_t)pass * 7919) % gr->gd.stride) : 0;
             e = launch_pass(s, rs, p, sv, npix, kk, fp64, hs, gr);
             if (e == hipSuccess && train) {
@@ -2080,7 +2098,7 @@ tt_status tt_guide_render(tt_scene *s, tt_guide *g, const tt_camera *cam, const 
     if (g->ctx != s->ctx) return fail(TT_ERR_INVALID, "tt_guide_render: guide and scene on different contexts");
     if (!(gp->alpha >= 0.0f && gp->alpha <= 1.0f)) return fail(TT_ERR_INVALID, "tt_guide_render: alpha outside [0, 1]");
     if (gp->train_paths < 0 || gp->train_batch < 0 || gp->train_spp < 0 || gp->train_batch > (1 << 22) ||
-        gp->max_bounce < 0)
+        gp->max_bounce < 0 || gp->pass_growth < 0 || gp->pass_growth > 16)
         return fail(TT_ERR_INVALID, "tt_guide_render: bad training sizes");
     tt_render_params q = *p;
     q.guided = 0;
@@ -2093,6 +2111,8 @@ tt_status tt_guide_render(tt_scene *s, tt_guide *g, const tt_camera *cam, const 
     gr.batch_rows = gp->train_batch > 0 ? gp->train_batch : (1 << 18);
     gr.train_spp = gp->train_spp;
     gr.max_bounce = gp->max_bounce;
+    gr.growth = std::max(1, gp->pass_growth);
+    gr.gd.learn_alpha = gp->learn_alpha ? 1 : 0;
     const int32_t root = 0;
     const int64_t n3 = (int64_t)std::max(0, cam->width) * std::max(0, cam->height) * 3;
     const OutSeg seg{rgb_out, rgb64_out, 0, n3};

@@ -50,7 +50,7 @@ static_assert(sizeof(tt_trace_cfg) == 24, "tt_trace_cfg must be 24 bytes");
 static_assert(sizeof(tt_camera) == 60, "tt_camera must be 60 bytes");
 static_assert(sizeof(tt_render_params) == 76, "tt_render_params must be 76 bytes");
 static_assert(sizeof(tt_render_stats) == 40, "tt_render_stats must be 40 bytes");
-static_assert(sizeof(tt_guide_params) == 28, "tt_guide_params must be 28 bytes");
+static_assert(sizeof(tt_guide_params) == 36, "tt_guide_params must be 36 bytes");
 static_assert(sizeof(tt_guide_stats) == 32, "tt_guide_stats must be 32 bytes");
 
 namespace tt {
@@ -113,7 +113,7 @@ static void copy_out(void *dst, const V &v, size_t elem_bytes) {
 extern "C" {
 
 const char *tt_last_error(void) { return tt::g_err.c_str(); }
-int tt_abi_version(void) { return 5; }   // 2: refill fields in tt_trace_cfg / tt_render_params; 3: max_bounce; 4: reorder; 5: tile, first_loss
+int tt_abi_version(void) { return 5; }   // 2: refill fields in tt_trace_cfg / tt_render_params; 3: max_bounce; 4: reorder; 5: tile, first_loss, pass_growth, learn_alpha
 
 tt_status tt_mesh_create(const float *verts, int32_t nverts, const int32_t *tris, int32_t ntris,
                          const float *normals, int32_t nnormals, const int32_t *tri_normals,
