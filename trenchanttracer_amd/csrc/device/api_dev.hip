@@ -21,6 +21,27 @@ using namespace tt;
 
 namespace tt {
 
+// Octahedral encoding of a unit vector in two 16-bit signed-normalised halves
+// (x in the low half): n / (|x| + |y| + |z|), the lower hemisphere folded over the
+// diagonals, each coordinate rounded to the nearest of 32767 steps. The shade stage
+// decodes it (oct_decode in render.hip).
+uint32_t oct_encode(float x, float y, float z) {
+    const float l1 = std::fabs(x) + std::fabs(y) + std::fabs(z);
+    if (!(l1 > 0.0f)) return 0x7FFF0000u;   // degenerate: +y
+    float u = x / l1, v = y / l1;
+    if (z < 0.0f) {
+        const float fu = (1.0f - std::fabs(v)) * (u >= 0.0f ? 1.0f : -1.0f);
+        const float fv = (1.0f - std::fabs(u)) * (v >= 0.0f ? 1.0f : -1.0f);
+        u = fu;
+        v = fv;
+    }
+    auto q = [](float t) -> uint32_t {
+        const float c = std::min(1.0f, std::max(-1.0f, t));
+        return (uint32_t)(uint16_t)(int16_t)std::lround(c * 32767.0f);
+    };
+    return q(u) | (q(v) << 16);
+}
+
 tt_status dev_alloc(tt_ctx *ctx, void **p, size_t bytes, const char *what) {
     (void)ctx;
     *p = nullptr;
@@ -407,6 +428,20 @@ tt_status tt_scene_create(tt_ctx *ctx, const tt_mesh *mesh, const tt_bvh *bvh, c
                 s->bmax[k] = hi[k];
             }
         up((void **)&s->tri_geo, geo.data(), geo.size() * 4, "scene geometric normals");
+        {
+            // the path-mode shading record: one 32-byte gather per hit instead of three
+            std::vector<uint32_t> rec((size_t)m.ntris() * 8, 0u);
+            for (int32_t i = 0; i < m.ntris(); ++i) {
+                uint32_t *r = &rec[(size_t)i * 8];
+                std::memcpy(r, &geo[(size_t)i * 4], 12);
+                r[3] = (uint32_t)m.tri_mat[i];
+                for (int k = 0; k < 3; ++k) {
+                    const V3 n = m.normals[m.tri_n[3 * (size_t)i + k]];
+                    r[4 + k] = oct_encode(n.x, n.y, n.z);
+                }
+            }
+            up((void **)&s->shade_tri, rec.data(), rec.size() * 4, "scene shading records");
+        }
         if (!etri.empty()) {
             for (float &c : ecdf) c = (float)(c / earea);
             ecdf.back() = 1.0f;
@@ -438,7 +473,7 @@ void tt_scene_destroy(tt_scene *s) {
     hipSetDevice(s->ctx->device);
     hipStreamSynchronize(s->ctx->stream);
     free_render_state(s);
-    void *ptrs[] = {s->nodes, s->tris, s->tri_normals, s->tri_mat, s->mats, s->tri_geo, s->emit_tri, s->emit_cdf,
+    void *ptrs[] = {s->nodes, s->tris, s->tri_normals, s->tri_mat, s->mats, s->tri_geo, s->shade_tri, s->emit_tri, s->emit_cdf,
                     s->emit_verts, s->env_rgb, s->env_marg_cdf, s->env_cond_cdf, s->env_pdf_scale, s->geoms,
                     s->env_marg_guide, s->env_cond_guide};
     for (void *p : ptrs)

@@ -278,6 +278,7 @@ struct SceneView {
     const float4 *tri_normals;
     const float4 *tri_geo;
     const int32_t *tri_mat;
+    const uint4 *shade_tri;   // 2 x uint4 per triangle: {ng.xyz, material}, {oct n0, n1, n2, 0}
     const tt_material *mats;
     int32_t num_mats;
     // environment
@@ -412,6 +413,37 @@ __device__ __forceinline__ Surf surface(const SceneView &sv, const tt_hit &h, f3
         s.mat = g.material;
         s.analytic = true;
     }
+    return s;
+}
+
+// Octahedral decoding of oct_encode (api_dev.hip): two 16-bit signed-normalised
+// coordinates, the lower hemisphere unfolded, normalised (angular error below 1e-4).
+__device__ __forceinline__ f3 oct_decode(uint32_t w) {
+    float u = (float)(int16_t)(w & 0xFFFFu) * (1.0f / 32767.0f);
+    float v = (float)(int16_t)(w >> 16) * (1.0f / 32767.0f);
+    const float z = 1.0f - fabsf(u) - fabsf(v);
+    if (z < 0.0f) {
+        const float fu = (1.0f - fabsf(v)) * (u >= 0.0f ? 1.0f : -1.0f);
+        const float fv = (1.0f - fabsf(u)) * (v >= 0.0f ? 1.0f : -1.0f);
+        u = fu;
+        v = fv;
+    }
+    return vnorm(F3(u, v, z));
+}
+
+// The path tracer's surface at a hit: one 32-byte record per triangle (geometric
+// normal, material, octahedral vertex normals) instead of the reference mode's float
+// normals, geometric normals and material ids in three arrays.
+__device__ __forceinline__ Surf surface_path(const SceneView &sv, const tt_hit &h, f3 point) {
+    if (h.tri_id < 0 || !sv.shade_tri) return surface(sv, h, point);
+    const uint4 *r = sv.shade_tri + 2 * (int64_t)h.tri_id;
+    const uint4 a = r[0], b = r[1];
+    Surf s;
+    const float w = 1.0f - h.u - h.v;
+    s.nrm = (oct_decode(b.x) * w + oct_decode(b.y) * h.u) + oct_decode(b.z) * h.v;
+    s.ng = F3(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z));
+    s.mat = (int32_t)a.w;
+    s.analytic = false;
     return s;
 }
 
@@ -921,7 +953,7 @@ __global__ __launch_bounds__(256) void k_guide_inputs(GuideInArgs a) {
             const tt_ray r = a.rays[i];
             const f3 rd = F3(r.dx, r.dy, r.dz), wo = rd * -1.0f;
             const f3 point = F3(r.ox, r.oy, r.oz) + rd * h.t;
-            const Surf sf = surface(a.sv, h, point);
+            const Surf sf = surface_path(a.sv, h, point);
             take = guide_eligible(a.sv.mats[sf.mat], a.gd.min_rough);
             if (take) {
                 const f3 ng = sf.ng;
@@ -971,7 +1003,7 @@ __device__ __forceinline__ void shade_path_one(const ShadeArgs &a, const int32_t
             add = thr * le * w;
         } else {
             const f3 point = ro + rd * h.t;
-            const Surf sf = surface(sv, h, point);
+            const Surf sf = surface_path(sv, h, point);
             const f3 ng = sf.ng;
             const f3 wo = rd * -1.0f;
             const bool front = dot(wo, ng) > 0.0f;
@@ -1734,6 +1766,7 @@ SceneView scene_view(const tt_scene *s, const tt_render_params *p) {
     sv.tri_normals = (const float4 *)s->tri_normals;
     sv.tri_geo = (const float4 *)s->tri_geo;
     sv.tri_mat = s->tri_mat;
+    sv.shade_tri = (const uint4 *)s->shade_tri;
     sv.mats = s->mats;
     sv.num_mats = s->num_mats;
     sv.env = (const float4 *)s->env_rgb;

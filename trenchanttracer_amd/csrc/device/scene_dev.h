@@ -34,6 +34,8 @@ struct tt_scene {
     int32_t *tri_mat = nullptr;
     int32_t num_orig_tris = 0;
     float *tri_geo = nullptr;     // float4 per triangle: unit geometric normal (e1 x e2), area
+    uint32_t *shade_tri = nullptr; // 32 B per triangle for path-mode shading: geometric normal (3 floats),
+                                   // material id, the 3 vertex normals octahedral-encoded (2 x 16 bits), pad
     tt_material *mats = nullptr;
     int32_t num_mats = 0;
     // emissive triangles (area-proportional light sampling)
