@@ -38,6 +38,9 @@ def main():
     ap.add_argument("--cpu-sample-stride", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-render", action="store_true", help="skip the render workloads (secondary metrics)")
+    ap.add_argument("--no-counters", action="store_true",
+                    help="skip the rocprofv3 counter runs (traffic then from the committed counter profile); "
+                         "for running bench.py itself under a profiler")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -164,7 +167,7 @@ def main():
     gather_ms = float(np.median(gather_ms))
     log(f"random 64-byte gather over {info['device_bytes'] / 1e6:.0f} MB: {gather_ms:.3f} ms for 2^26 reads")
 
-    traffic, traffic_src = pmc_call_traffic(args)
+    traffic, traffic_src = (None, "--no-counters") if args.no_counters else pmc_call_traffic(args)
     if traffic is None:
         log(f"live counter runs unavailable ({traffic_src}); using the committed counter profile")
         traffic, traffic_src = committed_traffic(args)
@@ -355,11 +358,11 @@ def committed_traffic(args):
     if (args.variant, args.lds_stack, args.block, args.refill, args.reorder, args.ntris) != (1, 16, 256, 32, 1,
                                                                                           1_000_000):
         return None, "no live counter runs and no committed profile for this configuration"
-    sr = 262_144 / 131_089.0
-    fetch = 6_597_740.5 + 129_627.1 + 16_251.6 + 148_430.5
-    write = 214_108.7 + 64_800.0 + 1_524.0 + 297_800.3
-    return ({"call_bytes": (fetch * sr + write) * 1024, "trace_bytes": (6_597_740.5 * sr + 214_108.7) * 1024,
-             "gather_bytes": None},
+    sr = 262_144 / 131_078.9
+    fetch = 6_598_328.8 + 129_623.2 + 16_249.3 + 148_397.1 + 50.4
+    write = 214_008.2 + 64_800.0 + 1_524.0 + 298_010.9 + 1_521.0
+    return ({"call_bytes": (fetch * sr + write) * 1024, "trace_bytes": (6_598_328.8 * sr + 214_008.2) * 1024,
+             "gather_bytes": 4_158_705.4 * sr * 1024},
             "profiles/k_trace_speculative_soup1M_pmc.md (committed counter runs)")
 
 

@@ -20,11 +20,17 @@ the camera's part of the room through a door, with a 4096-spp oracle.
 
     python -m trenchanttracer_amd.bench_guide [--oracle-spp 8192] [--seeds 4] [--skip-c5] [--quick]
     python -m trenchanttracer_amd.bench_guide --error-classes
+    python -m trenchanttracer_amd.bench_guide --sweep [--seeds 2]
 
 --error-classes: where the path-trace case's error sits. Plain 64-spp renders (seeds
 100, 101) against a 4096-spp FP64 oracle, pixels classed by the material the centre
 ray of the pixel hits first (diffuse, GGX conductor, glass, sky): share of pixels, share
 of the squared error, and the classes of the 400 worst pixels.
+
+--sweep: the path-trace case at 64 and 128 spp under the training schedules of
+SWEEP (pass growth 2 or 4, fixed or learned selection probability; 2^21 training
+paths, one path per pixel per pass): the wall ratios to the plain 64-spp RMSE and the
+64-spp points of each.
 """
 import argparse
 import json
@@ -164,6 +170,30 @@ def error_classes(tt, ctx, log):
     return out
 
 
+SWEEP = (("growth2_fixed_alpha", dict(pass_growth=2)),
+         ("growth2_learned_alpha", dict(pass_growth=2, learn_alpha=True)),
+         ("growth4_learned_alpha", dict(pass_growth=4, learn_alpha=True)))
+
+
+def sweep(tt, ctx, oracle_spp, seeds, log):
+    """The path-trace case under each training schedule of SWEEP, every one compared
+    with the same plain renders and FP64 oracle."""
+    from trenchanttracer_amd import workloads
+    mesh, mats, env, cam = workloads.pathtrace()
+    sc = tt.DeviceScene(ctx, mesh, tt.BVH.build(mesh, nthreads=16), mats, env=env)
+    _, ref, _ = sc.render(cam, tt.render_params(spp=oracle_spp, max_depth=16, rr_depth=4, seed=1234,
+                                                mode=tt.RENDER_PATH, fp64=True), want_fp64=True)
+    out = {}
+    for name, kw in SWEEP:
+        r = compare(tt, ctx, sc, cam, ref, 16, (64, 128), (64, 128), cam.width * cam.height, 1e-3, 262144,
+                    2 << 20, seeds, **kw)
+        out[name] = {k: v for k, v in r.items() if k.endswith("ratio")}
+        out[name].update(online64=r["guided_online"][0], frozen64=r["guided_pretrained"][0], plain64=r["plain"][0])
+        log(f"{name}: {out[name]}")
+    sc.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--oracle-spp", type=int, default=8192)
@@ -171,6 +201,7 @@ def main():
     ap.add_argument("--skip-c5", action="store_true")
     ap.add_argument("--quick", action="store_true", help="door room only, one training run (for profiling)")
     ap.add_argument("--error-classes", action="store_true", help="where the path-trace case's error sits")
+    ap.add_argument("--sweep", action="store_true", help="path-trace case under the training schedules of SWEEP")
     args = ap.parse_args()
     import trenchanttracer_amd as tt
     from trenchanttracer_amd import workloads
@@ -179,6 +210,12 @@ def main():
     if args.error_classes:
         print(json.dumps({"metric": "path-trace case: squared error by pixel class",
                           "pathtrace_1080p": error_classes(tt, ctx, log)}), flush=True)
+        ctx.close()
+        return
+    if args.sweep:
+        print(json.dumps({"metric": "path-trace case: wall ratio to the plain 64-spp RMSE by training schedule",
+                          "oracle_spp": args.oracle_spp, "seeds": args.seeds,
+                          "pathtrace_1080p": sweep(tt, ctx, args.oracle_spp, args.seeds, log)}), flush=True)
         ctx.close()
         return
     result = {"metric": "equal-time RMSE vs FP64 oracle, guided vs plain"}
