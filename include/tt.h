@@ -404,6 +404,18 @@ typedef struct tt_render_stats {
 tt_status tt_render(tt_scene *s, const tt_camera *cam, const tt_render_params *p, float *rgb_out,
                     double *rgb64_out, tt_render_stats *stats);
 
+/* Progressive accumulation and 8-bit display of one rendered frame (replaces the tail of
+ * pathTracingKernel, src/core/RenderKernel.cu:202-217, and the buffer reset of
+ * Renderer::display, src/core/Renderer.cpp:86-93). frame: device, npix x 3 floats (one
+ * frame's per-pixel mean, as tt_render returns it); sum: device, npix x 3 floats, the
+ * running sum, which frame_number 1 restarts from the reference's cudaMemset(buffer, 1)
+ * pattern (every float 0x01010101); rgba: device, npix packed colours, r | g << 8 |
+ * b << 16 | 1 << 24 with channel = (uint8)(pow(clamp(sum / frame_number, 0, 1), 1 / 2.2f)
+ * * 255) (NaN clamps to 1). frame_number >= 1. Enqueued on stream (NULL: the context
+ * stream); allocates nothing. */
+tt_status tt_display_frame(tt_ctx *ctx, const float *frame, float *sum, int64_t npix, int32_t frame_number,
+                           uint32_t *rgba, void *stream);
+
 /* Batched rendering (no reference counterpart; the reference renders one scene):
  * up to 64 scenes in one wavefront (one node / triangle array,
  * per-ray root nodes, per-scene camera and resolution), sharing one env map.

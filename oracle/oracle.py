@@ -76,6 +76,8 @@ def _load(path=LIB_PATH):
     lib.o_acosf.restype = C.c_float
     lib.o_math_batch.argtypes = [C.c_int, P, P, P, C.c_int64]
     lib.o_math_batch.restype = None
+    lib.o_display_frame.argtypes = [P, P, C.c_int64, C.c_int32, P]
+    lib.o_display_frame.restype = None
     lib.o_render_ref.restype = C.c_int
     lib.o_geom_t.argtypes = [C.POINTER(o_geometry), P]
     lib.o_geom_t.restype = C.c_float
@@ -224,6 +226,17 @@ def math(op, x, y=None, lib_=None):
     ya = np.ascontiguousarray(y if y is not None else np.zeros_like(xa), np.float32).ravel()
     out = np.empty_like(xa)
     L.o_math_batch(int(op), _p(xa), _p(ya), _p(out), len(xa))
+    return out
+
+
+def display_frame(frame, sum_, frame_number, lib_=None):
+    """The reference's progressive display step: updates sum_ ((..., 3) float32, in place) by
+    frame and returns the packed 8-bit colours (uint32, r in the low byte)."""
+    L = lib_ or lib
+    f = np.ascontiguousarray(frame, np.float32)
+    assert sum_.dtype == np.float32 and sum_.flags.c_contiguous and sum_.shape == f.shape and f.shape[-1] == 3
+    out = np.empty(f.shape[:-1], np.uint32)
+    L.o_display_frame(_p(f), _p(sum_), f.size // 3, int(frame_number), _p(out))
     return out
 
 

@@ -32,6 +32,11 @@
  *                        latitude of directions, the x2 scale, the linear-index
  *                        wrap and the end clamp (tests/test_oracle_env.py)
  *   o_atan2f/o_acosf     float64 numpy within 2 ulp, C99 special values
+ *   o_display_frame      closed forms: 8-bit value = floor(255 * c^(1/2.2)) in
+ *                        float64 away from integer boundaries, 0 and 255 at the
+ *                        clamp ends, NaN -> 255, alpha 1, the byte-1 starting
+ *                        sum, running sum / frame count (tests/test_oracle_display.py);
+ *                        planted rounding, starting-sum and exponent mistakes fail
  */
 #ifndef TT_ORACLE_H
 #define TT_ORACLE_H
@@ -123,6 +128,10 @@ float o_atan2f(float y, float x);
 float o_acosf(float x);
 /* op over n values: 0 sin x, 1 cos x, 2 pow(x, y), 3 atan2(x, y) (x the ordinate), 4 acos x */
 void o_math_batch(int op, const float *x, const float *y, float *out, int64_t n);
+/* Progressive display (RenderKernel.cu:202-217, Renderer.cpp:86-93): sum (npix x 3) +=
+ * frame (npix x 3), starting from the reference's byte-1 pattern when frame_number == 1;
+ * rgba[i] = the gamma-2.2 8-bit colour of sum / frame_number, alpha 1. */
+void o_display_frame(const float *frame, float *sum, int64_t npix, int32_t frame_number, uint32_t *rgba);
 
 #ifdef __cplusplus
 }

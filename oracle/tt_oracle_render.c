@@ -13,8 +13,9 @@
  *     past the end of its texture there (u2 + v2 * width >= width * height, reached
  *     when acosf(d.y) / PI exceeds 1 with PI = 3.14156265f < pi, or when the wrapped
  *     longitude rounds to TWO_PI in the last row);
- *   - the accumulation buffer starts at 0 (the reference memsets it to bytes of 1,
- *     i.e. 2.4e-38 per channel, Renderer.cpp:28).
+ *   - a render returns the mean of its samples; the reference's running sum over
+ *     frames, which starts at bytes of 1 (2.3694278e-38 per channel,
+ *     Renderer.cpp:27,88), and its 8-bit output are o_display_frame below.
  *
  * Planted mistakes: building with -DO_PLANT=<k> (k > 0) compiles one deliberate
  * error into this file (see the PLANT() sites below). Only the tests build such a
@@ -530,4 +531,34 @@ int o_render_ref(const o_bvh *b, const float *tri_normals, const int32_t *tri_ma
     }
     for (int k = 0; k < nthreads; ++k) pthread_join(th[k], NULL);
     return 0;
+}
+
+/* ---------------------------------------------------------------- display */
+/* The tail of pathTracingKernel (RenderKernel.cu:202-217) and the buffer reset of
+ * Renderer::display (Renderer.cpp:86-93): frame_number 1 starts the running sum
+ * from the cudaMemset(buffer, 1, ...) pattern (every float 0x01010101); then
+ * sum += frame, mean = sum / frame_number, c = fmaxf(0, fminf(mean, 1)) (clamp in
+ * CutilMath.h:44-46, so NaN gives 1), channel = (unsigned char)(powf(c, 1 / 2.2f) *
+ * 255), alpha 1, packed r | g << 8 | b << 16 | a << 24 as the Color union's uchar4. */
+static uint32_t display_channel(float sum, float frames) {
+    float mean = sum / frames;
+    float c = fmaxf(0.0f, fminf(mean, 1.0f));
+    float p = o_powf(c, PLANT(18) ? 2.2f : 1.0f / 2.2f) * 255.0f;
+    return (uint32_t)(uint8_t)(PLANT(16) ? p + 0.5f : p);
+}
+
+void o_display_frame(const float *frame, float *sum, int64_t npix, int32_t frame_number, uint32_t *rgba) {
+    const uint32_t bits = PLANT(17) ? 0u : 0x01010101u;
+    float fresh;
+    memcpy(&fresh, &bits, 4);
+    const float frames = (float)frame_number;
+    for (int64_t i = 0; i < npix; ++i) {
+        uint32_t c = 1u << 24;
+        for (int k = 0; k < 3; ++k) {
+            float v = (frame_number == 1 ? fresh : sum[3 * i + k]) + frame[3 * i + k];
+            sum[3 * i + k] = v;
+            c |= display_channel(v, frames) << (8 * k);
+        }
+        rgba[i] = c;
+    }
 }

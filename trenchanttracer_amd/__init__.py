@@ -476,6 +476,48 @@ class DeviceBuffer:
             pass
 
 
+class Display:
+    """Progressive accumulation and 8-bit display of rendered frames (tt_display_frame):
+    the reference's accumulatedBuffer / frameNumber average and gamma-2.2 colour output
+    (src/core/RenderKernel.cu:202-217, src/core/Renderer.cpp:86-93). frame(rgb) adds one
+    frame (host (h, w, 3) float32 as DeviceScene.render returns it, or a DeviceBuffer of
+    npix x 3 floats) and returns the (h, w, 4) uint8 image; reset() restarts the average
+    (the reference's bufferReset)."""
+
+    def __init__(self, ctx, width, height):
+        self.ctx, self.width, self.height = ctx, int(width), int(height)
+        npix = self.width * self.height
+        self._frame = DeviceBuffer(ctx, npix * 12)
+        self._sum = DeviceBuffer(ctx, npix * 12)
+        self._rgba = DeviceBuffer(ctx, npix * 4)
+        self.frame_number = 0
+
+    def reset(self):
+        self.frame_number = 0
+
+    def frame(self, rgb, stream=None):
+        npix = self.width * self.height
+        if isinstance(rgb, DeviceBuffer):
+            src = rgb
+        else:
+            a = np.ascontiguousarray(rgb, np.float32)
+            if a.size != npix * 3:
+                raise ValueError(f"frame has {a.size} floats, expected {npix * 3}")
+            self._frame.upload(a)
+            src = self._frame
+        self.frame_number += 1
+        check(lib.tt_display_frame(self.ctx._h, C.c_void_p(src.ptr), C.c_void_p(self._sum.ptr), npix,
+                                   self.frame_number, C.c_void_p(self._rgba.ptr), stream))
+        return self._rgba.download((self.height, self.width, 4), np.uint8)
+
+    def running_sum(self):
+        return self._sum.download((self.height, self.width, 3), np.float32)
+
+    def close(self):
+        for b in (self._frame, self._sum, self._rgba):
+            b.free()
+
+
 def gather64_gbps(ctx, buf, nbytes, fetches=1 << 26, seed=1):
     """Random 64-byte gather bandwidth (GB/s) over the first nbytes of a DeviceBuffer."""
     ms = C.c_float()

@@ -138,6 +138,7 @@ PROTOS = {
     "tt_memset_d": (C.c_int, [P, P, C.c_int, C.c_size_t]),
     "tt_bench_gather64": (C.c_int, [P, P, C.c_size_t, I64, U32, C.POINTER(C.c_float)]),
     "tt_dev_copy": (C.c_int, [P, P, P, C.c_size_t]),
+    "tt_display_frame": (C.c_int, [P, P, P, I64, I32, P, P]),
     "tt_timer_create": (C.c_int, [P, I32, C.POINTER(P)]),
     "tt_timer_record": (C.c_int, [P, I32]),
     "tt_timer_elapsed": (C.c_int, [P, I32, I32, C.POINTER(C.c_float)]),

@@ -13,7 +13,7 @@ LIB = os.path.join(HERE, "libtt.so")
 
 HOST_SRC = ["host/sbvh.cpp", "host/flatten.cpp", "host/gen.cpp", "host/api_host.cpp", "host/loaders.cpp"]
 DEV_SRC = ["device/trace.hip", "device/api_dev.hip", "device/env.hip", "device/render.hip", "device/guide.hip",
-           "device/reorder.hip"]
+           "device/reorder.hip", "device/display.hip"]
 
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
