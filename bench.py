@@ -330,12 +330,16 @@ def pmc_call_traffic(args, timeout=240):
         shutil.rmtree(out, ignore_errors=True)
         if not rows:
             return None, f"{counter}: no counter rows (rc {p.returncode}): {text[-300:]!r}"
+        rows.sort(key=lambda r: int(r["Dispatch_Id"]))
         per[counter] = [(r["Kernel_Name"], float(r["Counter_Value"])) for r in rows]
     kib = {}
     for counter, rows in per.items():
         copy = [v for k, v in rows if "k_copy16" in k]
         gather = [v for k, v in rows if "k_gather64" in k]
-        call = [(k, v) for k, v in rows if "k_copy16" not in k and "k_gather64" not in k]
+        # the call: its dispatches up to its traversal launch (the buffer fills after
+        # it belong to the gather probe and the copy)
+        last = max((i for i, (k, _) in enumerate(rows) if "k_trace" in k), default=-1)
+        call = rows[:last + 1]
         if len(copy) != 1 or len(gather) != 1 or not call:
             return None, f"{counter}: unexpected dispatches {[k for k, _ in rows]}"
         kib[counter] = (copy[0], call, gather[0])
@@ -359,8 +363,8 @@ def committed_traffic(args):
                                                                                           1_000_000):
         return None, "no live counter runs and no committed profile for this configuration"
     sr = 262_144 / 131_078.9
-    fetch = 6_598_328.8 + 129_623.2 + 16_249.3 + 148_397.1 + 50.4
-    write = 214_008.2 + 64_800.0 + 1_524.0 + 298_010.9 + 1_521.0
+    fetch = 6_598_328.8 + 129_623.2 + 16_249.3 + 148_397.1 + 51.8
+    write = 214_008.2 + 64_800.0 + 1_524.0 + 298_010.9 + 1_524.1
     return ({"call_bytes": (fetch * sr + write) * 1024, "trace_bytes": (6_598_328.8 * sr + 214_008.2) * 1024,
              "gather_bytes": 4_158_705.4 * sr * 1024},
             "profiles/k_trace_speculative_soup1M_pmc.md (committed counter runs)")
