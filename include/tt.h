@@ -565,6 +565,12 @@ tt_status tt_debug_env_sample(tt_scene *s, int64_t n, uint32_t seed, float *out)
  * of the marginal and conditional CDFs (tt_env_export_cdf) through their guide
  * tables: the largest index k with cdf[k] < u. Host buffers; synchronous. */
 tt_status tt_debug_env_invert(tt_scene *s, const float *u, int64_t n, int32_t *out);
+/* The order a reordering trace call (tt_trace_cfg.reorder = 1) traces the rays in:
+ * order[r] = index of the r-th ray, the stable sort of the ray indices by the 24-bit
+ * Morton code of the origins over the scene's bounds (8 bits per axis, x in bits
+ * 3k + 2, y in 3k + 1, z in 3k; cells clamped to 0..255). rays: device, n records;
+ * order: host, n int32. Synchronous. */
+tt_status tt_debug_ray_order(tt_scene *s, const tt_ray *rays, int64_t n, int32_t *order);
 
 #ifdef __cplusplus
 } /* extern "C" */

@@ -679,6 +679,13 @@ class DeviceScene:
         check(lib.tt_debug_env_invert(self._h, _vp(u), len(u), _vp(out)))
         return out
 
+    def ray_order(self, d_rays, n):
+        """The order a reordering trace call traces the rays of a DeviceBuffer in
+        (tt_debug_ray_order): int32 array, order[r] = index of the r-th ray."""
+        out = np.empty(int(n), np.int32)
+        check(lib.tt_debug_ray_order(self._h, C.c_void_p(d_rays.ptr), int(n), _vp(out)))
+        return out
+
     def env_samples(self, n, seed=1):
         out = np.empty((int(n), 8), np.float32)
         check(lib.tt_debug_env_sample(self._h, int(n), int(seed), _vp(out)))

@@ -475,7 +475,7 @@ void tt_scene_destroy(tt_scene *s) {
     free_render_state(s);
     void *ptrs[] = {s->nodes, s->tris, s->tri_normals, s->tri_mat, s->mats, s->tri_geo, s->shade_tri, s->emit_tri, s->emit_cdf,
                     s->emit_verts, s->env_rgb, s->env_marg_cdf, s->env_cond_cdf, s->env_pdf_scale, s->geoms,
-                    s->env_marg_guide, s->env_cond_guide};
+                    s->env_marg_cells, s->env_cells};
     for (void *p : ptrs)
         if (p) hipFree(p);
     delete s;
@@ -569,6 +569,25 @@ static tt_status trace_impl(tt_scene *s, const tt_ray *rays, int64_t n, tt_hit *
         TT_HIP(hipEventRecord(s->ctx->reorder_done, hs), "hipEventRecord");
         s->ctx->reorder_pending = true;
     }
+    return TT_OK;
+}
+
+tt_status tt_debug_ray_order(tt_scene *s, const tt_ray *rays, int64_t n, int32_t *order) {
+    if (!s || !rays || !order || n <= 0 || n >= ((int64_t)1 << 31))
+        return fail(TT_ERR_INVALID, "tt_debug_ray_order: bad arguments");
+    tt_ctx *ctx = s->ctx;
+    TT_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    size_t need = 0, sort_bytes = 0;
+    TT_HIP(reorder_workspace_bytes(n, &need, &sort_bytes), "tt_debug_ray_order: workspace size");
+    void *ws = nullptr;
+    tt_status st = dev_alloc(ctx, &ws, need, "tt_debug_ray_order workspace");
+    if (st != TT_OK) return st;
+    const uint32_t *d_order = nullptr;
+    hipError_t e = reorder_rays(rays, n, s->bmin, s->bmax, ws, sort_bytes, ctx->stream, &d_order);
+    if (e == hipSuccess) e = hipMemcpyAsync(order, d_order, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    hipFree(ws);
+    if (e != hipSuccess) return fail(TT_ERR_DEVICE, std::string("tt_debug_ray_order: ") + hipGetErrorString(e));
     return TT_OK;
 }
 

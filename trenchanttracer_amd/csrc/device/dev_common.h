@@ -72,6 +72,26 @@ __device__ __forceinline__ int cdf_search_guided(const float *cdf, const int2 *g
     return cdf_search_range(cdf, r.x, r.y, u);
 }
 
+// One cell of a row's conditional guide table together with everything a sample drawn
+// in it reads: the index range [lo, hi] the CDF search can return for a u of the cell,
+// the CDF entries lo + 1 and lo + 2, and the texels (rgb, sampling weight) of columns
+// lo, lo + 1 and lo + 2 (zero past the row's end). 64 bytes, so a next-event sample of
+// the environment reads one record; ranges of more than three entries fall back to the
+// CDF and texel arrays.
+struct alignas(64) EnvCell {
+    int32_t lo, hi;
+    float c1, c2;
+    float4 t[3];
+};
+static_assert(sizeof(EnvCell) == 64, "EnvCell is one 64-byte record");
+
+// One cell of the marginal CDF's guide table: its search range and the CDF entries
+// lo + 1, lo + 2 (16 bytes; ranges of more than three entries read the CDF).
+struct alignas(16) MargCell {
+    int32_t lo, hi;
+    float c1, c2;
+};
+
 // Launchers (trace.hip).
 hipError_t launch_trace(const TraceArgs &a, int variant, bool any, int lds_stack, int block, int xcd_order,
                         int refill, int num_cus, hipStream_t s, int tri_mode = kTriReference);

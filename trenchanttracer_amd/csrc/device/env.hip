@@ -113,19 +113,55 @@ __global__ __launch_bounds__(256) void k_env_guide(const float *__restrict__ cdf
     }
 }
 
+// Cell records of the rows' conditional guide tables: each cell's search range, the CDF
+// entries and texels a sample drawn in the cell can need (EnvCell).
+__global__ __launch_bounds__(256) void k_env_cells(const int2 *__restrict__ guide, const float *__restrict__ cond,
+                                                   const float4 *__restrict__ texel, int w, int h,
+                                                   EnvCell *__restrict__ cells) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= w) return;
+    for (int r = blockIdx.y; r < h; r += gridDim.y) {
+        const int2 g = guide[(int64_t)r * w + c];
+        const float *cd = cond + (int64_t)r * (w + 1);
+        const float4 *tr = texel + (int64_t)r * w;
+        EnvCell e;
+        e.lo = g.x;
+        e.hi = g.y;
+        e.c1 = g.x + 1 <= w ? cd[g.x + 1] : 0.0f;
+        e.c2 = g.x + 2 <= w ? cd[g.x + 2] : 0.0f;
+        for (int k = 0; k < 3; ++k) e.t[k] = g.x + k < w ? tr[g.x + k] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        cells[(int64_t)r * w + c] = e;
+    }
+}
+
+// Records of the marginal CDF's guide cells (MargCell).
+__global__ __launch_bounds__(256) void k_env_marg_cells(const int2 *__restrict__ guide, const float *__restrict__ marg,
+                                                        int h, MargCell *__restrict__ cells) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= h) return;
+    const int2 g = guide[c];
+    MargCell e;
+    e.lo = g.x;
+    e.hi = g.y;
+    e.c1 = g.x + 1 <= h ? marg[g.x + 1] : 0.0f;
+    e.c2 = g.x + 2 <= h ? marg[g.x + 2] : 0.0f;
+    cells[c] = e;
+}
+
 }  // namespace
 
 tt_status build_env(tt_scene *s, const float *rgb, int32_t w, int32_t h, float scale) {
     tt_ctx *ctx = s->ctx;
-    void *d_rgb = nullptr, *d_rows = nullptr;
+    void *d_rgb = nullptr, *d_rows = nullptr, *d_guide = nullptr;
     tt_status st = dev_alloc(ctx, &d_rgb, (size_t)w * h * 3 * sizeof(float), "env staging");
     if (st == TT_OK) st = dev_alloc(ctx, &d_rows, (size_t)h * sizeof(double), "env row sums");
     if (st == TT_OK) st = dev_alloc(ctx, (void **)&s->env_rgb, (size_t)w * h * sizeof(float4), "env texels");
     if (st == TT_OK) st = dev_alloc(ctx, (void **)&s->env_cond_cdf, (size_t)h * (w + 1) * sizeof(float), "env conditional cdf");
     if (st == TT_OK) st = dev_alloc(ctx, (void **)&s->env_marg_cdf, (size_t)(h + 1) * sizeof(float), "env marginal cdf");
     if (st == TT_OK) st = dev_alloc(ctx, (void **)&s->env_pdf_scale, sizeof(double), "env total");
-    if (st == TT_OK) st = dev_alloc(ctx, (void **)&s->env_cond_guide, (size_t)h * w * sizeof(int2), "env conditional guide");
-    if (st == TT_OK) st = dev_alloc(ctx, (void **)&s->env_marg_guide, (size_t)h * sizeof(int2), "env marginal guide");
+    if (st == TT_OK) st = dev_alloc(ctx, &d_guide, (size_t)h * w * sizeof(int2), "env conditional guide");
+    if (st == TT_OK) st = dev_alloc(ctx, (void **)&s->env_cells, (size_t)h * w * sizeof(EnvCell), "env cell records");
+    if (st == TT_OK) st = dev_alloc(ctx, (void **)&s->env_marg_cells, (size_t)h * sizeof(MargCell), "env marginal cells");
     hipError_t e = hipSuccess;
     if (st == TT_OK) e = hipMemcpy(d_rgb, rgb, (size_t)w * h * 3 * sizeof(float), hipMemcpyHostToDevice);
     if (st == TT_OK && e == hipSuccess) {
@@ -140,12 +176,23 @@ tt_status build_env(tt_scene *s, const float *rgb, int32_t w, int32_t h, float s
     }
     if (st == TT_OK && e == hipSuccess) {
         hipLaunchKernelGGL(k_env_guide, dim3((unsigned)((w + 255) / 256), (unsigned)std::min(h, 65535)), dim3(256), 0,
-                           ctx->stream, (const float *)s->env_cond_cdf, (int64_t)(w + 1), w, h, s->env_cond_guide);
+                           ctx->stream, (const float *)s->env_cond_cdf, (int64_t)(w + 1), w, h, (int2 *)d_guide);
+        e = hipGetLastError();
+    }
+    if (st == TT_OK && e == hipSuccess) {
+        hipLaunchKernelGGL(k_env_cells, dim3((unsigned)((w + 255) / 256), (unsigned)std::min(h, 65535)), dim3(256), 0,
+                           ctx->stream, (const int2 *)d_guide, (const float *)s->env_cond_cdf,
+                           (const float4 *)s->env_rgb, w, h, s->env_cells);
         e = hipGetLastError();
     }
     if (st == TT_OK && e == hipSuccess) {
         hipLaunchKernelGGL(k_env_guide, dim3((unsigned)((h + 255) / 256), 1), dim3(256), 0, ctx->stream,
-                           (const float *)s->env_marg_cdf, (int64_t)(h + 1), h, 1, s->env_marg_guide);
+                           (const float *)s->env_marg_cdf, (int64_t)(h + 1), h, 1, (int2 *)d_guide);
+        e = hipGetLastError();
+    }
+    if (st == TT_OK && e == hipSuccess) {
+        hipLaunchKernelGGL(k_env_marg_cells, dim3((unsigned)((h + 255) / 256)), dim3(256), 0, ctx->stream,
+                           (const int2 *)d_guide, (const float *)s->env_marg_cdf, h, s->env_marg_cells);
         e = hipGetLastError();
     }
     double total = 0.0;
@@ -154,6 +201,7 @@ tt_status build_env(tt_scene *s, const float *rgb, int32_t w, int32_t h, float s
     if (st == TT_OK && e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (d_rgb) hipFree(d_rgb);
     if (d_rows) hipFree(d_rows);
+    if (d_guide) hipFree(d_guide);
     if (st != TT_OK) return st;
     if (e != hipSuccess) return fail(TT_ERR_DEVICE, std::string("env build: ") + hipGetErrorString(e));
     s->env_w = w;
@@ -161,7 +209,7 @@ tt_status build_env(tt_scene *s, const float *rgb, int32_t w, int32_t h, float s
     s->env_scale = scale;
     s->env_total = total;
     s->bytes += (size_t)w * h * sizeof(float4) + (size_t)h * (w + 1) * 4 + (size_t)(h + 1) * 4 + 8 +
-                (size_t)(h * (int64_t)w + h) * sizeof(int2);
+                (size_t)h * sizeof(MargCell) + (size_t)h * w * sizeof(EnvCell);
     return TT_OK;
 }
 

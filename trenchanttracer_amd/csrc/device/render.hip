@@ -284,7 +284,8 @@ struct SceneView {
     // environment
     const float4 *env;
     const float *env_marg, *env_cond;
-    const int2 *env_gmarg, *env_gcond;   // guide tables of the two CDFs
+    const MargCell *env_gmarg;           // guide cells of the marginal CDF
+    const EnvCell *env_cells;            // the conditional CDFs' guide cells with their texels
     int32_t env_w, env_h;
     float env_scale, env_inv_total;
     // emitters
@@ -845,34 +846,63 @@ __device__ bool bsdf_sample(const tt_material &m, f3 wo, bool front, const float
     return false;
 }
 
-// Environment lookup (nearest texel) and the density of sampling direction d.
-__device__ f3 env_eval(const SceneView &sv, f3 d, float *pdf) {
-    *pdf = 0.0f;
-    if (!sv.env) return F3(0, 0, 0);
+// Texel (row j, column i) direction d looks up, nearest texel.
+__device__ __forceinline__ void env_dir_texel(const SceneView &sv, f3 d, int *j, int *i) {
     float phi = atan2f(d.x, d.z);
     if (phi < 0.0f) phi += 6.28318531f;
     float u = phi * 0.159154943f;
     float v = acosf(fminf(1.0f, fmaxf(-1.0f, d.y))) * 0.318309886f;
-    int i = min(max((int)(u * sv.env_w), 0), sv.env_w - 1);
-    int j = min(max((int)(v * sv.env_h), 0), sv.env_h - 1);
-    float4 t = sv.env[(int64_t)j * sv.env_w + i];
+    *i = min(max((int)(u * sv.env_w), 0), sv.env_w - 1);
+    *j = min(max((int)(v * sv.env_h), 0), sv.env_h - 1);
+}
+
+// Radiance of texel t and the solid-angle density of sampling direction d through it.
+__device__ __forceinline__ f3 env_texel_value(const SceneView &sv, f3 d, float4 t, float *pdf) {
     float sin_t = sqrtf(d.x * d.x + d.z * d.z);  // no cancellation near the poles
+    *pdf = 0.0f;
     if (sin_t > 0.0f && sv.env_inv_total > 0.0f)
         *pdf = t.w * sv.env_inv_total * (float)sv.env_w * (float)sv.env_h / (19.7392088f * sin_t);
     return F3(t.x, t.y, t.z) * sv.env_scale;
+}
+
+// Environment lookup (nearest texel) and the density of sampling direction d.
+__device__ f3 env_eval(const SceneView &sv, f3 d, float *pdf) {
+    *pdf = 0.0f;
+    if (!sv.env) return F3(0, 0, 0);
+    int j, i;
+    env_dir_texel(sv, d, &j, &i);
+    return env_texel_value(sv, d, sv.env[(int64_t)j * sv.env_w + i], pdf);
 }
 
 // Largest index k in [0, n) with cdf[k] < u (cdf[0] = 0 < u).
 __device__ __forceinline__ int cdf_search(const float *cdf, int n, float u) { return cdf_search_range(cdf, 0, n - 1, u); }
 
 // Texel (row j, column i) the env distribution gives for (u0, u1): inversion of the
-// marginal CDF, then of row j's conditional CDF, each through its guide table (a
-// few dependent loads where the two binary searches take 21 at 2048 x 1024; the
-// same texel for every u).
-__device__ __forceinline__ void env_texel(const SceneView &sv, float u0, float u1, int *j, int *i) {
-    *j = cdf_search_guided(sv.env_marg, sv.env_gmarg, sv.env_h, u0);
-    *i = cdf_search_guided(sv.env_cond + (int64_t)*j * (sv.env_w + 1), sv.env_gcond + (int64_t)*j * sv.env_w,
-                           sv.env_w, u1);
+// marginal CDF through its guide table (small: it stays in cache), then of row j's
+// conditional CDF through the cell record of u1, which holds the search range and,
+// when the range spans at most three entries, the CDF entries and texels the search
+// can need: one 64-byte read where the binary searches take 21 dependent loads at
+// 2048 x 1024. The same texel as the full search for every u. Returns true with the
+// texel's value in *t when it came with the record.
+__device__ __forceinline__ bool env_texel(const SceneView &sv, float u0, float u1, int *j, int *i, float4 *t) {
+    // cdf_search_range over [lo, hi] of a nondecreasing CDF, when the range spans at
+    // most three entries: one step past lo per entry below u
+    const float4 m = *(const float4 *)(sv.env_gmarg + guide_cell(u0, sv.env_h));
+    const int mlo = __float_as_int(m.x), mhi = __float_as_int(m.y);
+    *j = mhi - mlo <= 2 ? mlo + (mhi > mlo && m.z < u0 ? 1 : 0) + (mhi > mlo + 1 && m.w < u0 ? 1 : 0)
+                        : cdf_search_range(sv.env_marg, mlo, mhi, u0);
+    // the whole cell record at once (four independent 16-byte loads)
+    const float4 *rec = (const float4 *)(sv.env_cells + ((int64_t)*j * sv.env_w + guide_cell(u1, sv.env_w)));
+    const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2], r3 = rec[3];
+    const int lo = __float_as_int(r0.x), hi = __float_as_int(r0.y);
+    if (hi - lo <= 2) {
+        const int k = (hi > lo && r0.z < u1 ? 1 : 0) + (hi > lo + 1 && r0.w < u1 ? 1 : 0);
+        *i = lo + k;
+        *t = k == 0 ? r1 : k == 1 ? r2 : r3;
+        return true;
+    }
+    *i = cdf_search_range(sv.env_cond + (int64_t)*j * (sv.env_w + 1), lo, hi, u1);
+    return false;
 }
 
 __device__ f3 env_sample(const SceneView &sv, const float u[4], f3 *dir, float *pdf) {
@@ -881,13 +911,19 @@ __device__ f3 env_sample(const SceneView &sv, const float u[4], f3 *dir, float *
     // sampling and env_eval agree even where rounding moves a direction across a
     // texel edge
     int j, i;
-    env_texel(sv, u[0], u[1], &j, &i);
+    float4 t;
+    if (!env_texel(sv, u[0], u[1], &j, &i, &t)) t = sv.env[(int64_t)j * sv.env_w + i];
     float theta = 3.14159265f * ((float)j + (1.0f - u[2])) / (float)sv.env_h;
     float phi = 6.28318531f * ((float)i + (1.0f - u[3])) / (float)sv.env_w;
     float ct, st = tm_sincos(theta, &ct);
     float cp, sp = tm_sincos(phi, &cp);
     *dir = F3(st * sp, ct, st * cp);
-    return env_eval(sv, *dir, pdf);
+    // the texel the direction looks up: the sampled one, unless rounding carried the
+    // direction across a texel edge
+    int jd, id;
+    env_dir_texel(sv, *dir, &jd, &id);
+    if (jd != j || id != i) t = sv.env[(int64_t)jd * sv.env_w + id];
+    return env_texel_value(sv, *dir, t, pdf);
 }
 
 __device__ __forceinline__ int train_slot(const GuideShade &gd, int32_t p) {
@@ -1457,7 +1493,8 @@ __global__ void k_debug_env_invert(SceneView sv, int64_t n, const float *u, int3
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     int j, i;
-    env_texel(sv, u[2 * k], u[2 * k + 1], &j, &i);
+    float4 t;
+    env_texel(sv, u[2 * k], u[2 * k + 1], &j, &i, &t);
     out[2 * k] = j;
     out[2 * k + 1] = i;
 }
@@ -1772,8 +1809,8 @@ SceneView scene_view(const tt_scene *s, const tt_render_params *p) {
     sv.env = (const float4 *)s->env_rgb;
     sv.env_marg = s->env_marg_cdf;
     sv.env_cond = s->env_cond_cdf;
-    sv.env_gmarg = s->env_marg_guide;
-    sv.env_gcond = s->env_cond_guide;
+    sv.env_gmarg = s->env_marg_cells;
+    sv.env_cells = s->env_cells;
     sv.env_w = s->env_w;
     sv.env_h = s->env_h;
     sv.env_scale = s->env_scale;
@@ -2283,8 +2320,8 @@ tt_status tt_debug_env_sample(tt_scene *s, int64_t n, uint32_t seed, float *out)
     sv.env = (const float4 *)s->env_rgb;
     sv.env_marg = s->env_marg_cdf;
     sv.env_cond = s->env_cond_cdf;
-    sv.env_gmarg = s->env_marg_guide;
-    sv.env_gcond = s->env_cond_guide;
+    sv.env_gmarg = s->env_marg_cells;
+    sv.env_cells = s->env_cells;
     sv.env_w = s->env_w;
     sv.env_h = s->env_h;
     sv.env_scale = s->env_scale;
@@ -2311,8 +2348,8 @@ tt_status tt_debug_env_invert(tt_scene *s, const float *u, int64_t n, int32_t *o
     SceneView sv{};
     sv.env_marg = s->env_marg_cdf;
     sv.env_cond = s->env_cond_cdf;
-    sv.env_gmarg = s->env_marg_guide;
-    sv.env_gcond = s->env_cond_guide;
+    sv.env_gmarg = s->env_marg_cells;
+    sv.env_cells = s->env_cells;
     sv.env_w = s->env_w;
     sv.env_h = s->env_h;
     void *d = nullptr;

@@ -9,6 +9,11 @@
 
 #include "../../../include/tt.h"
 
+namespace tt {
+struct EnvCell;
+struct MargCell;
+}
+
 struct tt_ctx {
     int device = 0;
     int num_cus = 256;
@@ -48,8 +53,8 @@ struct tt_scene {
     float *env_rgb = nullptr;        // float4 per texel (rgb, luminance weight)
     float *env_marg_cdf = nullptr;   // H + 1 floats
     float *env_cond_cdf = nullptr;   // H * (W + 1) floats
-    int2 *env_marg_guide = nullptr;  // H cells: index range of the marginal CDF search
-    int2 *env_cond_guide = nullptr;  // H * W cells: the same for each row's conditional CDF
+    tt::MargCell *env_marg_cells = nullptr;  // H cells of the marginal CDF's guide table
+    tt::EnvCell *env_cells = nullptr;    // H * W cells of the conditional CDFs' guide tables, with their texels
     double *env_pdf_scale = nullptr; // [0] = total weight (FP64)
     int32_t env_w = 0, env_h = 0;
     float env_scale = 1.0f;
