@@ -270,6 +270,7 @@ def main():
             "value": round(cpu_mrays, 3), "unit": "Mrays/s", "cores": cores, "kind": "port",
             "sample": f"every {stride}th ray of the same set ({len(sample)} rays), oracle single-lane traversal, "
                       f"{st['seconds']:.2f}s wall on {cores} threads",
+            "host": host_cpu(),
         }
         result["parity_sample"] = {"rays": int(len(sample)), "hit_id_mismatches": mism, "bit_exact": exact,
                                    "oracle_algorithmic_bytes_per_ray": round(bpr_o, 1)}
@@ -353,6 +354,24 @@ def pmc_call_traffic(args, timeout=240):
            f"bench run ({len(kib['FETCH_SIZE'][1])} dispatches per call), scaled by a 256 MiB copy: "
            f"reads x {sr:.5f}, writes x {sw:.5f}")
     return {"call_bytes": call * 1024, "trace_bytes": trace * 1024, "gather_bytes": gather * 1024}, src
+
+
+def host_cpu():
+    """The host CPU model (from /proc/cpuinfo) and the logical CPUs the process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    return f"{model}, {usable} logical CPUs available to the process"
 
 
 def committed_traffic(args):

@@ -50,6 +50,25 @@ def test_fused_inference_matches_unfused_forward(tt, ctx, n):
     g.close()
 
 
+def test_gradient_is_bit_reproducible(tt, ctx):
+    """The loss and every gradient element are the same bits on repeated evaluation of
+    the same rows (the weight sum that scales the gradient and the loss are reduced in a
+    fixed order, without float atomics), at a row count spanning many reduction blocks."""
+    rng = np.random.default_rng(23)
+    n = 200_003
+    x = rng.random((n, 9), dtype=np.float32)
+    d = rng.normal(size=(n, 3))
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    w = rng.gamma(1.0, 1.0, n).astype(np.float32)
+    g = tt.Guide(ctx, seed=5, lr=1e-3)
+    l0, g0 = g.grad(x, d, w)
+    for _ in range(3):
+        l1, g1 = g.grad(x, d, w)
+        assert np.float32(l1).view(np.uint32) == np.float32(l0).view(np.uint32)
+        assert np.array_equal(g1.view(np.uint32), g0.view(np.uint32))
+    g.close()
+
+
 def test_gradient_matches_oracle(tt, ctx):
     from oracle import guide_ref as G
     g = tt.Guide(ctx, seed=5)

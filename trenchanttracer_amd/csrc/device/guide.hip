@@ -289,7 +289,7 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(InferArgs a) {
 // than the BSDF density does (Mueller et al. 2019, "Neural Importance Sampling").
 __global__ __launch_bounds__(256) void k_loss_grad(const float *__restrict__ raw, const float *__restrict__ dirs,
                                                    const float *__restrict__ wts, const float *__restrict__ pb,
-                                                   __bf16 *__restrict__ draw, float *__restrict__ loss_out,
+                                                   __bf16 *__restrict__ draw, float *__restrict__ loss_part,
                                                    const float *__restrict__ wsum, int64_t n) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     __shared__ float red[256];
@@ -349,10 +349,11 @@ __global__ __launch_bounds__(256) void k_loss_grad(const float *__restrict__ raw
         if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
         __syncthreads();
     }
-    if (threadIdx.x == 0) atomicAdd(loss_out, red[0]);
+    if (threadIdx.x == 0) loss_part[blockIdx.x] = red[0];
 }
 
-__global__ __launch_bounds__(256) void k_sum(const float *__restrict__ v, int64_t n, float *__restrict__ out) {
+// Per-block sums of v (grid-stride), one partial per block; k_sum_parts adds them.
+__global__ __launch_bounds__(256) void k_sum(const float *__restrict__ v, int64_t n, float *__restrict__ part) {
     __shared__ float red[256];
     float s = 0.0f;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -363,7 +364,22 @@ __global__ __launch_bounds__(256) void k_sum(const float *__restrict__ v, int64_
         if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
         __syncthreads();
     }
-    if (threadIdx.x == 0) atomicAdd(out, red[0]);
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// *out = the sum of n partials in a fixed order (one block): the loss and the weight
+// sum, which scales every gradient, are the same bits on every run.
+__global__ __launch_bounds__(256) void k_sum_parts(const float *__restrict__ part, int64_t n, float *__restrict__ out) {
+    __shared__ float red[256];
+    float s = 0.0f;
+    for (int64_t i = threadIdx.x; i < n; i += 256) s += part[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
 }
 
 // ------------------------------------------------------------------ weight gradients
@@ -550,7 +566,8 @@ static int64_t layer_offset(int k) {
 tt_status guide_alloc_batch(tt_guide *g, int64_t rows) {
     if (g->cap >= rows) return TT_OK;
     GuideBatch &b = g->batch;
-    void *ptrs[] = {b.in, b.x, b.h[0], b.h[1], b.h[2], b.raw, b.draw, b.dh[0], b.dh[1], b.dh[2], b.dirs, b.wts, b.pb};
+    void *ptrs[] = {b.in,   b.x,     b.h[0],  b.h[1],  b.h[2], b.raw, b.draw,
+                    b.dh[0], b.dh[1], b.dh[2], b.dirs, b.wts,  b.pb,  b.lpart};
     for (void *q : ptrs)
         if (q) hipFree(q);
     b = GuideBatch();
@@ -569,6 +586,7 @@ tt_status guide_alloc_batch(tt_guide *g, int64_t rows) {
     al((void **)&b.dirs, (size_t)R * 3 * 4, "guide directions");
     al((void **)&b.wts, (size_t)R * 4, "guide weights");
     al((void **)&b.pb, (size_t)R * 4, "guide BSDF densities");
+    al((void **)&b.lpart, (size_t)((R + 255) / 256) * 4, "guide loss partials");
     if (st == TT_OK) g->cap = R;
     return st;
 }
@@ -660,12 +678,15 @@ static hipError_t guide_backward(tt_guide *g, int64_t n, hipStream_t s) {
     const int64_t R = (n + 15) & ~15LL;
     hipError_t e = guide_forward(g, n, nullptr, s);
     if (e != hipSuccess) return e;
-    e = hipMemsetAsync(g->scalars, 0, 16, s);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_sum, dim3(256), dim3(256), 0, s, b.wts, n, g->scalars + 1);
-    hipLaunchKernelGGL(k_loss_grad, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, s, b.raw, b.dirs, b.wts,
-                       g->train_alpha ? (const float *)b.pb : (const float *)nullptr, b.draw, g->scalars,
+    // weight sum and loss: per-block partials added in a fixed order (no float atomics)
+    hipLaunchKernelGGL(k_sum, dim3(kGuideSumParts), dim3(256), 0, s, b.wts, n, g->scalars + kGuideSumParts);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, s, (const float *)(g->scalars + kGuideSumParts),
+                       (int64_t)kGuideSumParts, g->scalars + 1);
+    const unsigned lb = (unsigned)((R + 255) / 256);
+    hipLaunchKernelGGL(k_loss_grad, dim3(lb), dim3(256), 0, s, b.raw, b.dirs, b.wts,
+                       g->train_alpha ? (const float *)b.pb : (const float *)nullptr, b.draw, b.lpart,
                        (const float *)(g->scalars + 1), n);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, s, (const float *)b.lpart, (int64_t)lb, g->scalars);
     const unsigned gw = (unsigned)std::min<int64_t>(((R / 16) + 3) / 4 + 1, 8192);
     // dH3 = dRaw W4 (mask H3 > 0), dH2 = dH3 W3 (mask), dH1 = dH2 W2 (mask)
     hipLaunchKernelGGL((k_gemm<kGuideNOutPad, kGuideH, false, true>), dim3(gw), dim3(256), 0, s, b.draw, g->wb[3],
@@ -772,7 +793,7 @@ tt_status tt_guide_create(tt_ctx *ctx, uint32_t seed, float lr, tt_guide **out) 
     al((void **)&g->m1, (size_t)np * 4, "guide adam m");
     al((void **)&g->m2, (size_t)np * 4, "guide adam v");
     al((void **)&g->grad, (size_t)np * 4, "guide grad");
-    al((void **)&g->scalars, 16, "guide scalars");
+    al((void **)&g->scalars, (size_t)2 * kGuideSumParts * 4, "guide scalars");
     for (int k = 0; k < 4; ++k) {
         const GuideLayer &l = kGuideLayers[k];
         al((void **)&g->wf[k], (size_t)l.in * l.out * 2, "guide bf16 weights");
@@ -826,7 +847,7 @@ void tt_guide_destroy(tt_guide *g) {
     hipStreamSynchronize(g->ctx->stream);
     GuideBatch &b = g->batch;
     void *ptrs[] = {g->params, g->m1, g->m2, g->grad, g->scalars, g->partial, b.in, b.x, b.h[0], b.h[1], b.h[2],
-                    b.raw, b.draw, b.dh[0], b.dh[1], b.dh[2], b.dirs, b.wts, b.pb, g->wf[0], g->wf[1], g->wf[2], g->wf[3],
+                    b.raw, b.draw, b.dh[0], b.dh[1], b.dh[2], b.dirs, b.wts, b.pb, b.lpart, g->wf[0], g->wf[1], g->wf[2], g->wf[3],
                     g->wb[0], g->wb[1], g->wb[2], g->wb[3], g->train_buf, g->train_count, g->rec, g->iin, g->iraw,
                     g->irow};
     for (void *q : ptrs)

@@ -21,6 +21,7 @@ constexpr int kGuideAlphaOut = kGuideNOut;
 constexpr float kAlphaLo = 0.02f, kAlphaHi = 0.95f;
 constexpr int kGuideNOutPad16 = 48;  // output columns computed (multiple of 16); raw row stride
 constexpr int kGuideNOutPad = 64;    // output-gradient width (multiple of 32, the backward K)
+constexpr int kGuideSumParts = 256;  // blocks of the weight-sum reduction
 constexpr float kGuideLogKappaMin = -5.0f, kGuideLogKappaMax = 9.0f;
 
 struct GuideLayer {
@@ -39,6 +40,7 @@ struct GuideBatch {
     float *dirs = nullptr;    // [R][3] training directions
     float *wts = nullptr;     // [R] training weights (radiance / pdf)
     float *pb = nullptr;      // [R] BSDF density of each direction (learned alpha), or unset
+    float *lpart = nullptr;   // [(R + 255) / 256] per-block loss sums, added in a fixed order
 };
 
 __device__ __forceinline__ float sigmoidf_(float z) { return 1.0f / (1.0f + __expf(-z)); }
@@ -187,7 +189,7 @@ tt_status guide_alloc_infer(tt_guide *g, int64_t rows);
 struct tt_guide {
     tt_ctx *ctx = nullptr;
     float *params = nullptr, *m1 = nullptr, *m2 = nullptr, *grad = nullptr;
-    float *scalars = nullptr;  // [0] loss, [1] sum of weights
+    float *scalars = nullptr;  // [0] loss, [1] sum of weights, [kGuideSumParts ..] per-block weight sums
     __bf16 *wf[4] = {nullptr, nullptr, nullptr, nullptr};  // forward copies Bt[out][in]
     __bf16 *wb[4] = {nullptr, nullptr, nullptr, nullptr};  // backward copies W[in][out_pad]
     float *partial = nullptr;
