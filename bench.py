@@ -381,11 +381,12 @@ def committed_traffic(args):
     if (args.variant, args.lds_stack, args.block, args.refill, args.reorder, args.ntris) != (1, 16, 256, 32, 1,
                                                                                           1_000_000):
         return None, "no live counter runs and no committed profile for this configuration"
-    sr = 262_144 / 131_078.9
-    fetch = 6_598_328.8 + 129_623.2 + 16_249.3 + 148_397.1 + 51.8
-    write = 214_008.2 + 64_800.0 + 1_524.0 + 298_010.9 + 1_524.1
-    return ({"call_bytes": (fetch * sr + write) * 1024, "trace_bytes": (6_598_328.8 * sr + 214_008.2) * 1024,
-             "gather_bytes": 4_158_705.4 * sr * 1024},
+    sr = 262_144 / 131_079.5
+    # keys, 3 scans, 3 scatters, 2 histograms, traversal
+    fetch = 129_729.2 + 3_128.5 + 24_564.5 + 32_586.1 + 40_775.8 + 40_738.0 + 6_598_886.9
+    write = 49_932.7 + 6_120.0 + 78_975.1 + 32_480.9 + 79_058.8 + 39_029.7 + 214_054.2
+    return ({"call_bytes": (fetch * sr + write) * 1024, "trace_bytes": (6_598_886.9 * sr + 214_054.2) * 1024,
+             "gather_bytes": (4_157_602.9 * sr + 16_384.0) * 1024},
             "profiles/k_trace_speculative_soup1M_pmc.md (committed counter runs)")
 
 

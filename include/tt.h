@@ -542,6 +542,11 @@ tt_status tt_debug_guide_sample(tt_ctx *ctx, const float *raw, int64_t n, uint32
  * tt_guide_forward, computed by the training path's separate encoding and layer
  * kernels. The two agree bit for bit. Synchronous. */
 tt_status tt_debug_guide_forward_unfused(tt_guide *g, const float *inputs, int64_t n, float *raw_out);
+/* Check hook for the fused training kernels: the loss and gradient tt_guide_grad
+ * returns, computed with the separate encoding, layer and backward-product kernels.
+ * The two agree bit for bit. Synchronous. */
+tt_status tt_debug_guide_grad_unfused(tt_guide *g, const float *inputs, const float *dirs, const float *wts, int64_t n,
+                                      float *loss, float *grad_out);
 
 /* Reproducibility hook for the render math: applies op (0 sin, 1 cos, 2 pow(x, y),
  * 3 atan2(x, y) with x the ordinate, 4 acos) of the device sampler and env-lookup math

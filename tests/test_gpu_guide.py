@@ -69,6 +69,26 @@ def test_gradient_is_bit_reproducible(tt, ctx):
     g.close()
 
 
+@pytest.mark.parametrize("n", [1, 17, 1000, 70_001, 262_144])
+def test_fused_training_kernels_match_unfused(tt, ctx, n):
+    """The training step's fused forward (features and activations written by the
+    inference kernel) and fused backward products give the loss and gradient of the
+    separate encoding, layer and product kernels bit for bit."""
+    rng = np.random.default_rng(n)
+    x = rng.random((n, 9), dtype=np.float32)
+    d = rng.normal(size=(n, 3))
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    w = rng.gamma(1.0, 1.0, n).astype(np.float32)
+    g = tt.Guide(ctx, seed=7, lr=1e-3)
+    g.train_step(x, d, w)                      # weights away from their initial values
+    lf, gf = g.grad(x, d, w)
+    lu, gu = g.grad_unfused(x, d, w)
+    assert np.float32(lf).view(np.uint32) == np.float32(lu).view(np.uint32)
+    assert np.array_equal(gf.view(np.uint32), gu.view(np.uint32)), \
+        f"{np.count_nonzero(gf != gu)} gradient elements differ"
+    g.close()
+
+
 def test_gradient_matches_oracle(tt, ctx):
     from oracle import guide_ref as G
     g = tt.Guide(ctx, seed=5)

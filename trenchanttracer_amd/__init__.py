@@ -352,6 +352,16 @@ class Guide:
         check(lib.tt_guide_grad(self._h, _vp(x), _vp(d), _vp(w), len(x), C.byref(loss), _vp(g)))
         return loss.value, g
 
+    def grad_unfused(self, inputs, dirs, wts):
+        """grad() through the separate (unfused) training kernels, for checking the fused ones."""
+        x = np.ascontiguousarray(inputs, np.float32).reshape(-1, 9)
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        w = np.ascontiguousarray(wts, np.float32).reshape(-1)
+        g = np.empty(self.nparams, np.float32)
+        loss = C.c_float()
+        check(lib.tt_debug_guide_grad_unfused(self._h, _vp(x), _vp(d), _vp(w), len(x), C.byref(loss), _vp(g)))
+        return loss.value, g
+
     def train_step(self, inputs, dirs, wts):
         x = np.ascontiguousarray(inputs, np.float32).reshape(-1, 9)
         d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)

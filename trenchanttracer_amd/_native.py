@@ -154,6 +154,7 @@ PROTOS = {
     "tt_guide_set_params": (C.c_int, [P, P]),
     "tt_guide_forward": (C.c_int, [P, P, I64, P]),
     "tt_guide_grad": (C.c_int, [P, P, P, P, I64, PF, P]),
+    "tt_debug_guide_grad_unfused": (C.c_int, [P, P, P, P, I64, PF, P]),
     "tt_guide_train_step": (C.c_int, [P, P, P, P, I64, PF]),
     "tt_debug_guide_sample": (C.c_int, [P, P, I64, U32, P]),
     "tt_debug_guide_forward_unfused": (C.c_int, [P, P, I64, P]),

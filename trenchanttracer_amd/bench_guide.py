@@ -18,7 +18,7 @@ in the worst 0.1% of pixels.
 The same protocol on the door room (256x256, depth 12), where all light reaches
 the camera's part of the room through a door, with a 4096-spp oracle.
 
-    python -m trenchanttracer_amd.bench_guide [--oracle-spp 8192] [--seeds 4] [--skip-c5] [--quick]
+    python -m trenchanttracer_amd.bench_guide [--oracle-spp 8192] [--seeds 4] [--skip-pathtrace] [--quick]
     python -m trenchanttracer_amd.bench_guide --error-classes
     python -m trenchanttracer_amd.bench_guide --sweep [--seeds 2]
 
@@ -198,7 +198,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--oracle-spp", type=int, default=8192)
     ap.add_argument("--seeds", type=int, default=4, help="independent renders averaged per point")
-    ap.add_argument("--skip-c5", action="store_true")
+    ap.add_argument("--skip-pathtrace", action="store_true", help="door room only")
     ap.add_argument("--quick", action="store_true", help="door room only, one training run (for profiling)")
     ap.add_argument("--error-classes", action="store_true", help="where the path-trace case's error sits")
     ap.add_argument("--sweep", action="store_true", help="path-trace case under the training schedules of SWEEP")
@@ -233,13 +233,13 @@ def main():
     result["door_room_256"]["oracle_spp"] = 4096
     sc.close()
 
-    if not (args.skip_c5 or args.quick):
+    if not (args.skip_pathtrace or args.quick):
         mesh, mats, env, cam = workloads.pathtrace()
         sc = tt.DeviceScene(ctx, mesh, tt.BVH.build(mesh), mats, env=env)
         t0 = time.time()
         _, ref, st = sc.render(cam, tt.render_params(spp=args.oracle_spp, max_depth=16, rr_depth=4, seed=1234,
                                                      mode=tt.RENDER_PATH, fp64=True), want_fp64=True)
-        log(f"C5 oracle {args.oracle_spp} spp in {time.time() - t0:.1f}s")
+        log(f"path-trace case oracle {args.oracle_spp} spp in {time.time() - t0:.1f}s")
         result["pathtrace_1080p"] = compare(tt, ctx, sc, cam, ref, 16, (16, 32, 64, 128), (32, 64, 128),
                                             2 * 1920 * 1080, 1e-3, 262144, 65536, args.seeds)
         result["pathtrace_1080p"]["oracle_spp"] = args.oracle_spp

@@ -155,7 +155,24 @@ struct InferArgs {
     int64_t n;
     const int32_t *n_dev;
     const int32_t *live;   // if set: rows with live[r] == 0 are not needed; tiles of none are skipped
+    // training forward: if set, the features [rows][224] and the three hidden layers'
+    // activations [rows][64] of every 16-row tile are written too (the backward pass
+    // reads them), for all rows of the tile
+    __bf16 *x_out;
+    __bf16 *h_out[3];
 };
+
+// The wave's 16-row LDS tile (row stride LD) to global rows [tile * 16, tile * 16 + 16)
+// of a row-major [rows][K] array, 16 bytes per lane.
+template <int K, int LD>
+__device__ __forceinline__ void tile_to_global(const __bf16 *tile_lds, __bf16 *dst, int64_t tile, int lane) {
+    constexpr int VEC = 16 * K / 8;
+    for (int v = lane; v < VEC; v += 64) {
+        const int rr = v / (K / 8), c = v % (K / 8);
+        *reinterpret_cast<bf16x8 *>(dst + (tile * 16 + rr) * K + 8 * c) =
+            *reinterpret_cast<const bf16x8 *>(tile_lds + rr * LD + 8 * c);
+    }
+}
 
 template <int K, int LD>
 __device__ __forceinline__ void stage_weights(__bf16 *dst, const __bf16 *src, int rows) {
@@ -257,19 +274,23 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(InferArgs a) {
             *reinterpret_cast<bf16x8 *>(ft + lane * kLd1 + kGuideFeat) = z;   // features 216..223
         }
         wave_sync();
+        if (a.x_out) tile_to_global<kGuideK1, kLd1>(ft, a.x_out, tile, lane);
         f32x4 acc[kGuideH / 16];
         tile_mfma<kGuideH / 16, kGuideK1, kLd1>(ft, w1, lr, lq, acc);
         wave_sync();
         store_hidden(ft, acc, b1, lr, lq);
         wave_sync();
+        if (a.h_out[0]) tile_to_global<kGuideH, kLdH>(ft, a.h_out[0], tile, lane);
         tile_mfma<kGuideH / 16, kGuideH, kLdH>(ft, w2, lr, lq, acc);
         wave_sync();
         store_hidden(ft, acc, b2, lr, lq);
         wave_sync();
+        if (a.h_out[1]) tile_to_global<kGuideH, kLdH>(ft, a.h_out[1], tile, lane);
         tile_mfma<kGuideH / 16, kGuideH, kLdH>(ft, w3, lr, lq, acc);
         wave_sync();
         store_hidden(ft, acc, b3, lr, lq);
         wave_sync();
+        if (a.h_out[2]) tile_to_global<kGuideH, kLdH>(ft, a.h_out[2], tile, lane);
         f32x4 out[kGuideNOutPad16 / 16];
         tile_mfma<kGuideNOutPad16 / 16, kGuideH, kLdH>(ft, w4, lr, lq, out);
         wave_sync();
@@ -278,6 +299,94 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(InferArgs a) {
 #pragma unroll
             for (int j = 0; j < 4; ++j)
                 a.raw[(tile * 16 + 4 * lq + j) * kGuideNOutPad16 + 16 * t + lr] = out[t][j] + b4[16 * t + lr];
+    }
+}
+
+// ------------------------------------------------------------------ fused backward (data)
+// dH3 = (dRaw W4) * (H3 > 0), dH2 = (dH3 W3) * (H2 > 0), dH1 = (dH2 W2) * (H1 > 0) for
+// every 16-row tile of the batch in one kernel: the three backward weight matrices
+// (W[in][out_pad] copies, the Bt operand of these products) are staged in LDS once
+// per block; a wave reads its tile's dRaw rows from global, keeps each product's
+// output in its LDS tile as the next product's A operand, and writes dH3, dH2, dH1
+// (the weight gradients read them). Arithmetic, rounding points and accumulation
+// order are those of the three k_gemm launches it replaces, so the outputs are
+// identical to theirs.
+constexpr int kBwdWaves = 8;
+constexpr int kBwdLds = 2 * (3 * kGuideH * kLdH + kBwdWaves * 16 * kLdH);
+
+struct BwdArgs {
+    const __bf16 *draw;      // [R][64] output gradients
+    const __bf16 *wb[3];     // backward copies of layers 2, 3, 4: [in 64][out_pad 64]
+    const __bf16 *h[3];      // H1, H2, H3 [R][64] (ReLU masks)
+    __bf16 *dh[3];           // dH1, dH2, dH3 [R][64]
+    int64_t rows;            // R, a multiple of 16
+};
+
+// acc (D layout) -> bf16 into the wave's LDS tile; then, row-major with 16 bytes per
+// lane, the ReLU mask of the layer (h > 0) is applied in LDS and the tile written to
+// global rows. The same values as masking before the conversion (a masked value is 0
+// either way).
+__device__ __forceinline__ void store_grad(__bf16 *tile_lds, __bf16 *dst, const __bf16 *mask, const f32x4 *acc,
+                                           int64_t tile, int lane) {
+    const int lr = lane & 15, lq = lane >> 4;
+#pragma unroll
+    for (int t = 0; t < kGuideH / 16; ++t)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) tile_lds[(4 * lq + j) * kLdH + 16 * t + lr] = (__bf16)(acc[t][j] + 0.0f);
+    wave_sync();
+#pragma unroll
+    for (int v = lane; v < 16 * kGuideH / 8; v += 64) {
+        const int rr = v / (kGuideH / 8), c = 8 * (v % (kGuideH / 8));
+        const int64_t row = tile * 16 + rr;
+        bf16x8 x = *reinterpret_cast<const bf16x8 *>(tile_lds + rr * kLdH + c);
+        const bf16x8 m = *reinterpret_cast<const bf16x8 *>(mask + row * kGuideH + c);
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (!((float)m[j] > 0.0f)) x[j] = (__bf16)0.0f;
+        *reinterpret_cast<bf16x8 *>(tile_lds + rr * kLdH + c) = x;
+        *reinterpret_cast<bf16x8 *>(dst + row * kGuideH + c) = x;
+    }
+}
+
+__global__ __launch_bounds__(64 * kBwdWaves) void k_mlp_bwd(BwdArgs a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __bf16 *w2 = reinterpret_cast<__bf16 *>(lds_raw);   // layer 2 (dH2 -> dH1)
+    __bf16 *w3 = w2 + kGuideH * kLdH;                   // layer 3 (dH3 -> dH2)
+    __bf16 *w4 = w3 + kGuideH * kLdH;                   // layer 4 (dRaw -> dH3)
+    __bf16 *tiles = w4 + kGuideH * kLdH;
+    stage_weights<kGuideH, kLdH>(w2, a.wb[0], kGuideH);
+    stage_weights<kGuideH, kLdH>(w3, a.wb[1], kGuideH);
+    stage_weights<kGuideNOutPad, kLdH>(w4, a.wb[2], kGuideH);
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lr = lane & 15, lq = lane >> 4;
+    __bf16 *ft = tiles + wave * 16 * kLdH;
+    const int64_t ntiles = a.rows >> 4;
+    for (int64_t tile = (int64_t)blockIdx.x * kBwdWaves + wave; tile < ntiles; tile += (int64_t)gridDim.x * kBwdWaves) {
+        // dH3: A = dRaw rows straight from global (K = 64), as k_gemm reads them
+        f32x4 acc[kGuideH / 16];
+#pragma unroll
+        for (int t = 0; t < kGuideH / 16; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+        const __bf16 *arow = a.draw + (tile * 16 + lr) * kGuideNOutPad + 8 * lq;
+#pragma unroll
+        for (int s2 = 0; s2 < kGuideNOutPad / 32; ++s2) {
+            const bf16x8 av = *reinterpret_cast<const bf16x8 *>(arow + 32 * s2);
+#pragma unroll
+            for (int t = 0; t < kGuideH / 16; ++t) {
+                const bf16x8 b = *reinterpret_cast<const bf16x8 *>(w4 + (16 * t + lr) * kLdH + 32 * s2 + 8 * lq);
+                acc[t] = mfma(av, b, acc[t]);
+            }
+        }
+        store_grad(ft, a.dh[2], a.h[2], acc, tile, lane);
+        wave_sync();
+        tile_mfma<kGuideH / 16, kGuideH, kLdH>(ft, w3, lr, lq, acc);
+        wave_sync();
+        store_grad(ft, a.dh[1], a.h[1], acc, tile, lane);
+        wave_sync();
+        tile_mfma<kGuideH / 16, kGuideH, kLdH>(ft, w2, lr, lq, acc);
+        wave_sync();
+        store_grad(ft, a.dh[0], a.h[0], acc, tile, lane);
+        wave_sync();
     }
 }
 
@@ -295,10 +404,10 @@ __global__ __launch_bounds__(256) void k_loss_grad(const float *__restrict__ raw
     __shared__ float red[256];
     float contrib = 0.0f;
     const int64_t rows_pad = (n + 15) & ~15LL;
-    if (r < rows_pad) {
-        __bf16 *g = draw + r * kGuideNOutPad;
-        for (int c = 0; c < kGuideNOutPad; ++c) g[c] = (__bf16)0.0f;
-    }
+    // the row's 64 gradient columns (zero where nothing is written) go out as 8 x 16 B
+    __bf16 gv[kGuideNOutPad];
+#pragma unroll
+    for (int c = 0; c < kGuideNOutPad; ++c) gv[c] = (__bf16)0.0f;
     if (r < n) {
         const float *rr = raw + r * kGuideNOutPad16;
         Lobes L;
@@ -306,33 +415,35 @@ __global__ __launch_bounds__(256) void k_loss_grad(const float *__restrict__ raw
         const float wx = dirs[3 * r], wy = dirs[3 * r + 1], wz = dirs[3 * r + 2];
         const float scale = wts[r] / fmaxf(*wsum, 1e-30f);
         float lv[kGuideLobes], mx = -1e30f;
+#pragma unroll
         for (int k = 0; k < kGuideLobes; ++k) {
             float ca = L.mu[k][0] * wx + L.mu[k][1] * wy + L.mu[k][2] * wz;
             lv[k] = logf(L.w[k]) + vmf_log(L.kappa[k], ca);
             mx = fmaxf(mx, lv[k]);
         }
         float s = 0.0f;
+#pragma unroll
         for (int k = 0; k < kGuideLobes; ++k) s += expf(lv[k] - mx);
         const float logq = mx + logf(s);
         contrib = scale * (-logq);
-        __bf16 *g = draw + r * kGuideNOutPad;
         const float lk_min = kGuideLogKappaMin, lk_max = kGuideLogKappaMax;
+#pragma unroll
         for (int k = 0; k < kGuideLobes; ++k) {
             const float resp = expf(lv[k] - logq);
             // logits: d(-log q)/d logit_k = w_k - resp_k
-            g[kGuideLobes * 4 + k] = (__bf16)(scale * (L.w[k] - resp));
+            gv[kGuideLobes * 4 + k] = (__bf16)(scale * (L.w[k] - resp));
             // log kappa: d log v / d kappa = 1/k - 2/(e^{2k} - 1) + (mu.w - 1); d kappa / d lk = kappa
             const float kap = L.kappa[k];
             const float ca = L.mu[k][0] * wx + L.mu[k][1] * wy + L.mu[k][2] * wz;
             const float dlv = 1.0f / kap - 2.0f / expm1f(2.0f * kap) + (ca - 1.0f);
             const float lk = rr[kGuideLobes * 3 + k];
             const float gk = (lk > lk_min && lk < lk_max) ? -scale * resp * dlv * kap : 0.0f;
-            g[kGuideLobes * 3 + k] = (__bf16)gk;
+            gv[kGuideLobes * 3 + k] = (__bf16)gk;
             // raw mean m: d log v / d m = kappa (w - (mu.w) mu) / |m|
             const float f = -scale * resp * kap * L.inv_len[k];
-            g[3 * k] = (__bf16)(f * (wx - ca * L.mu[k][0]));
-            g[3 * k + 1] = (__bf16)(f * (wy - ca * L.mu[k][1]));
-            g[3 * k + 2] = (__bf16)(f * (wz - ca * L.mu[k][2]));
+            gv[3 * k] = (__bf16)(f * (wx - ca * L.mu[k][0]));
+            gv[3 * k + 1] = (__bf16)(f * (wy - ca * L.mu[k][1]));
+            gv[3 * k + 2] = (__bf16)(f * (wz - ca * L.mu[k][2]));
         }
         if (pb) {
             const float q = expf(logq), p_b = pb[r];
@@ -340,7 +451,17 @@ __global__ __launch_bounds__(256) void k_loss_grad(const float *__restrict__ raw
             const float al = kAlphaLo + (kAlphaHi - kAlphaLo) * sg;
             const float pc = fmaxf(al * q + (1.0f - al) * p_b, 1e-30f);
             const float ga = -scale * (q - p_b) / pc * (kAlphaHi - kAlphaLo) * sg * (1.0f - sg);
-            g[kGuideAlphaOut] = (__bf16)(isfinite(ga) ? ga : 0.0f);
+            gv[kGuideAlphaOut] = (__bf16)(isfinite(ga) ? ga : 0.0f);
+        }
+    }
+    if (r < rows_pad) {
+        bf16x8 *g8 = reinterpret_cast<bf16x8 *>(draw + r * kGuideNOutPad);
+#pragma unroll
+        for (int v = 0; v < kGuideNOutPad / 8; ++v) {
+            bf16x8 w;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = gv[8 * v + j];
+            g8[v] = w;
         }
     }
     red[threadIdx.x] = contrib;
@@ -468,35 +589,49 @@ __global__ __launch_bounds__(256) void k_wgrad(const __bf16 *__restrict__ H, int
     if (threadIdx.x < KO) pbias[(int64_t)blockIdx.x * KO + threadIdx.x] = bsum;
 }
 
-// Sum of the nb per-block partial gradients of each element, in a fixed order
-// (deterministic): a block takes 64 elements, its kRedParts waves each sum every
-// kRedParts-th partial (unrolled, so several loads are in flight per lane), and the
-// first wave adds the part sums in part order. dst transposes [in][out] partials to
-// the [out][in] master layout when transpose_ko is set.
+// Sum of the nb per-block partial gradients of every parameter, in a fixed order
+// (deterministic), all layers in one launch: 8 regions of partials (each layer's
+// weights [in][out] and biases [out], region r at partial + start[r], block b's values
+// at + b * count[r]). A block takes 64 elements of the concatenated regions, its
+// kRedParts waves each sum every kRedParts-th partial (unrolled, so several loads are
+// in flight per lane), and the first wave adds the part sums in part order; weights
+// are transposed to the [out][in] master layout.
 constexpr int kRedParts = 16;
-__global__ __launch_bounds__(64 * kRedParts) void k_reduce(const float *__restrict__ partial, int nb, int count,
-                                                           float *__restrict__ grad, int offset, bool transpose_ko,
-                                                           int ki, int ko) {
+struct RedRegions {
+    int64_t start[8];
+    int count[8], first[8];   // elements, and the first element of the region in the concatenation
+    int dst[8], ki[8], ko[8];
+    int transpose[8];
+};
+__global__ __launch_bounds__(64 * kRedParts) void k_reduce_all(const float *__restrict__ partial, int nb, int total,
+                                                               RedRegions R, float *__restrict__ grad) {
     __shared__ float red[kRedParts][64];
     const int el = threadIdx.x & 63, part = threadIdx.x >> 6;
     const int e = blockIdx.x * 64 + el;
+    int r = 0;
+#pragma unroll
+    for (int q = 1; q < 8; ++q)
+        if (e >= R.first[q]) r = q;
+    const int le = e - R.first[r];
     float s = 0.0f;
-    if (e < count) {
+    if (e < total) {
+        const float *src = partial + R.start[r] + le;
+        const int64_t stride = R.count[r];
 #pragma unroll 8
-        for (int b = part; b < nb; b += kRedParts) s += partial[(int64_t)b * count + e];
+        for (int b = part; b < nb; b += kRedParts) s += src[(int64_t)b * stride];
     }
     red[part][el] = s;
     __syncthreads();
-    if (part == 0 && e < count) {
+    if (part == 0 && e < total) {
         float t = red[0][el];
 #pragma unroll
         for (int q = 1; q < kRedParts; ++q) t += red[q][el];
-        int dst = e;
-        if (transpose_ko) {
-            const int i = e / ko, o = e % ko;
-            dst = o * ki + i;
+        int d = le;
+        if (R.transpose[r]) {
+            const int i = le / R.ko[r], o = le % R.ko[r];
+            d = o * R.ki[r] + i;
         }
-        grad[offset + dst] = t;
+        grad[R.dst[r] + d] = t;
     }
 }
 
@@ -648,10 +783,14 @@ hipError_t guide_forward(tt_guide *g, int64_t n, const int32_t *n_dev, hipStream
 // Inference over the first n rows of g->batch.in (device count n_dev optional):
 // the fused kernel, raw outputs (fp32, 48 wide) into g->batch.raw.
 hipError_t guide_infer(tt_guide *g, const float *in, float *raw, int64_t n, const int32_t *n_dev, hipStream_t s,
-                       const int32_t *live) {
+                       const int32_t *live, bool store_acts) {
     InferArgs a{};
     a.in = in;
     a.live = live;
+    if (store_acts) {   // the training forward pass: features and activations into the batch
+        a.x_out = g->batch.x;
+        for (int k = 0; k < 3; ++k) a.h_out[k] = g->batch.h[k];
+    }
     for (int k = 0; k < 4; ++k) {
         a.w[k] = g->wf[k];
         a.bias[k] = g->params + layer_offset(k) + (int64_t)kGuideLayers[k].in * kGuideLayers[k].out;
@@ -673,10 +812,13 @@ hipError_t guide_infer(tt_guide *g, const float *in, float *raw, int64_t n, cons
 }
 
 // Gradient of the loss over rows [0, n) of the batch (inputs, dirs, wts staged).
-static hipError_t guide_backward(tt_guide *g, int64_t n, hipStream_t s) {
+// fused: the forward pass in the fused kernel (writing the activations) and the
+// three backward products in k_mlp_bwd; otherwise k_encode and one k_gemm launch per
+// product (the same bits; kept as the check of the fused path).
+static hipError_t guide_backward(tt_guide *g, int64_t n, hipStream_t s, bool fused = true) {
     GuideBatch &b = g->batch;
     const int64_t R = (n + 15) & ~15LL;
-    hipError_t e = guide_forward(g, n, nullptr, s);
+    hipError_t e = fused ? guide_infer(g, b.in, b.raw, n, nullptr, s, nullptr, true) : guide_forward(g, n, nullptr, s);
     if (e != hipSuccess) return e;
     // weight sum and loss: per-block partials added in a fixed order (no float atomics)
     hipLaunchKernelGGL(k_sum, dim3(kGuideSumParts), dim3(256), 0, s, b.wts, n, g->scalars + kGuideSumParts);
@@ -687,30 +829,40 @@ static hipError_t guide_backward(tt_guide *g, int64_t n, hipStream_t s) {
                        g->train_alpha ? (const float *)b.pb : (const float *)nullptr, b.draw, b.lpart,
                        (const float *)(g->scalars + 1), n);
     hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, s, (const float *)b.lpart, (int64_t)lb, g->scalars);
-    const unsigned gw = (unsigned)std::min<int64_t>(((R / 16) + 3) / 4 + 1, 8192);
     // dH3 = dRaw W4 (mask H3 > 0), dH2 = dH3 W3 (mask), dH1 = dH2 W2 (mask)
-    hipLaunchKernelGGL((k_gemm<kGuideNOutPad, kGuideH, false, true>), dim3(gw), dim3(256), 0, s, b.draw, g->wb[3],
-                       (const float *)nullptr, (void *)b.dh[2], kGuideH, (const __bf16 *)b.h[2], R, (const int32_t *)nullptr);
-    hipLaunchKernelGGL((k_gemm<kGuideH, kGuideH, false, true>), dim3(gw), dim3(256), 0, s, b.dh[2], g->wb[2],
-                       (const float *)nullptr, (void *)b.dh[1], kGuideH, (const __bf16 *)b.h[1], R, (const int32_t *)nullptr);
-    hipLaunchKernelGGL((k_gemm<kGuideH, kGuideH, false, true>), dim3(gw), dim3(256), 0, s, b.dh[1], g->wb[1],
-                       (const float *)nullptr, (void *)b.dh[0], kGuideH, (const __bf16 *)b.h[0], R, (const int32_t *)nullptr);
+    if (fused) {
+        BwdArgs ba{};
+        ba.draw = b.draw;
+        for (int k = 0; k < 3; ++k) {
+            ba.wb[k] = g->wb[k + 1];
+            ba.h[k] = b.h[k];
+            ba.dh[k] = b.dh[k];
+        }
+        ba.rows = R;
+        e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_mlp_bwd), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                kBwdLds);
+        if (e != hipSuccess) return e;
+        const int64_t grid = std::max<int64_t>(
+            1, std::min<int64_t>((R / 16 + kBwdWaves - 1) / kBwdWaves, 2 * (int64_t)std::max(1, g->ctx->num_cus)));
+        hipLaunchKernelGGL(k_mlp_bwd, dim3((unsigned)grid), dim3(64 * kBwdWaves), kBwdLds, s, ba);
+    } else {
+        const unsigned gw = (unsigned)std::min<int64_t>(((R / 16) + 3) / 4 + 1, 8192);
+        hipLaunchKernelGGL((k_gemm<kGuideNOutPad, kGuideH, false, true>), dim3(gw), dim3(256), 0, s, b.draw, g->wb[3],
+                           (const float *)nullptr, (void *)b.dh[2], kGuideH, (const __bf16 *)b.h[2], R,
+                           (const int32_t *)nullptr);
+        hipLaunchKernelGGL((k_gemm<kGuideH, kGuideH, false, true>), dim3(gw), dim3(256), 0, s, b.dh[2], g->wb[2],
+                           (const float *)nullptr, (void *)b.dh[1], kGuideH, (const __bf16 *)b.h[1], R,
+                           (const int32_t *)nullptr);
+        hipLaunchKernelGGL((k_gemm<kGuideH, kGuideH, false, true>), dim3(gw), dim3(256), 0, s, b.dh[1], g->wb[1],
+                           (const float *)nullptr, (void *)b.dh[0], kGuideH, (const __bf16 *)b.h[0], R,
+                           (const int32_t *)nullptr);
+    }
     // weight gradients: dW_k = H_{k-1}^T dH_k and bias gradients, partial sums per
     // block (at most kWgBlocks blocks of whole kWgRows chunks), reduced in block order
     constexpr int64_t kWgBlocks = 1024, kWgMinRows = 256;
     const int64_t rpb =
         std::max<int64_t>(kWgMinRows, ((R + kWgBlocks - 1) / kWgBlocks + kWgRows - 1) / kWgRows * kWgRows);
     const int nb = (int)((R + rpb - 1) / rpb);
-    const int64_t need = (int64_t)nb * (kGuideK1 * kGuideH + kGuideNOutPad16 + kGuideH);
-    if (g->partial_cap < need) {
-        if (g->partial) hipFree(g->partial);
-        g->partial = nullptr;
-        g->partial_cap = 0;
-        if (dev_alloc(g->ctx, (void **)&g->partial, (size_t)need * 4, "guide partials") != TT_OK)
-            return hipErrorOutOfMemory;
-        g->partial_cap = need;
-    }
-    float *pbias = g->partial + (int64_t)nb * kGuideK1 * kGuideH;
     struct L {
         const __bf16 *h;
         int ldh;
@@ -721,23 +873,49 @@ static hipError_t guide_backward(tt_guide *g, int64_t n, hipStream_t s) {
                {b.h[0], kGuideH, b.dh[1], kGuideH, kGuideH, kGuideH},
                {b.h[1], kGuideH, b.dh[2], kGuideH, kGuideH, kGuideH},
                {b.h[2], kGuideH, b.draw, kGuideNOutPad, kGuideH, kGuideNOutPad16}};
+    // one partial region per layer's weights and per layer's biases, so that a single
+    // launch reduces them all
+    RedRegions rg{};
+    int64_t at = 0;
+    int first = 0;
+    for (int k = 0; k < 4; ++k) {
+        for (int part = 0; part < 2; ++part) {
+            const int r = 2 * k + part;
+            rg.count[r] = part == 0 ? ls[k].ki * ls[k].ko : ls[k].ko;
+            rg.start[r] = at;
+            rg.first[r] = first;
+            rg.dst[r] = (int)layer_offset(k) + (part == 0 ? 0 : ls[k].ki * ls[k].ko);
+            rg.ki[r] = ls[k].ki;
+            rg.ko[r] = ls[k].ko;
+            rg.transpose[r] = part == 0;
+            at += (int64_t)nb * rg.count[r];
+            first += rg.count[r];
+        }
+    }
+    const int64_t need = at;
+    if (g->partial_cap < need) {
+        if (g->partial) hipFree(g->partial);
+        g->partial = nullptr;
+        g->partial_cap = 0;
+        if (dev_alloc(g->ctx, (void **)&g->partial, (size_t)need * 4, "guide partials") != TT_OK)
+            return hipErrorOutOfMemory;
+        g->partial_cap = need;
+    }
     for (int k = 0; k < 4; ++k) {
         const L &l = ls[k];
+        float *pw = g->partial + rg.start[2 * k], *pbias = g->partial + rg.start[2 * k + 1];
         if (k == 0)
             hipLaunchKernelGGL((k_wgrad<kGuideK1, kGuideH>), dim3(nb), dim3(256), 0, s, l.h, l.ldh, l.d, l.ldd, R, rpb,
-                               g->partial, pbias);
+                               pw, pbias);
         else if (k < 3)
             hipLaunchKernelGGL((k_wgrad<kGuideH, kGuideH>), dim3(nb), dim3(256), 0, s, l.h, l.ldh, l.d, l.ldd, R, rpb,
-                               g->partial, pbias);
+                               pw, pbias);
         else
             hipLaunchKernelGGL((k_wgrad<kGuideH, kGuideNOutPad16>), dim3(nb), dim3(256), 0, s, l.h, l.ldh, l.d, l.ldd,
-                               R, rpb, g->partial, pbias);
-        const int cnt = l.ki * l.ko;
-        hipLaunchKernelGGL(k_reduce, dim3((cnt + 63) / 64), dim3(64 * kRedParts), 0, s, (const float *)g->partial, nb,
-                           cnt, g->grad, (int)layer_offset(k), true, l.ki, l.ko);
-        hipLaunchKernelGGL(k_reduce, dim3((l.ko + 63) / 64), dim3(64 * kRedParts), 0, s, (const float *)pbias, nb,
-                           l.ko, g->grad, (int)(layer_offset(k) + cnt), false, l.ki, l.ko);
+                               R, rpb, pw, pbias);
     }
+    hipLaunchKernelGGL(k_reduce_all, dim3((first + 63) / 64), dim3(64 * kRedParts), 0, s, (const float *)g->partial,
+                       nb, first, rg, g->grad);
     return hipGetLastError();
 }
 
@@ -920,12 +1098,12 @@ static tt_status stage_rows(tt_guide *g, const float *inputs, const float *dirs,
     return TT_OK;
 }
 
-tt_status tt_guide_grad(tt_guide *g, const float *inputs, const float *dirs, const float *wts, int64_t n, float *loss,
-                        float *grad_out) {
+static tt_status grad_impl(tt_guide *g, const float *inputs, const float *dirs, const float *wts, int64_t n,
+                           float *loss, float *grad_out, bool fused) {
     tt_status st = stage_rows(g, inputs, dirs, wts, n);
     if (st != TT_OK) return st;
     hipStream_t s = g->ctx->stream;
-    TT_HIPG(guide_backward(g, n, s), "guide backward");
+    TT_HIPG(guide_backward(g, n, s, fused), "guide backward");
     float sc[2];
     TT_HIPG(hipMemcpyAsync(sc, g->scalars, 8, hipMemcpyDeviceToHost, s), "copy loss");
     if (grad_out)
@@ -933,6 +1111,16 @@ tt_status tt_guide_grad(tt_guide *g, const float *inputs, const float *dirs, con
     TT_HIPG(hipStreamSynchronize(s), "sync");
     if (loss) *loss = sc[0];
     return TT_OK;
+}
+
+tt_status tt_guide_grad(tt_guide *g, const float *inputs, const float *dirs, const float *wts, int64_t n, float *loss,
+                        float *grad_out) {
+    return grad_impl(g, inputs, dirs, wts, n, loss, grad_out, true);
+}
+
+tt_status tt_debug_guide_grad_unfused(tt_guide *g, const float *inputs, const float *dirs, const float *wts, int64_t n,
+                                      float *loss, float *grad_out) {
+    return grad_impl(g, inputs, dirs, wts, n, loss, grad_out, false);
 }
 
 tt_status tt_guide_train_step(tt_guide *g, const float *inputs, const float *dirs, const float *wts, int64_t n,

@@ -177,9 +177,10 @@ __device__ __forceinline__ void mixture_sample(const Mix &L, float u0, float u1,
 }
 
 hipError_t guide_forward(tt_guide *g, int64_t n, const int32_t *n_dev, hipStream_t s);
-// live (may be null): per-row flags; 16-row tiles with no flagged row are skipped
+// live (may be null): per-row flags; 16-row tiles with no flagged row are skipped.
+// store_acts: also write the features and hidden activations into g->batch (training)
 hipError_t guide_infer(tt_guide *g, const float *in, float *raw, int64_t n, const int32_t *n_dev, hipStream_t s,
-                       const int32_t *live = nullptr);
+                       const int32_t *live = nullptr, bool store_acts = false);
 hipError_t guide_train_rows(tt_guide *g, int64_t n, hipStream_t s);
 tt_status guide_alloc_batch(tt_guide *g, int64_t rows);
 tt_status guide_alloc_infer(tt_guide *g, int64_t rows);
