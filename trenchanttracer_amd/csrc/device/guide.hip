@@ -155,10 +155,8 @@ struct InferArgs {
     int64_t n;
     const int32_t *n_dev;
     const int32_t *live;   // if set: rows with live[r] == 0 are not needed; tiles of none are skipped
-    // training forward: if set, the features [rows][224] and the three hidden layers'
-    // activations [rows][64] of every 16-row tile are written too (the backward pass
-    // reads them), for all rows of the tile
-    __bf16 *x_out;
+    // training forward: if set, the three hidden layers' activations [rows][64] of every
+    // 16-row tile are written too (the backward pass reads them), for all rows of the tile
     __bf16 *h_out[3];
 };
 
@@ -274,7 +272,6 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(InferArgs a) {
             *reinterpret_cast<bf16x8 *>(ft + lane * kLd1 + kGuideFeat) = z;   // features 216..223
         }
         wave_sync();
-        if (a.x_out) tile_to_global<kGuideK1, kLd1>(ft, a.x_out, tile, lane);
         f32x4 acc[kGuideH / 16];
         tile_mfma<kGuideH / 16, kGuideK1, kLd1>(ft, w1, lr, lq, acc);
         wave_sync();
@@ -539,10 +536,35 @@ __device__ __forceinline__ void stage_transposed(__bf16 (*dst)[kWgRows + 8], con
     }
 }
 
-template <int KI, int KO>
+// The first layer's input features of rows [rb, rb + kWgRows) straight from the
+// inputs, [feature][row] in LDS: the encoding of k_encode / k_mlp_infer (the same
+// bits), zero for rows at or past `valid` (the batch's rows, or the block's end).
+__device__ __forceinline__ void stage_features(__bf16 (*dst)[kWgRows + 8], const float *__restrict__ in, int64_t rb,
+                                               int64_t valid) {
+    for (int task = threadIdx.x; task < kWgRows * (kGuideIn + 1); task += blockDim.x) {
+        const int rr = task / (kGuideIn + 1), c = task % (kGuideIn + 1);
+        const int64_t row = rb + rr;
+        if (c == kGuideIn) {
+            for (int k = kGuideFeat; k < kGuideK1; ++k) dst[k][rr] = (__bf16)0.0f;
+            continue;
+        }
+        const bool ok = row < valid;
+        FreqState st = freq_base(ok ? in[row * kGuideIn + c] : 0.0f);
+        for (int f = 0; f < kGuideFreq; ++f) {
+            dst[2 * kGuideFreq * c + 2 * f][rr] = (__bf16)(ok ? st.s : 0.0f);
+            dst[2 * kGuideFreq * c + 2 * f + 1][rr] = (__bf16)(ok ? st.c : 0.0f);
+            freq_next(st);
+        }
+    }
+}
+
+// ENC (first layer only): H is not read; the features come from enc_in (rows below
+// enc_rows) through stage_features.
+template <int KI, int KO, bool ENC = false>
 __global__ __launch_bounds__(256) void k_wgrad(const __bf16 *__restrict__ H, int ldh, const __bf16 *__restrict__ G,
                                                int ldg, int64_t n, int64_t rows_per_block, float *__restrict__ partial,
-                                               float *__restrict__ pbias) {
+                                               float *__restrict__ pbias, const float *__restrict__ enc_in = nullptr,
+                                               int64_t enc_rows = 0) {
     constexpr int TI = KI / 16, TO = KO / 16, TILES = TI * TO;
     constexpr int PER_WAVE = (TILES + 3) / 4;
     __shared__ __align__(16) __bf16 hs[KI][kWgRows + 8];  // [feature][row]
@@ -557,7 +579,8 @@ __global__ __launch_bounds__(256) void k_wgrad(const __bf16 *__restrict__ H, int
     float bsum = 0.0f;
     for (int64_t rb = r0; rb < r1; rb += kWgRows) {
         __syncthreads();
-        stage_transposed<KI>(hs, H, ldh, rb, r1);
+        if (ENC) stage_features(hs, enc_in, rb, min(r1, enc_rows));
+        else stage_transposed<KI>(hs, H, ldh, rb, r1);
         stage_transposed<KO>(gs, G, ldg, rb, r1);
         __syncthreads();
         if (threadIdx.x < KO)
@@ -787,10 +810,9 @@ hipError_t guide_infer(tt_guide *g, const float *in, float *raw, int64_t n, cons
     InferArgs a{};
     a.in = in;
     a.live = live;
-    if (store_acts) {   // the training forward pass: features and activations into the batch
-        a.x_out = g->batch.x;
+    if (store_acts)   // the training forward pass: hidden activations into the batch (the
+                      // first layer's weight gradient recomputes the features)
         for (int k = 0; k < 3; ++k) a.h_out[k] = g->batch.h[k];
-    }
     for (int k = 0; k < 4; ++k) {
         a.w[k] = g->wf[k];
         a.bias[k] = g->params + layer_offset(k) + (int64_t)kGuideLayers[k].in * kGuideLayers[k].out;
@@ -904,15 +926,18 @@ static hipError_t guide_backward(tt_guide *g, int64_t n, hipStream_t s, bool fus
     for (int k = 0; k < 4; ++k) {
         const L &l = ls[k];
         float *pw = g->partial + rg.start[2 * k], *pbias = g->partial + rg.start[2 * k + 1];
-        if (k == 0)
+        if (k == 0 && fused)
+            hipLaunchKernelGGL((k_wgrad<kGuideK1, kGuideH, true>), dim3(nb), dim3(256), 0, s, l.h, l.ldh, l.d, l.ldd, R,
+                               rpb, pw, pbias, (const float *)b.in, n);
+        else if (k == 0)
             hipLaunchKernelGGL((k_wgrad<kGuideK1, kGuideH>), dim3(nb), dim3(256), 0, s, l.h, l.ldh, l.d, l.ldd, R, rpb,
-                               pw, pbias);
+                               pw, pbias, (const float *)nullptr, (int64_t)0);
         else if (k < 3)
             hipLaunchKernelGGL((k_wgrad<kGuideH, kGuideH>), dim3(nb), dim3(256), 0, s, l.h, l.ldh, l.d, l.ldd, R, rpb,
-                               pw, pbias);
+                               pw, pbias, (const float *)nullptr, (int64_t)0);
         else
             hipLaunchKernelGGL((k_wgrad<kGuideH, kGuideNOutPad16>), dim3(nb), dim3(256), 0, s, l.h, l.ldh, l.d, l.ldd,
-                               R, rpb, pw, pbias);
+                               R, rpb, pw, pbias, (const float *)nullptr, (int64_t)0);
     }
     hipLaunchKernelGGL(k_reduce_all, dim3((first + 63) / 64), dim3(64 * kRedParts), 0, s, (const float *)g->partial,
                        nb, first, rg, g->grad);
