@@ -574,8 +574,10 @@ tt_status tt_debug_env_invert(tt_scene *s, const float *u, int64_t n, int32_t *o
  * order[r] = index of the r-th ray, the stable sort of the ray indices by the 24-bit
  * Morton code of the origins over the scene's bounds (8 bits per axis, x in bits
  * 3k + 2, y in 3k + 1, z in 3k; cells clamped to 0..255). rays: device, n records;
- * order: host, n int32. Synchronous. */
-tt_status tt_debug_ray_order(tt_scene *s, const tt_ray *rays, int64_t n, int32_t *order);
+ * order: host, n int32. cap > n sorts as a queue of capacity cap whose length n is
+ * read on the device (the path tracer's form); cap <= n, with the length from the
+ * host. Synchronous. */
+tt_status tt_debug_ray_order(tt_scene *s, const tt_ray *rays, int64_t n, int64_t cap, int32_t *order);
 
 #ifdef __cplusplus
 } /* extern "C" */

@@ -3,7 +3,9 @@ call traces in) against numpy: the stable argsort of the 24-bit Morton keys of t
 origins over the scene's bounds, computed here in float32 with the same operations.
 Sizes around the 4096-key tile (1, 63, 4095, 4096, 4097), ragged large ones, all keys
 equal (camera rays: the identity), origins outside the box, infinite and NaN origins,
-and keys whose digits collide in every pass."""
+and keys whose digits collide in every pass; each with the length given on the host and
+as a queue of larger capacity whose length is read on the device (the path tracer's
+form), where the tiles past the length are skipped."""
 import sys
 
 import numpy as np
@@ -45,12 +47,15 @@ def soup(tt, ctx):
 
 
 def _check(tt, ctx, s, rays, lo, hi, what):
-    _say(f"ray order: {what}, {len(rays)} rays")
     d = tt.DeviceBuffer.from_array(ctx, np.ascontiguousarray(rays, np.float32))
-    got = s.ray_order(d, len(rays))
-    d.free()
     want = np.argsort(morton_keys(rays, lo, hi), kind="stable").astype(np.int32)
-    assert np.array_equal(got, want), f"{what}: {np.count_nonzero(got != want)} positions differ"
+    # the length from the host, then as a queue whose length is read on the device
+    # (capacities past the last tile and inside it)
+    for cap in (0, len(rays) + 1, len(rays) + 9000):
+        _say(f"ray order: {what}, {len(rays)} rays, capacity {cap}")
+        got = s.ray_order(d, len(rays), cap)
+        assert np.array_equal(got, want), f"{what}, cap {cap}: {np.count_nonzero(got != want)} positions differ"
+    d.free()
 
 
 @pytest.mark.parametrize("n", [1, 63, 4095, 4096, 4097, 100_003, 1_000_003])
@@ -68,6 +73,7 @@ def test_ray_order_equal_keys_is_identity(soup, ctx):
     _check(tt, ctx, s, rays, lo, hi, "one origin")
     d = tt.DeviceBuffer.from_array(ctx, rays)
     assert np.array_equal(s.ray_order(d, len(rays)), np.arange(len(rays), dtype=np.int32))
+    assert np.array_equal(s.ray_order(d, len(rays), 3 * len(rays)), np.arange(len(rays), dtype=np.int32))
     d.free()
 
 

@@ -689,11 +689,12 @@ class DeviceScene:
         check(lib.tt_debug_env_invert(self._h, _vp(u), len(u), _vp(out)))
         return out
 
-    def ray_order(self, d_rays, n):
+    def ray_order(self, d_rays, n, cap=0):
         """The order a reordering trace call traces the rays of a DeviceBuffer in
-        (tt_debug_ray_order): int32 array, order[r] = index of the r-th ray."""
+        (tt_debug_ray_order): int32 array, order[r] = index of the r-th ray. cap > n:
+        sorted as a queue of that capacity whose length is read on the device."""
         out = np.empty(int(n), np.int32)
-        check(lib.tt_debug_ray_order(self._h, C.c_void_p(d_rays.ptr), int(n), _vp(out)))
+        check(lib.tt_debug_ray_order(self._h, C.c_void_p(d_rays.ptr), int(n), int(cap), _vp(out)))
         return out
 
     def env_samples(self, n, seed=1):

@@ -173,7 +173,7 @@ PROTOS = {
     "tt_debug_bsdf_sample": (C.c_int, [P, C.POINTER(tt_material), PF, I32, I64, U32, P, P]),
     "tt_debug_env_sample": (C.c_int, [P, I64, U32, P]),
     "tt_debug_env_invert": (C.c_int, [P, P, I64, P]),
-    "tt_debug_ray_order": (C.c_int, [P, P, I64, P]),
+    "tt_debug_ray_order": (C.c_int, [P, P, I64, I64, P]),
     "tt_trace_closest": (C.c_int, [P, P, I64, P, C.POINTER(tt_trace_cfg), P]),
     "tt_trace_closest_noalloc": (C.c_int, [P, P, I64, P, C.POINTER(tt_trace_cfg), P]),
     "tt_bench_fetch_replay": (C.c_int, [P, P, I64, C.POINTER(tt_trace_cfg), C.c_int32, C.POINTER(C.c_float),

@@ -572,18 +572,27 @@ static tt_status trace_impl(tt_scene *s, const tt_ray *rays, int64_t n, tt_hit *
     return TT_OK;
 }
 
-tt_status tt_debug_ray_order(tt_scene *s, const tt_ray *rays, int64_t n, int32_t *order) {
-    if (!s || !rays || !order || n <= 0 || n >= ((int64_t)1 << 31))
+tt_status tt_debug_ray_order(tt_scene *s, const tt_ray *rays, int64_t n, int64_t cap, int32_t *order) {
+    if (!s || !rays || !order || n <= 0 || n >= ((int64_t)1 << 31) || cap >= ((int64_t)1 << 31))
         return fail(TT_ERR_INVALID, "tt_debug_ray_order: bad arguments");
     tt_ctx *ctx = s->ctx;
     TT_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    const bool dev_count = cap > n;
     size_t need = 0, sort_bytes = 0;
-    TT_HIP(reorder_workspace_bytes(n, &need, &sort_bytes), "tt_debug_ray_order: workspace size");
+    TT_HIP(reorder_workspace_bytes(dev_count ? cap : n, &need, &sort_bytes), "tt_debug_ray_order: workspace size");
     void *ws = nullptr;
-    tt_status st = dev_alloc(ctx, &ws, need, "tt_debug_ray_order workspace");
+    tt_status st = dev_alloc(ctx, &ws, need + 256, "tt_debug_ray_order workspace");
     if (st != TT_OK) return st;
+    int32_t *d_n = reinterpret_cast<int32_t *>(static_cast<char *>(ws) + need);
+    const int32_t n32 = (int32_t)n;
     const uint32_t *d_order = nullptr;
-    hipError_t e = reorder_rays(rays, n, s->bmin, s->bmax, ws, sort_bytes, ctx->stream, &d_order);
+    hipError_t e = hipSuccess;
+    if (dev_count) {
+        e = hipMemcpyAsync(d_n, &n32, 4, hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) e = reorder_rays_dev(rays, cap, d_n, s->bmin, s->bmax, ws, ctx->stream, &d_order);
+    } else {
+        e = reorder_rays(rays, n, s->bmin, s->bmax, ws, sort_bytes, ctx->stream, &d_order);
+    }
     if (e == hipSuccess) e = hipMemcpyAsync(order, d_order, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     hipFree(ws);

@@ -46,6 +46,11 @@ enum TriMode { kTriReference = 0, kTriTwoSided = 1 };
 hipError_t reorder_workspace_bytes(int64_t n, size_t *bytes, size_t *sort_bytes);
 hipError_t reorder_rays(const tt_ray *rays, int64_t n, const float lo[3], const float hi[3], void *ws,
                         size_t sort_bytes, hipStream_t s, const uint32_t **order);
+// The same for a queue whose length is read on the device (n_dev, capped at cap); the
+// workspace is reorder_workspace_bytes(cap); launches sized for cap, blocks past the
+// count leave at once.
+hipError_t reorder_rays_dev(const tt_ray *rays, int64_t cap, const int32_t *n_dev, const float lo[3],
+                            const float hi[3], void *ws, hipStream_t s, const uint32_t **order);
 
 // Inversion of a float CDF (cdf[0] = 0, nondecreasing): the largest k in [lo, hi]
 // with cdf[k] < u, or lo when there is none. Over [0, n - 1] this is the sampled

@@ -82,10 +82,18 @@ __device__ __forceinline__ void write_hist(const uint32_t *h, uint32_t *hist, in
 
 // Keys of the rays (24-bit Morton code of the origin) and the first pass's tile
 // histograms.
-__global__ __launch_bounds__(kSortThreads) void k_sort_keys(const tt_ray *__restrict__ rays, int64_t n, float lox,
-                                                            float loy, float loz, float sx, float sy, float sz,
-                                                            uint32_t *__restrict__ keys, uint32_t *__restrict__ hist,
-                                                            int64_t tiles) {
+// The number of keys: n, or the device count n_dev (capped at n) when given; tiles at or
+// past it are neither histogrammed nor scanned nor scattered.
+__device__ __forceinline__ int64_t sort_count(int64_t n, const int32_t *n_dev) {
+    return n_dev ? min((int64_t)max(*n_dev, 0), n) : n;
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_sort_keys(const tt_ray *__restrict__ rays, int64_t n_cap,
+                                                            const int32_t *n_dev, float lox, float loy, float loz,
+                                                            float sx, float sy, float sz, uint32_t *__restrict__ keys,
+                                                            uint32_t *__restrict__ hist, int64_t tiles) {
+    const int64_t n = sort_count(n_cap, n_dev);
+    if ((int64_t)blockIdx.x * kSortTile >= n) return;   // the whole block: past the keys
     __shared__ uint32_t h[256];
     for (int d = threadIdx.x; d < 256; d += kSortThreads) h[d] = 0u;
     __syncthreads();
@@ -111,8 +119,11 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_keys(const tt_ray *__rest
 }
 
 // Tile histograms of the digit at `shift`.
-__global__ __launch_bounds__(kSortThreads) void k_sort_hist(const uint32_t *__restrict__ keys, int64_t n, int shift,
-                                                            uint32_t *__restrict__ hist, int64_t tiles) {
+__global__ __launch_bounds__(kSortThreads) void k_sort_hist(const uint32_t *__restrict__ keys, int64_t n_cap,
+                                                            const int32_t *n_dev, int shift, uint32_t *__restrict__ hist,
+                                                            int64_t tiles) {
+    const int64_t n = sort_count(n_cap, n_dev);
+    if ((int64_t)blockIdx.x * kSortTile >= n) return;
     __shared__ uint32_t h[256];
     for (int d = threadIdx.x; d < 256; d += kSortThreads) h[d] = 0u;
     __syncthreads();
@@ -149,14 +160,15 @@ __device__ __forceinline__ uint32_t block_scan_u32(uint32_t x, uint32_t *sh, uin
     return incl - x;
 }
 
-// One block per digit: hist[d][*] becomes its exclusive prefix over the tiles, and
-// total[d] the digit's count.
-__global__ __launch_bounds__(kSortThreads) void k_sort_scan(uint32_t *__restrict__ hist, int64_t tiles,
-                                                            uint32_t *__restrict__ total) {
+// One block per digit: hist[d][*] (rows of `tiles` entries) becomes its exclusive
+// prefix over the tiles that hold keys, and total[d] the digit's count.
+__global__ __launch_bounds__(kSortThreads) void k_sort_scan(uint32_t *__restrict__ hist, int64_t tiles, int64_t n_cap,
+                                                            const int32_t *n_dev, uint32_t *__restrict__ total) {
     __shared__ uint32_t sh[kSortThreads];
     uint32_t *row = hist + (int64_t)blockIdx.x * tiles;
-    const int64_t per = (tiles + kSortThreads - 1) / kSortThreads;
-    const int64_t lo = min(tiles, (int64_t)threadIdx.x * per), hi = min(tiles, lo + per);
+    const int64_t used = (sort_count(n_cap, n_dev) + kSortTile - 1) / kSortTile;
+    const int64_t per = (used + kSortThreads - 1) / kSortThreads;
+    const int64_t lo = min(used, (int64_t)threadIdx.x * per), hi = min(used, lo + per);
     uint32_t s = 0u;
     for (int64_t t = lo; t < hi; ++t) s += row[t];
     uint32_t all;
@@ -172,11 +184,14 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scan(uint32_t *__restrict
 // One pass's stable scatter. vals_in null: the value is the key's own index (first
 // pass); keys_out null: only the values are written (last pass).
 __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const uint32_t *__restrict__ keys_in,
-                                                               const uint32_t *__restrict__ vals_in, int64_t n,
-                                                               int shift, const uint32_t *__restrict__ prefix,
+                                                               const uint32_t *__restrict__ vals_in, int64_t n_cap,
+                                                               const int32_t *n_dev, int shift,
+                                                               const uint32_t *__restrict__ prefix,
                                                                const uint32_t *__restrict__ total, int64_t tiles,
                                                                uint32_t *__restrict__ keys_out,
                                                                uint32_t *__restrict__ vals_out) {
+    const int64_t n = sort_count(n_cap, n_dev);
+    if ((int64_t)blockIdx.x * kSortTile >= n) return;
     __shared__ uint32_t s_keys[kSortTile];
     __shared__ uint32_t s_vals[kSortTile];
     __shared__ uint32_t wcnt[kSortWaves][256];   // per wave: running, then total count of each digit
@@ -267,13 +282,8 @@ hipError_t reorder_workspace_bytes(int64_t n, size_t *bytes, size_t *sort_bytes)
     return hipSuccess;
 }
 
-hipError_t reorder_rays(const tt_ray *rays, int64_t n, const float lo[3], const float hi[3], void *ws,
-                        size_t sort_bytes, hipStream_t s, const uint32_t **order) {
-    (void)sort_bytes;
-    if (n <= 0) {
-        *order = nullptr;
-        return hipSuccess;
-    }
+static hipError_t sort_rays(const tt_ray *rays, int64_t n, const int32_t *n_dev, const float lo[3], const float hi[3],
+                            void *ws, hipStream_t s, const uint32_t **order) {
     const size_t arr = align256((size_t)n * 4);
     const int64_t tiles = (n + kSortTile - 1) / kSortTile;
     char *p = static_cast<char *>(ws);
@@ -287,21 +297,40 @@ hipError_t reorder_rays(const tt_ray *rays, int64_t n, const float lo[3], const 
         scale[a] = ext > 0.0f ? 256.0f / ext : 0.0f;
     }
     const dim3 grid((unsigned)tiles), block(kSortThreads);
-    hipLaunchKernelGGL(k_sort_keys, grid, block, 0, s, rays, n, lo[0], lo[1], lo[2], scale[0], scale[1], scale[2], k0,
-                       hist, tiles);
+    hipLaunchKernelGGL(k_sort_keys, grid, block, 0, s, rays, n, n_dev, lo[0], lo[1], lo[2], scale[0], scale[1],
+                       scale[2], k0, hist, tiles);
     // pass 0: k0 -> (k1, v1); pass 1: (k1, v1) -> (k0, v0); pass 2: (k0, v0) -> v1
     const uint32_t *kin[3] = {k0, k1, k0};
     const uint32_t *vin[3] = {nullptr, v1, v0};
     uint32_t *kout[3] = {k1, k0, nullptr};
     uint32_t *vout[3] = {v1, v0, v1};
     for (int pass = 0; pass < 3; ++pass) {
-        if (pass > 0) hipLaunchKernelGGL(k_sort_hist, grid, block, 0, s, kin[pass], n, 8 * pass, hist, tiles);
-        hipLaunchKernelGGL(k_sort_scan, dim3(256), block, 0, s, hist, tiles, total);
-        hipLaunchKernelGGL(k_sort_scatter, grid, block, 0, s, kin[pass], vin[pass], n, 8 * pass,
+        if (pass > 0) hipLaunchKernelGGL(k_sort_hist, grid, block, 0, s, kin[pass], n, n_dev, 8 * pass, hist, tiles);
+        hipLaunchKernelGGL(k_sort_scan, dim3(256), block, 0, s, hist, tiles, n, n_dev, total);
+        hipLaunchKernelGGL(k_sort_scatter, grid, block, 0, s, kin[pass], vin[pass], n, n_dev, 8 * pass,
                            (const uint32_t *)hist, (const uint32_t *)total, tiles, kout[pass], vout[pass]);
     }
     *order = v1;
     return hipGetLastError();
+}
+
+hipError_t reorder_rays(const tt_ray *rays, int64_t n, const float lo[3], const float hi[3], void *ws,
+                        size_t sort_bytes, hipStream_t s, const uint32_t **order) {
+    (void)sort_bytes;
+    if (n <= 0) {
+        *order = nullptr;
+        return hipSuccess;
+    }
+    return sort_rays(rays, n, nullptr, lo, hi, ws, s, order);
+}
+
+hipError_t reorder_rays_dev(const tt_ray *rays, int64_t cap, const int32_t *n_dev, const float lo[3],
+                            const float hi[3], void *ws, hipStream_t s, const uint32_t **order) {
+    if (cap <= 0) {
+        *order = nullptr;
+        return hipSuccess;
+    }
+    return sort_rays(rays, cap, n_dev, lo, hi, ws, s, order);
 }
 
 }  // namespace tt
