@@ -387,63 +387,112 @@ __global__ __launch_bounds__(64 * kBwdWaves) void k_mlp_bwd(BwdArgs a) {
     }
 }
 
-// loss = sum_r s_r * (-log q(w_r)) / sum_r s_r; writes d loss / d raw (bf16, 48 wide + pad to 64).
+// loss = sum_r s_r * (-log q(w_r)) / sum_r s_r. Writes s_r d(-log q(w_r)) / d raw (bf16, 48
+// wide + pad to 64), the row's weighted gradient before the division by sum_r s_r, which
+// the reduction of the weight gradients applies, and per block the sums of
+// s_r (-log q(w_r)) and of s_r, which k_loss_parts adds.
 // With pb (the BSDF density of each row's direction), output kGuideAlphaOut also gets
 // the gradient of sum_r s_r * (-log(alpha q + (1 - alpha) p_b)) / sum_r s_r, the
 // cross-entropy of the one-sample MIS density, through alpha = learned_alpha(z): the
 // selection probability falls where the mixture describes the incident radiance worse
 // than the BSDF density does (Mueller et al. 2019, "Neural Importance Sampling").
-__global__ __launch_bounds__(256) void k_loss_grad(const float *__restrict__ raw, const float *__restrict__ dirs,
+// One thread per row; the kernel is bound by its VALU work, so exp, log, square root
+// and reciprocal use the hardware approximations (about 1e-6 relative error, far below
+// the BF16 rounding of the gradient), the log of each lobe's density is taken in the
+// form log w + log kappa - log(2 pi (1 - e^{-2 kappa})) + kappa (mu.w - 1) with log kappa
+// the clamped output itself, and only expm1 (in d/d kappa of the normaliser, where
+// 1/kappa - 2/(e^{2 kappa} - 1) cancels for small kappa) stays the accurate function.
+// A block's 256 rows of outputs are loaded into LDS with coalesced 16-byte loads and its
+// 256 gradient rows leave the same way (a thread reading or writing its own row alone
+// touches a different cache line in every lane).
+constexpr int kLossRows = 256;
+constexpr int kLossLdRaw = kGuideNOutPad16 + 1;   // floats per staged output row (odd: no bank conflicts)
+constexpr int kLossLdGrad = kGuideNOutPad + 8;     // bf16 per staged gradient row (16-byte rows)
+static_assert(kLossRows * kLossLdGrad * 2 <= kLossRows * kLossLdRaw * 4, "gradient rows reuse the output rows' LDS");
+__global__ __launch_bounds__(kLossRows) void k_loss_grad(const float *__restrict__ raw, const float *__restrict__ dirs,
                                                    const float *__restrict__ wts, const float *__restrict__ pb,
                                                    __bf16 *__restrict__ draw, float *__restrict__ loss_part,
-                                                   const float *__restrict__ wsum, int64_t n) {
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    __shared__ float red[256];
-    float contrib = 0.0f;
+                                                   float *__restrict__ w_part, int64_t n) {
+    const int64_t r0 = (int64_t)blockIdx.x * kLossRows;
+    const int64_t r = r0 + threadIdx.x;
+    __shared__ float red[kLossRows], redw[kLossRows];
+    __shared__ __align__(16) float stage[kLossRows * kLossLdRaw];
+    float contrib = 0.0f, wr = 0.0f;
     const int64_t rows_pad = (n + 15) & ~15LL;
+    {
+        constexpr int V = kGuideNOutPad16 / 4;   // 16-byte pieces per output row
+        const float4 *src = reinterpret_cast<const float4 *>(raw + r0 * kGuideNOutPad16);
+        for (int f = threadIdx.x; f < kLossRows * V; f += kLossRows) {
+            const int row = f / V, c = 4 * (f % V);
+            const float4 x = r0 + row < n ? src[f] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            float *d = stage + row * kLossLdRaw + c;
+            d[0] = x.x;
+            d[1] = x.y;
+            d[2] = x.z;
+            d[3] = x.w;
+        }
+    }
+    __syncthreads();
     // the row's 64 gradient columns (zero where nothing is written) go out as 8 x 16 B
     __bf16 gv[kGuideNOutPad];
 #pragma unroll
     for (int c = 0; c < kGuideNOutPad; ++c) gv[c] = (__bf16)0.0f;
     if (r < n) {
-        const float *rr = raw + r * kGuideNOutPad16;
-        Lobes L;
-        decode_lobes(rr, L);
+        const float *rr = stage + threadIdx.x * kLossLdRaw;
         const float wx = dirs[3 * r], wy = dirs[3 * r + 1], wz = dirs[3 * r + 2];
-        const float scale = wts[r] / fmaxf(*wsum, 1e-30f);
-        float lv[kGuideLobes], mx = -1e30f;
+        const float scale = wts[r];
+        wr = scale;
+        // mixture weights: softmax of the logits
+        float mxl = -1e30f;
+#pragma unroll
+        for (int k = 0; k < kGuideLobes; ++k) mxl = fmaxf(mxl, rr[kGuideLobes * 4 + k]);
+        float wk[kGuideLobes], wsum_k = 0.0f;
 #pragma unroll
         for (int k = 0; k < kGuideLobes; ++k) {
-            float ca = L.mu[k][0] * wx + L.mu[k][1] * wy + L.mu[k][2] * wz;
-            lv[k] = logf(L.w[k]) + vmf_log(L.kappa[k], ca);
+            wk[k] = __expf(rr[kGuideLobes * 4 + k] - mxl);
+            wsum_k += wk[k];
+        }
+        const float lws = __logf(wsum_k), iws = __builtin_amdgcn_rcpf(wsum_k);
+        float lv[kGuideLobes], kap[kGuideLobes], ca[kGuideLobes], il[kGuideLobes], mu[kGuideLobes][3];
+        float mx = -1e30f;
+#pragma unroll
+        for (int k = 0; k < kGuideLobes; ++k) {
+            const float m0 = rr[3 * k], m1 = rr[3 * k + 1], m2 = rr[3 * k + 2];
+            il[k] = __builtin_amdgcn_rcpf(__builtin_amdgcn_sqrtf(m0 * m0 + m1 * m1 + m2 * m2) + 1e-6f);
+            mu[k][0] = m0 * il[k];
+            mu[k][1] = m1 * il[k];
+            mu[k][2] = m2 * il[k];
+            const float lk = fminf(fmaxf(rr[kGuideLobes * 3 + k], kGuideLogKappaMin), kGuideLogKappaMax);
+            kap[k] = __expf(lk);
+            ca[k] = mu[k][0] * wx + mu[k][1] * wy + mu[k][2] * wz;
+            lv[k] = (rr[kGuideLobes * 4 + k] - mxl - lws) + lk - 1.83787706641f -
+                    __logf(1.0f - __expf(-2.0f * kap[k])) + kap[k] * (ca[k] - 1.0f);
             mx = fmaxf(mx, lv[k]);
         }
         float s = 0.0f;
 #pragma unroll
-        for (int k = 0; k < kGuideLobes; ++k) s += expf(lv[k] - mx);
-        const float logq = mx + logf(s);
+        for (int k = 0; k < kGuideLobes; ++k) s += __expf(lv[k] - mx);
+        const float logq = mx + __logf(s);
         contrib = scale * (-logq);
         const float lk_min = kGuideLogKappaMin, lk_max = kGuideLogKappaMax;
 #pragma unroll
         for (int k = 0; k < kGuideLobes; ++k) {
-            const float resp = expf(lv[k] - logq);
+            const float resp = __expf(lv[k] - logq);
             // logits: d(-log q)/d logit_k = w_k - resp_k
-            gv[kGuideLobes * 4 + k] = (__bf16)(scale * (L.w[k] - resp));
+            gv[kGuideLobes * 4 + k] = (__bf16)(scale * (wk[k] * iws - resp));
             // log kappa: d log v / d kappa = 1/k - 2/(e^{2k} - 1) + (mu.w - 1); d kappa / d lk = kappa
-            const float kap = L.kappa[k];
-            const float ca = L.mu[k][0] * wx + L.mu[k][1] * wy + L.mu[k][2] * wz;
-            const float dlv = 1.0f / kap - 2.0f / expm1f(2.0f * kap) + (ca - 1.0f);
+            const float dlv = __builtin_amdgcn_rcpf(kap[k]) - 2.0f / expm1f(2.0f * kap[k]) + (ca[k] - 1.0f);
             const float lk = rr[kGuideLobes * 3 + k];
-            const float gk = (lk > lk_min && lk < lk_max) ? -scale * resp * dlv * kap : 0.0f;
+            const float gk = (lk > lk_min && lk < lk_max) ? -scale * resp * dlv * kap[k] : 0.0f;
             gv[kGuideLobes * 3 + k] = (__bf16)gk;
             // raw mean m: d log v / d m = kappa (w - (mu.w) mu) / |m|
-            const float f = -scale * resp * kap * L.inv_len[k];
-            gv[3 * k] = (__bf16)(f * (wx - ca * L.mu[k][0]));
-            gv[3 * k + 1] = (__bf16)(f * (wy - ca * L.mu[k][1]));
-            gv[3 * k + 2] = (__bf16)(f * (wz - ca * L.mu[k][2]));
+            const float f = -scale * resp * kap[k] * il[k];
+            gv[3 * k] = (__bf16)(f * (wx - ca[k] * mu[k][0]));
+            gv[3 * k + 1] = (__bf16)(f * (wy - ca[k] * mu[k][1]));
+            gv[3 * k + 2] = (__bf16)(f * (wz - ca[k] * mu[k][2]));
         }
         if (pb) {
-            const float q = expf(logq), p_b = pb[r];
+            const float q = __expf(logq), p_b = pb[r];
             const float sg = sigmoidf_(rr[kGuideAlphaOut]);
             const float al = kAlphaLo + (kAlphaHi - kAlphaLo) * sg;
             const float pc = fmaxf(al * q + (1.0f - al) * p_b, 1e-30f);
@@ -451,109 +500,141 @@ __global__ __launch_bounds__(256) void k_loss_grad(const float *__restrict__ raw
             gv[kGuideAlphaOut] = (__bf16)(isfinite(ga) ? ga : 0.0f);
         }
     }
-    if (r < rows_pad) {
-        bf16x8 *g8 = reinterpret_cast<bf16x8 *>(draw + r * kGuideNOutPad);
+    __syncthreads();   // every row's outputs are read: the staging area takes the gradients
+    __bf16 *gs = reinterpret_cast<__bf16 *>(stage);
 #pragma unroll
-        for (int v = 0; v < kGuideNOutPad / 8; ++v) {
-            bf16x8 w;
+    for (int v = 0; v < kGuideNOutPad / 8; ++v) {
+        bf16x8 w;
 #pragma unroll
-            for (int j = 0; j < 8; ++j) w[j] = gv[8 * v + j];
-            g8[v] = w;
+        for (int j = 0; j < 8; ++j) w[j] = gv[8 * v + j];
+        *reinterpret_cast<bf16x8 *>(gs + threadIdx.x * kLossLdGrad + 8 * v) = w;
+    }
+    __syncthreads();
+    {
+        constexpr int V = kGuideNOutPad / 8;   // 16-byte pieces per gradient row
+        bf16x8 *dst = reinterpret_cast<bf16x8 *>(draw + r0 * kGuideNOutPad);
+        for (int f = threadIdx.x; f < kLossRows * V; f += kLossRows) {
+            const int row = f / V, c = 8 * (f % V);
+            if (r0 + row < rows_pad) dst[f] = *reinterpret_cast<const bf16x8 *>(gs + row * kLossLdGrad + c);
         }
     }
     red[threadIdx.x] = contrib;
+    redw[threadIdx.x] = wr;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        if ((int)threadIdx.x < o) {
+            red[threadIdx.x] += red[threadIdx.x + o];
+            redw[threadIdx.x] += redw[threadIdx.x + o];
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0) loss_part[blockIdx.x] = red[0];
+    if (threadIdx.x == 0) {
+        loss_part[blockIdx.x] = red[0];
+        w_part[blockIdx.x] = redw[0];
+    }
 }
 
-// Per-block sums of v (grid-stride), one partial per block; k_sum_parts adds them.
-__global__ __launch_bounds__(256) void k_sum(const float *__restrict__ v, int64_t n, float *__restrict__ part) {
-    __shared__ float red[256];
-    float s = 0.0f;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        s += v[i];
-    red[threadIdx.x] = s;
+// The loss and the weight sum from k_loss_grad's nb block partials, added in a fixed
+// order by one block (no float atomics), so both are the same bits on every run:
+// scalars[1] = sum_r s_r, scalars[0] = sum_r s_r (-log q(w_r)) / scalars[1].
+__global__ __launch_bounds__(256) void k_loss_parts(const float *__restrict__ loss_part,
+                                                    const float *__restrict__ w_part, int64_t nb,
+                                                    float *__restrict__ scalars) {
+    __shared__ float red[256], redw[256];
+    float sl = 0.0f, sw = 0.0f;
+    for (int64_t i = threadIdx.x; i < nb; i += 256) {
+        sl += loss_part[i];
+        sw += w_part[i];
+    }
+    red[threadIdx.x] = sl;
+    redw[threadIdx.x] = sw;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        if ((int)threadIdx.x < o) {
+            red[threadIdx.x] += red[threadIdx.x + o];
+            redw[threadIdx.x] += redw[threadIdx.x + o];
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
-}
-
-// *out = the sum of n partials in a fixed order (one block): the loss and the weight
-// sum, which scales every gradient, are the same bits on every run.
-__global__ __launch_bounds__(256) void k_sum_parts(const float *__restrict__ part, int64_t n, float *__restrict__ out) {
-    __shared__ float red[256];
-    float s = 0.0f;
-    for (int64_t i = threadIdx.x; i < n; i += 256) s += part[i];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
+    if (threadIdx.x == 0) {
+        scalars[1] = redw[0];
+        scalars[0] = red[0] / fmaxf(redw[0], 1e-30f);
     }
-    if (threadIdx.x == 0) *out = red[0];
 }
 
 // ------------------------------------------------------------------ weight gradients
 // partial[blk][i][o] = sum over the block's rows r of H[r][i] G[r][o]  (KI x KO) and
 // pbias[blk][o] = sum over the block's rows of G[r][o]. Rows are staged kWgRows at
 // a time in LDS, transposed ([feature][row]) so that each lane's 8 operand values
-// along the row (reduction) dimension are 16 contiguous bytes: every thread loads
-// 8 rows x 8 features with 16-byte loads, transposes the 8 x 8 block in registers
-// and writes 8 16-byte rows.
-constexpr int kWgRows = 64;
+// along the row (reduction) dimension are 16 contiguous bytes. A staging task moves
+// 8 rows x 8 features: eight 16-byte loads into registers, an 8 x 8 transpose in
+// registers, eight 16-byte LDS rows. The tasks of H and G are spread over all of the
+// block's threads, and the next chunk's loads are issued before the current chunk's
+// products, so that they are in flight while the matrix cores work. The bias sums
+// are products too: an all-ones A operand against a tile of G gives G's column sums
+// in every row of the result.
+constexpr int kWgRows = 128;
+constexpr int kWgThreads = 256;
 
-template <int KF>
-__device__ __forceinline__ void stage_transposed(__bf16 (*dst)[kWgRows + 8], const __bf16 *__restrict__ src, int ld,
-                                                 int64_t rb, int64_t r1) {
-    constexpr int FG = KF / 8, TASKS = (kWgRows / 8) * FG;
-    for (int task = threadIdx.x; task < TASKS; task += blockDim.x) {
-        const int rg = task / FG, fg = task % FG;
-        bf16x8 v[8];
+template <int KI, int KO, bool ENC>
+struct WgStage {
+    static constexpr int RG = kWgRows / 8;                 // row groups of a chunk
+    static constexpr int TH = ENC ? 0 : RG * (KI / 8);     // H tasks (none when the features are computed)
+    static constexpr int TG = RG * (KO / 8);               // G tasks
+    static constexpr int NT = (TH + TG + kWgThreads - 1) / kWgThreads;   // tasks per thread
+};
+
+// Rows 8 rg .. 8 rg + 7 of the chunk at rb, features 8 fg .. 8 fg + 7 (zero at or past r1).
+__device__ __forceinline__ void wg_load(bf16x8 (&v)[8], const __bf16 *__restrict__ src, int ld, int rg, int fg,
+                                        int64_t rb, int64_t r1) {
 #pragma unroll
-        for (int rr = 0; rr < 8; ++rr) {
-            const int64_t row = rb + 8 * rg + rr;
-            if (row < r1) {
-                v[rr] = *reinterpret_cast<const bf16x8 *>(src + row * ld + 8 * fg);
-            } else {
+    for (int rr = 0; rr < 8; ++rr) {
+        const int64_t row = rb + 8 * rg + rr;
+        if (row < r1) {
+            v[rr] = *reinterpret_cast<const bf16x8 *>(src + row * ld + 8 * fg);
+        } else {
 #pragma unroll
-                for (int q = 0; q < 8; ++q) v[rr][q] = (__bf16)0.0f;
-            }
+            for (int q = 0; q < 8; ++q) v[rr][q] = (__bf16)0.0f;
         }
+    }
+}
+
+__device__ __forceinline__ void wg_store(__bf16 (*dst)[kWgRows + 8], const bf16x8 (&v)[8], int rg, int fg) {
 #pragma unroll
-        for (int ff = 0; ff < 8; ++ff) {
-            bf16x8 w;
+    for (int ff = 0; ff < 8; ++ff) {
+        bf16x8 w;
 #pragma unroll
-            for (int rr = 0; rr < 8; ++rr) w[rr] = v[rr][ff];
-            *reinterpret_cast<bf16x8 *>(&dst[8 * fg + ff][8 * rg]) = w;
-        }
+        for (int rr = 0; rr < 8; ++rr) w[rr] = v[rr][ff];
+        *reinterpret_cast<bf16x8 *>(&dst[8 * fg + ff][8 * rg]) = w;
     }
 }
 
 // The first layer's input features of rows [rb, rb + kWgRows) straight from the
 // inputs, [feature][row] in LDS: the encoding of k_encode / k_mlp_infer (the same
-// bits), zero for rows at or past `valid` (the batch's rows, or the block's end).
+// bits), zero for rows at or past `valid` (the batch's rows, or the block's end). A
+// task is one input of two adjacent rows, rows fastest across the threads, and writes
+// each feature of the pair as one 4-byte word. The padding features are not written.
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ void stage_features(__bf16 (*dst)[kWgRows + 8], const float *__restrict__ in, int64_t rb,
                                                int64_t valid) {
-    for (int task = threadIdx.x; task < kWgRows * (kGuideIn + 1); task += blockDim.x) {
-        const int rr = task / (kGuideIn + 1), c = task % (kGuideIn + 1);
-        const int64_t row = rb + rr;
-        if (c == kGuideIn) {
-            for (int k = kGuideFeat; k < kGuideK1; ++k) dst[k][rr] = (__bf16)0.0f;
-            continue;
-        }
-        const bool ok = row < valid;
-        FreqState st = freq_base(ok ? in[row * kGuideIn + c] : 0.0f);
+    constexpr int PAIRS = kWgRows / 2;
+    for (int task = threadIdx.x; task < PAIRS * kGuideIn; task += blockDim.x) {
+        const int rp = task % PAIRS, c = task / PAIRS;
+        const int64_t row = rb + 2 * rp;
+        const bool ok0 = row < valid, ok1 = row + 1 < valid;
+        FreqState a = freq_base(ok0 ? in[row * kGuideIn + c] : 0.0f);
+        FreqState b = freq_base(ok1 ? in[(row + 1) * kGuideIn + c] : 0.0f);
         for (int f = 0; f < kGuideFreq; ++f) {
-            dst[2 * kGuideFreq * c + 2 * f][rr] = (__bf16)(ok ? st.s : 0.0f);
-            dst[2 * kGuideFreq * c + 2 * f + 1][rr] = (__bf16)(ok ? st.c : 0.0f);
-            freq_next(st);
+            const int k = 2 * kGuideFreq * c + 2 * f;
+            bf16x2 sv, cv;
+            sv[0] = (__bf16)(ok0 ? a.s : 0.0f);
+            sv[1] = (__bf16)(ok1 ? b.s : 0.0f);
+            cv[0] = (__bf16)(ok0 ? a.c : 0.0f);
+            cv[1] = (__bf16)(ok1 ? b.c : 0.0f);
+            *reinterpret_cast<bf16x2 *>(&dst[k][2 * rp]) = sv;
+            *reinterpret_cast<bf16x2 *>(&dst[k + 1][2 * rp]) = cv;
+            freq_next(a);
+            freq_next(b);
         }
     }
 }
@@ -561,30 +642,53 @@ __device__ __forceinline__ void stage_features(__bf16 (*dst)[kWgRows + 8], const
 // ENC (first layer only): H is not read; the features come from enc_in (rows below
 // enc_rows) through stage_features.
 template <int KI, int KO, bool ENC = false>
-__global__ __launch_bounds__(256) void k_wgrad(const __bf16 *__restrict__ H, int ldh, const __bf16 *__restrict__ G,
-                                               int ldg, int64_t n, int64_t rows_per_block, float *__restrict__ partial,
-                                               float *__restrict__ pbias, const float *__restrict__ enc_in = nullptr,
-                                               int64_t enc_rows = 0) {
+__global__ __launch_bounds__(kWgThreads) void k_wgrad(const __bf16 *__restrict__ H, int ldh,
+                                                      const __bf16 *__restrict__ G, int ldg, int64_t n,
+                                                      int64_t rows_per_block, float *__restrict__ partial,
+                                                      float *__restrict__ pbias, const float *__restrict__ enc_in = nullptr,
+                                                      int64_t enc_rows = 0) {
+    using S = WgStage<KI, KO, ENC>;
     constexpr int TI = KI / 16, TO = KO / 16, TILES = TI * TO;
-    constexpr int PER_WAVE = (TILES + 3) / 4;
+    constexpr int PER_WAVE = (TILES + TO + 3) / 4;   // the weight tiles, then one bias tile per output tile
     __shared__ __align__(16) __bf16 hs[KI][kWgRows + 8];  // [feature][row]
     __shared__ __align__(16) __bf16 gs[KO][kWgRows + 8];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lr = lane & 15, lq = lane >> 4;
     const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
     const int64_t r1 = min(n, r0 + rows_per_block);
+    if (ENC) {
+        // the padding features stay zero for every chunk
+        for (int i = threadIdx.x; i < (KI - kGuideFeat) * (kWgRows + 8); i += kWgThreads)
+            hs[kGuideFeat + i / (kWgRows + 8)][i % (kWgRows + 8)] = (__bf16)0.0f;
+    }
+    bf16x8 ones;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) ones[q] = (__bf16)1.0f;
     f32x4 acc[PER_WAVE];
 #pragma unroll
     for (int q = 0; q < PER_WAVE; ++q) acc[q] = f32x4{0.f, 0.f, 0.f, 0.f};
-    float bsum = 0.0f;
+
+    bf16x8 v[S::NT][8];
+    // the thread's staging tasks of the chunk at rb, into registers
+#define TT_WG_LOAD(rb_)                                                                            \
+    _Pragma("unroll") for (int k = 0; k < S::NT; ++k) {                                            \
+        const int t = threadIdx.x + k * kWgThreads;                                                \
+        if (t < S::TH) wg_load(v[k], H, ldh, t / (KI / 8), t % (KI / 8), (rb_), r1);               \
+        else if (t < S::TH + S::TG)                                                                \
+            wg_load(v[k], G, ldg, (t - S::TH) / (KO / 8), (t - S::TH) % (KO / 8), (rb_), r1);       \
+    }
+    if (r0 < r1) TT_WG_LOAD(r0)
     for (int64_t rb = r0; rb < r1; rb += kWgRows) {
-        __syncthreads();
+        __syncthreads();   // the previous chunk's products have read hs and gs
+#pragma unroll
+        for (int k = 0; k < S::NT; ++k) {
+            const int t = threadIdx.x + k * kWgThreads;
+            if (t < S::TH) wg_store(hs, v[k], t / (KI / 8), t % (KI / 8));
+            else if (t < S::TH + S::TG) wg_store(gs, v[k], (t - S::TH) / (KO / 8), (t - S::TH) % (KO / 8));
+        }
         if (ENC) stage_features(hs, enc_in, rb, min(r1, enc_rows));
-        else stage_transposed<KI>(hs, H, ldh, rb, r1);
-        stage_transposed<KO>(gs, G, ldg, rb, r1);
         __syncthreads();
-        if (threadIdx.x < KO)
-            for (int rr = 0; rr < kWgRows; ++rr) bsum += (float)gs[threadIdx.x][rr];
+        if (rb + kWgRows < r1) TT_WG_LOAD(rb + kWgRows)
 #pragma unroll
         for (int q = 0; q < PER_WAVE; ++q) {
             const int tile = wave * PER_WAVE + q;
@@ -596,9 +700,17 @@ __global__ __launch_bounds__(256) void k_wgrad(const __bf16 *__restrict__ H, int
                     const bf16x8 b = *reinterpret_cast<const bf16x8 *>(&gs[16 * to + lr][32 * s2 + 8 * lq]);
                     acc[q] = mfma(a, b, acc[q]);
                 }
+            } else if (tile < TILES + TO) {
+                const int to = tile - TILES;
+#pragma unroll
+                for (int s2 = 0; s2 < kWgRows / 32; ++s2) {
+                    const bf16x8 b = *reinterpret_cast<const bf16x8 *>(&gs[16 * to + lr][32 * s2 + 8 * lq]);
+                    acc[q] = mfma(ones, b, acc[q]);
+                }
             }
         }
     }
+#undef TT_WG_LOAD
     float *out = partial + (int64_t)blockIdx.x * KI * KO;
 #pragma unroll
     for (int q = 0; q < PER_WAVE; ++q) {
@@ -607,9 +719,10 @@ __global__ __launch_bounds__(256) void k_wgrad(const __bf16 *__restrict__ H, int
             const int ti = tile / TO, to = tile % TO;
 #pragma unroll
             for (int j = 0; j < 4; ++j) out[(16 * ti + 4 * lq + j) * KO + 16 * to + lr] = acc[q][j];
+        } else if (tile < TILES + TO && lq == 0) {
+            pbias[(int64_t)blockIdx.x * KO + 16 * (tile - TILES) + lr] = acc[q][0];
         }
     }
-    if (threadIdx.x < KO) pbias[(int64_t)blockIdx.x * KO + threadIdx.x] = bsum;
 }
 
 // Sum of the nb per-block partial gradients of every parameter, in a fixed order
@@ -617,8 +730,9 @@ __global__ __launch_bounds__(256) void k_wgrad(const __bf16 *__restrict__ H, int
 // weights [in][out] and biases [out], region r at partial + start[r], block b's values
 // at + b * count[r]). A block takes 64 elements of the concatenated regions, its
 // kRedParts waves each sum every kRedParts-th partial (unrolled, so several loads are
-// in flight per lane), and the first wave adds the part sums in part order; weights
-// are transposed to the [out][in] master layout.
+// in flight per lane), and the first wave adds the part sums in part order, divides by
+// the training weights' sum (*wsum: the rows' output gradients are weighted by s_r
+// alone) and transposes weights to the [out][in] master layout.
 constexpr int kRedParts = 16;
 struct RedRegions {
     int64_t start[8];
@@ -627,7 +741,8 @@ struct RedRegions {
     int transpose[8];
 };
 __global__ __launch_bounds__(64 * kRedParts) void k_reduce_all(const float *__restrict__ partial, int nb, int total,
-                                                               RedRegions R, float *__restrict__ grad) {
+                                                               RedRegions R, const float *__restrict__ wsum,
+                                                               float *__restrict__ grad) {
     __shared__ float red[kRedParts][64];
     const int el = threadIdx.x & 63, part = threadIdx.x >> 6;
     const int e = blockIdx.x * 64 + el;
@@ -654,7 +769,7 @@ __global__ __launch_bounds__(64 * kRedParts) void k_reduce_all(const float *__re
             const int i = le / R.ko[r], o = le % R.ko[r];
             d = o * R.ki[r] + i;
         }
-        grad[R.dst[r] + d] = t;
+        grad[R.dst[r] + d] = t / fmaxf(*wsum, 1e-30f);
     }
 }
 
@@ -744,7 +859,7 @@ tt_status guide_alloc_batch(tt_guide *g, int64_t rows) {
     al((void **)&b.dirs, (size_t)R * 3 * 4, "guide directions");
     al((void **)&b.wts, (size_t)R * 4, "guide weights");
     al((void **)&b.pb, (size_t)R * 4, "guide BSDF densities");
-    al((void **)&b.lpart, (size_t)((R + 255) / 256) * 4, "guide loss partials");
+    al((void **)&b.lpart, (size_t)2 * ((R + 255) / 256) * 4, "guide loss partials");
     if (st == TT_OK) g->cap = R;
     return st;
 }
@@ -842,15 +957,14 @@ static hipError_t guide_backward(tt_guide *g, int64_t n, hipStream_t s, bool fus
     const int64_t R = (n + 15) & ~15LL;
     hipError_t e = fused ? guide_infer(g, b.in, b.raw, n, nullptr, s, nullptr, true) : guide_forward(g, n, nullptr, s);
     if (e != hipSuccess) return e;
-    // weight sum and loss: per-block partials added in a fixed order (no float atomics)
-    hipLaunchKernelGGL(k_sum, dim3(kGuideSumParts), dim3(256), 0, s, b.wts, n, g->scalars + kGuideSumParts);
-    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, s, (const float *)(g->scalars + kGuideSumParts),
-                       (int64_t)kGuideSumParts, g->scalars + 1);
+    // loss and weight sum: per-block partials added in a fixed order (no float atomics);
+    // the weight-sum division of the gradient happens in k_reduce_all
     const unsigned lb = (unsigned)((R + 255) / 256);
     hipLaunchKernelGGL(k_loss_grad, dim3(lb), dim3(256), 0, s, b.raw, b.dirs, b.wts,
                        g->train_alpha ? (const float *)b.pb : (const float *)nullptr, b.draw, b.lpart,
-                       (const float *)(g->scalars + 1), n);
-    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, s, (const float *)b.lpart, (int64_t)lb, g->scalars);
+                       b.lpart + lb, n);
+    hipLaunchKernelGGL(k_loss_parts, dim3(1), dim3(256), 0, s, (const float *)b.lpart, (const float *)(b.lpart + lb),
+                       (int64_t)lb, g->scalars);
     // dH3 = dRaw W4 (mask H3 > 0), dH2 = dH3 W3 (mask), dH1 = dH2 W2 (mask)
     if (fused) {
         BwdArgs ba{};
@@ -881,7 +995,7 @@ static hipError_t guide_backward(tt_guide *g, int64_t n, hipStream_t s, bool fus
     }
     // weight gradients: dW_k = H_{k-1}^T dH_k and bias gradients, partial sums per
     // block (at most kWgBlocks blocks of whole kWgRows chunks), reduced in block order
-    constexpr int64_t kWgBlocks = 1024, kWgMinRows = 256;
+    constexpr int64_t kWgBlocks = 512, kWgMinRows = 512;
     const int64_t rpb =
         std::max<int64_t>(kWgMinRows, ((R + kWgBlocks - 1) / kWgBlocks + kWgRows - 1) / kWgRows * kWgRows);
     const int nb = (int)((R + rpb - 1) / rpb);
@@ -940,7 +1054,7 @@ static hipError_t guide_backward(tt_guide *g, int64_t n, hipStream_t s, bool fus
                                R, rpb, pw, pbias, (const float *)nullptr, (int64_t)0);
     }
     hipLaunchKernelGGL(k_reduce_all, dim3((first + 63) / 64), dim3(64 * kRedParts), 0, s, (const float *)g->partial,
-                       nb, first, rg, g->grad);
+                       nb, first, rg, (const float *)(g->scalars + 1), g->grad);
     return hipGetLastError();
 }
 
@@ -996,7 +1110,7 @@ tt_status tt_guide_create(tt_ctx *ctx, uint32_t seed, float lr, tt_guide **out) 
     al((void **)&g->m1, (size_t)np * 4, "guide adam m");
     al((void **)&g->m2, (size_t)np * 4, "guide adam v");
     al((void **)&g->grad, (size_t)np * 4, "guide grad");
-    al((void **)&g->scalars, (size_t)2 * kGuideSumParts * 4, "guide scalars");
+    al((void **)&g->scalars, 2 * 4, "guide scalars");
     for (int k = 0; k < 4; ++k) {
         const GuideLayer &l = kGuideLayers[k];
         al((void **)&g->wf[k], (size_t)l.in * l.out * 2, "guide bf16 weights");

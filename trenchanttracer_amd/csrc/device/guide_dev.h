@@ -21,7 +21,6 @@ constexpr int kGuideAlphaOut = kGuideNOut;
 constexpr float kAlphaLo = 0.02f, kAlphaHi = 0.95f;
 constexpr int kGuideNOutPad16 = 48;  // output columns computed (multiple of 16); raw row stride
 constexpr int kGuideNOutPad = 64;    // output-gradient width (multiple of 32, the backward K)
-constexpr int kGuideSumParts = 256;  // blocks of the weight-sum reduction
 constexpr float kGuideLogKappaMin = -5.0f, kGuideLogKappaMax = 9.0f;
 
 struct GuideLayer {
@@ -40,47 +39,13 @@ struct GuideBatch {
     float *dirs = nullptr;    // [R][3] training directions
     float *wts = nullptr;     // [R] training weights (radiance / pdf)
     float *pb = nullptr;      // [R] BSDF density of each direction (learned alpha), or unset
-    float *lpart = nullptr;   // [(R + 255) / 256] per-block loss sums, added in a fixed order
+    float *lpart = nullptr;   // 2 x [(R + 255) / 256]: per-block loss sums, then weight sums (fixed-order adds)
 };
 
 __device__ __forceinline__ float sigmoidf_(float z) { return 1.0f / (1.0f + __expf(-z)); }
 __device__ __forceinline__ float learned_alpha(float z) { return kAlphaLo + (kAlphaHi - kAlphaLo) * sigmoidf_(z); }
 
 // ------------------------------------------------------------------ vMF mixture
-struct Lobes {
-    float mu[kGuideLobes][3];
-    float kappa[kGuideLobes];
-    float w[kGuideLobes];
-    float inv_len[kGuideLobes];
-};
-
-// raw: 40 outputs (mean 3k..3k+2, log kappa 24+k, logit 32+k)
-__device__ __forceinline__ void decode_lobes(const float *raw, Lobes &L) {
-    float mx = -1e30f;
-#pragma unroll
-    for (int k = 0; k < kGuideLobes; ++k) mx = fmaxf(mx, raw[kGuideLobes * 4 + k]);
-    float sum = 0.0f;
-#pragma unroll
-    for (int k = 0; k < kGuideLobes; ++k) {
-        float m0 = raw[3 * k], m1 = raw[3 * k + 1], m2 = raw[3 * k + 2];
-        float len = sqrtf(m0 * m0 + m1 * m1 + m2 * m2) + 1e-6f;
-        L.inv_len[k] = 1.0f / len;
-        L.mu[k][0] = m0 * L.inv_len[k];
-        L.mu[k][1] = m1 * L.inv_len[k];
-        L.mu[k][2] = m2 * L.inv_len[k];
-        L.kappa[k] = expf(fminf(fmaxf(raw[kGuideLobes * 3 + k], kGuideLogKappaMin), kGuideLogKappaMax));
-        L.w[k] = expf(raw[kGuideLobes * 4 + k] - mx);
-        sum += L.w[k];
-    }
-#pragma unroll
-    for (int k = 0; k < kGuideLobes; ++k) L.w[k] /= sum;
-}
-
-// log of the vMF density kappa / (2 pi (1 - e^-2k)) exp(k (mu.w - 1))
-__device__ __forceinline__ float vmf_log(float kappa, float cosang) {
-    return logf(kappa) - 1.83787706641f - log1pf(-expf(-2.0f * kappa)) + kappa * (cosang - 1.0f);
-}
-
 // Render-side form of the mixture: per lobe the unit mean, kappa, the weight (for
 // picking a lobe) and lc = log(weight) + log of the vMF normaliser, so that the
 // density at a unit direction is sum_k exp(lc_k + kappa_k (mu_k . w - 1)) — one exp
@@ -190,7 +155,7 @@ tt_status guide_alloc_infer(tt_guide *g, int64_t rows);
 struct tt_guide {
     tt_ctx *ctx = nullptr;
     float *params = nullptr, *m1 = nullptr, *m2 = nullptr, *grad = nullptr;
-    float *scalars = nullptr;  // [0] loss, [1] sum of weights, [kGuideSumParts ..] per-block weight sums
+    float *scalars = nullptr;  // [0] loss, [1] sum of the training weights
     __bf16 *wf[4] = {nullptr, nullptr, nullptr, nullptr};  // forward copies Bt[out][in]
     __bf16 *wb[4] = {nullptr, nullptr, nullptr, nullptr};  // backward copies W[in][out_pad]
     float *partial = nullptr;

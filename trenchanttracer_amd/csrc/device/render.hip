@@ -40,9 +40,10 @@ struct RenderState {
     int32_t *sample_dev = nullptr;            // first sample index of the current pass
     void *table = nullptr;                    // device SceneTable
     tt_hit *hits = nullptr;
-    float4 *thr = nullptr;       // reference: mask | path: throughput, w = pdf of the last BSDF sample
+    // reference: mask | path: throughput, w = density of the last BSDF sample, or -1 when
+    // that sample was a specular (delta) one
+    float4 *thr = nullptr;
     float4 *L = nullptr;         // reference: path radiance
-    int32_t *flags = nullptr;    // path: bit 0 = last bounce was specular
     tt_ray *srays = nullptr;     // next-event rays
     float4 *scontrib = nullptr;  // contribution (rgb) and pixel (w, as int bits)
     int32_t *socc = nullptr;     // any-hit flags
@@ -228,7 +229,6 @@ struct GenArgs {
     tt_ray *rays;
     int32_t *path, *root;
     float4 *thr, *L;
-    int32_t *flags;
     int32_t *counts;
 };
 
@@ -265,10 +265,14 @@ __global__ __launch_bounds__(256) void k_generate(GenArgs a) {
     a.rays[p] = {ap.x, ap.y, ap.z, a.tmin, d.x, d.y, d.z, 1e20f};
     a.path[p] = p;
     a.root[p] = a.tab->root[sid];
-    a.thr[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    // reference mode: path radiance; path mode: L.x = Russian-roulette weight
-    if (a.L) a.L[p] = make_float4(a.mode == TT_RENDER_PATH ? 1.0f : 0.0f, 0.0f, 0.0f, 0.0f);
-    if (a.flags) a.flags[p] = 0;
+    // reference mode: mask and path radiance. Path mode starts every path from
+    // throughput 1 and roulette weight 1, which its camera-bounce shading takes as
+    // constants (every generated path is in that bounce's queue), so nothing is
+    // written here.
+    if (a.mode != TT_RENDER_PATH) {
+        a.thr[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+        if (a.L) a.L[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
 }
 
 __global__ void k_advance_sample(int32_t *sample_dev, int32_t k) { *sample_dev += k; }
@@ -473,7 +477,6 @@ struct ShadeArgs {
     int32_t *root_out, *sroot;
     const tt_hit *hits;
     float4 *thr, *L;
-    int32_t *flags;
     int32_t *counts;
     int32_t cur, nxt;
     tt_ray *srays;
@@ -1025,11 +1028,13 @@ __device__ __forceinline__ void shade_path_one(const ShadeArgs &a, const int32_t
         const uint32_t smp = (uint32_t)*a.sample_dev + (uint32_t)path_sample(p, a.k);
         const tt_ray r = a.rays_in[i];
         const tt_hit h = a.hits[i];
-        const float4 tp = a.thr[p];
+        // the camera bounce's state is the generated one: throughput 1, roulette weight 1
+        const bool first = a.bounce == 0;
+        const float4 tp = first ? make_float4(1.0f, 1.0f, 1.0f, 0.0f) : a.thr[p];
         f3 thr = xyz(tp);
         const float last_pdf = tp.w;
-        const bool last_delta = (a.flags[p] & 1) != 0;
-        float4 Ls = a.L[p];   // x: roulette weight, yzw: the path's radiance so far
+        const bool last_delta = tp.w < 0.0f;
+        float4 Ls = first ? make_float4(1.0f, 0.0f, 0.0f, 0.0f) : a.L[p];   // x: roulette weight, yzw: radiance
         const f3 ro = F3(r.ox, r.oy, r.oz), rd = F3(r.dx, r.dy, r.dz);
         f3 add = F3(0, 0, 0);
         if (h.tri_id == -1) {
@@ -1188,9 +1193,8 @@ __device__ __forceinline__ void shade_path_one(const ShadeArgs &a, const int32_t
                             f3 no = point + (side > 0.0f ? ngf : ngf * -1.0f) * eps;
                             nr = {no.x, no.y, no.z, 0.0f, wi.x, wi.y, wi.z, 1e20f};
                             cont = true;
-                            a.thr[p] = make_float4(thr.x, thr.y, thr.z, pdf_b);
+                            a.thr[p] = make_float4(thr.x, thr.y, thr.z, delta ? -1.0f : pdf_b);
                             Ls.x = rr;
-                            a.flags[p] = delta ? 1 : 0;
                             if (rec_b && !delta && pdf_b > 0.0f) {
                                 guide_inputs(a.gd, point, ns, wo, rec_b);
                                 rec_b[9] = wi.x;
@@ -1622,7 +1626,6 @@ tt_status ensure_state(tt_scene *s, int32_t cap, int32_t npix, bool fp64) {
     al((void **)&rs->hits, (size_t)cap * sizeof(tt_hit), "render hits");
     al((void **)&rs->thr, (size_t)cap * 16, "render throughput");
     al((void **)&rs->L, (size_t)cap * 16, "render radiance");
-    al((void **)&rs->flags, (size_t)cap * 4, "render flags");
     al((void **)&rs->srays, (size_t)cap * sizeof(tt_ray), "render shadow rays");
     al((void **)&rs->sroot, (size_t)cap * 4, "render shadow roots");
     al((void **)&rs->scontrib, (size_t)cap * 16, "render shadow contributions");
@@ -1712,7 +1715,6 @@ hipError_t launch_bounce(tt_scene *s, RenderState *rs, const tt_render_params *p
     sa.hits = rs->hits;
     sa.thr = rs->thr;
     sa.L = rs->L;
-    sa.flags = rs->flags;
     sa.counts = rs->counts;
     sa.cur = cur;
     sa.nxt = nxt;
@@ -1777,7 +1779,6 @@ hipError_t launch_pass(tt_scene *s, RenderState *rs, const tt_render_params *p, 
     g.root = rs->root[0];
     g.thr = rs->thr;
     g.L = rs->L;
-    g.flags = rs->flags;
     g.counts = rs->counts;
     hipLaunchKernelGGL(k_generate, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, hs, g);
     hipError_t e = hipGetLastError();
@@ -1946,9 +1947,9 @@ tt_status render_impl(tt_scene *s, int32_t nsc, const tt_camera *cams, const int
     // k samples per pixel are in flight in one pass (each path's radiance is kept in
     // its own slot and the pass ends by adding a pixel's samples in sample order, in
     // both modes). Each bounce's launches last as long as their slowest waves, so the
-    // fewer the passes the less that tail costs: by default up to 2^27 paths (188 B of state each, 25 GB)
-    // are in flight, guided renders with a frozen network included (228 B more per
-    // path for the network's inputs and mixture). Renders that train the network
+    // fewer the passes the less that tail costs: by default up to 2^27 paths (184 B of
+    // state each, 24.7 GB) are in flight, guided renders with a frozen network included
+    // (228 B more per path for the network's inputs and mixture). Renders that train the network
     // default to about 8M paths: it learns between passes.
     int32_t k = 1;
     {
@@ -2093,7 +2094,7 @@ void free_render_state(tt_scene *s) {
     RenderState *rs = s->rs;
     if (!rs) return;
     void *ptrs[] = {rs->rays[0], rs->rays[1], rs->path[0], rs->path[1], rs->root[0], rs->root[1], rs->hits, rs->thr,
-                    rs->L, rs->flags, rs->srays, rs->sroot, rs->scontrib, rs->socc, rs->counts, rs->stats,
+                    rs->L, rs->srays, rs->sroot, rs->scontrib, rs->socc, rs->counts, rs->stats,
                     rs->sample_dev, rs->table, rs->acc, rs->acc64};
     for (void *q : ptrs)
         if (q) hipFree(q);
