@@ -299,94 +299,6 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(InferArgs a) {
     }
 }
 
-// ------------------------------------------------------------------ fused backward (data)
-// dH3 = (dRaw W4) * (H3 > 0), dH2 = (dH3 W3) * (H2 > 0), dH1 = (dH2 W2) * (H1 > 0) for
-// every 16-row tile of the batch in one kernel: the three backward weight matrices
-// (W[in][out_pad] copies, the Bt operand of these products) are staged in LDS once
-// per block; a wave reads its tile's dRaw rows from global, keeps each product's
-// output in its LDS tile as the next product's A operand, and writes dH3, dH2, dH1
-// (the weight gradients read them). Arithmetic, rounding points and accumulation
-// order are those of the three k_gemm launches it replaces, so the outputs are
-// identical to theirs.
-constexpr int kBwdWaves = 8;
-constexpr int kBwdLds = 2 * (3 * kGuideH * kLdH + kBwdWaves * 16 * kLdH);
-
-struct BwdArgs {
-    const __bf16 *draw;      // [R][64] output gradients
-    const __bf16 *wb[3];     // backward copies of layers 2, 3, 4: [in 64][out_pad 64]
-    const __bf16 *h[3];      // H1, H2, H3 [R][64] (ReLU masks)
-    __bf16 *dh[3];           // dH1, dH2, dH3 [R][64]
-    int64_t rows;            // R, a multiple of 16
-};
-
-// acc (D layout) -> bf16 into the wave's LDS tile; then, row-major with 16 bytes per
-// lane, the ReLU mask of the layer (h > 0) is applied in LDS and the tile written to
-// global rows. The same values as masking before the conversion (a masked value is 0
-// either way).
-__device__ __forceinline__ void store_grad(__bf16 *tile_lds, __bf16 *dst, const __bf16 *mask, const f32x4 *acc,
-                                           int64_t tile, int lane) {
-    const int lr = lane & 15, lq = lane >> 4;
-#pragma unroll
-    for (int t = 0; t < kGuideH / 16; ++t)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) tile_lds[(4 * lq + j) * kLdH + 16 * t + lr] = (__bf16)(acc[t][j] + 0.0f);
-    wave_sync();
-#pragma unroll
-    for (int v = lane; v < 16 * kGuideH / 8; v += 64) {
-        const int rr = v / (kGuideH / 8), c = 8 * (v % (kGuideH / 8));
-        const int64_t row = tile * 16 + rr;
-        bf16x8 x = *reinterpret_cast<const bf16x8 *>(tile_lds + rr * kLdH + c);
-        const bf16x8 m = *reinterpret_cast<const bf16x8 *>(mask + row * kGuideH + c);
-#pragma unroll
-        for (int j = 0; j < 8; ++j)
-            if (!((float)m[j] > 0.0f)) x[j] = (__bf16)0.0f;
-        *reinterpret_cast<bf16x8 *>(tile_lds + rr * kLdH + c) = x;
-        *reinterpret_cast<bf16x8 *>(dst + row * kGuideH + c) = x;
-    }
-}
-
-__global__ __launch_bounds__(64 * kBwdWaves) void k_mlp_bwd(BwdArgs a) {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    __bf16 *w2 = reinterpret_cast<__bf16 *>(lds_raw);   // layer 2 (dH2 -> dH1)
-    __bf16 *w3 = w2 + kGuideH * kLdH;                   // layer 3 (dH3 -> dH2)
-    __bf16 *w4 = w3 + kGuideH * kLdH;                   // layer 4 (dRaw -> dH3)
-    __bf16 *tiles = w4 + kGuideH * kLdH;
-    stage_weights<kGuideH, kLdH>(w2, a.wb[0], kGuideH);
-    stage_weights<kGuideH, kLdH>(w3, a.wb[1], kGuideH);
-    stage_weights<kGuideNOutPad, kLdH>(w4, a.wb[2], kGuideH);
-    __syncthreads();
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int lr = lane & 15, lq = lane >> 4;
-    __bf16 *ft = tiles + wave * 16 * kLdH;
-    const int64_t ntiles = a.rows >> 4;
-    for (int64_t tile = (int64_t)blockIdx.x * kBwdWaves + wave; tile < ntiles; tile += (int64_t)gridDim.x * kBwdWaves) {
-        // dH3: A = dRaw rows straight from global (K = 64), as k_gemm reads them
-        f32x4 acc[kGuideH / 16];
-#pragma unroll
-        for (int t = 0; t < kGuideH / 16; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-        const __bf16 *arow = a.draw + (tile * 16 + lr) * kGuideNOutPad + 8 * lq;
-#pragma unroll
-        for (int s2 = 0; s2 < kGuideNOutPad / 32; ++s2) {
-            const bf16x8 av = *reinterpret_cast<const bf16x8 *>(arow + 32 * s2);
-#pragma unroll
-            for (int t = 0; t < kGuideH / 16; ++t) {
-                const bf16x8 b = *reinterpret_cast<const bf16x8 *>(w4 + (16 * t + lr) * kLdH + 32 * s2 + 8 * lq);
-                acc[t] = mfma(av, b, acc[t]);
-            }
-        }
-        store_grad(ft, a.dh[2], a.h[2], acc, tile, lane);
-        wave_sync();
-        tile_mfma<kGuideH / 16, kGuideH, kLdH>(ft, w3, lr, lq, acc);
-        wave_sync();
-        store_grad(ft, a.dh[1], a.h[1], acc, tile, lane);
-        wave_sync();
-        tile_mfma<kGuideH / 16, kGuideH, kLdH>(ft, w2, lr, lq, acc);
-        wave_sync();
-        store_grad(ft, a.dh[0], a.h[0], acc, tile, lane);
-        wave_sync();
-    }
-}
-
 // loss = sum_r s_r * (-log q(w_r)) / sum_r s_r. Writes s_r d(-log q(w_r)) / d raw (bf16, 48
 // wide + pad to 64), the row's weighted gradient before the division by sum_r s_r, which
 // the reduction of the weight gradients applies, and per block the sums of
@@ -568,8 +480,10 @@ __global__ __launch_bounds__(256) void k_loss_parts(const float *__restrict__ lo
 // a time in LDS, transposed ([feature][row]) so that each lane's 8 operand values
 // along the row (reduction) dimension are 16 contiguous bytes. A staging task moves
 // 8 rows x 8 features: eight 16-byte loads into registers, an 8 x 8 transpose in
-// registers, eight 16-byte LDS rows. The tasks of H and G are spread over all of the
-// block's threads, and the next chunk's loads are issued before the current chunk's
+// registers, eight 16-byte LDS rows. Consecutive threads take consecutive row groups
+// of one feature group, so the 16 lanes of a quarter wave store to 16 different LDS
+// bank groups (with the feature groups fastest, 4 lanes shared each group). The tasks of
+// H and G are spread over all of the block's threads, and the next chunk's loads are issued before the current chunk's
 // products, so that they are in flight while the matrix cores work. The bias sums
 // are products too: an all-ones A operand against a tile of G gives G's column sums
 // in every row of the result.
@@ -599,7 +513,8 @@ __device__ __forceinline__ void wg_load(bf16x8 (&v)[8], const __bf16 *__restrict
     }
 }
 
-__device__ __forceinline__ void wg_store(__bf16 (*dst)[kWgRows + 8], const bf16x8 (&v)[8], int rg, int fg) {
+template <int LD>
+__device__ __forceinline__ void wg_store(__bf16 (*dst)[LD], const bf16x8 (&v)[8], int rg, int fg) {
 #pragma unroll
     for (int ff = 0; ff < 8; ++ff) {
         bf16x8 w;
@@ -614,7 +529,6 @@ __device__ __forceinline__ void wg_store(__bf16 (*dst)[kWgRows + 8], const bf16x
 // bits), zero for rows at or past `valid` (the batch's rows, or the block's end). A
 // task is one input of two adjacent rows, rows fastest across the threads, and writes
 // each feature of the pair as one 4-byte word. The padding features are not written.
-typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ void stage_features(__bf16 (*dst)[kWgRows + 8], const float *__restrict__ in, int64_t rb,
                                                int64_t valid) {
     constexpr int PAIRS = kWgRows / 2;
@@ -673,9 +587,9 @@ __global__ __launch_bounds__(kWgThreads) void k_wgrad(const __bf16 *__restrict__
 #define TT_WG_LOAD(rb_)                                                                            \
     _Pragma("unroll") for (int k = 0; k < S::NT; ++k) {                                            \
         const int t = threadIdx.x + k * kWgThreads;                                                \
-        if (t < S::TH) wg_load(v[k], H, ldh, t / (KI / 8), t % (KI / 8), (rb_), r1);               \
+        if (t < S::TH) wg_load(v[k], H, ldh, t % S::RG, t / S::RG, (rb_), r1);                     \
         else if (t < S::TH + S::TG)                                                                \
-            wg_load(v[k], G, ldg, (t - S::TH) / (KO / 8), (t - S::TH) % (KO / 8), (rb_), r1);       \
+            wg_load(v[k], G, ldg, (t - S::TH) % S::RG, (t - S::TH) / S::RG, (rb_), r1);             \
     }
     if (r0 < r1) TT_WG_LOAD(r0)
     for (int64_t rb = r0; rb < r1; rb += kWgRows) {
@@ -683,8 +597,8 @@ __global__ __launch_bounds__(kWgThreads) void k_wgrad(const __bf16 *__restrict__
 #pragma unroll
         for (int k = 0; k < S::NT; ++k) {
             const int t = threadIdx.x + k * kWgThreads;
-            if (t < S::TH) wg_store(hs, v[k], t / (KI / 8), t % (KI / 8));
-            else if (t < S::TH + S::TG) wg_store(gs, v[k], (t - S::TH) / (KO / 8), (t - S::TH) % (KO / 8));
+            if (t < S::TH) wg_store(hs, v[k], t % S::RG, t / S::RG);
+            else if (t < S::TH + S::TG) wg_store(gs, v[k], (t - S::TH) % S::RG, (t - S::TH) / S::RG);
         }
         if (ENC) stage_features(hs, enc_in, rb, min(r1, enc_rows));
         __syncthreads();
@@ -724,6 +638,241 @@ __global__ __launch_bounds__(kWgThreads) void k_wgrad(const __bf16 *__restrict__
         }
     }
 }
+
+// ------------------------------------------------------------------ fused backward
+// The data gradients of layers 4, 3, 2 and the weight gradients of layers 4, 3, 2 in one
+// kernel. For every ROWS-row chunk of the block's rows:
+//   1. H3, H2, H1 and dRaw of the chunk are staged transposed ([feature][row]) in LDS, as
+//      k_wgrad stages its operands (the next chunk's loads are issued before the products);
+//   2. each wave takes one 16-row tile through the three backward products
+//      dH3 = (dRaw W4) * (H3 > 0), dH2 = (dH3 W3) * (H2 > 0), dH1 = (dH2 W2) * (H1 > 0): each
+//      product's output kept row-major in the wave's tile as the next product's A operand
+//      and written transposed into LDS (dH3, dH2) for the weight gradients; the ReLU masks
+//      come from the staged activations; dH1 goes to global, where the first layer's
+//      weight gradient reads it. The backward weight copies are read from LDS (WLDS) or
+//      straight from global memory, where all waves of the chip share them in cache;
+//   3. the weight gradients dW4 += H3^T dRaw, dW3 += H2^T dH3, dW2 += H1^T dH2 and their
+//      bias sums accumulate over the chunk, the 55 result tiles spread over the waves.
+// The per-tile arithmetic is that of k_gemm; each result tile takes its 32-row MFMA slices
+// in ascending row order over the block's rows, as k_wgrad does (for any chunk size), so
+// dH1 and the partials are the same bits as the separate kernels' (the unfused path
+// computes them that way), and dH3, dH2 never leave the CU.
+constexpr int kBwTiles = 3 * 4 + 4 * 4 + 4 * 4 + 3 + 4 + 4;   // weight tiles of layers 4, 3, 2, then bias tiles
+
+template <int ROWS, bool WLDS>
+struct BwShape {
+    static constexpr int WAVES = ROWS / 16;   // one 16-row tile of the chunk per wave
+    static constexpr int LDT = ROWS + 8;      // transposed staging row stride (bf16)
+    static constexpr int PER_WAVE = (kBwTiles + WAVES - 1) / WAVES;
+    static constexpr int LDW = WLDS ? kLdH : kGuideH;   // row stride of the backward weight copies read
+    static constexpr int LDS =
+        2 * ((WLDS ? 3 * kGuideH * kLdH : 0) + (5 * kGuideH + kGuideNOutPad16) * LDT + WAVES * 16 * kLdH);
+    static_assert(LDS <= 160 * 1024, "fused backward block exceeds the CU's LDS");
+    static_assert(ROWS % 32 == 0 && kWgRows % ROWS == 0, "whole MFMA row slices; chunks tile the block's rows");
+};
+
+struct BwdArgs {
+    const __bf16 *draw;      // [R][64] output gradients
+    const __bf16 *wb[3];     // backward copies of layers 2, 3, 4: [in 64][out_pad 64]
+    const __bf16 *h[3];      // H1, H2, H3 [R][64]
+    __bf16 *dh1;             // dH1 [R][64]
+    int64_t rows;            // R, a multiple of 16
+    int64_t rows_per_block;  // the weight gradients' block rows (a multiple of kWgRows)
+    float *pw[3], *pb[3];    // weight / bias partials of layers 2, 3, 4 (block b at + b * count)
+};
+
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+
+// One backward product's output (D layout: rows 4 lq + j, column 16 t + lr of the wave's
+// tile, chunk row `cr` for j = 0): rounded to BF16, masked where the layer's activation
+// is not positive, into the wave's row-major tile and, if td is set, transposed into td.
+template <int LDT>
+__device__ __forceinline__ void bwd_out(const f32x4 *acc, const __bf16 (*th)[LDT], __bf16 (*td)[LDT], __bf16 *ft,
+                                        int cr, int lr, int lq) {
+#pragma unroll
+    for (int t = 0; t < kGuideH / 16; ++t) {
+        const int col = 16 * t + lr;
+        const bf16x4 hm = *reinterpret_cast<const bf16x4 *>(&th[col][cr]);
+        bf16x4 o;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            __bf16 v = (__bf16)(acc[t][j] + 0.0f);
+            if (!((float)hm[j] > 0.0f)) v = (__bf16)0.0f;
+            o[j] = v;
+            ft[(4 * lq + j) * kLdH + col] = v;
+        }
+        if (td) *reinterpret_cast<bf16x4 *>(&td[col][cr]) = o;
+    }
+}
+
+// The wave's 16-row tile (row-major in LDS, stride kLdH) times W^T over K = 64, W rows
+// of stride LDW (LDS or global)
+template <int LDW>
+__device__ __forceinline__ void tile_mfma_w(const __bf16 *act, const __bf16 *w, int lr, int lq, f32x4 *acc) {
+#pragma unroll
+    for (int t = 0; t < kGuideH / 16; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < kGuideH / 32; ++s) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(act + lr * kLdH + 32 * s + 8 * lq);
+#pragma unroll
+        for (int t = 0; t < kGuideH / 16; ++t) {
+            const bf16x8 b = *reinterpret_cast<const bf16x8 *>(w + (16 * t + lr) * LDW + 32 * s + 8 * lq);
+            acc[t] = mfma(a, b, acc[t]);
+        }
+    }
+}
+
+template <int ROWS, bool WLDS>
+__global__ __launch_bounds__(ROWS * 4) void k_bwd_wgrad(BwdArgs a) {
+    using S = BwShape<ROWS, WLDS>;
+    constexpr int LDT = S::LDT, LDW = S::LDW;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __bf16 *wl = reinterpret_cast<__bf16 *>(lds_raw);
+    const __bf16 *w2 = WLDS ? wl : a.wb[0];                          // layer 2 (dH2 -> dH1)
+    const __bf16 *w3 = WLDS ? wl + kGuideH * kLdH : a.wb[1];         // layer 3 (dH3 -> dH2)
+    const __bf16 *w4 = WLDS ? wl + 2 * kGuideH * kLdH : a.wb[2];     // layer 4 (dRaw -> dH3)
+    typedef __bf16 TRow[LDT];
+    TRow *tH3 = reinterpret_cast<TRow *>(wl + (WLDS ? 3 * kGuideH * kLdH : 0));
+    TRow *tH2 = tH3 + kGuideH, *tH1 = tH2 + kGuideH, *tD3 = tH1 + kGuideH, *tD2 = tD3 + kGuideH;
+    TRow *tD4 = tD2 + kGuideH;   // dRaw, the head's kGuideNOutPad16 columns
+    __bf16 *tiles = reinterpret_cast<__bf16 *>(tD4 + kGuideNOutPad16);
+    if (WLDS) {
+        stage_weights<kGuideH, kLdH>(wl, a.wb[0], kGuideH);
+        stage_weights<kGuideH, kLdH>(wl + kGuideH * kLdH, a.wb[1], kGuideH);
+        stage_weights<kGuideNOutPad, kLdH>(wl + 2 * kGuideH * kLdH, a.wb[2], kGuideH);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lr = lane & 15, lq = lane >> 4;
+    __bf16 *ft = tiles + wave * 16 * kLdH;
+    const int64_t r0 = (int64_t)blockIdx.x * a.rows_per_block;
+    const int64_t r1 = min(a.rows, r0 + a.rows_per_block);
+    bf16x8 ones;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) ones[q] = (__bf16)1.0f;
+    f32x4 acc[S::PER_WAVE];
+#pragma unroll
+    for (int q = 0; q < S::PER_WAVE; ++q) acc[q] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    // staging tasks (8 rows x 8 features each): H3, H2, H1 (8 feature groups), dRaw (6)
+    constexpr int RG = ROWS / 8, FH = kGuideH / 8, FD = kGuideNOutPad16 / 8;
+    constexpr int TASKS = RG * (3 * FH + FD);
+    static_assert(TASKS <= 64 * S::WAVES, "one staging task per thread");
+    const int task = threadIdx.x;
+    const int which = task < RG * FH ? 0 : task < 2 * RG * FH ? 1 : task < 3 * RG * FH ? 2 : task < TASKS ? 3 : 4;
+    const int tt = which < 3 ? task - which * RG * FH : task - 3 * RG * FH;
+    const int rg = tt % RG, fg = tt / RG;
+    const __bf16 *src = which == 0 ? a.h[2] : which == 1 ? a.h[1] : which == 2 ? a.h[0] : a.draw;
+    const int ld = which < 3 ? kGuideH : kGuideNOutPad;
+    TRow *dst = which == 0 ? tH3 : which == 1 ? tH2 : which == 2 ? tH1 : tD4;
+    bf16x8 v[8];
+    if (which < 4 && r0 < r1) wg_load(v, src, ld, rg, fg, r0, r1);
+
+    for (int64_t rb = r0; rb < r1; rb += ROWS) {
+        __syncthreads();   // the weights are staged; the previous chunk's products are done with LDS
+        if (which < 4) wg_store(dst, v, rg, fg);
+        __syncthreads();
+        if (which < 4 && rb + ROWS < r1) wg_load(v, src, ld, rg, fg, rb + ROWS, r1);
+
+        // 2. the wave's tile through the three backward products
+        const int cr = 16 * wave + 4 * lq;   // chunk row of the lane's first output row
+        const int64_t trow = rb + 16 * wave;
+        if (trow < r1) {
+            f32x4 d[kGuideH / 16];
+#pragma unroll
+            for (int t = 0; t < kGuideH / 16; ++t) d[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+            const __bf16 *arow = a.draw + (trow + lr) * kGuideNOutPad + 8 * lq;
+#pragma unroll
+            for (int s2 = 0; s2 < kGuideNOutPad / 32; ++s2) {
+                const bf16x8 av = *reinterpret_cast<const bf16x8 *>(arow + 32 * s2);
+#pragma unroll
+                for (int t = 0; t < kGuideH / 16; ++t) {
+                    const bf16x8 b = *reinterpret_cast<const bf16x8 *>(w4 + (16 * t + lr) * LDW + 32 * s2 + 8 * lq);
+                    d[t] = mfma(av, b, d[t]);
+                }
+            }
+            bwd_out<LDT>(d, tH3, tD3, ft, cr, lr, lq);
+            wave_sync();
+            tile_mfma_w<LDW>(ft, w3, lr, lq, d);
+            wave_sync();
+            bwd_out<LDT>(d, tH2, tD2, ft, cr, lr, lq);
+            wave_sync();
+            tile_mfma_w<LDW>(ft, w2, lr, lq, d);
+            wave_sync();
+            bwd_out<LDT>(d, tH1, nullptr, ft, cr, lr, lq);
+            wave_sync();
+            tile_to_global<kGuideH, kLdH>(ft, a.dh1, trow >> 4, lane);
+        } else {
+            // rows past the batch: zero gradients for the weight products
+            bf16x4 z;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) z[j] = (__bf16)0.0f;
+#pragma unroll
+            for (int t = 0; t < kGuideH / 16; ++t) {
+                *reinterpret_cast<bf16x4 *>(&tD3[16 * t + lr][cr]) = z;
+                *reinterpret_cast<bf16x4 *>(&tD2[16 * t + lr][cr]) = z;
+            }
+        }
+        __syncthreads();   // dH3, dH2 of every tile are in LDS
+
+        // 3. weight gradients over the chunk: tiles 0..11 layer 4 (4 x 3), 12..27 layer 3,
+        // 28..43 layer 2 (4 x 4 each), then bias tiles 44..46, 47..50, 51..54
+#pragma unroll
+        for (int q = 0; q < S::PER_WAVE; ++q) {
+            const int tile = wave * S::PER_WAVE + q;
+            if (tile >= kBwTiles) continue;
+            const TRow *A = nullptr, *B;
+            int ti = 0, to;
+            if (tile < 12) {
+                A = tH3, B = tD4, ti = tile / 3, to = tile % 3;
+            } else if (tile < 28) {
+                A = tH2, B = tD3, ti = (tile - 12) / 4, to = (tile - 12) % 4;
+            } else if (tile < 44) {
+                A = tH1, B = tD2, ti = (tile - 28) / 4, to = (tile - 28) % 4;
+            } else if (tile < 47) {
+                B = tD4, to = tile - 44;
+            } else if (tile < 51) {
+                B = tD3, to = tile - 47;
+            } else {
+                B = tD2, to = tile - 51;
+            }
+#pragma unroll
+            for (int s2 = 0; s2 < ROWS / 32; ++s2) {
+                const bf16x8 b = *reinterpret_cast<const bf16x8 *>(&B[16 * to + lr][32 * s2 + 8 * lq]);
+                const bf16x8 av = A ? *reinterpret_cast<const bf16x8 *>(&A[16 * ti + lr][32 * s2 + 8 * lq]) : ones;
+                acc[q] = mfma(av, b, acc[q]);
+            }
+        }
+    }
+
+    // partials in k_wgrad's layout: weights [in][out] per block, biases [out] per block
+#pragma unroll
+    for (int q = 0; q < S::PER_WAVE; ++q) {
+        const int tile = wave * S::PER_WAVE + q;
+        if (tile >= kBwTiles) continue;
+        if (tile < 44) {
+            const int layer = tile < 12 ? 2 : tile < 28 ? 1 : 0;   // index into pw: layers 2, 3, 4
+            const int ko = layer == 2 ? kGuideNOutPad16 : kGuideH;
+            const int lt = tile < 12 ? tile : tile < 28 ? tile - 12 : tile - 28;
+            const int ti = lt / (ko / 16), to = lt % (ko / 16);
+            float *out = a.pw[layer] + (int64_t)blockIdx.x * kGuideH * ko;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) out[(16 * ti + 4 * lq + j) * ko + 16 * to + lr] = acc[q][j];
+        } else if (lq == 0) {
+            const int layer = tile < 47 ? 2 : tile < 51 ? 1 : 0;
+            const int ko = layer == 2 ? kGuideNOutPad16 : kGuideH;
+            const int to = tile < 47 ? tile - 44 : tile < 51 ? tile - 47 : tile - 51;
+            a.pb[layer][(int64_t)blockIdx.x * ko + 16 * to + lr] = acc[q][0];
+        }
+    }
+}
+
+// The fused backward's chunk rows and weight placement: 128-row chunks (8 waves) with
+// the weights in LDS took 77 us per 2^18-row step; 64-row chunks 113 us (weights in LDS)
+// and 136 us (from global), 32-row chunks 134 us, 128-row chunks with the weights read
+// from global 101 us: every wave reads each weight matrix once per product, which the
+// vector memory pipeline serves more slowly than LDS.
+constexpr int kBwRows = 128;
+constexpr bool kBwWLds = true;
 
 // Sum of the nb per-block partial gradients of every parameter, in a fixed order
 // (deterministic), all layers in one launch: 8 regions of partials (each layer's
@@ -949,9 +1098,10 @@ hipError_t guide_infer(tt_guide *g, const float *in, float *raw, int64_t n, cons
 }
 
 // Gradient of the loss over rows [0, n) of the batch (inputs, dirs, wts staged).
-// fused: the forward pass in the fused kernel (writing the activations) and the
-// three backward products in k_mlp_bwd; otherwise k_encode and one k_gemm launch per
-// product (the same bits; kept as the check of the fused path).
+// fused: the forward pass in the fused inference kernel (writing the activations), the
+// backward products with the weight gradients of layers 4..2 in k_bwd_wgrad, and the
+// first layer's weight gradient; otherwise k_encode, one k_gemm launch per product and
+// one k_wgrad launch per layer (the same bits; kept as the check of the fused path).
 static hipError_t guide_backward(tt_guide *g, int64_t n, hipStream_t s, bool fused = true) {
     GuideBatch &b = g->batch;
     const int64_t R = (n + 15) & ~15LL;
@@ -965,34 +1115,6 @@ static hipError_t guide_backward(tt_guide *g, int64_t n, hipStream_t s, bool fus
                        b.lpart + lb, n);
     hipLaunchKernelGGL(k_loss_parts, dim3(1), dim3(256), 0, s, (const float *)b.lpart, (const float *)(b.lpart + lb),
                        (int64_t)lb, g->scalars);
-    // dH3 = dRaw W4 (mask H3 > 0), dH2 = dH3 W3 (mask), dH1 = dH2 W2 (mask)
-    if (fused) {
-        BwdArgs ba{};
-        ba.draw = b.draw;
-        for (int k = 0; k < 3; ++k) {
-            ba.wb[k] = g->wb[k + 1];
-            ba.h[k] = b.h[k];
-            ba.dh[k] = b.dh[k];
-        }
-        ba.rows = R;
-        e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_mlp_bwd), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                kBwdLds);
-        if (e != hipSuccess) return e;
-        const int64_t grid = std::max<int64_t>(
-            1, std::min<int64_t>((R / 16 + kBwdWaves - 1) / kBwdWaves, 2 * (int64_t)std::max(1, g->ctx->num_cus)));
-        hipLaunchKernelGGL(k_mlp_bwd, dim3((unsigned)grid), dim3(64 * kBwdWaves), kBwdLds, s, ba);
-    } else {
-        const unsigned gw = (unsigned)std::min<int64_t>(((R / 16) + 3) / 4 + 1, 8192);
-        hipLaunchKernelGGL((k_gemm<kGuideNOutPad, kGuideH, false, true>), dim3(gw), dim3(256), 0, s, b.draw, g->wb[3],
-                           (const float *)nullptr, (void *)b.dh[2], kGuideH, (const __bf16 *)b.h[2], R,
-                           (const int32_t *)nullptr);
-        hipLaunchKernelGGL((k_gemm<kGuideH, kGuideH, false, true>), dim3(gw), dim3(256), 0, s, b.dh[2], g->wb[2],
-                           (const float *)nullptr, (void *)b.dh[1], kGuideH, (const __bf16 *)b.h[1], R,
-                           (const int32_t *)nullptr);
-        hipLaunchKernelGGL((k_gemm<kGuideH, kGuideH, false, true>), dim3(gw), dim3(256), 0, s, b.dh[1], g->wb[1],
-                           (const float *)nullptr, (void *)b.dh[0], kGuideH, (const __bf16 *)b.h[0], R,
-                           (const int32_t *)nullptr);
-    }
     // weight gradients: dW_k = H_{k-1}^T dH_k and bias gradients, partial sums per
     // block (at most kWgBlocks blocks of whole kWgRows chunks), reduced in block order
     constexpr int64_t kWgBlocks = 512, kWgMinRows = 512;
@@ -1037,21 +1159,54 @@ static hipError_t guide_backward(tt_guide *g, int64_t n, hipStream_t s, bool fus
             return hipErrorOutOfMemory;
         g->partial_cap = need;
     }
-    for (int k = 0; k < 4; ++k) {
-        const L &l = ls[k];
-        float *pw = g->partial + rg.start[2 * k], *pbias = g->partial + rg.start[2 * k + 1];
-        if (k == 0 && fused)
-            hipLaunchKernelGGL((k_wgrad<kGuideK1, kGuideH, true>), dim3(nb), dim3(256), 0, s, l.h, l.ldh, l.d, l.ldd, R,
-                               rpb, pw, pbias, (const float *)b.in, n);
-        else if (k == 0)
-            hipLaunchKernelGGL((k_wgrad<kGuideK1, kGuideH>), dim3(nb), dim3(256), 0, s, l.h, l.ldh, l.d, l.ldd, R, rpb,
-                               pw, pbias, (const float *)nullptr, (int64_t)0);
-        else if (k < 3)
-            hipLaunchKernelGGL((k_wgrad<kGuideH, kGuideH>), dim3(nb), dim3(256), 0, s, l.h, l.ldh, l.d, l.ldd, R, rpb,
-                               pw, pbias, (const float *)nullptr, (int64_t)0);
-        else
-            hipLaunchKernelGGL((k_wgrad<kGuideH, kGuideNOutPad16>), dim3(nb), dim3(256), 0, s, l.h, l.ldh, l.d, l.ldd,
-                               R, rpb, pw, pbias, (const float *)nullptr, (int64_t)0);
+    if (fused) {
+        // dH3, dH2, dH1 and the weight gradients of layers 4, 3, 2 in one launch; then
+        // the first layer's, its features recomputed from the inputs
+        BwdArgs ba{};
+        ba.draw = b.draw;
+        for (int k = 0; k < 3; ++k) {
+            ba.wb[k] = g->wb[k + 1];
+            ba.h[k] = b.h[k];
+            ba.pw[k] = g->partial + rg.start[2 * (k + 1)];
+            ba.pb[k] = g->partial + rg.start[2 * (k + 1) + 1];
+        }
+        ba.dh1 = b.dh[0];
+        ba.rows = R;
+        ba.rows_per_block = rpb;
+        using BS = BwShape<kBwRows, kBwWLds>;
+        e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_bwd_wgrad<kBwRows, kBwWLds>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, BS::LDS);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((k_bwd_wgrad<kBwRows, kBwWLds>), dim3(nb), dim3(64 * BS::WAVES), BS::LDS, s, ba);
+        const L &l = ls[0];
+        hipLaunchKernelGGL((k_wgrad<kGuideK1, kGuideH, true>), dim3(nb), dim3(256), 0, s, l.h, l.ldh, l.d, l.ldd, R,
+                           rpb, g->partial + rg.start[0], g->partial + rg.start[1], (const float *)b.in, n);
+    } else {
+        // dH3 = dRaw W4 (mask H3 > 0), dH2 = dH3 W3 (mask), dH1 = dH2 W2 (mask), one
+        // launch per product, then one weight-gradient launch per layer
+        const unsigned gw = (unsigned)std::min<int64_t>(((R / 16) + 3) / 4 + 1, 8192);
+        hipLaunchKernelGGL((k_gemm<kGuideNOutPad, kGuideH, false, true>), dim3(gw), dim3(256), 0, s, b.draw, g->wb[3],
+                           (const float *)nullptr, (void *)b.dh[2], kGuideH, (const __bf16 *)b.h[2], R,
+                           (const int32_t *)nullptr);
+        hipLaunchKernelGGL((k_gemm<kGuideH, kGuideH, false, true>), dim3(gw), dim3(256), 0, s, b.dh[2], g->wb[2],
+                           (const float *)nullptr, (void *)b.dh[1], kGuideH, (const __bf16 *)b.h[1], R,
+                           (const int32_t *)nullptr);
+        hipLaunchKernelGGL((k_gemm<kGuideH, kGuideH, false, true>), dim3(gw), dim3(256), 0, s, b.dh[1], g->wb[1],
+                           (const float *)nullptr, (void *)b.dh[0], kGuideH, (const __bf16 *)b.h[0], R,
+                           (const int32_t *)nullptr);
+        for (int k = 0; k < 4; ++k) {
+            const L &l = ls[k];
+            float *pw = g->partial + rg.start[2 * k], *pbias = g->partial + rg.start[2 * k + 1];
+            if (k == 0)
+                hipLaunchKernelGGL((k_wgrad<kGuideK1, kGuideH>), dim3(nb), dim3(256), 0, s, l.h, l.ldh, l.d, l.ldd, R,
+                                   rpb, pw, pbias, (const float *)nullptr, (int64_t)0);
+            else if (k < 3)
+                hipLaunchKernelGGL((k_wgrad<kGuideH, kGuideH>), dim3(nb), dim3(256), 0, s, l.h, l.ldh, l.d, l.ldd, R,
+                                   rpb, pw, pbias, (const float *)nullptr, (int64_t)0);
+            else
+                hipLaunchKernelGGL((k_wgrad<kGuideH, kGuideNOutPad16>), dim3(nb), dim3(256), 0, s, l.h, l.ldh, l.d,
+                                   l.ldd, R, rpb, pw, pbias, (const float *)nullptr, (int64_t)0);
+        }
     }
     hipLaunchKernelGGL(k_reduce_all, dim3((first + 63) / 64), dim3(64 * kRedParts), 0, s, (const float *)g->partial,
                        nb, first, rg, (const float *)(g->scalars + 1), g->grad);
