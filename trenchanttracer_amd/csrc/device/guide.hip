@@ -31,6 +31,7 @@ namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
@@ -182,9 +183,12 @@ __device__ __forceinline__ void stage_weights(__bf16 *dst, const __bf16 *src, in
     }
 }
 
-// A (16 x 32 s.. of a row-major LDS tile with row stride LD) times W^T over K
+// (A W^T)^T over K for a 16-row tile A (row-major in LDS, row stride LD) and W[out][K]
+// (stride LD): W's rows are the MFMA's A operand and the tile's rows its B operand, so
+// each output element is the dot product k_gemm computes, and the lane holds columns
+// 16 t + 4 lq + j of tile row lr.
 template <int NT, int K, int LD>
-__device__ __forceinline__ void tile_mfma(const __bf16 *act, const __bf16 *w, int lr, int lq, f32x4 *acc) {
+__device__ __forceinline__ void tile_mfma_t(const __bf16 *act, const __bf16 *w, int lr, int lq, f32x4 *acc) {
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -193,18 +197,23 @@ __device__ __forceinline__ void tile_mfma(const __bf16 *act, const __bf16 *w, in
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             const bf16x8 b = *reinterpret_cast<const bf16x8 *>(w + (16 * t + lr) * LD + 32 * s + 8 * lq);
-            acc[t] = mfma(a, b, acc[t]);
+            acc[t] = mfma(b, a, acc[t]);
         }
     }
 }
 
-// bias + ReLU, BF16, into the per-wave activation tile (D layout: row 4 lq + j, column 16 t + lr)
+// bias + ReLU, BF16, into the per-wave activation tile. The products come out transposed
+// (tile_mfma_t: lane holds columns 16 t + 4 lq + j of row lr), so each lane writes 4
+// adjacent columns of its row as one 8-byte store (a lane of the untransposed layout
+// holds one column of 4 rows: 16 two-byte stores, two lanes per bank word).
 __device__ __forceinline__ void store_hidden(__bf16 *act, const f32x4 *acc, const float *bias, int lr, int lq) {
 #pragma unroll
-    for (int t = 0; t < kGuideH / 16; ++t)
+    for (int t = 0; t < kGuideH / 16; ++t) {
+        bf16x4 o;
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-            act[(4 * lq + j) * kLdH + 16 * t + lr] = (__bf16)fmaxf(acc[t][j] + bias[16 * t + lr], 0.0f);
+        for (int j = 0; j < 4; ++j) o[j] = (__bf16)fmaxf(acc[t][j] + bias[16 * t + 4 * lq + j], 0.0f);
+        *reinterpret_cast<bf16x4 *>(act + lr * kLdH + 16 * t + 4 * lq) = o;
+    }
 }
 
 // orders a wave's LDS writes before its later reads (the tile is private to the wave)
@@ -273,29 +282,29 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(InferArgs a) {
         }
         wave_sync();
         f32x4 acc[kGuideH / 16];
-        tile_mfma<kGuideH / 16, kGuideK1, kLd1>(ft, w1, lr, lq, acc);
+        tile_mfma_t<kGuideH / 16, kGuideK1, kLd1>(ft, w1, lr, lq, acc);
         wave_sync();
         store_hidden(ft, acc, b1, lr, lq);
         wave_sync();
         if (a.h_out[0]) tile_to_global<kGuideH, kLdH>(ft, a.h_out[0], tile, lane);
-        tile_mfma<kGuideH / 16, kGuideH, kLdH>(ft, w2, lr, lq, acc);
+        tile_mfma_t<kGuideH / 16, kGuideH, kLdH>(ft, w2, lr, lq, acc);
         wave_sync();
         store_hidden(ft, acc, b2, lr, lq);
         wave_sync();
         if (a.h_out[1]) tile_to_global<kGuideH, kLdH>(ft, a.h_out[1], tile, lane);
-        tile_mfma<kGuideH / 16, kGuideH, kLdH>(ft, w3, lr, lq, acc);
+        tile_mfma_t<kGuideH / 16, kGuideH, kLdH>(ft, w3, lr, lq, acc);
         wave_sync();
         store_hidden(ft, acc, b3, lr, lq);
         wave_sync();
         if (a.h_out[2]) tile_to_global<kGuideH, kLdH>(ft, a.h_out[2], tile, lane);
         f32x4 out[kGuideNOutPad16 / 16];
-        tile_mfma<kGuideNOutPad16 / 16, kGuideH, kLdH>(ft, w4, lr, lq, out);
+        tile_mfma_t<kGuideNOutPad16 / 16, kGuideH, kLdH>(ft, w4, lr, lq, out);
         wave_sync();
 #pragma unroll
         for (int t = 0; t < kGuideNOutPad16 / 16; ++t)
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                a.raw[(tile * 16 + 4 * lq + j) * kGuideNOutPad16 + 16 * t + lr] = out[t][j] + b4[16 * t + lr];
+            *reinterpret_cast<float4 *>(a.raw + (tile * 16 + lr) * kGuideNOutPad16 + 16 * t + 4 * lq) =
+                make_float4(out[t][0] + b4[16 * t + 4 * lq], out[t][1] + b4[16 * t + 4 * lq + 1],
+                            out[t][2] + b4[16 * t + 4 * lq + 2], out[t][3] + b4[16 * t + 4 * lq + 3]);
     }
 }
 
@@ -680,8 +689,6 @@ struct BwdArgs {
     int64_t rows_per_block;  // the weight gradients' block rows (a multiple of kWgRows)
     float *pw[3], *pb[3];    // weight / bias partials of layers 2, 3, 4 (block b at + b * count)
 };
-
-typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
 // One backward product's output (D layout: rows 4 lq + j, column 16 t + lr of the wave's
 // tile, chunk row `cr` for j = 0): rounded to BF16, masked where the layer's activation
