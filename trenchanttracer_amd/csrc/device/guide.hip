@@ -139,7 +139,7 @@ __global__ __launch_bounds__(256) void k_gemm(const __bf16 *__restrict__ A, cons
 // carries each hidden layer's output (D layout -> A layout); the last layer writes
 // FP32. Arithmetic, rounding points and accumulation order are those of k_encode +
 // k_gemm, so the outputs are identical to the unfused forward pass.
-constexpr int kInferWaves = 12;
+constexpr int kInferWaves = 14;
 constexpr int kLd1 = kGuideK1 + 8, kLdH = kGuideH + 8;
 constexpr int kInferTile = 16 * kLd1;   // bf16 elements per wave tile (features, then activations)
 constexpr int kInferLds = 2 * (kGuideH * kLd1 + 2 * kGuideH * kLdH + kGuideNOutPad16 * kLdH + kInferWaves * kInferTile) +

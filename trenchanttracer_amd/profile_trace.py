@@ -10,10 +10,14 @@ pathtrace: the path-trace case (1,310,720-tri displaced icosphere, sky + sun,
 same calibration copy.
 train: 13 Adam steps of the guiding network on 262,144 seeded random rows (inputs in
 [0, 1], unit directions, Gamma(1) weights), the batch of the online training.
+guided: the path-trace case rendered at 64 spp with guiding, the network trained
+online during the render (Adam lr 1e-3, 2^18-row batches, 2^21 recorded paths per
+training pass, passes growing 4x, learned selection probability), then once more at
+64 spp with that network frozen.
 For profiler runs, one counter set per run:
 
     rocprofv3 --pmc FETCH_SIZE --output-format csv -d OUT -o run -- \\
-        python3 -m trenchanttracer_amd.profile_trace [--case traversal|pathtrace|train] [--variant 1] [--lds-stack 16]
+        python3 -m trenchanttracer_amd.profile_trace [--case traversal|pathtrace|train|guided] [--variant 1] [--lds-stack 16]
             [--block 64] [--refill 0] [--reorder 0] [--paths-in-flight N]
 """
 import argparse
@@ -43,7 +47,7 @@ def main():
     ap.add_argument("--block", type=int, default=64)
     ap.add_argument("--refill", type=int, default=0, help="traversal: refill threshold (0: one wave per group)")
     ap.add_argument("--reorder", type=int, default=0, help="traversal: 1 = trace in Morton order of the origins")
-    ap.add_argument("--case", choices=("traversal", "pathtrace", "train"), default="traversal")
+    ap.add_argument("--case", choices=("traversal", "pathtrace", "train", "guided"), default="traversal")
     ap.add_argument("--paths-in-flight", type=int, default=0, help="pathtrace: paths per pass (0: the default)")
     ap.add_argument("--ntris", type=int, default=1_000_000, help="traversal: triangles of the soup")
     args = ap.parse_args()
@@ -65,6 +69,25 @@ def main():
         ctx.sync()
         print(f"13 training steps of {n} rows, last loss {loss:.4f}", file=sys.stderr, flush=True)
         g.close()
+        ctx.close()
+        return
+
+    if args.case == "guided":
+        mesh, mats, env, cam = workloads.pathtrace()
+        ctx = tt.Context(0)
+        s = tt.DeviceScene(ctx, mesh, tt.BVH.build(mesh), mats, env=env)
+        g = tt.Guide(ctx, seed=1, lr=1e-3)
+
+        def rp(spp, seed, **kw):
+            return tt.render_params(spp=spp, max_depth=16, rr_depth=4, seed=seed, mode=tt.RENDER_PATH, **kw)
+
+        _, st, gs = g.render(s, cam, rp(64, 2000, paths_in_flight=cam.width * cam.height), alpha=0.5, train=True,
+                             train_paths=2 << 20, train_batch=1 << 18, pass_growth=4, learn_alpha=True)
+        print(f"guided 64 spp, trained online: {st} {gs}", file=sys.stderr, flush=True)
+        _, st, _ = g.render(s, cam, rp(64, 3000), alpha=0.5, train=False, learn_alpha=True)
+        print(f"guided 64 spp, frozen network: {st}", file=sys.stderr, flush=True)
+        g.close()
+        s.close()
         ctx.close()
         return
 
