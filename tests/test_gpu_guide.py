@@ -39,7 +39,7 @@ def test_forward_matches_oracle(tt, ctx):
 
 @pytest.mark.parametrize("n", [1, 16, 1000, 70001])
 def test_fused_inference_matches_unfused_forward(tt, ctx, n):
-    """The fused inference kernel (per-chain encoding into an LDS tile, 12-wave blocks)
+    """The fused inference kernel (per-chain encoding into an LDS tile, 14-wave blocks)
     and the training path's k_encode + k_gemm chain perform the same arithmetic in the
     same order: outputs identical bit for bit, ragged tails included."""
     g = tt.Guide(ctx, seed=5)
