@@ -174,3 +174,24 @@ def test_trained_guide_reduces_error(tt, ctx, door):
     assert gs["last_loss"] < 2.0          # uniform sphere density: log(4 pi) = 2.53
     assert mse_g < 0.6 * mse_plain
     g.close()
+
+
+def test_online_training_is_reproducible(tt, ctx, door):
+    """An online-trained guided render depends on its inputs alone: the training rows
+    are collected in record order (no atomics) and every reduction of the training
+    step runs in a fixed order, so two renders with fresh networks of the same seed
+    give the same image and the same trained parameters bit for bit."""
+    sc, cam, _ = door
+    pt = tt.render_params(spp=64, max_depth=12, rr_depth=4, seed=42, mode=tt.RENDER_PATH, paths_in_flight=32768)
+    outs = []
+    for _ in range(2):
+        g = tt.Guide(ctx, seed=3, lr=1e-2)
+        img, st, gs = g.render(sc, cam, pt, alpha=0.5, train=True, train_paths=16384, train_batch=8192,
+                               pass_growth=2, learn_alpha=True)
+        outs.append((img, g.params(), gs))
+        g.close()
+    (i0, p0, g0), (i1, p1, g1) = outs
+    _say(f"online twice: {g0['train_steps']} / {g1['train_steps']} steps, rows {g0['train_rows']} / {g1['train_rows']}")
+    assert g0["train_steps"] > 0 and g0["train_rows"] == g1["train_rows"]
+    assert np.array_equal(p0.view(np.uint32), p1.view(np.uint32))
+    assert np.array_equal(i0.view(np.uint32), i1.view(np.uint32))

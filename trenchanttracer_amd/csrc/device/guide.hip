@@ -1327,7 +1327,7 @@ void tt_guide_destroy(tt_guide *g) {
     GuideBatch &b = g->batch;
     void *ptrs[] = {g->params, g->m1, g->m2, g->grad, g->scalars, g->partial, b.in, b.x, b.h[0], b.h[1], b.h[2],
                     b.raw, b.draw, b.dh[0], b.dh[1], b.dh[2], b.dirs, b.wts, b.pb, b.lpart, g->wf[0], g->wf[1], g->wf[2], g->wf[3],
-                    g->wb[0], g->wb[1], g->wb[2], g->wb[3], g->train_buf, g->train_count, g->rec, g->iin, g->iraw,
+                    g->wb[0], g->wb[1], g->wb[2], g->wb[3], g->train_buf, g->train_off, g->rec, g->iin, g->iraw,
                     g->irow};
     for (void *q : ptrs)
         if (q) hipFree(q);

@@ -170,7 +170,8 @@ struct tt_guide {
     int64_t step = 0;
     // online training from the renderer
     float *train_buf = nullptr;   // [cap][16] samples: inputs (9), dir (3), weight (1), BSDF density (1)
-    int32_t *train_count = nullptr;
+    int32_t *train_off = nullptr;   // per block of 256 records: row offset; then the row total
+    int64_t train_off_cap = 0;
     int64_t train_cap = 0;
     float *rec = nullptr;         // per training path vertex records
     int64_t rec_cap = 0;
