@@ -448,7 +448,7 @@ def guided_bench(tt, ctx, np):
     Plain renders at 64 and 128 spp; guided renders (8-lobe vMF MLP, BF16 MFMA, Adam
     lr 1e-3 on 2^18-row batches, one-sample MIS with the BSDF, the selection
     probability learned per vertex) trained online from scratch throughout (passes
-    growing 4x from 1 spp, up to 2M recorded paths each), trained during the first
+    growing 8x from 1 spp, up to 2M recorded paths each), trained during the first
     half of the samples, and frozen after the 128-spp online run; two seeds per point.
     The time at which each reaches the plain 64-spp RMSE comes from a log-log fit of
     RMSE against time (trenchanttracer_amd/bench_guide.py, which also runs the longer
@@ -461,7 +461,7 @@ def guided_bench(tt, ctx, np):
                                                  fp64=True), want_fp64=True)
     log(f"guided leg: FP64 8192-spp oracle render in {time.time() - t0:.1f}s")
     r = bench_guide.compare(tt, ctx, sc, cam, ref, 16, (64, 128), (64, 128), 1920 * 1080, 1e-3, 262144, 2 << 20,
-                            seeds=2, pass_growth=4, learn_alpha=True)
+                            seeds=2, pass_growth=8, learn_alpha=True)
     sc.close()
     p64 = next(x for x in r["plain"] if x["spp"] == 64)
     g64 = next(x for x in r["guided_online"] if x["spp"] == 64)

@@ -172,7 +172,9 @@ def error_classes(tt, ctx, log):
 
 SWEEP = (("growth2_fixed_alpha", dict(pass_growth=2)),
          ("growth2_learned_alpha", dict(pass_growth=2, learn_alpha=True)),
-         ("growth4_learned_alpha", dict(pass_growth=4, learn_alpha=True)))
+         ("growth4_learned_alpha", dict(pass_growth=4, learn_alpha=True)),
+         ("growth8_learned_alpha", dict(pass_growth=8, learn_alpha=True)),
+         ("growth16_learned_alpha", dict(pass_growth=16, learn_alpha=True)))
 
 
 def sweep(tt, ctx, oracle_spp, seeds, log):

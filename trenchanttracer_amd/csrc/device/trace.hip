@@ -171,7 +171,7 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
         const unsigned long long m = __ballot(done);
         // new rays once enough lanes are idle (all idle always qualifies, so the
         // wave cannot wait on itself)
-        const bool take = __popcll(m) >= a.refill;
+        const bool take = (int32_t)__popcll(m) >= a.refill;
         if (take && done) {
             const int64_t q = cursor + __popcll(m & below);
             if (q < total) {

@@ -12,7 +12,7 @@ train: 13 Adam steps of the guiding network on 262,144 seeded random rows (input
 [0, 1], unit directions, Gamma(1) weights), the batch of the online training.
 guided: the path-trace case rendered at 64 spp with guiding, the network trained
 online during the render (Adam lr 1e-3, 2^18-row batches, 2^21 recorded paths per
-training pass, passes growing 4x, learned selection probability), then once more at
+training pass, passes growing 8x, learned selection probability), then once more at
 64 spp with that network frozen.
 For profiler runs, one counter set per run:
 
@@ -82,7 +82,7 @@ def main():
             return tt.render_params(spp=spp, max_depth=16, rr_depth=4, seed=seed, mode=tt.RENDER_PATH, **kw)
 
         _, st, gs = g.render(s, cam, rp(64, 2000, paths_in_flight=cam.width * cam.height), alpha=0.5, train=True,
-                             train_paths=2 << 20, train_batch=1 << 18, pass_growth=4, learn_alpha=True)
+                             train_paths=2 << 20, train_batch=1 << 18, pass_growth=8, learn_alpha=True)
         print(f"guided 64 spp, trained online: {st} {gs}", file=sys.stderr, flush=True)
         _, st, _ = g.render(s, cam, rp(64, 3000), alpha=0.5, train=False, learn_alpha=True)
         print(f"guided 64 spp, frozen network: {st}", file=sys.stderr, flush=True)
