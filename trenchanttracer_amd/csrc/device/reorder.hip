@@ -69,11 +69,11 @@ __device__ __forceinline__ uint64_t digit_peers(uint32_t d, bool valid) {
 
 __device__ __forceinline__ uint64_t lanes_below() { return (1ull << __lane_id()) - 1ull; }
 
-// Adds this wave's keys of one round to the block histogram: one LDS add per distinct
-// digit (by the lowest lane holding it).
+// Adds a key to the block histogram with one LDS atomic per lane. (Gathering each
+// digit's lanes first, with eight ballots per round as the scatter's ranking does, took
+// 27 us per histogram pass of 8.3M keys against 22 us.)
 __device__ __forceinline__ void hist_add(uint32_t *h, uint32_t d, bool valid) {
-    const uint64_t peers = digit_peers(d, valid);
-    if (valid && (peers & lanes_below()) == 0ull) atomicAdd(&h[d], (uint32_t)__popcll(peers));
+    if (valid) atomicAdd(&h[d], 1u);
 }
 
 __device__ __forceinline__ void write_hist(const uint32_t *h, uint32_t *hist, int64_t tiles, int64_t tile) {
