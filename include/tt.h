@@ -503,7 +503,9 @@ tt_status tt_config_get(const tt_config *c, tt_config_info *info, tt_material *m
  * network takes Adam steps on batches of train_batch rows between passes, until
  * train_spp samples per pixel are done (0: the whole render); the remaining samples
  * then run in passes as wide as a render without training (paths_in_flight, if
- * set, sizes the training passes only). */
+ * set, sizes the training passes only). The training rows are collected in record
+ * order and every reduction of a training step runs in a fixed order: the image and
+ * the trained network are the same on every run for the same inputs and seed. */
 typedef struct tt_guide_params {
     float alpha;          /* probability of sampling the network on guided vertices */
     float min_roughness;  /* conductors at least this rough are guided */
