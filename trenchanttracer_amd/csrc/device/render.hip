@@ -1893,6 +1893,8 @@ tt_status validate_params(const tt_render_params *p) {
 // Sizes the guide's record / training buffers for ntrain paths of max_depth vertices.
 tt_status guide_train_buffers(tt_guide *g, int32_t ntrain, int32_t max_depth, hipStream_t hs) {
     const int64_t ne = (int64_t)ntrain * max_depth;
+    // row offsets and counts are 32-bit
+    if (ne >= (int64_t)INT32_MAX) return fail(TT_ERR_INVALID, "guide: train_paths x max_depth must stay below 2^31");
     if (g->rec_cap < ne * kRecFloats || g->train_cap < ne) {
         if (g->rec) hipFree(g->rec);
         if (g->train_buf) hipFree(g->train_buf);
