@@ -442,7 +442,7 @@ __global__ __launch_bounds__(kLossRows) void k_loss_grad(const float *__restrict
     red[threadIdx.x] = contrib;
     redw[threadIdx.x] = wr;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
+    for (int o = kLossRows / 2; o > 0; o >>= 1) {
         if ((int)threadIdx.x < o) {
             red[threadIdx.x] += red[threadIdx.x + o];
             redw[threadIdx.x] += redw[threadIdx.x + o];
@@ -1015,7 +1015,7 @@ tt_status guide_alloc_batch(tt_guide *g, int64_t rows) {
     al((void **)&b.dirs, (size_t)R * 3 * 4, "guide directions");
     al((void **)&b.wts, (size_t)R * 4, "guide weights");
     al((void **)&b.pb, (size_t)R * 4, "guide BSDF densities");
-    al((void **)&b.lpart, (size_t)2 * ((R + 255) / 256) * 4, "guide loss partials");
+    al((void **)&b.lpart, (size_t)2 * ((R + kLossRows - 1) / kLossRows) * 4, "guide loss partials");
     if (st == TT_OK) g->cap = R;
     return st;
 }
@@ -1116,8 +1116,8 @@ static hipError_t guide_backward(tt_guide *g, int64_t n, hipStream_t s, bool fus
     if (e != hipSuccess) return e;
     // loss and weight sum: per-block partials added in a fixed order (no float atomics);
     // the weight-sum division of the gradient happens in k_reduce_all
-    const unsigned lb = (unsigned)((R + 255) / 256);
-    hipLaunchKernelGGL(k_loss_grad, dim3(lb), dim3(256), 0, s, b.raw, b.dirs, b.wts,
+    const unsigned lb = (unsigned)((R + kLossRows - 1) / kLossRows);
+    hipLaunchKernelGGL(k_loss_grad, dim3(lb), dim3(kLossRows), 0, s, b.raw, b.dirs, b.wts,
                        g->train_alpha ? (const float *)b.pb : (const float *)nullptr, b.draw, b.lpart,
                        b.lpart + lb, n);
     hipLaunchKernelGGL(k_loss_parts, dim3(1), dim3(256), 0, s, (const float *)b.lpart, (const float *)(b.lpart + lb),

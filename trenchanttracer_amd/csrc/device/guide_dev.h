@@ -39,7 +39,7 @@ struct GuideBatch {
     float *dirs = nullptr;    // [R][3] training directions
     float *wts = nullptr;     // [R] training weights (radiance / pdf)
     float *pb = nullptr;      // [R] BSDF density of each direction (learned alpha), or unset
-    float *lpart = nullptr;   // 2 x [(R + 255) / 256]: per-block loss sums, then weight sums (fixed-order adds)
+    float *lpart = nullptr;   // per loss-gradient block: loss sums, then weight sums (fixed-order adds)
 };
 
 __device__ __forceinline__ float sigmoidf_(float z) { return 1.0f / (1.0f + __expf(-z)); }
