@@ -34,6 +34,7 @@ def main():
     ap.add_argument("--refill", type=int, default=32,
                     help="persistent waves take new rays once this many lanes are idle (0: one wave per group)")
     ap.add_argument("--reorder", type=int, default=1, help="1: rays traced in Morton order of their origins")
+    ap.add_argument("--xcd-order", type=int, default=0, help="1: each XCD holds one contiguous range of waves")
     ap.add_argument("--ntris", type=int, default=1_000_000)
     ap.add_argument("--cpu-sample-stride", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -64,7 +65,7 @@ def main():
     d_rays = tt.DeviceBuffer.from_array(ctx, rays)
     d_hits = tt.DeviceBuffer(ctx, n * 16)
     cfg = tt.trace_cfg(variant=args.variant, lds_stack=args.lds_stack, block=args.block, refill=args.refill,
-                       reorder=args.reorder)
+                       xcd_order=args.xcd_order, reorder=args.reorder)
 
     for _ in range(args.warmup):
         scene.trace_closest(d_rays, n, d_hits, cfg)
@@ -94,7 +95,7 @@ def main():
     # order if the headline is speculative, and vice versa); its hits are compared
     other = 1 - args.variant
     other_cfg = tt.trace_cfg(variant=other, lds_stack=args.lds_stack, block=args.block, refill=args.refill,
-                             reorder=args.reorder)
+                             xcd_order=args.xcd_order, reorder=args.reorder)
     d_hits2 = tt.DeviceBuffer(ctx, n * 16)
     scene.trace_closest(d_rays, n, d_hits2, other_cfg)
     other_ms = []
@@ -136,7 +137,7 @@ def main():
     # (the exact loop's counting launch gives the algorithmic bytes: its node visits
     # and triangle tests are the single-lane traversal's)
     exact_cfg = tt.trace_cfg(variant=0, lds_stack=args.lds_stack, block=args.block, refill=args.refill,
-                             reorder=args.reorder)
+                             xcd_order=args.xcd_order, reorder=args.reorder)
     replay = scene.fetch_replay(d_rays, n, cfg, reps=5)
     algo = scene.fetch_replay(d_rays, n, exact_cfg, reps=1)
     log(f"fetch replay of the whole set: {replay['ms']:.3f} ms (call {kmean:.3f} ms)")
@@ -229,7 +230,8 @@ def main():
                         f"({n_prim} primary 1920x1080x2spp + {n - n_prim} incoherent), closest hit",
             "rays": n, "triangles": args.ntris, "tri_refs": info["num_tris"], "nodes": info["num_nodes"],
             "variant": "exact" if args.variant == 0 else "speculative", "lds_stack": args.lds_stack,
-            "block": args.block, "refill": args.refill, "reorder": args.reorder, "hit_fraction": round(hit_frac, 4),
+            "block": args.block, "refill": args.refill, "reorder": args.reorder, "xcd_order": args.xcd_order,
+            "hit_fraction": round(hit_frac, 4),
         },
         "kernel_ms": round(kmean, 4),
         "other_variant": spec,

@@ -283,8 +283,10 @@ tt_status tt_env_export_cdf(const tt_scene *s, float *marg, float *cond, double 
  * and a context-owned workspace of 16 bytes per ray plus the sort's temporary storage,
  * allocated on first use and grown when a larger n arrives; a call with reorder = 1
  * waits (on its stream) for the previous such call of the context to finish with that
- * workspace, whatever stream it ran on. Not meant for HIP graph capture (the sort
- * clears its state with memset operations). The Morton cells span the scene's mesh
+ * workspace, whatever stream it ran on. The sort issues no memset; still, a call is
+ * not meant for HIP graph capture, since the workspace may grow with hipMalloc and
+ * calls order themselves through a context event (tt_trace_closest_noalloc is the
+ * entry for callers that cannot allocate). The Morton cells span the scene's mesh
  * bounds, or the root node's box when the scene was created without a mesh. */
 typedef struct tt_trace_cfg {
     int32_t variant;
