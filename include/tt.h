@@ -546,6 +546,14 @@ tt_status tt_debug_guide_sample(tt_ctx *ctx, const float *raw, int64_t n, uint32
  * tt_guide_forward, computed by the training path's separate encoding and layer
  * kernels. The two agree bit for bit. Synchronous. */
 tt_status tt_debug_guide_forward_unfused(tt_guide *g, const float *inputs, int64_t n, float *raw_out);
+/* Check hook for the renderer's guided-row inference: n queue slots with the given
+ * inputs (n x 9) and guided flags (n, nonzero = guided) go through the render's own
+ * flag and count, slot-list and inference kernels. raw_out (n x 40) gets each guided
+ * slot's outputs, which equal tt_guide_forward's for its inputs bit for bit, and
+ * 0xFFFFFFFF words in every other row (the rows the renderer does not infer stay
+ * unwritten); *rows = the number of guided slots. Synchronous. */
+tt_status tt_debug_guide_infer_slots(tt_guide *g, const float *inputs, const int32_t *guided, int64_t n,
+                                     float *raw_out, int64_t *rows);
 /* Check hook for the fused training kernels: the loss and gradient tt_guide_grad
  * returns, computed with the separate encoding, layer and backward-product kernels.
  * The two agree bit for bit. Synchronous. */

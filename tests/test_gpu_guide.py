@@ -50,6 +50,31 @@ def test_fused_inference_matches_unfused_forward(tt, ctx, n):
     g.close()
 
 
+@pytest.mark.parametrize("n,density,runs", [(1, 1.0, 1), (255, 0.5, 1), (257, 0.03, 1), (4096, 0.0, 1),
+                                             (100003, 1.0, 1), (1000003, 0.4, 64), (3000017, 0.35, 64)])
+def test_renderer_inference_rows_match_forward(tt, ctx, n, density, runs):
+    """The renderer infers only its guided slots: ballot flags and block counts, the list
+    of guided slots in slot order, the inference kernel over that list reading its inputs
+    one tile ahead and writing each mixture back at its slot. Every guided slot gets the
+    outputs the plain inference gives its inputs, bit for bit; no other slot is written.
+    runs: the flag is drawn per run of that many slots (a pixel's samples share theirs);
+    at 3M slots each block of the list kernels walks several 256-slot steps."""
+    g = tt.Guide(ctx, seed=5)
+    rng = np.random.default_rng(n)
+    inp = rng.random((n, 9), dtype=np.float32)
+    f = np.repeat(rng.random((n + runs - 1) // runs) < density, runs)[:n]
+    if runs > 1:
+        f ^= rng.random(n) < 0.02   # a few slots that differ from their run
+    out, rows = g.infer_slots(inp, f)
+    _say(f"infer_slots n={n}: {rows} guided rows")
+    assert rows == int(f.sum())
+    assert np.all(out[~f].view(np.uint32) == 0xFFFFFFFF)
+    if rows:
+        ref = g.forward(inp[f])
+        assert np.array_equal(out[f].view(np.uint32), ref.view(np.uint32))
+    g.close()
+
+
 def test_gradient_is_bit_reproducible(tt, ctx):
     """The loss and every gradient element are the same bits on repeated evaluation of
     the same rows (the weight sum that scales the gradient and the loss are reduced in a

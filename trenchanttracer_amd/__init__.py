@@ -343,6 +343,19 @@ class Guide:
         check(lib.tt_debug_guide_forward_unfused(self._h, _vp(x), len(x), _vp(out)))
         return out
 
+    def infer_slots(self, inputs, guided):
+        """The renderer's guided-row inference over n queue slots with these inputs and
+        guided flags (tt_debug_guide_infer_slots): (outputs n x 40 with the guided rows
+        filled and 0xFFFFFFFF words elsewhere, number of guided rows)."""
+        x = np.ascontiguousarray(inputs, np.float32).reshape(-1, 9)
+        f = np.ascontiguousarray(guided, np.int32).reshape(-1)
+        if len(f) != len(x):
+            raise ValueError("one guided flag per input row")
+        out = np.empty((len(x), 40), np.float32)
+        rows = C.c_int64()
+        check(lib.tt_debug_guide_infer_slots(self._h, _vp(x), _vp(f), len(x), _vp(out), C.byref(rows)))
+        return out, rows.value
+
     def grad(self, inputs, dirs, wts):
         x = np.ascontiguousarray(inputs, np.float32).reshape(-1, 9)
         d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)

@@ -158,6 +158,7 @@ PROTOS = {
     "tt_guide_train_step": (C.c_int, [P, P, P, P, I64, PF]),
     "tt_debug_guide_sample": (C.c_int, [P, P, I64, U32, P]),
     "tt_debug_guide_forward_unfused": (C.c_int, [P, P, I64, P]),
+    "tt_debug_guide_infer_slots": (C.c_int, [P, P, P, I64, P, PI64]),
     "tt_scene_set_geometries": (C.c_int, [P, P, I32]),
     "tt_hdr_read": (C.c_int, [C.c_char_p, PI32, PI32, P, I64]),
     "tt_hdr_write": (C.c_int, [C.c_char_p, I32, I32, P]),

@@ -155,7 +155,9 @@ struct InferArgs {
     float *raw;
     int64_t n;
     const int32_t *n_dev;
-    const int32_t *live;   // if set: rows with live[r] == 0 are not needed; tiles of none are skipped
+    // if set: row r reads its inputs from and writes its outputs to row slot_of[r] of in
+    // and raw (a compacted list of the rows needed); rows past the count write nothing
+    const int32_t *slot_of;
     // training forward: if set, the three hidden layers' activations [rows][64] of every
     // 16-row tile are written too (the backward pass reads them), for all rows of the tile
     __bf16 *h_out[3];
@@ -248,30 +250,52 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(InferArgs a) {
     __bf16 *ft = tiles + wave * kInferTile;
     const int64_t rows = a.n_dev ? min((int64_t)*a.n_dev, a.n) : a.n;
     const int64_t ntiles = (rows + 15) >> 4;
+    const int64_t stride = (int64_t)gridDim.x * kInferWaves;
+    // A tile's inputs are loaded one tile ahead: the lane's 2 or 3 of the tile's 16 x 9
+    // inputs (chain q = lane + 64 k = (row rr, input c), row-major, so the loads
+    // coalesce), their rows' slots shuffled from the lanes that hold them (lane & 15 holds
+    // row lane & 15). The next tile's slots load before this tile's first product and its
+    // inputs after it, so neither the list nor the inputs stall the wave. Slot -1: a row
+    // past the end (zero features, no output).
+    auto row_slot = [&](int64_t tile) -> int32_t {
+        const int64_t r = tile * 16 + lr;
+        if (tile >= ntiles || r >= rows) return -1;
+        return a.slot_of ? a.slot_of[r] : (int32_t)r;
+    };
+    float xin[3];
+    auto load_inputs = [&](int32_t slot) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int q = lane + 64 * k, rr = q / kGuideIn, c = q - rr * kGuideIn;
+            const int32_t src = __shfl(slot, rr & 15);
+            xin[k] = (q < 16 * kGuideIn && src >= 0) ? a.in[(int64_t)src * kGuideIn + c] : 0.0f;
+        }
+    };
+    int64_t tile = (int64_t)blockIdx.x * kInferWaves + wave;
+    int32_t slot = row_slot(tile);
+    load_inputs(slot);
     // whole waves only ever touch their own tile, so no block barrier is needed
     // inside the loop; wave_sync orders the tile's LDS writes and reads
-    for (int64_t tile = (int64_t)blockIdx.x * kInferWaves + wave; tile < ntiles;
-         tile += (int64_t)gridDim.x * kInferWaves) {
-        if (a.live) {   // the wave skips a tile none of whose rows is needed (its outputs stay unwritten)
-            const int64_t r = tile * 16 + (lane & 15);
-            if (!__any(lane < 16 && r < rows && a.live[r] != 0)) continue;
-        }
-        // features x[rr][24 c + 2 f + {sin, cos}]: chain q = (row rr, input c), row-major
-        // so that the input loads coalesce; rows past the end get zero features
-        for (int q = lane; q < 16 * kGuideIn; q += 64) {
-            const int rr = q / kGuideIn, c = q - rr * kGuideIn;
-            const int64_t r = tile * 16 + rr;
-            bf16x2 *o = reinterpret_cast<bf16x2 *>(ft + rr * kLd1 + 2 * kGuideFreq * c);
-            if (r < rows) {
-                FreqState st = freq_base(a.in[r * kGuideIn + c]);
+    for (; tile < ntiles; tile += stride) {
+        const int32_t slot_next = row_slot(tile + stride);
+        // features x[rr][24 c + 2 f + {sin, cos}]; rows past the end get zero features
 #pragma unroll
-                for (int f = 0; f < kGuideFreq; ++f) {
-                    o[f] = bf16x2{(__bf16)st.s, (__bf16)st.c};
-                    freq_next(st);
+        for (int k = 0; k < 3; ++k) {
+            const int q = lane + 64 * k, rr = q / kGuideIn, c = q - rr * kGuideIn;
+            const int32_t src = __shfl(slot, rr & 15);   // every lane takes part in the shuffle
+            if (q < 16 * kGuideIn) {
+                bf16x2 *o = reinterpret_cast<bf16x2 *>(ft + rr * kLd1 + 2 * kGuideFreq * c);
+                if (src >= 0) {
+                    FreqState st = freq_base(xin[k]);
+#pragma unroll
+                    for (int f = 0; f < kGuideFreq; ++f) {
+                        o[f] = bf16x2{(__bf16)st.s, (__bf16)st.c};
+                        freq_next(st);
+                    }
+                } else {
+#pragma unroll
+                    for (int f = 0; f < kGuideFreq; ++f) o[f] = bf16x2{(__bf16)0.0f, (__bf16)0.0f};
                 }
-            } else {
-#pragma unroll
-                for (int f = 0; f < kGuideFreq; ++f) o[f] = bf16x2{(__bf16)0.0f, (__bf16)0.0f};
             }
         }
         if (lane < 16) {
@@ -283,6 +307,7 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(InferArgs a) {
         wave_sync();
         f32x4 acc[kGuideH / 16];
         tile_mfma_t<kGuideH / 16, kGuideK1, kLd1>(ft, w1, lr, lq, acc);
+        load_inputs(slot_next);
         wave_sync();
         store_hidden(ft, acc, b1, lr, lq);
         wave_sync();
@@ -300,11 +325,17 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(InferArgs a) {
         f32x4 out[kGuideNOutPad16 / 16];
         tile_mfma_t<kGuideNOutPad16 / 16, kGuideH, kLdH>(ft, w4, lr, lq, out);
         wave_sync();
+        // the lane's row of raw: its slot, or, without a slot list, the tile row itself
+        // also past the end (the buffers hold whole tiles)
+        const int64_t dst = a.slot_of ? (int64_t)slot : tile * 16 + lr;
+        if (dst >= 0) {
 #pragma unroll
-        for (int t = 0; t < kGuideNOutPad16 / 16; ++t)
-            *reinterpret_cast<float4 *>(a.raw + (tile * 16 + lr) * kGuideNOutPad16 + 16 * t + 4 * lq) =
-                make_float4(out[t][0] + b4[16 * t + 4 * lq], out[t][1] + b4[16 * t + 4 * lq + 1],
-                            out[t][2] + b4[16 * t + 4 * lq + 2], out[t][3] + b4[16 * t + 4 * lq + 3]);
+            for (int t = 0; t < kGuideNOutPad16 / 16; ++t)
+                *reinterpret_cast<float4 *>(a.raw + dst * kGuideNOutPad16 + 16 * t + 4 * lq) =
+                    make_float4(out[t][0] + b4[16 * t + 4 * lq], out[t][1] + b4[16 * t + 4 * lq + 1],
+                                out[t][2] + b4[16 * t + 4 * lq + 2], out[t][3] + b4[16 * t + 4 * lq + 3]);
+        }
+        slot = slot_next;
     }
 }
 
@@ -1020,22 +1051,32 @@ tt_status guide_alloc_batch(tt_guide *g, int64_t rows) {
     return st;
 }
 
-// Inference rows of the renderer: 228 B each (inputs and mixture), so a render can
-// keep as many paths in flight as the tracer itself; the training batch above
-// (about 1.6 KB a row) is sized by the Adam batch alone.
+// Inference rows of the renderer: about 232 B each (inputs, mixture, flag bit and slot
+// index), so a render can keep as many paths in flight as the tracer itself; the
+// training batch above (about 1.6 KB a row) is sized by the Adam batch alone. Rows are
+// counted in whole 256-slot steps of the guide-row kernels (each step's 4 flag words are
+// all written), which run at most 16 blocks of 256 threads per CU.
 tt_status guide_alloc_infer(tt_guide *g, int64_t rows) {
     if (g->icap >= rows) return TT_OK;
-    if (g->iin) hipFree(g->iin);
-    if (g->iraw) hipFree(g->iraw);
-    if (g->irow) hipFree(g->irow);
+    void *old[] = {g->iin, g->iraw, g->imask, g->iblock_n, g->islot};
+    for (void *q : old)
+        if (q) hipFree(q);
     g->iin = g->iraw = nullptr;
-    g->irow = nullptr;
-    g->icap = 0;
-    const int64_t R = (rows + 15) & ~15LL;
+    g->imask = nullptr;
+    g->iblock_n = g->islot = nullptr;
+    g->icap = g->iblocks = 0;
+    const int64_t R = (rows + 255) & ~255LL;
+    if (R >= INT32_MAX) return fail(TT_ERR_INVALID, "guide inference: too many queue slots");
+    const int64_t blocks = 16LL * std::max(1, g->ctx->num_cus);
     tt_status st = dev_alloc(g->ctx, (void **)&g->iin, (size_t)R * kGuideIn * 4, "guide inference inputs");
     if (st == TT_OK) st = dev_alloc(g->ctx, (void **)&g->iraw, (size_t)R * kGuideNOutPad16 * 4, "guide inference outputs");
-    if (st == TT_OK) st = dev_alloc(g->ctx, (void **)&g->irow, (size_t)R * 4, "guide inference row flags");
-    if (st == TT_OK) g->icap = R;
+    if (st == TT_OK) st = dev_alloc(g->ctx, (void **)&g->imask, (size_t)R / 8, "guide inference row flags");
+    if (st == TT_OK) st = dev_alloc(g->ctx, (void **)&g->iblock_n, (size_t)(blocks + 1) * 4, "guide inference row counts");
+    if (st == TT_OK) st = dev_alloc(g->ctx, (void **)&g->islot, (size_t)R * 4, "guide inference row slots");
+    if (st == TT_OK) {
+        g->icap = R;
+        g->iblocks = blocks;
+    }
     return st;
 }
 
@@ -1077,10 +1118,10 @@ hipError_t guide_forward(tt_guide *g, int64_t n, const int32_t *n_dev, hipStream
 // Inference over the first n rows of g->batch.in (device count n_dev optional):
 // the fused kernel, raw outputs (fp32, 48 wide) into g->batch.raw.
 hipError_t guide_infer(tt_guide *g, const float *in, float *raw, int64_t n, const int32_t *n_dev, hipStream_t s,
-                       const int32_t *live, bool store_acts) {
+                       const int32_t *slot_of, bool store_acts) {
     InferArgs a{};
     a.in = in;
-    a.live = live;
+    a.slot_of = slot_of;
     if (store_acts)   // the training forward pass: hidden activations into the batch (the
                       // first layer's weight gradient recomputes the features)
         for (int k = 0; k < 3; ++k) a.h_out[k] = g->batch.h[k];
@@ -1328,7 +1369,7 @@ void tt_guide_destroy(tt_guide *g) {
     void *ptrs[] = {g->params, g->m1, g->m2, g->grad, g->scalars, g->partial, b.in, b.x, b.h[0], b.h[1], b.h[2],
                     b.raw, b.draw, b.dh[0], b.dh[1], b.dh[2], b.dirs, b.wts, b.pb, b.lpart, g->wf[0], g->wf[1], g->wf[2], g->wf[3],
                     g->wb[0], g->wb[1], g->wb[2], g->wb[3], g->train_buf, g->train_off, g->rec, g->iin, g->iraw,
-                    g->irow};
+                    g->imask, g->iblock_n, g->islot};
     for (void *q : ptrs)
         if (q) hipFree(q);
     delete g;

@@ -142,10 +142,12 @@ __device__ __forceinline__ void mixture_sample(const Mix &L, float u0, float u1,
 }
 
 hipError_t guide_forward(tt_guide *g, int64_t n, const int32_t *n_dev, hipStream_t s);
-// live (may be null): per-row flags; 16-row tiles with no flagged row are skipped.
-// store_acts: also write the features and hidden activations into g->batch (training)
+// slot_of (may be null): row r of the inference reads inputs and writes outputs at
+// row slot_of[r] of in and raw (a compacted list of the rows needed); rows up to
+// min(*n_dev, n). store_acts: also write the features and hidden activations into
+// g->batch (training)
 hipError_t guide_infer(tt_guide *g, const float *in, float *raw, int64_t n, const int32_t *n_dev, hipStream_t s,
-                       const int32_t *live = nullptr, bool store_acts = false);
+                       const int32_t *slot_of = nullptr, bool store_acts = false);
 hipError_t guide_train_rows(tt_guide *g, int64_t n, hipStream_t s);
 tt_status guide_alloc_batch(tt_guide *g, int64_t rows);
 tt_status guide_alloc_infer(tt_guide *g, int64_t rows);
@@ -164,8 +166,10 @@ struct tt_guide {
     int64_t cap = 0;
     // renderer inference rows (one per queued path): inputs [icap][9], mixtures [icap][48]
     float *iin = nullptr, *iraw = nullptr;
-    int32_t *irow = nullptr;    // renderer: 1 where a queue slot's vertex is guided (its row is inferred)
-    int64_t icap = 0;
+    uint64_t *imask = nullptr;    // one bit per queue slot: its vertex is guided (its row is inferred)
+    int32_t *iblock_n = nullptr;  // guided slots of each block of the guide-row kernels; [iblocks]: the row total
+    int32_t *islot = nullptr;     // the guided slots in slot order (the inference's rows)
+    int64_t icap = 0, iblocks = 0;
     float lr = 1e-3f, beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
     int64_t step = 0;
     // online training from the renderer

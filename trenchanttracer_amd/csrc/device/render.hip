@@ -956,8 +956,10 @@ struct GuideInArgs {
     const tt_hit *hits;
     const int32_t *counts;
     int32_t cur;
-    float *in;        // [slot][9]
-    int32_t *live;    // [slot]: 1 if the vertex is guided (its mixture is needed)
+    float *in;          // [slot][9], written for the guided slots
+    uint64_t *mask;     // [slot / 64]: bit slot % 64 set where the vertex is guided
+    int32_t *block_n;   // [block]: guided slots of each block's span
+    const int32_t *given;   // check hook only: the guided flag of each slot (inputs already in in)
 };
 
 // The vertices the guide samples at: diffuse surfaces and conductors at least
@@ -966,33 +968,118 @@ __device__ __forceinline__ bool guide_eligible(const tt_material &m, float min_r
     return m.type == TT_MAT_DIFFUSE || (m.type == TT_MAT_CONDUCTOR && m.roughness >= min_rough);
 }
 
-// Network inputs of the hit queue, and a flag per slot for the vertices the guide
-// serves: inference skips every 16-row tile without a flagged row. Paths are
-// pixel-major, so a pixel's samples (all reaching the sky, or all one glass facet)
-// share their flag and whole tiles drop out: on the 1080p path-trace case most of the
-// queue (sky, glass, smooth conductors).
+// Whether slot i's vertex is guided; if so, its network inputs into a.in[i].
+__device__ __forceinline__ bool guide_vertex_inputs(const GuideInArgs &a, int64_t i) {
+    const tt_hit h = a.hits[i];
+    if (h.tri_id == -1) return false;
+    const tt_ray r = a.rays[i];
+    const f3 rd = F3(r.dx, r.dy, r.dz), wo = rd * -1.0f;
+    const f3 point = F3(r.ox, r.oy, r.oz) + rd * h.t;
+    const Surf sf = surface_path(a.sv, h, point);
+    if (!guide_eligible(a.sv.mats[sf.mat], a.gd.min_rough)) return false;
+    const f3 ng = sf.ng;
+    const f3 ngf = dot(wo, ng) > 0.0f ? ng : ng * -1.0f;
+    f3 ns = sf.analytic ? ngf : vnorm(sf.nrm);
+    if (dot(ns, ngf) < 0.0f) ns = ns * -1.0f;
+    guide_inputs(a.gd, point, ns, wo, a.in + i * kGuideIn);
+    return true;
+}
+
+// The queue split between the blocks of the guide-row kernels (k_guide_inputs and
+// k_guide_slots run the same grid): block b takes slots [b span, (b + 1) span), a span
+// of whole 256-slot steps, so a short queue still gets a thread per slot.
+__device__ __forceinline__ int64_t guide_span(int32_t n) {
+    const int64_t steps = ((int64_t)n + 255) / 256;
+    return (steps + gridDim.x - 1) / gridDim.x * 256;
+}
+
+// Network inputs of the hit queue for the vertices the guide serves, a flag bit per
+// slot (one ballot word per 64 slots) and each block's number of guided slots.
+// k_guide_slots turns the flags into the list of guided slots in slot order, and the
+// inference runs over that list alone: on the 1080p path-trace case only about a third
+// of the queue is guided (sky, glass and smooth conductors are not), and skipping only
+// the 16-row tiles without a guided row still left 41% of the inferred rows unneeded.
+// GIVEN (tt_debug_guide_infer_slots): the flags are given per slot instead.
+template <bool GIVEN>
 __global__ __launch_bounds__(256) void k_guide_inputs(GuideInArgs a) {
+    __shared__ int32_t wc[4];
     const int32_t n = a.counts[a.cur];
-    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const tt_hit h = a.hits[i];
-        bool take = false;
-        if (h.tri_id != -1) {
-            const tt_ray r = a.rays[i];
-            const f3 rd = F3(r.dx, r.dy, r.dz), wo = rd * -1.0f;
-            const f3 point = F3(r.ox, r.oy, r.oz) + rd * h.t;
-            const Surf sf = surface_path(a.sv, h, point);
-            take = guide_eligible(a.sv.mats[sf.mat], a.gd.min_rough);
-            if (take) {
-                const f3 ng = sf.ng;
-                const f3 ngf = dot(wo, ng) > 0.0f ? ng : ng * -1.0f;
-                f3 ns = sf.analytic ? ngf : vnorm(sf.nrm);
-                if (dot(ns, ngf) < 0.0f) ns = ns * -1.0f;
-                guide_inputs(a.gd, point, ns, wo, a.in + (int64_t)i * kGuideIn);
-            }
+    const int64_t span = guide_span(n);
+    const int64_t lo = (int64_t)blockIdx.x * span, hi = min((int64_t)n, lo + span);
+    const int wave = threadIdx.x >> 6, lane = __lane_id();
+    int32_t cnt = 0;
+    for (int64_t base = lo; base < hi; base += 256) {
+        const int64_t i = base + threadIdx.x;
+        const bool take = i < hi && (GIVEN ? a.given[i] != 0 : guide_vertex_inputs(a, i));
+        const uint64_t m = __ballot(take);
+        if (lane == 0) {
+            a.mask[i >> 6] = m;
+            cnt += __popcll(m);
         }
-        if (!take)   // unflagged rows of a tile that is inferred still read defined inputs
-            for (int k = 0; k < kGuideIn; ++k) a.in[(int64_t)i * kGuideIn + k] = 0.0f;
-        a.live[i] = take ? 1 : 0;
+    }
+    if (lane == 0) wc[wave] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) a.block_n[blockIdx.x] = wc[0] + wc[1] + wc[2] + wc[3];
+}
+
+// One block of 1024 threads: the counts off[0, nb) -> exclusive offsets in place (fixed
+// order); *total = their sum.
+__device__ __forceinline__ void scan_counts(int32_t *off, int64_t nb, int32_t *total) {
+    __shared__ int32_t sh[1024];
+    const int t = threadIdx.x;
+    const int64_t per = (nb + 1023) / 1024;
+    const int64_t lo = min(nb, (int64_t)t * per), hi = min(nb, lo + per);
+    int32_t sum = 0;
+#pragma unroll 8
+    for (int64_t i = lo; i < hi; ++i) sum += off[i];
+    sh[t] = sum;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const int32_t y = t >= o ? sh[t - o] : 0;
+        __syncthreads();
+        sh[t] += y;
+        __syncthreads();
+    }
+    int32_t run = sh[t] - sum;
+    for (int64_t i = lo; i < hi; ++i) {
+        const int32_t c = off[i];
+        off[i] = run;
+        run += c;
+    }
+    if (t == 1023) *total = sh[1023];
+}
+
+// The guided slots in slot order: slot_of[r] = the queue slot of inference row r. Each
+// block adds the guided counts of the blocks before it (at most 16 per CU, in a fixed
+// order) for its first row, then walks its span 256 slots at a time; the last block
+// writes the row total.
+__global__ __launch_bounds__(256) void k_guide_slots(const uint64_t *mask, const int32_t *block_n,
+                                                     const int32_t *n_dev, int32_t *slot_of, int32_t *total) {
+    __shared__ int32_t part[4];
+    __shared__ int32_t wc[2][4];
+    const int32_t n = *n_dev;
+    const int64_t span = guide_span(n);
+    const int64_t lo = (int64_t)blockIdx.x * span, hi = min((int64_t)n, lo + span);
+    const int wave = threadIdx.x >> 6, lane = __lane_id();
+    int32_t pre = 0;
+    for (int b = threadIdx.x; b < (int)blockIdx.x; b += 256) pre += block_n[b];
+    for (int o = 32; o > 0; o >>= 1) pre += __shfl_xor(pre, o);
+    if (lane == 0) part[wave] = pre;
+    __syncthreads();
+    int32_t run = part[0] + part[1] + part[2] + part[3];
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total = run + block_n[blockIdx.x];
+    int par = 0;
+    // wc alternates between steps: a block's next writes go to the other half, and the
+    // one after that follows the barrier every reader of this half has passed
+    for (int64_t base = lo; base < hi; base += 256, par ^= 1) {
+        const int64_t i = base + threadIdx.x;
+        const uint64_t m = mask[i >> 6];
+        if (lane == 0) wc[par][wave] = __popcll(m);
+        __syncthreads();
+        int32_t r = run;
+        for (int w = 0; w < wave; ++w) r += wc[par][w];
+        if ((m >> lane) & 1ull) slot_of[r + __popcll(m & ((1ull << lane) - 1ull))] = (int32_t)i;
+        run += wc[par][0] + wc[par][1] + wc[par][2] + wc[par][3];
     }
 }
 
@@ -1414,30 +1501,7 @@ __global__ __launch_bounds__(256) void k_collect_count(const float *rec, int64_t
 }
 
 // One block: off[0, nb) block counts -> exclusive row offsets; off[nb] = the row total.
-__global__ __launch_bounds__(1024) void k_collect_scan(int32_t *off, int64_t nb) {
-    __shared__ int32_t sh[1024];
-    const int t = threadIdx.x;
-    const int64_t per = (nb + 1023) / 1024;
-    const int64_t lo = min(nb, (int64_t)t * per), hi = min(nb, lo + per);
-    int32_t sum = 0;
-#pragma unroll 8
-    for (int64_t i = lo; i < hi; ++i) sum += off[i];
-    sh[t] = sum;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-        const int32_t y = t >= o ? sh[t - o] : 0;
-        __syncthreads();
-        sh[t] += y;
-        __syncthreads();
-    }
-    int32_t run = sh[t] - sum;
-    for (int64_t i = lo; i < hi; ++i) {
-        const int32_t c = off[i];
-        off[i] = run;
-        run += c;
-    }
-    if (t == 1023) off[nb] = sh[1023];
-}
+__global__ __launch_bounds__(1024) void k_collect_scan(int32_t *off, int64_t nb) { scan_counts(off, nb, off + nb); }
 
 __global__ __launch_bounds__(256) void k_collect_write(float *rec, int64_t ne, const int32_t *off, float *buf,
                                                        int32_t cap) {
@@ -1733,12 +1797,18 @@ hipError_t launch_bounce(tt_scene *s, RenderState *rs, const tt_render_params *p
     const bool guide_here = gr && (gr->max_bounce <= 0 || b < gr->max_bounce);
     const bool records = gr && gr->gd.rec != nullptr;
     if (guide_here) {
-        // guide inference over the hit queue: inputs and flags, then the MLP (rows up
-        // to the device count, tiles of unflagged rows skipped)
-        GuideInArgs ga{gr->gd, sv, rs->rays[cur], rs->hits, rs->counts, cur, gr->g->iin, gr->g->irow};
-        hipLaunchKernelGGL(k_guide_inputs, dim3(gblocks), dim3(256), 0, hs, ga);
+        // guide inference over the hit queue: inputs, flags and counts, the guided slots
+        // in slot order, then the MLP over that list
+        tt_guide *g = gr->g;
+        if ((int64_t)cap > g->icap) return hipErrorInvalidValue;   // the inference rows cover the queue
+        const unsigned rblocks = (unsigned)std::min<int64_t>(((int64_t)cap + 255) / 256, g->iblocks);
+        int32_t *rows_dev = g->iblock_n + g->iblocks;
+        GuideInArgs ga{gr->gd, sv, rs->rays[cur], rs->hits, rs->counts, cur, g->iin, g->imask, g->iblock_n, nullptr};
+        hipLaunchKernelGGL(k_guide_inputs<false>, dim3(rblocks), dim3(256), 0, hs, ga);
+        hipLaunchKernelGGL(k_guide_slots, dim3(rblocks), dim3(256), 0, hs, (const uint64_t *)g->imask,
+                           (const int32_t *)g->iblock_n, rs->counts + cur, g->islot, rows_dev);
         e = hipGetLastError();
-        if (e == hipSuccess) e = guide_infer(gr->g, gr->g->iin, gr->g->iraw, cap, rs->counts + cur, hs, gr->g->irow);
+        if (e == hipSuccess) e = guide_infer(g, g->iin, g->iraw, cap, rows_dev, hs, g->islot);
         if (e != hipSuccess) return e;
         sa.gd = gr->gd;
         sa.gd.raw = gr->g->iraw;
@@ -2451,6 +2521,56 @@ tt_status tt_debug_math(tt_ctx *ctx, int32_t op, const float *x, const float *y,
     hipLaunchKernelGGL(k_debug_math, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, op, x, y, out, n);
     TT_HIPR(hipGetLastError(), "debug math launch");
     TT_HIPR(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return TT_OK;
+}
+
+tt_status tt_debug_guide_infer_slots(tt_guide *g, const float *inputs, const int32_t *guided, int64_t n,
+                                     float *raw_out, int64_t *rows) {
+    if (!g || !inputs || !guided || !raw_out || !rows || n <= 0 || n > (1LL << 24))
+        return fail(TT_ERR_INVALID, "tt_debug_guide_infer_slots: bad arguments");
+    TT_HIPR(hipSetDevice(g->ctx->device), "hipSetDevice");
+    tt_status st = guide_alloc_infer(g, n);
+    if (st != TT_OK) return st;
+    int32_t *d_given = nullptr;
+    st = dev_alloc(g->ctx, (void **)&d_given, (size_t)n * 4 + 256, "debug guide flags");
+    if (st != TT_OK) return st;
+    int32_t *d_n = d_given + ((n + 63) & ~63LL);
+    const int32_t n32 = (int32_t)n;
+    hipStream_t hs = g->ctx->stream;
+    hipError_t e = hipMemcpyAsync(g->iin, inputs, (size_t)n * kGuideIn * 4, hipMemcpyHostToDevice, hs);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_given, guided, (size_t)n * 4, hipMemcpyHostToDevice, hs);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_n, &n32, 4, hipMemcpyHostToDevice, hs);
+    // every output word starts as 0xFFFFFFFF: the rows the inference skips keep it
+    if (e == hipSuccess) e = hipMemsetAsync(g->iraw, 0xFF, (size_t)n * kGuideNOutPad16 * 4, hs);
+    int32_t *rows_dev = g->iblock_n + g->iblocks;
+    if (e == hipSuccess) {
+        const unsigned rblocks = (unsigned)std::min<int64_t>((n + 255) / 256, g->iblocks);
+        GuideInArgs ga{};
+        ga.counts = d_n;
+        ga.cur = 0;
+        ga.in = g->iin;
+        ga.mask = g->imask;
+        ga.block_n = g->iblock_n;
+        ga.given = d_given;
+        hipLaunchKernelGGL(k_guide_inputs<true>, dim3(rblocks), dim3(256), 0, hs, ga);
+        hipLaunchKernelGGL(k_guide_slots, dim3(rblocks), dim3(256), 0, hs, (const uint64_t *)g->imask,
+                           (const int32_t *)g->iblock_n, (const int32_t *)d_n, g->islot, rows_dev);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = guide_infer(g, g->iin, g->iraw, n, rows_dev, hs, g->islot);
+    std::vector<float> pad;
+    int32_t r32 = 0;
+    if (e == hipSuccess) {
+        pad.resize((size_t)n * kGuideNOutPad16);
+        e = hipMemcpyAsync(pad.data(), g->iraw, pad.size() * 4, hipMemcpyDeviceToHost, hs);
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&r32, rows_dev, 4, hipMemcpyDeviceToHost, hs);
+    if (e == hipSuccess) e = hipStreamSynchronize(hs);
+    hipFree(d_given);
+    if (e != hipSuccess) return fail(TT_ERR_DEVICE, std::string("tt_debug_guide_infer_slots: ") + hipGetErrorString(e));
+    for (int64_t r = 0; r < n; ++r)
+        std::memcpy(raw_out + r * kGuideNOut, pad.data() + r * kGuideNOutPad16, kGuideNOut * 4);
+    *rows = r32;
     return TT_OK;
 }
 
