@@ -379,7 +379,7 @@ typedef struct tt_render_params {
     float tmin;               /* reference mode ray tmin (0.01, RenderKernel.cu:110) */
     int32_t use_graph;        /* capture one sample pass in a hipGraph and replay it */
     int32_t paths_in_flight;  /* target paths per pass, rounded to whole samples (0 = up to
-                                 2^27 paths, 184 B of state each, 412 B with a guide;
+                                 2^27 paths, 184 B of state each, 416 B with a guide;
                                  guided renders that train the network: about 8M) */
     int32_t lds_stack;        /* traversal LDS stack entries per lane (0 = 16) */
     int32_t block;            /* traversal block size (0 = 64) */
