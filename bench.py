@@ -444,14 +444,25 @@ def render_bench(tt, ctx, np, no_cpu):
     return out
 
 
+# Guided vertices of the path-trace case: the camera bounce's alone. Guiding deeper
+# vertices too costs inference on every bounce's queue and, on this scene, does not lower
+# the error: with 4 seeds per point and the same protocol, guiding the first 1 / 2 / all
+# bounces reached the plain 64-spp RMSE in 1.38 / 1.43 / 1.56x the plain time trained
+# online and 1.00 / 1.05 / 1.12x frozen. (The door room, lit through an opening, gains
+# from every bounce: the library's default guides all of them.)
+GUIDED_BOUNCES = 1
+
+
 def guided_bench(tt, ctx, np):
     """Learned guiding on the path-trace case (1,310,720 tris, 1920x1080, depth 16):
     equal-time RMSE against an FP64-accumulated 8192-spp render of the plain tracer.
     Plain renders at 64 and 128 spp; guided renders (8-lobe vMF MLP, BF16 MFMA, Adam
     lr 1e-3 on 2^18-row batches, one-sample MIS with the BSDF, the selection
-    probability learned per vertex) trained online from scratch throughout (passes
-    growing 8x from 1 spp, up to 2M recorded paths each), trained during the first
-    half of the samples, and frozen after the 128-spp online run; two seeds per point.
+    probability learned per vertex, the mixture used at the camera bounce's vertices:
+    GUIDED_BOUNCES) trained online from scratch throughout (passes growing 8x from
+    1 spp, up to 2M recorded paths each; every bounce recorded), trained during the
+    first half of the samples, and frozen after the 128-spp online run; two seeds per
+    point.
     The time at which each reaches the plain 64-spp RMSE comes from a log-log fit of
     RMSE against time (trenchanttracer_amd/bench_guide.py, which also runs the longer
     sweep)."""
@@ -463,13 +474,13 @@ def guided_bench(tt, ctx, np):
                                                  fp64=True), want_fp64=True)
     log(f"guided leg: FP64 8192-spp oracle render in {time.time() - t0:.1f}s")
     r = bench_guide.compare(tt, ctx, sc, cam, ref, 16, (64, 128), (64, 128), 1920 * 1080, 1e-3, 262144, 2 << 20,
-                            seeds=2, pass_growth=8, learn_alpha=True)
+                            seeds=2, pass_growth=8, learn_alpha=True, max_bounce=GUIDED_BOUNCES)
     sc.close()
     p64 = next(x for x in r["plain"] if x["spp"] == 64)
     g64 = next(x for x in r["guided_online"] if x["spp"] == 64)
     f64 = next(x for x in r["guided_pretrained"] if x["spp"] == 64)
     out = {
-        "oracle_spp": 8192, "seeds_per_point": 2,
+        "oracle_spp": 8192, "seeds_per_point": 2, "guided_bounces": GUIDED_BOUNCES,
         "plain_64spp": {"rmse": round(p64["rmse"], 5), "seconds": round(p64["seconds"], 4)},
         "wall_ratio_to_plain_64spp_rmse": {
             "online": round(r["guided_online_wall_ratio"], 3),

@@ -110,7 +110,8 @@ def compare(tt, ctx, sc, cam, ref, depth, plain_spp, guided_spp, paths, lr, batc
         runs = []
         for j in range(seeds):
             img, st, _ = g.render(sc, cam, rp(spp, 3000 * spp + j), alpha=0.5, train=False,
-                                  learn_alpha=guide_kw.get("learn_alpha", False))
+                                  learn_alpha=guide_kw.get("learn_alpha", False),
+                                  max_bounce=guide_kw.get("max_bounce", 0))
             runs.append(err(img) + (st["seconds"],))
         out["guided_pretrained"].append(point(spp, runs))
         log(f"guided pretrained {spp}: {out['guided_pretrained'][-1]}")
