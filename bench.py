@@ -445,11 +445,11 @@ def render_bench(tt, ctx, np, no_cpu):
 
 
 # Guided vertices of the path-trace case: the camera bounce's alone. Guiding deeper
-# vertices too costs inference on every bounce's queue and, on this scene, does not lower
-# the error: with 4 seeds per point and the same protocol, guiding the first 1 / 2 / all
-# bounces reached the plain 64-spp RMSE in 1.38 / 1.43 / 1.56x the plain time trained
-# online and 1.00 / 1.05 / 1.12x frozen. (The door room, lit through an opening, gains
-# from every bounce: the library's default guides all of them.)
+# vertices too costs inference on every bounce's queue and, on this scene, buys little or
+# nothing: with 4 seeds per point and the same protocol (bench_guide --sweep bounces),
+# guiding the first 1 / 2 / all bounces reached the plain 64-spp RMSE in 1.41 / 1.44 /
+# 1.53x the plain time trained online and 1.00 / 1.01 / 1.13x frozen. (The door room, lit
+# through an opening, gains from every bounce: the library's default guides all of them.)
 GUIDED_BOUNCES = 1
 
 

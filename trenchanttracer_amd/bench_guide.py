@@ -20,7 +20,7 @@ the camera's part of the room through a door, with a 4096-spp oracle.
 
     python -m trenchanttracer_amd.bench_guide [--oracle-spp 8192] [--seeds 4] [--skip-pathtrace] [--quick]
     python -m trenchanttracer_amd.bench_guide --error-classes
-    python -m trenchanttracer_amd.bench_guide --sweep [--seeds 2]
+    python -m trenchanttracer_amd.bench_guide --sweep [schedules|bounces] [--seeds 2]
 
 --error-classes: where the path-trace case's error sits. Plain 64-spp renders (seeds
 100, 101) against a 4096-spp FP64 oracle, pixels classed by the material the centre
@@ -28,9 +28,10 @@ ray of the pixel hits first (diffuse, GGX conductor, glass, sky): share of pixel
 of the squared error, and the classes of the 400 worst pixels.
 
 --sweep: the path-trace case at 64 and 128 spp under the training schedules of
-SWEEP (pass growth 2 or 4, fixed or learned selection probability; 2^21 training
-paths, one path per pixel per pass): the wall ratios to the plain 64-spp RMSE and the
-64-spp points of each.
+SWEEP (pass growth 2 to 16, fixed or learned selection probability; 2^21 training
+paths, one path per pixel per pass), or with `--sweep bounces` under the bench's
+schedule with the mixture used at the first 1, 2 or all bounces (BOUNCE_SWEEP): the
+wall ratios to the plain 64-spp RMSE and the 64-spp points of each.
 """
 import argparse
 import json
@@ -177,17 +178,21 @@ SWEEP = (("growth2_fixed_alpha", dict(pass_growth=2)),
          ("growth8_learned_alpha", dict(pass_growth=8, learn_alpha=True)),
          ("growth16_learned_alpha", dict(pass_growth=16, learn_alpha=True)))
 
+# how many bounces use the mixture (max_bounce; 0: every bounce), under the bench's schedule
+BOUNCE_SWEEP = tuple((f"guided_bounces_{b or 'all'}", dict(pass_growth=8, learn_alpha=True, max_bounce=b))
+                     for b in (0, 1, 2))
 
-def sweep(tt, ctx, oracle_spp, seeds, log):
-    """The path-trace case under each training schedule of SWEEP, every one compared
-    with the same plain renders and FP64 oracle."""
+
+def sweep(tt, ctx, oracle_spp, seeds, log, cases=SWEEP):
+    """The path-trace case under each setting of `cases` (SWEEP: training schedules;
+    BOUNCE_SWEEP: guided bounces), every one compared with the same FP64 oracle."""
     from trenchanttracer_amd import workloads
     mesh, mats, env, cam = workloads.pathtrace()
     sc = tt.DeviceScene(ctx, mesh, tt.BVH.build(mesh, nthreads=16), mats, env=env)
     _, ref, _ = sc.render(cam, tt.render_params(spp=oracle_spp, max_depth=16, rr_depth=4, seed=1234,
                                                 mode=tt.RENDER_PATH, fp64=True), want_fp64=True)
     out = {}
-    for name, kw in SWEEP:
+    for name, kw in cases:
         r = compare(tt, ctx, sc, cam, ref, 16, (64, 128), (64, 128), cam.width * cam.height, 1e-3, 262144,
                     2 << 20, seeds, **kw)
         out[name] = {k: v for k, v in r.items() if k.endswith("ratio")}
@@ -204,7 +209,8 @@ def main():
     ap.add_argument("--skip-pathtrace", action="store_true", help="door room only")
     ap.add_argument("--quick", action="store_true", help="door room only, one training run (for profiling)")
     ap.add_argument("--error-classes", action="store_true", help="where the path-trace case's error sits")
-    ap.add_argument("--sweep", action="store_true", help="path-trace case under the training schedules of SWEEP")
+    ap.add_argument("--sweep", nargs="?", const="schedules", choices=("schedules", "bounces"),
+                    help="path-trace case under the training schedules of SWEEP or the guided bounces of BOUNCE_SWEEP")
     args = ap.parse_args()
     import trenchanttracer_amd as tt
     from trenchanttracer_amd import workloads
@@ -216,9 +222,11 @@ def main():
         ctx.close()
         return
     if args.sweep:
-        print(json.dumps({"metric": "path-trace case: wall ratio to the plain 64-spp RMSE by training schedule",
+        by = "training schedule" if args.sweep == "schedules" else "guided bounces"
+        cases = SWEEP if args.sweep == "schedules" else BOUNCE_SWEEP
+        print(json.dumps({"metric": f"path-trace case: wall ratio to the plain 64-spp RMSE by {by}",
                           "oracle_spp": args.oracle_spp, "seeds": args.seeds,
-                          "pathtrace_1080p": sweep(tt, ctx, args.oracle_spp, args.seeds, log)}), flush=True)
+                          "pathtrace_1080p": sweep(tt, ctx, args.oracle_spp, args.seeds, log, cases)}), flush=True)
         ctx.close()
         return
     result = {"metric": "equal-time RMSE vs FP64 oracle, guided vs plain"}
