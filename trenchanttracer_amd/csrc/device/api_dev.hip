@@ -122,10 +122,16 @@ tt_status tt_ctx_create(int32_t device, tt_ctx **out) {
         return fail(TT_ERR_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
     }
     e = hipEventCreateWithFlags(&c->reorder_done, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->shade_done, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->shadow_done, hipEventDisableTiming);
     if (e != hipSuccess) {
+        if (c->shade_done) hipEventDestroy(c->shade_done);
+        if (c->aux) hipStreamDestroy(c->aux);
+        if (c->reorder_done) hipEventDestroy(c->reorder_done);
         hipStreamDestroy(c->stream);
         delete c;
-        return fail(TT_ERR_DEVICE, std::string("hipEventCreate: ") + hipGetErrorString(e));
+        return fail(TT_ERR_DEVICE, std::string("tt_ctx_create: ") + hipGetErrorString(e));
     }
     scratch_setup(device);
     *out = c;
@@ -136,9 +142,13 @@ void tt_ctx_destroy(tt_ctx *ctx) {
     if (!ctx) return;
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
+    hipStreamSynchronize(ctx->aux);
     if (ctx->scratch) hipFree(ctx->scratch);
     if (ctx->reorder_ws) hipFree(ctx->reorder_ws);
     if (ctx->reorder_done) hipEventDestroy(ctx->reorder_done);
+    if (ctx->shade_done) hipEventDestroy(ctx->shade_done);
+    if (ctx->shadow_done) hipEventDestroy(ctx->shadow_done);
+    hipStreamDestroy(ctx->aux);
     hipStreamDestroy(ctx->stream);
     delete ctx;
 }

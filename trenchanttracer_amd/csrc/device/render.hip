@@ -47,7 +47,7 @@ struct RenderState {
     tt_ray *srays = nullptr;     // next-event rays
     float4 *scontrib = nullptr;  // contribution (rgb) and pixel (w, as int bits)
     int32_t *socc = nullptr;     // any-hit flags
-    int32_t *counts = nullptr;   // [0], [1] queues, [2] next-event queue
+    int32_t *counts = nullptr;   // [0], [1] queues, [2], [3] next-event queues of even and odd bounces
     unsigned long long *stats = nullptr;  // [0] extension rays, [1] shadow rays
     float *acc = nullptr;        // per pixel rgb
     double *acc64 = nullptr;     // per pixel rgb (FP64 accumulation)
@@ -467,6 +467,7 @@ struct ShadeArgs {
     float4 *thr, *L;
     int32_t *counts;
     int32_t cur, nxt;
+    int32_t scount;    // counts[scount]: this bounce's next-event queue
     tt_ray *srays;
     float4 *scontrib;
     float *acc;
@@ -1297,7 +1298,7 @@ __device__ __forceinline__ void shade_path_one(const ShadeArgs &a, const int32_t
         a.L[p] = Ls;   // the path's own slot: no atomics; k_resolve sums a pixel's paths
     }
     int slot, sslot;
-    block_append2(cont, &a.counts[a.nxt], shadow, &a.counts[2], &slot, &sslot);
+    block_append2(cont, &a.counts[a.nxt], shadow, &a.counts[a.scount], &slot, &sslot);
     if (cont) {
         a.rays_out[slot] = nr;
         a.path_out[slot] = p;
@@ -1320,7 +1321,7 @@ __global__ __launch_bounds__(kAppendBlock) void k_shade_path(ShadeArgs a) {
 }
 
 struct ApplyArgs {
-    const int32_t *counts;
+    const int32_t *n_dev;   // the bounce's next-event queue length
     const int32_t *occ;
     const float4 *contrib;
     float4 *L;           // per-path radiance in yzw
@@ -1330,7 +1331,7 @@ struct ApplyArgs {
 };
 
 __global__ __launch_bounds__(256) void k_apply_shadow(ApplyArgs a) {
-    const int32_t n = a.counts[2];
+    const int32_t n = *a.n_dev;
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&a.stats[1], (unsigned long long)n);
     for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         if (a.occ[i]) continue;
@@ -1752,9 +1753,18 @@ tt_status ensure_state(tt_scene *s, int32_t cap, int32_t npix, bool fp64) {
 }
 
 // One bounce of the wavefront: extend -> shade (-> shadow -> apply).
+// as (path mode, may be null): the stream of the next-event work. A bounce's shadow
+// trace, occlusion and apply run there once its shade kernel is done, beside the next
+// bounce's extension trace (and guide inference) on hs; the next shade kernel, which
+// overwrites the next-event buffers and adds to the paths' radiance, waits for them.
+// The next-event queue length alternates between counts[2] and counts[3] by bounce, so
+// the extension trace that resets the one its shade kernel appends to leaves the one
+// the running shadow trace reads alone. Small late-bounce queues leave most of the GPU
+// idle behind their slowest rays; the two launches fill it together.
 hipError_t launch_bounce(tt_scene *s, RenderState *rs, const tt_render_params *p, const SceneView &sv, int32_t cap,
-                         int32_t npix, int b, bool fp64, hipStream_t hs, const GuideRun *gr) {
+                         int32_t npix, int b, bool fp64, hipStream_t hs, const GuideRun *gr, hipStream_t as) {
     const int cur = b & 1, nxt = cur ^ 1;
+    const int sc = 2 + (b & 1);
     const int tri_mode = p->mode == TT_RENDER_REFERENCE ? kTriReference : kTriTwoSided;
     // reference mode: the exact loop (hits identical to the host restatement);
     // path mode: the speculative while-while loop unless told otherwise
@@ -1776,7 +1786,7 @@ hipError_t launch_bounce(tt_scene *s, RenderState *rs, const tt_render_params *p
     ta.num_nodes = (int32_t)s->num_nodes;
     ta.n_dev = rs->counts + cur;
     ta.reset0 = rs->counts + nxt;
-    ta.reset1 = rs->counts + 2;
+    ta.reset1 = rs->counts + sc;
     ta.ray_root = rs->root[cur];
     // camera rays (bounce 0) are coherent and their queue is full: one wave per
     // 64-ray group; later extension and shadow queues: persistent waves that refill
@@ -1832,6 +1842,7 @@ hipError_t launch_bounce(tt_scene *s, RenderState *rs, const tt_render_params *p
     sa.counts = rs->counts;
     sa.cur = cur;
     sa.nxt = nxt;
+    sa.scount = sc;
     sa.srays = rs->srays;
     sa.scontrib = rs->scontrib;
     sa.acc = rs->acc;
@@ -1850,10 +1861,16 @@ hipError_t launch_bounce(tt_scene *s, RenderState *rs, const tt_render_params *p
         hipLaunchKernelGGL(k_shade_reference, dim3(sblocks), dim3(kAppendBlock), 0, hs, sa);
         return hipGetLastError();
     }
+    // the previous bounce's next-event work reads the buffers this shade kernel fills
+    if (as && b > 0) e = hipStreamWaitEvent(hs, s->ctx->shadow_done, 0);
+    if (e != hipSuccess) return e;
     if (guide_here || records) hipLaunchKernelGGL(k_shade_path<true>, dim3(sblocks), dim3(kAppendBlock), 0, hs, sa);
     else hipLaunchKernelGGL(k_shade_path<false>, dim3(sblocks), dim3(kAppendBlock), 0, hs, sa);
     e = hipGetLastError();
+    if (e == hipSuccess && as) e = hipEventRecord(s->ctx->shade_done, hs);
+    if (e == hipSuccess && as) e = hipStreamWaitEvent(as, s->ctx->shade_done, 0);
     if (e != hipSuccess) return e;
+    const hipStream_t ss = as ? as : hs;
     TraceArgs sh{};
     sh.nodes = (const float4 *)s->nodes;
     sh.tris = (const float4 *)s->tris;
@@ -1861,24 +1878,30 @@ hipError_t launch_bounce(tt_scene *s, RenderState *rs, const tt_render_params *p
     sh.flags = rs->socc;
     sh.n = cap;
     sh.num_nodes = (int32_t)s->num_nodes;
-    sh.n_dev = rs->counts + 2;
+    sh.n_dev = rs->counts + sc;
     sh.ray_root = rs->sroot;
-    e = launch_trace(sh, variant, true, lds, block, 0, refill_later, s->ctx->num_cus, hs, kTriTwoSided);
+    e = launch_trace(sh, variant, true, lds, block, 0, refill_later, s->ctx->num_cus, ss, kTriTwoSided);
     if (e != hipSuccess) return e;
     if (sv.num_geoms > 0) {
-        GeomArgs go{rs->srays, nullptr, rs->socc, rs->counts + 2, sv.geoms, sv.num_geoms};
-        hipLaunchKernelGGL(k_geom_occluded, dim3(gblocks), dim3(256), 0, hs, go);
+        GeomArgs go{rs->srays, nullptr, rs->socc, rs->counts + sc, sv.geoms, sv.num_geoms};
+        hipLaunchKernelGGL(k_geom_occluded, dim3(gblocks), dim3(256), 0, ss, go);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    ApplyArgs aa{rs->counts, rs->socc, rs->scontrib, rs->L, rs->stats, b, p->max_depth, gr ? gr->gd : GuideShade{}};
-    hipLaunchKernelGGL(k_apply_shadow, dim3(gblocks), dim3(256), 0, hs, aa);
-    return hipGetLastError();
+    ApplyArgs aa{rs->counts + sc, rs->socc, rs->scontrib, rs->L, rs->stats, b, p->max_depth, gr ? gr->gd : GuideShade{}};
+    hipLaunchKernelGGL(k_apply_shadow, dim3(gblocks), dim3(256), 0, ss, aa);
+    e = hipGetLastError();
+    if (e == hipSuccess && as) e = hipEventRecord(s->ctx->shadow_done, as);
+    return e;
 }
 
 // One sample pass (kk samples per pixel in flight) for all scenes of the table.
+// overlap: path mode runs each bounce's next-event work on the context's second stream
+// (off under graph capture, which records hs alone); the pass ends with hs waiting
+// for the last of it.
 hipError_t launch_pass(tt_scene *s, RenderState *rs, const tt_render_params *p, const SceneView &sv, int32_t npix,
-                       int32_t kk, bool fp64, hipStream_t hs, const GuideRun *gr = nullptr) {
+                       int32_t kk, bool fp64, hipStream_t hs, bool overlap, const GuideRun *gr = nullptr) {
+    const hipStream_t as = overlap && p->mode == TT_RENDER_PATH ? s->ctx->aux : nullptr;
     const int32_t cap = npix * kk;
     GenArgs g{};
     g.tab = (const SceneTable *)rs->table;
@@ -1896,7 +1919,12 @@ hipError_t launch_pass(tt_scene *s, RenderState *rs, const tt_render_params *p, 
     g.counts = rs->counts;
     hipLaunchKernelGGL(k_generate, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, hs, g);
     hipError_t e = hipGetLastError();
-    for (int32_t b = 0; e == hipSuccess && b < p->max_depth; ++b) e = launch_bounce(s, rs, p, sv, cap, npix, b, fp64, hs, gr);
+    for (int32_t b = 0; e == hipSuccess && b < p->max_depth; ++b)
+        e = launch_bounce(s, rs, p, sv, cap, npix, b, fp64, hs, gr, as);
+    if (as) {   // whatever was queued on as, also after an error
+        const hipError_t j = hipStreamWaitEvent(hs, s->ctx->shadow_done, 0);
+        if (e == hipSuccess) e = j;
+    }
     if (e == hipSuccess && p->mode == TT_RENDER_PATH) {
         hipLaunchKernelGGL(k_resolve, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, hs, g.tab,
                            (const float4 *)rs->L, npix, kk, fp64 ? nullptr : rs->acc, fp64 ? rs->acc64 : nullptr);
@@ -2138,7 +2166,7 @@ tt_status render_impl(tt_scene *s, int32_t nsc, const tt_camera *cams, const int
             // pass's whole width
             if (train) gr->gd.stride = std::max<int32_t>(1, (int32_t)(((int64_t)npix * kk) / std::max(1, gr->gd.ntrain)));
             gr->gd.toff = gr->gd.stride > 1 ? (int32_t)(((int64_t)pass * 7919) % gr->gd.stride) : 0;
-            e = launch_pass(s, rs, p, sv, npix, kk, fp64, hs, gr);
+            e = launch_pass(s, rs, p, sv, npix, kk, fp64, hs, true, gr);
             if (e == hipSuccess && train) {
                 hipEvent_t t0 = nullptr, t1 = nullptr;
                 e = hipEventCreate(&t0);
@@ -2161,7 +2189,7 @@ tt_status render_impl(tt_scene *s, int32_t nsc, const tt_camera *cams, const int
         // advance the sample index) and replay it
         e = hipStreamBeginCapture(hs, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess) {
-            hipError_t ce = launch_pass(s, rs, p, sv, npix, k, fp64, hs);
+            hipError_t ce = launch_pass(s, rs, p, sv, npix, k, fp64, hs, false);
             e = hipStreamEndCapture(hs, &graph);
             if (e == hipSuccess) e = ce;
         }
@@ -2169,11 +2197,11 @@ tt_status render_impl(tt_scene *s, int32_t nsc, const tt_camera *cams, const int
         for (int32_t i = 0; e == hipSuccess && i < full; ++i) e = hipGraphLaunch(gexec, hs);
         iters += full * p->max_depth;
     } else {
-        for (int32_t i = 0; e == hipSuccess && i < full; ++i) e = launch_pass(s, rs, p, sv, npix, k, fp64, hs);
+        for (int32_t i = 0; e == hipSuccess && i < full; ++i) e = launch_pass(s, rs, p, sv, npix, k, fp64, hs, true);
         iters += full * p->max_depth;
     }
     if (!gr && e == hipSuccess && rest > 0) {
-        e = launch_pass(s, rs, p, sv, npix, rest, fp64, hs);
+        e = launch_pass(s, rs, p, sv, npix, rest, fp64, hs, true);
         iters += p->max_depth;
     }
     if (e == hipSuccess) e = hipEventRecord(e1, hs);

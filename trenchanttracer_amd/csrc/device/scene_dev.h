@@ -25,6 +25,11 @@ struct tt_ctx {
     size_t reorder_ws_bytes = 0;
     hipEvent_t reorder_done = nullptr;  // recorded after the last launch that used reorder_ws
     bool reorder_pending = false;       // reorder_done has been recorded at least once
+    // path-mode renders: each bounce's next-event work (shadow trace, occlusion, apply)
+    // runs on aux, beside the next bounce's extension trace on stream (default priority)
+    hipStream_t aux = nullptr;
+    hipEvent_t shade_done = nullptr;    // recorded on stream after a bounce's shade kernel
+    hipEvent_t shadow_done = nullptr;   // recorded on aux after that bounce's next-event work
 };
 
 struct tt_scene {

@@ -13,12 +13,13 @@ train: 13 Adam steps of the guiding network on 262,144 seeded random rows (input
 guided: the path-trace case rendered at 64 spp with guiding, the network trained
 online during the render (Adam lr 1e-3, 2^18-row batches, 2^21 recorded paths per
 training pass, passes growing 8x, learned selection probability), then once more at
-64 spp with that network frozen.
+64 spp with that network frozen; --guided-bounces N uses the mixture at the first N
+bounces only (0: every bounce).
 For profiler runs, one counter set per run:
 
     rocprofv3 --pmc FETCH_SIZE --output-format csv -d OUT -o run -- \\
         python3 -m trenchanttracer_amd.profile_trace [--case traversal|pathtrace|train|guided] [--variant 1] [--lds-stack 16]
-            [--block 64] [--refill 0] [--reorder 0] [--paths-in-flight N]
+            [--block 64] [--refill 0] [--reorder 0] [--paths-in-flight N] [--guided-bounces 0]
 """
 import argparse
 import sys
@@ -50,6 +51,8 @@ def main():
     ap.add_argument("--case", choices=("traversal", "pathtrace", "train", "guided"), default="traversal")
     ap.add_argument("--paths-in-flight", type=int, default=0, help="pathtrace: paths per pass (0: the default)")
     ap.add_argument("--ntris", type=int, default=1_000_000, help="traversal: triangles of the soup")
+    ap.add_argument("--guided-bounces", type=int, default=0,
+                    help="guided: bounces that use the mixture (max_bounce; 0: every bounce)")
     args = ap.parse_args()
     import trenchanttracer_amd as tt
     from trenchanttracer_amd import workloads
@@ -82,9 +85,11 @@ def main():
             return tt.render_params(spp=spp, max_depth=16, rr_depth=4, seed=seed, mode=tt.RENDER_PATH, **kw)
 
         _, st, gs = g.render(s, cam, rp(64, 2000, paths_in_flight=cam.width * cam.height), alpha=0.5, train=True,
-                             train_paths=2 << 20, train_batch=1 << 18, pass_growth=8, learn_alpha=True)
+                             train_paths=2 << 20, train_batch=1 << 18, pass_growth=8, learn_alpha=True,
+                             max_bounce=args.guided_bounces)
         print(f"guided 64 spp, trained online: {st} {gs}", file=sys.stderr, flush=True)
-        _, st, _ = g.render(s, cam, rp(64, 3000), alpha=0.5, train=False, learn_alpha=True)
+        _, st, _ = g.render(s, cam, rp(64, 3000), alpha=0.5, train=False, learn_alpha=True,
+                            max_bounce=args.guided_bounces)
         print(f"guided 64 spp, frozen network: {st}", file=sys.stderr, flush=True)
         g.close()
         s.close()
