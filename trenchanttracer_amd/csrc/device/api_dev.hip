@@ -124,8 +124,10 @@ tt_status tt_ctx_create(int32_t device, tt_ctx **out) {
     e = hipEventCreateWithFlags(&c->reorder_done, hipEventDisableTiming);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->shade_done, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->shadow_done, hipEventDisableTiming);
+    for (int k = 0; k < 2; ++k)
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&c->shadow_done[k], hipEventDisableTiming);
     if (e != hipSuccess) {
+        if (c->shadow_done[0]) hipEventDestroy(c->shadow_done[0]);
         if (c->shade_done) hipEventDestroy(c->shade_done);
         if (c->aux) hipStreamDestroy(c->aux);
         if (c->reorder_done) hipEventDestroy(c->reorder_done);
@@ -147,7 +149,8 @@ void tt_ctx_destroy(tt_ctx *ctx) {
     if (ctx->reorder_ws) hipFree(ctx->reorder_ws);
     if (ctx->reorder_done) hipEventDestroy(ctx->reorder_done);
     if (ctx->shade_done) hipEventDestroy(ctx->shade_done);
-    if (ctx->shadow_done) hipEventDestroy(ctx->shadow_done);
+    for (hipEvent_t ev : ctx->shadow_done)
+        if (ev) hipEventDestroy(ev);
     hipStreamDestroy(ctx->aux);
     hipStreamDestroy(ctx->stream);
     delete ctx;

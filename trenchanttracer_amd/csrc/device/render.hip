@@ -36,18 +36,21 @@ struct RenderState {
     tt_ray *rays[2] = {nullptr, nullptr};
     int32_t *path[2] = {nullptr, nullptr};
     int32_t *root[2] = {nullptr, nullptr};   // root node of each queued ray's scene
-    int32_t *sroot = nullptr;
+    // next-event queues of even and odd bounces (a bounce's shadow work may still run
+    // while the next bounce fills the other set)
+    int32_t *sroot[2] = {nullptr, nullptr};
     int32_t *sample_dev = nullptr;            // first sample index of the current pass
     void *table = nullptr;                    // device SceneTable
     tt_hit *hits = nullptr;
     // reference: mask | path: throughput, w = density of the last BSDF sample, or -1 when
     // that sample was a specular (delta) one
     float4 *thr = nullptr;
-    float4 *L = nullptr;         // reference: path radiance
-    tt_ray *srays = nullptr;     // next-event rays
-    float4 *scontrib = nullptr;  // contribution (rgb) and pixel (w, as int bits)
-    int32_t *socc = nullptr;     // any-hit flags
-    int32_t *counts = nullptr;   // [0], [1] queues, [2], [3] next-event queues of even and odd bounces
+    float4 *L = nullptr;         // reference: path radiance | path: x roulette weight, yzw radiance
+    float4 *Lsh = nullptr;       // path: next-event radiance (xyz), added to L's at the end of the pass
+    tt_ray *srays[2] = {nullptr, nullptr};     // next-event rays
+    float4 *scontrib[2] = {nullptr, nullptr};  // contribution (rgb) and path id (w, as int bits)
+    int32_t *socc[2] = {nullptr, nullptr};     // any-hit flags
+    int32_t *counts = nullptr;   // [0], [1] queues, [2 + (b & 3)] next-event queue of bounce b
     unsigned long long *stats = nullptr;  // [0] extension rays, [1] shadow rays
     float *acc = nullptr;        // per pixel rgb
     double *acc64 = nullptr;     // per pixel rgb (FP64 accumulation)
@@ -465,6 +468,7 @@ struct ShadeArgs {
     int32_t *root_out, *sroot;
     const tt_hit *hits;
     float4 *thr, *L;
+    float4 *Lsh;       // zeroed for every path on the camera bounce
     int32_t *counts;
     int32_t cur, nxt;
     int32_t scount;    // counts[scount]: this bounce's next-event queue
@@ -1111,6 +1115,7 @@ __device__ __forceinline__ void shade_path_one(const ShadeArgs &a, const int32_t
         const float last_pdf = tp.w;
         const bool last_delta = tp.w < 0.0f;
         float4 Ls = first ? make_float4(1.0f, 0.0f, 0.0f, 0.0f) : a.L[p];   // x: roulette weight, yzw: radiance
+        if (first) a.Lsh[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         const f3 ro = F3(r.ox, r.oy, r.oz), rd = F3(r.dx, r.dy, r.dz);
         f3 add = F3(0, 0, 0);
         if (h.tri_id == -1) {
@@ -1324,7 +1329,7 @@ struct ApplyArgs {
     const int32_t *n_dev;   // the bounce's next-event queue length
     const int32_t *occ;
     const float4 *contrib;
-    float4 *L;           // per-path radiance in yzw
+    float4 *Lsh;         // per-path next-event radiance in xyz
     unsigned long long *stats;
     int32_t bounce, max_depth;
     GuideShade gd;
@@ -1339,12 +1344,13 @@ __global__ __launch_bounds__(256) void k_apply_shadow(ApplyArgs a) {
         const int32_t path = __float_as_int(c.w);
         const int tc = train_slot(a.gd, path);
         if (tc >= 0) credit(a.gd.rec, tc, a.bounce, a.max_depth, F3(c.x, c.y, c.z));
-        // a path has at most one next-event ray per bounce: plain read-modify-write
-        float4 l = a.L[path];
-        l.y += c.x;
-        l.z += c.y;
-        l.w += c.z;
-        a.L[path] = l;
+        // a path has at most one next-event ray per bounce, and the bounces' apply
+        // launches run in order: plain read-modify-write
+        float4 l = a.Lsh[path];
+        l.x += c.x;
+        l.y += c.y;
+        l.z += c.z;
+        a.Lsh[path] = l;
     }
 }
 
@@ -1372,12 +1378,23 @@ struct ResolveTile {
     float4 v[256][kResolveChunk + 1];
 };
 
-__device__ __forceinline__ void stage_chunk(ResolveTile &t, const float4 *L, int32_t q0, int32_t npix, int32_t kk,
-                                            int32_t s0) {
+// Lsh (path mode): each path's next-event radiance, added to its other radiance (yzw).
+__device__ __forceinline__ void stage_chunk(ResolveTile &t, const float4 *L, const float4 *Lsh, int32_t q0,
+                                            int32_t npix, int32_t kk, int32_t s0) {
     for (int i = threadIdx.x; i < 256 * kResolveChunk; i += 256) {
         const int pl = i / kResolveChunk, j = i - pl * kResolveChunk;
         const int32_t q = q0 + pl, s = s0 + j;
-        if (q < npix && s < kk) t.v[pl][j] = L[path_id(q, s, kk)];
+        if (q < npix && s < kk) {
+            const int32_t id = path_id(q, s, kk);
+            float4 v = L[id];
+            if (Lsh) {
+                const float4 e = Lsh[id];
+                v.y += e.x;
+                v.z += e.y;
+                v.w += e.z;
+            }
+            t.v[pl][j] = v;
+        }
     }
 }
 
@@ -1395,7 +1412,7 @@ __global__ __launch_bounds__(256) void k_resolve_reference(const SceneTable *tab
     }
     for (int32_t s0 = 0; s0 < kk; s0 += kResolveChunk) {
         __syncthreads();
-        stage_chunk(t, L, q0, npix, kk, s0);
+        stage_chunk(t, L, nullptr, q0, npix, kk, s0);
         __syncthreads();
         const int32_t m = min(kResolveChunk, kk - s0);
         for (int j = 0; j < m; ++j) {
@@ -1417,8 +1434,8 @@ __global__ __launch_bounds__(256) void k_resolve_reference(const SceneTable *tab
 // the sum is the same left fold over the pixel's samples for any pass structure
 // (a batch of scenes, whose passes split the samples differently, gives each scene
 // the bits of its own render).
-__global__ __launch_bounds__(256) void k_resolve(const SceneTable *tab, const float4 *L, int32_t npix, int32_t kk,
-                                                 float *acc, double *acc64) {
+__global__ __launch_bounds__(256) void k_resolve(const SceneTable *tab, const float4 *L, const float4 *Lsh,
+                                                 int32_t npix, int32_t kk, float *acc, double *acc64) {
     __shared__ ResolveTile t;
     const int32_t q0 = blockIdx.x * 256, q = q0 + threadIdx.x;
     const bool live = q < npix;
@@ -1440,7 +1457,7 @@ __global__ __launch_bounds__(256) void k_resolve(const SceneTable *tab, const fl
     }
     for (int32_t s0 = 0; s0 < kk; s0 += kResolveChunk) {
         __syncthreads();
-        stage_chunk(t, L, q0, npix, kk, s0);
+        stage_chunk(t, L, Lsh, q0, npix, kk, s0);
         __syncthreads();
         const int32_t m = min(kResolveChunk, kk - s0);
         for (int j = 0; j < m; ++j) {
@@ -1735,11 +1752,14 @@ tt_status ensure_state(tt_scene *s, int32_t cap, int32_t npix, bool fp64) {
     al((void **)&rs->hits, (size_t)cap * sizeof(tt_hit), "render hits");
     al((void **)&rs->thr, (size_t)cap * 16, "render throughput");
     al((void **)&rs->L, (size_t)cap * 16, "render radiance");
-    al((void **)&rs->srays, (size_t)cap * sizeof(tt_ray), "render shadow rays");
-    al((void **)&rs->sroot, (size_t)cap * 4, "render shadow roots");
-    al((void **)&rs->scontrib, (size_t)cap * 16, "render shadow contributions");
-    al((void **)&rs->socc, (size_t)cap * 4, "render shadow flags");
-    al((void **)&rs->counts, 16, "render queue counts");
+    al((void **)&rs->Lsh, (size_t)cap * 16, "render next-event radiance");
+    for (int k = 0; k < 2; ++k) {
+        al((void **)&rs->srays[k], (size_t)cap * sizeof(tt_ray), "render shadow rays");
+        al((void **)&rs->sroot[k], (size_t)cap * 4, "render shadow roots");
+        al((void **)&rs->scontrib[k], (size_t)cap * 16, "render shadow contributions");
+        al((void **)&rs->socc[k], (size_t)cap * 4, "render shadow flags");
+    }
+    al((void **)&rs->counts, 32, "render queue counts");
     al((void **)&rs->stats, 16, "render stats");
     al((void **)&rs->sample_dev, 16, "render sample index");
     al((void **)&rs->table, sizeof(SceneTable), "render scene table");
@@ -1755,16 +1775,20 @@ tt_status ensure_state(tt_scene *s, int32_t cap, int32_t npix, bool fp64) {
 // One bounce of the wavefront: extend -> shade (-> shadow -> apply).
 // as (path mode, may be null): the stream of the next-event work. A bounce's shadow
 // trace, occlusion and apply run there once its shade kernel is done, beside the next
-// bounce's extension trace (and guide inference) on hs; the next shade kernel, which
-// overwrites the next-event buffers and adds to the paths' radiance, waits for them.
-// The next-event queue length alternates between counts[2] and counts[3] by bounce, so
-// the extension trace that resets the one its shade kernel appends to leaves the one
-// the running shadow trace reads alone. Small late-bounce queues leave most of the GPU
-// idle behind their slowest rays; the two launches fill it together.
+// bounces' extension traces, guide inference and shade kernels on hs. The next-event
+// buffers alternate by bounce (set b & 1), and the apply adds into the paths' own
+// next-event radiance (Lsh, summed with L at the end of the pass), so a shade kernel
+// waits only for the next-event work two bounces back, the last user of its set. In a
+// pass that records training paths both the apply and the shade kernels credit the
+// records, so there each shade kernel waits for the previous bounce's apply. The
+// queue length of bounce b is counts[2 + (b & 3)]: the extension trace zeroes the one
+// its shade kernel appends to, three bounces past the last next-event work that read
+// it, which a shade kernel before it has waited for. Small late-bounce queues leave
+// most of the GPU idle behind their slowest rays; the launches fill it together.
 hipError_t launch_bounce(tt_scene *s, RenderState *rs, const tt_render_params *p, const SceneView &sv, int32_t cap,
                          int32_t npix, int b, bool fp64, hipStream_t hs, const GuideRun *gr, hipStream_t as) {
     const int cur = b & 1, nxt = cur ^ 1;
-    const int sc = 2 + (b & 1);
+    const int sc = 2 + (b & 3), set = b & 1;
     const int tri_mode = p->mode == TT_RENDER_REFERENCE ? kTriReference : kTriTwoSided;
     // reference mode: the exact loop (hits identical to the host restatement);
     // path mode: the speculative while-while loop unless told otherwise
@@ -1835,16 +1859,17 @@ hipError_t launch_bounce(tt_scene *s, RenderState *rs, const tt_render_params *p
     sa.path_out = rs->path[nxt];
     sa.root_in = rs->root[cur];
     sa.root_out = rs->root[nxt];
-    sa.sroot = rs->sroot;
+    sa.sroot = rs->sroot[set];
     sa.hits = rs->hits;
     sa.thr = rs->thr;
     sa.L = rs->L;
+    sa.Lsh = rs->Lsh;
     sa.counts = rs->counts;
     sa.cur = cur;
     sa.nxt = nxt;
     sa.scount = sc;
-    sa.srays = rs->srays;
-    sa.scontrib = rs->scontrib;
+    sa.srays = rs->srays[set];
+    sa.scontrib = rs->scontrib[set];
     sa.acc = rs->acc;
     sa.acc64 = fp64 ? rs->acc64 : nullptr;
     sa.stats = rs->stats;
@@ -1861,8 +1886,10 @@ hipError_t launch_bounce(tt_scene *s, RenderState *rs, const tt_render_params *p
         hipLaunchKernelGGL(k_shade_reference, dim3(sblocks), dim3(kAppendBlock), 0, hs, sa);
         return hipGetLastError();
     }
-    // the previous bounce's next-event work reads the buffers this shade kernel fills
-    if (as && b > 0) e = hipStreamWaitEvent(hs, s->ctx->shadow_done, 0);
+    // the next-event work that last read this bounce's set (two bounces back), or with
+    // training records the previous bounce's
+    if (as && records && b >= 1) e = hipStreamWaitEvent(hs, s->ctx->shadow_done[(b - 1) & 1], 0);
+    else if (as && b >= 2) e = hipStreamWaitEvent(hs, s->ctx->shadow_done[set], 0);
     if (e != hipSuccess) return e;
     if (guide_here || records) hipLaunchKernelGGL(k_shade_path<true>, dim3(sblocks), dim3(kAppendBlock), 0, hs, sa);
     else hipLaunchKernelGGL(k_shade_path<false>, dim3(sblocks), dim3(kAppendBlock), 0, hs, sa);
@@ -1874,24 +1901,25 @@ hipError_t launch_bounce(tt_scene *s, RenderState *rs, const tt_render_params *p
     TraceArgs sh{};
     sh.nodes = (const float4 *)s->nodes;
     sh.tris = (const float4 *)s->tris;
-    sh.rays = rs->srays;
-    sh.flags = rs->socc;
+    sh.rays = rs->srays[set];
+    sh.flags = rs->socc[set];
     sh.n = cap;
     sh.num_nodes = (int32_t)s->num_nodes;
     sh.n_dev = rs->counts + sc;
-    sh.ray_root = rs->sroot;
+    sh.ray_root = rs->sroot[set];
     e = launch_trace(sh, variant, true, lds, block, 0, refill_later, s->ctx->num_cus, ss, kTriTwoSided);
     if (e != hipSuccess) return e;
     if (sv.num_geoms > 0) {
-        GeomArgs go{rs->srays, nullptr, rs->socc, rs->counts + sc, sv.geoms, sv.num_geoms};
+        GeomArgs go{rs->srays[set], nullptr, rs->socc[set], rs->counts + sc, sv.geoms, sv.num_geoms};
         hipLaunchKernelGGL(k_geom_occluded, dim3(gblocks), dim3(256), 0, ss, go);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    ApplyArgs aa{rs->counts + sc, rs->socc, rs->scontrib, rs->L, rs->stats, b, p->max_depth, gr ? gr->gd : GuideShade{}};
+    ApplyArgs aa{rs->counts + sc, rs->socc[set], rs->scontrib[set], rs->Lsh, rs->stats, b, p->max_depth,
+                 gr ? gr->gd : GuideShade{}};
     hipLaunchKernelGGL(k_apply_shadow, dim3(gblocks), dim3(256), 0, ss, aa);
     e = hipGetLastError();
-    if (e == hipSuccess && as) e = hipEventRecord(s->ctx->shadow_done, as);
+    if (e == hipSuccess && as) e = hipEventRecord(s->ctx->shadow_done[set], as);
     return e;
 }
 
@@ -1921,13 +1949,14 @@ hipError_t launch_pass(tt_scene *s, RenderState *rs, const tt_render_params *p, 
     hipError_t e = hipGetLastError();
     for (int32_t b = 0; e == hipSuccess && b < p->max_depth; ++b)
         e = launch_bounce(s, rs, p, sv, cap, npix, b, fp64, hs, gr, as);
-    if (as) {   // whatever was queued on as, also after an error
-        const hipError_t j = hipStreamWaitEvent(hs, s->ctx->shadow_done, 0);
+    if (as) {   // whatever was queued on as (in order: the last bounce's event covers it), also after an error
+        const hipError_t j = hipStreamWaitEvent(hs, s->ctx->shadow_done[(p->max_depth - 1) & 1], 0);
         if (e == hipSuccess) e = j;
     }
     if (e == hipSuccess && p->mode == TT_RENDER_PATH) {
         hipLaunchKernelGGL(k_resolve, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, hs, g.tab,
-                           (const float4 *)rs->L, npix, kk, fp64 ? nullptr : rs->acc, fp64 ? rs->acc64 : nullptr);
+                           (const float4 *)rs->L, (const float4 *)rs->Lsh, npix, kk, fp64 ? nullptr : rs->acc,
+                           fp64 ? rs->acc64 : nullptr);
         e = hipGetLastError();
     } else if (e == hipSuccess) {
         hipLaunchKernelGGL(k_resolve_reference, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, hs, g.tab,
@@ -2244,7 +2273,8 @@ void free_render_state(tt_scene *s) {
     RenderState *rs = s->rs;
     if (!rs) return;
     void *ptrs[] = {rs->rays[0], rs->rays[1], rs->path[0], rs->path[1], rs->root[0], rs->root[1], rs->hits, rs->thr,
-                    rs->L, rs->srays, rs->sroot, rs->scontrib, rs->socc, rs->counts, rs->stats,
+                    rs->L, rs->Lsh, rs->srays[0], rs->srays[1], rs->sroot[0], rs->sroot[1], rs->scontrib[0],
+                    rs->scontrib[1], rs->socc[0], rs->socc[1], rs->counts, rs->stats,
                     rs->sample_dev, rs->table, rs->acc, rs->acc64};
     for (void *q : ptrs)
         if (q) hipFree(q);

@@ -29,7 +29,7 @@ struct tt_ctx {
     // runs on aux, beside the next bounce's extension trace on stream (default priority)
     hipStream_t aux = nullptr;
     hipEvent_t shade_done = nullptr;    // recorded on stream after a bounce's shade kernel
-    hipEvent_t shadow_done = nullptr;   // recorded on aux after that bounce's next-event work
+    hipEvent_t shadow_done[2] = {nullptr, nullptr};   // on aux after the next-event work of even / odd bounces
 };
 
 struct tt_scene {
