@@ -442,10 +442,16 @@ __device__ __forceinline__ Surf surface_path(const SceneView &sv, const tt_hit &
     return s;
 }
 
+// Training record of a path vertex: inputs (9), wi (3), pdf, 1 / throughput after the
+// vertex (3), radiance credited by the shade kernels (3), BSDF density of wi, radiance
+// credited by the next-event apply kernels (3), unused. The two kinds of credit land in
+// separate fields because a bounce's apply kernel may run beside the next bounce's shade
+// kernel.
+constexpr int kRecFloats = 24;
+constexpr int kRecNext = 20;   // first field of the next-event credit
+static_assert(kRecFloats == 24, "records are six 16-byte pieces");
+
 // Guiding inputs of the shade stage (all null / zero when unguided).
-constexpr int kRecFloats = 20;  // inputs (9), wi (3), pdf, 1 / throughput after the vertex (3), radiance (3),
-                                // BSDF density of wi
-static_assert(kRecFloats == 20, "records are read as five 16-byte pieces");
 struct GuideShade {
     const float *raw;        // [slot][48] mixture of this bounce's queue slots (null: unguided)
     float alpha;             // probability of sampling the guide (one-sample MIS with the BSDF)
@@ -930,13 +936,15 @@ __device__ __forceinline__ int train_slot(const GuideShade &gd, int32_t p) {
 
 // Adds a radiance contribution found at segment `bounce` to the incident-radiance
 // estimates of the path's earlier recorded vertices: L_i += c / T_{i+1}.
-__device__ __forceinline__ void credit(float *rec, int t, int bounce, int max_depth, f3 c) {
+// Adds contribution c, arriving at vertex `bounce`, to the radiance of the path's earlier
+// recorded vertices (field f: 16 from the shade kernels, kRecNext from the apply kernels).
+__device__ __forceinline__ void credit(float *rec, int t, int bounce, int max_depth, f3 c, int f) {
     for (int i = 0; i < bounce; ++i) {
         float *r = rec + ((int64_t)t * max_depth + i) * kRecFloats;
         if (r[12] > 0.0f) {
-            r[16] += c.x * r[13];
-            r[17] += c.y * r[14];
-            r[18] += c.z * r[15];
+            r[f] += c.x * r[13];
+            r[f + 1] += c.y * r[14];
+            r[f + 2] += c.z * r[15];
         }
     }
 }
@@ -1285,6 +1293,7 @@ __device__ __forceinline__ void shade_path_one(const ShadeArgs &a, const int32_t
                                 rec_b[14] = thr.y > 0.0f ? 1.0f / thr.y : 0.0f;
                                 rec_b[15] = thr.z > 0.0f ? 1.0f / thr.z : 0.0f;
                                 rec_b[16] = rec_b[17] = rec_b[18] = 0.0f;
+                                rec_b[kRecNext] = rec_b[kRecNext + 1] = rec_b[kRecNext + 2] = 0.0f;
                                 rec_b[19] = pdf_bsdf;
                                 rec_b[12] = pdf_b;
                             }
@@ -1295,7 +1304,7 @@ __device__ __forceinline__ void shade_path_one(const ShadeArgs &a, const int32_t
         }
         if (maxc(add) > 0.0f) {
             const int tc = GUIDED ? train_slot(a.gd, p) : -1;
-            if (tc >= 0) credit(a.gd.rec, tc, a.bounce, a.max_depth, add);
+            if (tc >= 0) credit(a.gd.rec, tc, a.bounce, a.max_depth, add, 16);
             Ls.y += add.x;
             Ls.z += add.y;
             Ls.w += add.z;
@@ -1343,7 +1352,7 @@ __global__ __launch_bounds__(256) void k_apply_shadow(ApplyArgs a) {
         const float4 c = a.contrib[i];
         const int32_t path = __float_as_int(c.w);
         const int tc = train_slot(a.gd, path);
-        if (tc >= 0) credit(a.gd.rec, tc, a.bounce, a.max_depth, F3(c.x, c.y, c.z));
+        if (tc >= 0) credit(a.gd.rec, tc, a.bounce, a.max_depth, F3(c.x, c.y, c.z), kRecNext);
         // a path has at most one next-event ray per bounce, and the bounces' apply
         // launches run in order: plain read-modify-write
         float4 l = a.Lsh[path];
@@ -1504,7 +1513,8 @@ __device__ __forceinline__ bool record_row(const float *rec, int64_t e, int64_t 
     if (!(r[12] > 0.0f)) return false;
     const float nx = 2.0f * r[3] - 1.0f, ny = 2.0f * r[4] - 1.0f, nz = 2.0f * r[5] - 1.0f;
     const float cosw = fabsf(nx * r[9] + ny * r[10] + nz * r[11]);
-    *w = (0.2126f * r[16] + 0.7152f * r[17] + 0.0722f * r[18]) * cosw / r[12];
+    *w = (0.2126f * (r[16] + r[kRecNext]) + 0.7152f * (r[17] + r[kRecNext + 1]) + 0.0722f * (r[18] + r[kRecNext + 2])) *
+         cosw / r[12];
     return *w > 0.0f && isfinite(*w);
 }
 
@@ -1534,7 +1544,7 @@ __global__ __launch_bounds__(256) void k_collect_write(float *rec, int64_t ne, c
     int32_t slot = off[blockIdx.x] + __popcll(m & ((1ull << lane) - 1ull));
     for (int k = 0; k < wave; ++k) slot += wc[k];
     if (valid && slot < cap) {
-        // 16-byte pieces: records are 80 bytes, rows 64
+        // 16-byte pieces: records are 96 bytes, rows 64
         const float4 *r = reinterpret_cast<const float4 *>(rec + e * kRecFloats);
         float4 *o = reinterpret_cast<float4 *>(buf + (int64_t)slot * 16);
         const float4 r0 = r[0], r1 = r[1], r2 = r[2], r4 = r[4];
@@ -1777,11 +1787,9 @@ tt_status ensure_state(tt_scene *s, int32_t cap, int32_t npix, bool fp64) {
 // trace, occlusion and apply run there once its shade kernel is done, beside the next
 // bounces' extension traces, guide inference and shade kernels on hs. The next-event
 // buffers alternate by bounce (set b & 1), and the apply adds into the paths' own
-// next-event radiance (Lsh, summed with L at the end of the pass), so a shade kernel
-// waits only for the next-event work two bounces back, the last user of its set. In a
-// pass that records training paths both the apply and the shade kernels credit the
-// records, so there each shade kernel waits for the previous bounce's apply. The
-// queue length of bounce b is counts[2 + (b & 3)]: the extension trace zeroes the one
+// next-event radiance (Lsh, summed with L at the end of the pass) and credits training
+// records in fields of their own, so a shade kernel waits only for the next-event work
+// two bounces back, the last user of its set. The queue length of bounce b is counts[2 + (b & 3)]: the extension trace zeroes the one
 // its shade kernel appends to, three bounces past the last next-event work that read
 // it, which a shade kernel before it has waited for. Small late-bounce queues leave
 // most of the GPU idle behind their slowest rays; the launches fill it together.
@@ -1886,10 +1894,8 @@ hipError_t launch_bounce(tt_scene *s, RenderState *rs, const tt_render_params *p
         hipLaunchKernelGGL(k_shade_reference, dim3(sblocks), dim3(kAppendBlock), 0, hs, sa);
         return hipGetLastError();
     }
-    // the next-event work that last read this bounce's set (two bounces back), or with
-    // training records the previous bounce's
-    if (as && records && b >= 1) e = hipStreamWaitEvent(hs, s->ctx->shadow_done[(b - 1) & 1], 0);
-    else if (as && b >= 2) e = hipStreamWaitEvent(hs, s->ctx->shadow_done[set], 0);
+    // the next-event work that last read this bounce's set (two bounces back)
+    if (as && b >= 2) e = hipStreamWaitEvent(hs, s->ctx->shadow_done[set], 0);
     if (e != hipSuccess) return e;
     if (guide_here || records) hipLaunchKernelGGL(k_shade_path<true>, dim3(sblocks), dim3(kAppendBlock), 0, hs, sa);
     else hipLaunchKernelGGL(k_shade_path<false>, dim3(sblocks), dim3(kAppendBlock), 0, hs, sa);
@@ -2126,9 +2132,9 @@ tt_status render_impl(tt_scene *s, int32_t nsc, const tt_camera *cams, const int
     // k samples per pixel are in flight in one pass (each path's radiance is kept in
     // its own slot and the pass ends by adding a pixel's samples in sample order, in
     // both modes). Each bounce's launches last as long as their slowest waves, so the
-    // fewer the passes the less that tail costs: by default up to 2^27 paths (184 B of
-    // state each, 24.7 GB) are in flight, guided renders with a frozen network included
-    // (228 B more per path for the network's inputs and mixture). Renders that train the network
+    // fewer the passes the less that tail costs: by default up to 2^27 paths (256 B of
+    // state each, 34.4 GB) are in flight, guided renders with a frozen network included
+    // (about 232 B more per path for the network's inputs, mixture and slot list). Renders that train the network
     // default to about 8M paths: it learns between passes.
     int32_t k = 1;
     {
