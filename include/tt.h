@@ -377,9 +377,11 @@ typedef struct tt_render_params {
     int32_t accumulate_fp64;  /* accumulate pixel sums in FP64 (oracle renders) */
     float pi, two_pi;         /* camera / env constants (reference: 3.14156265f and 6.2831853f) */
     float tmin;               /* reference mode ray tmin (0.01, RenderKernel.cu:110) */
-    int32_t use_graph;        /* capture one sample pass in a hipGraph and replay it */
+    int32_t use_graph;        /* capture one sample pass in a hipGraph and replay it (the captured
+                                 pass runs on one stream; otherwise path mode runs each bounce's
+                                 next-event work on the context's second stream) */
     int32_t paths_in_flight;  /* target paths per pass, rounded to whole samples (0 = up to
-                                 2^27 paths, 184 B of state each, 416 B with a guide;
+                                 2^27 paths, 256 B of state each, 488 B with a guide;
                                  guided renders that train the network: about 8M) */
     int32_t lds_stack;        /* traversal LDS stack entries per lane (0 = 16) */
     int32_t block;            /* traversal block size (0 = 64) */
